@@ -1,0 +1,291 @@
+"""dada2-rs compare hot path on MI355X (gfx950): ctypes binding of the C ABI.
+
+The library (lib/libdada2_amd.so, built from csrc/*.hip) is the product; this
+module only marshals numpy arrays through the C ABI declared in
+include/dada2_amd.h.  There is no CPU fallback: every call runs the HIP
+kernels, and loading fails loudly when the library is missing.
+
+Names follow the reference (dada2-rs v0.2.0): AlignParams (nwalign.rs:63-90),
+VectorizedAlignScores (nwalign.rs:96-103), the b_compare_parallel map
+(cluster.rs:279-330) and align_vectorized_with_buf (nwalign.rs:761).
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass
+from pathlib import Path
+
+import numpy as np
+
+PKG_DIR = Path(__file__).resolve().parent
+LIB_PATH = PKG_DIR / "lib" / "libdada2_amd.so"
+
+D2G_OK = 0
+HAMMING_NONE = 0xFFFFFFFF
+
+u8p = C.POINTER(C.c_uint8)
+u32p = C.POINTER(C.c_uint32)
+u64p = C.POINTER(C.c_uint64)
+f64p = C.POINTER(C.c_double)
+
+
+class AlignParams(C.Structure):
+    """AlignParams (nwalign.rs:63-90), NW backend."""
+    _fields_ = [("match_score", C.c_int32), ("mismatch", C.c_int32), ("gap_p", C.c_int32),
+                ("homo_gap_p", C.c_int32), ("use_kmers", C.c_int32), ("kmer_size", C.c_int32),
+                ("kdist_cutoff", C.c_double), ("band", C.c_int32), ("vectorized", C.c_int32),
+                ("gapless", C.c_int32), ("_reserved", C.c_int32)]
+
+
+class VecScores(C.Structure):
+    """VectorizedAlignScores (nwalign.rs:96-103)."""
+    _fields_ = [("match_score", C.c_int16), ("mismatch", C.c_int16), ("gap_p", C.c_int16),
+                ("end_gap_p", C.c_int16), ("band", C.c_int32)]
+
+
+class Stats(C.Structure):
+    _fields_ = [("comparisons", C.c_uint64), ("skipped", C.c_uint64), ("shrouded", C.c_uint64),
+                ("gapless", C.c_uint64), ("aligned_dp", C.c_uint64), ("dp_cells", C.c_uint64),
+                ("ms_screen", C.c_double), ("ms_plan", C.c_double), ("ms_dp", C.c_double),
+                ("ms_lambda", C.c_double), ("ms_total", C.c_double)]
+
+    def as_dict(self) -> dict:
+        return {name: getattr(self, name) for name, _ in self._fields_}
+
+
+def align_params(match=5, mismatch=-4, gap_p=-8, homo_gap_p=None, use_kmers=True, kmer_size=5,
+                 kdist_cutoff=0.42, band=16, vectorized=True, gapless=True) -> AlignParams:
+    """Defaults of resolve_dada_params (main.rs:4393-4433)."""
+    return AlignParams(match, mismatch, gap_p, gap_p if homo_gap_p is None else homo_gap_p,
+                       int(use_kmers), kmer_size, kdist_cutoff, band, int(vectorized),
+                       int(gapless), 0)
+
+
+class D2GError(RuntimeError):
+    def __init__(self, status: int, what: str):
+        self.status = status
+        msg = _lib().d2g_status_string(status).decode() if _LIB is not None else str(status)
+        super().__init__(f"{what}: status {status} ({msg})")
+
+
+_LIB = None
+
+
+def _lib():
+    global _LIB
+    if _LIB is None:
+        if not LIB_PATH.exists():
+            raise FileNotFoundError(
+                f"{LIB_PATH} is missing: build it first (__graft_entry__.build())")
+        L = C.CDLL(str(LIB_PATH))
+        vp = C.c_void_p
+        sig = {
+            "d2g_ctx_create": (C.c_int, [C.c_int, C.POINTER(vp)]),
+            "d2g_ctx_destroy": (C.c_int, [vp]),
+            "d2g_ctx_set_timing": (C.c_int, [vp, C.c_int]),
+            "d2g_ctx_stream": (vp, [vp]),
+            "d2g_device_info": (C.c_int, [vp, C.POINTER(C.c_int), C.POINTER(C.c_int),
+                                          C.POINTER(C.c_int), C.POINTER(C.c_int)]),
+            "d2g_status_string": (C.c_char_p, [C.c_int]),
+            "d2g_load_uniques": (C.c_int, [vp, C.c_uint32, u8p, u64p, u8p, u32p, C.c_uint32]),
+            "d2g_compare_center": (C.c_int, [vp, C.c_uint32, f64p, C.c_uint32,
+                                             C.POINTER(AlignParams), C.c_int, C.c_int, u8p, f64p,
+                                             u32p, u8p, C.POINTER(Stats)]),
+            "d2g_compare_batch": (C.c_int, [vp, u32p, C.c_uint32, f64p, C.c_uint32,
+                                            C.POINTER(AlignParams), C.c_int, C.c_int, u8p, f64p,
+                                            u32p, u8p, C.POINTER(Stats)]),
+            "d2g_compare_batch_device": (C.c_int, [vp, u32p, C.c_uint32, f64p, C.c_uint32,
+                                                   C.POINTER(AlignParams), C.c_int, C.c_int, u8p,
+                                                   C.POINTER(Stats)]),
+            "d2g_fetch_results": (C.c_int, [vp, f64p, u32p, u8p]),
+            "d2g_align_vectorized": (C.c_int, [vp, C.c_uint32, u8p, u64p, u8p, u64p,
+                                               C.POINTER(VecScores), u8p, u8p, u64p, u32p]),
+            "d2g_vectorized_cells": (C.c_uint64, [C.c_uint32, C.c_uint32, C.c_int32]),
+            "d2g_vectorized_supported": (C.c_int, [C.c_uint32, C.c_uint32, C.c_int32]),
+        }
+        for name, (res, args) in sig.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _LIB = L
+    return _LIB
+
+
+def exported_symbols() -> list[str]:
+    """Names of the C ABI entry points (include/dada2_amd.h)."""
+    return ["d2g_ctx_create", "d2g_ctx_destroy", "d2g_ctx_set_timing", "d2g_ctx_stream",
+            "d2g_device_info", "d2g_status_string", "d2g_load_uniques", "d2g_compare_center",
+            "d2g_compare_batch", "d2g_compare_batch_device", "d2g_fetch_results",
+            "d2g_align_vectorized", "d2g_vectorized_cells", "d2g_vectorized_supported"]
+
+
+def _p(a: np.ndarray, t):
+    return a.ctypes.data_as(t)
+
+
+def _check(rc: int, what: str):
+    if rc != D2G_OK:
+        raise D2GError(rc, what)
+
+
+def vectorized_cells(l1: int, l2: int, band: int) -> int:
+    return int(_lib().d2g_vectorized_cells(l1, l2, band))
+
+
+def vectorized_supported(l1: int, l2: int, band: int) -> bool:
+    return bool(_lib().d2g_vectorized_supported(l1, l2, band))
+
+
+def _concat(seqs) -> tuple[np.ndarray, np.ndarray]:
+    arrs = [np.frombuffer(bytes(s), np.uint8) if isinstance(s, (bytes, bytearray))
+            else np.asarray(s, np.uint8) for s in seqs]
+    off = np.zeros(len(arrs) + 1, np.uint64)
+    off[1:] = np.cumsum([len(a) for a in arrs])
+    cat = np.concatenate(arrs) if arrs else np.zeros(0, np.uint8)
+    return np.ascontiguousarray(cat, np.uint8), off
+
+
+@dataclass
+class CompareResult:
+    """Per-(center, unique) output of the compare map (cluster.rs:279-330)."""
+    lam: np.ndarray
+    hamming: np.ndarray
+    skipped: np.ndarray
+    stats: dict
+
+
+class Context:
+    """One device context: a HIP stream plus the resident uniques."""
+
+    def __init__(self, device: int = 0, timing: bool = False):
+        L = _lib()
+        h = C.c_void_p()
+        _check(L.d2g_ctx_create(device, C.byref(h)), "d2g_ctx_create")
+        self.h = h
+        self.n = 0
+        if timing:
+            _check(L.d2g_ctx_set_timing(self.h, 1), "d2g_ctx_set_timing")
+
+    def close(self):
+        if getattr(self, "h", None):
+            _lib().d2g_ctx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def set_timing(self, on: bool):
+        _check(_lib().d2g_ctx_set_timing(self.h, int(on)), "d2g_ctx_set_timing")
+
+    def device_info(self) -> dict:
+        a, b, c, d = C.c_int(), C.c_int(), C.c_int(), C.c_int()
+        _check(_lib().d2g_device_info(self.h, C.byref(a), C.byref(b), C.byref(c), C.byref(d)),
+               "d2g_device_info")
+        return {"cu_count": a.value, "clock_khz": b.value, "mem_clock_khz": c.value,
+                "mem_bus_width": d.value}
+
+    def load_uniques(self, seqs, quals=None, reads=None, k: int = 5):
+        """Resident uniques (dada_uniques_cached's Raw build, dada.rs:243-402)."""
+        cat, off = _concat(seqs)
+        q = None
+        if quals is not None:
+            q, qoff = _concat(quals)
+            if not np.array_equal(qoff, off):
+                raise ValueError("quality lengths must match sequence lengths")
+        r = np.ascontiguousarray(np.ones(len(seqs)) if reads is None else reads, np.uint32)
+        _check(_lib().d2g_load_uniques(self.h, len(seqs), _p(cat, u8p), _p(off, u64p),
+                                       _p(q, u8p) if q is not None else None, _p(r, u32p), k),
+               "d2g_load_uniques")
+        self.n = len(seqs)
+
+    def _common(self, err, params, lock):
+        e = np.ascontiguousarray(err, np.float64)
+        if e.ndim != 2 or e.shape[0] != 16:
+            raise ValueError("err must be a 16 x ncol matrix")
+        lk = None if lock is None else np.ascontiguousarray(lock, np.uint8)
+        return e, lk
+
+    def compare_batch(self, centers, err, params: AlignParams | None = None, use_quals=True,
+                      greedy=True, lock=None, device_only=False) -> CompareResult | dict:
+        """All centers x all uniques.  Rows follow `centers`."""
+        params = params or align_params()
+        e, lk = self._common(err, params, lock)
+        cs = np.ascontiguousarray(np.atleast_1d(centers), np.uint32)
+        st = Stats()
+        L = _lib()
+        if device_only:
+            _check(L.d2g_compare_batch_device(self.h, _p(cs, u32p), len(cs), _p(e, f64p),
+                                              e.shape[1], C.byref(params), int(use_quals),
+                                              int(greedy), _p(lk, u8p) if lk is not None else None,
+                                              C.byref(st)), "d2g_compare_batch_device")
+            return st.as_dict()
+        shape = (len(cs), self.n)
+        lam = np.zeros(shape, np.float64)
+        ham = np.zeros(shape, np.uint32)
+        skp = np.zeros(shape, np.uint8)
+        _check(L.d2g_compare_batch(self.h, _p(cs, u32p), len(cs), _p(e, f64p), e.shape[1],
+                                   C.byref(params), int(use_quals), int(greedy),
+                                   _p(lk, u8p) if lk is not None else None, _p(lam, f64p),
+                                   _p(ham, u32p), _p(skp, u8p), C.byref(st)),
+               "d2g_compare_batch")
+        return CompareResult(lam, ham, skp, st.as_dict())
+
+    def fetch_results(self, n_centers: int) -> CompareResult:
+        shape = (n_centers, self.n)
+        lam = np.zeros(shape, np.float64)
+        ham = np.zeros(shape, np.uint32)
+        skp = np.zeros(shape, np.uint8)
+        _check(_lib().d2g_fetch_results(self.h, _p(lam, f64p), _p(ham, u32p), _p(skp, u8p)),
+               "d2g_fetch_results")
+        return CompareResult(lam, ham, skp, {})
+
+    def compare_center(self, center: int, err, params: AlignParams | None = None,
+                       use_quals=True, greedy=True, lock=None) -> CompareResult:
+        """The plain path: one center vs every unique (b_compare_parallel's map)."""
+        params = params or align_params()
+        e, lk = self._common(err, params, lock)
+        lam = np.zeros(self.n, np.float64)
+        ham = np.zeros(self.n, np.uint32)
+        skp = np.zeros(self.n, np.uint8)
+        st = Stats()
+        _check(_lib().d2g_compare_center(self.h, center, _p(e, f64p), e.shape[1],
+                                         C.byref(params), int(use_quals), int(greedy),
+                                         _p(lk, u8p) if lk is not None else None, _p(lam, f64p),
+                                         _p(ham, u32p), _p(skp, u8p), C.byref(st)),
+               "d2g_compare_center")
+        return CompareResult(lam, ham, skp, st.as_dict())
+
+    def align_vectorized(self, s1s, s2s, match=5, mismatch=-4, gap_p=-8, end_gap_p=0,
+                         band=16) -> list[tuple[bytes, bytes]]:
+        """align_vectorized_with_buf (nwalign.rs:761) for each (s1, s2) pair."""
+        if len(s1s) != len(s2s):
+            raise ValueError("s1s and s2s differ in length")
+        n = len(s1s)
+        if n == 0:
+            return []
+        c1, o1 = _concat(s1s)
+        c2, o2 = _concat(s2s)
+        l1 = np.diff(o1)
+        l2 = np.diff(o2)
+        aoff = np.zeros(n + 1, np.uint64)
+        aoff[1:] = np.cumsum(l1 + l2)
+        total = int(aoff[-1]) + 1
+        a0 = np.zeros(total, np.uint8)
+        a1 = np.zeros(total, np.uint8)
+        alen = np.zeros(n, np.uint32)
+        sc = VecScores(match, mismatch, gap_p, end_gap_p, band)
+        _check(_lib().d2g_align_vectorized(self.h, n, _p(c1, u8p), _p(o1, u64p), _p(c2, u8p),
+                                           _p(o2, u64p), C.byref(sc), _p(a0, u8p), _p(a1, u8p),
+                                           _p(aoff, u64p), _p(alen, u32p)),
+               "d2g_align_vectorized")
+        out = []
+        for i in range(n):
+            s, m = int(aoff[i]), int(alen[i])
+            out.append((a0[s:s + m].tobytes(), a1[s:s + m].tobytes()))
+        return out

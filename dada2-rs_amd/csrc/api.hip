@@ -1,0 +1,616 @@
+// api.hip — the C ABI declared in include/dada2_amd.h.
+//
+// A context owns one HIP stream and every device buffer.  The compare
+// pipeline per call is: screen -> plan -> scatter -> vectorized DP ->
+// lambda, all on the context stream; host work is limited to argument
+// checks, uploading the center list / lock flags / error matrix, and the
+// final copy of results (skipped by the device-resident entry point).
+#include <algorithm>
+#include <cstring>
+#include <numeric>
+#include <vector>
+
+#include "../../include/dada2_amd.h"
+#include "d2g_launch.h"
+
+using namespace d2g;
+
+namespace {
+
+template <class T>
+struct DevBuf {
+  T* p = nullptr;
+  size_t cap = 0;
+  hipError_t ensure(size_t n) {
+    if (n <= cap && p) return hipSuccess;
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    cap = 0;
+    size_t bytes = std::max<size_t>(n, 1) * sizeof(T);
+    hipError_t e = hipMalloc((void**)&p, bytes);
+    if (e == hipSuccess) cap = std::max<size_t>(n, 1);
+    return e;
+  }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    cap = 0;
+  }
+};
+
+constexpr int kEvents = 6;
+
+}  // namespace
+
+struct d2g_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  bool timing = false;
+  hipEvent_t ev[kEvents] = {};
+  int cu_count = 0, clock_khz = 0, mem_clock_khz = 0, mem_bus_width = 0;
+  // uniques
+  bool loaded = false;
+  UniqueDev u;
+  std::vector<uint32_t> h_len;   // sorted order
+  std::vector<uint32_t> h_orig;  // sorted -> original
+  std::vector<uint32_t> h_inv;   // original -> sorted
+  uint32_t max_qual = 0;
+  bool has_qual = false;
+  DevBuf<uint8_t> seq, qual, has255;
+  DevBuf<uint64_t> off;
+  DevBuf<uint32_t> len, reads, orig, kmer_row, kmer_il, kord_row, kord_il, ksum;
+  // per call
+  DevBuf<uint32_t> centers;
+  DevBuf<uint8_t> lock;
+  DevBuf<double> err;
+  DevBuf<double> out_lambda;
+  DevBuf<uint32_t> out_hamming;
+  DevBuf<uint8_t> out_skipped;
+  DevBuf<GaplessTask> gl;
+  DevBuf<DpTask> dp_unsorted, dp_sorted;
+  DevBuf<uint32_t> bin_count, bin_start, wave_start, bin_fill;
+  DevBuf<uint32_t> sub_pool, sub_off, sub_cnt, sub_top;
+  DevBuf<Counters> ctr;
+  size_t sub_capacity = 0;
+  uint32_t last_nc = 0;
+  // cached LDS sizing of the DP kernel for a bin space
+  BinSpace lds_key = {-1, -1, -1, -1};
+  size_t lds_bytes = 0;
+  // align mode buffers
+  DevBuf<uint8_t> a_pool, a_al0, a_al1;
+  DevBuf<uint64_t> a_aloff;
+  DevBuf<uint32_t> a_allen;
+};
+
+#define HIP_TRY(x)                    \
+  do {                                \
+    hipError_t e_ = (x);              \
+    if (e_ != hipSuccess) return D2G_ERR_HIP; \
+  } while (0)
+
+extern "C" {
+
+const char* d2g_status_string(int s) {
+  switch (s) {
+    case D2G_OK: return "ok";
+    case D2G_ERR_ARG: return "invalid argument";
+    case D2G_ERR_HIP: return "HIP runtime error";
+    case D2G_ERR_NONACGT: return "non-ACGT base in the compare path";
+    case D2G_ERR_TRACEBACK: return "invalid traceback pointer";
+    case D2G_ERR_UNSUPPORTED: return "geometry or mode not supported by the kernels";
+    case D2G_ERR_STATE: return "call order (uniques not loaded)";
+    default: return "unknown status";
+  }
+}
+
+int d2g_ctx_create(int device, d2g_ctx** out) {
+  if (!out) return D2G_ERR_ARG;
+  *out = nullptr;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || device < 0 || device >= ndev) return D2G_ERR_HIP;
+  d2g_ctx* c = new d2g_ctx();
+  c->device = device;
+  if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+    delete c;
+    return D2G_ERR_HIP;
+  }
+  for (int i = 0; i < kEvents; ++i) (void)hipEventCreate(&c->ev[i]);
+  hipDeviceProp_t prop;
+  if (hipGetDeviceProperties(&prop, device) == hipSuccess) {
+    c->cu_count = prop.multiProcessorCount;
+    c->clock_khz = prop.clockRate;
+    c->mem_clock_khz = prop.memoryClockRate;
+    c->mem_bus_width = prop.memoryBusWidth;
+  }
+  if (c->cu_count <= 0) c->cu_count = 256;
+  if (c->ctr.ensure(1) != hipSuccess) {
+    d2g_ctx_destroy(c);
+    return D2G_ERR_HIP;
+  }
+  *out = c;
+  return D2G_OK;
+}
+
+int d2g_ctx_destroy(d2g_ctx* c) {
+  if (!c) return D2G_OK;
+  (void)hipSetDevice(c->device);
+  if (c->stream) (void)hipStreamSynchronize(c->stream);
+  for (auto* b : {&c->seq, &c->qual, &c->has255, &c->lock, &c->out_skipped, &c->a_pool, &c->a_al0, &c->a_al1}) b->release();
+  for (auto* b : {&c->len, &c->reads, &c->orig, &c->kmer_row, &c->kmer_il, &c->kord_row, &c->kord_il,
+                  &c->ksum, &c->centers, &c->out_hamming, &c->bin_count, &c->bin_start, &c->wave_start,
+                  &c->bin_fill, &c->sub_pool, &c->sub_off, &c->sub_cnt, &c->sub_top, &c->a_allen})
+    b->release();
+  c->off.release();
+  c->a_aloff.release();
+  c->err.release();
+  c->out_lambda.release();
+  c->gl.release();
+  c->dp_unsorted.release();
+  c->dp_sorted.release();
+  c->ctr.release();
+  for (int i = 0; i < kEvents; ++i)
+    if (c->ev[i]) (void)hipEventDestroy(c->ev[i]);
+  if (c->stream) (void)hipStreamDestroy(c->stream);
+  delete c;
+  return D2G_OK;
+}
+
+int d2g_ctx_set_timing(d2g_ctx* c, int enable) {
+  if (!c) return D2G_ERR_ARG;
+  c->timing = enable != 0;
+  return D2G_OK;
+}
+
+void* d2g_ctx_stream(d2g_ctx* c) { return c ? (void*)c->stream : nullptr; }
+
+int d2g_device_info(d2g_ctx* c, int* cu, int* clk, int* mclk, int* bus) {
+  if (!c) return D2G_ERR_ARG;
+  if (cu) *cu = c->cu_count;
+  if (clk) *clk = c->clock_khz;
+  if (mclk) *mclk = c->mem_clock_khz;
+  if (bus) *bus = c->mem_bus_width;
+  return D2G_OK;
+}
+
+uint64_t d2g_vectorized_cells(uint32_t l1_in, uint32_t l2_in, int32_t band) {
+  // replay of the index walk (nwalign.rs:865-1088): sum of active widths
+  VecGeom g = vec_geom((int)l1_in, (int)l2_in, band);
+  const int len1 = g.len1, len2 = g.len2, b = g.band, sc = g.start_col;
+  int row = 2, col_min = sc, col_max = sc, i_max = 0;
+  bool even = true;
+  uint64_t total = 0;
+  while (row <= len1 + len2) {
+    int n = col_max - col_min + 1;
+    if (n > 0) total += (uint64_t)n;
+    if (row == imin(b, len1)) {
+      col_min = imax(col_min - 1, 0);
+      i_max += 1;
+    }
+    if (row == g.top_limit) col_max += 1;
+    if (row < b && row < len1) {
+      if (even) col_min = imax(col_min - 1, 0);
+      i_max += 1;
+    } else if (i_max < len1 - 1) {
+      if (b % 2 == 0) {
+        if (!even) i_max += 1;
+      } else if (even) {
+        col_min = imax(col_min - 1, 0);
+        i_max += 1;
+      } else {
+        col_min += 1;
+      }
+    } else if (!even) {
+      col_min += 1;
+    }
+    if (row < g.top_limit) {
+      if (!even) col_max += 1;
+    } else if ((row + 1) / 2 + col_max - sc < len2) {
+      if ((b + len2 - len1) % 2 == 0) {
+        if (even) col_max = imax(col_max - 1, 0);
+        else col_max += 1;
+      }
+    } else if (even) {
+      col_max = imax(col_max - 1, 0);
+    }
+    ++row;
+    even = !even;
+  }
+  return total;
+}
+
+int d2g_vectorized_supported(uint32_t l1, uint32_t l2, int32_t band) {
+  if (l1 > kVectorizedMaxLen || l2 > kVectorizedMaxLen) return 0;
+  return nw_groups(vec_geom((int)l1, (int)l2, band)) > 0 ? 1 : 0;
+}
+
+int d2g_load_uniques(d2g_ctx* c, uint32_t n, const uint8_t* seq_concat, const uint64_t* offsets,
+                     const uint8_t* qual_concat, const uint32_t* reads, uint32_t k) {
+  if (!c || !seq_concat || !offsets || !reads || n == 0) return D2G_ERR_ARG;
+  if (k < 3 || k > 8) return D2G_ERR_ARG;  // dada.rs:259-263
+  (void)hipSetDevice(c->device);
+  c->loaded = false;
+  std::vector<uint32_t> len(n);
+  uint32_t maxq = 0;
+  for (uint32_t i = 0; i < n; ++i) {
+    if (offsets[i + 1] < offsets[i]) return D2G_ERR_ARG;
+    uint64_t L = offsets[i + 1] - offsets[i];
+    if (L <= k || L >= 9999) return D2G_ERR_ARG;  // dada.rs:265-275 (minlen > k, SEQLEN)
+    len[i] = (uint32_t)L;
+    for (uint64_t p = offsets[i]; p < offsets[i + 1]; ++p)
+      if (seq_concat[p] < 1 || seq_concat[p] > 4) return D2G_ERR_NONACGT;  // dada.rs:279-294
+    if (qual_concat)
+      for (uint64_t p = offsets[i]; p < offsets[i + 1]; ++p) maxq = std::max<uint32_t>(maxq, qual_concat[p]);
+  }
+  // length-sorted packing (stable, so equal lengths keep the input order)
+  std::vector<uint32_t> order(n);
+  std::iota(order.begin(), order.end(), 0u);
+  std::stable_sort(order.begin(), order.end(), [&](uint32_t a, uint32_t b) { return len[a] < len[b]; });
+  std::vector<uint64_t> soff(n + 1, 0);
+  std::vector<uint32_t> slen(n), sreads(n), inv(n);
+  for (uint32_t s = 0; s < n; ++s) {
+    slen[s] = len[order[s]];
+    sreads[s] = reads[order[s]];
+    inv[order[s]] = s;
+    soff[s + 1] = soff[s] + slen[s];
+  }
+  const uint64_t total = soff[n];
+  std::vector<uint8_t> sseq(total), squal(qual_concat ? total : 0);
+  for (uint32_t s = 0; s < n; ++s) {
+    std::memcpy(&sseq[soff[s]], seq_concat + offsets[order[s]], slen[s]);
+    if (qual_concat) std::memcpy(&squal[soff[s]], qual_concat + offsets[order[s]], slen[s]);
+  }
+  UniqueDev& u = c->u;
+  u.n = n;
+  u.npad = (n + 63) & ~63u;
+  u.k = k;
+  u.kwords = (1u << (2 * k)) / 4;
+  u.minlen = slen[0];
+  u.maxlen = slen[n - 1];
+  u.owords = (u.maxlen - k + 1 + 1) / 2;
+  HIP_TRY(c->seq.ensure(total));
+  HIP_TRY(c->off.ensure(n + 1));
+  HIP_TRY(c->len.ensure(n));
+  HIP_TRY(c->reads.ensure(n));
+  HIP_TRY(c->orig.ensure(n));
+  HIP_TRY(c->kmer_row.ensure((size_t)n * u.kwords));
+  HIP_TRY(c->kmer_il.ensure((size_t)u.npad * u.kwords));
+  HIP_TRY(c->kord_row.ensure((size_t)n * u.owords));
+  HIP_TRY(c->kord_il.ensure((size_t)u.npad * u.owords));
+  HIP_TRY(c->ksum.ensure(n));
+  HIP_TRY(c->has255.ensure(n));
+  if (qual_concat) HIP_TRY(c->qual.ensure(total));
+  hipStream_t s = c->stream;
+  HIP_TRY(hipMemcpyAsync(c->seq.p, sseq.data(), total, hipMemcpyHostToDevice, s));
+  if (qual_concat) HIP_TRY(hipMemcpyAsync(c->qual.p, squal.data(), total, hipMemcpyHostToDevice, s));
+  HIP_TRY(hipMemcpyAsync(c->off.p, soff.data(), (n + 1) * sizeof(uint64_t), hipMemcpyHostToDevice, s));
+  HIP_TRY(hipMemcpyAsync(c->len.p, slen.data(), n * sizeof(uint32_t), hipMemcpyHostToDevice, s));
+  HIP_TRY(hipMemcpyAsync(c->reads.p, sreads.data(), n * sizeof(uint32_t), hipMemcpyHostToDevice, s));
+  HIP_TRY(hipMemcpyAsync(c->orig.p, order.data(), n * sizeof(uint32_t), hipMemcpyHostToDevice, s));
+  HIP_TRY(launch_zero(c->kord_row.p, (size_t)n * u.owords, s));
+  HIP_TRY(launch_zero(c->kord_il.p, (size_t)u.npad * u.owords, s));
+  HIP_TRY(launch_zero(c->kmer_il.p, (size_t)u.npad * u.kwords, s));
+  u.seq = c->seq.p;
+  u.qual = qual_concat ? c->qual.p : nullptr;
+  u.off = c->off.p;
+  u.len = c->len.p;
+  u.reads = c->reads.p;
+  u.orig = c->orig.p;
+  u.kmer_row = c->kmer_row.p;
+  u.kmer_il = c->kmer_il.p;
+  u.kord_row = c->kord_row.p;
+  u.kord_il = c->kord_il.p;
+  u.ksum = c->ksum.p;
+  u.has255 = c->has255.p;
+  HIP_TRY(launch_prep(u, s));
+  HIP_TRY(hipStreamSynchronize(s));
+  c->h_len = std::move(slen);
+  c->h_orig = std::move(order);
+  c->h_inv = std::move(inv);
+  c->max_qual = maxq;
+  c->has_qual = qual_concat != nullptr;
+  c->loaded = true;
+  return D2G_OK;
+}
+
+static float ev_ms(d2g_ctx* c, int a, int b) {
+  float ms = 0.f;
+  if (hipEventElapsedTime(&ms, c->ev[a], c->ev[b]) != hipSuccess) return 0.f;
+  return ms;
+}
+
+static int compare_device(d2g_ctx* c, const uint32_t* centers, uint32_t nc, const double* err,
+                          uint32_t ncol, const d2g_align_params* p, int use_quals, int greedy,
+                          const uint8_t* lock, d2g_stats* stats) {
+  if (!c || !centers || !err || !p || nc == 0 || ncol == 0) return D2G_ERR_ARG;
+  if (!c->loaded) return D2G_ERR_STATE;
+  (void)hipSetDevice(c->device);
+  UniqueDev& u = c->u;
+  const uint32_t n = u.n;
+  if ((uint32_t)p->kmer_size != u.k && p->use_kmers) return D2G_ERR_ARG;
+  if (use_quals && c->has_qual && c->max_qual >= ncol) return D2G_ERR_ARG;  // pval.rs:289, err extension dada.rs:337-384
+  // dispatch (nwalign.rs:1335-1403): only the vectorized aligner is on the GPU here
+  const bool use_homo = p->homo_gap_p != p->gap_p && p->homo_gap_p <= 0;
+  const bool too_long = u.maxlen > kVectorizedMaxLen;
+  if (p->band != 0 && (use_homo || !p->vectorized || too_long)) return D2G_ERR_UNSUPPORTED;
+  for (uint32_t i = 0; i < nc; ++i)
+    if (centers[i] >= n) return D2G_ERR_ARG;
+  const size_t npairs = (size_t)nc * n;
+  if (npairs > 0xFFFFFFFFull) return D2G_ERR_UNSUPPORTED;
+  BinSpace bs;
+  bs.lmin = (int)u.minlen;
+  bs.range = (int)(u.maxlen - u.minlen + 1);
+  bs.band = p->band;
+  if ((size_t)bs.range * bs.range > (1u << 22)) return D2G_ERR_UNSUPPORTED;
+  bs.nbins = bs.range * bs.range;
+  if (bs.lmin != c->lds_key.lmin || bs.range != c->lds_key.range || bs.band != c->lds_key.band) {
+    c->lds_bytes = nw_vec_lds_for(bs);
+    c->lds_key = bs;
+  }
+  std::vector<uint32_t> hc(nc);
+  for (uint32_t i = 0; i < nc; ++i) hc[i] = c->h_inv[centers[i]];
+  HIP_TRY(c->centers.ensure(nc));
+  HIP_TRY(c->err.ensure((size_t)16 * ncol));
+  HIP_TRY(c->out_lambda.ensure(npairs));
+  HIP_TRY(c->out_hamming.ensure(npairs));
+  HIP_TRY(c->out_skipped.ensure(npairs));
+  HIP_TRY(c->gl.ensure(npairs));
+  HIP_TRY(c->dp_unsorted.ensure(npairs));
+  HIP_TRY(c->dp_sorted.ensure(npairs));
+  HIP_TRY(c->sub_off.ensure(npairs));
+  HIP_TRY(c->sub_cnt.ensure(npairs));
+  HIP_TRY(c->bin_count.ensure(bs.nbins + 1));
+  HIP_TRY(c->bin_start.ensure(bs.nbins + 1));
+  HIP_TRY(c->wave_start.ensure(bs.nbins + 1));
+  HIP_TRY(c->bin_fill.ensure(bs.nbins + 1));
+  HIP_TRY(c->sub_top.ensure(1));
+  if (c->sub_capacity < (size_t)1 << 20) c->sub_capacity = (size_t)1 << 20;
+  HIP_TRY(c->sub_pool.ensure(c->sub_capacity));
+  hipStream_t s = c->stream;
+  const uint8_t* dlock = nullptr;
+  if (lock) {
+    std::vector<uint8_t> sl(n);
+    for (uint32_t i = 0; i < n; ++i) sl[i] = lock[c->h_orig[i]];
+    HIP_TRY(c->lock.ensure(n));
+    HIP_TRY(hipMemcpyAsync(c->lock.p, sl.data(), n, hipMemcpyHostToDevice, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    dlock = c->lock.p;
+  }
+  HIP_TRY(hipMemcpyAsync(c->centers.p, hc.data(), nc * sizeof(uint32_t), hipMemcpyHostToDevice, s));
+  HIP_TRY(hipMemcpyAsync(c->err.p, err, (size_t)16 * ncol * sizeof(double), hipMemcpyHostToDevice, s));
+  HIP_TRY(hipStreamSynchronize(s));
+
+  for (int attempt = 0; attempt < 6; ++attempt) {
+    if (c->timing) HIP_TRY(hipEventRecord(c->ev[0], s));
+    HIP_TRY(launch_zero(reinterpret_cast<uint32_t*>(c->ctr.p), sizeof(Counters) / 4, s));
+    HIP_TRY(launch_zero(c->bin_count.p, bs.nbins + 1, s));
+    HIP_TRY(launch_zero(c->sub_top.p, 1, s));
+    ScreenArgs sa;
+    sa.u = &u;
+    sa.centers = c->centers.p;
+    sa.n_centers = nc;
+    sa.lock = dlock;
+    sa.greedy = greedy;
+    sa.use_kmers = p->use_kmers;
+    sa.gapless = p->gapless;
+    sa.kdist_cutoff = p->kdist_cutoff;
+    sa.band = p->band;
+    sa.bins = bs;
+    sa.out_lambda = c->out_lambda.p;
+    sa.out_hamming = c->out_hamming.p;
+    sa.out_skipped = c->out_skipped.p;
+    sa.gapless_tasks = c->gl.p;
+    sa.dp_unsorted = c->dp_unsorted.p;
+    sa.bin_count = c->bin_count.p;
+    sa.ctr = c->ctr.p;
+    HIP_TRY(launch_screen(sa, s));
+    if (c->timing) HIP_TRY(hipEventRecord(c->ev[1], s));
+    HIP_TRY(launch_plan(bs, c->bin_count.p, c->bin_start.p, c->wave_start.p, c->bin_fill.p, c->ctr.p, s));
+    HIP_TRY(launch_scatter(c->dp_unsorted.p, c->dp_sorted.p, c->bin_start.p, c->bin_fill.p, c->ctr.p,
+                           (uint32_t)npairs, s));
+    if (c->timing) HIP_TRY(hipEventRecord(c->ev[2], s));
+    NwArgs na;
+    std::memset(&na, 0, sizeof(na));
+    na.bins = bs;
+    na.bin_start = c->bin_start.p;
+    na.wave_start = c->wave_start.p;
+    na.tasks = c->dp_sorted.p;
+    na.pool = u.seq;
+    na.match = (int16_t)p->match_score;
+    na.mismatch = (int16_t)p->mismatch;
+    na.gap_p = (int16_t)p->gap_p;
+    na.end_gap_p = 0;  // raw_align_dp always aligns ends-free (nwalign.rs:1393)
+    na.ctr = c->ctr.p;
+    na.emit_alignment = 0;
+    na.sub_pool = c->sub_pool.p;
+    na.sub_capacity = (uint32_t)std::min<size_t>(c->sub_capacity, 0xFFFFFFF0u);
+    na.sub_top = c->sub_top.p;
+    na.sub_off = c->sub_off.p;
+    na.sub_cnt = c->sub_cnt.p;
+    na.grid = (uint32_t)c->cu_count * 16;
+    na.lds_bytes = c->lds_bytes;
+    HIP_TRY(launch_nw_vec(na, s));
+    if (c->timing) HIP_TRY(hipEventRecord(c->ev[3], s));
+    LambdaArgs la;
+    la.u = &u;
+    la.gapless_tasks = c->gl.p;
+    la.dp_tasks = c->dp_sorted.p;
+    la.sub_pool = c->sub_pool.p;
+    la.sub_off = c->sub_off.p;
+    la.sub_cnt = c->sub_cnt.p;
+    la.err = c->err.p;
+    la.ncol = ncol;
+    la.use_quals = use_quals;
+    la.ctr = c->ctr.p;
+    la.out_lambda = c->out_lambda.p;
+    la.out_hamming = c->out_hamming.p;
+    la.out_skipped = c->out_skipped.p;
+    la.grid = (uint32_t)c->cu_count * 4;
+    HIP_TRY(launch_lambda(la, s));
+    if (c->timing) HIP_TRY(hipEventRecord(c->ev[4], s));
+    Counters h;
+    HIP_TRY(hipMemcpyAsync(&h, c->ctr.p, sizeof(Counters), hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    if (h.err_flags & kErrSubOverflow) {
+      c->sub_capacity *= 4;
+      HIP_TRY(c->sub_pool.ensure(c->sub_capacity));
+      continue;
+    }
+    c->last_nc = nc;
+    if (stats) {
+      std::memset(stats, 0, sizeof(*stats));
+      stats->comparisons = npairs;
+      stats->skipped = h.n_skipped;
+      stats->shrouded = h.n_shrouded;
+      stats->gapless = h.n_gapless;
+      stats->aligned_dp = h.n_dp;
+      stats->dp_cells = h.dp_cells;
+      if (c->timing) {
+        stats->ms_screen = ev_ms(c, 0, 1);
+        stats->ms_plan = ev_ms(c, 1, 2);
+        stats->ms_dp = ev_ms(c, 2, 3);
+        stats->ms_lambda = ev_ms(c, 3, 4);
+        stats->ms_total = ev_ms(c, 0, 4);
+      }
+    }
+    if (h.err_flags & kErrTraceback) return D2G_ERR_TRACEBACK;
+    if (h.err_flags & kErrGeometry) return D2G_ERR_UNSUPPORTED;
+    return D2G_OK;
+  }
+  return D2G_ERR_HIP;
+}
+
+int d2g_compare_batch_device(d2g_ctx* c, const uint32_t* centers, uint32_t nc, const double* err,
+                             uint32_t ncol, const d2g_align_params* p, int use_quals, int greedy,
+                             const uint8_t* lock, d2g_stats* stats) {
+  return compare_device(c, centers, nc, err, ncol, p, use_quals, greedy, lock, stats);
+}
+
+int d2g_fetch_results(d2g_ctx* c, double* out_lambda, uint32_t* out_hamming, uint8_t* out_skipped) {
+  if (!c || !c->loaded || c->last_nc == 0) return D2G_ERR_STATE;
+  (void)hipSetDevice(c->device);
+  const size_t np = (size_t)c->last_nc * c->u.n;
+  hipStream_t s = c->stream;
+  if (out_lambda) HIP_TRY(hipMemcpyAsync(out_lambda, c->out_lambda.p, np * sizeof(double), hipMemcpyDeviceToHost, s));
+  if (out_hamming) HIP_TRY(hipMemcpyAsync(out_hamming, c->out_hamming.p, np * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+  if (out_skipped) HIP_TRY(hipMemcpyAsync(out_skipped, c->out_skipped.p, np, hipMemcpyDeviceToHost, s));
+  HIP_TRY(hipStreamSynchronize(s));
+  return D2G_OK;
+}
+
+int d2g_compare_batch(d2g_ctx* c, const uint32_t* centers, uint32_t nc, const double* err,
+                      uint32_t ncol, const d2g_align_params* p, int use_quals, int greedy,
+                      const uint8_t* lock, double* out_lambda, uint32_t* out_hamming,
+                      uint8_t* out_skipped, d2g_stats* stats) {
+  int rc = compare_device(c, centers, nc, err, ncol, p, use_quals, greedy, lock, stats);
+  if (rc != D2G_OK) return rc;
+  return d2g_fetch_results(c, out_lambda, out_hamming, out_skipped);
+}
+
+int d2g_compare_center(d2g_ctx* c, uint32_t center, const double* err, uint32_t ncol,
+                       const d2g_align_params* p, int use_quals, int greedy, const uint8_t* lock,
+                       double* out_lambda, uint32_t* out_hamming, uint8_t* out_skipped,
+                       d2g_stats* stats) {
+  return d2g_compare_batch(c, &center, 1, err, ncol, p, use_quals, greedy, lock, out_lambda,
+                           out_hamming, out_skipped, stats);
+}
+
+int d2g_align_vectorized(d2g_ctx* c, uint32_t n, const uint8_t* s1, const uint64_t* s1_off,
+                         const uint8_t* s2, const uint64_t* s2_off, const d2g_vec_scores* sc,
+                         uint8_t* al0, uint8_t* al1, const uint64_t* al_offsets, uint32_t* al_len) {
+  if (!c || !s1 || !s1_off || !s2 || !s2_off || !sc || !al0 || !al1 || !al_len) return D2G_ERR_ARG;
+  if (n == 0) return D2G_OK;
+  (void)hipSetDevice(c->device);
+  // pool: all s1 then all s2
+  std::vector<uint32_t> l1(n), l2(n);
+  uint32_t lmin = 0xFFFFFFFFu, lmax = 0;
+  for (uint32_t i = 0; i < n; ++i) {
+    if (s1_off[i + 1] < s1_off[i] || s2_off[i + 1] < s2_off[i]) return D2G_ERR_ARG;
+    l1[i] = (uint32_t)(s1_off[i + 1] - s1_off[i]);
+    l2[i] = (uint32_t)(s2_off[i + 1] - s2_off[i]);
+    if (l1[i] > kVectorizedMaxLen || l2[i] > kVectorizedMaxLen) return D2G_ERR_UNSUPPORTED;
+    lmin = std::min({lmin, l1[i], l2[i]});
+    lmax = std::max({lmax, l1[i], l2[i]});
+  }
+  const uint64_t n1 = s1_off[n] - s1_off[0], n2 = s2_off[n] - s2_off[0];
+  std::vector<uint64_t> aoff(n + 1, 0);
+  for (uint32_t i = 0; i < n; ++i) aoff[i + 1] = aoff[i] + l1[i] + l2[i];
+  if (al_offsets)
+    for (uint32_t i = 0; i < n; ++i) aoff[i] = al_offsets[i];
+  uint64_t al_total = 0;
+  for (uint32_t i = 0; i < n; ++i) al_total = std::max<uint64_t>(al_total, aoff[i] + l1[i] + l2[i]);
+  BinSpace bs;
+  bs.lmin = (int)lmin;
+  bs.range = (int)(lmax - lmin + 1);
+  bs.band = sc->band;
+  if ((size_t)bs.range * bs.range > (1u << 22)) return D2G_ERR_UNSUPPORTED;
+  bs.nbins = bs.range * bs.range;
+  std::vector<DpTask> tasks(n);
+  std::vector<uint32_t> counts(bs.nbins + 1, 0);
+  for (uint32_t i = 0; i < n; ++i) {
+    DpTask& t = tasks[i];
+    t.s1_off = s1_off[i] - s1_off[0];
+    t.s2_off = n1 + (s2_off[i] - s2_off[0]);
+    t.out_idx = i;
+    t.bin = (l1[i] - lmin) * (uint32_t)bs.range + (l2[i] - lmin);
+    t.u_idx = 0;
+    t.c_idx = 0;
+    counts[t.bin] += 1;
+    VecGeom g = vec_geom((int)l1[i], (int)l2[i], bs.band);
+    if (nw_groups(g) == 0) return D2G_ERR_UNSUPPORTED;
+  }
+  hipStream_t s = c->stream;
+  HIP_TRY(c->a_pool.ensure(n1 + n2 + 1));
+  HIP_TRY(c->dp_unsorted.ensure(n));
+  HIP_TRY(c->dp_sorted.ensure(n));
+  HIP_TRY(c->bin_count.ensure(bs.nbins + 1));
+  HIP_TRY(c->bin_start.ensure(bs.nbins + 1));
+  HIP_TRY(c->wave_start.ensure(bs.nbins + 1));
+  HIP_TRY(c->bin_fill.ensure(bs.nbins + 1));
+  HIP_TRY(c->a_al0.ensure(al_total + 1));
+  HIP_TRY(c->a_al1.ensure(al_total + 1));
+  HIP_TRY(c->a_aloff.ensure(n));
+  HIP_TRY(c->a_allen.ensure(n));
+  HIP_TRY(hipMemcpyAsync(c->a_pool.p, s1 + s1_off[0], n1, hipMemcpyHostToDevice, s));
+  HIP_TRY(hipMemcpyAsync(c->a_pool.p + n1, s2 + s2_off[0], n2, hipMemcpyHostToDevice, s));
+  HIP_TRY(hipMemcpyAsync(c->dp_unsorted.p, tasks.data(), n * sizeof(DpTask), hipMemcpyHostToDevice, s));
+  HIP_TRY(hipMemcpyAsync(c->bin_count.p, counts.data(), (bs.nbins + 1) * sizeof(uint32_t), hipMemcpyHostToDevice, s));
+  HIP_TRY(hipMemcpyAsync(c->a_aloff.p, aoff.data(), n * sizeof(uint64_t), hipMemcpyHostToDevice, s));
+  Counters h0;
+  std::memset(&h0, 0, sizeof(h0));
+  h0.n_dp = n;
+  HIP_TRY(hipMemcpyAsync(c->ctr.p, &h0, sizeof(Counters), hipMemcpyHostToDevice, s));
+  HIP_TRY(hipStreamSynchronize(s));
+  HIP_TRY(launch_plan(bs, c->bin_count.p, c->bin_start.p, c->wave_start.p, c->bin_fill.p, c->ctr.p, s));
+  HIP_TRY(launch_scatter(c->dp_unsorted.p, c->dp_sorted.p, c->bin_start.p, c->bin_fill.p, c->ctr.p, n, s));
+  NwArgs na;
+  std::memset(&na, 0, sizeof(na));
+  na.bins = bs;
+  na.bin_start = c->bin_start.p;
+  na.wave_start = c->wave_start.p;
+  na.tasks = c->dp_sorted.p;
+  na.pool = c->a_pool.p;
+  na.match = sc->match_score;
+  na.mismatch = sc->mismatch;
+  na.gap_p = sc->gap_p;
+  na.end_gap_p = sc->end_gap_p;
+  na.ctr = c->ctr.p;
+  na.emit_alignment = 1;
+  na.al0 = c->a_al0.p;
+  na.al1 = c->a_al1.p;
+  na.al_off = c->a_aloff.p;
+  na.al_len = c->a_allen.p;
+  na.grid = (uint32_t)c->cu_count * 16;
+  na.lds_bytes = nw_vec_lds_for(bs);
+  HIP_TRY(launch_nw_vec(na, s));
+  Counters h;
+  HIP_TRY(hipMemcpyAsync(&h, c->ctr.p, sizeof(Counters), hipMemcpyDeviceToHost, s));
+  HIP_TRY(hipMemcpyAsync(al_len, c->a_allen.p, n * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+  HIP_TRY(hipMemcpyAsync(al0, c->a_al0.p, al_total, hipMemcpyDeviceToHost, s));
+  HIP_TRY(hipMemcpyAsync(al1, c->a_al1.p, al_total, hipMemcpyDeviceToHost, s));
+  HIP_TRY(hipStreamSynchronize(s));
+  if (h.err_flags & kErrTraceback) return D2G_ERR_TRACEBACK;
+  if (h.err_flags & kErrGeometry) return D2G_ERR_UNSUPPORTED;
+  return D2G_OK;
+}
+
+}  // extern "C"

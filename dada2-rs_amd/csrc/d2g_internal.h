@@ -1,0 +1,123 @@
+// d2g_internal.h — records and geometry shared by the host driver and the
+// gfx950 kernels of the dada2-rs compare path.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#define D2G_HD __host__ __device__ __forceinline__
+
+namespace d2g {
+
+constexpr int kWave = 64;
+constexpr uint32_t kHammingNone = 0xFFFFFFFFu;
+// nwalign.rs:1383: above this the reference leaves the i16 vectorized path.
+constexpr uint32_t kVectorizedMaxLen = 3500;
+// error flags raised by kernels (OR-ed into a device word)
+constexpr uint32_t kErrTraceback = 1u;
+constexpr uint32_t kErrGeometry = 2u;
+constexpr uint32_t kErrQual = 4u;
+constexpr uint32_t kErrSubOverflow = 8u;
+
+D2G_HD int imin(int a, int b) { return a < b ? a : b; }
+D2G_HD int imax(int a, int b) { return a > b ? a : b; }
+D2G_HD int sat16(int v) { return v > 32767 ? 32767 : (v < -32768 ? -32768 : v); }
+
+// Compressed anti-diagonal geometry of align_vectorized_with_buf
+// (nwalign.rs:775-787, 833-838).  s1 is the shorter input after the swap.
+struct VecGeom {
+  int len1, len2, band, start_col, ncol, nrow, swap;
+  int left_fill_limit, top_fill_limit, top_limit;
+};
+
+D2G_HD VecGeom vec_geom(int l1_in, int l2_in, int band_in) {
+  VecGeom g;
+  g.swap = l1_in > l2_in;
+  g.len1 = g.swap ? l2_in : l1_in;
+  g.len2 = g.swap ? l1_in : l2_in;
+  g.band = band_in < 0 ? g.len2 : band_in;
+  int m = imin(g.band, g.len1);
+  g.start_col = 1 + (m + 1) / 2;
+  g.top_limit = imin(g.band + g.len2 - g.len1, g.len2);
+  g.ncol = 2 + g.start_col + g.top_limit / 2;
+  g.nrow = g.len1 + g.len2 + 1;
+  g.left_fill_limit = 1 + m;
+  g.top_fill_limit = 1 + g.top_limit;
+  return g;
+}
+
+// Pairs of one geometry share a wavefront: G = 64 / ncol groups of ncol
+// lanes, lane c of a group holding compressed column c.
+D2G_HD int vec_groups(int ncol) { return ncol <= kWave ? kWave / ncol : 0; }
+
+// LDS budget of one DP wavefront: nrow 16-byte pointer records, then per
+// pair the padded sequences and its substitution list.
+constexpr int kNwMaxGroups = 4;
+constexpr size_t kNwLdsSoft = 64 * 1024;
+constexpr size_t kNwLdsHard = 160 * 1024 - 1024;
+
+D2G_HD int nw_pad(int ncol) { return ((ncol + 4) + 3) & ~3; }
+D2G_HD size_t nw_group_bytes(const VecGeom& g) {
+  size_t seq = (size_t)g.len1 + g.len2 + 4 * nw_pad(g.ncol);
+  seq = (seq + 15) & ~(size_t)15;
+  return seq + (size_t)(g.len1 + 1) * 4;  // substitution list
+}
+D2G_HD size_t nw_wave_bytes(const VecGeom& g, int G) {
+  return (size_t)g.nrow * 16 + (size_t)G * nw_group_bytes(g);
+}
+// Pairs per wavefront for one geometry (0 = unsupported).
+D2G_HD int nw_groups(const VecGeom& g) {
+  int G = vec_groups(g.ncol);
+  if (G > kNwMaxGroups) G = kNwMaxGroups;
+  while (G > 1 && nw_wave_bytes(g, G) > kNwLdsSoft) --G;
+  if (G >= 1 && nw_wave_bytes(g, G) > kNwLdsHard) G = 0;
+  return G;
+}
+
+// One pair queued for the vectorized DP.  Sequences live in a byte pool; the
+// bin groups pairs of identical (len1_in, len2_in) so a wavefront's row
+// walk is uniform.
+struct DpTask {
+  uint64_t s1_off;   // s1_in (the center in the compare path)
+  uint64_t s2_off;   // s2_in (the query)
+  uint32_t out_idx;  // compare: c_slot * n + original unique index; align: pair index
+  uint32_t bin;
+  uint32_t u_idx;    // compare: query (sorted index)
+  uint32_t c_idx;    // compare: center (sorted index)
+};
+
+// One pair routed to the gapless aligner (nwalign.rs:1324-1327).
+struct GaplessTask {
+  uint32_t c_idx;
+  uint32_t u_idx;
+  uint32_t out_idx;
+  uint32_t _pad;
+};
+
+// Binning of DP tasks: bin = (len1_in - lmin) * range + (len2_in - lmin).
+struct BinSpace {
+  int lmin;
+  int range;
+  int nbins;
+  int band;
+};
+
+D2G_HD void bin_lengths(const BinSpace& bs, uint32_t bin, int* l1, int* l2) {
+  *l1 = bs.lmin + (int)(bin / (uint32_t)bs.range);
+  *l2 = bs.lmin + (int)(bin % (uint32_t)bs.range);
+}
+
+// Device-side counters of one compare call.
+struct Counters {
+  uint32_t n_gapless;
+  uint32_t n_dp;
+  uint32_t n_skipped;
+  uint32_t n_shrouded;
+  uint32_t n_exact_fallback;  // kmer_dist8 overflow pairs (kmers.rs:259-266)
+  uint32_t err_flags;
+  uint32_t wave_counter;      // persistent DP kernel work counter
+  uint32_t total_waves;
+  unsigned long long dp_cells;
+};
+
+}  // namespace d2g

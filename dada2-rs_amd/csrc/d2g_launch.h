@@ -1,0 +1,111 @@
+// d2g_launch.h — host-side launchers of the gfx950 kernels (one per .hip
+// translation unit), used by the C ABI in api.hip.
+#pragma once
+
+#include "d2g_internal.h"
+
+namespace d2g {
+
+// ---- prep.hip ----------------------------------------------------------
+// Builds the resident screen of every unique (raw_assign_kmers,
+// kmers.rs:213-221) in two layouts: row-major (centers, read with scalar
+// loads) and unique-interleaved (queries, coalesced across lanes).
+struct UniqueDev {
+  uint32_t n = 0, npad = 0, k = 0;
+  uint32_t kwords = 0;      // 4^k / 4 words of the dense kmer8 vector
+  uint32_t owords = 0;      // ceil((maxlen - k + 1) / 2) words of kord
+  uint32_t minlen = 0, maxlen = 0;
+  uint8_t* seq = nullptr;   // sorted-order concatenation
+  uint8_t* qual = nullptr;  // nullable
+  uint64_t* off = nullptr;  // n + 1
+  uint32_t* len = nullptr;
+  uint32_t* reads = nullptr;
+  uint32_t* orig = nullptr;  // sorted -> original index
+  uint32_t* kmer_row = nullptr;  // [n][kwords]
+  uint32_t* kmer_il = nullptr;   // [kwords][npad]
+  uint32_t* kord_row = nullptr;  // [n][owords]
+  uint32_t* kord_il = nullptr;   // [owords][npad]
+  uint32_t* ksum = nullptr;      // sum of kmer8 bytes
+  uint8_t* has255 = nullptr;
+  uint8_t* bad_base = nullptr;   // per unique: a base outside 1..4
+};
+
+hipError_t launch_prep(const UniqueDev& u, hipStream_t s);
+
+// ---- screen.hip --------------------------------------------------------
+struct ScreenArgs {
+  const UniqueDev* u;  // host copy of the pointers
+  const uint32_t* centers;  // sorted indices, n_centers
+  uint32_t n_centers;
+  const uint8_t* lock;  // sorted order, nullable
+  int greedy;
+  int use_kmers;
+  int gapless;
+  double kdist_cutoff;
+  int band;
+  BinSpace bins;
+  double* out_lambda;
+  uint32_t* out_hamming;
+  uint8_t* out_skipped;
+  GaplessTask* gapless_tasks;
+  DpTask* dp_unsorted;
+  uint32_t* bin_count;
+  Counters* ctr;
+};
+hipError_t launch_screen(const ScreenArgs& a, hipStream_t s);
+
+// ---- plan.hip ----------------------------------------------------------
+hipError_t launch_zero(uint32_t* p, size_t n_words, hipStream_t s);
+hipError_t launch_plan(const BinSpace& bs, const uint32_t* bin_count, uint32_t* bin_start,
+                       uint32_t* wave_start, uint32_t* bin_fill, Counters* ctr, hipStream_t s);
+hipError_t launch_scatter(const DpTask* unsorted, DpTask* sorted, const uint32_t* bin_start,
+                          uint32_t* bin_fill, const Counters* ctr, uint32_t capacity,
+                          hipStream_t s);
+
+// ---- nw_vec.hip --------------------------------------------------------
+struct NwArgs {
+  BinSpace bins;
+  const uint32_t* bin_start;   // nbins + 1
+  const uint32_t* wave_start;  // nbins + 1
+  const DpTask* tasks;         // sorted by bin
+  const uint8_t* pool;         // sequence bytes
+  int16_t match, mismatch, gap_p, end_gap_p;
+  Counters* ctr;
+  int emit_alignment;
+  // compare mode: per sorted task, substitution list (query_pos << 8 | trans)
+  uint32_t* sub_pool;
+  uint32_t sub_capacity;
+  uint32_t* sub_top;       // bump allocator
+  uint32_t* sub_off;       // per task
+  uint32_t* sub_cnt;       // per task
+  // align mode
+  uint8_t* al0;
+  uint8_t* al1;
+  const uint64_t* al_off;  // per pair (out_idx)
+  uint32_t* al_len;        // per pair
+  uint32_t grid;           // persistent workgroups
+  size_t lds_bytes;        // from nw_vec_lds_for
+};
+hipError_t launch_nw_vec(const NwArgs& a, hipStream_t s);
+size_t nw_vec_lds_for(const BinSpace& bs);
+
+// ---- lambda.hip --------------------------------------------------------
+struct LambdaArgs {
+  const UniqueDev* u;
+  const GaplessTask* gapless_tasks;
+  const DpTask* dp_tasks;  // sorted
+  const uint32_t* sub_pool;
+  const uint32_t* sub_off;
+  const uint32_t* sub_cnt;
+  const double* err;
+  uint32_t ncol;
+  int use_quals;
+  Counters* ctr;
+  double* out_lambda;
+  uint32_t* out_hamming;
+  uint8_t* out_skipped;
+  uint32_t grid;
+};
+hipError_t launch_lambda(const LambdaArgs& a, hipStream_t s);
+
+}  // namespace d2g

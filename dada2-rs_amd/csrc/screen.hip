@@ -1,0 +1,245 @@
+// screen.hip — the k-mer screen and method dispatch of the compare.
+//
+// Restates, per (center, unique) pair, cluster.rs:294-300 (greedy skip),
+// raw_align_with_buf (nwalign.rs:1261-1301: kmer_dist8 with the u16
+// overflow fallback, kord_dist when gapless, the strict `kdist > cutoff`
+// shroud) and the first dispatch test of raw_align_dp (nwalign.rs:1324:
+// band == 0 or |kodist - kdist| < f64::EPSILON -> gapless).
+//
+// Layout: a 256-lane workgroup owns 256 length-sorted uniques and CB centers.
+// Query vectors are read coalesced from the interleaved layout once per
+// workgroup; center vectors come through scalar loads from the row layout.
+// sum(min(a, b)) over the 4^k buckets is (sum a + sum b - sum |a - b|) / 2,
+// with sum |a - b| from v_sad_u8 (four buckets per instruction).
+//
+// Pairs that pass are appended (wave-aggregated atomics) to the gapless list
+// or to the DP list with their (len_center, len_query) bin.
+#include <float.h>
+
+#include "d2g_launch.h"
+
+namespace d2g {
+
+constexpr int kScreenBlock = 256;
+constexpr int kScreenCenters = 16;
+constexpr int kScreenChunk = 16;
+
+struct ScreenDev {
+  UniqueDev u;
+  const uint32_t* centers;
+  uint32_t n_centers;
+  const uint8_t* lock;
+  int greedy, use_kmers, gapless, band;
+  double kdist_cutoff;
+  BinSpace bins;
+  double* out_lambda;
+  uint32_t* out_hamming;
+  uint8_t* out_skipped;
+  GaplessTask* gapless_tasks;
+  DpTask* dp_unsorted;
+  uint32_t* bin_count;
+  Counters* ctr;
+};
+
+__device__ __forceinline__ uint32_t lane_id() { return __lane_id(); }
+
+// All lanes of the wavefront must call this (it is a wave collective).
+__device__ __forceinline__ uint32_t wave_append(uint32_t* counter, bool pred, bool* ok) {
+  uint64_t mask = __builtin_amdgcn_ballot_w64(pred);
+  uint32_t base = 0;
+  if (mask) {
+    int leader = __builtin_ffsll((long long)mask) - 1;
+    uint32_t cnt = (uint32_t)__builtin_popcountll(mask);
+    if ((int)lane_id() == leader) base = atomicAdd(counter, cnt);
+    base = __shfl(base, leader);
+  }
+  uint64_t below = mask & ((1ull << lane_id()) - 1ull);
+  *ok = pred;
+  return base + (uint32_t)__builtin_popcountll(below);
+}
+
+__device__ __forceinline__ uint16_t kord_il_at(const UniqueDev& u, uint32_t q, uint32_t s) {
+  uint32_t w = u.kord_il[(size_t)(s >> 1) * u.npad + q];
+  return (uint16_t)((s & 1) ? (w >> 16) : (w & 0xFFFF));
+}
+__device__ __forceinline__ uint16_t kord_row_at(const UniqueDev& u, uint32_t q, uint32_t s) {
+  return reinterpret_cast<const uint16_t*>(u.kord_row + (size_t)q * u.owords)[s];
+}
+
+// kmer_dist (kmers.rs:233-241) numerator sum_x min(cnt_a[x], cnt_b[x]) from
+// the positional k-mer lists (all windows valid: bases are checked at load).
+// Quadratic and lane-serial: reached only when kmer_dist8 saturates
+// (a k-mer seen >= 255 times in both sequences).
+__device__ uint32_t u16_min_dot(const UniqueDev& u, uint32_t qa, uint32_t qb) {
+  uint32_t ka = u.len[qa] - u.k + 1, kb = u.len[qb] - u.k + 1;
+  uint32_t dot = 0;
+  for (uint32_t i = 0; i < ka; ++i) {
+    uint16_t x = kord_il_at(u, qa, i);
+    bool seen = false;
+    for (uint32_t j = 0; j < i && !seen; ++j) seen = kord_il_at(u, qa, j) == x;
+    if (seen) continue;
+    uint32_t ca = 0, cb = 0;
+    for (uint32_t j = i; j < ka; ++j) ca += kord_il_at(u, qa, j) == x;
+    for (uint32_t j = 0; j < kb; ++j) cb += kord_row_at(u, qb, j) == x;
+    dot += ca < cb ? ca : cb;
+  }
+  return dot;
+}
+
+__global__ __launch_bounds__(kScreenBlock) void screen_kernel(ScreenDev a) {
+  const UniqueDev& u = a.u;
+  const uint32_t q = blockIdx.x * kScreenBlock + threadIdx.x;
+  const bool valid = q < u.n;
+  const uint32_t qq = valid ? q : u.n - 1;
+  const uint32_t c0 = blockIdx.y * kScreenCenters;
+  const uint32_t nc = min((uint32_t)kScreenCenters, a.n_centers - c0);
+  const uint32_t k = u.k;
+
+  uint32_t cidx[kScreenCenters];
+#pragma unroll
+  for (int c = 0; c < kScreenCenters; ++c) cidx[c] = a.centers[c0 + ((uint32_t)c < nc ? c : 0)];
+
+  const uint32_t my_len = u.len[qq];
+  const uint32_t my_reads = u.reads[qq];
+  const bool my_lock = a.lock ? a.lock[qq] != 0 : false;
+  const uint32_t my_sum = u.ksum[qq];
+  const bool my_255 = u.has255[qq] != 0;
+
+  // sum |a - b| over the dense vectors, four buckets per v_sad_u8
+  uint32_t sad[kScreenCenters];
+#pragma unroll
+  for (int c = 0; c < kScreenCenters; ++c) sad[c] = 0;
+  if (a.use_kmers) {
+    for (uint32_t w0 = 0; w0 < u.kwords; w0 += kScreenChunk) {
+      uint32_t qw[kScreenChunk];
+#pragma unroll
+      for (int t = 0; t < kScreenChunk; ++t) qw[t] = u.kmer_il[(size_t)(w0 + t) * u.npad + qq];
+#pragma unroll
+      for (int c = 0; c < kScreenCenters; ++c) {
+        if ((uint32_t)c < nc) {
+          const uint32_t* cw = u.kmer_row + (size_t)cidx[c] * u.kwords + w0;
+#pragma unroll
+          for (int t = 0; t < kScreenChunk; ++t) sad[c] = __builtin_amdgcn_sad_u8(qw[t], cw[t], sad[c]);
+        }
+      }
+    }
+  }
+
+  uint32_t n_skip = 0, n_shroud = 0;
+  for (uint32_t c = 0; c < nc; ++c) {
+    const uint32_t ci = cidx[c];
+    const uint32_t clen = u.len[ci];
+    const uint32_t creads = u.reads[ci];
+    const uint32_t out_idx = (c0 + c) * u.n + u.orig[qq];
+    // cluster.rs:295: greedy skip
+    const bool skip = a.greedy && (my_reads > creads || my_lock);
+    double kdist = 0.0;
+    if (a.use_kmers) {
+      uint32_t dot = (my_sum + u.ksum[ci] - sad[c]) >> 1;
+      if (my_255 && u.has255[ci] && !skip) {
+        // exact kmer_dist8 (kmers.rs:254-269): overflow when min == 255
+        bool ovf = false;
+        uint32_t d2 = 0;
+        for (uint32_t w = 0; w < u.kwords; ++w) {
+          uint32_t x = u.kmer_il[(size_t)w * u.npad + qq];
+          uint32_t y = u.kmer_row[(size_t)ci * u.kwords + w];
+          for (int b = 0; b < 4; ++b) {
+            uint32_t m = min((x >> (8 * b)) & 255u, (y >> (8 * b)) & 255u);
+            ovf |= m == 255;
+            d2 += m;
+          }
+        }
+        dot = ovf ? u16_min_dot(u, qq, ci) : d2;  // nwalign.rs:1270-1278
+        if (ovf && valid) atomicAdd(&a.ctr->n_exact_fallback, 1u);
+      }
+      uint32_t mn = my_len < clen ? my_len : clen;
+      kdist = 1.0 - (double)dot / (double)(mn - k + 1);
+    }
+    const bool shroud = !skip && a.use_kmers && kdist > a.kdist_cutoff;
+    bool route_gapless = false;
+    if (!skip && !shroud) {
+      double kodist = -1.0;
+      if (a.use_kmers && a.gapless && my_len == clen) {
+        // kord_dist, kmers.rs:314-328
+        uint32_t klen = my_len - (k - 1);
+        uint32_t eq = 0;
+        uint32_t nw = (klen + 1) / 2;
+        const uint32_t* crow = u.kord_row + (size_t)ci * u.owords;
+        for (uint32_t w = 0; w < nw; ++w) {
+          uint32_t x = u.kord_il[(size_t)w * u.npad + qq] ^ crow[w];
+          eq += (x & 0xFFFFu) == 0;
+          if (2 * w + 1 < klen) eq += (x >> 16) == 0;
+        }
+        kodist = 1.0 - (double)eq / (double)klen;
+      }
+      route_gapless = a.band == 0 || fabs(kodist - kdist) < DBL_EPSILON;
+    }
+    const bool route_dp = !skip && !shroud && !route_gapless;
+    if (valid && (skip || shroud)) {
+      a.out_lambda[out_idx] = 0.0;
+      a.out_hamming[out_idx] = kHammingNone;
+      a.out_skipped[out_idx] = skip ? 1 : 0;
+    }
+    n_skip += valid && skip;
+    n_shroud += valid && shroud;
+    bool ok;
+    uint32_t gpos = wave_append(&a.ctr->n_gapless, valid && route_gapless, &ok);
+    if (ok) {
+      GaplessTask t;
+      t.c_idx = ci;
+      t.u_idx = qq;
+      t.out_idx = out_idx;
+      t._pad = 0;
+      a.gapless_tasks[gpos] = t;
+    }
+    uint32_t dpos = wave_append(&a.ctr->n_dp, valid && route_dp, &ok);
+    if (ok) {
+      DpTask t;
+      t.s1_off = u.off[ci];
+      t.s2_off = u.off[qq];
+      t.out_idx = out_idx;
+      t.bin = (clen - (uint32_t)a.bins.lmin) * (uint32_t)a.bins.range + (my_len - (uint32_t)a.bins.lmin);
+      t.u_idx = qq;
+      t.c_idx = ci;
+      a.dp_unsorted[dpos] = t;
+      atomicAdd(&a.bin_count[t.bin], 1u);
+    }
+  }
+  // wave-level counter updates
+  uint32_t s1 = n_skip, s2 = n_shroud;
+  for (int off = 32; off > 0; off >>= 1) {
+    s1 += __shfl_xor(s1, off);
+    s2 += __shfl_xor(s2, off);
+  }
+  if (lane_id() == 0) {
+    if (s1) atomicAdd(&a.ctr->n_skipped, s1);
+    if (s2) atomicAdd(&a.ctr->n_shrouded, s2);
+  }
+}
+
+hipError_t launch_screen(const ScreenArgs& s, hipStream_t st) {
+  ScreenDev d;
+  d.u = *s.u;
+  d.centers = s.centers;
+  d.n_centers = s.n_centers;
+  d.lock = s.lock;
+  d.greedy = s.greedy;
+  d.use_kmers = s.use_kmers;
+  d.gapless = s.gapless;
+  d.band = s.band;
+  d.kdist_cutoff = s.kdist_cutoff;
+  d.bins = s.bins;
+  d.out_lambda = s.out_lambda;
+  d.out_hamming = s.out_hamming;
+  d.out_skipped = s.out_skipped;
+  d.gapless_tasks = s.gapless_tasks;
+  d.dp_unsorted = s.dp_unsorted;
+  d.bin_count = s.bin_count;
+  d.ctr = s.ctr;
+  dim3 grid((d.u.n + kScreenBlock - 1) / kScreenBlock,
+            (s.n_centers + kScreenCenters - 1) / kScreenCenters);
+  hipLaunchKernelGGL(screen_kernel, grid, dim3(kScreenBlock), 0, st, d);
+  return hipGetLastError();
+}
+
+}  // namespace d2g

@@ -1,0 +1,164 @@
+/*
+ * dada2_amd.h — C ABI of the MI355X (gfx950) compare hot path for dada2-rs.
+ *
+ * This is the drop-in boundary: plain pointers and sizes, no torch or Rust
+ * types.  Each entry point names the reference function it replaces
+ * (paths relative to the dada2-rs v0.2.0 source tree).
+ *
+ * Conventions
+ *  - Every entry point returns an int status: D2G_OK (0) or a negative code.
+ *    Invariant violations that panic in the reference (an invalid traceback
+ *    pointer, nwalign.rs:268/1130; a non-ACGT base reaching compute_lambda,
+ *    pval.rs:281-284) become error codes, never aborts.
+ *  - Host arrays are caller-owned and copied.  Device buffers are owned by the
+ *    context.  Outputs are caller-allocated.
+ *  - A context is not thread-safe; use one per host thread (the reference's
+ *    AlignBuffers are likewise per worker, nwalign.rs:112-113).
+ *  - Sequences for the compare path use the reference encoding A=1 C=2 G=3
+ *    T=4 (misc.rs:346-356); bimera sequences are ASCII (remove_bimera.rs:70-75).
+ */
+#ifndef DADA2_AMD_H
+#define DADA2_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define D2G_OK 0
+#define D2G_ERR_ARG (-1)         /* invalid argument / size */
+#define D2G_ERR_HIP (-2)         /* HIP runtime error (no device, OOM, launch) */
+#define D2G_ERR_NONACGT (-3)     /* non-ACGT base in the compare path */
+#define D2G_ERR_TRACEBACK (-4)   /* invalid traceback pointer (reference panics) */
+#define D2G_ERR_UNSUPPORTED (-5) /* geometry beyond the kernels' limits */
+#define D2G_ERR_STATE (-6)       /* call order (e.g. compare before load) */
+
+#define D2G_HAMMING_NONE 0xFFFFFFFFu /* shrouded or skipped (cluster.rs:262, 302) */
+
+typedef struct d2g_ctx d2g_ctx;
+
+/* AlignParams, nwalign.rs:63-90.  The WFA backend is not part of this
+ * library (it is off by default and not NW-identical, nwalign.rs:1340-1348). */
+typedef struct {
+  int32_t match_score;
+  int32_t mismatch;
+  int32_t gap_p;
+  int32_t homo_gap_p;
+  int32_t use_kmers;
+  int32_t kmer_size;
+  double kdist_cutoff;
+  int32_t band;
+  int32_t vectorized;
+  int32_t gapless;
+  int32_t _reserved;
+} d2g_align_params;
+
+/* VectorizedAlignScores, nwalign.rs:96-103. */
+typedef struct {
+  int16_t match_score;
+  int16_t mismatch;
+  int16_t gap_p;
+  int16_t end_gap_p;
+  int32_t band;
+} d2g_vec_scores;
+
+/* Per-call counters (the GPU counterpart of CompareTiming, cluster.rs:190-231).
+ * Kernel times are measured with HIP events on the context's stream. */
+typedef struct {
+  uint64_t comparisons; /* (center, unique) pairs processed */
+  uint64_t skipped;     /* greedy skips (cluster.rs:295) */
+  uint64_t shrouded;    /* k-mer screen rejections (nwalign.rs:1299) */
+  uint64_t gapless;     /* gapless dispatch (nwalign.rs:1324) */
+  uint64_t aligned_dp;  /* pairs that ran a DP aligner */
+  uint64_t dp_cells;    /* exact DP cells computed (reference loop count) */
+  double ms_screen;
+  double ms_plan;
+  double ms_dp;
+  double ms_lambda;
+  double ms_total;      /* first to last event on the stream */
+} d2g_stats;
+
+/* ---- context ---------------------------------------------------------- */
+int d2g_ctx_create(int device, d2g_ctx** out);
+int d2g_ctx_destroy(d2g_ctx* ctx);
+/* Enable per-kernel HIP-event timing in d2g_stats (default off). */
+int d2g_ctx_set_timing(d2g_ctx* ctx, int enable);
+/* The hipStream_t every kernel of this context is launched on. */
+void* d2g_ctx_stream(d2g_ctx* ctx);
+/* Device properties used for roofline figures. */
+int d2g_device_info(d2g_ctx* ctx, int* cu_count, int* clock_khz, int* mem_clock_khz,
+                    int* mem_bus_width);
+const char* d2g_status_string(int status);
+
+/* ---- uniques ------------------------------------------------------------
+ * Replaces the per-raw preparation of dada_uniques_cached (dada.rs:243-402):
+ * nt_encode'd sequences (1..4), rounded qualities (containers.rs:128-137),
+ * read counts, and raw_assign_kmers (kmers.rs:213-221) which the device
+ * builds itself (dense 4^k kmer8 screen + positional kord).
+ * seq_concat holds n sequences back to back; offsets has n+1 entries.
+ * qual_concat is nullable (no qualities: use_quals has no effect).
+ * Validation mirrors dada.rs:248-317: k in 3..=8, every length > k,
+ * every base in 1..4, lengths < 9999. */
+int d2g_load_uniques(d2g_ctx* ctx, uint32_t n, const uint8_t* seq_concat,
+                     const uint64_t* offsets, const uint8_t* qual_concat,
+                     const uint32_t* reads, uint32_t kmer_size);
+
+/* ---- the compare ---------------------------------------------------------
+ * Plain path: one center against every loaded unique.  Replaces the parallel
+ * map of b_compare_parallel (cluster.rs:279-330; the serial twin b_compare,
+ * cluster.rs:106-164).  For each unique index:
+ *   out_lambda[i]  compute_lambda(raw_i, sub) (pval.rs:253), 0.0 when
+ *                  shrouded or greedy-skipped
+ *   out_hamming[i] nsubs, or D2G_HAMMING_NONE when shrouded/skipped
+ *   out_skipped[i] 1 when greedy && (reads_i > reads_center || lock_i)
+ * err is row-major 16 x ncol f64 (pval.rs:246-250); every quality must be
+ * < ncol.  lock is nullable (all unlocked).  The serial store
+ * (cluster.rs:351-397) stays with the caller. */
+int d2g_compare_center(d2g_ctx* ctx, uint32_t center, const double* err, uint32_t ncol,
+                       const d2g_align_params* params, int use_quals, int greedy,
+                       const uint8_t* lock, double* out_lambda, uint32_t* out_hamming,
+                       uint8_t* out_skipped, d2g_stats* stats);
+
+/* Batched path: n_centers centers x all uniques in one device pipeline.
+ * Outputs are n_centers x n_uniques, row c for centers[c]. */
+int d2g_compare_batch(d2g_ctx* ctx, const uint32_t* centers, uint32_t n_centers,
+                      const double* err, uint32_t ncol, const d2g_align_params* params,
+                      int use_quals, int greedy, const uint8_t* lock, double* out_lambda,
+                      uint32_t* out_hamming, uint8_t* out_skipped, d2g_stats* stats);
+
+/* Device-resident variant of d2g_compare_batch: results stay in context-owned
+ * device buffers (fetch with d2g_fetch_results).  This is what a caller
+ * looping compare rounds on the device uses; it performs no D2H copy. */
+int d2g_compare_batch_device(d2g_ctx* ctx, const uint32_t* centers, uint32_t n_centers,
+                             const double* err, uint32_t ncol, const d2g_align_params* params,
+                             int use_quals, int greedy, const uint8_t* lock, d2g_stats* stats);
+int d2g_fetch_results(d2g_ctx* ctx, double* out_lambda, uint32_t* out_hamming,
+                      uint8_t* out_skipped);
+
+/* ---- pairwise alignment (parity / aux mode) -------------------------------
+ * Backs align_vectorized_with_buf (nwalign.rs:761-1140) for n arbitrary byte
+ * pairs (any alphabet: 1..4 or ASCII).  s1/s2 are concatenated with offsets
+ * (n+1 entries each).  al0/al1 receive each alignment at al_offsets[i]
+ * (caller supplies capacity len1+len2 per pair: al_offsets[i] =
+ * sum_{j<i} (len1_j + len2_j)); al_len[i] its length.  al_offsets may be
+ * NULL, meaning exactly that packing. */
+int d2g_align_vectorized(d2g_ctx* ctx, uint32_t n, const uint8_t* s1, const uint64_t* s1_off,
+                         const uint8_t* s2, const uint64_t* s2_off, const d2g_vec_scores* scores,
+                         uint8_t* al0, uint8_t* al1, const uint64_t* al_offsets,
+                         uint32_t* al_len);
+
+/* DP cell count of the vectorized loop for one pair (sum over anti-diagonals
+ * of the active width; nwalign.rs:875-1088) — the GCUPS unit. */
+uint64_t d2g_vectorized_cells(uint32_t len1, uint32_t len2, int32_t band);
+
+/* 1 when the vectorized DP kernel handles this geometry (compressed width
+ * ncol <= 64 and its wavefront fits in LDS), 0 otherwise.  Pairs outside it
+ * make d2g_align_vectorized / d2g_compare_* return D2G_ERR_UNSUPPORTED. */
+int d2g_vectorized_supported(uint32_t len1, uint32_t len2, int32_t band);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* DADA2_AMD_H */

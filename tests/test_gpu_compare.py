@@ -1,0 +1,94 @@
+"""GPU parity of the compare map (b_compare_parallel, cluster.rs:279-330)
+against the oracle: lambda bits, hamming and skipped flags must be identical
+for every (center, unique) pair.
+"""
+import numpy as np
+import pytest
+
+from dada2_rs_amd import synth
+
+pytestmark = pytest.mark.gpu
+
+
+def _oracle_rows(oracle, w, centers, params=None, use_quals=True, greedy=True, lock=None):
+    rs = oracle.RawSet(w.seqs, w.quals, w.reads, k=w.kmer_size, locks=lock)
+    params = params or oracle.make_params(band=w.band)
+    rows = [oracle.compare_center(rs, int(c), w.err, w.err.shape[1], params,
+                                  use_quals=use_quals, greedy=greedy, nthreads=8)
+            for c in centers]
+    lam = np.stack([r[0] for r in rows])
+    ham = np.stack([r[1] for r in rows])
+    skp = np.stack([r[2] for r in rows])
+    return lam, ham, skp
+
+
+def _assert_same(got, lam, ham, skp):
+    bad = np.nonzero(got.lam.view(np.uint64) != lam.view(np.uint64))
+    assert bad[0].size == 0, f"lambda bits differ at {list(zip(*bad))[:5]}"
+    assert np.array_equal(got.hamming, ham)
+    assert np.array_equal(got.skipped, skp)
+
+
+@pytest.fixture(scope="module")
+def plain():
+    return synth.workload_plain()
+
+
+@pytest.fixture(scope="module")
+def batched_small():
+    return synth.subsample(synth.workload_batched(), 3000, n_centers=24)
+
+
+def test_plain_center(ctx, oracle, pkg, plain):
+    ctx.load_uniques(plain.seqs, plain.quals, plain.reads, k=5)
+    c = int(plain.centers[0])
+    got = ctx.compare_center(c, plain.err)
+    _assert_same(got, *[x[0] for x in _oracle_rows(oracle, plain, [c])])
+    assert got.stats["comparisons"] == plain.n
+
+
+def test_plain_every_tenth_center_no_greedy(ctx, oracle, pkg, plain):
+    ctx.load_uniques(plain.seqs, plain.quals, plain.reads, k=5)
+    centers = np.arange(0, plain.n, 100)
+    got = ctx.compare_batch(centers, plain.err, greedy=False)
+    _assert_same(got, *_oracle_rows(oracle, plain, centers, greedy=False))
+
+
+def test_batched_uneven(ctx, oracle, pkg, batched_small):
+    w = batched_small
+    ctx.load_uniques(w.seqs, w.quals, w.reads, k=5)
+    got = ctx.compare_batch(w.centers, w.err)
+    lam, ham, skp = _oracle_rows(oracle, w, w.centers)
+    _assert_same(got, lam, ham, skp)
+    assert got.stats["aligned_dp"] > 100
+
+
+def test_batched_lock_and_no_quals(ctx, oracle, pkg, batched_small):
+    w = batched_small
+    rng = np.random.default_rng(3)
+    lock = (rng.random(w.n) < 0.1).astype(np.uint8)
+    ctx.load_uniques(w.seqs, w.quals, w.reads, k=5)
+    cs = w.centers[:8]
+    got = ctx.compare_batch(cs, w.err, use_quals=False, lock=lock)
+    _assert_same(got, *_oracle_rows(oracle, w, cs, use_quals=False, lock=lock))
+
+
+def test_cluster0_cutoff_one(ctx, oracle, pkg, plain):
+    """The cluster-0 compare uses kdist_cutoff = 1.0 (dada.rs:835-838): nothing
+    is shrouded, every pair aligns."""
+    sub = synth.subsample(plain, 300)
+    ctx.load_uniques(sub.seqs, sub.quals, sub.reads, k=5)
+    p = pkg.align_params(kdist_cutoff=1.0)
+    c = synth.plain_center(sub.reads)
+    got = ctx.compare_center(c, sub.err, params=p)
+    lam, ham, skp = _oracle_rows(oracle, sub, [c], params=oracle.make_params(kdist_cutoff=1.0))
+    _assert_same(got, lam[0], ham[0], skp[0])
+
+
+def test_results_are_deterministic(ctx, pkg, batched_small):
+    w = batched_small
+    ctx.load_uniques(w.seqs, w.quals, w.reads, k=5)
+    a = ctx.compare_batch(w.centers[:6], w.err)
+    b = ctx.compare_batch(w.centers[:6], w.err)
+    assert np.array_equal(a.lam.view(np.uint64), b.lam.view(np.uint64))
+    assert np.array_equal(a.hamming, b.hamming)
