@@ -63,9 +63,12 @@ struct d2g_ctx {
   DevBuf<uint32_t> centers;
   DevBuf<uint8_t> lock;
   DevBuf<double> err;
-  DevBuf<double> out_lambda;
+  DevBuf<double> out_lambda;  // length-sorted unique order
   DevBuf<uint32_t> out_hamming;
   DevBuf<uint8_t> out_skipped;
+  DevBuf<double> fin_lambda;  // caller's unique order (filled by fetch)
+  DevBuf<uint32_t> fin_hamming;
+  DevBuf<uint8_t> fin_skipped;
   DevBuf<GaplessTask> gl;
   DevBuf<DpTask> dp_unsorted, dp_sorted;
   DevBuf<uint32_t> bin_count, bin_start, wave_start, bin_fill;
@@ -73,9 +76,16 @@ struct d2g_ctx {
   DevBuf<Counters> ctr;
   size_t sub_capacity = 0;
   uint32_t last_nc = 0;
-  // cached LDS sizing of the DP kernel for a bin space
+  // cached per bin space: LDS sizing and the row schedule of the DP kernel
   BinSpace lds_key = {-1, -1, -1, -1};
+  int sched_gap = 0, sched_end_gap = 0;
   size_t lds_bytes = 0;
+  int sched_rows = 0;
+  DevBuf<RowRec> sched;
+  // pointer-history scratch of the DP: nw_grid slots of hist_stride words
+  uint32_t nw_grid = 0;
+  size_t hist_stride = 0;
+  DevBuf<uint32_t> hist;
   // align mode buffers
   DevBuf<uint8_t> a_pool, a_al0, a_al1;
   DevBuf<uint64_t> a_aloff;
@@ -135,10 +145,15 @@ int d2g_ctx_destroy(d2g_ctx* c) {
   if (!c) return D2G_OK;
   (void)hipSetDevice(c->device);
   if (c->stream) (void)hipStreamSynchronize(c->stream);
-  for (auto* b : {&c->seq, &c->qual, &c->has255, &c->lock, &c->out_skipped, &c->a_pool, &c->a_al0, &c->a_al1}) b->release();
+  for (auto* b : {&c->seq, &c->qual, &c->has255, &c->lock, &c->out_skipped, &c->fin_skipped,
+                  &c->a_pool, &c->a_al0, &c->a_al1})
+    b->release();
+  c->fin_lambda.release();
+  c->fin_hamming.release();
   for (auto* b : {&c->len, &c->reads, &c->orig, &c->kmer_row, &c->kmer_il, &c->kord_row, &c->kord_il,
                   &c->ksum, &c->centers, &c->out_hamming, &c->bin_count, &c->bin_start, &c->wave_start,
-                  &c->bin_fill, &c->sub_pool, &c->sub_off, &c->sub_cnt, &c->sub_top, &c->a_allen})
+                  &c->bin_fill, &c->sub_pool, &c->sub_off, &c->sub_cnt, &c->sub_top, &c->a_allen,
+                  &c->hist})
     b->release();
   c->off.release();
   c->a_aloff.release();
@@ -148,6 +163,7 @@ int d2g_ctx_destroy(d2g_ctx* c) {
   c->dp_unsorted.release();
   c->dp_sorted.release();
   c->ctr.release();
+  c->sched.release();
   for (int i = 0; i < kEvents; ++i)
     if (c->ev[i]) (void)hipEventDestroy(c->ev[i]);
   if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -251,10 +267,10 @@ int d2g_load_uniques(d2g_ctx* c, uint32_t n, const uint8_t* seq_concat, const ui
     slen[s] = len[order[s]];
     sreads[s] = reads[order[s]];
     inv[order[s]] = s;
-    soff[s + 1] = soff[s] + slen[s];
+    soff[s + 1] = soff[s] + ((slen[s] + 15u) & ~15u);  // 16-byte aligned rows
   }
   const uint64_t total = soff[n];
-  std::vector<uint8_t> sseq(total), squal(qual_concat ? total : 0);
+  std::vector<uint8_t> sseq(total, 0), squal(qual_concat ? total : 0, 0);
   for (uint32_t s = 0; s < n; ++s) {
     std::memcpy(&sseq[soff[s]], seq_concat + offsets[order[s]], slen[s]);
     if (qual_concat) std::memcpy(&squal[soff[s]], qual_concat + offsets[order[s]], slen[s]);
@@ -312,6 +328,35 @@ int d2g_load_uniques(d2g_ctx* c, uint32_t n, const uint8_t* seq_concat, const ui
   return D2G_OK;
 }
 
+#define HIP_TRY_INT(x)        \
+  do {                        \
+    int r_ = (x);             \
+    if (r_ != D2G_OK) return r_; \
+  } while (0)
+
+// LDS size and row schedule of the vectorized DP for a bin space, cached
+// across calls with the same space and scores.
+static int prepare_schedule(d2g_ctx* c, const BinSpace& bs, int end_gap, int gap) {
+  if (bs.lmin == c->lds_key.lmin && bs.range == c->lds_key.range && bs.band == c->lds_key.band &&
+      end_gap == c->sched_end_gap && gap == c->sched_gap && c->sched.p)
+    return D2G_OK;
+  c->lds_bytes = nw_vec_lds_for(bs);
+  const int lmax = bs.lmin + bs.range - 1;
+  c->sched_rows = nw_sched_rows(lmax);
+  const size_t recs = (size_t)bs.nbins * (size_t)c->sched_rows;
+  if (c->sched.ensure(recs) != hipSuccess) return D2G_ERR_HIP;
+  c->nw_grid = (uint32_t)c->cu_count * 16;
+  c->hist_stride = nw_hist_p_words(2 * lmax + 1);
+  if (c->hist.ensure((size_t)c->nw_grid * c->hist_stride) != hipSuccess) return D2G_ERR_HIP;
+  if (launch_sched(bs, c->sched_rows, end_gap, gap, c->sched.p, c->stream) != hipSuccess)
+    return D2G_ERR_HIP;
+  if (hipStreamSynchronize(c->stream) != hipSuccess) return D2G_ERR_HIP;
+  c->lds_key = bs;
+  c->sched_end_gap = end_gap;
+  c->sched_gap = gap;
+  return D2G_OK;
+}
+
 static float ev_ms(d2g_ctx* c, int a, int b) {
   float ms = 0.f;
   if (hipEventElapsedTime(&ms, c->ev[a], c->ev[b]) != hipSuccess) return 0.f;
@@ -342,10 +387,7 @@ static int compare_device(d2g_ctx* c, const uint32_t* centers, uint32_t nc, cons
   bs.band = p->band;
   if ((size_t)bs.range * bs.range > (1u << 22)) return D2G_ERR_UNSUPPORTED;
   bs.nbins = bs.range * bs.range;
-  if (bs.lmin != c->lds_key.lmin || bs.range != c->lds_key.range || bs.band != c->lds_key.band) {
-    c->lds_bytes = nw_vec_lds_for(bs);
-    c->lds_key = bs;
-  }
+  HIP_TRY_INT(prepare_schedule(c, bs, 0, p->gap_p));
   std::vector<uint32_t> hc(nc);
   for (uint32_t i = 0; i < nc; ++i) hc[i] = c->h_inv[centers[i]];
   HIP_TRY(c->centers.ensure(nc));
@@ -426,8 +468,12 @@ static int compare_device(d2g_ctx* c, const uint32_t* centers, uint32_t nc, cons
     na.sub_top = c->sub_top.p;
     na.sub_off = c->sub_off.p;
     na.sub_cnt = c->sub_cnt.p;
-    na.grid = (uint32_t)c->cu_count * 16;
+    na.grid = c->nw_grid;
+    na.hist = c->hist.p;
+    na.hist_stride = c->hist_stride;
     na.lds_bytes = c->lds_bytes;
+    na.sched = c->sched.p;
+    na.sched_rows = c->sched_rows;
     HIP_TRY(launch_nw_vec(na, s));
     if (c->timing) HIP_TRY(hipEventRecord(c->ev[3], s));
     LambdaArgs la;
@@ -451,7 +497,10 @@ static int compare_device(d2g_ctx* c, const uint32_t* centers, uint32_t nc, cons
     HIP_TRY(hipMemcpyAsync(&h, c->ctr.p, sizeof(Counters), hipMemcpyDeviceToHost, s));
     HIP_TRY(hipStreamSynchronize(s));
     if (h.err_flags & kErrSubOverflow) {
-      c->sub_capacity *= 4;
+      // each DP pair reserves (shorter length + 1) records: this bound fits all
+      const size_t need = (size_t)h.n_dp * (u.maxlen + 1);
+      c->sub_capacity = std::max(c->sub_capacity * 4, need);
+      if (c->sub_capacity > 0xFFFFFFF0u) return D2G_ERR_UNSUPPORTED;
       HIP_TRY(c->sub_pool.ensure(c->sub_capacity));
       continue;
     }
@@ -490,9 +539,14 @@ int d2g_fetch_results(d2g_ctx* c, double* out_lambda, uint32_t* out_hamming, uin
   (void)hipSetDevice(c->device);
   const size_t np = (size_t)c->last_nc * c->u.n;
   hipStream_t s = c->stream;
-  if (out_lambda) HIP_TRY(hipMemcpyAsync(out_lambda, c->out_lambda.p, np * sizeof(double), hipMemcpyDeviceToHost, s));
-  if (out_hamming) HIP_TRY(hipMemcpyAsync(out_hamming, c->out_hamming.p, np * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
-  if (out_skipped) HIP_TRY(hipMemcpyAsync(out_skipped, c->out_skipped.p, np, hipMemcpyDeviceToHost, s));
+  HIP_TRY(c->fin_lambda.ensure(np));
+  HIP_TRY(c->fin_hamming.ensure(np));
+  HIP_TRY(c->fin_skipped.ensure(np));
+  HIP_TRY(launch_unpermute(c->last_nc, c->u.n, c->u.orig, c->out_lambda.p, c->out_hamming.p,
+                           c->out_skipped.p, c->fin_lambda.p, c->fin_hamming.p, c->fin_skipped.p, s));
+  if (out_lambda) HIP_TRY(hipMemcpyAsync(out_lambda, c->fin_lambda.p, np * sizeof(double), hipMemcpyDeviceToHost, s));
+  if (out_hamming) HIP_TRY(hipMemcpyAsync(out_hamming, c->fin_hamming.p, np * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+  if (out_skipped) HIP_TRY(hipMemcpyAsync(out_skipped, c->fin_skipped.p, np, hipMemcpyDeviceToHost, s));
   HIP_TRY(hipStreamSynchronize(s));
   return D2G_OK;
 }
@@ -599,8 +653,13 @@ int d2g_align_vectorized(d2g_ctx* c, uint32_t n, const uint8_t* s1, const uint64
   na.al1 = c->a_al1.p;
   na.al_off = c->a_aloff.p;
   na.al_len = c->a_allen.p;
-  na.grid = (uint32_t)c->cu_count * 16;
-  na.lds_bytes = nw_vec_lds_for(bs);
+  HIP_TRY_INT(prepare_schedule(c, bs, sc->end_gap_p, sc->gap_p));
+  na.grid = c->nw_grid;
+  na.hist = c->hist.p;
+  na.hist_stride = c->hist_stride;
+  na.lds_bytes = c->lds_bytes;
+  na.sched = c->sched.p;
+  na.sched_rows = c->sched_rows;
   HIP_TRY(launch_nw_vec(na, s));
   Counters h;
   HIP_TRY(hipMemcpyAsync(&h, c->ctr.p, sizeof(Counters), hipMemcpyDeviceToHost, s));

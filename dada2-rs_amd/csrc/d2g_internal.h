@@ -50,21 +50,42 @@ D2G_HD VecGeom vec_geom(int l1_in, int l2_in, int band_in) {
 // lanes, lane c of a group holding compressed column c.
 D2G_HD int vec_groups(int ncol) { return ncol <= kWave ? kWave / ncol : 0; }
 
-// LDS budget of one DP wavefront: nrow 16-byte pointer records, then per
-// pair the padded sequences and its substitution list.
-constexpr int kNwMaxGroups = 4;
+// LDS budget of one DP wavefront: per pair the padded sequences.  The
+// pointer history lives in a global slot per workgroup.
+constexpr int kNwMaxGroups = 16;
 constexpr size_t kNwLdsSoft = 64 * 1024;
 constexpr size_t kNwLdsHard = 160 * 1024 - 1024;
 
 D2G_HD int nw_pad(int ncol) { return ((ncol + 4) + 3) & ~3; }
+// pointer history: one word per lane per 16 rows
+D2G_HD size_t nw_hist_p_words(int nrow) { return (size_t)((nrow + 15) / 16) * 64; }
+// per pair: both sequences with padding either side
 D2G_HD size_t nw_group_bytes(const VecGeom& g) {
   size_t seq = (size_t)g.len1 + g.len2 + 4 * nw_pad(g.ncol);
-  seq = (seq + 15) & ~(size_t)15;
-  return seq + (size_t)(g.len1 + 1) * 4;  // substitution list
+  return (seq + 15) & ~(size_t)15;
 }
-D2G_HD size_t nw_wave_bytes(const VecGeom& g, int G) {
-  return (size_t)g.nrow * 16 + (size_t)G * nw_group_bytes(g);
-}
+D2G_HD size_t nw_wave_bytes(const VecGeom& g, int G) { return (size_t)G * nw_group_bytes(g); }
+// One anti-diagonal of the vectorized aligner's row walk, precomputed per
+// geometry (the schedule of nwalign.rs:875-1088 depends only on (len1, len2,
+// band, end_gap_p > gap_p)).  Fields are in compressed columns.
+struct RowRec {
+  int32_t a1;       // i_max + col_min: s1 byte offset of column 0's cell
+  int32_t a2;       // j_min - col_min: s2 byte offset of column 0's cell
+  int32_t inr_lo;   // col_min + k_lo: first column whose cell is inside both sequences
+  uint32_t inr_n;   // k_hi - k_lo
+  int32_t cmin;     // col_min
+  uint32_t n;       // active width col_max - col_min + 1
+  int32_t edge_l;   // left ends-free edge column, -1 when none (nwalign.rs:891-894)
+  int32_t edge_t;   // top ends-free edge column, -1 when none (nwalign.rs:895-898)
+  int32_t row_free; // accumulated ends-free edge score (nwalign.rs:884)
+  uint32_t flags;   // bit0 left recalc, bit1 right recalc, bit2 band wedge,
+                    // bits 8-15 col_min after the wedge, bits 16-23 col_max after it
+};
+
+// Records of the schedule table per bin of a BinSpace whose longest length
+// is lmax: rows 2..len1+len2 and two lookahead records.
+D2G_HD int nw_sched_rows(int lmax) { return 2 * lmax + 1; }
+
 // Pairs per wavefront for one geometry (0 = unsupported).
 D2G_HD int nw_groups(const VecGeom& g) {
   int G = vec_groups(g.ncol);
@@ -73,6 +94,59 @@ D2G_HD int nw_groups(const VecGeom& g) {
   if (G >= 1 && nw_wave_bytes(g, G) > kNwLdsHard) G = 0;
   return G;
 }
+
+// Row-band geometry of the scalar aligners (align_endsfree_with_buf,
+// nwalign.rs:336-417; align_endsfree_homo_with_buf, nwalign.rs:435-528):
+// row i computes j in [max(1, i-lband), min(len2, i+rband)] (band_adjust,
+// nwalign.rs:276-284), i.e. diagonals d = j - i in [-lband, rband], with
+// -9999 sentinels on d = -lband-1 and d = rband+1 (fill_band_sentinels,
+// nwalign.rs:287-308).  Diagonal index D = d + lband + 1 runs over
+// [0, dmax]; lane c holds D = 2c or 2c+1 (whichever matches the parity of
+// the anti-diagonal i + j).
+struct BandGeom {
+  int len1, len2, lband, rband, dmax, lanes, nrow;
+  bool sentinels;
+};
+
+D2G_HD BandGeom band_geom(int l1, int l2, int band) {
+  BandGeom g;
+  g.len1 = l1;
+  g.len2 = l2;
+  if (band < 0) {  // unbanded: every row spans j in [1, len2]
+    g.lband = l1;
+    g.rband = l2;
+    g.sentinels = false;
+  } else {
+    g.lband = band + (l1 > l2 ? l1 - l2 : 0);
+    g.rband = band + (l2 > l1 ? l2 - l1 : 0);
+    g.sentinels = band < l1 || band < l2;
+  }
+  g.dmax = g.lband + g.rband + 2;
+  g.lanes = g.dmax / 2 + 1;
+  g.nrow = l1 + l2 + 1;
+  return g;
+}
+
+D2G_HD int band_pad(const BandGeom& g) { return ((g.lanes + 4) + 3) & ~3; }
+D2G_HD size_t band_group_bytes(const BandGeom& g) {
+  size_t seq = (size_t)g.len1 + g.len2 + 4 * band_pad(g);
+  seq = (seq + 15) & ~(size_t)15;
+  return seq + (size_t)(g.len1 + 1) * 4;  // substitution list
+}
+D2G_HD size_t band_wave_bytes(const BandGeom& g, int G) {
+  return (size_t)g.nrow * 16 + (size_t)G * band_group_bytes(g);
+}
+D2G_HD int band_groups(const BandGeom& g) {
+  int G = g.lanes <= kWave ? kWave / g.lanes : 0;
+  if (G > kNwMaxGroups) G = kNwMaxGroups;
+  while (G > 1 && band_wave_bytes(g, G) > kNwLdsSoft) --G;
+  if (G >= 1 && band_wave_bytes(g, G) > kNwLdsHard) G = 0;
+  return G;
+}
+
+// Which DP kernel a bin space feeds.
+constexpr int kDpVectorized = 0;
+constexpr int kDpRowBand = 1;
 
 // One pair queued for the vectorized DP.  Sequences live in a byte pool; the
 // bin groups pairs of identical (len1_in, len2_in) so a wavefront's row

@@ -61,6 +61,9 @@ hipError_t launch_plan(const BinSpace& bs, const uint32_t* bin_count, uint32_t* 
 hipError_t launch_scatter(const DpTask* unsorted, DpTask* sorted, const uint32_t* bin_start,
                           uint32_t* bin_fill, const Counters* ctr, uint32_t capacity,
                           hipStream_t s);
+hipError_t launch_unpermute(uint32_t nc, uint32_t n, const uint32_t* orig, const double* src_lam,
+                            const uint32_t* src_ham, const uint8_t* src_skp, double* dst_lam,
+                            uint32_t* dst_ham, uint8_t* dst_skp, hipStream_t s);
 
 // ---- nw_vec.hip --------------------------------------------------------
 struct NwArgs {
@@ -69,15 +72,20 @@ struct NwArgs {
   const uint32_t* wave_start;  // nbins + 1
   const DpTask* tasks;         // sorted by bin
   const uint8_t* pool;         // sequence bytes
+  const RowRec* sched;         // per-bin row schedule (launch_sched)
+  int sched_rows;              // records per bin
   int16_t match, mismatch, gap_p, end_gap_p;
   Counters* ctr;
   int emit_alignment;
-  // compare mode: per sorted task, substitution list (query_pos << 8 | trans)
+  // compare mode: per sorted task, substitution list (query_pos << 16 | center_pos)
   uint32_t* sub_pool;
   uint32_t sub_capacity;
   uint32_t* sub_top;       // bump allocator
   uint32_t* sub_off;       // per task
   uint32_t* sub_cnt;       // per task
+  // pointer history scratch: grid slots of hist_stride words
+  uint32_t* hist;
+  size_t hist_stride;
   // align mode
   uint8_t* al0;
   uint8_t* al1;
@@ -87,6 +95,9 @@ struct NwArgs {
   size_t lds_bytes;        // from nw_vec_lds_for
 };
 hipError_t launch_nw_vec(const NwArgs& a, hipStream_t s);
+// Replays the vectorized row walk of every bin into table[bin * rows + row - 2].
+hipError_t launch_sched(const BinSpace& bs, int rows, int end_gap, int gap, RowRec* table,
+                        hipStream_t s);
 size_t nw_vec_lds_for(const BinSpace& bs);
 
 // ---- lambda.hip --------------------------------------------------------

@@ -9,8 +9,8 @@
 //
 // Gapless pairs (align_gapless_with_buf, nwalign.rs:645-655) pair positions
 // directly: substitutions are the positions < min(len) where bases differ.
-// DP pairs read the substitution list the traceback wrote, which is in
-// descending query position.
+// DP pairs read the substitution list the traceback wrote: (query position,
+// center position) records in descending query position.
 #include "d2g_launch.h"
 
 namespace d2g {
@@ -47,45 +47,70 @@ __global__ __launch_bounds__(256) void lambda_kernel(LambdaDev a) {
     if (t < n_g) {
       const GaplessTask task = a.gl[t];
       out = task.out_idx;
-      const uint8_t* q = a.u.seq + a.u.off[task.u_idx];
-      const uint8_t* qq = quals ? a.u.qual + a.u.off[task.u_idx] : nullptr;
-      const uint8_t* c = a.u.seq + a.u.off[task.c_idx];
+      const uint4* q16 = reinterpret_cast<const uint4*>(a.u.seq + a.u.off[task.u_idx]);
+      const uint4* qq16 = quals ? reinterpret_cast<const uint4*>(a.u.qual + a.u.off[task.u_idx]) : nullptr;
+      const uint4* c16 = reinterpret_cast<const uint4*>(a.u.seq + a.u.off[task.c_idx]);
       const uint32_t lq = a.u.len[task.u_idx], lc = a.u.len[task.c_idx];
-      for (uint32_t pos = 0; pos < lq; ++pos) {
-        const uint32_t qn = q[pos];
-        uint32_t tr = 5u * (qn - 1u);
-        if (pos < lc) {
-          const uint32_t cn = c[pos];
-          if (cn != qn) {
-            tr = 4u * (cn - 1u) + (qn - 1u);
-            ++ham;
+      for (uint32_t base = 0; base < lq; base += 16) {
+        const uint4 qv = q16[base >> 4];
+        // rows are padded to 16 bytes; a longer query reads no center bytes
+        const uint4 cv = base < lc ? c16[base >> 4] : make_uint4(0, 0, 0, 0);
+        const uint4 qu = qq16 ? qq16[base >> 4] : make_uint4(0, 0, 0, 0);
+        const uint32_t qw[4] = {qv.x, qv.y, qv.z, qv.w};
+        const uint32_t cw[4] = {cv.x, cv.y, cv.z, cv.w};
+        const uint32_t uw[4] = {qu.x, qu.y, qu.z, qu.w};
+#pragma unroll
+        for (int b = 0; b < 16; ++b) {
+          const uint32_t pos = base + (uint32_t)b;
+          if (pos < lq) {
+            const uint32_t qn = (qw[b >> 2] >> (8 * (b & 3))) & 0xFFu;
+            const uint32_t cn = (cw[b >> 2] >> (8 * (b & 3))) & 0xFFu;
+            const uint32_t qi = (uw[b >> 2] >> (8 * (b & 3))) & 0xFFu;
+            uint32_t tr = 5u * (qn - 1u);
+            if (pos < lc && cn != qn) {  // gapless pairing, nwalign.rs:645-655
+              tr = 4u * (cn - 1u) + (qn - 1u);
+              ++ham;
+            }
+            lam *= err_lds[tr * ncol + qi];
           }
         }
-        const uint32_t qi = qq ? qq[pos] : 0u;
-        lam *= err_lds[tr * ncol + qi];
       }
     } else {
       const uint32_t di = t - n_g;
       const DpTask task = a.dp[di];
       out = task.out_idx;
-      const uint8_t* q = a.u.seq + a.u.off[task.u_idx];
-      const uint8_t* qq = quals ? a.u.qual + a.u.off[task.u_idx] : nullptr;
+      const uint4* q16 = reinterpret_cast<const uint4*>(a.u.seq + a.u.off[task.u_idx]);
+      const uint4* qq16 = quals ? reinterpret_cast<const uint4*>(a.u.qual + a.u.off[task.u_idx]) : nullptr;
       const uint32_t lq = a.u.len[task.u_idx];
       const uint32_t cnt_raw = a.sub_cnt[di];
       if (cnt_raw & 0x80000000u) atomicOr(&a.ctr->err_flags, kErrTraceback);
       const uint32_t cnt = cnt_raw & 0x7FFFFFFFu;
       const uint32_t* subs = a.sub_pool + a.sub_off[di];
+      const uint8_t* cseq = a.u.seq + a.u.off[task.c_idx];
+      // records are (query position << 16 | center position), descending
       int idx = (int)cnt - 1;
       uint32_t next = idx >= 0 ? subs[idx] : 0xFFFFFFFFu;
-      for (uint32_t pos = 0; pos < lq; ++pos) {
-        uint32_t tr = 5u * ((uint32_t)q[pos] - 1u);
-        if ((next >> 8) == pos) {
-          tr = next & 0xFFu;
-          --idx;
-          next = idx >= 0 ? subs[idx] : 0xFFFFFFFFu;
+      for (uint32_t base = 0; base < lq; base += 16) {
+        const uint4 qv = q16[base >> 4];
+        const uint4 qu = qq16 ? qq16[base >> 4] : make_uint4(0, 0, 0, 0);
+        const uint32_t qw[4] = {qv.x, qv.y, qv.z, qv.w};
+        const uint32_t uw[4] = {qu.x, qu.y, qu.z, qu.w};
+#pragma unroll
+        for (int b = 0; b < 16; ++b) {
+          const uint32_t pos = base + (uint32_t)b;
+          if (pos < lq) {
+            const uint32_t qn = (qw[b >> 2] >> (8 * (b & 3))) & 0xFFu;
+            const uint32_t qi = (uw[b >> 2] >> (8 * (b & 3))) & 0xFFu;
+            uint32_t tr = 5u * (qn - 1u);
+            if ((next >> 16) == pos) {
+              const uint32_t cn = cseq[next & 0xFFFFu];
+              tr = 4u * (cn - 1u) + (qn - 1u);
+              --idx;
+              next = idx >= 0 ? subs[idx] : 0xFFFFFFFFu;
+            }
+            lam *= err_lds[tr * ncol + qi];
+          }
         }
-        const uint32_t qi = qq ? qq[pos] : 0u;
-        lam *= err_lds[tr * ncol + qi];
       }
       ham = cnt;
     }

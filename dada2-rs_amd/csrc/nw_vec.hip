@@ -3,26 +3,36 @@
 //
 // Mapping.  The reference stores cell (i, j) of anti-diagonal row = i + j in
 // compressed column (2*start_col + j - i) / 2 of an ncol-wide row
-// (nwalign.rs:783-787).  Here a wavefront holds G = 64 / ncol pairs; lane c
-// of group g owns column c of pair g for every row, so a column's values
-// stay in registers: prev2 (row-2, the diagonal source) and prev (row-1).
-// The left/up operands (nwalign.rs:957-964) are column c-1/c on even rows
-// and c/c+1 on odd rows: a one-lane DPP wavefront shift.  Columns 0 and
-// ncol-1 never become active (their lanes hold the sentinel), so a shift
-// across a group boundary is never consumed.
+// (nwalign.rs:783-787).  A wavefront holds G <= 64 / ncol pairs of one
+// (len1, len2) geometry; lane c of group g owns column c of pair g for every
+// row, so a column's scores stay in registers: prev2 (row-2, the diagonal
+// source) and prev (row-1).  The left/up operands (nwalign.rs:957-964) are
+// columns c-1/c on even rows and c/c+1 on odd rows: a one-lane DPP wavefront
+// shift.  Columns 0 and ncol-1 never become active, so a shift across a
+// group boundary is never consumed.
 //
-// The row schedule (col_min, col_max, i_max, j_min, band wedges, ends-free
-// recalculation flags; nwalign.rs:865-1088) depends only on (len1, len2,
-// band); every pair of a wavefront shares them, so it runs in scalar
-// registers.  Saturating i16 arithmetic is emulated on i32 with clamps.
+// Schedule.  The row walk (col_min, col_max, i_max, j_min, band wedges,
+// ends-free recalculation; nwalign.rs:865-1088) depends only on the
+// geometry.  sched_kernel replays it once per bin into a table of RowRec
+// records; the DP reads one record per row through scalar loads, so the
+// per-row scalar work is a load instead of the walk.  The cell update is
+// branch-free and specialised on the swap flag (the tie precedence of
+// dploop, nwalign.rs:700-737).  Saturating i16 arithmetic is emulated on
+// i32 with clamps.
 //
-// Traceback pointers (2 bits per cell) are ballots: two 64-bit masks per row
-// collected one row per lane and flushed to LDS every 64 rows as one 16-byte
-// record per row.  One lane per pair then walks the pointers
-// (nwalign.rs:1101-1139) and either emits the alignment strings (parity
-// mode) or the substitution list al2subs would build (nwalign.rs:1169-1224),
-// as (query position << 8 | 4*(center nt-1) + (query nt-1)) records for the
-// lambda kernel.
+// Traceback.  Each lane appends its 2-bit pointer per row to a private
+// history word flushed to LDS every 16 rows.  A diagonal step over a
+// mismatch is stored as pointer 0; only active cells carry diagonal
+// pointers, so the traceback tells it from the reference's invalid pointer
+// by the row's active range.  The traceback (nwalign.rs:1101-1139) sweeps
+// the rows downwards once: per row two ballots rebuild the row's pointer
+// mask and every lane advances its pair's path cell (the path state is
+// replicated across the pair's lanes).  Mismatching diagonal steps become
+// al2subs substitutions (nwalign.rs:1169-1224), written straight to global
+// memory as (query position << 16 | center position) for the lambda
+// kernel; in parity mode the alignment strings are written instead.
+#include <type_traits>
+
 #include "d2g_launch.h"
 
 namespace d2g {
@@ -33,6 +43,8 @@ struct NwDev {
   const uint32_t* wave_start;
   const DpTask* tasks;
   const uint8_t* pool;
+  const RowRec* sched;
+  int sched_rows;
   int match, mismatch, gap, end_gap;
   Counters* ctr;
   int emit;
@@ -41,11 +53,124 @@ struct NwDev {
   uint32_t* sub_top;
   uint32_t* sub_off;
   uint32_t* sub_cnt;
+  uint32_t* hist;
+  size_t hist_stride;
   uint8_t* al0;
   uint8_t* al1;
   const uint64_t* al_off;
   uint32_t* al_len;
 };
+
+// ---------------------------------------------------------------------------
+// Schedule: a replay of the reference's row walk, one record per row 2..last.
+// ---------------------------------------------------------------------------
+D2G_HD void vec_schedule(const VecGeom& geo, int end_gap, int gap, RowRec* out) {
+  const int len1 = geo.len1, len2 = geo.len2, band = geo.band, start_col = geo.start_col;
+  const int lfl = geo.left_fill_limit, tfl = geo.top_fill_limit, top_limit = geo.top_limit;
+  const int band_min_len1 = imin(band, len1);
+  const bool do_recalc = end_gap > gap;
+  int col_min = start_col, col_max = start_col, i_max = 0, j_min = 0;
+  bool recalc_left = false, recalc_right = false, even = true;
+  int row_free = end_gap;
+  for (int row = 2; row <= len1 + len2; ++row) {
+    RowRec R;
+    const int n = col_max - col_min + 1;
+    row_free = sat16(row_free + end_gap);
+    int k_lo = i_max - len1 + 1;
+    k_lo = k_lo < 0 ? 0 : (k_lo > n ? n : k_lo);
+    int k_hi = imin(i_max + 1, len2 - j_min);
+    k_hi = k_hi < 0 ? 0 : (k_hi > n ? n : k_hi);
+    if (k_hi < k_lo) k_hi = k_lo;
+    R.a1 = i_max + col_min;
+    R.a2 = j_min - col_min;
+    R.inr_lo = col_min + k_lo;
+    R.inr_n = (uint32_t)(k_hi - k_lo);
+    R.cmin = col_min;
+    R.n = (uint32_t)n;
+    R.edge_l = row < lfl ? imax(start_col - 1 - (row - 1) / 2, 0) : -1;
+    R.edge_t = row < tfl ? start_col + row / 2 : -1;
+    R.row_free = row_free;
+    uint32_t flags = 0;
+    // band transition (nwalign.rs:980-987)
+    const bool trans_min = row == band_min_len1;
+    if (trans_min) {
+      col_min = imax(col_min - 1, 0);
+      i_max += 1;
+      j_min -= 1;
+      flags |= 4u;
+    }
+    if (row == top_limit) col_max += 1;
+    // ends-free recalculation flags (nwalign.rs:993-1026)
+    if (do_recalc) {
+      if (recalc_left) flags |= 1u;
+      if (i_max == len1 - 1) recalc_left = true;
+      if (recalc_right) flags |= 2u;
+      if ((row + 1) / 2 + col_max - start_col == len2) recalc_right = true;
+    }
+    flags |= ((uint32_t)col_min & 0xFFu) << 8;
+    flags |= ((uint32_t)col_max & 0xFFu) << 16;
+    R.flags = flags;
+    out[row - 2] = R;
+    // index walk (nwalign.rs:1032-1079)
+    if (row < band && row < len1) {
+      if (even) col_min = imax(col_min - 1, 0);
+      i_max += 1;
+    } else if (i_max < len1 - 1) {
+      if (band % 2 == 0) {
+        if (even) j_min += 1;
+        else i_max += 1;
+      } else if (even) {
+        col_min = imax(col_min - 1, 0);
+        i_max += 1;
+      } else {
+        col_min += 1;
+        j_min += 1;
+      }
+    } else {
+      if (!even) col_min += 1;
+      j_min += 1;
+    }
+    if (row < top_limit) {
+      if (!even) col_max += 1;
+    } else if ((row + 1) / 2 + col_max - start_col < len2) {
+      if ((band + len2 - len1) % 2 == 0) {
+        if (even) col_max = imax(col_max - 1, 0);
+        else col_max += 1;
+      }
+    } else if (even) {
+      col_max = imax(col_max - 1, 0);
+    }
+    even = !even;
+  }
+}
+
+__global__ void sched_kernel(BinSpace bs, int rows, int end_gap, int gap, RowRec* table) {
+  const int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= bs.nbins) return;
+  int l1, l2;
+  bin_lengths(bs, (uint32_t)b, &l1, &l2);
+  const VecGeom g = vec_geom(l1, l2, bs.band);
+  if (nw_groups(g) == 0 || g.nrow > rows) return;
+  RowRec* t = table + (size_t)b * rows;
+  vec_schedule(g, end_gap, gap, t);
+  // two lookahead records past the last row (rows last+1, last+2)
+  const int nlast = g.nrow - 3;  // index of row len1 + len2
+  t[nlast + 1] = t[nlast];
+  t[nlast + 2] = t[nlast];
+}
+
+hipError_t launch_sched(const BinSpace& bs, int rows, int end_gap, int gap, RowRec* table,
+                        hipStream_t s) {
+  const int threads = 64;
+  const int blocks = (bs.nbins + threads - 1) / threads;
+  hipLaunchKernelGGL(sched_kernel, dim3(blocks), dim3(threads), 0, s, bs, rows, end_gap, gap,
+                     table);
+  return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// DP
+// ---------------------------------------------------------------------------
 
 // lane i <- lane i-1 (DPP wave_shr:1); lane 0 keeps `old`
 __device__ __forceinline__ int from_lower(int v, int old) {
@@ -58,149 +183,135 @@ __device__ __forceinline__ int from_upper(int v, int old) {
 
 __device__ __forceinline__ int uni(int v) { return __builtin_amdgcn_readfirstlane(v); }
 
+__device__ __forceinline__ int clamp16(int v) { return v > 32767 ? 32767 : (v < -32768 ? -32768 : v); }
+
 // Row constants of one wavefront (uniform).
 struct RowConst {
-  int len1, len2, start_col, lfl, tfl, top_limit, band_min_len1, band_mod2, full_band, last, band;
-  int match, mismatch, gap, end_gap, fill;
-  bool swap, do_recalc;
-};
-// Row walk state (uniform).
-struct RowState {
-  int col_min, col_max, i_max, j_min, row_free;
-  bool recalc_left, recalc_right;
-  uint64_t cells;
+  int match, mismatch, gap, end_gap, fill, last;
 };
 // Per-lane column state.
 struct LaneState {
-  int prev2, prev, acc0, acc1, acc2, acc3;
+  int prev2, prev;
+  uint32_t hp;  // pointer history, 2 bits x 16 rows
 };
 
-// One anti-diagonal (nwalign.rs:875-1088) for every pair of the wavefront.
-template <bool EVEN>
-__device__ __forceinline__ void nw_row(const int row, RowState& st, LaneState& ls, const RowConst K,
-                                       const uint8_t* s1b, const uint8_t* s2b, const int c,
-                                       const int lane, uint4* lds_ptr) {
-  const int n = st.col_max - st.col_min + 1;
-  st.row_free = sat16(st.row_free + K.end_gap);
-  const int edgeL = row < K.lfl ? imax(K.start_col - 1 - (row - 1) / 2, 0) : -1;
-  const int edgeT = row < K.tfl ? K.start_col + row / 2 : -1;
-  int k_lo = st.i_max - K.len1 + 1;
-  k_lo = k_lo < 0 ? 0 : (k_lo > n ? n : k_lo);
-  int k_hi = imin(st.i_max + 1, K.len2 - st.j_min);
-  k_hi = k_hi < 0 ? 0 : (k_hi > n ? n : k_hi);
-  if (k_hi < k_lo) k_hi = k_lo;
-  st.cells += (uint64_t)n;
-
-  // cell (i, j) of this lane: si = i_max - k, sj = j_min + k (nwalign.rs:922-953)
-  const int k = c - st.col_min;
-  const int ca = s1b[st.i_max - k];
-  const int cb = s2b[st.j_min + k];
-  const bool inr = (unsigned)(k - k_lo) < (unsigned)(k_hi - k_lo);
-  const int score = inr ? (ca == cb ? K.match : K.mismatch) : K.fill;
-  const int dg = sat16(ls.prev2 + score);
+// One anti-diagonal for every pair of the wavefront (nwalign.rs:875-1026).
+// ca, cb: this lane's bases s1[i_max - k] and s2[j_min + k], k = c - col_min
+// (nwalign.rs:922-953), loaded a row ahead by the caller.
+template <bool EVEN, bool SWAP>
+__device__ __forceinline__ void nw_row(const int row, const RowRec R, LaneState& ls,
+                                       const RowConst K, const int ca, const int cb, const int c,
+                                       const int lane, uint32_t* hist_p) {
+  const bool inr = (unsigned)(c - R.inr_lo) < R.inr_n;
+  const bool differ = ca != cb;
+  const int score = inr ? (differ ? K.mismatch : K.match) : K.fill;
+  const int dg = clamp16(ls.prev2 + score);
   const int lraw = EVEN ? from_lower(ls.prev, K.fill) : ls.prev;
   const int uraw = EVEN ? ls.prev : from_upper(ls.prev, K.fill);
-  const int l = sat16(lraw + K.gap), u = sat16(uraw + K.gap);
-  // dploop precedence (nwalign.rs:700-737)
-  const bool upf = K.swap ? (l < u) : (u >= l);
-  int e = upf ? u : l;
+  const int l = clamp16(lraw + K.gap), u = clamp16(uraw + K.gap);
+  // dploop precedence (nwalign.rs:700-737); max(l, u) is the entry either way
+  const bool upf = SWAP ? (l < u) : (u >= l);
+  int e = l > u ? l : u;
   int p = upf ? 3 : 2;
-  if (dg > e) {
-    e = dg;
-    p = 1;
-  }
-  if ((unsigned)k >= (unsigned)n) {
-    // outside [col_min, col_max]: sentinel or ends-free edge (nwalign.rs:884-899)
-    const bool isT = c == edgeT, isL = c == edgeL;
-    e = (isT || isL) ? st.row_free : K.fill;
-    p = isT ? 2 : (isL ? 3 : 0);
-  }
-  // band transition (nwalign.rs:980-987)
-  const bool trans_min = row == K.band_min_len1;
-  if (trans_min) {
-    st.col_min = imax(st.col_min - 1, 0);
-    st.i_max += 1;
-    st.j_min -= 1;
-  }
-  if (row == K.top_limit) st.col_max += 1;
-  // ends-free recalculation (nwalign.rs:993-1026)
-  if (K.do_recalc) {
-    const bool rl = st.recalc_left;
-    if (st.i_max == K.len1 - 1) st.recalc_left = true;
-    const bool rr = st.recalc_right;
-    if ((row + 1) / 2 + st.col_max - K.start_col == K.len2) st.recalc_right = true;
-    if (rl) {
-      const int d_free = sat16((trans_min ? uraw : lraw) + K.end_gap);
-      if (c == st.col_min) {
-        if (d_free > e) {
-          e = d_free;
-          p = 2;
-        } else if (d_free == e && (K.swap ? p != 2 : p == 1)) {
-          p = 2;
-        }
-      }
+  p = dg > e ? 1 : p;
+  e = dg > e ? dg : e;
+  // outside [col_min, col_max]: sentinel or ends-free edge (nwalign.rs:884-899)
+  const bool act = (unsigned)(c - R.cmin) < R.n;
+  const bool isT = c == R.edge_t, isL = c == R.edge_l;
+  e = act ? e : ((isT || isL) ? R.row_free : K.fill);
+  p = act ? p : (isT ? 2 : (isL ? 3 : 0));
+  // ends-free recalculation (nwalign.rs:993-1026), only on the last rows
+  if (R.flags & 3u) {
+    const bool trans_min = (R.flags & 4u) != 0;
+    const int cmin_new = (int)((R.flags >> 8) & 0xFFu);
+    const int cmax_new = (int)((R.flags >> 16) & 0xFFu);
+    if (R.flags & 1u) {
+      const int d_free = clamp16((trans_min ? uraw : lraw) + K.end_gap);
+      const bool here = c == cmin_new;
+      const bool take = d_free > e;
+      const bool tie = d_free == e && (SWAP ? p != 2 : p == 1);
+      e = (here && take) ? d_free : e;
+      p = (here && (take || tie)) ? 2 : p;
     }
-    if (rr) {
+    if (R.flags & 2u) {
       const int src = trans_min ? from_upper(uraw, K.fill) : uraw;
-      const int d_free = sat16(src + K.end_gap);
-      if (c == st.col_max) {
-        if (d_free > e) {
-          e = d_free;
-          p = 3;
-        } else if (d_free == e && (K.swap ? p == 1 : p != 3)) {
-          p = 3;
-        }
-      }
+      const int d_free = clamp16(src + K.end_gap);
+      const bool here = c == cmax_new;
+      const bool take = d_free > e;
+      const bool tie = d_free == e && (SWAP ? p == 1 : p != 3);
+      e = (here && take) ? d_free : e;
+      p = (here && (take || tie)) ? 3 : p;
     }
   }
-  // pointer record of this row
-  const uint64_t b0 = __builtin_amdgcn_ballot_w64((p & 1) != 0);
-  const uint64_t b1 = __builtin_amdgcn_ballot_w64((p & 2) != 0);
-  const int slot = row & 63;
-  const bool mine = lane == slot;
-  ls.acc0 = mine ? (int)(uint32_t)b0 : ls.acc0;
-  ls.acc1 = mine ? (int)(uint32_t)b1 : ls.acc1;
-  ls.acc2 = mine ? (int)(uint32_t)(b0 >> 32) : ls.acc2;
-  ls.acc3 = mine ? (int)(uint32_t)(b1 >> 32) : ls.acc3;
-  if (slot == 63 || row == K.last) {
-    const int rbase = row & ~63;
-    if (rbase + lane <= row) lds_ptr[rbase + lane] = make_uint4(ls.acc0, ls.acc1, ls.acc2, ls.acc3);
+  // pointer history.  A diagonal pointer over a mismatch (an al2subs
+  // substitution: both bases in 1..4 and different) is stored as 0; the
+  // traceback tells it from an unwritten cell because only active cells
+  // carry diagonal pointers.
+  const bool mm = inr && differ && (unsigned)(ca - 1) < 4u && (unsigned)(cb - 1) < 4u;
+  p = (p == 1 && mm) ? 0 : p;
+  ls.hp |= (uint32_t)p << (2 * (row & 15));
+  if ((row & 15) == 15 || row == K.last) {
+    hist_p[(row >> 4) * 64 + lane] = ls.hp;
+    ls.hp = 0;
   }
   ls.prev2 = ls.prev;
   ls.prev = e;
-  // index walk (nwalign.rs:1032-1079)
-  if (row < K.band && row < K.len1) {
-    if (EVEN) st.col_min = imax(st.col_min - 1, 0);
-    st.i_max += 1;
-  } else if (st.i_max < K.len1 - 1) {
-    if (K.band_mod2 == 0) {
-      if (EVEN) st.j_min += 1;
-      else st.i_max += 1;
-    } else if (EVEN) {
-      st.col_min = imax(st.col_min - 1, 0);
-      st.i_max += 1;
-    } else {
-      st.col_min += 1;
-      st.j_min += 1;
-    }
-  } else {
-    if (!EVEN) st.col_min += 1;
-    st.j_min += 1;
+}
+
+// The schedule is read-only for the DP: reading it through the constant
+// address space lets every record arrive by scalar loads.
+typedef const __attribute__((address_space(4))) uint32_t CWord;
+typedef CWord CRowRec;
+static_assert(sizeof(RowRec) == 40, "RowRec is ten words");
+
+__device__ __forceinline__ RowRec load_rec(CWord* w) {
+  RowRec R;
+  R.a1 = (int32_t)w[0];
+  R.a2 = (int32_t)w[1];
+  R.inr_lo = (int32_t)w[2];
+  R.inr_n = w[3];
+  R.cmin = (int32_t)w[4];
+  R.n = w[5];
+  R.edge_l = (int32_t)w[6];
+  R.edge_t = (int32_t)w[7];
+  R.row_free = (int32_t)w[8];
+  R.flags = w[9];
+  return R;
+}
+
+// Returns the number of DP cells of one pair (sum of the active widths).
+// Software pipelined: the bases of row r+1 and the record of row r+2 are
+// requested before row r is computed, so the one wait per row (LDS and
+// scalar loads share a counter) finds both long arrived.  The schedule holds
+// two padding records past the last row, copies of it, for the lookahead.
+template <bool SWAP>
+__device__ __forceinline__ uint32_t forward(CRowRec* rec, LaneState& ls, const RowConst K,
+                                            const uint8_t* s1l, const uint8_t* s2l, const int c,
+                                            const int lane, uint32_t* hist_p) {
+  const int last = K.last;
+  uint32_t cells = 0;
+  RowRec cur = load_rec(rec);
+  RowRec nxt = load_rec(rec + 10);
+  int ca = s1l[cur.a1], cb = s2l[cur.a2];
+  for (int row = 2; row <= last; row += 2) {
+    const int ca1 = s1l[nxt.a1], cb1 = s2l[nxt.a2];
+    const RowRec nx2 = load_rec(rec + 10 * row);  // row + 2
+    cells += cur.n;
+    nw_row<true, SWAP>(row, cur, ls, K, ca, cb, c, lane, hist_p);
+    if (row + 1 > last) break;
+    ca = s1l[nx2.a1];
+    cb = s2l[nx2.a2];
+    const RowRec nx3 = load_rec(rec + 10 * (row + 1));  // row + 3
+    cells += nxt.n;
+    nw_row<false, SWAP>(row + 1, nxt, ls, K, ca1, cb1, c, lane, hist_p);
+    cur = nx2;
+    nxt = nx3;
   }
-  if (row < K.top_limit) {
-    if (!EVEN) st.col_max += 1;
-  } else if ((row + 1) / 2 + st.col_max - K.start_col < K.len2) {
-    if (K.full_band % 2 == 0) {
-      if (EVEN) st.col_max = imax(st.col_max - 1, 0);
-      else st.col_max += 1;
-    }
-  } else if (EVEN) {
-    st.col_max = imax(st.col_max - 1, 0);
-  }
+  return cells;
 }
 
 __global__ __launch_bounds__(64) void nw_vec_kernel(NwDev a) {
-  extern __shared__ uint4 lds_ptr[];  // [nrow] pointer records, then group buffers
+  extern __shared__ uint32_t lds_words[];
   const int lane = (int)__lane_id();
   const BinSpace bs = a.bins;
   for (;;) {
@@ -223,197 +334,163 @@ __global__ __launch_bounds__(64) void nw_vec_kernel(NwDev a) {
     const uint32_t wave_in_bin = w - a.wave_start[bin];
     const uint32_t first = a.bin_start[bin] + wave_in_bin * (uint32_t)G;
     const uint32_t bin_end = a.bin_start[bin + 1];
-    const int count = (int)min((uint32_t)G, bin_end - first);
+    const int count = uni((int)min((uint32_t)G, bin_end - first));
 
     const int len1 = geo.len1, len2 = geo.len2, ncol = geo.ncol, nrow = geo.nrow;
-    const int start_col = geo.start_col, band = geo.band;
+    const int start_col = geo.start_col;
     const bool swap = geo.swap != 0;
     const int g_raw = lane / ncol;
     const int c = lane - g_raw * ncol;
-    const int g = g_raw < count ? g_raw : 0;  // idle lanes shadow group 0
+    const bool real = g_raw < count;
+    const int g = real ? g_raw : 0;  // idle lanes shadow group 0
     const DpTask task = a.tasks[first + (uint32_t)g];
 
-    // ---- stage the pair's sequences (internal s1 = shorter) in LDS ----
-    uint8_t* lbytes = reinterpret_cast<uint8_t*>(lds_ptr);
+    // ---- pointer history in this workgroup's global slot; LDS holds the
+    // padded sequences of every pair ----
+    uint32_t* hist_p = a.hist + (size_t)blockIdx.x * a.hist_stride;
+    uint8_t* gbase0 = reinterpret_cast<uint8_t*>(lds_words);
     const int pad = nw_pad(ncol);
     const size_t gbytes = nw_group_bytes(geo);
-    uint8_t* gbase = lbytes + (size_t)nrow * 16 + (size_t)g * gbytes;
+    uint8_t* gbase = gbase0 + (size_t)g * gbytes;
     uint8_t* s1b = gbase + pad;
     uint8_t* s2b = gbase + pad + len1 + 2 * pad;
-    uint32_t* subl = reinterpret_cast<uint32_t*>(
-        gbase + (((size_t)len1 + len2 + 4 * pad + 15) & ~(size_t)15));
     const uint8_t* src1 = a.pool + (swap ? task.s2_off : task.s1_off);
     const uint8_t* src2 = a.pool + (swap ? task.s1_off : task.s2_off);
-    if (g_raw < count) {
+    if (real) {
       for (int t = c; t < len1; t += ncol) s1b[t] = src1[t];
       for (int t = c; t < len2; t += ncol) s2b[t] = src2[t];
     }
     __syncthreads();
 
     // ---- scoring constants (nwalign.rs:817-818) ----
-    const int match = a.match, mismatch = a.mismatch, gap = a.gap, end_gap = a.end_gap;
-    int min_score = mismatch < gap ? mismatch : gap;
-    min_score = match < min_score ? match : min_score;
+    RowConst K;
+    K.match = a.match;
+    K.mismatch = a.mismatch;
+    K.gap = a.gap;
+    K.end_gap = a.end_gap;
+    int min_score = K.mismatch < K.gap ? K.mismatch : K.gap;
+    min_score = K.match < min_score ? K.match : min_score;
     min_score = min_score < 0 ? min_score : 0;
-    const int fill = (int)(int16_t)(uint16_t)(uint32_t)(-32768 - min_score);
-
-    const int lfl = geo.left_fill_limit, tfl = geo.top_fill_limit, top_limit = geo.top_limit;
-    const int band_min_len1 = imin(band, len1);
-    const int band_mod2 = band % 2;
-    const int full_band = band + len2 - len1;
-    const bool do_recalc = end_gap > gap;
-    const int last = len1 + len2;
+    K.fill = (int)(int16_t)(uint16_t)(uint32_t)(-32768 - min_score);
+    K.last = len1 + len2;
+    const int last = K.last;
 
     // ---- rows 0 and 1 (nwalign.rs:842-861) ----
-    int prev2 = (c == start_col) ? 0 : fill;
-    int prev, p1;
-    {
-      const bool isL = 1 < lfl && c == imax(start_col - 1, 0);
-      const bool isT = 1 < tfl && c == start_col;
-      prev = (isL || isT) ? end_gap : fill;
-      p1 = isT ? 2 : (isL ? 3 : 0);
-    }
-    int acc0 = 0, acc1 = 0, acc2 = 0, acc3 = 0;
-    {
-      const uint64_t b0 = __builtin_amdgcn_ballot_w64((p1 & 1) != 0);
-      const uint64_t b1 = __builtin_amdgcn_ballot_w64((p1 & 2) != 0);
-      if (lane == 1) {
-        acc0 = (int)(uint32_t)b0;
-        acc1 = (int)(uint32_t)b1;
-        acc2 = (int)(uint32_t)(b0 >> 32);
-        acc3 = (int)(uint32_t)(b1 >> 32);
-      }
-    }
-    if (last < 2) {  // no DP rows: flush rows 0..last
-      if (lane <= last) lds_ptr[lane] = make_uint4(acc0, acc1, acc2, acc3);
-    }
-
-    // ---- main anti-diagonal loop (nwalign.rs:875-1088) ----
-    int col_min = start_col, col_max = start_col, i_max = 0, j_min = 0;
-    bool recalc_left = false, recalc_right = false;
-    int row_free = end_gap;
-    uint64_t cells = 0;
-
-    RowConst K;
-    K.len1 = len1;
-    K.len2 = len2;
-    K.start_col = start_col;
-    K.lfl = lfl;
-    K.tfl = tfl;
-    K.top_limit = top_limit;
-    K.band_min_len1 = band_min_len1;
-    K.band_mod2 = band_mod2;
-    K.full_band = full_band;
-    K.last = last;
-    K.band = band;
-    K.match = match;
-    K.mismatch = mismatch;
-    K.gap = gap;
-    K.end_gap = end_gap;
-    K.fill = fill;
-    K.swap = swap;
-    K.do_recalc = do_recalc;
-    RowState st;
-    st.col_min = col_min;
-    st.col_max = col_max;
-    st.i_max = i_max;
-    st.j_min = j_min;
-    st.row_free = row_free;
-    st.recalc_left = recalc_left;
-    st.recalc_right = recalc_right;
-    st.cells = cells;
     LaneState ls;
-    ls.prev2 = prev2;
-    ls.prev = prev;
-    ls.acc0 = acc0;
-    ls.acc1 = acc1;
-    ls.acc2 = acc2;
-    ls.acc3 = acc3;
-    for (int row = 2; row <= last; row += 2) {
-      nw_row<true>(row, st, ls, K, s1b, s2b, c, lane, lds_ptr);
-      if (row + 1 <= last) nw_row<false>(row + 1, st, ls, K, s1b, s2b, c, lane, lds_ptr);
+    ls.prev2 = (c == start_col) ? 0 : K.fill;
+    {
+      const bool isL = 1 < geo.left_fill_limit && c == imax(start_col - 1, 0);
+      const bool isT = 1 < geo.top_fill_limit && c == start_col;
+      ls.prev = (isL || isT) ? K.end_gap : K.fill;
+      const int p1 = isT ? 2 : (isL ? 3 : 0);
+      ls.hp = (uint32_t)p1 << 2;  // row 0 holds only 0 pointers
     }
-    cells = st.cells;
+    if (last < 2) hist_p[lane] = ls.hp;
+
+    // ---- forward: anti-diagonals 2..len1+len2 ----
+    CRowRec* rec = (CRowRec*)(a.sched + (size_t)bin * a.sched_rows);
+    const uint8_t* s1l = s1b - c;
+    const uint8_t* s2l = s2b + c;
+    const uint32_t cells = swap ? forward<true>(rec, ls, K, s1l, s2l, c, lane, hist_p)
+                                : forward<false>(rec, ls, K, s1l, s2l, c, lane, hist_p);
+    if (lane == 0)
+      atomicAdd(&a.ctr->dp_cells, (unsigned long long)cells * (unsigned long long)count);
     __syncthreads();
 
-    if (lane == 0) atomicAdd(&a.ctr->dp_cells, (unsigned long long)(cells * (uint64_t)count));
-
-    // ---- traceback, one lane per pair (nwalign.rs:1101-1139) ----
-    if (c == 0 && g_raw < count) {
-      int i = len1, j = len2, nstep = 0, nsub = 0;
-      bool bad = false;
-      uint8_t* out0 = nullptr;
-      uint8_t* out1 = nullptr;
-      if (a.emit) {
-        const uint64_t o = a.al_off[task.out_idx];
-        out0 = a.al0 + o;
-        out1 = a.al1 + o;
+    // ---- backward sweep: every pair's traceback (nwalign.rs:1101-1139) ----
+    // Substitution records go straight to the pool: each pair reserves
+    // len1 + 1 slots (a bound on its diagonal steps) before the sweep.
+    uint32_t sub_base = 0;
+    bool sub_ok = true;
+    if (!a.emit) {
+      if (lane == 0) sub_base = atomicAdd(a.sub_top, (uint32_t)(count * (len1 + 1)));
+      sub_base = (uint32_t)uni((int)sub_base);
+      sub_ok = (uint64_t)sub_base + (uint64_t)count * (uint64_t)(len1 + 1) <= a.sub_capacity;
+      if (!sub_ok && lane == 0) atomicOr(&a.ctr->err_flags, kErrSubOverflow);
+    }
+    const uint32_t my_sub = sub_base + (uint32_t)g * (uint32_t)(len1 + 1);
+    // path state replicated over the pair's lanes
+    int rc = real ? last : -1;
+    int cc = (2 * start_col + len2 - len1) >> 1;
+    int ns = 0, nt = 0;
+    bool bad = false;
+    const int gpos = g_raw * ncol;  // bit of column 0 of this pair in a row mask
+    const uint64_t ao = (a.emit && real) ? a.al_off[task.out_idx] : 0;
+    // history words are read one 16-row block ahead of their use
+    uint32_t hp = hist_p[(last >> 4) * 64 + lane];
+    uint32_t hp_next = (last >> 4) >= 1 ? hist_p[((last >> 4) - 1) * 64 + lane] : 0u;
+    for (int r = last; r >= 1; --r) {
+      if ((r & 15) == 15 && r != last) {
+        hp = hp_next;
+        hp_next = (r >> 4) >= 1 ? hist_p[((r >> 4) - 1) * 64 + lane] : 0u;
       }
-      while (i > 0 || j > 0) {
-        const int cs = 2 * start_col + j - i;
-        const int col = cs >> 1;
-        if (cs < 0 || col >= ncol) {
-          bad = true;
-          break;
-        }
-        const int pos = g * ncol + col;
-        const uint4 m = lds_ptr[i + j];
-        const uint32_t w0 = pos < 32 ? m.x : m.z;
-        const uint32_t w1 = pos < 32 ? m.y : m.w;
-        const int sh = pos & 31;
-        const int pp = (int)((w0 >> sh) & 1u) | (int)(((w1 >> sh) & 1u) << 1);
-        int x0, x1;  // internal s1 side, s2 side
-        if (pp == 1) {
-          x0 = s1b[i - 1];
-          x1 = s2b[j - 1];
-          if (!a.emit && x0 != x1 && x0 >= 1 && x0 <= 4 && x1 >= 1 && x1 <= 4) {
+      const uint64_t B0 = __builtin_amdgcn_ballot_w64(((hp >> (2 * (r & 15))) & 1u) != 0);
+      const uint64_t B1 = __builtin_amdgcn_ballot_w64(((hp >> (2 * (r & 15) + 1)) & 1u) != 0);
+      const bool on = rc == r;
+      const int pos = gpos + cc;
+      const bool inb = (unsigned)cc < (unsigned)ncol;
+      const int b0 = inb ? (int)((B0 >> pos) & 1u) : 0;
+      const int b1 = inb ? (int)((B1 >> pos) & 1u) : 0;
+      const int pf = b0 | (b1 << 1);
+      // active columns of row r, looked up only when a path cell holds 0
+      // (row 1 has none: it holds only ends-free edge pointers)
+      bool cell_act = false;
+      if (r >= 2 && __builtin_amdgcn_ballot_w64(on && pf == 0)) {
+        const int rcmin = (int)rec[10 * (r - 2) + 4];
+        const uint32_t rn = rec[10 * (r - 2) + 5];
+        cell_act = (unsigned)(cc - rcmin) < rn;
+      }
+      // decode the folded pointer: 0 on an active cell is a mismatching diagonal
+      const bool sub = pf == 0 && cell_act;
+      const int pp = sub ? 1 : pf;
+      if (a.emit || __builtin_amdgcn_ballot_w64(on && sub)) {
+        const int dd = 2 * (cc - start_col) + (r & 1);  // j - i on this row
+        const int i = (r - dd) >> 1, j = (r + dd) >> 1;
+        if (on && c == cc && real) {
+          if (sub && !a.emit && sub_ok) {
             const int qp = swap ? i - 1 : j - 1;
-            const int cn = swap ? x1 : x0, qn = swap ? x0 : x1;
-            subl[nsub++] = ((uint32_t)qp << 8) | (uint32_t)(4 * (cn - 1) + (qn - 1));
+            const int cp = swap ? j - 1 : i - 1;
+            a.sub_pool[my_sub + ns] = ((uint32_t)qp << 16) | (uint32_t)cp;
           }
-          --i;
-          --j;
-        } else if (pp == 2) {
-          x0 = '-';
-          x1 = s2b[j - 1];
-          --j;
-        } else if (pp == 3) {
-          x0 = s1b[i - 1];
-          x1 = '-';
-          --i;
-        } else {
-          bad = true;
-          break;
-        }
-        if (a.emit) {
-          out0[nstep] = (uint8_t)(swap ? x1 : x0);
-          out1[nstep] = (uint8_t)(swap ? x0 : x1);
-        }
-        ++nstep;
-      }
-      if (bad) atomicOr(&a.ctr->err_flags, kErrTraceback);
-      const uint32_t ti = first + (uint32_t)g;
-      if (a.emit) {
-        for (int x = 0, y = nstep - 1; x < y; ++x, --y) {
-          uint8_t t0 = out0[x], t1 = out1[x];
-          out0[x] = out0[y];
-          out1[x] = out1[y];
-          out0[y] = t0;
-          out1[y] = t1;
-        }
-        a.al_len[task.out_idx] = bad ? 0u : (uint32_t)nstep;
-      } else {
-        const uint32_t off = atomicAdd(a.sub_top, (uint32_t)nsub);
-        if (off + (uint32_t)nsub > a.sub_capacity) {
-          atomicOr(&a.ctr->err_flags, kErrSubOverflow);
-          a.sub_off[ti] = 0;
-          a.sub_cnt[ti] = 0;
-        } else {
-          for (int s = 0; s < nsub; ++s) a.sub_pool[off + s] = subl[s];
-          a.sub_off[ti] = off;
-          a.sub_cnt[ti] = (uint32_t)nsub | (bad ? 0x80000000u : 0u);
+          if (a.emit && pp >= 1) {
+            const uint8_t x0 = pp == 2 ? (uint8_t)'-' : s1b[i - 1];
+            const uint8_t x1 = pp == 3 ? (uint8_t)'-' : s2b[j - 1];
+            a.al0[ao + nt] = swap ? x1 : x0;
+            a.al1[ao + nt] = swap ? x0 : x1;
+          }
         }
       }
+      ns += (on && sub) ? 1 : 0;
+      bad = bad || (on && pp == 0);
+      nt += (on && pp != 0) ? 1 : 0;
+      // step: diag (i-1, j-1) two rows up; left (i, j-1) and up (i-1, j) one row up
+      const int odd = r & 1;
+      cc = on ? (pp == 2 ? cc - (1 - odd) : (pp == 3 ? cc + odd : cc)) : cc;
+      rc = on ? (pp == 1 ? r - 2 : (pp == 0 ? -1 : r - 1)) : rc;
+    }
+    if (__builtin_amdgcn_ballot_w64(bad && real) && lane == 0)
+      atomicOr(&a.ctr->err_flags, kErrTraceback);
+    __syncthreads();
+
+    // ---- outputs (column 0 lane of each pair) ----
+    const uint32_t ti = first + (uint32_t)g;
+    if (a.emit) {
+      if (real) {
+        uint8_t* o0 = a.al0 + ao;
+        uint8_t* o1 = a.al1 + ao;
+        for (int x = c; x < nt / 2; x += ncol) {
+          const int y = nt - 1 - x;
+          const uint8_t t0 = o0[x], t1 = o1[x];
+          o0[x] = o0[y];
+          o1[x] = o1[y];
+          o0[y] = t0;
+          o1[y] = t1;
+        }
+        if (c == 0) a.al_len[task.out_idx] = bad ? 0u : (uint32_t)nt;
+      }
+    } else if (real && c == 0) {
+      a.sub_off[ti] = sub_ok ? my_sub : 0u;
+      a.sub_cnt[ti] = sub_ok ? ((uint32_t)ns | (bad ? 0x80000000u : 0u)) : 0u;
     }
     __syncthreads();
   }
@@ -426,6 +503,8 @@ hipError_t launch_nw_vec(const NwArgs& s, hipStream_t st) {
   d.wave_start = s.wave_start;
   d.tasks = s.tasks;
   d.pool = s.pool;
+  d.sched = s.sched;
+  d.sched_rows = s.sched_rows;
   d.match = s.match;
   d.mismatch = s.mismatch;
   d.gap = s.gap_p;
@@ -437,10 +516,13 @@ hipError_t launch_nw_vec(const NwArgs& s, hipStream_t st) {
   d.sub_top = s.sub_top;
   d.sub_off = s.sub_off;
   d.sub_cnt = s.sub_cnt;
+  d.hist = s.hist;
+  d.hist_stride = s.hist_stride;
   d.al0 = s.al0;
   d.al1 = s.al1;
   d.al_off = s.al_off;
   d.al_len = s.al_len;
+  if (s.grid == 0 || s.hist == nullptr || s.sched == nullptr) return hipErrorInvalidValue;
   const size_t lds = s.lds_bytes;
   if (lds > 48 * 1024) {
     hipError_t e = hipFuncSetAttribute((const void*)nw_vec_kernel,

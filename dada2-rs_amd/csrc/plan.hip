@@ -111,3 +111,36 @@ hipError_t launch_scatter(const DpTask* unsorted, DpTask* sorted, const uint32_t
 }
 
 }  // namespace d2g
+
+namespace d2g {
+
+// Results are produced in the length-sorted unique order; this restores the
+// caller's order: dst[c * n + orig[s]] = src[c * n + s].
+__global__ void unpermute_kernel(uint32_t nc, uint32_t n, const uint32_t* orig,
+                                 const double* src_lam, const uint32_t* src_ham,
+                                 const uint8_t* src_skp, double* dst_lam, uint32_t* dst_ham,
+                                 uint8_t* dst_skp) {
+  const size_t total = (size_t)nc * n;
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < total;
+       i += (size_t)gridDim.x * blockDim.x) {
+    const size_t c = i / n, s = i - c * n;
+    const size_t d = c * n + orig[s];
+    dst_lam[d] = src_lam[i];
+    dst_ham[d] = src_ham[i];
+    dst_skp[d] = src_skp[i];
+  }
+}
+
+hipError_t launch_unpermute(uint32_t nc, uint32_t n, const uint32_t* orig, const double* src_lam,
+                            const uint32_t* src_ham, const uint8_t* src_skp, double* dst_lam,
+                            uint32_t* dst_ham, uint8_t* dst_skp, hipStream_t s) {
+  size_t total = (size_t)nc * n;
+  size_t blocks = (total + 255) / 256;
+  if (blocks > 8192) blocks = 8192;
+  if (blocks == 0) blocks = 1;
+  hipLaunchKernelGGL(unpermute_kernel, dim3((unsigned)blocks), dim3(256), 0, s, nc, n, orig,
+                     src_lam, src_ham, src_skp, dst_lam, dst_ham, dst_skp);
+  return hipGetLastError();
+}
+
+}  // namespace d2g

@@ -130,7 +130,9 @@ __global__ __launch_bounds__(kScreenBlock) void screen_kernel(ScreenDev a) {
     const uint32_t ci = cidx[c];
     const uint32_t clen = u.len[ci];
     const uint32_t creads = u.reads[ci];
-    const uint32_t out_idx = (c0 + c) * u.n + u.orig[qq];
+    // results stay in the length-sorted order (coalesced); d2g_fetch_results
+    // restores the caller's unique order
+    const uint32_t out_idx = (c0 + c) * u.n + qq;
     // cluster.rs:295: greedy skip
     const bool skip = a.greedy && (my_reads > creads || my_lock);
     double kdist = 0.0;
