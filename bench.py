@@ -38,28 +38,72 @@ def parse():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--plain-centers", type=int, default=20,
                     help="centers timed one call each for the plain path")
-    ap.add_argument("--cpu-centers", type=int, default=6,
-                    help="centers of the CPU baseline sample")
+    ap.add_argument("--cpu-seconds", type=float, default=10.0,
+                    help="minimum CPU time of the baseline sample (whole passes)")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-calibration-mib", type=int, default=0,
+                    help="after the run, one device-to-device copy of this size (a plain copy "
+                         "kernel of known bytes that scales the HBM byte counters)")
     return ap.parse_args()
 
 
-def cpu_baseline(w, n_centers: int) -> dict:
+def cpu_baseline(w, min_seconds: float) -> dict:
+    """The oracle's compare (C, threads over chunks of raws) on whole passes of
+    the same workload: every center against every unique, repeated until at
+    least min_seconds of wall time have passed."""
     import oracle as O
 
     threads = O.cpu_count_allowed()
     rs = O.RawSet(w.seqs, w.quals, w.reads, k=w.kmer_size)
     p = O.make_params(band=w.band)
-    cs = [int(c) for c in w.centers[:n_centers]]
+    cs = [int(c) for c in w.centers]
     O.compare_center(rs, cs[0], w.err, w.err.shape[1], p, nthreads=threads)  # warm
+    passes = 0
     t0 = time.perf_counter()
-    for c in cs:
-        O.compare_center(rs, c, w.err, w.err.shape[1], p, nthreads=threads)
-    dt = time.perf_counter() - t0
-    comps = len(cs) * w.n
+    while True:
+        for c in cs:
+            O.compare_center(rs, c, w.err, w.err.shape[1], p, nthreads=threads)
+        passes += 1
+        dt = time.perf_counter() - t0
+        if dt >= min_seconds:
+            break
+    comps = passes * len(cs) * w.n
     return {"value": comps / dt, "unit": "comparisons/s", "cores": threads, "kind": "port",
-            "sample": f"{len(cs)} centers x {w.n} uniques of the same workload "
-                      f"({comps} comparisons, {dt:.2f} s, C oracle with {threads} threads)"}
+            "sample": f"{passes} passes of {len(cs)} centers x {w.n} uniques of the same workload "
+                      f"({comps} comparisons, {dt:.1f} s, C oracle with {threads} threads)"}
+
+
+def copy_calibration(mib: int) -> None:
+    """One hipMemcpy device-to-device of mib MiB: the runtime's copy kernel
+    moves exactly that many bytes each way, which calibrates FETCH_SIZE and
+    WRITE_SIZE in a profiled run."""
+    import ctypes as C
+
+    hip = C.CDLL("libamdhip64.so")
+    n = mib << 20
+    a, b = C.c_void_p(), C.c_void_p()
+    assert hip.hipMalloc(C.byref(a), C.c_size_t(n)) == 0
+    assert hip.hipMalloc(C.byref(b), C.c_size_t(n)) == 0
+    assert hip.hipMemset(a, 1, C.c_size_t(n)) == 0
+    assert hip.hipDeviceSynchronize() == 0
+    assert hip.hipMemcpy(b, a, C.c_size_t(n), 3) == 0  # hipMemcpyDeviceToDevice
+    assert hip.hipDeviceSynchronize() == 0
+    hip.hipFree(a)
+    hip.hipFree(b)
+
+
+def traffic_from_profile(kernel: str):
+    """HBM bytes per launch of `kernel` from the committed PMC summary
+    (profiles/traffic_batched.txt: 'kernel fetch_kib write_kib' lines already
+    scaled by the copy-kernel factor), or None."""
+    path = ROOT / "profiles" / "traffic_batched.txt"
+    if not path.exists():
+        return None
+    for line in path.read_text().splitlines():
+        f = line.split()
+        if len(f) == 3 and f[0] == kernel:
+            return (float(f[1]) + float(f[2])) * 1024.0
+    return None
 
 
 def main():
@@ -104,34 +148,41 @@ def main():
     kernels = {"screen": ms_screen, "plan+scatter": ms_plan, "nw_vec": ms_dp, "lambda": ms_lam}
     dominant = max(kernels, key=kernels.get)
     roof_screen = {"bound": "hbm", "achieved": screen_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                   "frac": screen_gbs / HBM_PEAK_GBS, "traffic": None,
+                   "frac": screen_gbs / HBM_PEAK_GBS, "traffic": traffic_from_profile("screen_kernel"),
                    "kernel": "screen_kernel", "algorithmic_bytes_per_launch": 1024 * screened,
                    "avg_ms": ms_screen}
     roof_dp = {"bound": "valu", "achieved": gcups * 8 / 1e3, "peak": valu_cells_peak * 8 / 1e12,
-               "unit": "TOP/s", "frac": gcups * 1e9 / valu_cells_peak, "traffic": None,
+               "unit": "TOP/s", "frac": gcups * 1e9 / valu_cells_peak,
+               "traffic": traffic_from_profile("nw_vec_kernel"),
                "kernel": "nw_vec_kernel", "gcups": gcups, "cells_per_launch": st["dp_cells"],
                "avg_ms": ms_dp}
     roofline = roof_screen if dominant in ("screen", "plan+scatter", "lambda") else roof_dp
 
     # plain path: one center per call, results copied to the host
-    pc = centers[:args.plain_centers]
-    ctx.set_timing(False)
-    ctx.compare_center(int(pc[0]), w.err, params)
-    t1 = time.perf_counter()
-    for c in pc:
-        ctx.compare_center(int(c), w.err, params)
-    plain_s = time.perf_counter() - t1
-    plain_value = len(pc) * w.n / plain_s
+    plain = None
+    if args.plain_centers > 0:
+        pc = centers[:args.plain_centers]
+        ctx.set_timing(False)
+        ctx.compare_center(int(pc[0]), w.err, params)
+        t1 = time.perf_counter()
+        for c in pc:
+            ctx.compare_center(int(c), w.err, params)
+        plain_s = time.perf_counter() - t1
+        plain_value = len(pc) * w.n / plain_s
+        plain = {"value": plain_value, "unit": "comparisons/s", "centers": int(len(pc)),
+                 "batched_over_plain": value / plain_value}
 
     # batched end to end, results copied to the host (PCIe inclusive)
     t2 = time.perf_counter()
     ctx.compare_batch(centers, w.err, params)
     e2e_value = n_pairs / (time.perf_counter() - t2)
     ctx.close()
+    if args.copy_calibration_mib > 0:
+        copy_calibration(args.copy_calibration_mib)
 
-    cpu = None if args.no_cpu else cpu_baseline(w, args.cpu_centers)
+    cpu = None if args.no_cpu else cpu_baseline(w, args.cpu_seconds)
     line = {
-        "metric": "center-unique comparisons/sec (batched compare, bit-exact align)",
+        "metric": "center-unique comparisons/sec + GCUPS, 1 GPU, batched vs plain, bit-exact align",
         "value": value, "unit": "comparisons/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": ms_step, "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None, "dtype": "i16 DP / f64 lambda",
@@ -145,8 +196,7 @@ def main():
         "kernel_ms": kernels,
         "roofline_screen": roof_screen,
         "roofline_dp": roof_dp,
-        "plain_path": {"value": plain_value, "unit": "comparisons/s",
-                       "centers": int(len(pc)), "batched_over_plain": value / plain_value},
+        "plain_path": plain,
         "batched_e2e_with_d2h": e2e_value,
         "counts": {k: st[k] for k in ("comparisons", "skipped", "shrouded", "gapless",
                                       "aligned_dp", "dp_cells")},

@@ -82,6 +82,7 @@ struct d2g_ctx {
   size_t lds_bytes = 0;
   int sched_rows = 0;
   DevBuf<RowRec> sched;
+  DevBuf<BinInfo> sched_info;
   // pointer-history scratch of the DP: nw_grid slots of hist_stride words
   uint32_t nw_grid = 0;
   size_t hist_stride = 0;
@@ -164,6 +165,7 @@ int d2g_ctx_destroy(d2g_ctx* c) {
   c->dp_sorted.release();
   c->ctr.release();
   c->sched.release();
+  c->sched_info.release();
   for (int i = 0; i < kEvents; ++i)
     if (c->ev[i]) (void)hipEventDestroy(c->ev[i]);
   if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -345,10 +347,19 @@ static int prepare_schedule(d2g_ctx* c, const BinSpace& bs, int end_gap, int gap
   c->sched_rows = nw_sched_rows(lmax);
   const size_t recs = (size_t)bs.nbins * (size_t)c->sched_rows;
   if (c->sched.ensure(recs) != hipSuccess) return D2G_ERR_HIP;
-  c->nw_grid = (uint32_t)c->cu_count * 16;
+  if (c->sched_info.ensure((size_t)bs.nbins) != hipSuccess) return D2G_ERR_HIP;
+  // persistent DP workgroups: 8 one-wave groups per SIMD where LDS allows,
+  // and at most 1 GiB of pointer-history scratch
+  int per_cu = 32;
+  if (c->lds_bytes > 0) per_cu = std::min<int>(per_cu, (int)((160 * 1024) / c->lds_bytes));
+  per_cu = std::max(per_cu, 1);
+  c->nw_grid = (uint32_t)c->cu_count * (uint32_t)per_cu;
   c->hist_stride = nw_hist_p_words(2 * lmax + 1);
+  while (c->nw_grid > (uint32_t)c->cu_count && (size_t)c->nw_grid * c->hist_stride * 4 > ((size_t)1 << 30))
+    c->nw_grid /= 2;
   if (c->hist.ensure((size_t)c->nw_grid * c->hist_stride) != hipSuccess) return D2G_ERR_HIP;
-  if (launch_sched(bs, c->sched_rows, end_gap, gap, c->sched.p, c->stream) != hipSuccess)
+  if (launch_sched(bs, c->sched_rows, end_gap, gap, c->sched.p, c->sched_info.p, c->stream) !=
+      hipSuccess)
     return D2G_ERR_HIP;
   if (hipStreamSynchronize(c->stream) != hipSuccess) return D2G_ERR_HIP;
   c->lds_key = bs;
@@ -473,6 +484,7 @@ static int compare_device(d2g_ctx* c, const uint32_t* centers, uint32_t nc, cons
     na.hist_stride = c->hist_stride;
     na.lds_bytes = c->lds_bytes;
     na.sched = c->sched.p;
+    na.info = c->sched_info.p;
     na.sched_rows = c->sched_rows;
     HIP_TRY(launch_nw_vec(na, s));
     if (c->timing) HIP_TRY(hipEventRecord(c->ev[3], s));
@@ -659,6 +671,7 @@ int d2g_align_vectorized(d2g_ctx* c, uint32_t n, const uint8_t* s1, const uint64
   na.hist_stride = c->hist_stride;
   na.lds_bytes = c->lds_bytes;
   na.sched = c->sched.p;
+  na.info = c->sched_info.p;
   na.sched_rows = c->sched_rows;
   HIP_TRY(launch_nw_vec(na, s));
   Counters h;

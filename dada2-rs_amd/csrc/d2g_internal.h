@@ -71,15 +71,23 @@ D2G_HD size_t nw_wave_bytes(const VecGeom& g, int G) { return (size_t)G * nw_gro
 struct RowRec {
   int32_t a1;       // i_max + col_min: s1 byte offset of column 0's cell
   int32_t a2;       // j_min - col_min: s2 byte offset of column 0's cell
-  int32_t inr_lo;   // col_min + k_lo: first column whose cell is inside both sequences
-  uint32_t inr_n;   // k_hi - k_lo
   int32_t cmin;     // col_min
   uint32_t n;       // active width col_max - col_min + 1
+  int32_t inr_lo;   // col_min + k_lo: first column whose cell is inside both sequences
+  uint32_t inr_n;   // k_hi - k_lo
   int32_t edge_l;   // left ends-free edge column, -1 when none (nwalign.rs:891-894)
   int32_t edge_t;   // top ends-free edge column, -1 when none (nwalign.rs:895-898)
   int32_t row_free; // accumulated ends-free edge score (nwalign.rs:884)
   uint32_t flags;   // bit0 left recalc, bit1 right recalc, bit2 band wedge,
                     // bits 8-15 col_min after the wedge, bits 16-23 col_max after it
+};
+
+// Per bin: the rows [steady_lo, steady_hi) (multiples of 16) where no
+// ends-free edge or recalculation applies and every active cell lies inside
+// both sequences, so the DP runs its short row update there; and the DP
+// cells of one pair (sum of the active widths).
+struct BinInfo {
+  uint32_t steady_lo, steady_hi, cells, _pad;
 };
 
 // Records of the schedule table per bin of a BinSpace whose longest length

@@ -73,6 +73,7 @@ struct NwArgs {
   const DpTask* tasks;         // sorted by bin
   const uint8_t* pool;         // sequence bytes
   const RowRec* sched;         // per-bin row schedule (launch_sched)
+  const BinInfo* info;         // per-bin steady range and cell count (launch_sched)
   int sched_rows;              // records per bin
   int16_t match, mismatch, gap_p, end_gap_p;
   Counters* ctr;
@@ -97,7 +98,7 @@ struct NwArgs {
 hipError_t launch_nw_vec(const NwArgs& a, hipStream_t s);
 // Replays the vectorized row walk of every bin into table[bin * rows + row - 2].
 hipError_t launch_sched(const BinSpace& bs, int rows, int end_gap, int gap, RowRec* table,
-                        hipStream_t s);
+                        BinInfo* info, hipStream_t s);
 size_t nw_vec_lds_for(const BinSpace& bs);
 
 // ---- lambda.hip --------------------------------------------------------

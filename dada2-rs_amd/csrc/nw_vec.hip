@@ -44,6 +44,7 @@ struct NwDev {
   const DpTask* tasks;
   const uint8_t* pool;
   const RowRec* sched;
+  const BinInfo* info;
   int sched_rows;
   int match, mismatch, gap, end_gap;
   Counters* ctr;
@@ -144,7 +145,8 @@ D2G_HD void vec_schedule(const VecGeom& geo, int end_gap, int gap, RowRec* out) 
   }
 }
 
-__global__ void sched_kernel(BinSpace bs, int rows, int end_gap, int gap, RowRec* table) {
+__global__ void sched_kernel(BinSpace bs, int rows, int end_gap, int gap, RowRec* table,
+                             BinInfo* info) {
   const int b = blockIdx.x * blockDim.x + threadIdx.x;
   if (b >= bs.nbins) return;
   int l1, l2;
@@ -154,17 +156,43 @@ __global__ void sched_kernel(BinSpace bs, int rows, int end_gap, int gap, RowRec
   RowRec* t = table + (size_t)b * rows;
   vec_schedule(g, end_gap, gap, t);
   // two lookahead records past the last row (rows last+1, last+2)
-  const int nlast = g.nrow - 3;  // index of row len1 + len2
-  t[nlast + 1] = t[nlast];
-  t[nlast + 2] = t[nlast];
+  const int last = g.len1 + g.len2;
+  t[last - 1] = t[last - 2];
+  t[last] = t[last - 2];
+  // longest run of steady rows, trimmed to whole 16-row history words
+  int best_lo = 0, best_hi = 0, run_lo = -1;
+  uint32_t cells = 0;
+  for (int row = 2; row <= last; ++row) {
+    const RowRec& R = t[row - 2];
+    cells += R.n;
+    const bool steady = R.edge_l < 0 && R.edge_t < 0 && (R.flags & 3u) == 0 &&
+                        R.inr_lo == R.cmin && R.inr_n == R.n;
+    if (!steady) {
+      run_lo = -1;
+      continue;
+    }
+    if (run_lo < 0) run_lo = row;
+    if (row + 1 - run_lo > best_hi - best_lo) {
+      best_lo = run_lo;
+      best_hi = row + 1;
+    }
+  }
+  int lo = (best_lo + 15) & ~15, hi = best_hi & ~15;
+  if (hi <= lo) lo = hi = last + 1;
+  BinInfo bi;
+  bi.steady_lo = (uint32_t)lo;
+  bi.steady_hi = (uint32_t)hi;
+  bi.cells = cells;
+  bi._pad = 0;
+  info[b] = bi;
 }
 
 hipError_t launch_sched(const BinSpace& bs, int rows, int end_gap, int gap, RowRec* table,
-                        hipStream_t s) {
+                        BinInfo* info, hipStream_t s) {
   const int threads = 64;
   const int blocks = (bs.nbins + threads - 1) / threads;
   hipLaunchKernelGGL(sched_kernel, dim3(blocks), dim3(threads), 0, s, bs, rows, end_gap, gap,
-                     table);
+                     table, info);
   return hipGetLastError();
 }
 
@@ -243,12 +271,11 @@ __device__ __forceinline__ void nw_row(const int row, const RowRec R, LaneState&
       p = (here && (take || tie)) ? 3 : p;
     }
   }
-  // pointer history.  A diagonal pointer over a mismatch (an al2subs
-  // substitution: both bases in 1..4 and different) is stored as 0; the
-  // traceback tells it from an unwritten cell because only active cells
-  // carry diagonal pointers.
-  const bool mm = inr && differ && (unsigned)(ca - 1) < 4u && (unsigned)(cb - 1) < 4u;
-  p = (p == 1 && mm) ? 0 : p;
+  // pointer history.  A diagonal pointer over a mismatch is stored as 0;
+  // the traceback tells it from an unwritten cell because only active cells
+  // carry diagonal pointers.  (In the compare path every base is in 1..4,
+  // so each such step is an al2subs substitution.)
+  p = (p == 1 && inr && differ) ? 0 : p;
   ls.hp |= (uint32_t)p << (2 * (row & 15));
   if ((row & 15) == 15 || row == K.last) {
     hist_p[(row >> 4) * 64 + lane] = ls.hp;
@@ -268,10 +295,10 @@ __device__ __forceinline__ RowRec load_rec(CWord* w) {
   RowRec R;
   R.a1 = (int32_t)w[0];
   R.a2 = (int32_t)w[1];
-  R.inr_lo = (int32_t)w[2];
-  R.inr_n = w[3];
-  R.cmin = (int32_t)w[4];
-  R.n = w[5];
+  R.cmin = (int32_t)w[2];
+  R.n = w[3];
+  R.inr_lo = (int32_t)w[4];
+  R.inr_n = w[5];
   R.edge_l = (int32_t)w[6];
   R.edge_t = (int32_t)w[7];
   R.row_free = (int32_t)w[8];
@@ -279,38 +306,109 @@ __device__ __forceinline__ RowRec load_rec(CWord* w) {
   return R;
 }
 
-// Returns the number of DP cells of one pair (sum of the active widths).
-// Software pipelined: the bases of row r+1 and the record of row r+2 are
-// requested before row r is computed, so the one wait per row (LDS and
-// scalar loads share a counter) finds both long arrived.  The schedule holds
-// two padding records past the last row, copies of it, for the lookahead.
-template <bool SWAP>
-__device__ __forceinline__ uint32_t forward(CRowRec* rec, LaneState& ls, const RowConst K,
-                                            const uint8_t* s1l, const uint8_t* s2l, const int c,
-                                            const int lane, uint32_t* hist_p) {
-  const int last = K.last;
-  uint32_t cells = 0;
-  RowRec cur = load_rec(rec);
-  RowRec nxt = load_rec(rec + 10);
-  int ca = s1l[cur.a1], cb = s2l[cur.a2];
-  for (int row = 2; row <= last; row += 2) {
-    const int ca1 = s1l[nxt.a1], cb1 = s2l[nxt.a2];
-    const RowRec nx2 = load_rec(rec + 10 * row);  // row + 2
-    cells += cur.n;
-    nw_row<true, SWAP>(row, cur, ls, K, ca, cb, c, lane, hist_p);
-    if (row + 1 > last) break;
-    ca = s1l[nx2.a1];
-    cb = s2l[nx2.a2];
-    const RowRec nx3 = load_rec(rec + 10 * (row + 1));  // row + 3
-    cells += nxt.n;
-    nw_row<false, SWAP>(row + 1, nxt, ls, K, ca1, cb1, c, lane, hist_p);
-    cur = nx2;
-    nxt = nx3;
-  }
-  return cells;
+// The four fields a steady row needs.
+struct Rec4 {
+  int32_t a1, a2, cmin;
+  uint32_t n;
+};
+__device__ __forceinline__ Rec4 load_rec4(CWord* w) {
+  Rec4 R;
+  R.a1 = (int32_t)w[0];
+  R.a2 = (int32_t)w[1];
+  R.cmin = (int32_t)w[2];
+  R.n = w[3];
+  return R;
 }
 
-__global__ __launch_bounds__(64) void nw_vec_kernel(NwDev a) {
+// A steady row (no ends-free edge, no recalculation, every active cell
+// inside both sequences): nw_row reduced to what such a row can reach.
+// Returns the cell's pointer code.
+template <bool EVEN, bool SWAP>
+__device__ __forceinline__ uint32_t steady_cell(const Rec4 R, const int ca, const int cb,
+                                                LaneState& ls, const RowConst K, const int c) {
+  const bool differ = ca != cb;
+  const int dg = clamp16(ls.prev2 + (differ ? K.mismatch : K.match));
+  const int lraw = EVEN ? from_lower(ls.prev, K.fill) : ls.prev;
+  const int uraw = EVEN ? ls.prev : from_upper(ls.prev, K.fill);
+  const int l = clamp16(lraw + K.gap), u = clamp16(uraw + K.gap);
+  const bool upf = SWAP ? (l < u) : (u >= l);
+  const int m = l > u ? l : u;
+  const bool dw = dg > m;
+  const bool act = (unsigned)(c - R.cmin) < R.n;
+  int p = dw ? (differ ? 0 : 1) : (upf ? 3 : 2);
+  p = act ? p : 0;
+  ls.prev2 = ls.prev;
+  ls.prev = act ? (dw ? dg : m) : K.fill;
+  return (uint32_t)p;
+}
+
+// Rows [r0, r1) through the full row update.  Software pipelined: the bases
+// of row r+1 and the record of row r+2 are requested before row r is
+// computed, so the one wait per row (LDS and scalar loads share a counter)
+// finds both long arrived.  The schedule holds two lookahead records past
+// the last row.
+template <bool SWAP>
+__device__ __forceinline__ void generic_rows(const int r0, const int r1, CRowRec* rec,
+                                             LaneState& ls, const RowConst K, const uint8_t* s1l,
+                                             const uint8_t* s2l, const int c, const int lane,
+                                             uint32_t* hist_p) {
+  if (r0 >= r1) return;
+  RowRec cur = load_rec(rec + 10 * (r0 - 2));
+  RowRec nxt = load_rec(rec + 10 * (r0 - 1));
+  int ca = s1l[cur.a1], cb = s2l[cur.a2];
+  for (int row = r0; row < r1; ++row) {
+    const int ca1 = s1l[nxt.a1], cb1 = s2l[nxt.a2];
+    const RowRec nx2 = load_rec(rec + 10 * row);  // row + 2
+    if (row & 1)
+      nw_row<false, SWAP>(row, cur, ls, K, ca, cb, c, lane, hist_p);
+    else
+      nw_row<true, SWAP>(row, cur, ls, K, ca, cb, c, lane, hist_p);
+    cur = nxt;
+    nxt = nx2;
+    ca = ca1;
+    cb = cb1;
+  }
+}
+
+// Steady rows [b0, b1), both multiples of 16: one history word per 16-row
+// block, fully unrolled, pipelined like generic_rows.
+template <bool SWAP>
+__device__ __forceinline__ void steady_rows(const int b0, const int b1, CRowRec* rec,
+                                            LaneState& ls, const RowConst K, const uint8_t* s1l,
+                                            const uint8_t* s2l, const int c, const int lane,
+                                            uint32_t* hist_p) {
+  if (b0 >= b1) return;
+  Rec4 cur = load_rec4(rec + 10 * (b0 - 2));
+  Rec4 nxt = load_rec4(rec + 10 * (b0 - 1));
+  int ca = s1l[cur.a1], cb = s2l[cur.a2];
+  for (int row = b0; row < b1; row += 16) {
+    uint32_t hp = 0;
+#pragma unroll
+    for (int k = 0; k < 16; ++k) {
+      const int ca1 = s1l[nxt.a1], cb1 = s2l[nxt.a2];
+      const Rec4 nx2 = load_rec4(rec + 10 * (row + k));  // row + k + 2
+      const uint32_t p = (k & 1) ? steady_cell<false, SWAP>(cur, ca, cb, ls, K, c)
+                                 : steady_cell<true, SWAP>(cur, ca, cb, ls, K, c);
+      hp |= p << (2 * k);
+      cur = nxt;
+      nxt = nx2;
+      ca = ca1;
+      cb = cb1;
+    }
+    hist_p[(row >> 4) * 64 + lane] = hp;
+  }
+}
+
+template <bool SWAP>
+__device__ __forceinline__ void forward(CRowRec* rec, const int b0, const int b1, LaneState& ls,
+                                        const RowConst K, const uint8_t* s1l, const uint8_t* s2l,
+                                        const int c, const int lane, uint32_t* hist_p) {
+  generic_rows<SWAP>(2, b0, rec, ls, K, s1l, s2l, c, lane, hist_p);
+  steady_rows<SWAP>(b0, b1, rec, ls, K, s1l, s2l, c, lane, hist_p);
+  generic_rows<SWAP>(b1, K.last + 1, rec, ls, K, s1l, s2l, c, lane, hist_p);
+}
+
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(8, 8))) void nw_vec_kernel(NwDev a) {
   extern __shared__ uint32_t lds_words[];
   const int lane = (int)__lane_id();
   const BinSpace bs = a.bins;
@@ -389,12 +487,16 @@ __global__ __launch_bounds__(64) void nw_vec_kernel(NwDev a) {
 
     // ---- forward: anti-diagonals 2..len1+len2 ----
     CRowRec* rec = (CRowRec*)(a.sched + (size_t)bin * a.sched_rows);
+    const BinInfo bi = a.info[bin];
+    const int b0 = uni((int)bi.steady_lo), b1 = uni((int)bi.steady_hi);
     const uint8_t* s1l = s1b - c;
     const uint8_t* s2l = s2b + c;
-    const uint32_t cells = swap ? forward<true>(rec, ls, K, s1l, s2l, c, lane, hist_p)
-                                : forward<false>(rec, ls, K, s1l, s2l, c, lane, hist_p);
+    if (swap)
+      forward<true>(rec, b0, b1, ls, K, s1l, s2l, c, lane, hist_p);
+    else
+      forward<false>(rec, b0, b1, ls, K, s1l, s2l, c, lane, hist_p);
     if (lane == 0)
-      atomicAdd(&a.ctr->dp_cells, (unsigned long long)cells * (unsigned long long)count);
+      atomicAdd(&a.ctr->dp_cells, (unsigned long long)bi.cells * (unsigned long long)count);
     __syncthreads();
 
     // ---- backward sweep: every pair's traceback (nwalign.rs:1101-1139) ----
@@ -436,8 +538,8 @@ __global__ __launch_bounds__(64) void nw_vec_kernel(NwDev a) {
       // (row 1 has none: it holds only ends-free edge pointers)
       bool cell_act = false;
       if (r >= 2 && __builtin_amdgcn_ballot_w64(on && pf == 0)) {
-        const int rcmin = (int)rec[10 * (r - 2) + 4];
-        const uint32_t rn = rec[10 * (r - 2) + 5];
+        const int rcmin = (int)rec[10 * (r - 2) + 2];
+        const uint32_t rn = rec[10 * (r - 2) + 3];
         cell_act = (unsigned)(cc - rcmin) < rn;
       }
       // decode the folded pointer: 0 on an active cell is a mismatching diagonal
@@ -504,6 +606,7 @@ hipError_t launch_nw_vec(const NwArgs& s, hipStream_t st) {
   d.tasks = s.tasks;
   d.pool = s.pool;
   d.sched = s.sched;
+  d.info = s.info;
   d.sched_rows = s.sched_rows;
   d.match = s.match;
   d.mismatch = s.mismatch;
