@@ -408,6 +408,102 @@ __device__ __forceinline__ void forward(CRowRec* rec, const int b0, const int b1
   generic_rows<SWAP>(b1, K.last + 1, rec, ls, K, s1l, s2l, c, lane, hist_p);
 }
 
+// ---------------------------------------------------------------------------
+// Traceback
+// ---------------------------------------------------------------------------
+// Per-wave constants of the traceback.
+struct BwConst {
+  CRowRec* rec;
+  int gpos;  // lane of column 0 of this lane's pair
+  int c, start_col;
+  bool swap, real, sub_ok;
+  uint32_t my_sub;  // first substitution slot of this lane's pair
+  const uint8_t* s1b;
+  const uint8_t* s2b;
+  uint64_t ao;  // alignment string offset (parity mode)
+};
+// Path state of this lane's pair, replicated over the pair's lanes.
+struct PathState {
+  int rc;    // row of the path cell; -1 once the path has ended
+  int addr;  // 4 x the lane owning the path cell's column (a ds_bpermute address)
+  int ns, nt;
+  bool bad;
+};
+
+// One row of the traceback (nwalign.rs:1101-1139) for every pair: each
+// lane fetches the history word of its pair's path column from the owning
+// lane (ds_bpermute) and advances the path when the path cell is on row r.
+// k = r mod 16 selects the pointer within the word; odd = r mod 2.
+template <bool EMIT>
+__device__ __forceinline__ void bw_row(const int r, const int k, const int odd, const uint32_t hp,
+                                       PathState& ps, const BwConst& B, const NwDev& a) {
+  const uint32_t w = (uint32_t)__builtin_amdgcn_ds_bpermute(ps.addr, (int)hp);
+  int pp = (int)((w >> (2 * k)) & 3u);
+  const bool on = ps.rc == r;
+  const int cc = (ps.addr >> 2) - B.gpos;
+  // pointer 0: a mismatching diagonal on an active cell, else the
+  // reference's invalid pointer (row 1 holds no active cells)
+  if (__builtin_amdgcn_ballot_w64(on && pp == 0)) {
+    bool act = false;
+    if (r >= 2) {
+      const int rcmin = (int)B.rec[10 * (r - 2) + 2];
+      const uint32_t rn = B.rec[10 * (r - 2) + 3];
+      act = (unsigned)(cc - rcmin) < rn;
+    }
+    const bool sub = on && pp == 0 && act;
+    if (!EMIT && sub && B.c == cc && B.real && B.sub_ok) {
+      const int dd = 2 * (cc - B.start_col) + odd;  // j - i on this row
+      const int i = (r - dd) >> 1, j = (r + dd) >> 1;
+      const int qp = B.swap ? i - 1 : j - 1;
+      const int cp = B.swap ? j - 1 : i - 1;
+      a.sub_pool[B.my_sub + ps.ns] = ((uint32_t)qp << 16) | (uint32_t)cp;
+    }
+    ps.ns += sub ? 1 : 0;
+    pp = sub ? 1 : pp;
+  }
+  if (EMIT) {
+    if (on && pp != 0 && B.c == cc && B.real) {
+      const int dd = 2 * (cc - B.start_col) + odd;
+      const int i = (r - dd) >> 1, j = (r + dd) >> 1;
+      const uint8_t x0 = pp == 2 ? (uint8_t)'-' : B.s1b[i - 1];
+      const uint8_t x1 = pp == 3 ? (uint8_t)'-' : B.s2b[j - 1];
+      a.al0[B.ao + ps.nt] = B.swap ? x1 : x0;
+      a.al1[B.ao + ps.nt] = B.swap ? x0 : x1;
+    }
+    ps.nt += (on && pp != 0) ? 1 : 0;
+  }
+  ps.bad = ps.bad || (on && pp == 0);
+  // step: diag (i-1, j-1) two rows up; left (i, j-1) and up (i-1, j) one row
+  // up, i.e. one column down on even rows / up on odd rows
+  if (odd)
+    ps.addr += (on && pp == 3) ? 4 : 0;
+  else
+    ps.addr -= (on && pp == 2) ? 4 : 0;
+  ps.rc = on ? (pp == 1 ? r - 2 : (pp == 0 ? -1 : r - 1)) : ps.rc;
+}
+
+// Rows last..1 in 16-row history blocks; each block's words are loaded one
+// block ahead, and whole blocks run fully unrolled.
+template <bool EMIT>
+__device__ __forceinline__ void backward(const int last, const uint32_t* hist_p, const int lane,
+                                         PathState& ps, const BwConst& B, const NwDev& a) {
+  const int top = last >> 4;
+  uint32_t hp = hist_p[top * 64 + lane];
+  uint32_t hp_next = top >= 1 ? hist_p[(top - 1) * 64 + lane] : 0u;
+  const int rlo = top * 16 > 1 ? top * 16 : 1;
+  for (int r = last; r >= rlo; --r) bw_row<EMIT>(r, r & 15, r & 1, hp, ps, B, a);
+  for (int b = top - 1; b >= 1; --b) {
+    hp = hp_next;
+    hp_next = hist_p[(b - 1) * 64 + lane];
+#pragma unroll
+    for (int k = 15; k >= 0; --k) bw_row<EMIT>(b * 16 + k, k, k & 1, hp, ps, B, a);
+  }
+  if (top >= 1) {
+    hp = hp_next;
+    for (int r = 15; r >= 1; --r) bw_row<EMIT>(r, r, r & 1, hp, ps, B, a);
+  }
+}
+
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(8, 8))) void nw_vec_kernel(NwDev a) {
   extern __shared__ uint32_t lds_words[];
   const int lane = (int)__lane_id();
@@ -510,66 +606,30 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(8, 8))) void
       sub_ok = (uint64_t)sub_base + (uint64_t)count * (uint64_t)(len1 + 1) <= a.sub_capacity;
       if (!sub_ok && lane == 0) atomicOr(&a.ctr->err_flags, kErrSubOverflow);
     }
-    const uint32_t my_sub = sub_base + (uint32_t)g * (uint32_t)(len1 + 1);
-    // path state replicated over the pair's lanes
-    int rc = real ? last : -1;
-    int cc = (2 * start_col + len2 - len1) >> 1;
-    int ns = 0, nt = 0;
-    bool bad = false;
-    const int gpos = g_raw * ncol;  // bit of column 0 of this pair in a row mask
-    const uint64_t ao = (a.emit && real) ? a.al_off[task.out_idx] : 0;
-    // history words are read one 16-row block ahead of their use
-    uint32_t hp = hist_p[(last >> 4) * 64 + lane];
-    uint32_t hp_next = (last >> 4) >= 1 ? hist_p[((last >> 4) - 1) * 64 + lane] : 0u;
-    for (int r = last; r >= 1; --r) {
-      if ((r & 15) == 15 && r != last) {
-        hp = hp_next;
-        hp_next = (r >> 4) >= 1 ? hist_p[((r >> 4) - 1) * 64 + lane] : 0u;
-      }
-      const uint64_t B0 = __builtin_amdgcn_ballot_w64(((hp >> (2 * (r & 15))) & 1u) != 0);
-      const uint64_t B1 = __builtin_amdgcn_ballot_w64(((hp >> (2 * (r & 15) + 1)) & 1u) != 0);
-      const bool on = rc == r;
-      const int pos = gpos + cc;
-      const bool inb = (unsigned)cc < (unsigned)ncol;
-      const int b0 = inb ? (int)((B0 >> pos) & 1u) : 0;
-      const int b1 = inb ? (int)((B1 >> pos) & 1u) : 0;
-      const int pf = b0 | (b1 << 1);
-      // active columns of row r, looked up only when a path cell holds 0
-      // (row 1 has none: it holds only ends-free edge pointers)
-      bool cell_act = false;
-      if (r >= 2 && __builtin_amdgcn_ballot_w64(on && pf == 0)) {
-        const int rcmin = (int)rec[10 * (r - 2) + 2];
-        const uint32_t rn = rec[10 * (r - 2) + 3];
-        cell_act = (unsigned)(cc - rcmin) < rn;
-      }
-      // decode the folded pointer: 0 on an active cell is a mismatching diagonal
-      const bool sub = pf == 0 && cell_act;
-      const int pp = sub ? 1 : pf;
-      if (a.emit || __builtin_amdgcn_ballot_w64(on && sub)) {
-        const int dd = 2 * (cc - start_col) + (r & 1);  // j - i on this row
-        const int i = (r - dd) >> 1, j = (r + dd) >> 1;
-        if (on && c == cc && real) {
-          if (sub && !a.emit && sub_ok) {
-            const int qp = swap ? i - 1 : j - 1;
-            const int cp = swap ? j - 1 : i - 1;
-            a.sub_pool[my_sub + ns] = ((uint32_t)qp << 16) | (uint32_t)cp;
-          }
-          if (a.emit && pp >= 1) {
-            const uint8_t x0 = pp == 2 ? (uint8_t)'-' : s1b[i - 1];
-            const uint8_t x1 = pp == 3 ? (uint8_t)'-' : s2b[j - 1];
-            a.al0[ao + nt] = swap ? x1 : x0;
-            a.al1[ao + nt] = swap ? x0 : x1;
-          }
-        }
-      }
-      ns += (on && sub) ? 1 : 0;
-      bad = bad || (on && pp == 0);
-      nt += (on && pp != 0) ? 1 : 0;
-      // step: diag (i-1, j-1) two rows up; left (i, j-1) and up (i-1, j) one row up
-      const int odd = r & 1;
-      cc = on ? (pp == 2 ? cc - (1 - odd) : (pp == 3 ? cc + odd : cc)) : cc;
-      rc = on ? (pp == 1 ? r - 2 : (pp == 0 ? -1 : r - 1)) : rc;
-    }
+    BwConst B;
+    B.rec = rec;
+    B.gpos = g_raw * ncol;
+    B.c = c;
+    B.start_col = start_col;
+    B.swap = swap;
+    B.real = real;
+    B.sub_ok = sub_ok;
+    B.my_sub = sub_base + (uint32_t)g * (uint32_t)(len1 + 1);
+    B.s1b = s1b;
+    B.s2b = s2b;
+    B.ao = (a.emit && real) ? a.al_off[task.out_idx] : 0;
+    PathState ps;
+    ps.rc = real ? last : -1;
+    ps.addr = (B.gpos + ((2 * start_col + len2 - len1) >> 1)) * 4;
+    ps.ns = 0;
+    ps.nt = 0;
+    ps.bad = false;
+    if (a.emit)
+      backward<true>(last, hist_p, lane, ps, B, a);
+    else
+      backward<false>(last, hist_p, lane, ps, B, a);
+    const int ns = ps.ns, nt = ps.nt;
+    const bool bad = ps.bad;
     if (__builtin_amdgcn_ballot_w64(bad && real) && lane == 0)
       atomicOr(&a.ctr->err_flags, kErrTraceback);
     __syncthreads();
@@ -578,8 +638,8 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(8, 8))) void
     const uint32_t ti = first + (uint32_t)g;
     if (a.emit) {
       if (real) {
-        uint8_t* o0 = a.al0 + ao;
-        uint8_t* o1 = a.al1 + ao;
+        uint8_t* o0 = a.al0 + B.ao;
+        uint8_t* o1 = a.al1 + B.ao;
         for (int x = c; x < nt / 2; x += ncol) {
           const int y = nt - 1 - x;
           const uint8_t t0 = o0[x], t1 = o1[x];
@@ -591,7 +651,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(8, 8))) void
         if (c == 0) a.al_len[task.out_idx] = bad ? 0u : (uint32_t)nt;
       }
     } else if (real && c == 0) {
-      a.sub_off[ti] = sub_ok ? my_sub : 0u;
+      a.sub_off[ti] = sub_ok ? B.my_sub : 0u;
       a.sub_cnt[ti] = sub_ok ? ((uint32_t)ns | (bad ? 0x80000000u : 0u)) : 0u;
     }
     __syncthreads();
