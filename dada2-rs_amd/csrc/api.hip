@@ -72,7 +72,7 @@ struct d2g_ctx {
   DevBuf<GaplessTask> gl;
   DevBuf<DpTask> dp_unsorted, dp_sorted;
   DevBuf<uint32_t> bin_count, bin_start, wave_start, bin_fill;
-  DevBuf<uint32_t> sub_pool, sub_off, sub_cnt, sub_top;
+  DevBuf<uint32_t> sub_pool, sub_cnt, screen_partials;
   DevBuf<Counters> ctr;
   size_t sub_capacity = 0;
   uint32_t last_nc = 0;
@@ -153,7 +153,7 @@ int d2g_ctx_destroy(d2g_ctx* c) {
   c->fin_hamming.release();
   for (auto* b : {&c->len, &c->reads, &c->orig, &c->kmer_row, &c->kmer_il, &c->kord_row, &c->kord_il,
                   &c->ksum, &c->centers, &c->out_hamming, &c->bin_count, &c->bin_start, &c->wave_start,
-                  &c->bin_fill, &c->sub_pool, &c->sub_off, &c->sub_cnt, &c->sub_top, &c->a_allen,
+                  &c->bin_fill, &c->sub_pool, &c->sub_cnt, &c->screen_partials, &c->a_allen,
                   &c->hist})
     b->release();
   c->off.release();
@@ -409,13 +409,14 @@ static int compare_device(d2g_ctx* c, const uint32_t* centers, uint32_t nc, cons
   HIP_TRY(c->gl.ensure(npairs));
   HIP_TRY(c->dp_unsorted.ensure(npairs));
   HIP_TRY(c->dp_sorted.ensure(npairs));
-  HIP_TRY(c->sub_off.ensure(npairs));
   HIP_TRY(c->sub_cnt.ensure(npairs));
   HIP_TRY(c->bin_count.ensure(bs.nbins + 1));
   HIP_TRY(c->bin_start.ensure(bs.nbins + 1));
   HIP_TRY(c->wave_start.ensure(bs.nbins + 1));
   HIP_TRY(c->bin_fill.ensure(bs.nbins + 1));
-  HIP_TRY(c->sub_top.ensure(1));
+  const uint32_t n_part = screen_blocks(n, nc);
+  HIP_TRY(c->screen_partials.ensure((size_t)2 * n_part));
+  const uint32_t sub_stride = (uint32_t)(bs.lmin + bs.range);  // longest length + 1
   if (c->sub_capacity < (size_t)1 << 20) c->sub_capacity = (size_t)1 << 20;
   HIP_TRY(c->sub_pool.ensure(c->sub_capacity));
   hipStream_t s = c->stream;
@@ -436,7 +437,6 @@ static int compare_device(d2g_ctx* c, const uint32_t* centers, uint32_t nc, cons
     if (c->timing) HIP_TRY(hipEventRecord(c->ev[0], s));
     HIP_TRY(launch_zero(reinterpret_cast<uint32_t*>(c->ctr.p), sizeof(Counters) / 4, s));
     HIP_TRY(launch_zero(c->bin_count.p, bs.nbins + 1, s));
-    HIP_TRY(launch_zero(c->sub_top.p, 1, s));
     ScreenArgs sa;
     sa.u = &u;
     sa.centers = c->centers.p;
@@ -454,10 +454,12 @@ static int compare_device(d2g_ctx* c, const uint32_t* centers, uint32_t nc, cons
     sa.gapless_tasks = c->gl.p;
     sa.dp_unsorted = c->dp_unsorted.p;
     sa.bin_count = c->bin_count.p;
+    sa.partials = c->screen_partials.p;
     sa.ctr = c->ctr.p;
     HIP_TRY(launch_screen(sa, s));
     if (c->timing) HIP_TRY(hipEventRecord(c->ev[1], s));
-    HIP_TRY(launch_plan(bs, c->bin_count.p, c->bin_start.p, c->wave_start.p, c->bin_fill.p, c->ctr.p, s));
+    HIP_TRY(launch_plan(bs, c->bin_count.p, c->bin_start.p, c->wave_start.p, c->bin_fill.p, c->ctr.p,
+                        c->sched_info.p, c->screen_partials.p, n_part, s));
     HIP_TRY(launch_scatter(c->dp_unsorted.p, c->dp_sorted.p, c->bin_start.p, c->bin_fill.p, c->ctr.p,
                            (uint32_t)npairs, s));
     if (c->timing) HIP_TRY(hipEventRecord(c->ev[2], s));
@@ -475,9 +477,8 @@ static int compare_device(d2g_ctx* c, const uint32_t* centers, uint32_t nc, cons
     na.ctr = c->ctr.p;
     na.emit_alignment = 0;
     na.sub_pool = c->sub_pool.p;
-    na.sub_capacity = (uint32_t)std::min<size_t>(c->sub_capacity, 0xFFFFFFF0u);
-    na.sub_top = c->sub_top.p;
-    na.sub_off = c->sub_off.p;
+    na.sub_capacity = c->sub_capacity;
+    na.sub_stride = sub_stride;
     na.sub_cnt = c->sub_cnt.p;
     na.grid = c->nw_grid;
     na.hist = c->hist.p;
@@ -493,7 +494,7 @@ static int compare_device(d2g_ctx* c, const uint32_t* centers, uint32_t nc, cons
     la.gapless_tasks = c->gl.p;
     la.dp_tasks = c->dp_sorted.p;
     la.sub_pool = c->sub_pool.p;
-    la.sub_off = c->sub_off.p;
+    la.sub_stride = sub_stride;
     la.sub_cnt = c->sub_cnt.p;
     la.err = c->err.p;
     la.ncol = ncol;
@@ -509,10 +510,8 @@ static int compare_device(d2g_ctx* c, const uint32_t* centers, uint32_t nc, cons
     HIP_TRY(hipMemcpyAsync(&h, c->ctr.p, sizeof(Counters), hipMemcpyDeviceToHost, s));
     HIP_TRY(hipStreamSynchronize(s));
     if (h.err_flags & kErrSubOverflow) {
-      // each DP pair reserves (shorter length + 1) records: this bound fits all
-      const size_t need = (size_t)h.n_dp * (u.maxlen + 1);
-      c->sub_capacity = std::max(c->sub_capacity * 4, need);
-      if (c->sub_capacity > 0xFFFFFFF0u) return D2G_ERR_UNSUPPORTED;
+      // every DP pair owns sub_stride records
+      c->sub_capacity = std::max(c->sub_capacity * 2, (size_t)h.n_dp * sub_stride);
       HIP_TRY(c->sub_pool.ensure(c->sub_capacity));
       continue;
     }
@@ -646,7 +645,9 @@ int d2g_align_vectorized(d2g_ctx* c, uint32_t n, const uint8_t* s1, const uint64
   h0.n_dp = n;
   HIP_TRY(hipMemcpyAsync(c->ctr.p, &h0, sizeof(Counters), hipMemcpyHostToDevice, s));
   HIP_TRY(hipStreamSynchronize(s));
-  HIP_TRY(launch_plan(bs, c->bin_count.p, c->bin_start.p, c->wave_start.p, c->bin_fill.p, c->ctr.p, s));
+  HIP_TRY_INT(prepare_schedule(c, bs, sc->end_gap_p, sc->gap_p));
+  HIP_TRY(launch_plan(bs, c->bin_count.p, c->bin_start.p, c->wave_start.p, c->bin_fill.p, c->ctr.p,
+                      c->sched_info.p, nullptr, 0, s));
   HIP_TRY(launch_scatter(c->dp_unsorted.p, c->dp_sorted.p, c->bin_start.p, c->bin_fill.p, c->ctr.p, n, s));
   NwArgs na;
   std::memset(&na, 0, sizeof(na));
@@ -665,7 +666,6 @@ int d2g_align_vectorized(d2g_ctx* c, uint32_t n, const uint8_t* s1, const uint64
   na.al1 = c->a_al1.p;
   na.al_off = c->a_aloff.p;
   na.al_len = c->a_allen.p;
-  HIP_TRY_INT(prepare_schedule(c, bs, sc->end_gap_p, sc->gap_p));
   na.grid = c->nw_grid;
   na.hist = c->hist.p;
   na.hist_stride = c->hist_stride;

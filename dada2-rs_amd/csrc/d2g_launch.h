@@ -50,14 +50,20 @@ struct ScreenArgs {
   GaplessTask* gapless_tasks;
   DpTask* dp_unsorted;
   uint32_t* bin_count;
+  uint32_t* partials;  // per block: skipped, shrouded (screen_blocks entries)
   Counters* ctr;
 };
 hipError_t launch_screen(const ScreenArgs& a, hipStream_t s);
+uint32_t screen_blocks(uint32_t n_uniques, uint32_t n_centers);
 
 // ---- plan.hip ----------------------------------------------------------
 hipError_t launch_zero(uint32_t* p, size_t n_words, hipStream_t s);
+// Also totals the DP cells (info: per-bin cells, nullable) and the screen's
+// per-block skip/shroud partials (pairs of words, nullable).
 hipError_t launch_plan(const BinSpace& bs, const uint32_t* bin_count, uint32_t* bin_start,
-                       uint32_t* wave_start, uint32_t* bin_fill, Counters* ctr, hipStream_t s);
+                       uint32_t* wave_start, uint32_t* bin_fill, Counters* ctr,
+                       const BinInfo* info, const uint32_t* partials, uint32_t n_partials,
+                       hipStream_t s);
 hipError_t launch_scatter(const DpTask* unsorted, DpTask* sorted, const uint32_t* bin_start,
                           uint32_t* bin_fill, const Counters* ctr, uint32_t capacity,
                           hipStream_t s);
@@ -78,12 +84,12 @@ struct NwArgs {
   int16_t match, mismatch, gap_p, end_gap_p;
   Counters* ctr;
   int emit_alignment;
-  // compare mode: per sorted task, substitution list (query_pos << 16 | center_pos)
+  // compare mode: per sorted task ti, substitution list (query_pos << 16 |
+  // center_pos) at sub_pool[ti * sub_stride]
   uint32_t* sub_pool;
-  uint32_t sub_capacity;
-  uint32_t* sub_top;       // bump allocator
-  uint32_t* sub_off;       // per task
-  uint32_t* sub_cnt;       // per task
+  size_t sub_capacity;
+  uint32_t sub_stride;     // longest sequence of the bin space + 1
+  uint32_t* sub_cnt;       // per task: count, bit 31 = invalid traceback
   // pointer history scratch: grid slots of hist_stride words
   uint32_t* hist;
   size_t hist_stride;
@@ -106,8 +112,8 @@ struct LambdaArgs {
   const UniqueDev* u;
   const GaplessTask* gapless_tasks;
   const DpTask* dp_tasks;  // sorted
-  const uint32_t* sub_pool;
-  const uint32_t* sub_off;
+  const uint32_t* sub_pool;  // task di's list at sub_pool[di * sub_stride]
+  uint32_t sub_stride;
   const uint32_t* sub_cnt;
   const double* err;
   uint32_t ncol;

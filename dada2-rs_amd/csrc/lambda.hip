@@ -20,7 +20,7 @@ struct LambdaDev {
   const GaplessTask* gl;
   const DpTask* dp;
   const uint32_t* sub_pool;
-  const uint32_t* sub_off;
+  uint32_t sub_stride;
   const uint32_t* sub_cnt;
   const double* err;
   uint32_t ncol;
@@ -85,7 +85,7 @@ __global__ __launch_bounds__(256) void lambda_kernel(LambdaDev a) {
       const uint32_t cnt_raw = a.sub_cnt[di];
       if (cnt_raw & 0x80000000u) atomicOr(&a.ctr->err_flags, kErrTraceback);
       const uint32_t cnt = cnt_raw & 0x7FFFFFFFu;
-      const uint32_t* subs = a.sub_pool + a.sub_off[di];
+      const uint32_t* subs = a.sub_pool + (size_t)di * a.sub_stride;
       const uint8_t* cseq = a.u.seq + a.u.off[task.c_idx];
       // records are (query position << 16 | center position), descending
       int idx = (int)cnt - 1;
@@ -126,7 +126,7 @@ hipError_t launch_lambda(const LambdaArgs& s, hipStream_t st) {
   d.gl = s.gapless_tasks;
   d.dp = s.dp_tasks;
   d.sub_pool = s.sub_pool;
-  d.sub_off = s.sub_off;
+  d.sub_stride = s.sub_stride;
   d.sub_cnt = s.sub_cnt;
   d.err = s.err;
   d.ncol = s.ncol;

@@ -50,9 +50,8 @@ struct NwDev {
   Counters* ctr;
   int emit;
   uint32_t* sub_pool;
-  uint32_t sub_capacity;
-  uint32_t* sub_top;
-  uint32_t* sub_off;
+  size_t sub_capacity;
+  uint32_t sub_stride;
   uint32_t* sub_cnt;
   uint32_t* hist;
   size_t hist_stride;
@@ -417,7 +416,7 @@ struct BwConst {
   int gpos;  // lane of column 0 of this lane's pair
   int c, start_col;
   bool swap, real, sub_ok;
-  uint32_t my_sub;  // first substitution slot of this lane's pair
+  size_t my_sub;  // first substitution slot of this lane's pair
   const uint8_t* s1b;
   const uint8_t* s2b;
   uint64_t ao;  // alignment string offset (parity mode)
@@ -508,12 +507,10 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(8, 8))) void
   extern __shared__ uint32_t lds_words[];
   const int lane = (int)__lane_id();
   const BinSpace bs = a.bins;
-  for (;;) {
-    uint32_t w = 0;
-    if (lane == 0) w = atomicAdd(&a.ctr->wave_counter, 1u);
-    w = (uint32_t)uni((int)w);
-    const uint32_t total = (uint32_t)uni((int)a.ctr->total_waves);
-    if (w >= total) break;
+  // wavefronts are assigned statically (the grid covers the usual batch in
+  // one pass; pairs of one launch cost about the same)
+  const uint32_t total = (uint32_t)uni((int)a.ctr->total_waves);
+  for (uint32_t w = blockIdx.x; w < total; w += gridDim.x) {
     // bin of wavefront w: last b with wave_start[b] <= w
     int lo = 0, hi = bs.nbins;  // invariant wave_start[lo] <= w < wave_start[hi]
     while (hi - lo > 1) {
@@ -591,21 +588,16 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(8, 8))) void
       forward<true>(rec, b0, b1, ls, K, s1l, s2l, c, lane, hist_p);
     else
       forward<false>(rec, b0, b1, ls, K, s1l, s2l, c, lane, hist_p);
-    if (lane == 0)
-      atomicAdd(&a.ctr->dp_cells, (unsigned long long)bi.cells * (unsigned long long)count);
     __syncthreads();
 
     // ---- backward sweep: every pair's traceback (nwalign.rs:1101-1139) ----
-    // Substitution records go straight to the pool: each pair reserves
-    // len1 + 1 slots (a bound on its diagonal steps) before the sweep.
-    uint32_t sub_base = 0;
-    bool sub_ok = true;
-    if (!a.emit) {
-      if (lane == 0) sub_base = atomicAdd(a.sub_top, (uint32_t)(count * (len1 + 1)));
-      sub_base = (uint32_t)uni((int)sub_base);
-      sub_ok = (uint64_t)sub_base + (uint64_t)count * (uint64_t)(len1 + 1) <= a.sub_capacity;
-      if (!sub_ok && lane == 0) atomicOr(&a.ctr->err_flags, kErrSubOverflow);
-    }
+    // Substitution records go straight to the pool: sorted task ti owns
+    // sub_stride slots (more than its diagonal steps) at ti * sub_stride.
+    const uint32_t ti = first + (uint32_t)g;
+    const size_t sub0 = (size_t)ti * a.sub_stride;
+    const bool sub_ok = a.emit || sub0 + a.sub_stride <= a.sub_capacity;
+    if (!a.emit && __builtin_amdgcn_ballot_w64(!sub_ok && real) && lane == 0)
+      atomicOr(&a.ctr->err_flags, kErrSubOverflow);
     BwConst B;
     B.rec = rec;
     B.gpos = g_raw * ncol;
@@ -614,7 +606,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(8, 8))) void
     B.swap = swap;
     B.real = real;
     B.sub_ok = sub_ok;
-    B.my_sub = sub_base + (uint32_t)g * (uint32_t)(len1 + 1);
+    B.my_sub = sub_ok ? sub0 : 0;
     B.s1b = s1b;
     B.s2b = s2b;
     B.ao = (a.emit && real) ? a.al_off[task.out_idx] : 0;
@@ -635,7 +627,6 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(8, 8))) void
     __syncthreads();
 
     // ---- outputs (column 0 lane of each pair) ----
-    const uint32_t ti = first + (uint32_t)g;
     if (a.emit) {
       if (real) {
         uint8_t* o0 = a.al0 + B.ao;
@@ -651,7 +642,6 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(8, 8))) void
         if (c == 0) a.al_len[task.out_idx] = bad ? 0u : (uint32_t)nt;
       }
     } else if (real && c == 0) {
-      a.sub_off[ti] = sub_ok ? B.my_sub : 0u;
       a.sub_cnt[ti] = sub_ok ? ((uint32_t)ns | (bad ? 0x80000000u : 0u)) : 0u;
     }
     __syncthreads();
@@ -676,8 +666,7 @@ hipError_t launch_nw_vec(const NwArgs& s, hipStream_t st) {
   d.emit = s.emit_alignment;
   d.sub_pool = s.sub_pool;
   d.sub_capacity = s.sub_capacity;
-  d.sub_top = s.sub_top;
-  d.sub_off = s.sub_off;
+  d.sub_stride = s.sub_stride;
   d.sub_cnt = s.sub_cnt;
   d.hist = s.hist;
   d.hist_stride = s.hist_stride;
@@ -686,6 +675,7 @@ hipError_t launch_nw_vec(const NwArgs& s, hipStream_t st) {
   d.al_off = s.al_off;
   d.al_len = s.al_len;
   if (s.grid == 0 || s.hist == nullptr || s.sched == nullptr) return hipErrorInvalidValue;
+  if (!s.emit_alignment && s.sub_stride == 0) return hipErrorInvalidValue;
   const size_t lds = s.lds_bytes;
   if (lds > 48 * 1024) {
     hipError_t e = hipFuncSetAttribute((const void*)nw_vec_kernel,

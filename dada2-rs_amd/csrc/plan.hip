@@ -26,19 +26,29 @@ hipError_t launch_zero(uint32_t* p, size_t n_words, hipStream_t s) {
 
 constexpr int kPlanThreads = 1024;
 
-// Exclusive scans of the pair counts and of the wavefront counts over bins.
+// Exclusive scans of the pair counts and of the wavefront counts over bins;
+// also the call's DP cell total (per-bin cells from the schedule) and the
+// screen's skip/shroud totals from its per-block partial counts (written
+// without atomics: device-scope atomics on one word serialise across the
+// chip).
 __global__ __launch_bounds__(kPlanThreads) void plan_kernel(BinSpace bs,
                                                             const uint32_t* bin_count,
                                                             uint32_t* bin_start,
                                                             uint32_t* wave_start,
-                                                            uint32_t* bin_fill, Counters* ctr) {
+                                                            uint32_t* bin_fill, Counters* ctr,
+                                                            const BinInfo* info,
+                                                            const uint32_t* partials,
+                                                            uint32_t n_partials) {
   __shared__ uint32_t s_pairs[kPlanThreads];
   __shared__ uint32_t s_waves[kPlanThreads];
+  __shared__ unsigned long long s_cells[kPlanThreads / 64];
+  __shared__ uint32_t s_skip[kPlanThreads / 64], s_shroud[kPlanThreads / 64];
   const int tid = threadIdx.x;
   const uint32_t nb = (uint32_t)bs.nbins;
   const uint32_t chunk = (nb + kPlanThreads - 1) / kPlanThreads;
   const uint32_t b0 = min(nb, tid * chunk), b1 = min(nb, b0 + chunk);
   uint32_t sp = 0, sw = 0, bad = 0;
+  unsigned long long cells = 0;
   for (uint32_t b = b0; b < b1; ++b) {
     uint32_t cnt = bin_count[b];
     int l1, l2;
@@ -48,6 +58,22 @@ __global__ __launch_bounds__(kPlanThreads) void plan_kernel(BinSpace bs,
     if (cnt && !G) bad = 1;
     sp += cnt;
     sw += G ? (cnt + G - 1) / G : 0;
+    if (cnt && info) cells += (unsigned long long)cnt * info[b].cells;
+  }
+  uint32_t skip = 0, shroud = 0;
+  for (uint32_t i = tid; i < n_partials; i += kPlanThreads) {
+    skip += partials[2 * i];
+    shroud += partials[2 * i + 1];
+  }
+  for (int off = 32; off > 0; off >>= 1) {
+    cells += __shfl_xor(cells, off);
+    skip += __shfl_xor(skip, off);
+    shroud += __shfl_xor(shroud, off);
+  }
+  if ((tid & 63) == 0) {
+    s_cells[tid >> 6] = cells;
+    s_skip[tid >> 6] = skip;
+    s_shroud[tid >> 6] = shroud;
   }
   s_pairs[tid] = sp;
   s_waves[tid] = sw;
@@ -78,14 +104,28 @@ __global__ __launch_bounds__(kPlanThreads) void plan_kernel(BinSpace bs,
     wave_start[nb] = s_waves[tid];
     ctr->total_waves = s_waves[tid];
     ctr->wave_counter = 0;
+    unsigned long long tc = 0;
+    uint32_t ts = 0, th = 0;
+    for (int w = 0; w < kPlanThreads / 64; ++w) {
+      tc += s_cells[w];
+      ts += s_skip[w];
+      th += s_shroud[w];
+    }
+    ctr->dp_cells = tc;
+    if (partials) {
+      ctr->n_skipped = ts;
+      ctr->n_shrouded = th;
+    }
   }
   if (bad) atomicOr(&ctr->err_flags, kErrGeometry);
 }
 
 hipError_t launch_plan(const BinSpace& bs, const uint32_t* bin_count, uint32_t* bin_start,
-                       uint32_t* wave_start, uint32_t* bin_fill, Counters* ctr, hipStream_t s) {
+                       uint32_t* wave_start, uint32_t* bin_fill, Counters* ctr,
+                       const BinInfo* info, const uint32_t* partials, uint32_t n_partials,
+                       hipStream_t s) {
   hipLaunchKernelGGL(plan_kernel, dim3(1), dim3(kPlanThreads), 0, s, bs, bin_count, bin_start,
-                     wave_start, bin_fill, ctr);
+                     wave_start, bin_fill, ctr, info, partials, n_partials);
   return hipGetLastError();
 }
 

@@ -38,6 +38,7 @@ struct ScreenDev {
   GaplessTask* gapless_tasks;
   DpTask* dp_unsorted;
   uint32_t* bin_count;
+  uint32_t* partials;
   Counters* ctr;
 };
 
@@ -207,16 +208,34 @@ __global__ __launch_bounds__(kScreenBlock) void screen_kernel(ScreenDev a) {
       atomicAdd(&a.bin_count[t.bin], 1u);
     }
   }
-  // wave-level counter updates
+  // block totals of skipped / shrouded pairs, summed by the plan kernel
+  // (no device-scope atomics on shared counter words)
+  __shared__ uint32_t s_cnt[2][kScreenBlock / 64];
   uint32_t s1 = n_skip, s2 = n_shroud;
   for (int off = 32; off > 0; off >>= 1) {
     s1 += __shfl_xor(s1, off);
     s2 += __shfl_xor(s2, off);
   }
   if (lane_id() == 0) {
-    if (s1) atomicAdd(&a.ctr->n_skipped, s1);
-    if (s2) atomicAdd(&a.ctr->n_shrouded, s2);
+    s_cnt[0][threadIdx.x >> 6] = s1;
+    s_cnt[1][threadIdx.x >> 6] = s2;
   }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint32_t t1 = 0, t2 = 0;
+    for (int w = 0; w < kScreenBlock / 64; ++w) {
+      t1 += s_cnt[0][w];
+      t2 += s_cnt[1][w];
+    }
+    const uint32_t blk = blockIdx.y * gridDim.x + blockIdx.x;
+    a.partials[2 * blk] = t1;
+    a.partials[2 * blk + 1] = t2;
+  }
+}
+
+uint32_t screen_blocks(uint32_t n_uniques, uint32_t n_centers) {
+  return ((n_uniques + kScreenBlock - 1) / kScreenBlock) *
+         ((n_centers + kScreenCenters - 1) / kScreenCenters);
 }
 
 hipError_t launch_screen(const ScreenArgs& s, hipStream_t st) {
@@ -237,6 +256,7 @@ hipError_t launch_screen(const ScreenArgs& s, hipStream_t st) {
   d.gapless_tasks = s.gapless_tasks;
   d.dp_unsorted = s.dp_unsorted;
   d.bin_count = s.bin_count;
+  d.partials = s.partials;
   d.ctr = s.ctr;
   dim3 grid((d.u.n + kScreenBlock - 1) / kScreenBlock,
             (s.n_centers + kScreenCenters - 1) / kScreenCenters);
