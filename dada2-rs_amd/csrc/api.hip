@@ -69,7 +69,8 @@ struct d2g_ctx {
   DevBuf<double> fin_lambda;  // caller's unique order (filled by fetch)
   DevBuf<uint32_t> fin_hamming;
   DevBuf<uint8_t> fin_skipped;
-  DevBuf<GaplessTask> gl;
+  DevBuf<GaplessTask> gl, gl_unsorted;
+  DevBuf<uint32_t> gl_off;
   DevBuf<DpTask> dp_unsorted, dp_sorted;
   DevBuf<uint32_t> bin_count, bin_start, wave_start, bin_fill;
   DevBuf<uint32_t> sub_pool, sub_cnt, screen_partials;
@@ -161,6 +162,8 @@ int d2g_ctx_destroy(d2g_ctx* c) {
   c->err.release();
   c->out_lambda.release();
   c->gl.release();
+  c->gl_unsorted.release();
+  c->gl_off.release();
   c->dp_unsorted.release();
   c->dp_sorted.release();
   c->ctr.release();
@@ -406,16 +409,19 @@ static int compare_device(d2g_ctx* c, const uint32_t* centers, uint32_t nc, cons
   HIP_TRY(c->out_lambda.ensure(npairs));
   HIP_TRY(c->out_hamming.ensure(npairs));
   HIP_TRY(c->out_skipped.ensure(npairs));
+  // per screen block: a region of kScreenRegion task slots in each list
+  const uint32_t n_part = screen_blocks(n, nc);
   HIP_TRY(c->gl.ensure(npairs));
-  HIP_TRY(c->dp_unsorted.ensure(npairs));
+  HIP_TRY(c->gl_unsorted.ensure((size_t)n_part * kScreenRegion));
+  HIP_TRY(c->gl_off.ensure(n_part));
+  HIP_TRY(c->dp_unsorted.ensure((size_t)n_part * kScreenRegion));
   HIP_TRY(c->dp_sorted.ensure(npairs));
   HIP_TRY(c->sub_cnt.ensure(npairs));
   HIP_TRY(c->bin_count.ensure(bs.nbins + 1));
   HIP_TRY(c->bin_start.ensure(bs.nbins + 1));
   HIP_TRY(c->wave_start.ensure(bs.nbins + 1));
   HIP_TRY(c->bin_fill.ensure(bs.nbins + 1));
-  const uint32_t n_part = screen_blocks(n, nc);
-  HIP_TRY(c->screen_partials.ensure((size_t)2 * n_part));
+  HIP_TRY(c->screen_partials.ensure((size_t)4 * n_part));
   const uint32_t sub_stride = (uint32_t)(bs.lmin + bs.range);  // longest length + 1
   if (c->sub_capacity < (size_t)1 << 20) c->sub_capacity = (size_t)1 << 20;
   HIP_TRY(c->sub_pool.ensure(c->sub_capacity));
@@ -451,7 +457,7 @@ static int compare_device(d2g_ctx* c, const uint32_t* centers, uint32_t nc, cons
     sa.out_lambda = c->out_lambda.p;
     sa.out_hamming = c->out_hamming.p;
     sa.out_skipped = c->out_skipped.p;
-    sa.gapless_tasks = c->gl.p;
+    sa.gapless_tasks = c->gl_unsorted.p;
     sa.dp_unsorted = c->dp_unsorted.p;
     sa.bin_count = c->bin_count.p;
     sa.partials = c->screen_partials.p;
@@ -459,9 +465,9 @@ static int compare_device(d2g_ctx* c, const uint32_t* centers, uint32_t nc, cons
     HIP_TRY(launch_screen(sa, s));
     if (c->timing) HIP_TRY(hipEventRecord(c->ev[1], s));
     HIP_TRY(launch_plan(bs, c->bin_count.p, c->bin_start.p, c->wave_start.p, c->bin_fill.p, c->ctr.p,
-                        c->sched_info.p, c->screen_partials.p, n_part, s));
-    HIP_TRY(launch_scatter(c->dp_unsorted.p, c->dp_sorted.p, c->bin_start.p, c->bin_fill.p, c->ctr.p,
-                           (uint32_t)npairs, s));
+                        c->sched_info.p, c->screen_partials.p, n_part, c->gl_off.p, s));
+    HIP_TRY(launch_scatter(c->dp_unsorted.p, c->dp_sorted.p, c->bin_start.p, c->bin_fill.p,
+                           c->screen_partials.p, n_part, 0, c->gl_unsorted.p, c->gl.p, c->gl_off.p, s));
     if (c->timing) HIP_TRY(hipEventRecord(c->ev[2], s));
     NwArgs na;
     std::memset(&na, 0, sizeof(na));
@@ -625,7 +631,8 @@ int d2g_align_vectorized(d2g_ctx* c, uint32_t n, const uint8_t* s1, const uint64
   }
   hipStream_t s = c->stream;
   HIP_TRY(c->a_pool.ensure(n1 + n2 + 1));
-  HIP_TRY(c->dp_unsorted.ensure(n));
+  const uint32_t n_regions = (n + kScreenRegion - 1) / kScreenRegion;
+  HIP_TRY(c->dp_unsorted.ensure((size_t)n_regions * kScreenRegion));
   HIP_TRY(c->dp_sorted.ensure(n));
   HIP_TRY(c->bin_count.ensure(bs.nbins + 1));
   HIP_TRY(c->bin_start.ensure(bs.nbins + 1));
@@ -647,8 +654,9 @@ int d2g_align_vectorized(d2g_ctx* c, uint32_t n, const uint8_t* s1, const uint64
   HIP_TRY(hipStreamSynchronize(s));
   HIP_TRY_INT(prepare_schedule(c, bs, sc->end_gap_p, sc->gap_p));
   HIP_TRY(launch_plan(bs, c->bin_count.p, c->bin_start.p, c->wave_start.p, c->bin_fill.p, c->ctr.p,
-                      c->sched_info.p, nullptr, 0, s));
-  HIP_TRY(launch_scatter(c->dp_unsorted.p, c->dp_sorted.p, c->bin_start.p, c->bin_fill.p, c->ctr.p, n, s));
+                      c->sched_info.p, nullptr, 0, nullptr, s));
+  HIP_TRY(launch_scatter(c->dp_unsorted.p, c->dp_sorted.p, c->bin_start.p, c->bin_fill.p, nullptr,
+                         n_regions, n, nullptr, nullptr, nullptr, s));
   NwArgs na;
   std::memset(&na, 0, sizeof(na));
   na.bins = bs;

@@ -33,6 +33,10 @@ struct UniqueDev {
 hipError_t launch_prep(const UniqueDev& u, hipStream_t s);
 
 // ---- screen.hip --------------------------------------------------------
+// Task slots per screen workgroup (256 uniques x 16 centers): each block
+// appends its gapless and DP tasks to its own region of that many slots.
+constexpr uint32_t kScreenRegion = 4096;
+
 struct ScreenArgs {
   const UniqueDev* u;  // host copy of the pointers
   const uint32_t* centers;  // sorted indices, n_centers
@@ -47,10 +51,10 @@ struct ScreenArgs {
   double* out_lambda;
   uint32_t* out_hamming;
   uint8_t* out_skipped;
-  GaplessTask* gapless_tasks;
-  DpTask* dp_unsorted;
+  GaplessTask* gapless_tasks;  // per block region (unsorted)
+  DpTask* dp_unsorted;         // per block region
   uint32_t* bin_count;
-  uint32_t* partials;  // per block: skipped, shrouded (screen_blocks entries)
+  uint32_t* partials;  // per block: skipped, shrouded, gapless, DP (screen_blocks entries)
   Counters* ctr;
 };
 hipError_t launch_screen(const ScreenArgs& a, hipStream_t s);
@@ -58,15 +62,20 @@ uint32_t screen_blocks(uint32_t n_uniques, uint32_t n_centers);
 
 // ---- plan.hip ----------------------------------------------------------
 hipError_t launch_zero(uint32_t* p, size_t n_words, hipStream_t s);
-// Also totals the DP cells (info: per-bin cells, nullable) and the screen's
-// per-block skip/shroud partials (pairs of words, nullable).
+// Also totals the DP cells (info: per-bin cells, nullable) and, for a
+// compare, the screen's per-block counts (partials: 4 words per block,
+// nullable), writing each block's gapless offset to gl_off.
 hipError_t launch_plan(const BinSpace& bs, const uint32_t* bin_count, uint32_t* bin_start,
                        uint32_t* wave_start, uint32_t* bin_fill, Counters* ctr,
                        const BinInfo* info, const uint32_t* partials, uint32_t n_partials,
-                       hipStream_t s);
+                       uint32_t* gl_off, hipStream_t s);
+// Bins the DP tasks of n_regions regions of kScreenRegion slots (counts from
+// partials, or `count` tasks filling the regions in order when partials is
+// null) and compacts the gapless tasks (compare only).
 hipError_t launch_scatter(const DpTask* unsorted, DpTask* sorted, const uint32_t* bin_start,
-                          uint32_t* bin_fill, const Counters* ctr, uint32_t capacity,
-                          hipStream_t s);
+                          uint32_t* bin_fill, const uint32_t* partials, uint32_t n_regions,
+                          uint32_t count, const GaplessTask* gl_unsorted, GaplessTask* gl,
+                          const uint32_t* gl_off, hipStream_t s);
 hipError_t launch_unpermute(uint32_t nc, uint32_t n, const uint32_t* orig, const double* src_lam,
                             const uint32_t* src_ham, const uint8_t* src_skp, double* dst_lam,
                             uint32_t* dst_ham, uint8_t* dst_skp, hipStream_t s);

@@ -26,11 +26,30 @@ hipError_t launch_zero(uint32_t* p, size_t n_words, hipStream_t s) {
 
 constexpr int kPlanThreads = 1024;
 
+// Single-workgroup exclusive scan of v over the 1024 threads (returns the
+// exclusive prefix; *total gets the sum).  Every thread must call it.
+__device__ uint32_t block_scan(uint32_t v, uint32_t* sh, uint32_t* total) {
+  const int tid = threadIdx.x;
+  sh[tid] = v;
+  __syncthreads();
+  for (int off = 1; off < kPlanThreads; off <<= 1) {
+    const uint32_t x = tid >= off ? sh[tid - off] : 0;
+    __syncthreads();
+    sh[tid] += x;
+    __syncthreads();
+  }
+  const uint32_t incl = sh[tid];
+  *total = sh[kPlanThreads - 1];
+  __syncthreads();
+  return incl - v;
+}
+
 // Exclusive scans of the pair counts and of the wavefront counts over bins;
-// also the call's DP cell total (per-bin cells from the schedule) and the
-// screen's skip/shroud totals from its per-block partial counts (written
-// without atomics: device-scope atomics on one word serialise across the
-// chip).
+// the call's DP cell total (per-bin cells from the schedule); and, for a
+// compare, the screen's per-block counts (skipped, shrouded, gapless, DP):
+// totals, and each block's offset into the compacted gapless list.  Totals
+// are written here rather than by atomics in the screen: device-scope
+// atomics on one word serialise across the chip.
 __global__ __launch_bounds__(kPlanThreads) void plan_kernel(BinSpace bs,
                                                             const uint32_t* bin_count,
                                                             uint32_t* bin_start,
@@ -38,11 +57,10 @@ __global__ __launch_bounds__(kPlanThreads) void plan_kernel(BinSpace bs,
                                                             uint32_t* bin_fill, Counters* ctr,
                                                             const BinInfo* info,
                                                             const uint32_t* partials,
-                                                            uint32_t n_partials) {
-  __shared__ uint32_t s_pairs[kPlanThreads];
-  __shared__ uint32_t s_waves[kPlanThreads];
+                                                            uint32_t n_partials,
+                                                            uint32_t* gl_off) {
+  __shared__ uint32_t sh[kPlanThreads];
   __shared__ unsigned long long s_cells[kPlanThreads / 64];
-  __shared__ uint32_t s_skip[kPlanThreads / 64], s_shroud[kPlanThreads / 64];
   const int tid = threadIdx.x;
   const uint32_t nb = (uint32_t)bs.nbins;
   const uint32_t chunk = (nb + kPlanThreads - 1) / kPlanThreads;
@@ -60,33 +78,11 @@ __global__ __launch_bounds__(kPlanThreads) void plan_kernel(BinSpace bs,
     sw += G ? (cnt + G - 1) / G : 0;
     if (cnt && info) cells += (unsigned long long)cnt * info[b].cells;
   }
-  uint32_t skip = 0, shroud = 0;
-  for (uint32_t i = tid; i < n_partials; i += kPlanThreads) {
-    skip += partials[2 * i];
-    shroud += partials[2 * i + 1];
-  }
-  for (int off = 32; off > 0; off >>= 1) {
-    cells += __shfl_xor(cells, off);
-    skip += __shfl_xor(skip, off);
-    shroud += __shfl_xor(shroud, off);
-  }
-  if ((tid & 63) == 0) {
-    s_cells[tid >> 6] = cells;
-    s_skip[tid >> 6] = skip;
-    s_shroud[tid >> 6] = shroud;
-  }
-  s_pairs[tid] = sp;
-  s_waves[tid] = sw;
-  __syncthreads();
-  for (int off = 1; off < kPlanThreads; off <<= 1) {
-    uint32_t vp = tid >= off ? s_pairs[tid - off] : 0;
-    uint32_t vw = tid >= off ? s_waves[tid - off] : 0;
-    __syncthreads();
-    s_pairs[tid] += vp;
-    s_waves[tid] += vw;
-    __syncthreads();
-  }
-  uint32_t rp = s_pairs[tid] - sp, rw = s_waves[tid] - sw;  // exclusive prefix
+  for (int off = 32; off > 0; off >>= 1) cells += __shfl_xor(cells, off);
+  if ((tid & 63) == 0) s_cells[tid >> 6] = cells;
+  uint32_t tot_pairs, tot_waves;
+  uint32_t rp = block_scan(sp, sh, &tot_pairs);
+  uint32_t rw = block_scan(sw, sh, &tot_waves);
   for (uint32_t b = b0; b < b1; ++b) {
     uint32_t cnt = bin_count[b];
     int l1, l2;
@@ -99,22 +95,40 @@ __global__ __launch_bounds__(kPlanThreads) void plan_kernel(BinSpace bs,
     rp += cnt;
     rw += G ? (cnt + G - 1) / G : 0;
   }
+  // screen blocks: skip / shroud totals, gapless offsets
+  uint32_t skip = 0, shroud = 0, ngl = 0;
+  const uint32_t pchunk = (n_partials + kPlanThreads - 1) / kPlanThreads;
+  const uint32_t p0 = min(n_partials, tid * pchunk), p1 = min(n_partials, p0 + pchunk);
+  if (partials) {
+    for (uint32_t i = p0; i < p1; ++i) {
+      skip += partials[4 * i];
+      shroud += partials[4 * i + 1];
+      ngl += partials[4 * i + 2];
+    }
+  }
+  uint32_t tot_skip, tot_shroud, tot_gl;
+  (void)block_scan(skip, sh, &tot_skip);
+  (void)block_scan(shroud, sh, &tot_shroud);
+  uint32_t go = block_scan(ngl, sh, &tot_gl);
+  if (partials) {
+    for (uint32_t i = p0; i < p1; ++i) {
+      gl_off[i] = go;
+      go += partials[4 * i + 2];
+    }
+  }
   if (tid == kPlanThreads - 1) {
-    bin_start[nb] = s_pairs[tid];
-    wave_start[nb] = s_waves[tid];
-    ctr->total_waves = s_waves[tid];
+    bin_start[nb] = tot_pairs;
+    wave_start[nb] = tot_waves;
+    ctr->total_waves = tot_waves;
     ctr->wave_counter = 0;
     unsigned long long tc = 0;
-    uint32_t ts = 0, th = 0;
-    for (int w = 0; w < kPlanThreads / 64; ++w) {
-      tc += s_cells[w];
-      ts += s_skip[w];
-      th += s_shroud[w];
-    }
+    for (int w = 0; w < kPlanThreads / 64; ++w) tc += s_cells[w];
     ctr->dp_cells = tc;
     if (partials) {
-      ctr->n_skipped = ts;
-      ctr->n_shrouded = th;
+      ctr->n_skipped = tot_skip;
+      ctr->n_shrouded = tot_shroud;
+      ctr->n_gapless = tot_gl;
+      ctr->n_dp = tot_pairs;
     }
   }
   if (bad) atomicOr(&ctr->err_flags, kErrGeometry);
@@ -123,30 +137,45 @@ __global__ __launch_bounds__(kPlanThreads) void plan_kernel(BinSpace bs,
 hipError_t launch_plan(const BinSpace& bs, const uint32_t* bin_count, uint32_t* bin_start,
                        uint32_t* wave_start, uint32_t* bin_fill, Counters* ctr,
                        const BinInfo* info, const uint32_t* partials, uint32_t n_partials,
-                       hipStream_t s) {
+                       uint32_t* gl_off, hipStream_t s) {
   hipLaunchKernelGGL(plan_kernel, dim3(1), dim3(kPlanThreads), 0, s, bs, bin_count, bin_start,
-                     wave_start, bin_fill, ctr, info, partials, n_partials);
+                     wave_start, bin_fill, ctr, info, partials, n_partials, gl_off);
   return hipGetLastError();
 }
 
-__global__ void scatter_kernel(const DpTask* unsorted, DpTask* sorted, const uint32_t* bin_start,
-                               uint32_t* bin_fill, const Counters* ctr, uint32_t capacity) {
-  const uint32_t n = min(ctr->n_dp, capacity);
-  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-    DpTask t = unsorted[i];
-    uint32_t pos = bin_start[t.bin] + atomicAdd(&bin_fill[t.bin], 1u);
+// One workgroup per task region: DP tasks go to their bin (bin_start plus a
+// per-bin fill counter); gapless tasks are compacted at gl_off[region].
+// Region r holds partials[4r+3] DP tasks and partials[4r+2] gapless tasks;
+// without partials, `count` DP tasks fill the regions in order.
+__global__ __launch_bounds__(256) void scatter_kernel(const DpTask* unsorted, DpTask* sorted,
+                                                      const uint32_t* bin_start, uint32_t* bin_fill,
+                                                      const uint32_t* partials, uint32_t count,
+                                                      const GaplessTask* gl_unsorted,
+                                                      GaplessTask* gl, const uint32_t* gl_off) {
+  const uint32_t r = blockIdx.x;
+  const uint32_t ndp =
+      partials ? partials[4 * r + 3] : min(kScreenRegion, count - r * kScreenRegion);
+  const DpTask* src = unsorted + (size_t)r * kScreenRegion;
+  for (uint32_t i = threadIdx.x; i < ndp; i += blockDim.x) {
+    const DpTask t = src[i];
+    const uint32_t pos = bin_start[t.bin] + atomicAdd(&bin_fill[t.bin], 1u);
     sorted[pos] = t;
+  }
+  if (partials) {
+    const uint32_t ngl = partials[4 * r + 2];
+    const GaplessTask* gsrc = gl_unsorted + (size_t)r * kScreenRegion;
+    GaplessTask* gdst = gl + gl_off[r];
+    for (uint32_t i = threadIdx.x; i < ngl; i += blockDim.x) gdst[i] = gsrc[i];
   }
 }
 
 hipError_t launch_scatter(const DpTask* unsorted, DpTask* sorted, const uint32_t* bin_start,
-                          uint32_t* bin_fill, const Counters* ctr, uint32_t capacity,
-                          hipStream_t s) {
-  uint32_t blocks = (capacity + 255) / 256;
-  if (blocks > 2048) blocks = 2048;
-  if (blocks == 0) blocks = 1;
-  hipLaunchKernelGGL(scatter_kernel, dim3(blocks), dim3(256), 0, s, unsorted, sorted, bin_start,
-                     bin_fill, ctr, capacity);
+                          uint32_t* bin_fill, const uint32_t* partials, uint32_t n_regions,
+                          uint32_t count, const GaplessTask* gl_unsorted, GaplessTask* gl,
+                          const uint32_t* gl_off, hipStream_t s) {
+  if (n_regions == 0) return hipSuccess;
+  hipLaunchKernelGGL(scatter_kernel, dim3(n_regions), dim3(256), 0, s, unsorted, sorted,
+                     bin_start, bin_fill, partials, count, gl_unsorted, gl, gl_off);
   return hipGetLastError();
 }
 

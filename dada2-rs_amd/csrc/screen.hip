@@ -12,8 +12,10 @@
 // sum(min(a, b)) over the 4^k buckets is (sum a + sum b - sum |a - b|) / 2,
 // with sum |a - b| from v_sad_u8 (four buckets per instruction).
 //
-// Pairs that pass are appended (wave-aggregated atomics) to the gapless list
-// or to the DP list with their (len_center, len_query) bin.
+// Pairs that pass are appended to the block's own gapless or DP list (LDS
+// counters; each block owns kScreenRegion slots of each list) with their
+// (len_center, len_query) bin; per-block counts go to `partials` and the
+// scatter kernel compacts the lists.
 #include <float.h>
 
 #include "d2g_launch.h"
@@ -44,18 +46,18 @@ struct ScreenDev {
 
 __device__ __forceinline__ uint32_t lane_id() { return __lane_id(); }
 
-// All lanes of the wavefront must call this (it is a wave collective).
-__device__ __forceinline__ uint32_t wave_append(uint32_t* counter, bool pred, bool* ok) {
-  uint64_t mask = __builtin_amdgcn_ballot_w64(pred);
+// Appends to a workgroup-local list: one LDS atomic per wavefront, ranks
+// from the ballot.  All lanes of the wavefront must call this.
+__device__ __forceinline__ uint32_t wave_append(uint32_t* lds_counter, bool pred) {
+  const uint64_t mask = __builtin_amdgcn_ballot_w64(pred);
   uint32_t base = 0;
   if (mask) {
-    int leader = __builtin_ffsll((long long)mask) - 1;
-    uint32_t cnt = (uint32_t)__builtin_popcountll(mask);
-    if ((int)lane_id() == leader) base = atomicAdd(counter, cnt);
+    const int leader = __builtin_ffsll((long long)mask) - 1;
+    const uint32_t cnt = (uint32_t)__builtin_popcountll(mask);
+    if ((int)lane_id() == leader) base = atomicAdd(lds_counter, cnt);
     base = __shfl(base, leader);
   }
-  uint64_t below = mask & ((1ull << lane_id()) - 1ull);
-  *ok = pred;
+  const uint64_t below = mask & ((1ull << lane_id()) - 1ull);
   return base + (uint32_t)__builtin_popcountll(below);
 }
 
@@ -87,8 +89,20 @@ __device__ uint32_t u16_min_dot(const UniqueDev& u, uint32_t qa, uint32_t qb) {
   return dot;
 }
 
+static_assert(kScreenBlock * kScreenCenters == kScreenRegion, "one task slot per pair of a block");
+
 __global__ __launch_bounds__(kScreenBlock) void screen_kernel(ScreenDev a) {
   const UniqueDev& u = a.u;
+  // this block's task lists: kScreenRegion slots each, counted in LDS
+  __shared__ uint32_t s_ngl, s_ndp;
+  if (threadIdx.x == 0) {
+    s_ngl = 0;
+    s_ndp = 0;
+  }
+  __syncthreads();
+  const uint32_t blk = blockIdx.y * gridDim.x + blockIdx.x;
+  GaplessTask* gl_region = a.gapless_tasks + (size_t)blk * kScreenRegion;
+  DpTask* dp_region = a.dp_unsorted + (size_t)blk * kScreenRegion;
   const uint32_t q = blockIdx.x * kScreenBlock + threadIdx.x;
   const bool valid = q < u.n;
   const uint32_t qq = valid ? q : u.n - 1;
@@ -185,18 +199,19 @@ __global__ __launch_bounds__(kScreenBlock) void screen_kernel(ScreenDev a) {
     }
     n_skip += valid && skip;
     n_shroud += valid && shroud;
-    bool ok;
-    uint32_t gpos = wave_append(&a.ctr->n_gapless, valid && route_gapless, &ok);
-    if (ok) {
+    const bool to_gl = valid && route_gapless;
+    const uint32_t gpos = wave_append(&s_ngl, to_gl);
+    if (to_gl) {
       GaplessTask t;
       t.c_idx = ci;
       t.u_idx = qq;
       t.out_idx = out_idx;
       t._pad = 0;
-      a.gapless_tasks[gpos] = t;
+      gl_region[gpos] = t;
     }
-    uint32_t dpos = wave_append(&a.ctr->n_dp, valid && route_dp, &ok);
-    if (ok) {
+    const bool to_dp = valid && route_dp;
+    const uint32_t dpos = wave_append(&s_ndp, to_dp);
+    if (to_dp) {
       DpTask t;
       t.s1_off = u.off[ci];
       t.s2_off = u.off[qq];
@@ -204,7 +219,7 @@ __global__ __launch_bounds__(kScreenBlock) void screen_kernel(ScreenDev a) {
       t.bin = (clen - (uint32_t)a.bins.lmin) * (uint32_t)a.bins.range + (my_len - (uint32_t)a.bins.lmin);
       t.u_idx = qq;
       t.c_idx = ci;
-      a.dp_unsorted[dpos] = t;
+      dp_region[dpos] = t;
       atomicAdd(&a.bin_count[t.bin], 1u);
     }
   }
@@ -227,9 +242,10 @@ __global__ __launch_bounds__(kScreenBlock) void screen_kernel(ScreenDev a) {
       t1 += s_cnt[0][w];
       t2 += s_cnt[1][w];
     }
-    const uint32_t blk = blockIdx.y * gridDim.x + blockIdx.x;
-    a.partials[2 * blk] = t1;
-    a.partials[2 * blk + 1] = t2;
+    a.partials[4 * blk] = t1;
+    a.partials[4 * blk + 1] = t2;
+    a.partials[4 * blk + 2] = s_ngl;
+    a.partials[4 * blk + 3] = s_ndp;
   }
 }
 
