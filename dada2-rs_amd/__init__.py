@@ -102,6 +102,14 @@ def _lib():
                                                C.POINTER(VecScores), u8p, u8p, u64p, u32p]),
             "d2g_vectorized_cells": (C.c_uint64, [C.c_uint32, C.c_uint32, C.c_int32]),
             "d2g_vectorized_supported": (C.c_int, [C.c_uint32, C.c_uint32, C.c_int32]),
+            "d2g_align_endsfree": (C.c_int, [vp, C.c_uint32, u8p, u64p, u8p, u64p, C.c_int32,
+                                             C.c_int32, C.c_int32, C.c_int32, C.c_int32, u8p,
+                                             u8p, u64p, u32p]),
+            "d2g_band_supported": (C.c_int, [C.c_uint32, C.c_uint32, C.c_int32, C.c_int]),
+            "d2g_pairs_load": (C.c_int, [vp, C.c_uint32, u8p, u64p, u8p, u32p, u64p, u32p, u32p]),
+            "d2g_pairs_lambda": (C.c_int, [vp, f64p, C.c_uint32, C.c_int, C.c_double,
+                                           C.POINTER(PairStats)]),
+            "d2g_pairs_fetch": (C.c_int, [vp, C.POINTER(C.c_float), f64p, f64p, u8p]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -116,7 +124,9 @@ def exported_symbols() -> list[str]:
     return ["d2g_ctx_create", "d2g_ctx_destroy", "d2g_ctx_set_timing", "d2g_ctx_stream",
             "d2g_device_info", "d2g_status_string", "d2g_load_uniques", "d2g_compare_center",
             "d2g_compare_batch", "d2g_compare_batch_device", "d2g_fetch_results",
-            "d2g_align_vectorized", "d2g_vectorized_cells", "d2g_vectorized_supported"]
+            "d2g_align_vectorized", "d2g_vectorized_cells", "d2g_vectorized_supported",
+            "d2g_align_endsfree", "d2g_band_supported", "d2g_pairs_load", "d2g_pairs_lambda",
+            "d2g_pairs_fetch"]
 
 
 def _p(a: np.ndarray, t):
@@ -136,6 +146,10 @@ def vectorized_supported(l1: int, l2: int, band: int) -> bool:
     return bool(_lib().d2g_vectorized_supported(l1, l2, band))
 
 
+def band_supported(l1: int, l2: int, band: int, homo: bool = False) -> bool:
+    return bool(_lib().d2g_band_supported(l1, l2, band, int(homo)))
+
+
 def _concat(seqs) -> tuple[np.ndarray, np.ndarray]:
     arrs = [np.frombuffer(bytes(s), np.uint8) if isinstance(s, (bytes, bytearray))
             else np.asarray(s, np.uint8) for s in seqs]
@@ -143,6 +157,14 @@ def _concat(seqs) -> tuple[np.ndarray, np.ndarray]:
     off[1:] = np.cumsum([len(a) for a in arrs])
     cat = np.concatenate(arrs) if arrs else np.zeros(0, np.uint8)
     return np.ascontiguousarray(cat, np.uint8), off
+
+
+class PairStats(C.Structure):
+    _fields_ = [("pairs", C.c_uint64), ("recomputed", C.c_uint64), ("ms_loglam", C.c_float),
+                ("ms_pval", C.c_float), ("ms_total", C.c_float), ("_pad", C.c_float)]
+
+    def as_dict(self) -> dict:
+        return {name: getattr(self, name) for name, _ in self._fields_ if name != "_pad"}
 
 
 @dataclass
@@ -261,14 +283,49 @@ class Context:
                "d2g_compare_center")
         return CompareResult(lam, ham, skp, st.as_dict())
 
-    def align_vectorized(self, s1s, s2s, match=5, mismatch=-4, gap_p=-8, end_gap_p=0,
-                         band=16) -> list[tuple[bytes, bytes]]:
-        """align_vectorized_with_buf (nwalign.rs:761) for each (s1, s2) pair."""
+    def pairs_load(self, seqs, quals, subs, reads, center_reads):
+        """Resident aligned pairs: query i with its substitution list subs[i]
+        (records (query position << 16 | center base), ascending)."""
+        cat, off = _concat(seqs)
+        q = None
+        if quals is not None:
+            q, _ = _concat(quals)
+        sc = np.ascontiguousarray(np.concatenate([np.asarray(x, np.uint32) for x in subs])
+                                  if len(subs) else np.zeros(0, np.uint32), np.uint32)
+        so = np.zeros(len(subs) + 1, np.uint64)
+        so[1:] = np.cumsum([len(x) for x in subs])
+        if sc.size == 0:
+            sc = np.zeros(1, np.uint32)
+        r = np.ascontiguousarray(reads, np.uint32)
+        cr = np.ascontiguousarray(center_reads, np.uint32)
+        _check(_lib().d2g_pairs_load(self.h, len(seqs), _p(cat, u8p), _p(off, u64p),
+                                     _p(q, u8p) if q is not None else None, _p(sc, u32p),
+                                     _p(so, u64p), _p(r, u32p), _p(cr, u32p)), "d2g_pairs_load")
+        self.n_pairs = len(seqs)
+
+    def pairs_lambda(self, err, use_quals=True, omega_a=1e-40, fetch=True):
+        """FP32 Kahan log-lambda + FP64 p-values (FP64 lambda recompute within
+        10x of omega_a).  Returns (loglam32, lambda, pval, recomputed, stats)."""
+        e = np.ascontiguousarray(err, np.float64)
+        st = PairStats()
+        _check(_lib().d2g_pairs_lambda(self.h, _p(e, f64p), e.shape[1], int(use_quals),
+                                       float(omega_a), C.byref(st)), "d2g_pairs_lambda")
+        if not fetch:
+            return st.as_dict()
+        n = self.n_pairs
+        ll = np.zeros(n, np.float32)
+        lam = np.zeros(n, np.float64)
+        pv = np.zeros(n, np.float64)
+        fl = np.zeros(n, np.uint8)
+        _check(_lib().d2g_pairs_fetch(self.h, ll.ctypes.data_as(C.POINTER(C.c_float)),
+                                      _p(lam, f64p), _p(pv, f64p), _p(fl, u8p)), "d2g_pairs_fetch")
+        return ll, lam, pv, fl, st.as_dict()
+
+    @staticmethod
+    def _pair_buffers(s1s, s2s):
         if len(s1s) != len(s2s):
             raise ValueError("s1s and s2s differ in length")
         n = len(s1s)
-        if n == 0:
-            return []
         c1, o1 = _concat(s1s)
         c2, o2 = _concat(s2s)
         l1 = np.diff(o1)
@@ -276,16 +333,41 @@ class Context:
         aoff = np.zeros(n + 1, np.uint64)
         aoff[1:] = np.cumsum(l1 + l2)
         total = int(aoff[-1]) + 1
-        a0 = np.zeros(total, np.uint8)
-        a1 = np.zeros(total, np.uint8)
-        alen = np.zeros(n, np.uint32)
-        sc = VecScores(match, mismatch, gap_p, end_gap_p, band)
-        _check(_lib().d2g_align_vectorized(self.h, n, _p(c1, u8p), _p(o1, u64p), _p(c2, u8p),
-                                           _p(o2, u64p), C.byref(sc), _p(a0, u8p), _p(a1, u8p),
-                                           _p(aoff, u64p), _p(alen, u32p)),
-               "d2g_align_vectorized")
+        return (n, c1, o1, c2, o2, aoff, np.zeros(total, np.uint8), np.zeros(total, np.uint8),
+                np.zeros(n, np.uint32))
+
+    @staticmethod
+    def _pairs_out(n, aoff, a0, a1, alen):
         out = []
         for i in range(n):
             s, m = int(aoff[i]), int(alen[i])
             out.append((a0[s:s + m].tobytes(), a1[s:s + m].tobytes()))
         return out
+
+    def align_vectorized(self, s1s, s2s, match=5, mismatch=-4, gap_p=-8, end_gap_p=0,
+                         band=16) -> list[tuple[bytes, bytes]]:
+        """align_vectorized_with_buf (nwalign.rs:761) for each (s1, s2) pair."""
+        n, c1, o1, c2, o2, aoff, a0, a1, alen = self._pair_buffers(s1s, s2s)
+        if n == 0:
+            return []
+        sc = VecScores(match, mismatch, gap_p, end_gap_p, band)
+        _check(_lib().d2g_align_vectorized(self.h, n, _p(c1, u8p), _p(o1, u64p), _p(c2, u8p),
+                                           _p(o2, u64p), C.byref(sc), _p(a0, u8p), _p(a1, u8p),
+                                           _p(aoff, u64p), _p(alen, u32p)),
+               "d2g_align_vectorized")
+        return self._pairs_out(n, aoff, a0, a1, alen)
+
+    def align_endsfree(self, s1s, s2s, match=5, mismatch=-4, gap_p=-8, homo_gap_p=None,
+                       band=16) -> list[tuple[bytes, bytes]]:
+        """align_endsfree_with_buf (nwalign.rs:336), or with homo_gap_p the
+        homopolymer variant align_endsfree_homo_with_buf (nwalign.rs:435),
+        for each (s1, s2) pair."""
+        n, c1, o1, c2, o2, aoff, a0, a1, alen = self._pair_buffers(s1s, s2s)
+        if n == 0:
+            return []
+        hg = gap_p if homo_gap_p is None else homo_gap_p
+        _check(_lib().d2g_align_endsfree(self.h, n, _p(c1, u8p), _p(o1, u64p), _p(c2, u8p),
+                                         _p(o2, u64p), match, mismatch, gap_p, hg, band,
+                                         _p(a0, u8p), _p(a1, u8p), _p(aoff, u64p), _p(alen, u32p)),
+               "d2g_align_endsfree")
+        return self._pairs_out(n, aoff, a0, a1, alen)

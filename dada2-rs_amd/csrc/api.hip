@@ -78,7 +78,7 @@ struct d2g_ctx {
   size_t sub_capacity = 0;
   uint32_t last_nc = 0;
   // cached per bin space: LDS sizing and the row schedule of the DP kernel
-  BinSpace lds_key = {-1, -1, -1, -1};
+  BinSpace lds_key = {-1, -1, -1, -1, -1, -1};
   int sched_gap = 0, sched_end_gap = 0;
   size_t lds_bytes = 0;
   int sched_rows = 0;
@@ -88,6 +88,14 @@ struct d2g_ctx {
   uint32_t nw_grid = 0;
   size_t hist_stride = 0;
   DevBuf<uint32_t> hist;
+  // resident aligned-pair set (mixed-precision lambda / p-values)
+  uint32_t p_n = 0, p_max_qual = 0;
+  bool p_has_qual = false;
+  DevBuf<uint8_t> p_seq, p_qual, p_flag;
+  DevBuf<uint64_t> p_off, p_suboff;
+  DevBuf<uint32_t> p_len, p_subs, p_reads, p_creads, p_nrec;
+  DevBuf<float> p_loglam;
+  DevBuf<double> p_lambda, p_pval, p_err;
   // align mode buffers
   DevBuf<uint8_t> a_pool, a_al0, a_al1;
   DevBuf<uint64_t> a_aloff;
@@ -169,6 +177,11 @@ int d2g_ctx_destroy(d2g_ctx* c) {
   c->ctr.release();
   c->sched.release();
   c->sched_info.release();
+  for (auto* b : {&c->p_seq, &c->p_qual, &c->p_flag}) b->release();
+  for (auto* b : {&c->p_off, &c->p_suboff}) b->release();
+  for (auto* b : {&c->p_len, &c->p_subs, &c->p_reads, &c->p_creads, &c->p_nrec}) b->release();
+  c->p_loglam.release();
+  for (auto* b : {&c->p_lambda, &c->p_pval, &c->p_err}) b->release();
   for (int i = 0; i < kEvents; ++i)
     if (c->ev[i]) (void)hipEventDestroy(c->ev[i]);
   if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -339,18 +352,24 @@ int d2g_load_uniques(d2g_ctx* c, uint32_t n, const uint8_t* seq_concat, const ui
     if (r_ != D2G_OK) return r_; \
   } while (0)
 
-// LDS size and row schedule of the vectorized DP for a bin space, cached
-// across calls with the same space and scores.
+// LDS size, per-bin info (cells; steady rows) and, for the vectorized
+// kernel, the row schedule of a bin space, cached across calls with the same
+// space and scores; also sizes the persistent grid and its pointer-history
+// scratch.
 static int prepare_schedule(d2g_ctx* c, const BinSpace& bs, int end_gap, int gap) {
   if (bs.lmin == c->lds_key.lmin && bs.range == c->lds_key.range && bs.band == c->lds_key.band &&
-      end_gap == c->sched_end_gap && gap == c->sched_gap && c->sched.p)
+      bs.kind == c->lds_key.kind && bs.homo == c->lds_key.homo && end_gap == c->sched_end_gap &&
+      gap == c->sched_gap && c->sched_info.p)
     return D2G_OK;
-  c->lds_bytes = nw_vec_lds_for(bs);
+  const bool vec = bs.kind == kDpVectorized;
+  c->lds_bytes = vec ? nw_vec_lds_for(bs) : nw_band_lds_for(bs);
   const int lmax = bs.lmin + bs.range - 1;
-  c->sched_rows = nw_sched_rows(lmax);
-  const size_t recs = (size_t)bs.nbins * (size_t)c->sched_rows;
-  if (c->sched.ensure(recs) != hipSuccess) return D2G_ERR_HIP;
   if (c->sched_info.ensure((size_t)bs.nbins) != hipSuccess) return D2G_ERR_HIP;
+  if (vec) {
+    c->sched_rows = nw_sched_rows(lmax);
+    const size_t recs = (size_t)bs.nbins * (size_t)c->sched_rows;
+    if (c->sched.ensure(recs) != hipSuccess) return D2G_ERR_HIP;
+  }
   // persistent DP workgroups: 8 one-wave groups per SIMD where LDS allows,
   // and at most 1 GiB of pointer-history scratch
   int per_cu = 32;
@@ -361,14 +380,31 @@ static int prepare_schedule(d2g_ctx* c, const BinSpace& bs, int end_gap, int gap
   while (c->nw_grid > (uint32_t)c->cu_count && (size_t)c->nw_grid * c->hist_stride * 4 > ((size_t)1 << 30))
     c->nw_grid /= 2;
   if (c->hist.ensure((size_t)c->nw_grid * c->hist_stride) != hipSuccess) return D2G_ERR_HIP;
-  if (launch_sched(bs, c->sched_rows, end_gap, gap, c->sched.p, c->sched_info.p, c->stream) !=
-      hipSuccess)
-    return D2G_ERR_HIP;
+  const hipError_t e = vec ? launch_sched(bs, c->sched_rows, end_gap, gap, c->sched.p, c->sched_info.p, c->stream)
+                           : launch_band_info(bs, c->sched_info.p, c->stream);
+  if (e != hipSuccess) return D2G_ERR_HIP;
   if (hipStreamSynchronize(c->stream) != hipSuccess) return D2G_ERR_HIP;
   c->lds_key = bs;
   c->sched_end_gap = end_gap;
   c->sched_gap = gap;
   return D2G_OK;
+}
+
+// The DP kernel of the compare for these parameters (raw_align_dp,
+// nwalign.rs:1335-1414): the vectorized aligner when vectorized, no
+// homopolymer penalty and both lengths <= 3500; the homopolymer aligner when
+// homo_gap_p differs from gap_p and is <= 0; the plain ends-free aligner
+// otherwise.  Returns -1 when one call would need both kernels (vectorized
+// parameters with some uniques above 3500 nt and some not).
+static int dp_kind(const d2g_align_params* p, uint32_t minlen, uint32_t maxlen, int* homo) {
+  const bool use_homo = p->homo_gap_p != p->gap_p && p->homo_gap_p <= 0;
+  *homo = use_homo ? 1 : 0;
+  if (p->vectorized && !use_homo) {
+    if (maxlen <= kVectorizedMaxLen) return kDpVectorized;
+    if (minlen > kVectorizedMaxLen) return kDpRowBand;
+    return -1;
+  }
+  return kDpRowBand;
 }
 
 static float ev_ms(d2g_ctx* c, int a, int b) {
@@ -387,10 +423,9 @@ static int compare_device(d2g_ctx* c, const uint32_t* centers, uint32_t nc, cons
   const uint32_t n = u.n;
   if ((uint32_t)p->kmer_size != u.k && p->use_kmers) return D2G_ERR_ARG;
   if (use_quals && c->has_qual && c->max_qual >= ncol) return D2G_ERR_ARG;  // pval.rs:289, err extension dada.rs:337-384
-  // dispatch (nwalign.rs:1335-1403): only the vectorized aligner is on the GPU here
-  const bool use_homo = p->homo_gap_p != p->gap_p && p->homo_gap_p <= 0;
-  const bool too_long = u.maxlen > kVectorizedMaxLen;
-  if (p->band != 0 && (use_homo || !p->vectorized || too_long)) return D2G_ERR_UNSUPPORTED;
+  int homo = 0;
+  const int kind = dp_kind(p, u.minlen, u.maxlen, &homo);
+  if (p->band != 0 && kind < 0) return D2G_ERR_UNSUPPORTED;
   for (uint32_t i = 0; i < nc; ++i)
     if (centers[i] >= n) return D2G_ERR_ARG;
   const size_t npairs = (size_t)nc * n;
@@ -399,8 +434,11 @@ static int compare_device(d2g_ctx* c, const uint32_t* centers, uint32_t nc, cons
   bs.lmin = (int)u.minlen;
   bs.range = (int)(u.maxlen - u.minlen + 1);
   bs.band = p->band;
+  bs.kind = kind < 0 ? kDpVectorized : kind;  // band 0: every pair is gapless
+  bs.homo = homo;
   if ((size_t)bs.range * bs.range > (1u << 22)) return D2G_ERR_UNSUPPORTED;
   bs.nbins = bs.range * bs.range;
+  const bool vec = bs.kind == kDpVectorized;
   HIP_TRY_INT(prepare_schedule(c, bs, 0, p->gap_p));
   std::vector<uint32_t> hc(nc);
   for (uint32_t i = 0; i < nc; ++i) hc[i] = c->h_inv[centers[i]];
@@ -476,9 +514,12 @@ static int compare_device(d2g_ctx* c, const uint32_t* centers, uint32_t nc, cons
     na.wave_start = c->wave_start.p;
     na.tasks = c->dp_sorted.p;
     na.pool = u.seq;
-    na.match = (int16_t)p->match_score;
-    na.mismatch = (int16_t)p->mismatch;
-    na.gap_p = (int16_t)p->gap_p;
+    // the vectorized aligner takes i16 scores (nwalign.rs:1388-1394), the
+    // scalar ones i32
+    na.match = vec ? (int)(int16_t)p->match_score : p->match_score;
+    na.mismatch = vec ? (int)(int16_t)p->mismatch : p->mismatch;
+    na.gap_p = vec ? (int)(int16_t)p->gap_p : p->gap_p;
+    na.homo_gap_p = p->homo_gap_p;
     na.end_gap_p = 0;  // raw_align_dp always aligns ends-free (nwalign.rs:1393)
     na.ctr = c->ctr.p;
     na.emit_alignment = 0;
@@ -493,7 +534,7 @@ static int compare_device(d2g_ctx* c, const uint32_t* centers, uint32_t nc, cons
     na.sched = c->sched.p;
     na.info = c->sched_info.p;
     na.sched_rows = c->sched_rows;
-    HIP_TRY(launch_nw_vec(na, s));
+    HIP_TRY(vec ? launch_nw_vec(na, s) : launch_nw_band(na, s));
     if (c->timing) HIP_TRY(hipEventRecord(c->ev[3], s));
     LambdaArgs la;
     la.u = &u;
@@ -585,10 +626,13 @@ int d2g_compare_center(d2g_ctx* c, uint32_t center, const double* err, uint32_t 
                            out_hamming, out_skipped, stats);
 }
 
-int d2g_align_vectorized(d2g_ctx* c, uint32_t n, const uint8_t* s1, const uint64_t* s1_off,
-                         const uint8_t* s2, const uint64_t* s2_off, const d2g_vec_scores* sc,
-                         uint8_t* al0, uint8_t* al1, const uint64_t* al_offsets, uint32_t* al_len) {
-  if (!c || !s1 || !s1_off || !s2 || !s2_off || !sc || !al0 || !al1 || !al_len) return D2G_ERR_ARG;
+// n independent pairs through one DP kernel with alignment strings out
+// (parity mode); shared by the align entry points.
+static int align_pairs(d2g_ctx* c, uint32_t n, const uint8_t* s1, const uint64_t* s1_off,
+                       const uint8_t* s2, const uint64_t* s2_off, int kind, int homo, int band,
+                       int match, int mismatch, int gap, int end_gap, int homo_gap, uint8_t* al0,
+                       uint8_t* al1, const uint64_t* al_offsets, uint32_t* al_len) {
+  if (!c || !s1 || !s1_off || !s2 || !s2_off || !al0 || !al1 || !al_len) return D2G_ERR_ARG;
   if (n == 0) return D2G_OK;
   (void)hipSetDevice(c->device);
   // pool: all s1 then all s2
@@ -598,7 +642,10 @@ int d2g_align_vectorized(d2g_ctx* c, uint32_t n, const uint8_t* s1, const uint64
     if (s1_off[i + 1] < s1_off[i] || s2_off[i + 1] < s2_off[i]) return D2G_ERR_ARG;
     l1[i] = (uint32_t)(s1_off[i + 1] - s1_off[i]);
     l2[i] = (uint32_t)(s2_off[i + 1] - s2_off[i]);
-    if (l1[i] > kVectorizedMaxLen || l2[i] > kVectorizedMaxLen) return D2G_ERR_UNSUPPORTED;
+    if (l1[i] == 0 || l2[i] == 0) return D2G_ERR_ARG;
+    if (kind == kDpVectorized && (l1[i] > kVectorizedMaxLen || l2[i] > kVectorizedMaxLen))
+      return D2G_ERR_UNSUPPORTED;
+    if (l1[i] > 65535 || l2[i] > 65535) return D2G_ERR_UNSUPPORTED;
     lmin = std::min({lmin, l1[i], l2[i]});
     lmax = std::max({lmax, l1[i], l2[i]});
   }
@@ -612,7 +659,9 @@ int d2g_align_vectorized(d2g_ctx* c, uint32_t n, const uint8_t* s1, const uint64
   BinSpace bs;
   bs.lmin = (int)lmin;
   bs.range = (int)(lmax - lmin + 1);
-  bs.band = sc->band;
+  bs.band = band;
+  bs.kind = kind;
+  bs.homo = homo;
   if ((size_t)bs.range * bs.range > (1u << 22)) return D2G_ERR_UNSUPPORTED;
   bs.nbins = bs.range * bs.range;
   std::vector<DpTask> tasks(n);
@@ -626,8 +675,7 @@ int d2g_align_vectorized(d2g_ctx* c, uint32_t n, const uint8_t* s1, const uint64
     t.u_idx = 0;
     t.c_idx = 0;
     counts[t.bin] += 1;
-    VecGeom g = vec_geom((int)l1[i], (int)l2[i], bs.band);
-    if (nw_groups(g) == 0) return D2G_ERR_UNSUPPORTED;
+    if (bin_groups(bs, (int)l1[i], (int)l2[i]) == 0) return D2G_ERR_UNSUPPORTED;
   }
   hipStream_t s = c->stream;
   HIP_TRY(c->a_pool.ensure(n1 + n2 + 1));
@@ -652,7 +700,7 @@ int d2g_align_vectorized(d2g_ctx* c, uint32_t n, const uint8_t* s1, const uint64
   h0.n_dp = n;
   HIP_TRY(hipMemcpyAsync(c->ctr.p, &h0, sizeof(Counters), hipMemcpyHostToDevice, s));
   HIP_TRY(hipStreamSynchronize(s));
-  HIP_TRY_INT(prepare_schedule(c, bs, sc->end_gap_p, sc->gap_p));
+  HIP_TRY_INT(prepare_schedule(c, bs, end_gap, gap));
   HIP_TRY(launch_plan(bs, c->bin_count.p, c->bin_start.p, c->wave_start.p, c->bin_fill.p, c->ctr.p,
                       c->sched_info.p, nullptr, 0, nullptr, s));
   HIP_TRY(launch_scatter(c->dp_unsorted.p, c->dp_sorted.p, c->bin_start.p, c->bin_fill.p, nullptr,
@@ -664,10 +712,11 @@ int d2g_align_vectorized(d2g_ctx* c, uint32_t n, const uint8_t* s1, const uint64
   na.wave_start = c->wave_start.p;
   na.tasks = c->dp_sorted.p;
   na.pool = c->a_pool.p;
-  na.match = sc->match_score;
-  na.mismatch = sc->mismatch;
-  na.gap_p = sc->gap_p;
-  na.end_gap_p = sc->end_gap_p;
+  na.match = match;
+  na.mismatch = mismatch;
+  na.gap_p = gap;
+  na.end_gap_p = end_gap;
+  na.homo_gap_p = homo_gap;
   na.ctr = c->ctr.p;
   na.emit_alignment = 1;
   na.al0 = c->a_al0.p;
@@ -681,7 +730,7 @@ int d2g_align_vectorized(d2g_ctx* c, uint32_t n, const uint8_t* s1, const uint64
   na.sched = c->sched.p;
   na.info = c->sched_info.p;
   na.sched_rows = c->sched_rows;
-  HIP_TRY(launch_nw_vec(na, s));
+  HIP_TRY(kind == kDpVectorized ? launch_nw_vec(na, s) : launch_nw_band(na, s));
   Counters h;
   HIP_TRY(hipMemcpyAsync(&h, c->ctr.p, sizeof(Counters), hipMemcpyDeviceToHost, s));
   HIP_TRY(hipMemcpyAsync(al_len, c->a_allen.p, n * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
@@ -690,6 +739,161 @@ int d2g_align_vectorized(d2g_ctx* c, uint32_t n, const uint8_t* s1, const uint64
   HIP_TRY(hipStreamSynchronize(s));
   if (h.err_flags & kErrTraceback) return D2G_ERR_TRACEBACK;
   if (h.err_flags & kErrGeometry) return D2G_ERR_UNSUPPORTED;
+  return D2G_OK;
+}
+
+int d2g_align_vectorized(d2g_ctx* c, uint32_t n, const uint8_t* s1, const uint64_t* s1_off,
+                         const uint8_t* s2, const uint64_t* s2_off, const d2g_vec_scores* sc,
+                         uint8_t* al0, uint8_t* al1, const uint64_t* al_offsets, uint32_t* al_len) {
+  if (!sc) return D2G_ERR_ARG;
+  return align_pairs(c, n, s1, s1_off, s2, s2_off, kDpVectorized, 0, sc->band, sc->match_score,
+                     sc->mismatch, sc->gap_p, sc->end_gap_p, sc->gap_p, al0, al1, al_offsets, al_len);
+}
+
+int d2g_align_endsfree(d2g_ctx* c, uint32_t n, const uint8_t* s1, const uint64_t* s1_off,
+                       const uint8_t* s2, const uint64_t* s2_off, int32_t match, int32_t mismatch,
+                       int32_t gap_p, int32_t homo_gap_p, int32_t band, uint8_t* al0, uint8_t* al1,
+                       const uint64_t* al_offsets, uint32_t* al_len) {
+  // with homo_gap_p == gap_p the homopolymer aligner equals the plain one
+  const int homo = homo_gap_p != gap_p ? 1 : 0;
+  return align_pairs(c, n, s1, s1_off, s2, s2_off, kDpRowBand, homo, band, match, mismatch, gap_p,
+                     0, homo_gap_p, al0, al1, al_offsets, al_len);
+}
+
+int d2g_band_supported(uint32_t len1, uint32_t len2, int32_t band, int homo) {
+  return band_groups(band_geom((int)len1, (int)len2, band), homo != 0) > 0 ? 1 : 0;
+}
+
+int d2g_pairs_load(d2g_ctx* c, uint32_t n, const uint8_t* seq_concat, const uint64_t* offsets,
+                   const uint8_t* qual_concat, const uint32_t* subs, const uint64_t* sub_offsets,
+                   const uint32_t* reads, const uint32_t* center_reads) {
+  if (!c || !seq_concat || !offsets || !subs || !sub_offsets || !reads || !center_reads)
+    return D2G_ERR_ARG;
+  (void)hipSetDevice(c->device);
+  c->p_n = 0;
+  if (n == 0) return D2G_OK;
+  // rows padded to 16 bytes (vector loads); validation as for load_uniques
+  std::vector<uint64_t> off(n + 1, 0);
+  std::vector<uint32_t> len(n);
+  uint32_t maxq = 0;
+  for (uint32_t i = 0; i < n; ++i) {
+    if (offsets[i + 1] < offsets[i] || sub_offsets[i + 1] < sub_offsets[i]) return D2G_ERR_ARG;
+    len[i] = (uint32_t)(offsets[i + 1] - offsets[i]);
+    if (len[i] == 0 || len[i] > 65535 || reads[i] == 0) return D2G_ERR_ARG;
+    off[i + 1] = off[i] + ((len[i] + 15) & ~15u);
+    for (uint64_t k = offsets[i]; k < offsets[i + 1]; ++k) {
+      const uint8_t b = seq_concat[k];
+      if (b < 1 || b > 4) return D2G_ERR_NONACGT;
+      if (qual_concat) maxq = std::max<uint32_t>(maxq, qual_concat[k]);
+    }
+    uint32_t prev = 0;
+    for (uint64_t k = sub_offsets[i]; k < sub_offsets[i + 1]; ++k) {
+      const uint32_t pos = subs[k] >> 16, nt0 = subs[k] & 0xFFFFu;
+      if (pos >= len[i] || nt0 < 1 || nt0 > 4) return D2G_ERR_ARG;
+      if (k > sub_offsets[i] && pos <= prev) return D2G_ERR_ARG;  // ascending
+      prev = pos;
+    }
+  }
+  std::vector<uint8_t> sq(off[n], 1), ql;
+  if (qual_concat) ql.assign(off[n], 0);
+  for (uint32_t i = 0; i < n; ++i) {
+    std::memcpy(sq.data() + off[i], seq_concat + offsets[i], len[i]);
+    if (qual_concat) std::memcpy(ql.data() + off[i], qual_concat + offsets[i], len[i]);
+  }
+  const uint64_t ns = sub_offsets[n] - sub_offsets[0];
+  std::vector<uint64_t> so(n + 1);
+  for (uint32_t i = 0; i <= n; ++i) so[i] = sub_offsets[i] - sub_offsets[0];
+  hipStream_t s = c->stream;
+  HIP_TRY(c->p_seq.ensure(off[n] + 16));
+  if (qual_concat) HIP_TRY(c->p_qual.ensure(off[n] + 16));
+  HIP_TRY(c->p_off.ensure(n + 1));
+  HIP_TRY(c->p_len.ensure(n));
+  HIP_TRY(c->p_subs.ensure(ns + 1));
+  HIP_TRY(c->p_suboff.ensure(n + 1));
+  HIP_TRY(c->p_reads.ensure(n));
+  HIP_TRY(c->p_creads.ensure(n));
+  HIP_TRY(c->p_loglam.ensure(n));
+  HIP_TRY(c->p_lambda.ensure(n));
+  HIP_TRY(c->p_pval.ensure(n));
+  HIP_TRY(c->p_flag.ensure(n));
+  HIP_TRY(c->p_nrec.ensure(1));
+  HIP_TRY(hipMemcpyAsync(c->p_seq.p, sq.data(), off[n], hipMemcpyHostToDevice, s));
+  if (qual_concat) HIP_TRY(hipMemcpyAsync(c->p_qual.p, ql.data(), off[n], hipMemcpyHostToDevice, s));
+  HIP_TRY(hipMemcpyAsync(c->p_off.p, off.data(), (n + 1) * 8, hipMemcpyHostToDevice, s));
+  HIP_TRY(hipMemcpyAsync(c->p_len.p, len.data(), n * 4, hipMemcpyHostToDevice, s));
+  if (ns) HIP_TRY(hipMemcpyAsync(c->p_subs.p, subs + sub_offsets[0], ns * 4, hipMemcpyHostToDevice, s));
+  HIP_TRY(hipMemcpyAsync(c->p_suboff.p, so.data(), (n + 1) * 8, hipMemcpyHostToDevice, s));
+  HIP_TRY(hipMemcpyAsync(c->p_reads.p, reads, n * 4, hipMemcpyHostToDevice, s));
+  HIP_TRY(hipMemcpyAsync(c->p_creads.p, center_reads, n * 4, hipMemcpyHostToDevice, s));
+  HIP_TRY(hipStreamSynchronize(s));
+  c->p_n = n;
+  c->p_has_qual = qual_concat != nullptr;
+  c->p_max_qual = maxq;
+  return D2G_OK;
+}
+
+int d2g_pairs_lambda(d2g_ctx* c, const double* err, uint32_t ncol, int use_quals, double omega_a,
+                     d2g_pair_stats* stats) {
+  if (!c || !err || ncol == 0) return D2G_ERR_ARG;
+  if (c->p_n == 0) return D2G_ERR_STATE;
+  if (use_quals && c->p_has_qual && c->p_max_qual >= ncol) return D2G_ERR_ARG;
+  (void)hipSetDevice(c->device);
+  hipStream_t s = c->stream;
+  HIP_TRY(c->p_err.ensure((size_t)16 * ncol));
+  HIP_TRY(hipMemcpyAsync(c->p_err.p, err, (size_t)16 * ncol * 8, hipMemcpyHostToDevice, s));
+  if (c->timing) HIP_TRY(hipEventRecord(c->ev[0], s));
+  HIP_TRY(launch_zero(c->p_nrec.p, 1, s));
+  PairArgs pa;
+  pa.n = c->p_n;
+  pa.seq = c->p_seq.p;
+  pa.qual = c->p_has_qual ? c->p_qual.p : nullptr;
+  pa.off = c->p_off.p;
+  pa.len = c->p_len.p;
+  pa.subs = c->p_subs.p;
+  pa.sub_off = c->p_suboff.p;
+  pa.reads = c->p_reads.p;
+  pa.center_reads = c->p_creads.p;
+  pa.err = c->p_err.p;
+  pa.ncol = ncol;
+  pa.use_quals = use_quals;
+  pa.omega = omega_a;
+  pa.out_loglam = c->p_loglam.p;
+  pa.out_lambda = c->p_lambda.p;
+  pa.out_p = c->p_pval.p;
+  pa.out_flag = c->p_flag.p;
+  pa.n_recomputed = c->p_nrec.p;
+  pa.max_blocks = (uint32_t)c->cu_count * 8;
+  pa.ev_mid = c->timing ? c->ev[1] : nullptr;
+  HIP_TRY(launch_pairs_lambda(pa, s));
+  if (c->timing) HIP_TRY(hipEventRecord(c->ev[2], s));
+  uint32_t nrec = 0;
+  HIP_TRY(hipMemcpyAsync(&nrec, c->p_nrec.p, 4, hipMemcpyDeviceToHost, s));
+  HIP_TRY(hipStreamSynchronize(s));
+  if (stats) {
+    std::memset(stats, 0, sizeof(*stats));
+    stats->pairs = c->p_n;
+    stats->recomputed = nrec;
+    if (c->timing) {
+      stats->ms_loglam = ev_ms(c, 0, 1);
+      stats->ms_pval = ev_ms(c, 1, 2);
+      stats->ms_total = ev_ms(c, 0, 2);
+    }
+  }
+  return D2G_OK;
+}
+
+int d2g_pairs_fetch(d2g_ctx* c, float* out_loglam32, double* out_lambda, double* out_pval,
+                    uint8_t* out_recomputed) {
+  if (!c) return D2G_ERR_ARG;
+  if (c->p_n == 0) return D2G_ERR_STATE;
+  (void)hipSetDevice(c->device);
+  hipStream_t s = c->stream;
+  const uint32_t n = c->p_n;
+  if (out_loglam32) HIP_TRY(hipMemcpyAsync(out_loglam32, c->p_loglam.p, n * 4, hipMemcpyDeviceToHost, s));
+  if (out_lambda) HIP_TRY(hipMemcpyAsync(out_lambda, c->p_lambda.p, n * 8, hipMemcpyDeviceToHost, s));
+  if (out_pval) HIP_TRY(hipMemcpyAsync(out_pval, c->p_pval.p, n * 8, hipMemcpyDeviceToHost, s));
+  if (out_recomputed) HIP_TRY(hipMemcpyAsync(out_recomputed, c->p_flag.p, n, hipMemcpyDeviceToHost, s));
+  HIP_TRY(hipStreamSynchronize(s));
   return D2G_OK;
 }
 

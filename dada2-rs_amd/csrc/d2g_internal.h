@@ -135,21 +135,32 @@ D2G_HD BandGeom band_geom(int l1, int l2, int band) {
   return g;
 }
 
-D2G_HD int band_pad(const BandGeom& g) { return ((g.lanes + 4) + 3) & ~3; }
-D2G_HD size_t band_group_bytes(const BandGeom& g) {
-  size_t seq = (size_t)g.len1 + g.len2 + 4 * band_pad(g);
-  seq = (seq + 15) & ~(size_t)15;
-  return seq + (size_t)(g.len1 + 1) * 4;  // substitution list
+// Per pair in LDS: both sequences, and with homopolymer gaps their run
+// masks (homopolymer_mask_into, nwalign.rs:211-227).
+D2G_HD size_t band_group_bytes(const BandGeom& g, bool homo) {
+  size_t b = (size_t)(g.len1 + g.len2) * (homo ? 2 : 1);
+  return (b + 15) & ~(size_t)15;
 }
-D2G_HD size_t band_wave_bytes(const BandGeom& g, int G) {
-  return (size_t)g.nrow * 16 + (size_t)G * band_group_bytes(g);
+D2G_HD size_t band_wave_bytes(const BandGeom& g, int G, bool homo) {
+  return (size_t)G * band_group_bytes(g, homo);
 }
-D2G_HD int band_groups(const BandGeom& g) {
+D2G_HD int band_groups(const BandGeom& g, bool homo) {
   int G = g.lanes <= kWave ? kWave / g.lanes : 0;
   if (G > kNwMaxGroups) G = kNwMaxGroups;
-  while (G > 1 && band_wave_bytes(g, G) > kNwLdsSoft) --G;
-  if (G >= 1 && band_wave_bytes(g, G) > kNwLdsHard) G = 0;
+  while (G > 1 && band_wave_bytes(g, G, homo) > kNwLdsSoft) --G;
+  if (G >= 1 && band_wave_bytes(g, G, homo) > kNwLdsHard) G = 0;
   return G;
+}
+// Cells the reference's row-band loop computes (sum over rows i = 1..len1
+// of the band width, nwalign.rs:381-384 / 481-484).
+D2G_HD uint32_t band_cells(const BandGeom& g, int band) {
+  uint32_t cells = 0;
+  for (int i = 1; i <= g.len1; ++i) {
+    const int l = band >= 0 ? imax(i - g.lband, 1) : 1;
+    const int r = band >= 0 ? imin(i + g.rband, g.len2) : g.len2;
+    if (r >= l) cells += (uint32_t)(r - l + 1);
+  }
+  return cells;
 }
 
 // Which DP kernel a bin space feeds.
@@ -177,16 +188,26 @@ struct GaplessTask {
 };
 
 // Binning of DP tasks: bin = (len1_in - lmin) * range + (len2_in - lmin).
+// kind: the DP kernel of every bin (kDpVectorized or kDpRowBand); homo: the
+// row-band kernel applies homopolymer gap penalties.
 struct BinSpace {
   int lmin;
   int range;
   int nbins;
   int band;
+  int kind;
+  int homo;
 };
 
 D2G_HD void bin_lengths(const BinSpace& bs, uint32_t bin, int* l1, int* l2) {
   *l1 = bs.lmin + (int)(bin / (uint32_t)bs.range);
   *l2 = bs.lmin + (int)(bin % (uint32_t)bs.range);
+}
+
+// Pairs per wavefront of a bin for its kernel (0 = unsupported geometry).
+D2G_HD int bin_groups(const BinSpace& bs, int l1, int l2) {
+  if (bs.kind == kDpRowBand) return band_groups(band_geom(l1, l2, bs.band), bs.homo != 0);
+  return nw_groups(vec_geom(l1, l2, bs.band));
 }
 
 // Device-side counters of one compare call.
@@ -197,7 +218,7 @@ struct Counters {
   uint32_t n_shrouded;
   uint32_t n_exact_fallback;  // kmer_dist8 overflow pairs (kmers.rs:259-266)
   uint32_t err_flags;
-  uint32_t wave_counter;      // persistent DP kernel work counter
+  uint32_t wave_counter;      // unused (kept for layout)
   uint32_t total_waves;
   unsigned long long dp_cells;
 };

@@ -90,7 +90,8 @@ struct NwArgs {
   const RowRec* sched;         // per-bin row schedule (launch_sched)
   const BinInfo* info;         // per-bin steady range and cell count (launch_sched)
   int sched_rows;              // records per bin
-  int16_t match, mismatch, gap_p, end_gap_p;
+  int match, mismatch, gap_p, end_gap_p;
+  int homo_gap_p;              // row-band kernel with bins.homo
   Counters* ctr;
   int emit_alignment;
   // compare mode: per sorted task ti, substitution list (query_pos << 16 |
@@ -116,6 +117,14 @@ hipError_t launch_sched(const BinSpace& bs, int rows, int end_gap, int gap, RowR
                         BinInfo* info, hipStream_t s);
 size_t nw_vec_lds_for(const BinSpace& bs);
 
+// ---- nw_band.hip -------------------------------------------------------
+// The scalar ends-free aligners (plain / homopolymer by bins.homo) over the
+// same binned tasks and arguments as launch_nw_vec (sched unused).
+hipError_t launch_nw_band(const NwArgs& a, hipStream_t s);
+// BinInfo.cells of every bin for the row-band kernel.
+hipError_t launch_band_info(const BinSpace& bs, BinInfo* info, hipStream_t s);
+size_t nw_band_lds_for(const BinSpace& bs);
+
 // ---- lambda.hip --------------------------------------------------------
 struct LambdaArgs {
   const UniqueDev* u;
@@ -134,5 +143,32 @@ struct LambdaArgs {
   uint32_t grid;
 };
 hipError_t launch_lambda(const LambdaArgs& a, hipStream_t s);
+
+// ---- lambda32.hip ------------------------------------------------------
+// FP32 Kahan log-lambda and FP64 p-values (with FP64 lambda recompute near
+// omega) over a resident set of aligned pairs.
+struct PairArgs {
+  uint32_t n;
+  const uint8_t* seq;       // rows padded to 16 bytes at off[i]
+  const uint8_t* qual;      // nullable
+  const uint64_t* off;      // n + 1
+  const uint32_t* len;
+  const uint32_t* subs;     // (query pos << 16 | center base), ascending
+  const uint64_t* sub_off;  // n + 1
+  const uint32_t* reads;
+  const uint32_t* center_reads;
+  const double* err;
+  uint32_t ncol;
+  int use_quals;
+  double omega;
+  float* out_loglam;
+  double* out_lambda;
+  double* out_p;
+  uint8_t* out_flag;
+  uint32_t* n_recomputed;
+  uint32_t max_blocks;
+  hipEvent_t ev_mid;        // nullable: recorded between the two kernels
+};
+hipError_t launch_pairs_lambda(const PairArgs& a, hipStream_t s);
 
 }  // namespace d2g

@@ -1,0 +1,433 @@
+// nw_band.hip — the scalar ends-free aligners of dada2-rs on gfx950,
+// bit-exact: align_endsfree_with_buf (nwalign.rs:336-417) and
+// align_endsfree_homo_with_buf (nwalign.rs:435-528), i32 arithmetic.
+//
+// Reference semantics restated (s1 = the center, s2 = the query; no swap):
+//   * matrix (len1+1) x (len2+1), d zeroed, row 0 pointers 2 (left), column 0
+//     pointers 3 (up) (nwalign.rs:456-466);
+//   * row i computes j in [max(1, i-lband), min(len2, i+rband)], the band
+//     widened by the length difference (band_adjust, nwalign.rs:276-284);
+//     band < 0 computes every j;
+//   * when band < len1 or band < len2, d = -9999 at j = i-lband-1 and
+//     j = i+rband+1 for i = 0..len1 (fill_band_sentinels, nwalign.rs:287-308),
+//     pointers there keep their initial value;
+//   * left = d[i][j-1] + (i == len1 ? 0 : gap), up = d[i-1][j] + (j == len2 ?
+//     0 : gap), where the homopolymer variant uses homo_gap instead of gap when
+//     the base opposite the gap (s2[j-1] for left, s1[i-1] for up) lies in a
+//     run of >= 3 identical bytes (homopolymer_mask_into, nwalign.rs:211-227);
+//   * up if up >= diag && up >= left, else left if left >= diag, else diag;
+//   * traceback from (len1, len2) (traceback_into, nwalign.rs:234-273).
+//
+// Mapping.  Cells of one anti-diagonal t = i + j are independent.  Band
+// diagonals d = j - i run over [-lband-1, rband+1] (sentinels included),
+// indexed D = d + lband + 1 in [0, dmax].  On anti-diagonal t only diagonals
+// with D = t + lband + 1 (mod 2) hold cells, so lane c of a pair's group
+// owns D = 2c or 2c+1 — one cell per lane per step.  The up neighbour (D+1)
+// and left neighbour (D-1) of the previous step are the lane itself or its
+// neighbour (DPP shift); the diagonal neighbour is the lane's own value two
+// steps back.  Every lane classifies its cell (interior, row 0 / column 0,
+// sentinel, outside the matrix) from (i, j).
+//
+// Pointers go to a per-workgroup history slot (2 bits per cell, a word per
+// 16 steps); a diagonal step over a mismatch is stored as 0 and told apart
+// from an unset pointer by the cell's class.  The traceback sweeps the
+// steps downwards, reading the path cell's word from its owner lane with
+// ds_bpermute, and writes al2subs records (compare) or alignment strings.
+#include <algorithm>
+
+#include "d2g_device.h"
+#include "d2g_launch.h"
+
+namespace d2g {
+
+constexpr int kBandSentinel = -9999;  // nwalign.rs:287-308
+
+struct BandDev {
+  BinSpace bins;
+  const uint32_t* bin_start;
+  const uint32_t* wave_start;
+  const DpTask* tasks;
+  const uint8_t* pool;
+  int match, mismatch, gap, homo_gap;
+  Counters* ctr;
+  int emit;
+  uint32_t* sub_pool;
+  size_t sub_capacity;
+  uint32_t sub_stride;
+  uint32_t* sub_cnt;
+  uint32_t* hist;
+  size_t hist_stride;
+  uint8_t* al0;
+  uint8_t* al1;
+  const uint64_t* al_off;
+  uint32_t* al_len;
+};
+
+// Per-bin cell counts for the plan's GCUPS total (BinInfo.cells).
+__global__ void band_info_kernel(BinSpace bs, BinInfo* info) {
+  const int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= bs.nbins) return;
+  int l1, l2;
+  bin_lengths(bs, (uint32_t)b, &l1, &l2);
+  const BandGeom g = band_geom(l1, l2, bs.band);
+  BinInfo bi;
+  bi.steady_lo = 0;
+  bi.steady_hi = 0;
+  bi.cells = band_cells(g, bs.band);
+  bi._pad = 0;
+  info[b] = bi;
+}
+
+hipError_t launch_band_info(const BinSpace& bs, BinInfo* info, hipStream_t s) {
+  const int threads = 64;
+  const int blocks = (bs.nbins + threads - 1) / threads;
+  hipLaunchKernelGGL(band_info_kernel, dim3(blocks), dim3(threads), 0, s, bs, info);
+  return hipGetLastError();
+}
+
+// Per-wavefront constants.
+struct BandK {
+  int len1, len2, lband, dmax;
+  int match, mismatch, gap, homo;
+};
+// Per-lane state: the lane's two diagonals (d for D = 2c and 2c+1), their
+// class flags, and the scores of the last two steps.
+struct BandLane {
+  int dE, dO;
+  bool sentE, sentO, okE, okO;
+  int prev2, prev;
+};
+
+// One anti-diagonal t for every pair of the wavefront; PAR = parity of the
+// lane's diagonal D on this step.  Returns the cell's pointer code.
+template <bool HOMO, int PAR>
+__device__ __forceinline__ uint32_t band_step(const int t, BandLane& L, const BandK K,
+                                              const uint8_t* s1, const uint8_t* s2,
+                                              const uint8_t* h1, const uint8_t* h2) {
+  const int d = PAR ? L.dO : L.dE;
+  const int i = (t - d) >> 1;  // t - d is even on this step
+  const int j = i + d;
+  const bool inm = (PAR ? L.okO : L.okE) && (unsigned)i <= (unsigned)K.len1 &&
+                   (unsigned)j <= (unsigned)K.len2;
+  const int ii = min(max(i - 1, 0), K.len1 - 1), jj = min(max(j - 1, 0), K.len2 - 1);
+  const int ca = s1[ii], cb = s2[jj];
+  const bool differ = ca != cb;
+  const int diag = L.prev2 + (differ ? K.mismatch : K.match);
+  const int lraw = PAR ? L.prev : from_lower(L.prev, 0);
+  const int uraw = PAR ? from_upper(L.prev, 0) : L.prev;
+  int gl = K.gap, gu = K.gap;
+  if (HOMO) {
+    gl = h2[jj] ? K.homo : K.gap;
+    gu = h1[ii] ? K.homo : K.gap;
+  }
+  gl = (i == K.len1) ? 0 : gl;  // free end gaps on the last row / column
+  gu = (j == K.len2) ? 0 : gu;
+  const int left = lraw + gl, up = uraw + gu;
+  const bool upw = up >= diag && up >= left;
+  const bool lw = left >= diag;
+  int e = upw ? up : (lw ? left : diag);
+  int p = upw ? 3 : (lw ? 2 : (differ ? 0 : 1));
+  const bool sent = PAR ? L.sentO : L.sentE;
+  const bool interior = inm && i >= 1 && j >= 1 && !sent;
+  const bool edge = inm && (i == 0 || j == 0);
+  e = interior ? e : (sent ? kBandSentinel : 0);
+  p = interior ? p : (edge ? (i == 0 ? 2 : 3) : 0);
+  L.prev2 = L.prev;
+  L.prev = e;
+  return (uint32_t)p;
+}
+
+// Steps [16 blk, 16 blk + 16) ∩ [0, last]; P0 = parity of D at even steps.
+template <bool HOMO, int P0>
+__device__ __forceinline__ void band_block(const int blk, const int last, BandLane& L,
+                                           const BandK K, const uint8_t* s1, const uint8_t* s2,
+                                           const uint8_t* h1, const uint8_t* h2,
+                                           uint32_t* hist_p, const int lane) {
+  const int t0 = blk * 16;
+  uint32_t hp = 0;
+  if (t0 + 15 <= last) {
+#pragma unroll
+    for (int k = 0; k < 16; ++k) {
+      const uint32_t p = (k & 1) ? band_step<HOMO, 1 - P0>(t0 + k, L, K, s1, s2, h1, h2)
+                                 : band_step<HOMO, P0>(t0 + k, L, K, s1, s2, h1, h2);
+      hp |= p << (2 * k);
+    }
+  } else {
+    for (int k = 0; t0 + k <= last; ++k) {
+      const uint32_t p = (k & 1) ? band_step<HOMO, 1 - P0>(t0 + k, L, K, s1, s2, h1, h2)
+                                 : band_step<HOMO, P0>(t0 + k, L, K, s1, s2, h1, h2);
+      hp |= p << (2 * k);
+    }
+  }
+  hist_p[blk * 64 + lane] = hp;
+}
+
+// Per-wave constants of the traceback.
+struct BandBw {
+  int gpos, c, lband, len1, len2, dmax;
+  bool sentinels, real, sub_ok;
+  size_t my_sub;
+  const uint8_t* s1;
+  const uint8_t* s2;
+  uint64_t ao;
+};
+struct BandPath {
+  int rc;  // step of the path cell; -1 once ended
+  int D;   // its diagonal index
+  int ns, nt;
+  bool bad;
+};
+
+// One step t of the traceback (traceback_into, nwalign.rs:234-273) for
+// every pair of the wavefront.
+template <bool EMIT>
+__device__ __forceinline__ void band_bw_step(const int t, const int k, const uint32_t hp,
+                                             BandPath& ps, const BandBw& B, const BandDev& a) {
+  const int owner = ps.D >> 1;
+  const uint32_t w = (uint32_t)__builtin_amdgcn_ds_bpermute((B.gpos + owner) * 4, (int)hp);
+  int pp = (int)((w >> (2 * k)) & 3u);
+  const bool on = ps.rc == t;
+  const int d = ps.D - B.lband - 1;
+  const int i = (t - d) >> 1, j = i + d;
+  // pointer 0: a mismatching diagonal on an interior cell, else unset
+  if (__builtin_amdgcn_ballot_w64(on && pp == 0)) {
+    const bool sentd = B.sentinels && (ps.D == 0 || ps.D == B.dmax);
+    const bool interior = i >= 1 && j >= 1 && i <= B.len1 && j <= B.len2 && !sentd;
+    const bool sub = on && pp == 0 && interior;
+    if (!EMIT && sub && B.c == owner && B.real && B.sub_ok)
+      a.sub_pool[B.my_sub + ps.ns] = ((uint32_t)(j - 1) << 16) | (uint32_t)(i - 1);
+    ps.ns += sub ? 1 : 0;
+    pp = sub ? 1 : pp;
+  }
+  if (EMIT) {
+    if (on && pp != 0 && B.c == owner && B.real) {
+      a.al0[B.ao + ps.nt] = pp == 2 ? (uint8_t)'-' : B.s1[max(i - 1, 0)];
+      a.al1[B.ao + ps.nt] = pp == 3 ? (uint8_t)'-' : B.s2[max(j - 1, 0)];
+    }
+    ps.nt += (on && pp != 0) ? 1 : 0;
+  }
+  ps.bad = ps.bad || (on && pp == 0);
+  // diag: (i-1, j-1), same diagonal two steps back; left: (i, j-1), D-1;
+  // up: (i-1, j), D+1
+  ps.D += on ? (pp == 2 ? -1 : (pp == 3 ? 1 : 0)) : 0;
+  ps.rc = on ? (pp == 1 ? t - 2 : (pp == 0 ? -1 : t - 1)) : ps.rc;
+}
+
+template <bool EMIT>
+__device__ __forceinline__ void band_backward(const int last, const uint32_t* hist_p,
+                                              const int lane, BandPath& ps, const BandBw& B,
+                                              const BandDev& a) {
+  const int top = last >> 4;
+  uint32_t hp = hist_p[top * 64 + lane];
+  uint32_t hp_next = top >= 1 ? hist_p[(top - 1) * 64 + lane] : 0u;
+  const int tlo = top * 16 > 1 ? top * 16 : 1;
+  for (int t = last; t >= tlo; --t) band_bw_step<EMIT>(t, t & 15, hp, ps, B, a);
+  for (int b = top - 1; b >= 1; --b) {
+    hp = hp_next;
+    hp_next = hist_p[(b - 1) * 64 + lane];
+#pragma unroll
+    for (int k = 15; k >= 0; --k) band_bw_step<EMIT>(b * 16 + k, k, hp, ps, B, a);
+  }
+  if (top >= 1) {
+    hp = hp_next;
+    for (int t = 15; t >= 1; --t) band_bw_step<EMIT>(t, t, hp, ps, B, a);
+  }
+}
+
+template <bool HOMO>
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(8, 8))) void nw_band_kernel(
+    BandDev a) {
+  extern __shared__ uint8_t band_lds[];
+  const int lane = (int)__lane_id();
+  const BinSpace bs = a.bins;
+  const uint32_t total = (uint32_t)uni((int)a.ctr->total_waves);
+  for (uint32_t w = blockIdx.x; w < total; w += gridDim.x) {
+    // bin of wavefront w: last b with wave_start[b] <= w
+    int lo = 0, hi = bs.nbins;
+    while (hi - lo > 1) {
+      const int mid = (lo + hi) >> 1;
+      if (a.wave_start[mid] <= w) lo = mid; else hi = mid;
+    }
+    const int bin = uni(lo);
+    int l1, l2;
+    bin_lengths(bs, (uint32_t)bin, &l1, &l2);
+    const BandGeom geo = band_geom(l1, l2, bs.band);
+    const int G = band_groups(geo, HOMO);
+    const uint32_t first = a.bin_start[bin] + (w - a.wave_start[bin]) * (uint32_t)G;
+    const int count = uni((int)min((uint32_t)G, a.bin_start[bin + 1] - first));
+    const int lanes = geo.lanes;
+    const int g_raw = lane / lanes;
+    const int c = lane - g_raw * lanes;
+    const bool real = g_raw < count;
+    const int g = real ? g_raw : 0;  // idle lanes shadow group 0
+    const DpTask task = a.tasks[first + (uint32_t)g];
+    const int len1 = geo.len1, len2 = geo.len2, last = len1 + len2;
+
+    // ---- stage s1 (center), s2 (query) and their homopolymer masks ----
+    uint8_t* gb = band_lds + (size_t)g * band_group_bytes(geo, HOMO);
+    uint8_t* s1 = gb;
+    uint8_t* s2 = gb + len1;
+    uint8_t* h1 = gb + len1 + len2;
+    uint8_t* h2 = h1 + len1;
+    if (real) {
+      const uint8_t* src1 = a.pool + task.s1_off;
+      const uint8_t* src2 = a.pool + task.s2_off;
+      for (int x = c; x < len1; x += lanes) s1[x] = src1[x];
+      for (int x = c; x < len2; x += lanes) s2[x] = src2[x];
+    }
+    __syncthreads();
+    if (HOMO && real) {
+      // position x lies in a run of >= 3 identical bytes
+      for (int x = c; x < len1 + len2; x += lanes) {
+        const uint8_t* s = x < len1 ? s1 : s2;
+        const int n = x < len1 ? len1 : len2;
+        const int y = x < len1 ? x : x - len1;
+        const uint8_t v = s[y];
+        const bool m1 = y >= 1 && s[y - 1] == v;
+        const bool m2 = y >= 2 && m1 && s[y - 2] == v;
+        const bool p1 = y + 1 < n && s[y + 1] == v;
+        const bool p2 = y + 2 < n && p1 && s[y + 2] == v;
+        (x < len1 ? h1 : h2)[y] = (m2 || (m1 && p1) || p2) ? 1 : 0;
+      }
+    }
+    __syncthreads();
+
+    BandK K;
+    K.len1 = len1;
+    K.len2 = len2;
+    K.lband = geo.lband;
+    K.dmax = geo.dmax;
+    K.match = a.match;
+    K.mismatch = a.mismatch;
+    K.gap = a.gap;
+    K.homo = a.homo_gap;
+    BandLane L;
+    L.dE = 2 * c - geo.lband - 1;
+    L.dO = L.dE + 1;
+    L.okE = 2 * c <= geo.dmax;
+    L.okO = 2 * c + 1 <= geo.dmax;
+    L.sentE = geo.sentinels && (2 * c == 0 || 2 * c == geo.dmax);
+    L.sentO = geo.sentinels && (2 * c + 1 == geo.dmax);
+    L.prev2 = 0;
+    L.prev = 0;
+
+    // ---- forward: anti-diagonals 0..last ----
+    uint32_t* hist_p = a.hist + (size_t)blockIdx.x * a.hist_stride;
+    const int P0 = (geo.lband + 1) & 1;  // parity of D on even steps
+    for (int blk = 0; blk * 16 <= last; ++blk) {
+      if (P0)
+        band_block<HOMO, 1>(blk, last, L, K, s1, s2, h1, h2, hist_p, lane);
+      else
+        band_block<HOMO, 0>(blk, last, L, K, s1, s2, h1, h2, hist_p, lane);
+    }
+    __syncthreads();
+
+    // ---- traceback ----
+    const uint32_t ti = first + (uint32_t)g;
+    const size_t sub0 = (size_t)ti * a.sub_stride;
+    const bool sub_ok = a.emit || sub0 + a.sub_stride <= a.sub_capacity;
+    if (!a.emit && __builtin_amdgcn_ballot_w64(!sub_ok && real) && lane == 0)
+      atomicOr(&a.ctr->err_flags, kErrSubOverflow);
+    BandBw B;
+    B.gpos = g_raw * lanes;
+    B.c = c;
+    B.lband = geo.lband;
+    B.len1 = len1;
+    B.len2 = len2;
+    B.dmax = geo.dmax;
+    B.sentinels = geo.sentinels;
+    B.real = real;
+    B.sub_ok = sub_ok;
+    B.my_sub = sub_ok ? sub0 : 0;
+    B.s1 = s1;
+    B.s2 = s2;
+    B.ao = (a.emit && real) ? a.al_off[task.out_idx] : 0;
+    BandPath ps;
+    ps.rc = real ? last : -1;
+    ps.D = len2 - len1 + geo.lband + 1;
+    ps.ns = 0;
+    ps.nt = 0;
+    ps.bad = false;
+    if (a.emit)
+      band_backward<true>(last, hist_p, lane, ps, B, a);
+    else
+      band_backward<false>(last, hist_p, lane, ps, B, a);
+    if (__builtin_amdgcn_ballot_w64(ps.bad && real) && lane == 0)
+      atomicOr(&a.ctr->err_flags, kErrTraceback);
+    __syncthreads();
+
+    // ---- outputs ----
+    if (a.emit) {
+      if (real) {
+        uint8_t* o0 = a.al0 + B.ao;
+        uint8_t* o1 = a.al1 + B.ao;
+        const int nt = ps.nt;
+        for (int x = c; x < nt / 2; x += lanes) {
+          const int y = nt - 1 - x;
+          const uint8_t t0 = o0[x], t1 = o1[x];
+          o0[x] = o0[y];
+          o1[x] = o1[y];
+          o0[y] = t0;
+          o1[y] = t1;
+        }
+        if (c == 0) a.al_len[task.out_idx] = ps.bad ? 0u : (uint32_t)nt;
+      }
+    } else if (real && c == 0) {
+      a.sub_cnt[ti] = sub_ok ? ((uint32_t)ps.ns | (ps.bad ? 0x80000000u : 0u)) : 0u;
+    }
+    __syncthreads();
+  }
+}
+
+hipError_t launch_nw_band(const NwArgs& s, hipStream_t st) {
+  if (s.grid == 0 || s.hist == nullptr) return hipErrorInvalidValue;
+  if (!s.emit_alignment && s.sub_stride == 0) return hipErrorInvalidValue;
+  BandDev d;
+  d.bins = s.bins;
+  d.bin_start = s.bin_start;
+  d.wave_start = s.wave_start;
+  d.tasks = s.tasks;
+  d.pool = s.pool;
+  d.match = s.match;
+  d.mismatch = s.mismatch;
+  d.gap = s.gap_p;
+  d.homo_gap = s.homo_gap_p;
+  d.ctr = s.ctr;
+  d.emit = s.emit_alignment;
+  d.sub_pool = s.sub_pool;
+  d.sub_capacity = s.sub_capacity;
+  d.sub_stride = s.sub_stride;
+  d.sub_cnt = s.sub_cnt;
+  d.hist = s.hist;
+  d.hist_stride = s.hist_stride;
+  d.al0 = s.al0;
+  d.al1 = s.al1;
+  d.al_off = s.al_off;
+  d.al_len = s.al_len;
+  const size_t lds = s.lds_bytes;
+  const void* fn = s.bins.homo ? (const void*)nw_band_kernel<true> : (const void*)nw_band_kernel<false>;
+  if (lds > 48 * 1024) {
+    hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    if (e != hipSuccess) return e;
+  }
+  if (s.bins.homo)
+    hipLaunchKernelGGL(nw_band_kernel<true>, dim3(s.grid), dim3(64), lds, st, d);
+  else
+    hipLaunchKernelGGL(nw_band_kernel<false>, dim3(s.grid), dim3(64), lds, st, d);
+  return hipGetLastError();
+}
+
+size_t nw_band_lds_for(const BinSpace& bs) {
+  size_t lds = 0;
+  for (int b = 0; b < bs.nbins; ++b) {
+    int l1, l2;
+    bin_lengths(bs, (uint32_t)b, &l1, &l2);
+    const BandGeom g = band_geom(l1, l2, bs.band);
+    const int G = band_groups(g, bs.homo != 0);
+    if (G > 0) lds = std::max(lds, band_wave_bytes(g, G, bs.homo != 0));
+  }
+  lds = (lds + 255) & ~(size_t)255;
+  return lds < 256 ? 256 : lds;
+}
+
+}  // namespace d2g

@@ -33,6 +33,7 @@
 // kernel; in parity mode the alignment strings are written instead.
 #include <type_traits>
 
+#include "d2g_device.h"
 #include "d2g_launch.h"
 
 namespace d2g {
@@ -198,17 +199,6 @@ hipError_t launch_sched(const BinSpace& bs, int rows, int end_gap, int gap, RowR
 // ---------------------------------------------------------------------------
 // DP
 // ---------------------------------------------------------------------------
-
-// lane i <- lane i-1 (DPP wave_shr:1); lane 0 keeps `old`
-__device__ __forceinline__ int from_lower(int v, int old) {
-  return __builtin_amdgcn_update_dpp(old, v, 0x138, 0xF, 0xF, false);
-}
-// lane i <- lane i+1 (DPP wave_shl:1); lane 63 keeps `old`
-__device__ __forceinline__ int from_upper(int v, int old) {
-  return __builtin_amdgcn_update_dpp(old, v, 0x130, 0xF, 0xF, false);
-}
-
-__device__ __forceinline__ int uni(int v) { return __builtin_amdgcn_readfirstlane(v); }
 
 __device__ __forceinline__ int clamp16(int v) { return v > 32767 ? 32767 : (v < -32768 ? -32768 : v); }
 
@@ -521,13 +511,13 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(8, 8))) void
     int l1_in, l2_in;
     bin_lengths(bs, (uint32_t)bin, &l1_in, &l2_in);
     const VecGeom geo = vec_geom(l1_in, l2_in, bs.band);
-    const int G = nw_groups(geo);
+    const int G = bin_groups(bs, l1_in, l2_in);
     const uint32_t wave_in_bin = w - a.wave_start[bin];
     const uint32_t first = a.bin_start[bin] + wave_in_bin * (uint32_t)G;
     const uint32_t bin_end = a.bin_start[bin + 1];
     const int count = uni((int)min((uint32_t)G, bin_end - first));
 
-    const int len1 = geo.len1, len2 = geo.len2, ncol = geo.ncol, nrow = geo.nrow;
+    const int len1 = geo.len1, len2 = geo.len2, ncol = geo.ncol;
     const int start_col = geo.start_col;
     const bool swap = geo.swap != 0;
     const int g_raw = lane / ncol;

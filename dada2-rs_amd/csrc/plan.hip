@@ -1,11 +1,12 @@
 // plan.hip — binning of DP pairs by geometry, on the device.
 //
-// The vectorized DP walks anti-diagonals with a row schedule fixed by
-// (len1, len2, band) (nwalign.rs:875-1088).  Pairs with identical
-// (len_center, len_query) share a wavefront so that schedule is uniform
+// Both DP kernels walk anti-diagonals with a geometry fixed by (len1, len2,
+// band) (nwalign.rs:875-1088 for the vectorized aligner; the row band of
+// nwalign.rs:276-308 for the scalar ones).  Pairs with identical
+// (len_center, len_query) share a wavefront so that the walk is uniform
 // (scalar registers) while lanes compute cells.  plan_kernel turns per-bin
-// pair counts into bin offsets and wavefront offsets (G = 64 / ncol pairs
-// per wavefront); scatter_kernel places each pair into its bin.
+// pair counts into bin offsets and wavefront offsets (G pairs per
+// wavefront, bin_groups); scatter_kernel places each pair into its bin.
 #include "d2g_launch.h"
 
 namespace d2g {
@@ -71,8 +72,7 @@ __global__ __launch_bounds__(kPlanThreads) void plan_kernel(BinSpace bs,
     uint32_t cnt = bin_count[b];
     int l1, l2;
     bin_lengths(bs, b, &l1, &l2);
-    VecGeom g = vec_geom(l1, l2, bs.band);
-    int G = nw_groups(g);
+    const int G = bin_groups(bs, l1, l2);
     if (cnt && !G) bad = 1;
     sp += cnt;
     sw += G ? (cnt + G - 1) / G : 0;
@@ -87,8 +87,7 @@ __global__ __launch_bounds__(kPlanThreads) void plan_kernel(BinSpace bs,
     uint32_t cnt = bin_count[b];
     int l1, l2;
     bin_lengths(bs, b, &l1, &l2);
-    VecGeom g = vec_geom(l1, l2, bs.band);
-    int G = nw_groups(g);
+    const int G = bin_groups(bs, l1, l2);
     bin_start[b] = rp;
     wave_start[b] = rw;
     bin_fill[b] = 0;

@@ -195,3 +195,109 @@ def subsample(w: Workload, n_uniques: int, n_centers: int | None = None) -> Work
         keep = keep[:n_centers]
     return Workload(w.name + f"[:{n_uniques}]", w.seqs[:n_uniques], w.quals[:n_uniques],
                     w.reads[:n_uniques], w.err, keep, w.band, w.kmer_size, dict(w.extra))
+
+
+def _homopolymer_indels(rng, s: np.ndarray, rate: float) -> np.ndarray:
+    """At each position with probability `rate`, lengthen the homopolymer run
+    there by one base, or shorten it when the run has length >= 2."""
+    hits = np.nonzero(rng.random(len(s)) < rate)[0]
+    if len(hits) == 0:
+        return s
+    out = list(s)
+    for pos in sorted(hits.tolist(), reverse=True):
+        if pos >= len(out):
+            continue
+        in_run = (pos > 0 and out[pos - 1] == out[pos]) or \
+                 (pos + 1 < len(out) and out[pos + 1] == out[pos])
+        if in_run and rng.random() < 0.5 and len(out) > 1:
+            del out[pos]
+        else:
+            out.insert(pos, out[pos])
+    return np.asarray(out, np.uint8)
+
+
+def workload_longread(n_variants=100, n_uniques=5000, lmin=1420, lmax=1480, sub=0.001,
+                      homo_indel=0.003, seed=SEED_BASE + 4) -> Workload:
+    """Long-read (PacBio-shape) compare: uniques are variant prefixes of length
+    U[lmin, lmax] with substitutions (sub/base) and homopolymer-length indels
+    (homo_indel/base); Phred 0-93 with mean 60, sd 10, and the fixed 16 x 94
+    error matrix.  Band 32; the compare runs with homo_gap_p = -1.  Centers:
+    the most abundant unique of each variant."""
+    rng = np.random.default_rng(seed)
+    err = fixed_err(94)
+    variants = [rng.integers(1, 5, size=lmax).astype(np.uint8) for _ in range(n_variants)]
+    seqs, quals, origin = [], [], []
+    seen: set = set()
+    while len(seqs) < n_uniques:
+        v = int(rng.integers(0, n_variants))
+        Lu = int(rng.integers(lmin, lmax + 1))
+        s = _substitute(rng, variants[v][:Lu], np.full(Lu, sub))
+        s = _homopolymer_indels(rng, s, homo_indel)
+        if s.tobytes() in seen:
+            continue
+        seen.add(s.tobytes())
+        seqs.append(s)
+        quals.append(np.clip(np.rint(rng.normal(60.0, 10.0, size=len(s))), 0, 93).astype(np.uint8))
+        origin.append(v)
+    reads = abundance(rng, len(seqs))
+    origin = np.asarray(origin)
+
+    def centers(inv, r):
+        o = origin[np.argsort(inv)]
+        best = {}
+        for i in range(len(o)):
+            best.setdefault(int(o[i]), i)
+        return sorted(best.values())
+
+    return _finish("longread", seqs, quals, reads, err, centers, band=32,
+                   extra={"origin": origin, "homo_gap_p": -1})
+
+
+@dataclass
+class PairSet:
+    """Aligned (query, center) pairs with their substitution lists."""
+    seqs: list[np.ndarray]
+    quals: list[np.ndarray]
+    subs: list[np.ndarray]       # (query pos << 16 | center base), ascending
+    reads: np.ndarray
+    center_reads: np.ndarray
+    err: np.ndarray
+
+    @property
+    def n(self) -> int:
+        return len(self.seqs)
+
+
+def workload_precision(n_pairs=1_000_000, L=250, n_centers=2000, seed=SEED_BASE + 2) -> PairSet:
+    """Aligned pairs at L nt for the mixed-precision lambda: centers uniform
+    over ACGT; each query is a center copy with substitutions drawn from the
+    seeded error matrix (off-diagonal rates log-uniform in [1e-5, 1e-1], rows
+    normalised) at the query's per-position quality (the plain workload's
+    profile); abundances lognormal(2, 1.5) for queries, lognormal(6, 1.5)
+    for centers."""
+    rng = np.random.default_rng(seed)
+    err = seeded_err(rng, 41)
+    centers = rng.integers(1, 5, size=(n_centers, L)).astype(np.uint8)
+    cid = rng.integers(0, n_centers, size=n_pairs)
+    ref = centers[cid]                                            # (n, L)
+    pos = np.arange(L, dtype=np.float64)
+    mu = 36.0 - 8.0 * pos / max(L - 1, 1)
+    qual = np.clip(np.rint(rng.normal(mu, 3.0, size=(n_pairs, L))), 0, 40).astype(np.uint8)
+    # query base given the reference base and quality: rows 4*ref + query
+    r0 = (ref.astype(np.int64) - 1) * 4
+    cum = np.zeros((n_pairs, L), np.float64)
+    u = rng.random((n_pairs, L))
+    qry = np.zeros((n_pairs, L), np.uint8)
+    for qb in range(4):
+        cum += err[r0 + qb, qual]
+        pick = (qry == 0) & (u < cum)
+        qry[pick] = qb + 1
+    qry[qry == 0] = ref[qry == 0]
+    seqs = list(qry)
+    quals = list(qual)
+    rows, cols = np.nonzero(qry != ref)
+    recs = (cols.astype(np.uint32) << 16) | ref[rows, cols].astype(np.uint32)
+    subs = np.split(recs, np.cumsum(np.bincount(rows, minlength=n_pairs))[:-1])
+    reads = abundance(rng, n_pairs)
+    center_reads = np.maximum(1, np.rint(rng.lognormal(6.0, 1.5, size=n_pairs))).astype(np.uint32)
+    return PairSet(seqs, quals, subs, reads, center_reads, err)

@@ -158,6 +158,57 @@ uint64_t d2g_vectorized_cells(uint32_t len1, uint32_t len2, int32_t band);
  * make d2g_align_vectorized / d2g_compare_* return D2G_ERR_UNSUPPORTED. */
 int d2g_vectorized_supported(uint32_t len1, uint32_t len2, int32_t band);
 
+/* Backs align_endsfree_with_buf (nwalign.rs:336-417, when homo_gap_p ==
+ * gap_p) and align_endsfree_homo_with_buf (nwalign.rs:435-528, homopolymer
+ * runs of >= 3 bytes take homo_gap_p) for n pairs: i32 scores, band < 0
+ * unbanded, free end gaps on the last row and column, no swap.  Buffers as
+ * for d2g_align_vectorized. */
+int d2g_align_endsfree(d2g_ctx* ctx, uint32_t n, const uint8_t* s1, const uint64_t* s1_off,
+                       const uint8_t* s2, const uint64_t* s2_off, int32_t match, int32_t mismatch,
+                       int32_t gap_p, int32_t homo_gap_p, int32_t band, uint8_t* al0,
+                       uint8_t* al1, const uint64_t* al_offsets, uint32_t* al_len);
+
+/* 1 when the row-band kernel handles this geometry (band diagonals plus the
+ * two sentinel diagonals fit 128 = 2 x 64 lanes), 0 otherwise.  The compare
+ * routes to it for homopolymer gap penalties (homo = 1), for vectorized = 0,
+ * and when every unique is longer than 3500 nt (nwalign.rs:1335-1414). */
+int d2g_band_supported(uint32_t len1, uint32_t len2, int32_t band, int homo);
+
+/* ---- mixed-precision lambda and abundance p-values ----------------------
+ * A resident set of n aligned pairs: query i (bases 1..4, qualities, read
+ * count reads[i]) with its al2subs list against its center (records
+ * (query position << 16 | center base 1..4), ascending query position;
+ * sub_offsets has n+1 entries) and the center's read count center_reads[i].
+ * d2g_pairs_lambda computes, on the device:
+ *   loglam32[i]  log compute_lambda (pval.rs:253-320) as an FP32
+ *                compensated (Kahan) sum of FP32-rounded log err[t][q]
+ *   lambda[i]    exp(loglam32[i]) in FP64, or the exact ordered FP64 product
+ *                when recomputed
+ *   pval[i]      get_pA(reads, prior = false, lambda, hamming = #subs,
+ *                center_reads, detect_singletons = false) (pval.rs:376-398,
+ *                calc_pA pval.rs:208-241) in FP64
+ *   recomputed[i] 1 when the first p-value fell within a factor 10 of
+ *                omega_a, so lambda was recomputed exactly in FP64 and the
+ *                p-value taken again.
+ * The reference computes lambda only as the FP64 product; this is the
+ * reduced-precision variant with its FP64 safeguard. */
+typedef struct d2g_pair_stats {
+  uint64_t pairs;
+  uint64_t recomputed;
+  float ms_loglam;  /* FP32 log-lambda kernel */
+  float ms_pval;    /* p-value kernel (incl. recomputes) */
+  float ms_total;
+  float _pad;
+} d2g_pair_stats;
+
+int d2g_pairs_load(d2g_ctx* ctx, uint32_t n, const uint8_t* seq_concat, const uint64_t* offsets,
+                   const uint8_t* qual_concat, const uint32_t* subs, const uint64_t* sub_offsets,
+                   const uint32_t* reads, const uint32_t* center_reads);
+int d2g_pairs_lambda(d2g_ctx* ctx, const double* err, uint32_t ncol, int use_quals,
+                     double omega_a, d2g_pair_stats* stats);
+int d2g_pairs_fetch(d2g_ctx* ctx, float* out_loglam32, double* out_lambda, double* out_pval,
+                    uint8_t* out_recomputed);
+
 #ifdef __cplusplus
 }
 #endif

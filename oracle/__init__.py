@@ -389,6 +389,33 @@ def calc_pA(reads: int, e_reads: float, prior: bool) -> float:
     return lib().orc_calc_pA(reads, e_reads, int(prior))
 
 
+def get_pA(reads: int, lam: float, hamming: int, bi_reads: int, prior=False,
+           detect_singletons=False) -> float:
+    """get_pA, pval.rs:376-398."""
+    return lib().orc_get_pA(reads, int(prior), lam, hamming, bi_reads, int(detect_singletons))
+
+
+def pairs_lambda_fp64(seqs, quals, subs, err: np.ndarray, use_quals=True) -> np.ndarray:
+    """compute_lambda (pval.rs:253-320) for equal-length aligned pairs, as the
+    ordered FP64 left fold from 1.0 over query positions (pval.rs:311-313),
+    vectorised over pairs: t = 5 (nt-1) on the diagonal, 4 (nt0-1) + (nt-1)
+    at a substitution record (query pos << 16 | center base)."""
+    q = np.stack([np.asarray(s, np.int64) for s in seqs])
+    n, L = q.shape
+    t = 5 * (q - 1)
+    for i, rec in enumerate(subs):
+        rec = np.asarray(rec, np.int64)
+        if rec.size:
+            pos, nt0 = rec >> 16, rec & 0xFFFF
+            t[i, pos] = 4 * (nt0 - 1) + (q[i, pos] - 1)
+    qi = np.stack([np.asarray(x, np.int64) for x in quals]) if (use_quals and quals is not None) \
+        else np.zeros((n, L), np.int64)
+    lam = np.ones(n, np.float64)
+    for pos in range(L):
+        lam *= err[t[:, pos], qi[:, pos]]
+    return lam
+
+
 def poisson_upper_tail_direct(reads: int, lam: float) -> float:
     return lib().orc_poisson_upper_tail_direct(reads, lam)
 
