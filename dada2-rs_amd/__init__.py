@@ -110,6 +110,13 @@ def _lib():
             "d2g_pairs_lambda": (C.c_int, [vp, f64p, C.c_uint32, C.c_int, C.c_double,
                                            C.POINTER(PairStats)]),
             "d2g_pairs_fetch": (C.c_int, [vp, C.POINTER(C.c_float), f64p, f64p, u8p]),
+            "d2g_bimera_pairs": (C.c_int, [vp, C.c_uint32, u8p, u64p, u8p, u64p,
+                                           C.POINTER(BimeraParams), u32p, u32p]),
+            "d2g_is_bimera_batch": (C.c_int, [vp, C.c_uint32, u8p, u64p, u8p, u64p, u32p,
+                                              C.POINTER(BimeraParams), u8p]),
+            "d2g_table_bimera2": (C.c_int, [vp, u32p, C.c_uint32, C.c_uint32, u8p, u64p,
+                                            C.c_double, C.c_uint32, C.POINTER(BimeraParams),
+                                            u32p, u32p]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -126,7 +133,7 @@ def exported_symbols() -> list[str]:
             "d2g_compare_batch", "d2g_compare_batch_device", "d2g_fetch_results",
             "d2g_align_vectorized", "d2g_vectorized_cells", "d2g_vectorized_supported",
             "d2g_align_endsfree", "d2g_band_supported", "d2g_pairs_load", "d2g_pairs_lambda",
-            "d2g_pairs_fetch"]
+            "d2g_pairs_fetch", "d2g_bimera_pairs", "d2g_is_bimera_batch", "d2g_table_bimera2"]
 
 
 def _p(a: np.ndarray, t):
@@ -157,6 +164,19 @@ def _concat(seqs) -> tuple[np.ndarray, np.ndarray]:
     off[1:] = np.cumsum([len(a) for a in arrs])
     cat = np.concatenate(arrs) if arrs else np.zeros(0, np.uint8)
     return np.ascontiguousarray(cat, np.uint8), off
+
+
+class BimeraParams(C.Structure):
+    """BimeraAlignParams (chimera.rs:183-196), NW backend."""
+    _fields_ = [("allow_one_off", C.c_int32), ("min_one_off_par_dist", C.c_int32),
+                ("match_score", C.c_int16), ("mismatch", C.c_int16), ("gap_p", C.c_int16),
+                ("_pad", C.c_int16), ("max_shift", C.c_int32)]
+
+
+def bimera_params(allow_one_off=False, min_one_off_par_dist=4, match=5, mismatch=-4, gap_p=-8,
+                  max_shift=16) -> BimeraParams:
+    return BimeraParams(int(allow_one_off), min_one_off_par_dist, match, mismatch, gap_p, 0,
+                        max_shift)
 
 
 class PairStats(C.Structure):
@@ -320,6 +340,52 @@ class Context:
         _check(_lib().d2g_pairs_fetch(self.h, ll.ctypes.data_as(C.POINTER(C.c_float)),
                                       _p(lam, f64p), _p(pv, f64p), _p(fl, u8p)), "d2g_pairs_fetch")
         return ll, lam, pv, fl, st.as_dict()
+
+    def bimera_pairs(self, queries, parents, params: BimeraParams | None = None):
+        """get_lr + get_ham_endsfree per (query, parent) pair: returns
+        (lr (n x 4: left, right, left_oo, right_oo), ham (n))."""
+        params = params or bimera_params()
+        n = len(queries)
+        qc, qo = _concat(queries)
+        pc, po = _concat(parents)
+        lr = np.zeros((n, 4), np.uint32)
+        ham = np.zeros(n, np.uint32)
+        _check(_lib().d2g_bimera_pairs(self.h, n, _p(qc, u8p), _p(qo, u64p), _p(pc, u8p),
+                                       _p(po, u64p), C.byref(params), _p(lr, u32p),
+                                       _p(ham, u32p)), "d2g_bimera_pairs")
+        return lr, ham
+
+    def is_bimera_batch(self, queries, parent_lists, params: BimeraParams | None = None):
+        """is_bimera_with_buf (chimera.rs:218) for each query against its own
+        parent list."""
+        params = params or bimera_params()
+        nq = len(queries)
+        qc, qo = _concat(queries)
+        flat = [p for ps in parent_lists for p in ps]
+        pc, po = _concat(flat) if flat else (np.zeros(1, np.uint8), np.zeros(1, np.uint64))
+        ps_ = np.zeros(nq + 1, np.uint32)
+        ps_[1:] = np.cumsum([len(ps) for ps in parent_lists])
+        out = np.zeros(nq, np.uint8)
+        _check(_lib().d2g_is_bimera_batch(self.h, nq, _p(qc, u8p), _p(qo, u64p), _p(pc, u8p),
+                                          _p(po, u64p), _p(ps_, u32p), C.byref(params),
+                                          _p(out, u8p)), "d2g_is_bimera_batch")
+        return out.astype(bool)
+
+    def table_bimera2(self, mat, seqs, min_fold=1.5, min_abund=2,
+                      params: BimeraParams | None = None):
+        """table_bimera2 (chimera.rs:317): mat is (samples, sequences)."""
+        params = params or bimera_params()
+        mat = np.asarray(mat)
+        nrow, ncol = mat.shape
+        colmajor = np.ascontiguousarray(mat.T.reshape(-1), np.uint32)
+        cat, off = _concat(seqs)
+        nflag = np.zeros(ncol, np.uint32)
+        nsam = np.zeros(ncol, np.uint32)
+        _check(_lib().d2g_table_bimera2(self.h, _p(colmajor, u32p), nrow, ncol, _p(cat, u8p),
+                                        _p(off, u64p), float(min_fold), int(min_abund),
+                                        C.byref(params), _p(nflag, u32p), _p(nsam, u32p)),
+               "d2g_table_bimera2")
+        return nflag, nsam
 
     @staticmethod
     def _pair_buffers(s1s, s2s):

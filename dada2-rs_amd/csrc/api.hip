@@ -95,6 +95,7 @@ struct d2g_ctx {
   DevBuf<uint64_t> p_off, p_suboff;
   DevBuf<uint32_t> p_len, p_subs, p_reads, p_creads, p_nrec;
   DevBuf<float> p_loglam;
+  DevBuf<uint32_t> b_out;  // bimera: 5 words per pair
   DevBuf<double> p_lambda, p_pval, p_err;
   // align mode buffers
   DevBuf<uint8_t> a_pool, a_al0, a_al1;
@@ -181,6 +182,7 @@ int d2g_ctx_destroy(d2g_ctx* c) {
   for (auto* b : {&c->p_off, &c->p_suboff}) b->release();
   for (auto* b : {&c->p_len, &c->p_subs, &c->p_reads, &c->p_creads, &c->p_nrec}) b->release();
   c->p_loglam.release();
+  c->b_out.release();
   for (auto* b : {&c->p_lambda, &c->p_pval, &c->p_err}) b->release();
   for (int i = 0; i < kEvents; ++i)
     if (c->ev[i]) (void)hipEventDestroy(c->ev[i]);
@@ -626,13 +628,16 @@ int d2g_compare_center(d2g_ctx* c, uint32_t center, const double* err, uint32_t 
                            out_hamming, out_skipped, stats);
 }
 
-// n independent pairs through one DP kernel with alignment strings out
-// (parity mode); shared by the align entry points.
-static int align_pairs(d2g_ctx* c, uint32_t n, const uint8_t* s1, const uint64_t* s1_off,
-                       const uint8_t* s2, const uint64_t* s2_off, int kind, int homo, int band,
-                       int match, int mismatch, int gap, int end_gap, int homo_gap, uint8_t* al0,
-                       uint8_t* al1, const uint64_t* al_offsets, uint32_t* al_len) {
-  if (!c || !s1 || !s1_off || !s2 || !s2_off || !al0 || !al1 || !al_len) return D2G_ERR_ARG;
+// n independent pairs through one DP kernel with the alignment strings left
+// in the context's device buffers (a_al0/a_al1 at a_aloff, lengths a_allen);
+// *al_total receives the bytes spanned.  Shared by the align and bimera
+// entry points.
+static int align_pairs_device(d2g_ctx* c, uint32_t n, const uint8_t* s1, const uint64_t* s1_off,
+                              const uint8_t* s2, const uint64_t* s2_off, int kind, int homo,
+                              int band, int match, int mismatch, int gap, int end_gap,
+                              int homo_gap, const uint64_t* al_offsets, uint64_t* al_total_out) {
+  if (!c || !s1 || !s1_off || !s2 || !s2_off) return D2G_ERR_ARG;
+  *al_total_out = 0;
   if (n == 0) return D2G_OK;
   (void)hipSetDevice(c->device);
   // pool: all s1 then all s2
@@ -733,12 +738,28 @@ static int align_pairs(d2g_ctx* c, uint32_t n, const uint8_t* s1, const uint64_t
   HIP_TRY(kind == kDpVectorized ? launch_nw_vec(na, s) : launch_nw_band(na, s));
   Counters h;
   HIP_TRY(hipMemcpyAsync(&h, c->ctr.p, sizeof(Counters), hipMemcpyDeviceToHost, s));
+  HIP_TRY(hipStreamSynchronize(s));
+  *al_total_out = al_total;
+  if (h.err_flags & kErrTraceback) return D2G_ERR_TRACEBACK;
+  if (h.err_flags & kErrGeometry) return D2G_ERR_UNSUPPORTED;
+  return D2G_OK;
+}
+
+// align_pairs_device, then the strings and lengths to the caller.
+static int align_pairs(d2g_ctx* c, uint32_t n, const uint8_t* s1, const uint64_t* s1_off,
+                       const uint8_t* s2, const uint64_t* s2_off, int kind, int homo, int band,
+                       int match, int mismatch, int gap, int end_gap, int homo_gap, uint8_t* al0,
+                       uint8_t* al1, const uint64_t* al_offsets, uint32_t* al_len) {
+  if (!al0 || !al1 || !al_len) return D2G_ERR_ARG;
+  uint64_t al_total = 0;
+  const int rc = align_pairs_device(c, n, s1, s1_off, s2, s2_off, kind, homo, band, match,
+                                    mismatch, gap, end_gap, homo_gap, al_offsets, &al_total);
+  if (rc != D2G_OK || n == 0) return rc;
+  hipStream_t s = c->stream;
   HIP_TRY(hipMemcpyAsync(al_len, c->a_allen.p, n * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
   HIP_TRY(hipMemcpyAsync(al0, c->a_al0.p, al_total, hipMemcpyDeviceToHost, s));
   HIP_TRY(hipMemcpyAsync(al1, c->a_al1.p, al_total, hipMemcpyDeviceToHost, s));
   HIP_TRY(hipStreamSynchronize(s));
-  if (h.err_flags & kErrTraceback) return D2G_ERR_TRACEBACK;
-  if (h.err_flags & kErrGeometry) return D2G_ERR_UNSUPPORTED;
   return D2G_OK;
 }
 
@@ -762,6 +783,205 @@ int d2g_align_endsfree(d2g_ctx* c, uint32_t n, const uint8_t* s1, const uint64_t
 
 int d2g_band_supported(uint32_t len1, uint32_t len2, int32_t band, int homo) {
   return band_groups(band_geom((int)len1, (int)len2, band), homo != 0) > 0 ? 1 : 0;
+}
+
+// ---- bimera ---------------------------------------------------------------
+// (query, parent) pairs -> get_lr + get_ham_endsfree, 5 words per pair.
+static int bimera_lr_pairs(d2g_ctx* c, uint32_t n, const uint8_t* q, const uint64_t* q_off,
+                           const uint8_t* par, const uint64_t* p_off, const d2g_bimera_params* p,
+                           uint32_t* out5) {
+  if (n == 0) return D2G_OK;
+  uint64_t al_total = 0;
+  // the vectorized aligner with end_gap 0 and band = max_shift (chimera.rs:234-240)
+  const int rc = align_pairs_device(c, n, q, q_off, par, p_off, kDpVectorized, 0, p->max_shift,
+                                    p->match_score, p->mismatch, p->gap_p, 0, p->gap_p, nullptr,
+                                    &al_total);
+  if (rc != D2G_OK) return rc;
+  hipStream_t s = c->stream;
+  HIP_TRY(c->b_out.ensure((size_t)5 * n));
+  HIP_TRY(launch_bimera_lr(n, c->a_al0.p, c->a_al1.p, c->a_aloff.p, c->a_allen.p,
+                           p->allow_one_off, (uint32_t)p->max_shift, c->b_out.p, s));
+  HIP_TRY(hipMemcpyAsync(out5, c->b_out.p, (size_t)5 * n * 4, hipMemcpyDeviceToHost, s));
+  HIP_TRY(hipStreamSynchronize(s));
+  return D2G_OK;
+}
+
+// Pairs per device batch: bounds the alignment-string buffers.
+static uint32_t bimera_chunk(const uint64_t* q_off, const uint64_t* p_off, uint32_t i0,
+                             uint32_t n) {
+  uint64_t bytes = 0;
+  uint32_t k = 0;
+  while (i0 + k < n && k < (1u << 18)) {
+    bytes += (q_off[i0 + k + 1] - q_off[i0 + k]) + (p_off[i0 + k + 1] - p_off[i0 + k]);
+    if (bytes > ((uint64_t)1 << 30) && k > 0) break;
+    ++k;
+  }
+  return k;
+}
+
+int d2g_bimera_pairs(d2g_ctx* c, uint32_t n, const uint8_t* q, const uint64_t* q_off,
+                     const uint8_t* par, const uint64_t* p_off, const d2g_bimera_params* p,
+                     uint32_t* out_lr, uint32_t* out_ham) {
+  if (!c || !q || !q_off || !par || !p_off || !p || !out_lr || !out_ham) return D2G_ERR_ARG;
+  std::vector<uint32_t> o5;
+  for (uint32_t i0 = 0; i0 < n;) {
+    const uint32_t k = bimera_chunk(q_off, p_off, i0, n);
+    std::vector<uint64_t> qo(k + 1), po(k + 1);
+    for (uint32_t t = 0; t <= k; ++t) {
+      qo[t] = q_off[i0 + t];
+      po[t] = p_off[i0 + t];
+    }
+    o5.resize((size_t)5 * k);
+    const int rc = bimera_lr_pairs(c, k, q, qo.data(), par, po.data(), p, o5.data());
+    if (rc != D2G_OK) return rc;
+    for (uint32_t t = 0; t < k; ++t) {
+      for (int f = 0; f < 4; ++f) out_lr[4 * (size_t)(i0 + t) + f] = o5[5 * (size_t)t + f];
+      out_ham[i0 + t] = o5[5 * (size_t)t + 4];
+    }
+    i0 += k;
+  }
+  return D2G_OK;
+}
+
+int d2g_is_bimera_batch(d2g_ctx* c, uint32_t nq, const uint8_t* q, const uint64_t* q_off,
+                        const uint8_t* par, const uint64_t* p_off, const uint32_t* par_start,
+                        const d2g_bimera_params* p, uint8_t* out_flag) {
+  if (!c || !q || !q_off || !par || !p_off || !par_start || !p || !out_flag) return D2G_ERR_ARG;
+  const uint32_t npairs = par_start[nq];
+  // pair t = (query i, parent t) for t in [par_start[i], par_start[i+1])
+  std::vector<uint64_t> qo(npairs + 1), po(npairs + 1);
+  std::vector<uint8_t> qcat;
+  uint64_t tot = 0;
+  for (uint32_t i = 0; i < nq; ++i) {
+    if (par_start[i + 1] < par_start[i]) return D2G_ERR_ARG;
+    tot += (uint64_t)(par_start[i + 1] - par_start[i]) * (q_off[i + 1] - q_off[i]);
+  }
+  qcat.resize(tot + 1);
+  uint64_t w = 0;
+  for (uint32_t i = 0; i < nq; ++i)
+    for (uint32_t t = par_start[i]; t < par_start[i + 1]; ++t) {
+      qo[t] = w;
+      const uint64_t l = q_off[i + 1] - q_off[i];
+      std::memcpy(qcat.data() + w, q + q_off[i], l);
+      w += l;
+    }
+  qo[npairs] = w;
+  for (uint32_t t = 0; t <= npairs; ++t) po[t] = p_off[t];
+  std::vector<uint32_t> lr((size_t)4 * npairs + 4), ham(npairs + 1);
+  int rc = d2g_bimera_pairs(c, npairs, qcat.data(), qo.data(), par, po.data(), p, lr.data(),
+                            ham.data());
+  if (rc != D2G_OK) return rc;
+  // is_bimera_with_buf (chimera.rs:218-292): maxima over the parents; the
+  // early exits of the reference only stop a search already flagged
+  for (uint32_t i = 0; i < nq; ++i) {
+    const uint32_t sqlen = (uint32_t)(q_off[i + 1] - q_off[i]);
+    uint32_t ml = 0, mr = 0, oml = 0, omr = 0, omloo = 0, omroo = 0;
+    bool flag = false;
+    for (uint32_t t = par_start[i]; t < par_start[i + 1] && !flag; ++t) {
+      const uint32_t* e = lr.data() + 4 * (size_t)t;
+      if (e[0] + e[1] >= sqlen) continue;
+      ml = std::max(ml, e[0]);
+      mr = std::max(mr, e[1]);
+      if (p->allow_one_off && ham[t] >= (uint32_t)p->min_one_off_par_dist) {
+        oml = std::max(oml, e[0]);
+        omr = std::max(omr, e[1]);
+        omloo = std::max(omloo, e[2]);
+        omroo = std::max(omroo, e[3]);
+      }
+      if (ml + mr >= sqlen) flag = true;
+      if (p->allow_one_off && (oml + omroo >= sqlen || omloo + omr >= sqlen)) flag = true;
+    }
+    out_flag[i] = flag ? 1 : 0;
+  }
+  return D2G_OK;
+}
+
+int d2g_table_bimera2(d2g_ctx* c, const uint32_t* mat, uint32_t nrow, uint32_t ncol,
+                      const uint8_t* seqs, const uint64_t* off, double min_fold,
+                      uint32_t min_abund, const d2g_bimera_params* p, uint32_t* out_nflag,
+                      uint32_t* out_nsam) {
+  if (!c || !mat || !seqs || !off || !p || !out_nflag || !out_nsam) return D2G_ERR_ARG;
+  // every (j, k) the reference would align: k qualifies as a parent of j in
+  // some sample (chimera.rs:351-374)
+  std::vector<std::vector<uint32_t>> need(ncol);
+  std::vector<uint8_t> mark(ncol);
+  for (uint32_t j = 0; j < ncol; ++j) {
+    std::fill(mark.begin(), mark.end(), 0);
+    for (uint32_t i = 0; i < nrow; ++i) {
+      const uint32_t ja = mat[i + (size_t)j * nrow];
+      if (ja == 0) continue;
+      for (uint32_t k = 0; k < ncol; ++k) {
+        const uint32_t ka = mat[i + (size_t)k * nrow];
+        if ((double)ka <= min_fold * (double)ja || ka < min_abund) continue;
+        if (!mark[k]) {
+          mark[k] = 1;
+          need[j].push_back(k);
+        }
+      }
+    }
+  }
+  std::vector<uint32_t> pj, pk;
+  for (uint32_t j = 0; j < ncol; ++j)
+    for (uint32_t k : need[j]) {
+      pj.push_back(j);
+      pk.push_back(k);
+    }
+  const uint32_t npairs = (uint32_t)pj.size();
+  std::vector<uint64_t> qo(npairs + 1), po(npairs + 1);
+  std::vector<uint8_t> qcat, pcat;
+  for (uint32_t t = 0; t < npairs; ++t) {
+    qo[t] = qcat.size();
+    po[t] = pcat.size();
+    qcat.insert(qcat.end(), seqs + off[pj[t]], seqs + off[pj[t] + 1]);
+    pcat.insert(pcat.end(), seqs + off[pk[t]], seqs + off[pk[t] + 1]);
+  }
+  qo[npairs] = qcat.size();
+  po[npairs] = pcat.size();
+  qcat.push_back(0);
+  pcat.push_back(0);
+  std::vector<uint32_t> lr((size_t)4 * npairs + 4), ham(npairs + 1);
+  int rc = d2g_bimera_pairs(c, npairs, qcat.data(), qo.data(), pcat.data(), po.data(), p,
+                            lr.data(), ham.data());
+  if (rc != D2G_OK) return rc;
+  // per column: the sample loop of chimera.rs:351-431 over the cached pairs
+  std::vector<int64_t> slot(ncol, -1);
+  uint32_t t0 = 0;
+  for (uint32_t j = 0; j < ncol; ++j) {
+    for (uint32_t x = 0; x < need[j].size(); ++x) slot[need[j][x]] = (int64_t)(t0 + x);
+    const uint32_t sqlen = (uint32_t)(off[j + 1] - off[j]);
+    uint32_t nsam = 0, nflag = 0;
+    for (uint32_t i = 0; i < nrow; ++i) {
+      const uint32_t ja = mat[i + (size_t)j * nrow];
+      if (ja == 0) continue;
+      ++nsam;
+      uint32_t ml = 0, mr = 0, oml = 0, omr = 0, omloo = 0, omroo = 0;
+      for (uint32_t k = 0; k < ncol; ++k) {
+        const uint32_t ka = mat[i + (size_t)k * nrow];
+        if ((double)ka <= min_fold * (double)ja || ka < min_abund) continue;
+        const size_t t = (size_t)slot[k];
+        const uint32_t* e = lr.data() + 4 * t;
+        const bool keep = e[0] + e[1] < sqlen;
+        const uint32_t l = keep ? e[0] : 0, r = keep ? e[1] : 0;
+        const uint32_t loo = keep ? e[2] : 0, roo = keep ? e[3] : 0;
+        const bool allowed = p->allow_one_off && ham[t] >= (uint32_t)p->min_one_off_par_dist;
+        ml = std::max(ml, l);
+        mr = std::max(mr, r);
+        if (p->allow_one_off && allowed) {
+          oml = std::max(oml, l);
+          omr = std::max(omr, r);
+          omloo = std::max(omloo, loo);
+          omroo = std::max(omroo, roo);
+        }
+      }
+      if (ml + mr >= sqlen || (p->allow_one_off && (oml + omroo >= sqlen || omloo + omr >= sqlen)))
+        ++nflag;
+    }
+    for (uint32_t k : need[j]) slot[k] = -1;
+    t0 += (uint32_t)need[j].size();
+    out_nflag[j] = nflag;
+    out_nsam[j] = nsam;
+  }
+  return D2G_OK;
 }
 
 int d2g_pairs_load(d2g_ctx* c, uint32_t n, const uint8_t* seq_concat, const uint64_t* offsets,

@@ -144,6 +144,12 @@ struct LambdaArgs {
 };
 hipError_t launch_lambda(const LambdaArgs& a, hipStream_t s);
 
+// ---- bimera.hip --------------------------------------------------------
+// get_lr + get_ham_endsfree of n device-resident alignments (5 words each).
+hipError_t launch_bimera_lr(uint32_t n, const uint8_t* al0, const uint8_t* al1,
+                            const uint64_t* al_off, const uint32_t* al_len, int allow_one_off,
+                            uint32_t max_shift, uint32_t* out, hipStream_t s);
+
 // ---- lambda32.hip ------------------------------------------------------
 // FP32 Kahan log-lambda and FP64 p-values (with FP64 lambda recompute near
 // omega) over a resident set of aligned pairs.

@@ -301,3 +301,47 @@ def workload_precision(n_pairs=1_000_000, L=250, n_centers=2000, seed=SEED_BASE 
     reads = abundance(rng, n_pairs)
     center_reads = np.maximum(1, np.rint(rng.lognormal(6.0, 1.5, size=n_pairs))).astype(np.uint32)
     return PairSet(seqs, quals, subs, reads, center_reads, err)
+
+
+@dataclass
+class BimeraSet:
+    """ASCII queries, each with its list of more-abundant parents."""
+    queries: list[bytes]
+    parent_lists: list[list[bytes]]
+    chimeric: np.ndarray  # construction truth (not the reference's verdict)
+
+
+def workload_bimera(n_pool=200, n_queries=2000, parents_per_query=50, lmin=1420, lmax=1480,
+                    div=0.01, sub=0.001, chimera_frac=0.4, seed=SEED_BASE + 5) -> BimeraSet:
+    """Bimera parent search at the long-read shape: a pool of related parents
+    (an ancestor trimmed to U[lmin, lmax] with `div` substitutions per base);
+    queries are two-parent chimeras at a random breakpoint (chimera_frac) or
+    pool parents with `sub` substitutions; each query gets parents_per_query
+    distinct pool members, including its true parents.  ASCII bytes."""
+    rng = np.random.default_rng(seed)
+    ascii_ = np.frombuffer(b"ACGT", np.uint8)
+    anc = rng.integers(1, 5, size=lmax).astype(np.uint8)
+    pool = []
+    for _ in range(n_pool):
+        L = int(rng.integers(lmin, lmax + 1))
+        pool.append(_substitute(rng, anc[:L], np.full(L, div)))
+    queries, lists, truth = [], [], []
+    for _ in range(n_queries):
+        chim = rng.random() < chimera_frac
+        if chim:
+            a, b = (int(x) for x in rng.choice(n_pool, size=2, replace=False))
+            m = min(len(pool[a]), len(pool[b]))
+            bp = int(rng.integers(int(0.2 * m), int(0.8 * m)))
+            q = np.concatenate([pool[a][:bp], pool[b][bp:]])
+            must = [a, b]
+        else:
+            a = int(rng.integers(0, n_pool))
+            q = _substitute(rng, pool[a], np.full(len(pool[a]), sub))
+            must = [a]
+        others = [int(x) for x in rng.permutation(n_pool) if int(x) not in must]
+        par = must + others[:parents_per_query - len(must)]
+        par = [par[i] for i in rng.permutation(len(par))]
+        queries.append(ascii_[q - 1].tobytes())
+        lists.append([ascii_[pool[k] - 1].tobytes() for k in par])
+        truth.append(chim)
+    return BimeraSet(queries, lists, np.asarray(truth))

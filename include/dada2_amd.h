@@ -174,6 +174,44 @@ int d2g_align_endsfree(d2g_ctx* ctx, uint32_t n, const uint8_t* s1, const uint64
  * and when every unique is longer than 3500 nt (nwalign.rs:1335-1414). */
 int d2g_band_supported(uint32_t len1, uint32_t len2, int32_t band, int homo);
 
+/* ---- bimera parent search -------------------------------------------------
+ * BimeraAlignParams (chimera.rs:183-196) with the NW backend.  Sequences are
+ * ASCII; each (query, parent) pair is aligned by the vectorized aligner with
+ * end_gap 0 and band = max_shift (chimera.rs:234-240, 336-342). */
+typedef struct d2g_bimera_params {
+  int32_t allow_one_off;
+  int32_t min_one_off_par_dist;
+  int16_t match_score;
+  int16_t mismatch;
+  int16_t gap_p;
+  int16_t _pad;
+  int32_t max_shift;
+} d2g_bimera_params;
+
+/* Per pair i (query q[q_off[i]..q_off[i+1]), parent p[...]): get_lr
+ * (chimera.rs:99-175) into out_lr[4i..4i+3] = left, right, left_oo,
+ * right_oo, and get_ham_endsfree (chimera.rs:54-86) into out_ham[i]. */
+int d2g_bimera_pairs(d2g_ctx* ctx, uint32_t n, const uint8_t* q, const uint64_t* q_off,
+                     const uint8_t* p, const uint64_t* p_off, const d2g_bimera_params* params,
+                     uint32_t* out_lr, uint32_t* out_ham);
+
+/* is_bimera_with_buf (chimera.rs:218-292) for nq queries: query i's parents
+ * are p entries par_start[i] .. par_start[i+1]-1 (par_start has nq+1
+ * entries; p_off has par_start[nq]+1).  out_flag[i] = 1 when bimera. */
+int d2g_is_bimera_batch(d2g_ctx* ctx, uint32_t nq, const uint8_t* q, const uint64_t* q_off,
+                        const uint8_t* p, const uint64_t* p_off, const uint32_t* par_start,
+                        const d2g_bimera_params* params, uint8_t* out_flag);
+
+/* table_bimera2 (chimera.rs:317-433): mat is column-major nrow (samples) x
+ * ncol (sequences); seqs holds the ncol sequences with off[ncol+1].  Per
+ * column j: out_nsam[j] samples where j is present, out_nflag[j] of them
+ * where j is a bimera of more abundant parents (> min_fold x and >=
+ * min_abund reads in that sample). */
+int d2g_table_bimera2(d2g_ctx* ctx, const uint32_t* mat, uint32_t nrow, uint32_t ncol,
+                      const uint8_t* seqs, const uint64_t* off, double min_fold,
+                      uint32_t min_abund, const d2g_bimera_params* params,
+                      uint32_t* out_nflag, uint32_t* out_nsam);
+
 /* ---- mixed-precision lambda and abundance p-values ----------------------
  * A resident set of n aligned pairs: query i (bases 1..4, qualities, read
  * count reads[i]) with its al2subs list against its center (records
