@@ -117,6 +117,10 @@ def _lib():
             "d2g_table_bimera2": (C.c_int, [vp, u32p, C.c_uint32, C.c_uint32, u8p, u64p,
                                             C.c_double, C.c_uint32, C.POINTER(BimeraParams),
                                             u32p, u32p]),
+            "d2g_loglik_forward": (C.c_int, [vp, C.POINTER(C.c_float), C.c_uint32, C.c_int,
+                                             f64p, C.POINTER(C.c_float)]),
+            "d2g_loglik_backward": (C.c_int, [vp, C.POINTER(C.c_float), f64p,
+                                              C.POINTER(C.c_float)]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -133,7 +137,8 @@ def exported_symbols() -> list[str]:
             "d2g_compare_batch", "d2g_compare_batch_device", "d2g_fetch_results",
             "d2g_align_vectorized", "d2g_vectorized_cells", "d2g_vectorized_supported",
             "d2g_align_endsfree", "d2g_band_supported", "d2g_pairs_load", "d2g_pairs_lambda",
-            "d2g_pairs_fetch", "d2g_bimera_pairs", "d2g_is_bimera_batch", "d2g_table_bimera2"]
+            "d2g_pairs_fetch", "d2g_bimera_pairs", "d2g_is_bimera_batch", "d2g_table_bimera2",
+            "d2g_loglik_forward", "d2g_loglik_backward"]
 
 
 def _p(a: np.ndarray, t):
@@ -340,6 +345,29 @@ class Context:
         _check(_lib().d2g_pairs_fetch(self.h, ll.ctypes.data_as(C.POINTER(C.c_float)),
                                       _p(lam, f64p), _p(pv, f64p), _p(fl, u8p)), "d2g_pairs_fetch")
         return ll, lam, pv, fl, st.as_dict()
+
+    def loglik_forward(self, logE, use_quals=True) -> float:
+        """Poisson log-likelihood of the abundances over the last compare's
+        aligned pairs for the 16 x ncol FP32 log error matrix logE."""
+        e = np.ascontiguousarray(logE, np.float32)
+        loss = C.c_double()
+        ms = C.c_float()
+        _check(_lib().d2g_loglik_forward(self.h, e.ctypes.data_as(C.POINTER(C.c_float)),
+                                         e.shape[1], int(use_quals), C.byref(loss), C.byref(ms)),
+               "d2g_loglik_forward")
+        self.last_loglik_ms = ms.value
+        self._ll_shape = e.shape
+        return loss.value
+
+    def loglik_backward(self):
+        """Gradient of the last forward's loss in logE: (float32, float64)."""
+        g32 = np.zeros(self._ll_shape, np.float32)
+        g64 = np.zeros(self._ll_shape, np.float64)
+        ms = C.c_float()
+        _check(_lib().d2g_loglik_backward(self.h, g32.ctypes.data_as(C.POINTER(C.c_float)),
+                                          _p(g64, f64p), C.byref(ms)), "d2g_loglik_backward")
+        self.last_loglik_bwd_ms = ms.value
+        return g32, g64
 
     def bimera_pairs(self, queries, parents, params: BimeraParams | None = None):
         """get_lr + get_ham_endsfree per (query, parent) pair: returns

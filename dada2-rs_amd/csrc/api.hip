@@ -96,6 +96,16 @@ struct d2g_ctx {
   DevBuf<uint32_t> p_len, p_subs, p_reads, p_creads, p_nrec;
   DevBuf<float> p_loglam;
   DevBuf<uint32_t> b_out;  // bimera: 5 words per pair
+  // the last compare's aligned pairs (for the log-likelihood and gradient)
+  bool cmp_valid = false;
+  uint32_t cmp_sub_stride = 0;
+  uint64_t cmp_pairs = 0;
+  DevBuf<Counters> last_ctr;
+  DevBuf<double> ll_w, ll_part, ll_out;
+  DevBuf<float> ll_logE;
+  uint32_t ll_ncol = 0;
+  int ll_use_quals = 0;
+  bool ll_have_w = false;
   DevBuf<double> p_lambda, p_pval, p_err;
   // align mode buffers
   DevBuf<uint8_t> a_pool, a_al0, a_al1;
@@ -183,6 +193,9 @@ int d2g_ctx_destroy(d2g_ctx* c) {
   for (auto* b : {&c->p_len, &c->p_subs, &c->p_reads, &c->p_creads, &c->p_nrec}) b->release();
   c->p_loglam.release();
   c->b_out.release();
+  c->last_ctr.release();
+  for (auto* b : {&c->ll_w, &c->ll_part, &c->ll_out}) b->release();
+  c->ll_logE.release();
   for (auto* b : {&c->p_lambda, &c->p_pval, &c->p_err}) b->release();
   for (int i = 0; i < kEvents; ++i)
     if (c->ev[i]) (void)hipEventDestroy(c->ev[i]);
@@ -345,6 +358,7 @@ int d2g_load_uniques(d2g_ctx* c, uint32_t n, const uint8_t* seq_concat, const ui
   c->max_qual = maxq;
   c->has_qual = qual_concat != nullptr;
   c->loaded = true;
+  c->cmp_valid = false;
   return D2G_OK;
 }
 
@@ -565,6 +579,12 @@ static int compare_device(d2g_ctx* c, const uint32_t* centers, uint32_t nc, cons
       continue;
     }
     c->last_nc = nc;
+    // the aligned pairs of this call stay resident for d2g_loglik_*
+    HIP_TRY(c->last_ctr.ensure(1));
+    HIP_TRY(hipMemcpyAsync(c->last_ctr.p, c->ctr.p, sizeof(Counters), hipMemcpyDeviceToDevice, s));
+    c->cmp_valid = true;
+    c->cmp_sub_stride = sub_stride;
+    c->cmp_pairs = (uint64_t)h.n_gapless + h.n_dp;
     if (stats) {
       std::memset(stats, 0, sizeof(*stats));
       stats->comparisons = npairs;
@@ -638,6 +658,7 @@ static int align_pairs_device(d2g_ctx* c, uint32_t n, const uint8_t* s1, const u
                               int homo_gap, const uint64_t* al_offsets, uint64_t* al_total_out) {
   if (!c || !s1 || !s1_off || !s2 || !s2_off) return D2G_ERR_ARG;
   *al_total_out = 0;
+  c->cmp_valid = false;  // the task buffers are reused below
   if (n == 0) return D2G_OK;
   (void)hipSetDevice(c->device);
   // pool: all s1 then all s2
@@ -783,6 +804,72 @@ int d2g_align_endsfree(d2g_ctx* c, uint32_t n, const uint8_t* s1, const uint64_t
 
 int d2g_band_supported(uint32_t len1, uint32_t len2, int32_t band, int homo) {
   return band_groups(band_geom((int)len1, (int)len2, band), homo != 0) > 0 ? 1 : 0;
+}
+
+// ---- log-likelihood and gradient ------------------------------------------
+static int loglik_args(d2g_ctx* c, uint32_t ncol, int use_quals, LoglikArgs* la) {
+  if (!c->cmp_valid) return D2G_ERR_STATE;
+  if (use_quals && c->has_qual && c->max_qual >= ncol) return D2G_ERR_ARG;
+  const uint32_t grid = (uint32_t)c->cu_count * 4;
+  HIP_TRY(c->ll_w.ensure(c->cmp_pairs + 1));
+  HIP_TRY(c->ll_part.ensure((size_t)grid * 16 * ncol));
+  HIP_TRY(c->ll_out.ensure((size_t)16 * ncol + 1));
+  la->u = &c->u;
+  la->gapless_tasks = c->gl.p;
+  la->dp_tasks = c->dp_sorted.p;
+  la->sub_pool = c->sub_pool.p;
+  la->sub_stride = c->cmp_sub_stride;
+  la->sub_cnt = c->sub_cnt.p;
+  la->ncol = ncol;
+  la->use_quals = use_quals;
+  la->ctr = c->last_ctr.p;
+  la->w = c->ll_w.p;
+  la->block_part = c->ll_part.p;
+  la->grid = grid;
+  return D2G_OK;
+}
+
+int d2g_loglik_forward(d2g_ctx* c, const float* logE, uint32_t ncol, int use_quals,
+                       double* out_loss, float* ms) {
+  if (!c || !logE || ncol == 0 || !out_loss) return D2G_ERR_ARG;
+  (void)hipSetDevice(c->device);
+  LoglikArgs la;
+  HIP_TRY_INT(loglik_args(c, ncol, use_quals, &la));
+  hipStream_t s = c->stream;
+  HIP_TRY(c->ll_logE.ensure((size_t)16 * ncol));
+  HIP_TRY(hipMemcpyAsync(c->ll_logE.p, logE, (size_t)16 * ncol * 4, hipMemcpyHostToDevice, s));
+  HIP_TRY(hipEventRecord(c->ev[0], s));
+  HIP_TRY(launch_loglik_fwd(la, c->ll_logE.p, c->ll_out.p, s));
+  HIP_TRY(hipEventRecord(c->ev[1], s));
+  HIP_TRY(hipMemcpyAsync(out_loss, c->ll_out.p, 8, hipMemcpyDeviceToHost, s));
+  HIP_TRY(hipStreamSynchronize(s));
+  if (ms) *ms = ev_ms(c, 0, 1);
+  c->ll_ncol = ncol;
+  c->ll_use_quals = use_quals;
+  c->ll_have_w = true;
+  return D2G_OK;
+}
+
+int d2g_loglik_backward(d2g_ctx* c, float* out_grad, double* out_grad64, float* ms) {
+  if (!c || (!out_grad && !out_grad64)) return D2G_ERR_ARG;
+  if (!c->ll_have_w || !c->cmp_valid) return D2G_ERR_STATE;
+  (void)hipSetDevice(c->device);
+  LoglikArgs la;
+  HIP_TRY_INT(loglik_args(c, c->ll_ncol, c->ll_use_quals, &la));
+  hipStream_t s = c->stream;
+  const size_t m = (size_t)16 * c->ll_ncol;
+  HIP_TRY(hipEventRecord(c->ev[0], s));
+  HIP_TRY(launch_loglik_bwd(la, c->ll_out.p, s));
+  HIP_TRY(hipEventRecord(c->ev[1], s));
+  std::vector<double> g(m);
+  HIP_TRY(hipMemcpyAsync(g.data(), c->ll_out.p, m * 8, hipMemcpyDeviceToHost, s));
+  HIP_TRY(hipStreamSynchronize(s));
+  if (ms) *ms = ev_ms(c, 0, 1);
+  for (size_t i = 0; i < m; ++i) {
+    if (out_grad) out_grad[i] = (float)g[i];
+    if (out_grad64) out_grad64[i] = g[i];
+  }
+  return D2G_OK;
 }
 
 // ---- bimera ---------------------------------------------------------------

@@ -150,6 +150,27 @@ hipError_t launch_bimera_lr(uint32_t n, const uint8_t* al0, const uint8_t* al1,
                             const uint64_t* al_off, const uint32_t* al_len, int allow_one_off,
                             uint32_t max_shift, uint32_t* out, hipStream_t s);
 
+// ---- loglik.hip --------------------------------------------------------
+// Poisson log-likelihood of the abundances over the last compare's aligned
+// pairs (gapless + DP tasks) and its gradient in the 16 x ncol log error
+// matrix.
+struct LoglikArgs {
+  const UniqueDev* u;
+  const GaplessTask* gapless_tasks;
+  const DpTask* dp_tasks;  // sorted, with their substitution lists
+  const uint32_t* sub_pool;
+  uint32_t sub_stride;
+  const uint32_t* sub_cnt;
+  uint32_t ncol;
+  int use_quals;
+  const Counters* ctr;     // n_gapless, n_dp of that compare
+  double* w;               // per task scratch: a - E
+  double* block_part;      // grid * 16 * ncol doubles
+  uint32_t grid;
+};
+hipError_t launch_loglik_fwd(const LoglikArgs& a, const float* logE, double* loss, hipStream_t s);
+hipError_t launch_loglik_bwd(const LoglikArgs& a, double* grad, hipStream_t s);
+
 // ---- lambda32.hip ------------------------------------------------------
 // FP32 Kahan log-lambda and FP64 p-values (with FP64 lambda recompute near
 // omega) over a resident set of aligned pairs.

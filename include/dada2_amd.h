@@ -174,6 +174,24 @@ int d2g_align_endsfree(d2g_ctx* ctx, uint32_t n, const uint8_t* s1, const uint64
  * and when every unique is longer than 3500 nt (nwalign.rs:1335-1414). */
 int d2g_band_supported(uint32_t len1, uint32_t len2, int32_t band, int homo);
 
+/* ---- differentiable likelihood ---------------------------------------------
+ * Over the aligned pairs of the last d2g_compare_* call (gapless and DP
+ * pairs; skipped and shrouded pairs take no part), with logE a 16 x ncol
+ * row-major FP32 log error matrix (row t = 4*ref + query, column = quality):
+ *   log lambda_j = sum over query positions of logE[t][q] with the
+ *                  transitions of compute_lambda (pval.rs:294-309),
+ *   E_j = reads[center] * lambda_j,
+ *   loss = sum_j reads_j log E_j - E_j - lgamma(reads_j + 1)
+ * (the Poisson log-likelihood of the abundances).  d2g_loglik_backward
+ * returns d loss / d logE[t][q] = sum_j (reads_j - E_j) count_j[t][q] for the
+ * logE of the last forward; the counts are recomputed from the alignments.
+ * Sums are FP64.  No counterpart in the reference, which fits the error
+ * model outside this path (learn_errors.rs); ms (nullable) receives the
+ * kernel time. */
+int d2g_loglik_forward(d2g_ctx* ctx, const float* logE, uint32_t ncol, int use_quals,
+                       double* out_loss, float* ms);
+int d2g_loglik_backward(d2g_ctx* ctx, float* out_grad, double* out_grad64, float* ms);
+
 /* ---- bimera parent search -------------------------------------------------
  * BimeraAlignParams (chimera.rs:183-196) with the NW backend.  Sequences are
  * ASCII; each (query, parent) pair is aligned by the vectorized aligner with
