@@ -41,6 +41,11 @@ def parse():
     ap.add_argument("--cpu-seconds", type=float, default=10.0,
                     help="minimum CPU time of the baseline sample (whole passes)")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--workload", default="batched",
+                    choices=["batched", "plain", "precision", "gradient", "longread"],
+                    help="batched: the headline line (default); the others measure the "
+                         "plain single-center compare, the mixed-precision lambda, the "
+                         "log-likelihood gradient and the long-read compare + bimera search")
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
                     help="after the run, one device-to-device copy of this size (a plain copy "
                          "kernel of known bytes that scales the HBM byte counters)")
@@ -106,6 +111,225 @@ def traffic_from_profile(kernel: str):
     return None
 
 
+def _line(metric, value, unit, args, ms, dtype, config, **kw):
+    d = {"metric": metric, "value": value, "unit": unit, "n_gpus": 1, "steps": args.steps,
+         "warmup": args.warmup, "ms_per_step": ms, "higher_is_better": True, "scaling": "weak",
+         "vs_baseline": None, "dtype": dtype, "data": "synthetic", "config": config}
+    d.update(kw)
+    return d
+
+
+def _timed_cpu(fn, min_seconds: float):
+    """Whole calls of fn() until min_seconds have passed: (calls, seconds)."""
+    fn()
+    n, t0 = 0, time.perf_counter()
+    while True:
+        fn()
+        n += 1
+        dt = time.perf_counter() - t0
+        if dt >= min_seconds:
+            return n, dt
+
+
+def bench_plain(args, pkg):
+    """The plain single-center compare: 20 variants x 250 nt, 2,000 uniques,
+    the most abundant unique as center; one call per step with the results
+    copied to the host (b_compare_parallel's map)."""
+    from dada2_rs_amd import synth
+    w = synth.workload_plain()
+    ctx = pkg.Context(0, timing=True)
+    ctx.load_uniques(w.seqs, w.quals, w.reads, k=w.kmer_size)
+    c = int(w.centers[0])
+    for _ in range(args.warmup):
+        ctx.compare_center(c, w.err)
+    t0 = time.perf_counter()
+    st = [ctx.compare_center(c, w.err).stats for _ in range(args.steps)]
+    wall = time.perf_counter() - t0
+    ctx.close()
+    cpu = None
+    if not args.no_cpu:
+        import oracle as O
+        rs = O.RawSet(w.seqs, w.quals, w.reads, k=w.kmer_size)
+        p = O.make_params(band=w.band)
+        th = O.cpu_count_allowed()
+        calls, dt = _timed_cpu(lambda: O.compare_center(rs, c, w.err, 41, p, nthreads=th),
+                               args.cpu_seconds)
+        cpu = {"value": calls * w.n / dt, "unit": "comparisons/s", "cores": th, "kind": "port",
+               "sample": f"{calls} calls of the full step ({w.n} comparisons each), {dt:.1f} s"}
+    ms = wall / args.steps * 1e3
+    return _line("center-unique comparisons/sec, plain single-center compare", w.n / (ms * 1e-3),
+                 "comparisons/s", args, ms, "i16 DP / f64 lambda",
+                 {"workload": "plain: 1 center x 2000 uniques, 250 nt, band 16, (5,-4,-8), "
+                              "k=5 cutoff 0.42, FP64 lambda, results to host"},
+                 cpu_baseline=cpu,
+                 kernel_ms={k: float(np.mean([x[k] for x in st]))
+                            for k in ("ms_screen", "ms_plan", "ms_dp", "ms_lambda", "ms_total")},
+                 counts={k: st[-1][k] for k in ("comparisons", "shrouded", "gapless", "aligned_dp")})
+
+
+def bench_precision(args, pkg):
+    """1e6 aligned pairs at 250 nt: FP32 Kahan log-lambda + FP64 p-values
+    (FP64 lambda recompute within 10x of omega_a = 1e-40), against the FP64
+    oracle (ordered product, get_pA)."""
+    import oracle as O
+    from dada2_rs_amd import synth
+    ps = synth.workload_precision()
+    ctx = pkg.Context(0, timing=True)
+    ctx.pairs_load(ps.seqs, ps.quals, ps.subs, ps.reads, ps.center_reads)
+    for _ in range(args.warmup):
+        ctx.pairs_lambda(ps.err, fetch=False)
+    t0 = time.perf_counter()
+    st = [ctx.pairs_lambda(ps.err, fetch=False) for _ in range(args.steps)]
+    wall = time.perf_counter() - t0
+    ll, lam, pv, fl, last = ctx.pairs_lambda(ps.err)
+    ctx.close()
+    t1 = time.perf_counter()
+    exact = O.pairs_lambda_fp64(ps.seqs, ps.quals, ps.subs, ps.err)
+    ham = np.array([len(x) for x in ps.subs])
+    want = np.array([O.get_pA(int(r), float(l), int(h), int(c))
+                     for r, l, h, c in zip(ps.reads, exact, ham, ps.center_reads)])
+    cpu_s = time.perf_counter() - t1
+    ref = np.log(exact)
+    rel = np.abs(ll.astype(np.float64) - ref) / np.abs(ref)
+    omega = 1e-40
+    flips = int(np.count_nonzero((pv < omega) != (want < omega)))
+    ms = wall / args.steps * 1e3
+    L = 250
+    hbm = ps.n * (2 * L + 4 * float(ham.mean()) + 8 + 8 + 4 + 8 + 8 + 1)  # in + out bytes
+    kms = float(np.mean([x["ms_total"] for x in st]))
+    return _line("aligned pairs/sec, FP32 Kahan log-lambda + FP64 p-values", ps.n / (ms * 1e-3),
+                 "pairs/s", args, ms, "f32 log-lambda / f64 p-values",
+                 {"workload": "precision: 1e6 aligned pairs, 250 nt, seeded 16x41 error matrix, "
+                              "omega_a 1e-40"},
+                 loglam_rel_err={"max": float(rel.max()), "mean": float(rel.mean())},
+                 asv_call_flips=flips, recomputed=int(fl.sum()),
+                 kernel_ms={"loglam32": float(np.mean([x["ms_loglam"] for x in st])),
+                            "pval": float(np.mean([x["ms_pval"] for x in st])), "total": kms},
+                 roofline={"bound": "hbm", "achieved": hbm / (kms * 1e-3) / 1e9,
+                           "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                           "frac": hbm / (kms * 1e-3) / 1e9 / HBM_PEAK_GBS, "traffic": None,
+                           "kernel": "loglam32_kernel + pval_kernel",
+                           "algorithmic_bytes_per_launch": hbm},
+                 cpu_baseline={"value": ps.n / cpu_s, "unit": "pairs/s", "cores": 1,
+                               "kind": "port",
+                               "sample": f"all {ps.n} pairs: numpy FP64 ordered product + C "
+                                         f"get_pA per pair, {cpu_s:.1f} s"})
+
+
+def bench_gradient(args, pkg):
+    """50,000 uniques x 500 centers at 250 nt: one batched compare, then the
+    Poisson log-likelihood forward and its backward kernel over the aligned
+    pairs (16 x 41 FP32 log error matrix)."""
+    from dada2_rs_amd import synth
+    w = synth.workload_batched(n_variants=500, L=250, n_uniques=50000, lmin=250, lmax=250,
+                               seed=synth.SEED_BASE + 3)
+    ctx = pkg.Context(0, timing=True)
+    ctx.load_uniques(w.seqs, w.quals, w.reads, k=w.kmer_size)
+    centers = np.asarray(w.centers, np.uint32)
+    cst = ctx.compare_batch(centers, w.err, device_only=True)
+    logE = np.log(w.err).astype(np.float32)
+    for _ in range(args.warmup):
+        ctx.loglik_forward(logE)
+        ctx.loglik_backward()
+    fwd, bwd = [], []
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        ctx.loglik_forward(logE)
+        fwd.append(ctx.last_loglik_ms)
+        ctx.loglik_backward()
+        bwd.append(ctx.last_loglik_bwd_ms)
+    wall = time.perf_counter() - t0
+    # finite differences on the 4 largest gradient entries (h = 1e-3)
+    _, g = ctx.loglik_backward()
+    fd_err = 0.0
+    for k in np.argsort(-np.abs(g).ravel())[:4]:
+        t, q = divmod(int(k), logE.shape[1])
+        lp, lm = logE.copy(), logE.copy()
+        lp[t, q] += np.float32(1e-3)
+        lm[t, q] -= np.float32(1e-3)
+        h = (float(lp[t, q]) - float(lm[t, q])) / 2
+        fd = (ctx.loglik_forward(lp) - ctx.loglik_forward(lm)) / (2 * h)
+        fd_err = max(fd_err, abs(fd - g[t, q]) / abs(g[t, q]))
+    ctx.close()
+    npairs = cst["gapless"] + cst["aligned_dp"]
+    ms = wall / args.steps * 1e3
+    kms = float(np.mean(fwd)) + float(np.mean(bwd))
+    hbm = npairs * 2 * 250.0 * 2  # seq + qual per pair, read by forward and backward
+    return _line("aligned pairs/sec, log-likelihood forward + backward", npairs / (ms * 1e-3),
+                 "pairs/s", args, ms, "f32 parameter / f64 sums",
+                 {"workload": "gradient: 500 centers x 50000 uniques, 250 nt, 16x41 log error "
+                              "matrix", "aligned_pairs": int(npairs),
+                  "comparisons": int(cst["comparisons"])},
+                 kernel_ms={"forward": float(np.mean(fwd)), "backward": float(np.mean(bwd))},
+                 fd_rel_err_top4=fd_err,
+                 roofline={"bound": "hbm", "achieved": hbm / (kms * 1e-3) / 1e9,
+                           "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                           "frac": hbm / (kms * 1e-3) / 1e9 / HBM_PEAK_GBS, "traffic": None,
+                           "kernel": "loglik_fwd_kernel + loglik_bwd_kernel",
+                           "algorithmic_bytes_per_launch": hbm},
+                 cpu_baseline=None)
+
+
+def bench_longread(args, pkg):
+    """Long reads: 100 centers x 5,000 uniques, 1,420-1,480 nt, band 32,
+    homopolymer gap -1 (the i32 row-band aligner), then the bimera parent
+    search: 2,000 queries x 50 parents through the vectorized aligner."""
+    from dada2_rs_amd import synth
+    w = synth.workload_longread()
+    params = pkg.align_params(band=32, homo_gap_p=-1)
+    ctx = pkg.Context(0, timing=True)
+    info = ctx.device_info()
+    ctx.load_uniques(w.seqs, w.quals, w.reads, k=w.kmer_size)
+    centers = np.asarray(w.centers, np.uint32)
+    for _ in range(args.warmup):
+        ctx.compare_batch(centers, w.err, params, device_only=True)
+    t0 = time.perf_counter()
+    st = [ctx.compare_batch(centers, w.err, params, device_only=True) for _ in range(args.steps)]
+    wall = time.perf_counter() - t0
+    bs = synth.workload_bimera()
+    ctx.is_bimera_batch(bs.queries[:50], bs.parent_lists[:50])
+    t1 = time.perf_counter()
+    flags = ctx.is_bimera_batch(bs.queries, bs.parent_lists)
+    bim_s = time.perf_counter() - t1
+    ctx.close()
+    n_pairs = len(centers) * w.n
+    ms = wall / args.steps * 1e3
+    ms_dp = float(np.mean([x["ms_dp"] for x in st]))
+    gcups = st[-1]["dp_cells"] / (ms_dp * 1e-3) / 1e9
+    cu, clk = info["cu_count"], info["clock_khz"] * 1e3
+    peak_cells = cu * 64 * clk / 8.0
+    cpu = None
+    if not args.no_cpu:
+        import oracle as O
+        rs = O.RawSet(w.seqs, w.quals, w.reads, k=w.kmer_size)
+        p = O.make_params(band=32, homo_gap_p=-1)
+        th = O.cpu_count_allowed()
+        cs = [int(c) for c in centers[:2]]
+        calls, dt = _timed_cpu(lambda: [O.compare_center(rs, c, w.err, w.err.shape[1], p,
+                                                         nthreads=th) for c in cs],
+                               args.cpu_seconds)
+        cpu = {"value": calls * len(cs) * w.n / dt, "unit": "comparisons/s", "cores": th,
+               "kind": "port", "sample": f"{calls} x {len(cs)} centers x {w.n} uniques, {dt:.1f} s"}
+    return _line("center-unique comparisons/sec, long-read compare (homopolymer i32 DP)",
+                 n_pairs / (ms * 1e-3), "comparisons/s", args, ms, "i32 DP / f64 lambda",
+                 {"workload": "longread: 100 centers x 5000 uniques, 1420-1480 nt, band 32, "
+                              "homo_gap -1, (5,-4,-8), Phred 0-93"},
+                 gcups=gcups,
+                 roofline={"bound": "valu", "achieved": gcups * 8 / 1e3,
+                           "peak": peak_cells * 8 / 1e12, "unit": "TOP/s",
+                           "frac": gcups * 1e9 / peak_cells, "traffic": None,
+                           "kernel": "nw_band_kernel", "cells_per_launch": st[-1]["dp_cells"],
+                           "avg_ms": ms_dp},
+                 kernel_ms={k: float(np.mean([x[k] for x in st]))
+                            for k in ("ms_screen", "ms_plan", "ms_dp", "ms_lambda")},
+                 counts={k: st[-1][k] for k in ("comparisons", "skipped", "shrouded", "gapless",
+                                                "aligned_dp", "dp_cells")},
+                 bimera={"queries": len(bs.queries), "parents_per_query": 50,
+                         "alignments_per_s": len(bs.queries) * 50 / bim_s,
+                         "seconds": bim_s, "flagged": int(flags.sum())},
+                 cpu_baseline=cpu)
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -114,6 +338,9 @@ def main():
               file=sys.stderr)
         sys.exit(2)
     pkg = __graft_entry__.load_package()
+    if args.workload != "batched":
+        print(json.dumps(globals()["bench_" + args.workload](args, pkg)))
+        return
     from dada2_rs_amd import synth
 
     w = synth.workload_batched()

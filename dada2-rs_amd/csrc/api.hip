@@ -392,7 +392,15 @@ static int prepare_schedule(d2g_ctx* c, const BinSpace& bs, int end_gap, int gap
   if (c->lds_bytes > 0) per_cu = std::min<int>(per_cu, (int)((160 * 1024) / c->lds_bytes));
   per_cu = std::max(per_cu, 1);
   c->nw_grid = (uint32_t)c->cu_count * (uint32_t)per_cu;
-  c->hist_stride = nw_hist_p_words(2 * lmax + 1);
+  // history words per slot: K per 16 anti-diagonals for the row-band kernel
+  int kmax = 1;
+  if (!vec)
+    for (int b = 0; b < bs.nbins; ++b) {
+      int l1, l2;
+      bin_lengths(bs, (uint32_t)b, &l1, &l2);
+      kmax = std::max(kmax, band_geom(l1, l2, bs.band).k);
+    }
+  c->hist_stride = nw_hist_p_words(2 * lmax + 1) * (size_t)kmax;
   while (c->nw_grid > (uint32_t)c->cu_count && (size_t)c->nw_grid * c->hist_stride * 4 > ((size_t)1 << 30))
     c->nw_grid /= 2;
   if (c->hist.ensure((size_t)c->nw_grid * c->hist_stride) != hipSuccess) return D2G_ERR_HIP;

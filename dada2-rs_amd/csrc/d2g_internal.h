@@ -109,10 +109,13 @@ D2G_HD int nw_groups(const VecGeom& g) {
 // nwalign.rs:276-284), i.e. diagonals d = j - i in [-lband, rband], with
 // -9999 sentinels on d = -lband-1 and d = rband+1 (fill_band_sentinels,
 // nwalign.rs:287-308).  Diagonal index D = d + lband + 1 runs over
-// [0, dmax]; lane c holds D = 2c or 2c+1 (whichever matches the parity of
-// the anti-diagonal i + j).
+// [0, dmax].  Lane c holds the 2k diagonals D in [2kc, 2kc + 2k): on each
+// anti-diagonal the k of them whose parity matches it (k = 1, 2 or 4,
+// the smallest that fits one wavefront).
+constexpr int kBandMaxK = 4;
+
 struct BandGeom {
-  int len1, len2, lband, rband, dmax, lanes, nrow;
+  int len1, len2, lband, rband, dmax, lanes, nrow, k;
   bool sentinels;
 };
 
@@ -130,7 +133,9 @@ D2G_HD BandGeom band_geom(int l1, int l2, int band) {
     g.sentinels = band < l1 || band < l2;
   }
   g.dmax = g.lband + g.rband + 2;
-  g.lanes = g.dmax / 2 + 1;
+  g.k = 1;
+  while (g.k < kBandMaxK && (g.dmax + 1 + 2 * g.k - 1) / (2 * g.k) > kWave) g.k *= 2;
+  g.lanes = (g.dmax + 1 + 2 * g.k - 1) / (2 * g.k);
   g.nrow = l1 + l2 + 1;
   return g;
 }

@@ -22,13 +22,15 @@
 // diagonals d = j - i run over [-lband-1, rband+1] (sentinels included),
 // indexed D = d + lband + 1 in [0, dmax].  On anti-diagonal t only diagonals
 // with D = t + lband + 1 (mod 2) hold cells, so lane c of a pair's group
-// owns D = 2c or 2c+1 — one cell per lane per step.  The up neighbour (D+1)
-// and left neighbour (D-1) of the previous step are the lane itself or its
-// neighbour (DPP shift); the diagonal neighbour is the lane's own value two
-// steps back.  Every lane classifies its cell (interior, row 0 / column 0,
+// owns the 2K diagonals D in [2Kc, 2Kc + 2K) and computes K cells per step
+// (K = 1, 2 or 4: the smallest that fits the band into one wavefront).  The
+// up neighbour (D+1) and left neighbour (D-1) of the previous step are the
+// lane's own cells except at the lane's ends, where one DPP shift brings the
+// neighbour lane's; the diagonal neighbour is the cell's own value two steps
+// back.  Every lane classifies its cells (interior, row 0 / column 0,
 // sentinel, outside the matrix) from (i, j).
 //
-// Pointers go to a per-workgroup history slot (2 bits per cell, a word per
+// Pointers go to a per-workgroup history slot (2 bits per cell, K words per
 // 16 steps); a diagonal step over a mismatch is stored as 0 and told apart
 // from an unset pointer by the cell's class.  The traceback sweeps the
 // steps downwards, reading the path cell's word from its owner lane with
@@ -90,76 +92,101 @@ struct BandK {
   int len1, len2, lband, dmax;
   int match, mismatch, gap, homo;
 };
-// Per-lane state: the lane's two diagonals (d for D = 2c and 2c+1), their
-// class flags, and the scores of the last two steps.
+// Per-lane state for K cells per step: cell m of the lane is diagonal
+// D = 2Kc + 2m + parity; bit m of the masks flags D <= dmax (ok) and the
+// sentinel diagonals (sent); prev / prev2 hold the scores of the last two
+// steps.
+template <int K>
 struct BandLane {
-  int dE, dO;
-  bool sentE, sentO, okE, okO;
-  int prev2, prev;
+  int d0;  // d of D = 2Kc
+  uint32_t okE, okO, sentE, sentO;
+  int prev2[K], prev[K];
 };
 
 // One anti-diagonal t for every pair of the wavefront; PAR = parity of the
-// lane's diagonal D on this step.  Returns the cell's pointer code.
-template <bool HOMO, int PAR>
-__device__ __forceinline__ uint32_t band_step(const int t, BandLane& L, const BandK K,
-                                              const uint8_t* s1, const uint8_t* s2,
-                                              const uint8_t* h1, const uint8_t* h2) {
-  const int d = PAR ? L.dO : L.dE;
-  const int i = (t - d) >> 1;  // t - d is even on this step
-  const int j = i + d;
-  const bool inm = (PAR ? L.okO : L.okE) && (unsigned)i <= (unsigned)K.len1 &&
-                   (unsigned)j <= (unsigned)K.len2;
-  const int ii = min(max(i - 1, 0), K.len1 - 1), jj = min(max(j - 1, 0), K.len2 - 1);
-  const int ca = s1[ii], cb = s2[jj];
-  const bool differ = ca != cb;
-  const int diag = L.prev2 + (differ ? K.mismatch : K.match);
-  const int lraw = PAR ? L.prev : from_lower(L.prev, 0);
-  const int uraw = PAR ? from_upper(L.prev, 0) : L.prev;
-  int gl = K.gap, gu = K.gap;
-  if (HOMO) {
-    gl = h2[jj] ? K.homo : K.gap;
-    gu = h1[ii] ? K.homo : K.gap;
+// lane's diagonals on this step.  Writes the K cells' pointer codes.
+template <bool HOMO, int PAR, int K>
+__device__ __forceinline__ void band_step(const int t, BandLane<K>& L, const BandK Kc,
+                                          const uint8_t* s1, const uint8_t* s2, const uint8_t* h1,
+                                          const uint8_t* h2, uint32_t* pout) {
+  // the one neighbour value from another lane: D+1 of cell K-1 (odd steps)
+  // or D-1 of cell 0 (even steps)
+  const int ext = PAR ? from_upper(L.prev[0], 0) : from_lower(L.prev[K - 1], 0);
+  int nv[K];
+#pragma unroll
+  for (int m = 0; m < K; ++m) {
+    const int d = L.d0 + 2 * m + PAR;
+    const int i = (t - d) >> 1;  // t - d is even on this step
+    const int j = i + d;
+    const bool ok = (((PAR ? L.okO : L.okE) >> m) & 1u) != 0;
+    const bool inm = ok && (unsigned)i <= (unsigned)Kc.len1 && (unsigned)j <= (unsigned)Kc.len2;
+    const int ii = min(max(i - 1, 0), Kc.len1 - 1), jj = min(max(j - 1, 0), Kc.len2 - 1);
+    const int ca = s1[ii], cb = s2[jj];
+    const bool differ = ca != cb;
+    const int diag = L.prev2[m] + (differ ? Kc.mismatch : Kc.match);
+    const int lraw = PAR ? L.prev[m] : (m >= 1 ? L.prev[m > 0 ? m - 1 : 0] : ext);
+    const int uraw = PAR ? (m + 1 < K ? L.prev[m + 1 < K ? m + 1 : 0] : ext) : L.prev[m];
+    int gl = Kc.gap, gu = Kc.gap;
+    if (HOMO) {
+      gl = h2[jj] ? Kc.homo : Kc.gap;
+      gu = h1[ii] ? Kc.homo : Kc.gap;
+    }
+    gl = (i == Kc.len1) ? 0 : gl;  // free end gaps on the last row / column
+    gu = (j == Kc.len2) ? 0 : gu;
+    const int left = lraw + gl, up = uraw + gu;
+    const bool upw = up >= diag && up >= left;
+    const bool lw = left >= diag;
+    int e = upw ? up : (lw ? left : diag);
+    int p = upw ? 3 : (lw ? 2 : (differ ? 0 : 1));
+    const bool sent = (((PAR ? L.sentO : L.sentE) >> m) & 1u) != 0;
+    const bool interior = inm && i >= 1 && j >= 1 && !sent;
+    const bool edge = inm && (i == 0 || j == 0);
+    e = interior ? e : (sent ? kBandSentinel : 0);
+    p = interior ? p : (edge ? (i == 0 ? 2 : 3) : 0);
+    nv[m] = e;
+    pout[m] = (uint32_t)p;
   }
-  gl = (i == K.len1) ? 0 : gl;  // free end gaps on the last row / column
-  gu = (j == K.len2) ? 0 : gu;
-  const int left = lraw + gl, up = uraw + gu;
-  const bool upw = up >= diag && up >= left;
-  const bool lw = left >= diag;
-  int e = upw ? up : (lw ? left : diag);
-  int p = upw ? 3 : (lw ? 2 : (differ ? 0 : 1));
-  const bool sent = PAR ? L.sentO : L.sentE;
-  const bool interior = inm && i >= 1 && j >= 1 && !sent;
-  const bool edge = inm && (i == 0 || j == 0);
-  e = interior ? e : (sent ? kBandSentinel : 0);
-  p = interior ? p : (edge ? (i == 0 ? 2 : 3) : 0);
-  L.prev2 = L.prev;
-  L.prev = e;
-  return (uint32_t)p;
+#pragma unroll
+  for (int m = 0; m < K; ++m) {
+    L.prev2[m] = L.prev[m];
+    L.prev[m] = nv[m];
+  }
 }
 
 // Steps [16 blk, 16 blk + 16) ∩ [0, last]; P0 = parity of D at even steps.
-template <bool HOMO, int P0>
-__device__ __forceinline__ void band_block(const int blk, const int last, BandLane& L,
-                                           const BandK K, const uint8_t* s1, const uint8_t* s2,
+// History word m of the block holds cell m's pointers.
+template <bool HOMO, int P0, int K>
+__device__ __forceinline__ void band_block(const int blk, const int last, BandLane<K>& L,
+                                           const BandK Kc, const uint8_t* s1, const uint8_t* s2,
                                            const uint8_t* h1, const uint8_t* h2,
                                            uint32_t* hist_p, const int lane) {
   const int t0 = blk * 16;
-  uint32_t hp = 0;
+  uint32_t hp[K];
+#pragma unroll
+  for (int m = 0; m < K; ++m) hp[m] = 0;
+  uint32_t p[K];
   if (t0 + 15 <= last) {
 #pragma unroll
     for (int k = 0; k < 16; ++k) {
-      const uint32_t p = (k & 1) ? band_step<HOMO, 1 - P0>(t0 + k, L, K, s1, s2, h1, h2)
-                                 : band_step<HOMO, P0>(t0 + k, L, K, s1, s2, h1, h2);
-      hp |= p << (2 * k);
+      if (k & 1)
+        band_step<HOMO, 1 - P0, K>(t0 + k, L, Kc, s1, s2, h1, h2, p);
+      else
+        band_step<HOMO, P0, K>(t0 + k, L, Kc, s1, s2, h1, h2, p);
+#pragma unroll
+      for (int m = 0; m < K; ++m) hp[m] |= p[m] << (2 * k);
     }
   } else {
     for (int k = 0; t0 + k <= last; ++k) {
-      const uint32_t p = (k & 1) ? band_step<HOMO, 1 - P0>(t0 + k, L, K, s1, s2, h1, h2)
-                                 : band_step<HOMO, P0>(t0 + k, L, K, s1, s2, h1, h2);
-      hp |= p << (2 * k);
+      if (k & 1)
+        band_step<HOMO, 1 - P0, K>(t0 + k, L, Kc, s1, s2, h1, h2, p);
+      else
+        band_step<HOMO, P0, K>(t0 + k, L, Kc, s1, s2, h1, h2, p);
+#pragma unroll
+      for (int m = 0; m < K; ++m) hp[m] |= p[m] << (2 * k);
     }
   }
-  hist_p[blk * 64 + lane] = hp;
+#pragma unroll
+  for (int m = 0; m < K; ++m) hist_p[(blk * K + m) * 64 + lane] = hp[m];
 }
 
 // Per-wave constants of the traceback.
@@ -180,11 +207,17 @@ struct BandPath {
 
 // One step t of the traceback (traceback_into, nwalign.rs:234-273) for
 // every pair of the wavefront.
-template <bool EMIT>
-__device__ __forceinline__ void band_bw_step(const int t, const int k, const uint32_t hp,
+template <bool EMIT, int K>
+__device__ __forceinline__ void band_bw_step(const int t, const int k, const uint32_t* hp,
                                              BandPath& ps, const BandBw& B, const BandDev& a) {
-  const int owner = ps.D >> 1;
-  const uint32_t w = (uint32_t)__builtin_amdgcn_ds_bpermute((B.gpos + owner) * 4, (int)hp);
+  const int owner = (int)((unsigned)ps.D / (unsigned)(2 * K));
+  const int mi = (ps.D - owner * 2 * K) >> 1;
+  uint32_t w = 0;
+#pragma unroll
+  for (int m = 0; m < K; ++m) {
+    const uint32_t x = (uint32_t)__builtin_amdgcn_ds_bpermute((B.gpos + owner) * 4, (int)hp[m]);
+    w = mi == m ? x : w;
+  }
   int pp = (int)((w >> (2 * k)) & 3u);
   const bool on = ps.rc == t;
   const int d = ps.D - B.lband - 1;
@@ -213,29 +246,135 @@ __device__ __forceinline__ void band_bw_step(const int t, const int k, const uin
   ps.rc = on ? (pp == 1 ? t - 2 : (pp == 0 ? -1 : t - 1)) : ps.rc;
 }
 
-template <bool EMIT>
+template <bool EMIT, int K>
 __device__ __forceinline__ void band_backward(const int last, const uint32_t* hist_p,
                                               const int lane, BandPath& ps, const BandBw& B,
                                               const BandDev& a) {
   const int top = last >> 4;
-  uint32_t hp = hist_p[top * 64 + lane];
-  uint32_t hp_next = top >= 1 ? hist_p[(top - 1) * 64 + lane] : 0u;
-  const int tlo = top * 16 > 1 ? top * 16 : 1;
-  for (int t = last; t >= tlo; --t) band_bw_step<EMIT>(t, t & 15, hp, ps, B, a);
-  for (int b = top - 1; b >= 1; --b) {
-    hp = hp_next;
-    hp_next = hist_p[(b - 1) * 64 + lane];
+  uint32_t hp[K], hn[K];
 #pragma unroll
-    for (int k = 15; k >= 0; --k) band_bw_step<EMIT>(b * 16 + k, k, hp, ps, B, a);
+  for (int m = 0; m < K; ++m) {
+    hp[m] = hist_p[(top * K + m) * 64 + lane];
+    hn[m] = top >= 1 ? hist_p[((top - 1) * K + m) * 64 + lane] : 0u;
+  }
+  const int tlo = top * 16 > 1 ? top * 16 : 1;
+  for (int t = last; t >= tlo; --t) band_bw_step<EMIT, K>(t, t & 15, hp, ps, B, a);
+  for (int b = top - 1; b >= 1; --b) {
+#pragma unroll
+    for (int m = 0; m < K; ++m) {
+      hp[m] = hn[m];
+      hn[m] = hist_p[((b - 1) * K + m) * 64 + lane];
+    }
+#pragma unroll
+    for (int k = 15; k >= 0; --k) band_bw_step<EMIT, K>(b * 16 + k, k, hp, ps, B, a);
   }
   if (top >= 1) {
-    hp = hp_next;
-    for (int t = 15; t >= 1; --t) band_bw_step<EMIT>(t, t, hp, ps, B, a);
+#pragma unroll
+    for (int m = 0; m < K; ++m) hp[m] = hn[m];
+    for (int t = 15; t >= 1; --t) band_bw_step<EMIT, K>(t, t, hp, ps, B, a);
   }
 }
 
+// Forward, traceback and outputs of one wavefront whose bin has K cells
+// per lane.
+template <bool HOMO, int K>
+__device__ __forceinline__ void band_pairs(const BandDev& a, const BandGeom& geo, const int c,
+                                           const int lane, const int g_raw, const bool real,
+                                           const DpTask& task, const uint32_t ti,
+                                           const uint8_t* s1, const uint8_t* s2,
+                                           const uint8_t* h1, const uint8_t* h2,
+                                           uint32_t* hist_p) {
+  const int len1 = geo.len1, len2 = geo.len2, last = len1 + len2, lanes = geo.lanes;
+  BandK Kc;
+  Kc.len1 = len1;
+  Kc.len2 = len2;
+  Kc.lband = geo.lband;
+  Kc.dmax = geo.dmax;
+  Kc.match = a.match;
+  Kc.mismatch = a.mismatch;
+  Kc.gap = a.gap;
+  Kc.homo = a.homo_gap;
+  BandLane<K> L;
+  L.d0 = 2 * K * c - geo.lband - 1;
+  L.okE = L.okO = L.sentE = L.sentO = 0;
+#pragma unroll
+  for (int m = 0; m < K; ++m) {
+    const int De = 2 * K * c + 2 * m, Do = De + 1;
+    L.okE |= (De <= geo.dmax ? 1u : 0u) << m;
+    L.okO |= (Do <= geo.dmax ? 1u : 0u) << m;
+    L.sentE |= (geo.sentinels && (De == 0 || De == geo.dmax) ? 1u : 0u) << m;
+    L.sentO |= (geo.sentinels && Do == geo.dmax ? 1u : 0u) << m;
+    L.prev2[m] = 0;
+    L.prev[m] = 0;
+  }
+
+  // ---- forward: anti-diagonals 0..last ----
+  const int P0 = (geo.lband + 1) & 1;  // parity of D on even steps
+  for (int blk = 0; blk * 16 <= last; ++blk) {
+    if (P0)
+      band_block<HOMO, 1, K>(blk, last, L, Kc, s1, s2, h1, h2, hist_p, lane);
+    else
+      band_block<HOMO, 0, K>(blk, last, L, Kc, s1, s2, h1, h2, hist_p, lane);
+  }
+  __syncthreads();
+
+  // ---- traceback ----
+  const size_t sub0 = (size_t)ti * a.sub_stride;
+  const bool sub_ok = a.emit || sub0 + a.sub_stride <= a.sub_capacity;
+  if (!a.emit && __builtin_amdgcn_ballot_w64(!sub_ok && real) && lane == 0)
+    atomicOr(&a.ctr->err_flags, kErrSubOverflow);
+  BandBw B;
+  B.gpos = g_raw * lanes;
+  B.c = c;
+  B.lband = geo.lband;
+  B.len1 = len1;
+  B.len2 = len2;
+  B.dmax = geo.dmax;
+  B.sentinels = geo.sentinels;
+  B.real = real;
+  B.sub_ok = sub_ok;
+  B.my_sub = sub_ok ? sub0 : 0;
+  B.s1 = s1;
+  B.s2 = s2;
+  B.ao = (a.emit && real) ? a.al_off[task.out_idx] : 0;
+  BandPath ps;
+  ps.rc = real ? last : -1;
+  ps.D = len2 - len1 + geo.lband + 1;
+  ps.ns = 0;
+  ps.nt = 0;
+  ps.bad = false;
+  if (a.emit)
+    band_backward<true, K>(last, hist_p, lane, ps, B, a);
+  else
+    band_backward<false, K>(last, hist_p, lane, ps, B, a);
+  if (__builtin_amdgcn_ballot_w64(ps.bad && real) && lane == 0)
+    atomicOr(&a.ctr->err_flags, kErrTraceback);
+  __syncthreads();
+
+  // ---- outputs ----
+  if (a.emit) {
+    if (real) {
+      uint8_t* o0 = a.al0 + B.ao;
+      uint8_t* o1 = a.al1 + B.ao;
+      const int nt = ps.nt;
+      for (int x = c; x < nt / 2; x += lanes) {
+        const int y = nt - 1 - x;
+        const uint8_t t0 = o0[x], t1 = o1[x];
+        o0[x] = o0[y];
+        o1[x] = o1[y];
+        o0[y] = t0;
+        o1[y] = t1;
+      }
+      if (c == 0) a.al_len[task.out_idx] = ps.bad ? 0u : (uint32_t)nt;
+    }
+  } else if (real && c == 0) {
+    a.sub_cnt[ti] = sub_ok ? ((uint32_t)ps.ns | (ps.bad ? 0x80000000u : 0u)) : 0u;
+  }
+  __syncthreads();
+}
+
 template <bool HOMO>
-__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(8, 8))) void nw_band_kernel(
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4, 8))) void nw_band_kernel(
     BandDev a) {
   extern __shared__ uint8_t band_lds[];
   const int lane = (int)__lane_id();
@@ -261,7 +400,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(8, 8))) void
     const bool real = g_raw < count;
     const int g = real ? g_raw : 0;  // idle lanes shadow group 0
     const DpTask task = a.tasks[first + (uint32_t)g];
-    const int len1 = geo.len1, len2 = geo.len2, last = len1 + len2;
+    const int len1 = geo.len1, len2 = geo.len2;
 
     // ---- stage s1 (center), s2 (query) and their homopolymer masks ----
     uint8_t* gb = band_lds + (size_t)g * band_group_bytes(geo, HOMO);
@@ -292,90 +431,14 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(8, 8))) void
     }
     __syncthreads();
 
-    BandK K;
-    K.len1 = len1;
-    K.len2 = len2;
-    K.lband = geo.lband;
-    K.dmax = geo.dmax;
-    K.match = a.match;
-    K.mismatch = a.mismatch;
-    K.gap = a.gap;
-    K.homo = a.homo_gap;
-    BandLane L;
-    L.dE = 2 * c - geo.lband - 1;
-    L.dO = L.dE + 1;
-    L.okE = 2 * c <= geo.dmax;
-    L.okO = 2 * c + 1 <= geo.dmax;
-    L.sentE = geo.sentinels && (2 * c == 0 || 2 * c == geo.dmax);
-    L.sentO = geo.sentinels && (2 * c + 1 == geo.dmax);
-    L.prev2 = 0;
-    L.prev = 0;
-
-    // ---- forward: anti-diagonals 0..last ----
     uint32_t* hist_p = a.hist + (size_t)blockIdx.x * a.hist_stride;
-    const int P0 = (geo.lband + 1) & 1;  // parity of D on even steps
-    for (int blk = 0; blk * 16 <= last; ++blk) {
-      if (P0)
-        band_block<HOMO, 1>(blk, last, L, K, s1, s2, h1, h2, hist_p, lane);
-      else
-        band_block<HOMO, 0>(blk, last, L, K, s1, s2, h1, h2, hist_p, lane);
-    }
-    __syncthreads();
-
-    // ---- traceback ----
     const uint32_t ti = first + (uint32_t)g;
-    const size_t sub0 = (size_t)ti * a.sub_stride;
-    const bool sub_ok = a.emit || sub0 + a.sub_stride <= a.sub_capacity;
-    if (!a.emit && __builtin_amdgcn_ballot_w64(!sub_ok && real) && lane == 0)
-      atomicOr(&a.ctr->err_flags, kErrSubOverflow);
-    BandBw B;
-    B.gpos = g_raw * lanes;
-    B.c = c;
-    B.lband = geo.lband;
-    B.len1 = len1;
-    B.len2 = len2;
-    B.dmax = geo.dmax;
-    B.sentinels = geo.sentinels;
-    B.real = real;
-    B.sub_ok = sub_ok;
-    B.my_sub = sub_ok ? sub0 : 0;
-    B.s1 = s1;
-    B.s2 = s2;
-    B.ao = (a.emit && real) ? a.al_off[task.out_idx] : 0;
-    BandPath ps;
-    ps.rc = real ? last : -1;
-    ps.D = len2 - len1 + geo.lband + 1;
-    ps.ns = 0;
-    ps.nt = 0;
-    ps.bad = false;
-    if (a.emit)
-      band_backward<true>(last, hist_p, lane, ps, B, a);
+    if (geo.k == 1)
+      band_pairs<HOMO, 1>(a, geo, c, lane, g_raw, real, task, ti, s1, s2, h1, h2, hist_p);
+    else if (geo.k == 2)
+      band_pairs<HOMO, 2>(a, geo, c, lane, g_raw, real, task, ti, s1, s2, h1, h2, hist_p);
     else
-      band_backward<false>(last, hist_p, lane, ps, B, a);
-    if (__builtin_amdgcn_ballot_w64(ps.bad && real) && lane == 0)
-      atomicOr(&a.ctr->err_flags, kErrTraceback);
-    __syncthreads();
-
-    // ---- outputs ----
-    if (a.emit) {
-      if (real) {
-        uint8_t* o0 = a.al0 + B.ao;
-        uint8_t* o1 = a.al1 + B.ao;
-        const int nt = ps.nt;
-        for (int x = c; x < nt / 2; x += lanes) {
-          const int y = nt - 1 - x;
-          const uint8_t t0 = o0[x], t1 = o1[x];
-          o0[x] = o0[y];
-          o1[x] = o1[y];
-          o0[y] = t0;
-          o1[y] = t1;
-        }
-        if (c == 0) a.al_len[task.out_idx] = ps.bad ? 0u : (uint32_t)nt;
-      }
-    } else if (real && c == 0) {
-      a.sub_cnt[ti] = sub_ok ? ((uint32_t)ps.ns | (ps.bad ? 0x80000000u : 0u)) : 0u;
-    }
-    __syncthreads();
+      band_pairs<HOMO, 4>(a, geo, c, lane, g_raw, real, task, ti, s1, s2, h1, h2, hist_p);
   }
 }
 

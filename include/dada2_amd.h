@@ -169,7 +169,8 @@ int d2g_align_endsfree(d2g_ctx* ctx, uint32_t n, const uint8_t* s1, const uint64
                        uint8_t* al1, const uint64_t* al_offsets, uint32_t* al_len);
 
 /* 1 when the row-band kernel handles this geometry (band diagonals plus the
- * two sentinel diagonals fit 128 = 2 x 64 lanes), 0 otherwise.  The compare
+ * two sentinel diagonals fit 512 = 4 cells x 2 parities x 64 lanes), 0
+ * otherwise.  The compare
  * routes to it for homopolymer gap penalties (homo = 1), for vectorized = 0,
  * and when every unique is longer than 3500 nt (nwalign.rs:1335-1414). */
 int d2g_band_supported(uint32_t len1, uint32_t len2, int32_t band, int homo);

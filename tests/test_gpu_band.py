@@ -97,3 +97,25 @@ def test_compare_homopolymer_long_reads(ctx, oracle, pkg):
         assert np.array_equal(got.lam[r].view(np.uint64), lam.view(np.uint64)), c
         assert np.array_equal(got.hamming[r], ham), c
         assert np.array_equal(got.skipped[r], skp), c
+
+
+def test_band_wide_geometries(ctx, oracle, pkg):
+    """Bands wider than one wavefront of lanes: 2 and 4 cells per lane."""
+    rng = np.random.default_rng(17)
+    cases = []
+    for la, lb, band in [(200, 300, 32), (300, 200, 32), (240, 330, 16), (150, 150, -1),
+                         (180, 120, -1), (1450, 1380, 32), (1380, 1460, 32)]:
+        for _ in range(3):
+            a = rng.integers(1, 5, la).astype(np.uint8)
+            b = np.concatenate([a[: min(la, lb)], rng.integers(1, 5, max(0, lb - la))])[:lb]
+            b = b.astype(np.uint8).copy()
+            hit = rng.random(lb) < 0.05
+            b[hit] = rng.integers(1, 5, int(hit.sum()))
+            cases.append((a.tobytes(), b.tobytes(), band))
+    for band in sorted({c[2] for c in cases}):
+        pairs = [(a, b) for a, b, bd in cases if bd == band]
+        for a, b in pairs:
+            assert pkg.band_supported(len(a), len(b), band, True)
+        _check_pairs(ctx, oracle, pairs, band)
+        _check_pairs(ctx, oracle, pairs, band, homo_gap=-1)
+    assert not pkg.band_supported(600, 600, -1, True)  # 1203 diagonals: beyond 4 x 128
