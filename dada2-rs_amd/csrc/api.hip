@@ -96,6 +96,11 @@ struct d2g_ctx {
   DevBuf<uint32_t> p_len, p_subs, p_reads, p_creads, p_nrec;
   DevBuf<float> p_loglam;
   DevBuf<uint32_t> b_out;  // bimera: 5 words per pair
+  // pinned host staging for per-call uploads (centers, lock flags) and the
+  // last error matrix uploaded (skipped when unchanged)
+  uint8_t* h_pin = nullptr;
+  size_t h_pin_cap = 0;
+  std::vector<double> h_err;
   // the last compare's aligned pairs (for the log-likelihood and gradient)
   bool cmp_valid = false;
   uint32_t cmp_sub_stride = 0;
@@ -194,6 +199,7 @@ int d2g_ctx_destroy(d2g_ctx* c) {
   c->p_loglam.release();
   c->b_out.release();
   c->last_ctr.release();
+  if (c->h_pin) (void)hipHostFree(c->h_pin);
   for (auto* b : {&c->ll_w, &c->ll_part, &c->ll_out}) b->release();
   c->ll_logE.release();
   for (auto* b : {&c->p_lambda, &c->p_pval, &c->p_err}) b->release();
@@ -437,9 +443,39 @@ static float ev_ms(d2g_ctx* c, int a, int b) {
   return ms;
 }
 
+// Copies the last compare's results to the host in the caller's unique
+// order (unpermuted on the device); enqueued only.
+static hipError_t enqueue_fetch(d2g_ctx* c, uint32_t nc, double* out_lambda,
+                                uint32_t* out_hamming, uint8_t* out_skipped) {
+  const size_t np = (size_t)nc * c->u.n;
+  hipStream_t s = c->stream;
+  hipError_t e;
+  if ((e = c->fin_lambda.ensure(np)) != hipSuccess) return e;
+  if ((e = c->fin_hamming.ensure(np)) != hipSuccess) return e;
+  if ((e = c->fin_skipped.ensure(np)) != hipSuccess) return e;
+  if ((e = launch_unpermute(nc, c->u.n, c->u.orig, c->out_lambda.p, c->out_hamming.p,
+                            c->out_skipped.p, c->fin_lambda.p, c->fin_hamming.p,
+                            c->fin_skipped.p, s)) != hipSuccess)
+    return e;
+  if (out_lambda &&
+      (e = hipMemcpyAsync(out_lambda, c->fin_lambda.p, np * sizeof(double), hipMemcpyDeviceToHost, s)) != hipSuccess)
+    return e;
+  if (out_hamming &&
+      (e = hipMemcpyAsync(out_hamming, c->fin_hamming.p, np * sizeof(uint32_t), hipMemcpyDeviceToHost, s)) != hipSuccess)
+    return e;
+  if (out_skipped &&
+      (e = hipMemcpyAsync(out_skipped, c->fin_skipped.p, np, hipMemcpyDeviceToHost, s)) != hipSuccess)
+    return e;
+  return hipSuccess;
+}
+
+// The compare pipeline; with fetch, the results are copied to the host
+// before the call's single synchronisation.
 static int compare_device(d2g_ctx* c, const uint32_t* centers, uint32_t nc, const double* err,
                           uint32_t ncol, const d2g_align_params* p, int use_quals, int greedy,
-                          const uint8_t* lock, d2g_stats* stats) {
+                          const uint8_t* lock, d2g_stats* stats, bool fetch = false,
+                          double* out_lambda = nullptr, uint32_t* out_hamming = nullptr,
+                          uint8_t* out_skipped = nullptr) {
   if (!c || !centers || !err || !p || nc == 0 || ncol == 0) return D2G_ERR_ARG;
   if (!c->loaded) return D2G_ERR_STATE;
   (void)hipSetDevice(c->device);
@@ -464,9 +500,8 @@ static int compare_device(d2g_ctx* c, const uint32_t* centers, uint32_t nc, cons
   bs.nbins = bs.range * bs.range;
   const bool vec = bs.kind == kDpVectorized;
   HIP_TRY_INT(prepare_schedule(c, bs, 0, p->gap_p));
-  std::vector<uint32_t> hc(nc);
-  for (uint32_t i = 0; i < nc; ++i) hc[i] = c->h_inv[centers[i]];
   HIP_TRY(c->centers.ensure(nc));
+  if (c->err.cap < (size_t)16 * ncol) c->h_err.clear();  // reallocation loses the upload
   HIP_TRY(c->err.ensure((size_t)16 * ncol));
   HIP_TRY(c->out_lambda.ensure(npairs));
   HIP_TRY(c->out_hamming.ensure(npairs));
@@ -488,18 +523,33 @@ static int compare_device(d2g_ctx* c, const uint32_t* centers, uint32_t nc, cons
   if (c->sub_capacity < (size_t)1 << 20) c->sub_capacity = (size_t)1 << 20;
   HIP_TRY(c->sub_pool.ensure(c->sub_capacity));
   hipStream_t s = c->stream;
+  // uploads through pinned staging (the previous call ended with a stream
+  // synchronisation, so the staging area is free)
+  const size_t pin_need = (size_t)nc * 4 + (lock ? n : 0) + 16;
+  if (c->h_pin_cap < pin_need) {
+    if (c->h_pin) (void)hipHostFree(c->h_pin);
+    c->h_pin = nullptr;
+    c->h_pin_cap = 0;
+    HIP_TRY(hipHostMalloc((void**)&c->h_pin, pin_need, hipHostMallocDefault));
+    c->h_pin_cap = pin_need;
+  }
+  uint32_t* hc = reinterpret_cast<uint32_t*>(c->h_pin);
+  for (uint32_t i = 0; i < nc; ++i) hc[i] = c->h_inv[centers[i]];
+  HIP_TRY(hipMemcpyAsync(c->centers.p, hc, nc * sizeof(uint32_t), hipMemcpyHostToDevice, s));
   const uint8_t* dlock = nullptr;
   if (lock) {
-    std::vector<uint8_t> sl(n);
+    uint8_t* sl = c->h_pin + (size_t)nc * 4;
     for (uint32_t i = 0; i < n; ++i) sl[i] = lock[c->h_orig[i]];
     HIP_TRY(c->lock.ensure(n));
-    HIP_TRY(hipMemcpyAsync(c->lock.p, sl.data(), n, hipMemcpyHostToDevice, s));
-    HIP_TRY(hipStreamSynchronize(s));
+    HIP_TRY(hipMemcpyAsync(c->lock.p, sl, n, hipMemcpyHostToDevice, s));
     dlock = c->lock.p;
   }
-  HIP_TRY(hipMemcpyAsync(c->centers.p, hc.data(), nc * sizeof(uint32_t), hipMemcpyHostToDevice, s));
-  HIP_TRY(hipMemcpyAsync(c->err.p, err, (size_t)16 * ncol * sizeof(double), hipMemcpyHostToDevice, s));
-  HIP_TRY(hipStreamSynchronize(s));
+  const size_t ne = (size_t)16 * ncol;
+  if (c->h_err.size() != ne || std::memcmp(c->h_err.data(), err, ne * sizeof(double)) != 0) {
+    c->h_err.assign(err, err + ne);
+    HIP_TRY(hipMemcpyAsync(c->err.p, c->h_err.data(), ne * sizeof(double), hipMemcpyHostToDevice, s));
+    HIP_TRY(hipStreamSynchronize(s));  // h_err is pageable: finish before it can change
+  }
 
   for (int attempt = 0; attempt < 6; ++attempt) {
     if (c->timing) HIP_TRY(hipEventRecord(c->ev[0], s));
@@ -551,7 +601,8 @@ static int compare_device(d2g_ctx* c, const uint32_t* centers, uint32_t nc, cons
     na.sub_capacity = c->sub_capacity;
     na.sub_stride = sub_stride;
     na.sub_cnt = c->sub_cnt.p;
-    na.grid = c->nw_grid;
+    // no more wavefronts than pairs can exist: small calls launch small grids
+    na.grid = (uint32_t)std::min<size_t>(c->nw_grid, std::max<size_t>(npairs, 1));
     na.hist = c->hist.p;
     na.hist_stride = c->hist_stride;
     na.lds_bytes = c->lds_bytes;
@@ -574,9 +625,10 @@ static int compare_device(d2g_ctx* c, const uint32_t* centers, uint32_t nc, cons
     la.out_lambda = c->out_lambda.p;
     la.out_hamming = c->out_hamming.p;
     la.out_skipped = c->out_skipped.p;
-    la.grid = (uint32_t)c->cu_count * 4;
+    la.grid = (uint32_t)std::min<size_t>((size_t)c->cu_count * 4, (npairs + 255) / 256);
     HIP_TRY(launch_lambda(la, s));
     if (c->timing) HIP_TRY(hipEventRecord(c->ev[4], s));
+    if (fetch) HIP_TRY(enqueue_fetch(c, nc, out_lambda, out_hamming, out_skipped));
     Counters h;
     HIP_TRY(hipMemcpyAsync(&h, c->ctr.p, sizeof(Counters), hipMemcpyDeviceToHost, s));
     HIP_TRY(hipStreamSynchronize(s));
@@ -625,17 +677,8 @@ int d2g_compare_batch_device(d2g_ctx* c, const uint32_t* centers, uint32_t nc, c
 int d2g_fetch_results(d2g_ctx* c, double* out_lambda, uint32_t* out_hamming, uint8_t* out_skipped) {
   if (!c || !c->loaded || c->last_nc == 0) return D2G_ERR_STATE;
   (void)hipSetDevice(c->device);
-  const size_t np = (size_t)c->last_nc * c->u.n;
-  hipStream_t s = c->stream;
-  HIP_TRY(c->fin_lambda.ensure(np));
-  HIP_TRY(c->fin_hamming.ensure(np));
-  HIP_TRY(c->fin_skipped.ensure(np));
-  HIP_TRY(launch_unpermute(c->last_nc, c->u.n, c->u.orig, c->out_lambda.p, c->out_hamming.p,
-                           c->out_skipped.p, c->fin_lambda.p, c->fin_hamming.p, c->fin_skipped.p, s));
-  if (out_lambda) HIP_TRY(hipMemcpyAsync(out_lambda, c->fin_lambda.p, np * sizeof(double), hipMemcpyDeviceToHost, s));
-  if (out_hamming) HIP_TRY(hipMemcpyAsync(out_hamming, c->fin_hamming.p, np * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
-  if (out_skipped) HIP_TRY(hipMemcpyAsync(out_skipped, c->fin_skipped.p, np, hipMemcpyDeviceToHost, s));
-  HIP_TRY(hipStreamSynchronize(s));
+  HIP_TRY(enqueue_fetch(c, c->last_nc, out_lambda, out_hamming, out_skipped));
+  HIP_TRY(hipStreamSynchronize(c->stream));
   return D2G_OK;
 }
 
@@ -643,9 +686,8 @@ int d2g_compare_batch(d2g_ctx* c, const uint32_t* centers, uint32_t nc, const do
                       uint32_t ncol, const d2g_align_params* p, int use_quals, int greedy,
                       const uint8_t* lock, double* out_lambda, uint32_t* out_hamming,
                       uint8_t* out_skipped, d2g_stats* stats) {
-  int rc = compare_device(c, centers, nc, err, ncol, p, use_quals, greedy, lock, stats);
-  if (rc != D2G_OK) return rc;
-  return d2g_fetch_results(c, out_lambda, out_hamming, out_skipped);
+  return compare_device(c, centers, nc, err, ncol, p, use_quals, greedy, lock, stats, true,
+                        out_lambda, out_hamming, out_skipped);
 }
 
 int d2g_compare_center(d2g_ctx* c, uint32_t center, const double* err, uint32_t ncol,
