@@ -140,11 +140,15 @@ def bench_plain(args, pkg):
     ctx = pkg.Context(0, timing=True)
     ctx.load_uniques(w.seqs, w.quals, w.reads, k=w.kmer_size)
     c = int(w.centers[0])
+    ctx.set_timing(False)
     for _ in range(args.warmup):
         ctx.compare_center(c, w.err)
     t0 = time.perf_counter()
-    st = [ctx.compare_center(c, w.err).stats for _ in range(args.steps)]
+    for _ in range(args.steps):
+        ctx.compare_center(c, w.err)
     wall = time.perf_counter() - t0
+    ctx.set_timing(True)
+    st = [ctx.compare_center(c, w.err).stats for _ in range(max(5, args.steps // 2))]
     ctx.close()
     cpu = None
     if not args.no_cpu:
@@ -281,11 +285,16 @@ def bench_longread(args, pkg):
     info = ctx.device_info()
     ctx.load_uniques(w.seqs, w.quals, w.reads, k=w.kmer_size)
     centers = np.asarray(w.centers, np.uint32)
+    ctx.set_timing(False)
     for _ in range(args.warmup):
         ctx.compare_batch(centers, w.err, params, device_only=True)
     t0 = time.perf_counter()
-    st = [ctx.compare_batch(centers, w.err, params, device_only=True) for _ in range(args.steps)]
+    for _ in range(args.steps):
+        ctx.compare_batch(centers, w.err, params, device_only=True)
     wall = time.perf_counter() - t0
+    ctx.set_timing(True)
+    st = [ctx.compare_batch(centers, w.err, params, device_only=True)
+          for _ in range(max(3, args.steps // 2))]
     bs = synth.workload_bimera()
     ctx.is_bimera_batch(bs.queries[:50], bs.parent_lists[:50])
     t1 = time.perf_counter()
@@ -351,15 +360,22 @@ def main():
     centers = np.asarray(w.centers, np.uint32)
     n_pairs = len(centers) * w.n
 
+    # timed region: the production path (launches replayed from a captured
+    # HIP graph, no events)
+    ctx.set_timing(False)
     for _ in range(args.warmup):
         ctx.compare_batch(centers, w.err, params, device_only=True)
-    stats = []
     t0 = time.perf_counter()
     for _ in range(args.steps):
-        stats.append(ctx.compare_batch(centers, w.err, params, device_only=True))
+        ctx.compare_batch(centers, w.err, params, device_only=True)
     wall = time.perf_counter() - t0
     ms_step = wall / args.steps * 1e3
     value = n_pairs * args.steps / wall
+    # per-kernel durations: the same step with HIP events around each kernel
+    # on the context stream (eager launches), right after the timed region
+    ctx.set_timing(True)
+    stats = [ctx.compare_batch(centers, w.err, params, device_only=True)
+             for _ in range(max(5, args.steps // 2))]
 
     def avg(key):
         return float(np.mean([s[key] for s in stats]))

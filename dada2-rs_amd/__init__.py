@@ -121,6 +121,7 @@ def _lib():
                                              f64p, C.POINTER(C.c_float)]),
             "d2g_loglik_backward": (C.c_int, [vp, C.POINTER(C.c_float), f64p,
                                               C.POINTER(C.c_float)]),
+            "d2g_ctx_set_graphs": (C.c_int, [vp, C.c_int]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -138,7 +139,7 @@ def exported_symbols() -> list[str]:
             "d2g_align_vectorized", "d2g_vectorized_cells", "d2g_vectorized_supported",
             "d2g_align_endsfree", "d2g_band_supported", "d2g_pairs_load", "d2g_pairs_lambda",
             "d2g_pairs_fetch", "d2g_bimera_pairs", "d2g_is_bimera_batch", "d2g_table_bimera2",
-            "d2g_loglik_forward", "d2g_loglik_backward"]
+            "d2g_loglik_forward", "d2g_loglik_backward", "d2g_ctx_set_graphs"]
 
 
 def _p(a: np.ndarray, t):
@@ -229,6 +230,10 @@ class Context:
 
     def set_timing(self, on: bool):
         _check(_lib().d2g_ctx_set_timing(self.h, int(on)), "d2g_ctx_set_timing")
+
+    def set_graphs(self, on: bool):
+        """Replay the compare's launches as a captured HIP graph (default on)."""
+        _check(_lib().d2g_ctx_set_graphs(self.h, int(on)), "d2g_ctx_set_graphs")
 
     def device_info(self) -> dict:
         a, b, c, d = C.c_int(), C.c_int(), C.c_int(), C.c_int()

@@ -96,6 +96,10 @@ struct d2g_ctx {
   DevBuf<uint32_t> p_len, p_subs, p_reads, p_creads, p_nrec;
   DevBuf<float> p_loglam;
   DevBuf<uint32_t> b_out;  // bimera: 5 words per pair
+  // the compare's launch sequence as a replayable HIP graph, per shape key
+  bool graphs = true;
+  hipGraphExec_t graph_exec = nullptr;
+  std::vector<uint64_t> graph_key;
   // pinned host staging for per-call uploads (centers, lock flags) and the
   // last error matrix uploaded (skipped when unchanged)
   uint8_t* h_pin = nullptr;
@@ -200,6 +204,7 @@ int d2g_ctx_destroy(d2g_ctx* c) {
   c->b_out.release();
   c->last_ctr.release();
   if (c->h_pin) (void)hipHostFree(c->h_pin);
+  if (c->graph_exec) (void)hipGraphExecDestroy(c->graph_exec);
   for (auto* b : {&c->ll_w, &c->ll_part, &c->ll_out}) b->release();
   c->ll_logE.release();
   for (auto* b : {&c->p_lambda, &c->p_pval, &c->p_err}) b->release();
@@ -213,6 +218,12 @@ int d2g_ctx_destroy(d2g_ctx* c) {
 int d2g_ctx_set_timing(d2g_ctx* c, int enable) {
   if (!c) return D2G_ERR_ARG;
   c->timing = enable != 0;
+  return D2G_OK;
+}
+
+int d2g_ctx_set_graphs(d2g_ctx* c, int enable) {
+  if (!c) return D2G_ERR_ARG;
+  c->graphs = enable != 0;
   return D2G_OK;
 }
 
@@ -365,6 +376,9 @@ int d2g_load_uniques(d2g_ctx* c, uint32_t n, const uint8_t* seq_concat, const ui
   c->has_qual = qual_concat != nullptr;
   c->loaded = true;
   c->cmp_valid = false;
+  if (c->graph_exec) (void)hipGraphExecDestroy(c->graph_exec);  // unique buffers changed
+  c->graph_exec = nullptr;
+  c->graph_key.clear();
   return D2G_OK;
 }
 
@@ -528,6 +542,7 @@ static int compare_device(d2g_ctx* c, const uint32_t* centers, uint32_t nc, cons
   const size_t pin_need = (size_t)nc * 4 + (lock ? n : 0) + 16;
   if (c->h_pin_cap < pin_need) {
     if (c->h_pin) (void)hipHostFree(c->h_pin);
+  if (c->graph_exec) (void)hipGraphExecDestroy(c->graph_exec);
     c->h_pin = nullptr;
     c->h_pin_cap = 0;
     HIP_TRY(hipHostMalloc((void**)&c->h_pin, pin_need, hipHostMallocDefault));
@@ -551,8 +566,11 @@ static int compare_device(d2g_ctx* c, const uint32_t* centers, uint32_t nc, cons
     HIP_TRY(hipStreamSynchronize(s));  // h_err is pageable: finish before it can change
   }
 
-  for (int attempt = 0; attempt < 6; ++attempt) {
-    if (c->timing) HIP_TRY(hipEventRecord(c->ev[0], s));
+  // The launch sequence of one compare (screen -> plan -> scatter -> DP ->
+  // lambda).  Timed runs bracket the kernels with events; otherwise the
+  // sequence is captured once per shape into a HIP graph and replayed.
+  auto enqueue = [&](bool timed) -> int {
+    if (timed) HIP_TRY(hipEventRecord(c->ev[0], s));
     HIP_TRY(launch_zero(reinterpret_cast<uint32_t*>(c->ctr.p), sizeof(Counters) / 4, s));
     HIP_TRY(launch_zero(c->bin_count.p, bs.nbins + 1, s));
     ScreenArgs sa;
@@ -575,12 +593,12 @@ static int compare_device(d2g_ctx* c, const uint32_t* centers, uint32_t nc, cons
     sa.partials = c->screen_partials.p;
     sa.ctr = c->ctr.p;
     HIP_TRY(launch_screen(sa, s));
-    if (c->timing) HIP_TRY(hipEventRecord(c->ev[1], s));
+    if (timed) HIP_TRY(hipEventRecord(c->ev[1], s));
     HIP_TRY(launch_plan(bs, c->bin_count.p, c->bin_start.p, c->wave_start.p, c->bin_fill.p, c->ctr.p,
                         c->sched_info.p, c->screen_partials.p, n_part, c->gl_off.p, s));
     HIP_TRY(launch_scatter(c->dp_unsorted.p, c->dp_sorted.p, c->bin_start.p, c->bin_fill.p,
                            c->screen_partials.p, n_part, 0, c->gl_unsorted.p, c->gl.p, c->gl_off.p, s));
-    if (c->timing) HIP_TRY(hipEventRecord(c->ev[2], s));
+    if (timed) HIP_TRY(hipEventRecord(c->ev[2], s));
     NwArgs na;
     std::memset(&na, 0, sizeof(na));
     na.bins = bs;
@@ -610,7 +628,7 @@ static int compare_device(d2g_ctx* c, const uint32_t* centers, uint32_t nc, cons
     na.info = c->sched_info.p;
     na.sched_rows = c->sched_rows;
     HIP_TRY(vec ? launch_nw_vec(na, s) : launch_nw_band(na, s));
-    if (c->timing) HIP_TRY(hipEventRecord(c->ev[3], s));
+    if (timed) HIP_TRY(hipEventRecord(c->ev[3], s));
     LambdaArgs la;
     la.u = &u;
     la.gapless_tasks = c->gl.p;
@@ -627,7 +645,55 @@ static int compare_device(d2g_ctx* c, const uint32_t* centers, uint32_t nc, cons
     la.out_skipped = c->out_skipped.p;
     la.grid = (uint32_t)std::min<size_t>((size_t)c->cu_count * 4, (npairs + 255) / 256);
     HIP_TRY(launch_lambda(la, s));
-    if (c->timing) HIP_TRY(hipEventRecord(c->ev[4], s));
+    if (timed) HIP_TRY(hipEventRecord(c->ev[4], s));
+    return D2G_OK;
+  };
+
+  for (int attempt = 0; attempt < 6; ++attempt) {
+    const bool graph = c->graphs && !c->timing;
+    if (graph) {
+      // everything a kernel argument depends on
+      std::vector<uint64_t> key = {
+          nc, n, (uint64_t)(uintptr_t)u.seq, (uint64_t)(uintptr_t)dlock, (uint64_t)greedy,
+          (uint64_t)use_quals, ncol, (uint64_t)bs.lmin, (uint64_t)bs.range, (uint64_t)bs.band,
+          (uint64_t)bs.kind, (uint64_t)bs.homo, (uint64_t)p->use_kmers, (uint64_t)p->gapless,
+          (uint64_t)p->match_score, (uint64_t)(uint32_t)p->mismatch, (uint64_t)(uint32_t)p->gap_p,
+          (uint64_t)(uint32_t)p->homo_gap_p, (uint64_t)p->kmer_size, sub_stride, c->sub_capacity,
+          c->nw_grid, c->lds_bytes, c->hist_stride, (uint64_t)c->sched_rows};
+      uint64_t cut;
+      std::memcpy(&cut, &p->kdist_cutoff, 8);
+      key.push_back(cut);
+      for (const void* q : {(const void*)c->centers.p, (const void*)c->err.p,
+                            (const void*)c->out_lambda.p, (const void*)c->out_hamming.p,
+                            (const void*)c->out_skipped.p, (const void*)c->gl.p,
+                            (const void*)c->gl_unsorted.p, (const void*)c->gl_off.p,
+                            (const void*)c->dp_unsorted.p, (const void*)c->dp_sorted.p,
+                            (const void*)c->sub_cnt.p, (const void*)c->sub_pool.p,
+                            (const void*)c->bin_count.p, (const void*)c->bin_start.p,
+                            (const void*)c->wave_start.p, (const void*)c->bin_fill.p,
+                            (const void*)c->screen_partials.p, (const void*)c->hist.p,
+                            (const void*)c->sched.p, (const void*)c->sched_info.p,
+                            (const void*)c->ctr.p})
+        key.push_back((uint64_t)(uintptr_t)q);
+      if (!c->graph_exec || key != c->graph_key) {
+        if (c->graph_exec) (void)hipGraphExecDestroy(c->graph_exec);
+        c->graph_exec = nullptr;
+        c->graph_key.clear();
+        hipGraph_t gr = nullptr;
+        HIP_TRY(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+        const int rc = enqueue(false);
+        const hipError_t ec = hipStreamEndCapture(s, &gr);
+        if (rc != D2G_OK) return rc;
+        HIP_TRY(ec);
+        const hipError_t ei = hipGraphInstantiate(&c->graph_exec, gr, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(gr);
+        HIP_TRY(ei);
+        c->graph_key = key;
+      }
+      HIP_TRY(hipGraphLaunch(c->graph_exec, s));
+    } else {
+      HIP_TRY_INT(enqueue(c->timing));
+    }
     if (fetch) HIP_TRY(enqueue_fetch(c, nc, out_lambda, out_hamming, out_skipped));
     Counters h;
     HIP_TRY(hipMemcpyAsync(&h, c->ctr.p, sizeof(Counters), hipMemcpyDeviceToHost, s));

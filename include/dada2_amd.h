@@ -85,6 +85,10 @@ int d2g_ctx_create(int device, d2g_ctx** out);
 int d2g_ctx_destroy(d2g_ctx* ctx);
 /* Enable per-kernel HIP-event timing in d2g_stats (default off). */
 int d2g_ctx_set_timing(d2g_ctx* ctx, int enable);
+/* Replay the compare's launch sequence as a HIP graph, captured once per
+ * shape (centers count, bin space, parameters, buffers) (default on; timed
+ * calls always launch eagerly). */
+int d2g_ctx_set_graphs(d2g_ctx* ctx, int enable);
 /* The hipStream_t every kernel of this context is launched on. */
 void* d2g_ctx_stream(d2g_ctx* ctx);
 /* Device properties used for roofline figures. */

@@ -92,3 +92,30 @@ def test_results_are_deterministic(ctx, pkg, batched_small):
     b = ctx.compare_batch(w.centers[:6], w.err)
     assert np.array_equal(a.lam.view(np.uint64), b.lam.view(np.uint64))
     assert np.array_equal(a.hamming, b.hamming)
+
+
+def test_graph_replays_match_eager(pkg, oracle, batched_small):
+    """The captured launch sequence, replayed with new center lists of the
+    same length and with eager calls of other shapes in between, gives the
+    eager results every time."""
+    w = batched_small
+    g = pkg.Context(0)                 # graphs on (default), no timing
+    e = pkg.Context(0, timing=True)    # timed calls always launch eagerly
+    try:
+        for c in (g, e):
+            c.load_uniques(w.seqs, w.quals, w.reads, k=5)
+        rng = np.random.default_rng(8)
+        for rep in range(5):
+            cs = np.sort(rng.choice(w.centers, size=8, replace=False))
+            got = g.compare_batch(cs, w.err)
+            want = e.compare_batch(cs, w.err)
+            assert np.array_equal(got.lam.view(np.uint64), want.lam.view(np.uint64)), rep
+            assert np.array_equal(got.hamming, want.hamming) and np.array_equal(got.skipped, want.skipped)
+            # an eager call of another shape in between
+            g.compare_center(int(cs[0]), w.err)
+            e.compare_batch(cs[:3], w.err)
+        lam, ham, skp = _oracle_rows(oracle, w, cs)
+        _assert_same(got, lam, ham, skp)
+    finally:
+        g.close()
+        e.close()
