@@ -122,6 +122,7 @@ def _lib():
             "d2g_loglik_backward": (C.c_int, [vp, C.POINTER(C.c_float), f64p,
                                               C.POINTER(C.c_float)]),
             "d2g_ctx_set_graphs": (C.c_int, [vp, C.c_int]),
+            "d2g_ctx_set_screen_shape": (C.c_int, [vp, C.c_int, C.c_int]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -139,7 +140,8 @@ def exported_symbols() -> list[str]:
             "d2g_align_vectorized", "d2g_vectorized_cells", "d2g_vectorized_supported",
             "d2g_align_endsfree", "d2g_band_supported", "d2g_pairs_load", "d2g_pairs_lambda",
             "d2g_pairs_fetch", "d2g_bimera_pairs", "d2g_is_bimera_batch", "d2g_table_bimera2",
-            "d2g_loglik_forward", "d2g_loglik_backward", "d2g_ctx_set_graphs"]
+            "d2g_loglik_forward", "d2g_loglik_backward", "d2g_ctx_set_graphs",
+            "d2g_ctx_set_screen_shape"]
 
 
 def _p(a: np.ndarray, t):
@@ -234,6 +236,12 @@ class Context:
     def set_graphs(self, on: bool):
         """Replay the compare's launches as a captured HIP graph (default on)."""
         _check(_lib().d2g_ctx_set_graphs(self.h, int(on)), "d2g_ctx_set_graphs")
+
+    def set_screen_shape(self, qblock: int = 256, split: int = 2):
+        """Screen launch shape: uniques per workgroup (64, 128, 256) and the
+        wavefront groups (1, 2, 4) sharing its 16 centers."""
+        _check(_lib().d2g_ctx_set_screen_shape(self.h, int(qblock), int(split)),
+               "d2g_ctx_set_screen_shape")
 
     def device_info(self) -> dict:
         a, b, c, d = C.c_int(), C.c_int(), C.c_int(), C.c_int()

@@ -98,6 +98,10 @@ struct d2g_ctx {
   DevBuf<uint32_t> b_out;  // bimera: 5 words per pair
   // the compare's launch sequence as a replayable HIP graph, per shape key
   bool graphs = true;
+  // screen_kernel's launch shape: uniques per workgroup, and wavefront
+  // groups sharing its 16 centers
+  uint32_t screen_qblock = 256;
+  int screen_split = 2;
   hipGraphExec_t graph_exec = nullptr;
   std::vector<uint64_t> graph_key;
   // pinned host staging for per-call uploads (centers, lock flags) and the
@@ -224,6 +228,14 @@ int d2g_ctx_set_timing(d2g_ctx* c, int enable) {
 int d2g_ctx_set_graphs(d2g_ctx* c, int enable) {
   if (!c) return D2G_ERR_ARG;
   c->graphs = enable != 0;
+  return D2G_OK;
+}
+
+int d2g_ctx_set_screen_shape(d2g_ctx* c, int qblock, int split) {
+  if (!c || (split != 1 && split != 2 && split != 4)) return D2G_ERR_ARG;
+  if (qblock != 64 && qblock != 128 && qblock != 256) return D2G_ERR_ARG;
+  c->screen_qblock = (uint32_t)qblock;
+  c->screen_split = split;
   return D2G_OK;
 }
 
@@ -520,12 +532,13 @@ static int compare_device(d2g_ctx* c, const uint32_t* centers, uint32_t nc, cons
   HIP_TRY(c->out_lambda.ensure(npairs));
   HIP_TRY(c->out_hamming.ensure(npairs));
   HIP_TRY(c->out_skipped.ensure(npairs));
-  // per screen block: a region of kScreenRegion task slots in each list
-  const uint32_t n_part = screen_blocks(n, nc);
+  // per screen block: a region of `region` task slots in each list
+  const uint32_t n_part = screen_blocks(n, nc, c->screen_qblock);
+  const uint32_t region = screen_region(c->screen_qblock);
   HIP_TRY(c->gl.ensure(npairs));
-  HIP_TRY(c->gl_unsorted.ensure((size_t)n_part * kScreenRegion));
+  HIP_TRY(c->gl_unsorted.ensure((size_t)n_part * region));
   HIP_TRY(c->gl_off.ensure(n_part));
-  HIP_TRY(c->dp_unsorted.ensure((size_t)n_part * kScreenRegion));
+  HIP_TRY(c->dp_unsorted.ensure((size_t)n_part * region));
   HIP_TRY(c->dp_sorted.ensure(npairs));
   HIP_TRY(c->sub_cnt.ensure(npairs));
   HIP_TRY(c->bin_count.ensure(bs.nbins + 1));
@@ -592,12 +605,15 @@ static int compare_device(d2g_ctx* c, const uint32_t* centers, uint32_t nc, cons
     sa.bin_count = c->bin_count.p;
     sa.partials = c->screen_partials.p;
     sa.ctr = c->ctr.p;
+    sa.qblock = c->screen_qblock;
+    sa.split = c->screen_split;
     HIP_TRY(launch_screen(sa, s));
     if (timed) HIP_TRY(hipEventRecord(c->ev[1], s));
     HIP_TRY(launch_plan(bs, c->bin_count.p, c->bin_start.p, c->wave_start.p, c->bin_fill.p, c->ctr.p,
                         c->sched_info.p, c->screen_partials.p, n_part, c->gl_off.p, s));
     HIP_TRY(launch_scatter(c->dp_unsorted.p, c->dp_sorted.p, c->bin_start.p, c->bin_fill.p,
-                           c->screen_partials.p, n_part, 0, c->gl_unsorted.p, c->gl.p, c->gl_off.p, s));
+                           c->screen_partials.p, n_part, region, 0, c->gl_unsorted.p, c->gl.p,
+                           c->gl_off.p, s));
     if (timed) HIP_TRY(hipEventRecord(c->ev[2], s));
     NwArgs na;
     std::memset(&na, 0, sizeof(na));
@@ -659,7 +675,8 @@ static int compare_device(d2g_ctx* c, const uint32_t* centers, uint32_t nc, cons
           (uint64_t)bs.kind, (uint64_t)bs.homo, (uint64_t)p->use_kmers, (uint64_t)p->gapless,
           (uint64_t)p->match_score, (uint64_t)(uint32_t)p->mismatch, (uint64_t)(uint32_t)p->gap_p,
           (uint64_t)(uint32_t)p->homo_gap_p, (uint64_t)p->kmer_size, sub_stride, c->sub_capacity,
-          c->nw_grid, c->lds_bytes, c->hist_stride, (uint64_t)c->sched_rows};
+          c->nw_grid, c->lds_bytes, c->hist_stride, (uint64_t)c->sched_rows,
+          (uint64_t)c->screen_split, (uint64_t)c->screen_qblock};
       uint64_t cut;
       std::memcpy(&cut, &p->kdist_cutoff, 8);
       key.push_back(cut);
@@ -821,8 +838,8 @@ static int align_pairs_device(d2g_ctx* c, uint32_t n, const uint8_t* s1, const u
   }
   hipStream_t s = c->stream;
   HIP_TRY(c->a_pool.ensure(n1 + n2 + 1));
-  const uint32_t n_regions = (n + kScreenRegion - 1) / kScreenRegion;
-  HIP_TRY(c->dp_unsorted.ensure((size_t)n_regions * kScreenRegion));
+  const uint32_t n_regions = (n + kAlignRegion - 1) / kAlignRegion;
+  HIP_TRY(c->dp_unsorted.ensure((size_t)n_regions * kAlignRegion));
   HIP_TRY(c->dp_sorted.ensure(n));
   HIP_TRY(c->bin_count.ensure(bs.nbins + 1));
   HIP_TRY(c->bin_start.ensure(bs.nbins + 1));
@@ -846,7 +863,7 @@ static int align_pairs_device(d2g_ctx* c, uint32_t n, const uint8_t* s1, const u
   HIP_TRY(launch_plan(bs, c->bin_count.p, c->bin_start.p, c->wave_start.p, c->bin_fill.p, c->ctr.p,
                       c->sched_info.p, nullptr, 0, nullptr, s));
   HIP_TRY(launch_scatter(c->dp_unsorted.p, c->dp_sorted.p, c->bin_start.p, c->bin_fill.p, nullptr,
-                         n_regions, n, nullptr, nullptr, nullptr, s));
+                         n_regions, kAlignRegion, n, nullptr, nullptr, nullptr, s));
   NwArgs na;
   std::memset(&na, 0, sizeof(na));
   na.bins = bs;

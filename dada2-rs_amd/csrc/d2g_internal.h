@@ -80,11 +80,14 @@ struct RowRec {
   int32_t row_free; // accumulated ends-free edge score (nwalign.rs:884)
   uint32_t flags;   // bit0 left recalc, bit1 right recalc, bit2 band wedge,
                     // bits 8-15 col_min after the wedge, bits 16-23 col_max after it
+  uint32_t act_lo;  // lanes of the bin's wavefront whose column is active
+  uint32_t act_hi;  // (columns col_min..col_max of each of its pairs), bits 0-31 / 32-63
 };
 
 // Per bin: the rows [steady_lo, steady_hi) (multiples of 16) where no
-// ends-free edge or recalculation applies and every active cell lies inside
-// both sequences, so the DP runs its short row update there; and the DP
+// ends-free edge or recalculation applies, every active cell lies inside
+// both sequences, and the s1 / s2 offsets advance by one after each odd /
+// even row, so the DP runs its short row update there; and the DP
 // cells of one pair (sum of the active widths).
 struct BinInfo {
   uint32_t steady_lo, steady_hi, cells, _pad;

@@ -33,9 +33,8 @@ struct UniqueDev {
 hipError_t launch_prep(const UniqueDev& u, hipStream_t s);
 
 // ---- screen.hip --------------------------------------------------------
-// Task slots per screen workgroup (256 uniques x 16 centers): each block
-// appends its gapless and DP tasks to its own region of that many slots.
-constexpr uint32_t kScreenRegion = 4096;
+// Task slots per region of the pairwise align path's DP task list.
+constexpr uint32_t kAlignRegion = 4096;
 
 struct ScreenArgs {
   const UniqueDev* u;  // host copy of the pointers
@@ -56,9 +55,15 @@ struct ScreenArgs {
   uint32_t* bin_count;
   uint32_t* partials;  // per block: skipped, shrouded, gapless, DP (screen_blocks entries)
   Counters* ctr;
+  uint32_t qblock = 256;  // uniques per workgroup: 64, 128 or 256
+  int split = 2;          // wavefront groups per workgroup sharing its 16 centers: 1, 2 or 4
 };
 hipError_t launch_screen(const ScreenArgs& a, hipStream_t s);
-uint32_t screen_blocks(uint32_t n_uniques, uint32_t n_centers);
+// Workgroups of the screen grid; each owns screen_region(qblock) task slots
+// in each list.
+uint32_t screen_blocks(uint32_t n_uniques, uint32_t n_centers, uint32_t qblock);
+constexpr uint32_t kScreenCenters = 16;  // centers per screen workgroup
+inline uint32_t screen_region(uint32_t qblock) { return qblock * kScreenCenters; }
 
 // ---- plan.hip ----------------------------------------------------------
 hipError_t launch_zero(uint32_t* p, size_t n_words, hipStream_t s);
@@ -69,13 +74,13 @@ hipError_t launch_plan(const BinSpace& bs, const uint32_t* bin_count, uint32_t* 
                        uint32_t* wave_start, uint32_t* bin_fill, Counters* ctr,
                        const BinInfo* info, const uint32_t* partials, uint32_t n_partials,
                        uint32_t* gl_off, hipStream_t s);
-// Bins the DP tasks of n_regions regions of kScreenRegion slots (counts from
+// Bins the DP tasks of n_regions regions of `region` slots (counts from
 // partials, or `count` tasks filling the regions in order when partials is
 // null) and compacts the gapless tasks (compare only).
 hipError_t launch_scatter(const DpTask* unsorted, DpTask* sorted, const uint32_t* bin_start,
                           uint32_t* bin_fill, const uint32_t* partials, uint32_t n_regions,
-                          uint32_t count, const GaplessTask* gl_unsorted, GaplessTask* gl,
-                          const uint32_t* gl_off, hipStream_t s);
+                          uint32_t region, uint32_t count, const GaplessTask* gl_unsorted,
+                          GaplessTask* gl, const uint32_t* gl_off, hipStream_t s);
 hipError_t launch_unpermute(uint32_t nc, uint32_t n, const uint32_t* orig, const double* src_lam,
                             const uint32_t* src_ham, const uint8_t* src_skp, double* dst_lam,
                             uint32_t* dst_ham, uint8_t* dst_skp, hipStream_t s);

@@ -20,8 +20,15 @@
 // dploop, nwalign.rs:700-737).  Saturating i16 arithmetic is emulated on
 // i32 with clamps.
 //
-// Traceback.  Each lane appends its 2-bit pointer per row to a private
-// history word flushed to LDS every 16 rows.  A diagonal step over a
+// Steady rows (the bulk of the anti-diagonals) take a short update: their
+// sequence bytes are read at fixed LDS offsets from a per-block base, the
+// active lanes come as a 64-bit mask from the schedule, and where the scores
+// provably stay inside i16 the saturation clamps are dropped (a clamp only
+// changes a sentinel-derived candidate, which never wins there).
+//
+// Traceback.  Each lane shifts its 2-bit pointer per row into a private
+// history word (word = 4 word + pointer, so row r sits at bits 2 (15 - r mod
+// 16)) flushed to global memory every 16 rows.  A diagonal step over a
 // mismatch is stored as pointer 0; only active cells carry diagonal
 // pointers, so the traceback tells it from the reference's invalid pointer
 // by the row's active range.  The traceback (nwalign.rs:1101-1139) sweeps
@@ -31,6 +38,7 @@
 // al2subs substitutions (nwalign.rs:1169-1224), written straight to global
 // memory as (query position << 16 | center position) for the lambda
 // kernel; in parity mode the alignment strings are written instead.
+#include <algorithm>
 #include <type_traits>
 
 #include "d2g_device.h"
@@ -48,6 +56,7 @@ struct NwDev {
   const BinInfo* info;
   int sched_rows;
   int match, mismatch, gap, end_gap;
+  int steady_clamp;  // keep the i16 clamps in steady rows
   Counters* ctr;
   int emit;
   uint32_t* sub_pool;
@@ -155,23 +164,51 @@ __global__ void sched_kernel(BinSpace bs, int rows, int end_gap, int gap, RowRec
   if (nw_groups(g) == 0 || g.nrow > rows) return;
   RowRec* t = table + (size_t)b * rows;
   vec_schedule(g, end_gap, gap, t);
-  // two lookahead records past the last row (rows last+1, last+2)
   const int last = g.len1 + g.len2;
+  // active-lane masks over the G pairs of a wavefront (pair gi on lanes
+  // gi * ncol .. gi * ncol + ncol - 1)
+  const int G = nw_groups(g);
+  for (int row = 2; row <= last; ++row) {
+    RowRec& R = t[row - 2];
+    uint64_t base = 0;
+    if (R.n > 0 && R.cmin < 64)
+      base = (R.n >= 64 ? ~0ull : ((1ull << R.n) - 1ull)) << R.cmin;
+    uint64_t m = 0;
+    for (int gi = 0; gi < G; ++gi) m |= base << (gi * g.ncol);
+    R.act_lo = (uint32_t)m;
+    R.act_hi = (uint32_t)(m >> 32);
+  }
+  // two lookahead records past the last row (rows last+1, last+2)
   t[last - 1] = t[last - 2];
   t[last] = t[last - 2];
-  // longest run of steady rows, trimmed to whole 16-row history words
+  // longest run of steady rows whose s1 / s2 offsets advance by one after
+  // each odd / even row, trimmed to whole 16-row history words
   int best_lo = 0, best_hi = 0, run_lo = -1;
   uint32_t cells = 0;
   for (int row = 2; row <= last; ++row) {
     const RowRec& R = t[row - 2];
     cells += R.n;
+    // a steady row's active cells keep off each pair's outer columns (a
+    // lane-shift that leaves the wavefront lands on an inactive lane), and
+    // its active lanes are those of the row two up (so every active cell has
+    // an active diagonal predecessor, and a run needs one mask per parity)
+    const uint64_t act = ((uint64_t)R.act_hi << 32) | R.act_lo;
+    const uint64_t act2 = row >= 4 ? (((uint64_t)t[row - 4].act_hi << 32) | t[row - 4].act_lo) : 0ull;
     const bool steady = R.edge_l < 0 && R.edge_t < 0 && (R.flags & 3u) == 0 &&
-                        R.inr_lo == R.cmin && R.inr_n == R.n;
+                        R.inr_lo == R.cmin && R.inr_n == R.n && R.cmin >= 1 &&
+                        R.cmin + (int)R.n <= g.ncol - 1 && act == act2;
     if (!steady) {
       run_lo = -1;
       continue;
     }
-    if (run_lo < 0) run_lo = row;
+    if (run_lo >= 0) {
+      const RowRec& P = t[row - 3];
+      const bool odd_prev = ((row - 1) & 1) != 0;
+      const bool linked = R.a1 - P.a1 == (odd_prev ? 1 : 0) && R.a2 - P.a2 == (odd_prev ? 0 : 1);
+      if (!linked) run_lo = row;
+    } else {
+      run_lo = row;
+    }
     if (row + 1 - run_lo > best_hi - best_lo) {
       best_lo = run_lo;
       best_hi = row + 1;
@@ -265,9 +302,9 @@ __device__ __forceinline__ void nw_row(const int row, const RowRec R, LaneState&
   // carry diagonal pointers.  (In the compare path every base is in 1..4,
   // so each such step is an al2subs substitution.)
   p = (p == 1 && inr && differ) ? 0 : p;
-  ls.hp |= (uint32_t)p << (2 * (row & 15));
+  ls.hp = ls.hp * 4u + (uint32_t)p;
   if ((row & 15) == 15 || row == K.last) {
-    hist_p[(row >> 4) * 64 + lane] = ls.hp;
+    hist_p[(row >> 4) * 64 + lane] = ls.hp << (2 * (15 - (row & 15)));
     ls.hp = 0;
   }
   ls.prev2 = ls.prev;
@@ -278,7 +315,8 @@ __device__ __forceinline__ void nw_row(const int row, const RowRec R, LaneState&
 // address space lets every record arrive by scalar loads.
 typedef const __attribute__((address_space(4))) uint32_t CWord;
 typedef CWord CRowRec;
-static_assert(sizeof(RowRec) == 40, "RowRec is ten words");
+constexpr int kRecWords = 12;
+static_assert(sizeof(RowRec) == 4 * kRecWords, "RowRec is twelve words");
 
 __device__ __forceinline__ RowRec load_rec(CWord* w) {
   RowRec R;
@@ -295,42 +333,6 @@ __device__ __forceinline__ RowRec load_rec(CWord* w) {
   return R;
 }
 
-// The four fields a steady row needs.
-struct Rec4 {
-  int32_t a1, a2, cmin;
-  uint32_t n;
-};
-__device__ __forceinline__ Rec4 load_rec4(CWord* w) {
-  Rec4 R;
-  R.a1 = (int32_t)w[0];
-  R.a2 = (int32_t)w[1];
-  R.cmin = (int32_t)w[2];
-  R.n = w[3];
-  return R;
-}
-
-// A steady row (no ends-free edge, no recalculation, every active cell
-// inside both sequences): nw_row reduced to what such a row can reach.
-// Returns the cell's pointer code.
-template <bool EVEN, bool SWAP>
-__device__ __forceinline__ uint32_t steady_cell(const Rec4 R, const int ca, const int cb,
-                                                LaneState& ls, const RowConst K, const int c) {
-  const bool differ = ca != cb;
-  const int dg = clamp16(ls.prev2 + (differ ? K.mismatch : K.match));
-  const int lraw = EVEN ? from_lower(ls.prev, K.fill) : ls.prev;
-  const int uraw = EVEN ? ls.prev : from_upper(ls.prev, K.fill);
-  const int l = clamp16(lraw + K.gap), u = clamp16(uraw + K.gap);
-  const bool upf = SWAP ? (l < u) : (u >= l);
-  const int m = l > u ? l : u;
-  const bool dw = dg > m;
-  const bool act = (unsigned)(c - R.cmin) < R.n;
-  int p = dw ? (differ ? 0 : 1) : (upf ? 3 : 2);
-  p = act ? p : 0;
-  ls.prev2 = ls.prev;
-  ls.prev = act ? (dw ? dg : m) : K.fill;
-  return (uint32_t)p;
-}
-
 // Rows [r0, r1) through the full row update.  Software pipelined: the bases
 // of row r+1 and the record of row r+2 are requested before row r is
 // computed, so the one wait per row (LDS and scalar loads share a counter)
@@ -342,12 +344,12 @@ __device__ __forceinline__ void generic_rows(const int r0, const int r1, CRowRec
                                              const uint8_t* s2l, const int c, const int lane,
                                              uint32_t* hist_p) {
   if (r0 >= r1) return;
-  RowRec cur = load_rec(rec + 10 * (r0 - 2));
-  RowRec nxt = load_rec(rec + 10 * (r0 - 1));
+  RowRec cur = load_rec(rec + kRecWords * (r0 - 2));
+  RowRec nxt = load_rec(rec + kRecWords * (r0 - 1));
   int ca = s1l[cur.a1], cb = s2l[cur.a2];
   for (int row = r0; row < r1; ++row) {
     const int ca1 = s1l[nxt.a1], cb1 = s2l[nxt.a2];
-    const RowRec nx2 = load_rec(rec + 10 * row);  // row + 2
+    const RowRec nx2 = load_rec(rec + kRecWords * row);  // row + 2
     if (row & 1)
       nw_row<false, SWAP>(row, cur, ls, K, ca, cb, c, lane, hist_p);
     else
@@ -359,41 +361,93 @@ __device__ __forceinline__ void generic_rows(const int r0, const int r1, CRowRec
   }
 }
 
+// hp = 2 hp + (1 on the lanes of mask): one add with the mask as carry-in.
+__device__ __forceinline__ uint32_t shift_in(uint32_t hp, uint64_t mask) {
+  uint32_t r;
+  uint64_t carry_out;
+  asm("v_addc_co_u32_e64 %0, %1, %2, %2, %3" : "=v"(r), "=s"(carry_out) : "v"(hp), "s"(mask));
+  return r;
+}
+
+// A steady row: no ends-free edge or recalculation, every active cell
+// inside both sequences and off its pair's outer columns, with an active
+// diagonal predecessor.  nw_row reduced to what such a row can reach:
+//  - act (the schedule's lane mask) sends inactive lanes to fill, and their
+//    pointer bits are never read: no path of such a row enters them;
+//  - without CLAMP every candidate is an active cell's score or fill plus
+//    one increment, inside i16 by the launch's score bound, so the clamps
+//    are the identity and the gap is added once, after max(left, up).
+// The pointer (dploop precedence, nwalign.rs:700-737; a diagonal over a
+// mismatch is 0) is shifted into hp as two bits.
+template <bool EVEN, bool SWAP, bool CLAMP>
+__device__ __forceinline__ void steady_cell(const int ca, const int cb, const uint64_t act,
+                                            LaneState& ls, const RowConst K, uint32_t& hp) {
+  const bool eq = ca == cb;
+  int dg = ls.prev2 + (eq ? K.match : K.mismatch);
+  const int sh = EVEN ? shift_from_lower(ls.prev) : shift_from_upper(ls.prev);
+  const int lraw = EVEN ? sh : ls.prev;
+  const int uraw = EVEN ? ls.prev : sh;
+  bool upf;
+  int m;
+  if (CLAMP) {
+    dg = clamp16(dg);
+    const int l = clamp16(lraw + K.gap), u = clamp16(uraw + K.gap);
+    upf = SWAP ? (l < u) : (u >= l);
+    m = l > u ? l : u;
+  } else {
+    upf = SWAP ? (lraw < uraw) : (uraw >= lraw);
+    m = (lraw > uraw ? lraw : uraw) + K.gap;
+  }
+  const bool dw = dg > m;
+  const int val = dw ? dg : m;
+  ls.prev2 = ls.prev;
+  ls.prev = __builtin_amdgcn_inverse_ballot_w64(act) ? val : K.fill;
+  // pointer bits as lane masks: hi = !dw, lo = dw ? eq : upf
+  const uint64_t mdw = __builtin_amdgcn_ballot_w64(dw);
+  const uint64_t meq = __builtin_amdgcn_ballot_w64(eq);
+  const uint64_t mup = __builtin_amdgcn_ballot_w64(upf);
+  hp = shift_in(hp, ~mdw);
+  hp = shift_in(hp, (mdw & meq) | (~mdw & mup));
+}
+
 // Steady rows [b0, b1), both multiples of 16: one history word per 16-row
-// block, fully unrolled, pipelined like generic_rows.
-template <bool SWAP>
+// block, fully unrolled.  Within a run the s1 offset advances after odd rows
+// and the s2 offset after even rows, so row b0 + k reads s1 at a1(b0) + k/2
+// and s2 at a2(b0) + (k+1)/2 (immediate LDS offsets from per-block bases),
+// and the active lanes of every even / odd row are those of row b0 / b0 + 1.
+// Nothing is read from the schedule inside the run.
+template <bool SWAP, bool CLAMP>
 __device__ __forceinline__ void steady_rows(const int b0, const int b1, CRowRec* rec,
                                             LaneState& ls, const RowConst K, const uint8_t* s1l,
-                                            const uint8_t* s2l, const int c, const int lane,
-                                            uint32_t* hist_p) {
+                                            const uint8_t* s2l, const int lane, uint32_t* hist_p) {
   if (b0 >= b1) return;
-  Rec4 cur = load_rec4(rec + 10 * (b0 - 2));
-  Rec4 nxt = load_rec4(rec + 10 * (b0 - 1));
-  int ca = s1l[cur.a1], cb = s2l[cur.a2];
+  CWord* r0 = rec + kRecWords * (b0 - 2);
+  const uint8_t* p1 = s1l + (int)r0[0];
+  const uint8_t* p2 = s2l + (int)r0[1];
+  const uint64_t act_even = ((uint64_t)r0[11] << 32) | r0[10];
+  const uint64_t act_odd = ((uint64_t)r0[kRecWords + 11] << 32) | r0[kRecWords + 10];
   for (int row = b0; row < b1; row += 16) {
     uint32_t hp = 0;
 #pragma unroll
     for (int k = 0; k < 16; ++k) {
-      const int ca1 = s1l[nxt.a1], cb1 = s2l[nxt.a2];
-      const Rec4 nx2 = load_rec4(rec + 10 * (row + k));  // row + k + 2
-      const uint32_t p = (k & 1) ? steady_cell<false, SWAP>(cur, ca, cb, ls, K, c)
-                                 : steady_cell<true, SWAP>(cur, ca, cb, ls, K, c);
-      hp |= p << (2 * k);
-      cur = nxt;
-      nxt = nx2;
-      ca = ca1;
-      cb = cb1;
+      const int ca = p1[k >> 1], cb = p2[(k + 1) >> 1];
+      if (k & 1)
+        steady_cell<false, SWAP, CLAMP>(ca, cb, act_odd, ls, K, hp);
+      else
+        steady_cell<true, SWAP, CLAMP>(ca, cb, act_even, ls, K, hp);
     }
     hist_p[(row >> 4) * 64 + lane] = hp;
+    p1 += 8;
+    p2 += 8;
   }
 }
 
-template <bool SWAP>
+template <bool SWAP, bool CLAMP>
 __device__ __forceinline__ void forward(CRowRec* rec, const int b0, const int b1, LaneState& ls,
                                         const RowConst K, const uint8_t* s1l, const uint8_t* s2l,
                                         const int c, const int lane, uint32_t* hist_p) {
   generic_rows<SWAP>(2, b0, rec, ls, K, s1l, s2l, c, lane, hist_p);
-  steady_rows<SWAP>(b0, b1, rec, ls, K, s1l, s2l, c, lane, hist_p);
+  steady_rows<SWAP, CLAMP>(b0, b1, rec, ls, K, s1l, s2l, lane, hist_p);
   generic_rows<SWAP>(b1, K.last + 1, rec, ls, K, s1l, s2l, c, lane, hist_p);
 }
 
@@ -427,7 +481,7 @@ template <bool EMIT>
 __device__ __forceinline__ void bw_row(const int r, const int k, const int odd, const uint32_t hp,
                                        PathState& ps, const BwConst& B, const NwDev& a) {
   const uint32_t w = (uint32_t)__builtin_amdgcn_ds_bpermute(ps.addr, (int)hp);
-  int pp = (int)((w >> (2 * k)) & 3u);
+  int pp = (int)((w >> (30 - 2 * k)) & 3u);
   const bool on = ps.rc == r;
   const int cc = (ps.addr >> 2) - B.gpos;
   // pointer 0: a mismatching diagonal on an active cell, else the
@@ -435,8 +489,8 @@ __device__ __forceinline__ void bw_row(const int r, const int k, const int odd, 
   if (__builtin_amdgcn_ballot_w64(on && pp == 0)) {
     bool act = false;
     if (r >= 2) {
-      const int rcmin = (int)B.rec[10 * (r - 2) + 2];
-      const uint32_t rn = B.rec[10 * (r - 2) + 3];
+      const int rcmin = (int)B.rec[kRecWords * (r - 2) + 2];
+      const uint32_t rn = B.rec[kRecWords * (r - 2) + 3];
       act = (unsigned)(cc - rcmin) < rn;
     }
     const bool sub = on && pp == 0 && act;
@@ -564,9 +618,9 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(8, 8))) void
       const bool isT = 1 < geo.top_fill_limit && c == start_col;
       ls.prev = (isL || isT) ? K.end_gap : K.fill;
       const int p1 = isT ? 2 : (isL ? 3 : 0);
-      ls.hp = (uint32_t)p1 << 2;  // row 0 holds only 0 pointers
+      ls.hp = (uint32_t)p1;  // rows 0 (only 0 pointers) and 1 shifted in
     }
-    if (last < 2) hist_p[lane] = ls.hp;
+    if (last < 2) hist_p[lane] = ls.hp << 28;
 
     // ---- forward: anti-diagonals 2..len1+len2 ----
     CRowRec* rec = (CRowRec*)(a.sched + (size_t)bin * a.sched_rows);
@@ -574,10 +628,17 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(8, 8))) void
     const int b0 = uni((int)bi.steady_lo), b1 = uni((int)bi.steady_hi);
     const uint8_t* s1l = s1b - c;
     const uint8_t* s2l = s2b + c;
-    if (swap)
-      forward<true>(rec, b0, b1, ls, K, s1l, s2l, c, lane, hist_p);
-    else
-      forward<false>(rec, b0, b1, ls, K, s1l, s2l, c, lane, hist_p);
+    if (a.steady_clamp) {
+      if (swap)
+        forward<true, true>(rec, b0, b1, ls, K, s1l, s2l, c, lane, hist_p);
+      else
+        forward<false, true>(rec, b0, b1, ls, K, s1l, s2l, c, lane, hist_p);
+    } else {
+      if (swap)
+        forward<true, false>(rec, b0, b1, ls, K, s1l, s2l, c, lane, hist_p);
+      else
+        forward<false, false>(rec, b0, b1, ls, K, s1l, s2l, c, lane, hist_p);
+    }
     __syncthreads();
 
     // ---- backward sweep: every pair's traceback (nwalign.rs:1101-1139) ----
@@ -638,6 +699,21 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(8, 8))) void
   }
 }
 
+// Whether a steady row's i16 clamps can change a result for this bin space:
+// a cell (i, j) of a banded alignment scores within [-(i + j) pen, min(i, j)
+// match] (pen the largest penalty), so when that range, widened by one
+// increment, fits i16 every clamp of a real score is the identity (fill plus
+// one increment never saturates by the choice of fill).
+bool steady_needs_clamp(const BinSpace& bs, int match, int mismatch, int gap, int end_gap) {
+  const long long lmax = (long long)bs.lmin + bs.range - 1;
+  long long pen = 0;
+  for (int x : {mismatch, gap, end_gap, match}) pen = std::max(pen, (long long)-x);
+  const long long up = std::max(0, match);
+  const long long lo_bound = (2 * lmax + 1) * pen;
+  const long long hi_bound = (lmax + 1) * up;
+  return lo_bound > 32768 - 64 || hi_bound > 32767 - 64;
+}
+
 hipError_t launch_nw_vec(const NwArgs& s, hipStream_t st) {
   NwDev d;
   d.bins = s.bins;
@@ -652,6 +728,7 @@ hipError_t launch_nw_vec(const NwArgs& s, hipStream_t st) {
   d.mismatch = s.mismatch;
   d.gap = s.gap_p;
   d.end_gap = s.end_gap_p;
+  d.steady_clamp = steady_needs_clamp(s.bins, s.match, s.mismatch, s.gap_p, s.end_gap_p) ? 1 : 0;
   d.ctr = s.ctr;
   d.emit = s.emit_alignment;
   d.sub_pool = s.sub_pool;

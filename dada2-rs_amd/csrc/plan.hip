@@ -149,12 +149,12 @@ hipError_t launch_plan(const BinSpace& bs, const uint32_t* bin_count, uint32_t* 
 __global__ __launch_bounds__(256) void scatter_kernel(const DpTask* unsorted, DpTask* sorted,
                                                       const uint32_t* bin_start, uint32_t* bin_fill,
                                                       const uint32_t* partials, uint32_t count,
+                                                      uint32_t region,
                                                       const GaplessTask* gl_unsorted,
                                                       GaplessTask* gl, const uint32_t* gl_off) {
   const uint32_t r = blockIdx.x;
-  const uint32_t ndp =
-      partials ? partials[4 * r + 3] : min(kScreenRegion, count - r * kScreenRegion);
-  const DpTask* src = unsorted + (size_t)r * kScreenRegion;
+  const uint32_t ndp = partials ? partials[4 * r + 3] : min(region, count - r * region);
+  const DpTask* src = unsorted + (size_t)r * region;
   for (uint32_t i = threadIdx.x; i < ndp; i += blockDim.x) {
     const DpTask t = src[i];
     const uint32_t pos = bin_start[t.bin] + atomicAdd(&bin_fill[t.bin], 1u);
@@ -162,7 +162,7 @@ __global__ __launch_bounds__(256) void scatter_kernel(const DpTask* unsorted, Dp
   }
   if (partials) {
     const uint32_t ngl = partials[4 * r + 2];
-    const GaplessTask* gsrc = gl_unsorted + (size_t)r * kScreenRegion;
+    const GaplessTask* gsrc = gl_unsorted + (size_t)r * region;
     GaplessTask* gdst = gl + gl_off[r];
     for (uint32_t i = threadIdx.x; i < ngl; i += blockDim.x) gdst[i] = gsrc[i];
   }
@@ -170,11 +170,11 @@ __global__ __launch_bounds__(256) void scatter_kernel(const DpTask* unsorted, Dp
 
 hipError_t launch_scatter(const DpTask* unsorted, DpTask* sorted, const uint32_t* bin_start,
                           uint32_t* bin_fill, const uint32_t* partials, uint32_t n_regions,
-                          uint32_t count, const GaplessTask* gl_unsorted, GaplessTask* gl,
-                          const uint32_t* gl_off, hipStream_t s) {
+                          uint32_t region, uint32_t count, const GaplessTask* gl_unsorted,
+                          GaplessTask* gl, const uint32_t* gl_off, hipStream_t s) {
   if (n_regions == 0) return hipSuccess;
   hipLaunchKernelGGL(scatter_kernel, dim3(n_regions), dim3(256), 0, s, unsorted, sorted,
-                     bin_start, bin_fill, partials, count, gl_unsorted, gl, gl_off);
+                     bin_start, bin_fill, partials, count, region, gl_unsorted, gl, gl_off);
   return hipGetLastError();
 }
 

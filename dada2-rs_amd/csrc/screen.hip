@@ -6,25 +6,31 @@
 // shroud) and the first dispatch test of raw_align_dp (nwalign.rs:1324:
 // band == 0 or |kodist - kdist| < f64::EPSILON -> gapless).
 //
-// Layout: a 256-lane workgroup owns 256 length-sorted uniques and CB centers.
-// Query vectors are read coalesced from the interleaved layout once per
-// workgroup; center vectors come through scalar loads from the row layout.
+// Layout: a workgroup owns QB = 64, 128 or 256 length-sorted uniques and 16
+// centers, split over SPLIT groups of QB/64 wavefronts: wavefront group h
+// compares the QB uniques with centers 16/SPLIT * h .. 16/SPLIT * (h+1) - 1.
+// Query
+// vectors are read coalesced from the interleaved layout (the groups of one
+// workgroup read the same lines, so all but the first come from L2); center
+// vectors come through scalar loads from the row layout.  A scalar load is
+// waited for before its words are used (scalar loads return out of order),
+// so a wavefront stalls once per (center, 16-word chunk); splitting the
+// centers over more wavefronts gives the SIMD more of them to interleave.
 // sum(min(a, b)) over the 4^k buckets is (sum a + sum b - sum |a - b|) / 2,
 // with sum |a - b| from v_sad_u8 (four buckets per instruction).
 //
 // Pairs that pass are appended to the block's own gapless or DP list (LDS
-// counters; each block owns kScreenRegion slots of each list) with their
+// counters; each block owns QB x 16 slots of each list) with their
 // (len_center, len_query) bin; per-block counts go to `partials` and the
 // scatter kernel compacts the lists.
 #include <float.h>
 
+#include "d2g_device.h"
 #include "d2g_launch.h"
 
 namespace d2g {
 
-constexpr int kScreenBlock = 256;
-constexpr int kScreenCenters = 16;
-constexpr int kScreenChunk = 16;
+constexpr int kScreenChunk = 16;  // k-mer words per scalar load
 
 struct ScreenDev {
   UniqueDev u;
@@ -89,11 +95,33 @@ __device__ uint32_t u16_min_dot(const UniqueDev& u, uint32_t qa, uint32_t qb) {
   return dot;
 }
 
-static_assert(kScreenBlock * kScreenCenters == kScreenRegion, "one task slot per pair of a block");
+// The query's k-mer words w0 .. w0 + N - 1 (interleaved layout, coalesced).
+template <int N>
+__device__ __forceinline__ void load_chunk(const UniqueDev& u, uint32_t qq, uint32_t w0,
+                                           uint32_t (&q)[N]) {
+#pragma unroll
+  for (int t = 0; t < N; ++t) q[t] = u.kmer_il[(size_t)(w0 + t) * u.npad + qq];
+}
 
-__global__ __launch_bounds__(kScreenBlock) void screen_kernel(ScreenDev a) {
+// sad[c] += sum |q - center c| over words w0 .. w0 + N - 1; the center words
+// are wavefront-uniform (scalar loads).
+template <int CW, int N>
+__device__ __forceinline__ void sad_chunk(const UniqueDev& u, const uint32_t (&cidx)[CW],
+                                          uint32_t w0, const uint32_t (&q)[N], uint32_t (&sad)[CW]) {
+#pragma unroll
+  for (int c = 0; c < CW; ++c) {
+    const uint32_t* cw = u.kmer_row + (size_t)cidx[c] * u.kwords + w0;
+#pragma unroll
+    for (int t = 0; t < N; ++t) sad[c] = __builtin_amdgcn_sad_u8(q[t], cw[t], sad[c]);
+  }
+}
+
+template <int QB, int SPLIT>
+__global__ __launch_bounds__(QB* SPLIT) void screen_kernel(ScreenDev a) {
+  constexpr int CW = (int)kScreenCenters / SPLIT;  // centers per wavefront
+  constexpr uint32_t kRegion = QB * kScreenCenters;  // one task slot per pair of the block
   const UniqueDev& u = a.u;
-  // this block's task lists: kScreenRegion slots each, counted in LDS
+  // this block's task lists: kRegion slots each, counted in LDS
   __shared__ uint32_t s_ngl, s_ndp;
   if (threadIdx.x == 0) {
     s_ngl = 0;
@@ -101,18 +129,20 @@ __global__ __launch_bounds__(kScreenBlock) void screen_kernel(ScreenDev a) {
   }
   __syncthreads();
   const uint32_t blk = blockIdx.y * gridDim.x + blockIdx.x;
-  GaplessTask* gl_region = a.gapless_tasks + (size_t)blk * kScreenRegion;
-  DpTask* dp_region = a.dp_unsorted + (size_t)blk * kScreenRegion;
-  const uint32_t q = blockIdx.x * kScreenBlock + threadIdx.x;
+  GaplessTask* gl_region = a.gapless_tasks + (size_t)blk * kRegion;
+  DpTask* dp_region = a.dp_unsorted + (size_t)blk * kRegion;
+  const uint32_t part = uni(threadIdx.x / QB);
+  const uint32_t q = blockIdx.x * QB + threadIdx.x % QB;
   const bool valid = q < u.n;
   const uint32_t qq = valid ? q : u.n - 1;
-  const uint32_t c0 = blockIdx.y * kScreenCenters;
-  const uint32_t nc = min((uint32_t)kScreenCenters, a.n_centers - c0);
+  const uint32_t c0 = blockIdx.y * kScreenCenters + part * CW;
+  const uint32_t nc = c0 < a.n_centers ? min((uint32_t)CW, a.n_centers - c0) : 0u;
   const uint32_t k = u.k;
 
-  uint32_t cidx[kScreenCenters];
+  // centers past the end repeat the last one (their results are unused)
+  uint32_t cidx[CW];
 #pragma unroll
-  for (int c = 0; c < kScreenCenters; ++c) cidx[c] = a.centers[c0 + ((uint32_t)c < nc ? c : 0)];
+  for (int c = 0; c < CW; ++c) cidx[c] = a.centers[min(c0 + (uint32_t)c, a.n_centers - 1)];
 
   const uint32_t my_len = u.len[qq];
   const uint32_t my_reads = u.reads[qq];
@@ -121,22 +151,27 @@ __global__ __launch_bounds__(kScreenBlock) void screen_kernel(ScreenDev a) {
   const bool my_255 = u.has255[qq] != 0;
 
   // sum |a - b| over the dense vectors, four buckets per v_sad_u8
-  uint32_t sad[kScreenCenters];
+  uint32_t sad[CW];
 #pragma unroll
-  for (int c = 0; c < kScreenCenters; ++c) sad[c] = 0;
-  if (a.use_kmers) {
-    for (uint32_t w0 = 0; w0 < u.kwords; w0 += kScreenChunk) {
-      uint32_t qw[kScreenChunk];
-#pragma unroll
-      for (int t = 0; t < kScreenChunk; ++t) qw[t] = u.kmer_il[(size_t)(w0 + t) * u.npad + qq];
-#pragma unroll
-      for (int c = 0; c < kScreenCenters; ++c) {
-        if ((uint32_t)c < nc) {
-          const uint32_t* cw = u.kmer_row + (size_t)cidx[c] * u.kwords + w0;
-#pragma unroll
-          for (int t = 0; t < kScreenChunk; ++t) sad[c] = __builtin_amdgcn_sad_u8(qw[t], cw[t], sad[c]);
-        }
-      }
+  for (int c = 0; c < CW; ++c) sad[c] = 0;
+  if (a.use_kmers && nc > 0) {
+    // Ping-pong over two chunk buffers: the query words of the next chunk
+    // are requested before this chunk's sums (vector loads complete in
+    // order, so the wait for this chunk leaves the next one in flight).  A
+    // request past the last chunk repeats the last chunk rather than branch.
+    const uint32_t last = u.kwords - kScreenChunk;
+    uint32_t qa[kScreenChunk], qb[kScreenChunk];
+    // (sched_barrier keeps the scheduler from sinking the requests below
+    // the sums)
+    load_chunk<kScreenChunk>(u, qq, 0, qa);
+    for (uint32_t w0 = 0; w0 < u.kwords; w0 += 2 * kScreenChunk) {
+      load_chunk<kScreenChunk>(u, qq, min(w0 + kScreenChunk, last), qb);
+      __builtin_amdgcn_sched_barrier(0);
+      sad_chunk<CW, kScreenChunk>(u, cidx, w0, qa, sad);
+      __builtin_amdgcn_sched_barrier(0);
+      load_chunk<kScreenChunk>(u, qq, min(w0 + 2 * kScreenChunk, last), qa);
+      __builtin_amdgcn_sched_barrier(0);
+      if (w0 + kScreenChunk < u.kwords) sad_chunk<CW, kScreenChunk>(u, cidx, w0 + kScreenChunk, qb, sad);
     }
   }
 
@@ -225,7 +260,7 @@ __global__ __launch_bounds__(kScreenBlock) void screen_kernel(ScreenDev a) {
   }
   // block totals of skipped / shrouded pairs, summed by the plan kernel
   // (no device-scope atomics on shared counter words)
-  __shared__ uint32_t s_cnt[2][kScreenBlock / 64];
+  __shared__ uint32_t s_cnt[2][QB * SPLIT / 64];
   uint32_t s1 = n_skip, s2 = n_shroud;
   for (int off = 32; off > 0; off >>= 1) {
     s1 += __shfl_xor(s1, off);
@@ -238,7 +273,7 @@ __global__ __launch_bounds__(kScreenBlock) void screen_kernel(ScreenDev a) {
   __syncthreads();
   if (threadIdx.x == 0) {
     uint32_t t1 = 0, t2 = 0;
-    for (int w = 0; w < kScreenBlock / 64; ++w) {
+    for (int w = 0; w < QB * SPLIT / 64; ++w) {
       t1 += s_cnt[0][w];
       t2 += s_cnt[1][w];
     }
@@ -249,9 +284,17 @@ __global__ __launch_bounds__(kScreenBlock) void screen_kernel(ScreenDev a) {
   }
 }
 
-uint32_t screen_blocks(uint32_t n_uniques, uint32_t n_centers) {
-  return ((n_uniques + kScreenBlock - 1) / kScreenBlock) *
-         ((n_centers + kScreenCenters - 1) / kScreenCenters);
+uint32_t screen_blocks(uint32_t n_uniques, uint32_t n_centers, uint32_t qblock) {
+  return ((n_uniques + qblock - 1) / qblock) * ((n_centers + kScreenCenters - 1) / kScreenCenters);
+}
+
+template <int QB>
+static void launch_qb(const ScreenDev& d, dim3 grid, int split, hipStream_t st) {
+  switch (split) {
+    case 1: hipLaunchKernelGGL((screen_kernel<QB, 1>), grid, dim3(QB), 0, st, d); break;
+    case 2: hipLaunchKernelGGL((screen_kernel<QB, 2>), grid, dim3(QB * 2), 0, st, d); break;
+    default: hipLaunchKernelGGL((screen_kernel<QB, 4>), grid, dim3(QB * 4), 0, st, d); break;
+  }
 }
 
 hipError_t launch_screen(const ScreenArgs& s, hipStream_t st) {
@@ -274,9 +317,15 @@ hipError_t launch_screen(const ScreenArgs& s, hipStream_t st) {
   d.bin_count = s.bin_count;
   d.partials = s.partials;
   d.ctr = s.ctr;
-  dim3 grid((d.u.n + kScreenBlock - 1) / kScreenBlock,
-            (s.n_centers + kScreenCenters - 1) / kScreenCenters);
-  hipLaunchKernelGGL(screen_kernel, grid, dim3(kScreenBlock), 0, st, d);
+  if (s.split != 1 && s.split != 2 && s.split != 4) return hipErrorInvalidValue;
+  const dim3 grid((d.u.n + s.qblock - 1) / s.qblock,
+                  (s.n_centers + kScreenCenters - 1) / kScreenCenters);
+  switch (s.qblock) {
+    case 64: launch_qb<64>(d, grid, s.split, st); break;
+    case 128: launch_qb<128>(d, grid, s.split, st); break;
+    case 256: launch_qb<256>(d, grid, s.split, st); break;
+    default: return hipErrorInvalidValue;
+  }
   return hipGetLastError();
 }
 

@@ -89,6 +89,11 @@ int d2g_ctx_set_timing(d2g_ctx* ctx, int enable);
  * shape (centers count, bin space, parameters, buffers) (default on; timed
  * calls always launch eagerly). */
 int d2g_ctx_set_graphs(d2g_ctx* ctx, int enable);
+/* Launch shape of the k-mer screen: a workgroup compares qblock = 64, 128
+ * or 256 uniques with 16 centers, split over split = 1, 2 or 4 wavefront
+ * groups (16/split centers each).  Results do not depend on it (default
+ * 256, 2).  No counterpart in the reference. */
+int d2g_ctx_set_screen_shape(d2g_ctx* ctx, int qblock, int split);
 /* The hipStream_t every kernel of this context is launched on. */
 void* d2g_ctx_stream(d2g_ctx* ctx);
 /* Device properties used for roofline figures. */

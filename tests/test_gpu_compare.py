@@ -119,3 +119,20 @@ def test_graph_replays_match_eager(pkg, oracle, batched_small):
     finally:
         g.close()
         e.close()
+
+
+@pytest.mark.parametrize("qblock,split", [(256, 1), (256, 4), (128, 2), (64, 1), (64, 4)])
+def test_screen_shapes_same_results(ctx, oracle, pkg, batched_small, qblock, split):
+    """Each launch shape of the screen (64, 128 or 256 uniques x 16 centers
+    per workgroup, over 1, 2 or 4 wavefront groups) gives the oracle's
+    results; 21 centers leave the last workgroup's groups partly or wholly
+    without centers."""
+    w = batched_small
+    ctx.load_uniques(w.seqs, w.quals, w.reads, k=5)
+    cs = w.centers[:21]
+    ctx.set_screen_shape(qblock, split)
+    try:
+        got = ctx.compare_batch(cs, w.err)
+    finally:
+        ctx.set_screen_shape()
+    _assert_same(got, *_oracle_rows(oracle, w, cs))
