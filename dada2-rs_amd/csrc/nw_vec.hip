@@ -481,12 +481,13 @@ template <bool EMIT>
 __device__ __forceinline__ void bw_row(const int r, const int k, const int odd, const uint32_t hp,
                                        PathState& ps, const BwConst& B, const NwDev& a) {
   const uint32_t w = (uint32_t)__builtin_amdgcn_ds_bpermute(ps.addr, (int)hp);
-  int pp = (int)((w >> (30 - 2 * k)) & 3u);
-  const bool on = ps.rc == r;
-  const int cc = (ps.addr >> 2) - B.gpos;
+  int pp = (int)__builtin_amdgcn_ubfe(w, (uint32_t)(30 - 2 * k), 2u);
+  bool on = ps.rc == r;
   // pointer 0: a mismatching diagonal on an active cell, else the
-  // reference's invalid pointer (row 1 holds no active cells)
+  // reference's invalid pointer (row 1 holds no active cells), which ends
+  // the path here
   if (__builtin_amdgcn_ballot_w64(on && pp == 0)) {
+    const int cc = (ps.addr >> 2) - B.gpos;
     bool act = false;
     if (r >= 2) {
       const int rcmin = (int)B.rec[kRecWords * (r - 2) + 2];
@@ -503,9 +504,14 @@ __device__ __forceinline__ void bw_row(const int r, const int k, const int odd, 
     }
     ps.ns += sub ? 1 : 0;
     pp = sub ? 1 : pp;
+    const bool invalid = on && pp == 0;
+    ps.bad = ps.bad || invalid;
+    ps.rc = invalid ? -1 : ps.rc;
+    on = on && !invalid;
   }
   if (EMIT) {
-    if (on && pp != 0 && B.c == cc && B.real) {
+    const int cc = (ps.addr >> 2) - B.gpos;
+    if (on && B.c == cc && B.real) {
       const int dd = 2 * (cc - B.start_col) + odd;
       const int i = (r - dd) >> 1, j = (r + dd) >> 1;
       const uint8_t x0 = pp == 2 ? (uint8_t)'-' : B.s1b[i - 1];
@@ -513,16 +519,16 @@ __device__ __forceinline__ void bw_row(const int r, const int k, const int odd, 
       a.al0[B.ao + ps.nt] = B.swap ? x1 : x0;
       a.al1[B.ao + ps.nt] = B.swap ? x0 : x1;
     }
-    ps.nt += (on && pp != 0) ? 1 : 0;
+    ps.nt += on ? 1 : 0;
   }
-  ps.bad = ps.bad || (on && pp == 0);
-  // step: diag (i-1, j-1) two rows up; left (i, j-1) and up (i-1, j) one row
-  // up, i.e. one column down on even rows / up on odd rows
+  // step (pointer 1, 2 or 3 where on): diag (i-1, j-1) two rows up; left
+  // (i, j-1) and up (i-1, j) one row up, i.e. one column down on even rows
+  // / up on odd rows
   if (odd)
     ps.addr += (on && pp == 3) ? 4 : 0;
   else
     ps.addr -= (on && pp == 2) ? 4 : 0;
-  ps.rc = on ? (pp == 1 ? r - 2 : (pp == 0 ? -1 : r - 1)) : ps.rc;
+  ps.rc = on ? r - 1 - (pp == 1 ? 1 : 0) : ps.rc;
 }
 
 // Rows last..1 in 16-row history blocks; each block's words are loaded one
