@@ -332,6 +332,8 @@ int d2g_load_uniques(d2g_ctx* c, uint32_t n, const uint8_t* seq_concat, const ui
     soff[s + 1] = soff[s] + ((slen[s] + 15u) & ~15u);  // 16-byte aligned rows
   }
   const uint64_t total = soff[n];
+  // kernels address the padded pool with 32-bit byte offsets
+  if (total >= ((uint64_t)1 << 32)) return D2G_ERR_UNSUPPORTED;
   std::vector<uint8_t> sseq(total, 0), squal(qual_concat ? total : 0, 0);
   for (uint32_t s = 0; s < n; ++s) {
     std::memcpy(&sseq[soff[s]], seq_concat + offsets[order[s]], slen[s]);
@@ -659,7 +661,7 @@ static int compare_device(d2g_ctx* c, const uint32_t* centers, uint32_t nc, cons
     la.out_lambda = c->out_lambda.p;
     la.out_hamming = c->out_hamming.p;
     la.out_skipped = c->out_skipped.p;
-    la.grid = (uint32_t)std::min<size_t>((size_t)c->cu_count * 4, (npairs + 255) / 256);
+    la.grid = (uint32_t)std::min<size_t>((size_t)c->cu_count * 3, (npairs + 63) / 64);  // 64 pairs each
     HIP_TRY(launch_lambda(la, s));
     if (timed) HIP_TRY(hipEventRecord(c->ev[4], s));
     return D2G_OK;
@@ -718,6 +720,9 @@ static int compare_device(d2g_ctx* c, const uint32_t* centers, uint32_t nc, cons
     if (h.err_flags & kErrSubOverflow) {
       // every DP pair owns sub_stride records
       c->sub_capacity = std::max(c->sub_capacity * 2, (size_t)h.n_dp * sub_stride);
+      // the lambda kernel addresses the pool with 32-bit word offsets
+      if ((size_t)h.n_dp * sub_stride >= ((size_t)1 << 32)) return D2G_ERR_UNSUPPORTED;
+      c->sub_capacity = std::min(c->sub_capacity, ((size_t)1 << 32) - 1);
       HIP_TRY(c->sub_pool.ensure(c->sub_capacity));
       continue;
     }

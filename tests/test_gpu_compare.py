@@ -136,3 +136,21 @@ def test_screen_shapes_same_results(ctx, oracle, pkg, batched_small, qblock, spl
     finally:
         ctx.set_screen_shape()
     _assert_same(got, *_oracle_rows(oracle, w, cs))
+
+
+def test_first_call_outgrows_substitution_pool(pkg, oracle):
+    """A fresh context starts with a small substitution pool; a first
+    compare with more DP pairs than it holds (cutoff 1.0: every pair aligns)
+    is rerun after the pool grows.  Kernels of the overflowing pass must stay
+    inside the pool, and the results must be the oracle's."""
+    w = synth.subsample(synth.workload_batched(), 2000, n_centers=12)
+    print("case: 2000 uniques x 12 centers, kdist_cutoff 1.0", flush=True)
+    c = pkg.Context(0)
+    try:
+        c.load_uniques(w.seqs, w.quals, w.reads, k=5)
+        p = pkg.align_params(kdist_cutoff=1.0)
+        got = c.compare_batch(w.centers, w.err, params=p)
+        assert got.stats["aligned_dp"] * 262 > (1 << 20)
+        _assert_same(got, *_oracle_rows(oracle, w, w.centers, params=oracle.make_params(kdist_cutoff=1.0)))
+    finally:
+        c.close()
