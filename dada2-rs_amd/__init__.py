@@ -122,7 +122,7 @@ def _lib():
             "d2g_loglik_backward": (C.c_int, [vp, C.POINTER(C.c_float), f64p,
                                               C.POINTER(C.c_float)]),
             "d2g_ctx_set_graphs": (C.c_int, [vp, C.c_int]),
-            "d2g_ctx_set_screen_shape": (C.c_int, [vp, C.c_int, C.c_int]),
+            "d2g_ctx_set_screen_shape": (C.c_int, [vp, C.c_int, C.c_int, C.c_int]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -237,10 +237,11 @@ class Context:
         """Replay the compare's launches as a captured HIP graph (default on)."""
         _check(_lib().d2g_ctx_set_graphs(self.h, int(on)), "d2g_ctx_set_graphs")
 
-    def set_screen_shape(self, qblock: int = 256, split: int = 2):
-        """Screen launch shape: uniques per workgroup (64, 128, 256) and the
-        wavefront groups (1, 2, 4) sharing its 16 centers."""
-        _check(_lib().d2g_ctx_set_screen_shape(self.h, int(qblock), int(split)),
+    def set_screen_shape(self, qblock: int = 64, split: int = 4, per_lane: int = 1):
+        """Screen launch shape: uniques per workgroup (64, 128, 256), the
+        wavefront groups (1, 2, 4) sharing its 16 centers, and uniques per
+        lane (1, 2)."""
+        _check(_lib().d2g_ctx_set_screen_shape(self.h, int(qblock), int(split), int(per_lane)),
                "d2g_ctx_set_screen_shape")
 
     def device_info(self) -> dict:

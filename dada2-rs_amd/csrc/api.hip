@@ -100,8 +100,14 @@ struct d2g_ctx {
   bool graphs = true;
   // screen_kernel's launch shape: uniques per workgroup, and wavefront
   // groups sharing its 16 centers
-  uint32_t screen_qblock = 256;
-  int screen_split = 2;
+  uint32_t screen_qblock = 64;
+  int screen_split = 4;
+  int screen_per_lane = 1;
+  // the screen's shroud thresholds, for one (cutoff, k, longest unique)
+  DevBuf<uint32_t> dstar;
+  uint64_t dstar_cut = 0;
+  uint32_t dstar_n = 0;
+  bool dstar_valid = false;
   hipGraphExec_t graph_exec = nullptr;
   std::vector<uint64_t> graph_key;
   // pinned host staging for per-call uploads (centers, lock flags) and the
@@ -231,11 +237,13 @@ int d2g_ctx_set_graphs(d2g_ctx* c, int enable) {
   return D2G_OK;
 }
 
-int d2g_ctx_set_screen_shape(d2g_ctx* c, int qblock, int split) {
+int d2g_ctx_set_screen_shape(d2g_ctx* c, int qblock, int split, int per_lane) {
   if (!c || (split != 1 && split != 2 && split != 4)) return D2G_ERR_ARG;
   if (qblock != 64 && qblock != 128 && qblock != 256) return D2G_ERR_ARG;
+  if (per_lane != 1 && !(per_lane == 2 && qblock >= 128)) return D2G_ERR_ARG;
   c->screen_qblock = (uint32_t)qblock;
   c->screen_split = split;
+  c->screen_per_lane = per_lane;
   return D2G_OK;
 }
 
@@ -580,6 +588,23 @@ static int compare_device(d2g_ctx* c, const uint32_t* centers, uint32_t nc, cons
     HIP_TRY(hipMemcpyAsync(c->err.p, c->h_err.data(), ne * sizeof(double), hipMemcpyHostToDevice, s));
     HIP_TRY(hipStreamSynchronize(s));  // h_err is pageable: finish before it can change
   }
+  // shroud thresholds of the screen (cached per cutoff and longest unique)
+  if (p->use_kmers) {
+    uint64_t cut;
+    std::memcpy(&cut, &p->kdist_cutoff, 8);
+    const uint32_t nmax = u.maxlen - u.k + 1;
+    if (!c->dstar_valid || c->dstar_cut != cut || c->dstar_n != nmax) {
+      std::vector<uint32_t> t(nmax + 1);
+      screen_thresholds(p->kdist_cutoff, nmax, t.data());
+      HIP_TRY(c->dstar.ensure(nmax + 1));
+      HIP_TRY(hipMemcpyAsync(c->dstar.p, t.data(), (nmax + 1) * sizeof(uint32_t),
+                             hipMemcpyHostToDevice, s));
+      HIP_TRY(hipStreamSynchronize(s));  // t is pageable
+      c->dstar_cut = cut;
+      c->dstar_n = nmax;
+      c->dstar_valid = true;
+    }
+  }
 
   // The launch sequence of one compare (screen -> plan -> scatter -> DP ->
   // lambda).  Timed runs bracket the kernels with events; otherwise the
@@ -609,6 +634,8 @@ static int compare_device(d2g_ctx* c, const uint32_t* centers, uint32_t nc, cons
     sa.ctr = c->ctr.p;
     sa.qblock = c->screen_qblock;
     sa.split = c->screen_split;
+    sa.per_lane = c->screen_per_lane;
+    sa.dstar = c->dstar.p;
     HIP_TRY(launch_screen(sa, s));
     if (timed) HIP_TRY(hipEventRecord(c->ev[1], s));
     HIP_TRY(launch_plan(bs, c->bin_count.p, c->bin_start.p, c->wave_start.p, c->bin_fill.p, c->ctr.p,
@@ -678,7 +705,8 @@ static int compare_device(d2g_ctx* c, const uint32_t* centers, uint32_t nc, cons
           (uint64_t)p->match_score, (uint64_t)(uint32_t)p->mismatch, (uint64_t)(uint32_t)p->gap_p,
           (uint64_t)(uint32_t)p->homo_gap_p, (uint64_t)p->kmer_size, sub_stride, c->sub_capacity,
           c->nw_grid, c->lds_bytes, c->hist_stride, (uint64_t)c->sched_rows,
-          (uint64_t)c->screen_split, (uint64_t)c->screen_qblock};
+          (uint64_t)c->screen_split, (uint64_t)c->screen_qblock, (uint64_t)c->screen_per_lane,
+          (uint64_t)(uintptr_t)c->dstar.p};
       uint64_t cut;
       std::memcpy(&cut, &p->kdist_cutoff, 8);
       key.push_back(cut);

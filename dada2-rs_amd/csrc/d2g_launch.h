@@ -55,9 +55,15 @@ struct ScreenArgs {
   uint32_t* bin_count;
   uint32_t* partials;  // per block: skipped, shrouded, gapless, DP (screen_blocks entries)
   Counters* ctr;
-  uint32_t qblock = 256;  // uniques per workgroup: 64, 128 or 256
-  int split = 2;          // wavefront groups per workgroup sharing its 16 centers: 1, 2 or 4
+  uint32_t qblock = 64;   // uniques per workgroup: 64, 128 or 256
+  int split = 4;          // wavefront groups per workgroup sharing its 16 centers: 1, 2 or 4
+  int per_lane = 1;       // uniques per lane: 1, or 2 (qblock 128 or 256)
+  const uint32_t* dstar = nullptr;  // [maxlen - k + 2]: shroud iff dot < dstar[n] (use_kmers)
 };
+// dstar[n] for n = 0 .. nmax: the smallest dot in 0 .. n with
+// 1.0 - (double)dot / (double)n <= cutoff in f64 (n + 1 when there is none);
+// dstar[0] = 0.
+void screen_thresholds(double cutoff, uint32_t nmax, uint32_t* out);
 hipError_t launch_screen(const ScreenArgs& a, hipStream_t s);
 // Workgroups of the screen grid; each owns screen_region(qblock) task slots
 // in each list.

@@ -7,17 +7,23 @@
 // band == 0 or |kodist - kdist| < f64::EPSILON -> gapless).
 //
 // Layout: a workgroup owns QB = 64, 128 or 256 length-sorted uniques and 16
-// centers, split over SPLIT groups of QB/64 wavefronts: wavefront group h
-// compares the QB uniques with centers 16/SPLIT * h .. 16/SPLIT * (h+1) - 1.
-// Query
+// centers, split over SPLIT groups of QB/(64 QL) wavefronts: wavefront group
+// h compares the QB uniques with centers 16/SPLIT * h .. 16/SPLIT * (h+1) - 1,
+// each lane taking QL = 1 or 2 uniques (64 apart).  Query
 // vectors are read coalesced from the interleaved layout (the groups of one
 // workgroup read the same lines, so all but the first come from L2); center
 // vectors come through scalar loads from the row layout.  A scalar load is
 // waited for before its words are used (scalar loads return out of order),
 // so a wavefront stalls once per (center, 16-word chunk); splitting the
-// centers over more wavefronts gives the SIMD more of them to interleave.
+// centers over more wavefronts gives the SIMD more of them to interleave,
+// and a second unique per lane gives each stall twice the work.
 // sum(min(a, b)) over the 4^k buckets is (sum a + sum b - sum |a - b|) / 2,
 // with sum |a - b| from v_sad_u8 (four buckets per instruction).
+//
+// The shroud test kdist > cutoff, kdist = 1 - dot / n in f64 (n = min(len)
+// - k + 1), is non-increasing in dot, so it is dot < dstar[n] with dstar
+// tabulated on the host from the same f64 expression; kdist itself is
+// formed only for the pairs that pass (for the gapless dispatch).
 //
 // Pairs that pass are appended to the block's own gapless or DP list (LDS
 // counters; each block owns QB x 16 slots of each list) with their
@@ -34,6 +40,7 @@ constexpr int kScreenChunk = 16;  // k-mer words per scalar load
 
 struct ScreenDev {
   UniqueDev u;
+  const uint32_t* dstar;  // [n]: smallest dot with kdist <= cutoff
   const uint32_t* centers;
   uint32_t n_centers;
   const uint8_t* lock;
@@ -95,30 +102,23 @@ __device__ uint32_t u16_min_dot(const UniqueDev& u, uint32_t qa, uint32_t qb) {
   return dot;
 }
 
-// The query's k-mer words w0 .. w0 + N - 1 (interleaved layout, coalesced).
-template <int N>
-__device__ __forceinline__ void load_chunk(const UniqueDev& u, uint32_t qq, uint32_t w0,
-                                           uint32_t (&q)[N]) {
-#pragma unroll
-  for (int t = 0; t < N; ++t) q[t] = u.kmer_il[(size_t)(w0 + t) * u.npad + qq];
-}
+// Per-unique fields a lane needs for each of its uniques.
+struct QueryFields {
+  uint32_t qq;   // the unique (sorted index; idle lanes repeat the last)
+  bool valid;
+  uint32_t len, reads, sum, thr;  // thr = dstar[len - k + 1]
+  bool lock, has255;
+};
+// Per-center fields, wavefront-uniform.
+struct CenterFields {
+  uint32_t ci, len, reads, sum, thr;
+  bool has255;
+};
 
-// sad[c] += sum |q - center c| over words w0 .. w0 + N - 1; the center words
-// are wavefront-uniform (scalar loads).
-template <int CW, int N>
-__device__ __forceinline__ void sad_chunk(const UniqueDev& u, const uint32_t (&cidx)[CW],
-                                          uint32_t w0, const uint32_t (&q)[N], uint32_t (&sad)[CW]) {
-#pragma unroll
-  for (int c = 0; c < CW; ++c) {
-    const uint32_t* cw = u.kmer_row + (size_t)cidx[c] * u.kwords + w0;
-#pragma unroll
-    for (int t = 0; t < N; ++t) sad[c] = __builtin_amdgcn_sad_u8(q[t], cw[t], sad[c]);
-  }
-}
-
-template <int QB, int SPLIT>
-__global__ __launch_bounds__(QB* SPLIT) void screen_kernel(ScreenDev a) {
-  constexpr int CW = (int)kScreenCenters / SPLIT;  // centers per wavefront
+template <int QB, int SPLIT, int QL>
+__global__ __launch_bounds__(QB / QL * SPLIT) void screen_kernel(ScreenDev a) {
+  constexpr int CW = (int)kScreenCenters / SPLIT;    // centers per wavefront
+  constexpr int GT = QB / QL;                          // threads per wavefront group
   constexpr uint32_t kRegion = QB * kScreenCenters;  // one task slot per pair of the block
   const UniqueDev& u = a.u;
   // this block's task lists: kRegion slots each, counted in LDS
@@ -131,136 +131,167 @@ __global__ __launch_bounds__(QB* SPLIT) void screen_kernel(ScreenDev a) {
   const uint32_t blk = blockIdx.y * gridDim.x + blockIdx.x;
   GaplessTask* gl_region = a.gapless_tasks + (size_t)blk * kRegion;
   DpTask* dp_region = a.dp_unsorted + (size_t)blk * kRegion;
-  const uint32_t part = uni(threadIdx.x / QB);
-  const uint32_t q = blockIdx.x * QB + threadIdx.x % QB;
-  const bool valid = q < u.n;
-  const uint32_t qq = valid ? q : u.n - 1;
+  const uint32_t part = uni(threadIdx.x / GT);
+  const uint32_t in_group = threadIdx.x % GT;
+  const uint32_t wave_base = blockIdx.x * QB + (in_group / 64u) * 64u * QL;
+  const uint32_t lane = lane_id();
   const uint32_t c0 = blockIdx.y * kScreenCenters + part * CW;
   const uint32_t nc = c0 < a.n_centers ? min((uint32_t)CW, a.n_centers - c0) : 0u;
   const uint32_t k = u.k;
 
-  // centers past the end repeat the last one (their results are unused)
-  uint32_t cidx[CW];
+  // center fields, requested together (centers past the end repeat the
+  // last one; their results are unused)
+  CenterFields cf[CW];
 #pragma unroll
-  for (int c = 0; c < CW; ++c) cidx[c] = a.centers[min(c0 + (uint32_t)c, a.n_centers - 1)];
-
-  const uint32_t my_len = u.len[qq];
-  const uint32_t my_reads = u.reads[qq];
-  const bool my_lock = a.lock ? a.lock[qq] != 0 : false;
-  const uint32_t my_sum = u.ksum[qq];
-  const bool my_255 = u.has255[qq] != 0;
+  for (int c = 0; c < CW; ++c) cf[c].ci = a.centers[min(c0 + (uint32_t)c, a.n_centers - 1)];
+#pragma unroll
+  for (int c = 0; c < CW; ++c) {
+    cf[c].len = u.len[cf[c].ci];
+    cf[c].reads = u.reads[cf[c].ci];
+    cf[c].sum = u.ksum[cf[c].ci];
+    cf[c].has255 = u.has255[cf[c].ci] != 0;
+  }
+  QueryFields qf[QL];
+#pragma unroll
+  for (int j = 0; j < QL; ++j) {
+    const uint32_t q = wave_base + 64u * (uint32_t)j + lane;
+    qf[j].valid = q < u.n;
+    qf[j].qq = qf[j].valid ? q : u.n - 1;
+    qf[j].len = u.len[qf[j].qq];
+    qf[j].reads = u.reads[qf[j].qq];
+    qf[j].lock = a.lock ? a.lock[qf[j].qq] != 0 : false;
+    qf[j].sum = u.ksum[qf[j].qq];
+    qf[j].has255 = u.has255[qf[j].qq] != 0;
+  }
+  if (a.use_kmers) {
+#pragma unroll
+    for (int c = 0; c < CW; ++c) cf[c].thr = a.dstar[cf[c].len - k + 1];
+#pragma unroll
+    for (int j = 0; j < QL; ++j) qf[j].thr = a.dstar[qf[j].len - k + 1];
+  }
 
   // sum |a - b| over the dense vectors, four buckets per v_sad_u8
-  uint32_t sad[CW];
+  uint32_t sad[QL][CW];
 #pragma unroll
-  for (int c = 0; c < CW; ++c) sad[c] = 0;
+  for (int j = 0; j < QL; ++j)
+#pragma unroll
+    for (int c = 0; c < CW; ++c) sad[j][c] = 0;
   if (a.use_kmers && nc > 0) {
-    // Ping-pong over two chunk buffers: the query words of the next chunk
-    // are requested before this chunk's sums (vector loads complete in
-    // order, so the wait for this chunk leaves the next one in flight).  A
-    // request past the last chunk repeats the last chunk rather than branch.
-    const uint32_t last = u.kwords - kScreenChunk;
-    uint32_t qa[kScreenChunk], qb[kScreenChunk];
-    // (sched_barrier keeps the scheduler from sinking the requests below
-    // the sums)
-    load_chunk<kScreenChunk>(u, qq, 0, qa);
-    for (uint32_t w0 = 0; w0 < u.kwords; w0 += 2 * kScreenChunk) {
-      load_chunk<kScreenChunk>(u, qq, min(w0 + kScreenChunk, last), qb);
-      __builtin_amdgcn_sched_barrier(0);
-      sad_chunk<CW, kScreenChunk>(u, cidx, w0, qa, sad);
-      __builtin_amdgcn_sched_barrier(0);
-      load_chunk<kScreenChunk>(u, qq, min(w0 + 2 * kScreenChunk, last), qa);
-      __builtin_amdgcn_sched_barrier(0);
-      if (w0 + kScreenChunk < u.kwords) sad_chunk<CW, kScreenChunk>(u, cidx, w0 + kScreenChunk, qb, sad);
+    for (uint32_t w0 = 0; w0 < u.kwords; w0 += kScreenChunk) {
+      uint32_t qw[QL][kScreenChunk];
+#pragma unroll
+      for (int j = 0; j < QL; ++j)
+#pragma unroll
+        for (int t = 0; t < kScreenChunk; ++t)
+          qw[j][t] = u.kmer_il[(size_t)(w0 + t) * u.npad + qf[j].qq];
+#pragma unroll
+      for (int c = 0; c < CW; ++c) {
+        const uint32_t* cw = u.kmer_row + (size_t)cf[c].ci * u.kwords + w0;
+#pragma unroll
+        for (int t = 0; t < kScreenChunk; ++t)
+#pragma unroll
+          for (int j = 0; j < QL; ++j) sad[j][c] = __builtin_amdgcn_sad_u8(qw[j][t], cw[t], sad[j][c]);
+      }
     }
   }
 
   uint32_t n_skip = 0, n_shroud = 0;
   for (uint32_t c = 0; c < nc; ++c) {
-    const uint32_t ci = cidx[c];
-    const uint32_t clen = u.len[ci];
-    const uint32_t creads = u.reads[ci];
-    // results stay in the length-sorted order (coalesced); d2g_fetch_results
-    // restores the caller's unique order
-    const uint32_t out_idx = (c0 + c) * u.n + qq;
-    // cluster.rs:295: greedy skip
-    const bool skip = a.greedy && (my_reads > creads || my_lock);
-    double kdist = 0.0;
-    if (a.use_kmers) {
-      uint32_t dot = (my_sum + u.ksum[ci] - sad[c]) >> 1;
-      if (my_255 && u.has255[ci] && !skip) {
-        // exact kmer_dist8 (kmers.rs:254-269): overflow when min == 255
-        bool ovf = false;
-        uint32_t d2 = 0;
-        for (uint32_t w = 0; w < u.kwords; ++w) {
-          uint32_t x = u.kmer_il[(size_t)w * u.npad + qq];
-          uint32_t y = u.kmer_row[(size_t)ci * u.kwords + w];
-          for (int b = 0; b < 4; ++b) {
-            uint32_t m = min((x >> (8 * b)) & 255u, (y >> (8 * b)) & 255u);
-            ovf |= m == 255;
-            d2 += m;
+    // (the per-center arrays are read with a uniform index: select chains)
+    CenterFields C = cf[0];
+#pragma unroll
+    for (int cc = 1; cc < CW; ++cc)
+      if ((uint32_t)cc == c) C = cf[cc];
+#pragma unroll
+    for (int j = 0; j < QL; ++j) {
+      const QueryFields& Q = qf[j];
+      uint32_t sd = sad[j][0];
+#pragma unroll
+      for (int cc = 1; cc < CW; ++cc) sd = (uint32_t)cc == c ? sad[j][cc] : sd;
+      // results stay in the length-sorted order (coalesced); d2g_fetch_results
+      // restores the caller's unique order
+      const uint32_t out_idx = (c0 + c) * u.n + Q.qq;
+      // cluster.rs:295: greedy skip
+      const bool skip = a.greedy && (Q.reads > C.reads || Q.lock);
+      uint32_t dot = 0;
+      bool shroud = false;
+      if (a.use_kmers) {
+        dot = (Q.sum + C.sum - sd) >> 1;
+        if (Q.has255 && C.has255 && !skip) {
+          // exact kmer_dist8 (kmers.rs:254-269): overflow when min == 255
+          bool ovf = false;
+          uint32_t d2 = 0;
+          for (uint32_t w = 0; w < u.kwords; ++w) {
+            uint32_t x = u.kmer_il[(size_t)w * u.npad + Q.qq];
+            uint32_t y = u.kmer_row[(size_t)C.ci * u.kwords + w];
+            for (int b = 0; b < 4; ++b) {
+              uint32_t m = min((x >> (8 * b)) & 255u, (y >> (8 * b)) & 255u);
+              ovf |= m == 255;
+              d2 += m;
+            }
           }
+          dot = ovf ? u16_min_dot(u, Q.qq, C.ci) : d2;  // nwalign.rs:1270-1278
+          if (ovf && Q.valid) atomicAdd(&a.ctr->n_exact_fallback, 1u);
         }
-        dot = ovf ? u16_min_dot(u, qq, ci) : d2;  // nwalign.rs:1270-1278
-        if (ovf && valid) atomicAdd(&a.ctr->n_exact_fallback, 1u);
+        // kdist > cutoff (raw_align_with_buf, nwalign.rs:1296)
+        shroud = !skip && dot < (Q.len <= C.len ? Q.thr : C.thr);
       }
-      uint32_t mn = my_len < clen ? my_len : clen;
-      kdist = 1.0 - (double)dot / (double)(mn - k + 1);
-    }
-    const bool shroud = !skip && a.use_kmers && kdist > a.kdist_cutoff;
-    bool route_gapless = false;
-    if (!skip && !shroud) {
-      double kodist = -1.0;
-      if (a.use_kmers && a.gapless && my_len == clen) {
-        // kord_dist, kmers.rs:314-328
-        uint32_t klen = my_len - (k - 1);
-        uint32_t eq = 0;
-        uint32_t nw = (klen + 1) / 2;
-        const uint32_t* crow = u.kord_row + (size_t)ci * u.owords;
-        for (uint32_t w = 0; w < nw; ++w) {
-          uint32_t x = u.kord_il[(size_t)w * u.npad + qq] ^ crow[w];
-          eq += (x & 0xFFFFu) == 0;
-          if (2 * w + 1 < klen) eq += (x >> 16) == 0;
+      bool route_gapless = false;
+      if (!skip && !shroud) {
+        const uint32_t mn = Q.len < C.len ? Q.len : C.len;
+        const double kdist = a.use_kmers ? 1.0 - (double)dot / (double)(mn - k + 1) : 0.0;
+        double kodist = -1.0;
+        if (a.use_kmers && a.gapless && Q.len == C.len) {
+          // kord_dist, kmers.rs:314-328
+          uint32_t klen = Q.len - (k - 1);
+          uint32_t eq = 0;
+          uint32_t nw = (klen + 1) / 2;
+          const uint32_t* crow = u.kord_row + (size_t)C.ci * u.owords;
+          for (uint32_t w = 0; w < nw; ++w) {
+            uint32_t x = u.kord_il[(size_t)w * u.npad + Q.qq] ^ crow[w];
+            eq += (x & 0xFFFFu) == 0;
+            if (2 * w + 1 < klen) eq += (x >> 16) == 0;
+          }
+          kodist = 1.0 - (double)eq / (double)klen;
         }
-        kodist = 1.0 - (double)eq / (double)klen;
+        route_gapless = a.band == 0 || fabs(kodist - kdist) < DBL_EPSILON;
       }
-      route_gapless = a.band == 0 || fabs(kodist - kdist) < DBL_EPSILON;
-    }
-    const bool route_dp = !skip && !shroud && !route_gapless;
-    if (valid && (skip || shroud)) {
-      a.out_lambda[out_idx] = 0.0;
-      a.out_hamming[out_idx] = kHammingNone;
-      a.out_skipped[out_idx] = skip ? 1 : 0;
-    }
-    n_skip += valid && skip;
-    n_shroud += valid && shroud;
-    const bool to_gl = valid && route_gapless;
-    const uint32_t gpos = wave_append(&s_ngl, to_gl);
-    if (to_gl) {
-      GaplessTask t;
-      t.c_idx = ci;
-      t.u_idx = qq;
-      t.out_idx = out_idx;
-      t._pad = 0;
-      gl_region[gpos] = t;
-    }
-    const bool to_dp = valid && route_dp;
-    const uint32_t dpos = wave_append(&s_ndp, to_dp);
-    if (to_dp) {
-      DpTask t;
-      t.s1_off = u.off[ci];
-      t.s2_off = u.off[qq];
-      t.out_idx = out_idx;
-      t.bin = (clen - (uint32_t)a.bins.lmin) * (uint32_t)a.bins.range + (my_len - (uint32_t)a.bins.lmin);
-      t.u_idx = qq;
-      t.c_idx = ci;
-      dp_region[dpos] = t;
-      atomicAdd(&a.bin_count[t.bin], 1u);
+      const bool route_dp = !skip && !shroud && !route_gapless;
+      if (Q.valid && (skip || shroud)) {
+        a.out_lambda[out_idx] = 0.0;
+        a.out_hamming[out_idx] = kHammingNone;
+        a.out_skipped[out_idx] = skip ? 1 : 0;
+      }
+      n_skip += Q.valid && skip;
+      n_shroud += Q.valid && shroud;
+      const bool to_gl = Q.valid && route_gapless;
+      const uint32_t gpos = wave_append(&s_ngl, to_gl);
+      if (to_gl) {
+        GaplessTask t;
+        t.c_idx = C.ci;
+        t.u_idx = Q.qq;
+        t.out_idx = out_idx;
+        t._pad = 0;
+        gl_region[gpos] = t;
+      }
+      const bool to_dp = Q.valid && route_dp;
+      const uint32_t dpos = wave_append(&s_ndp, to_dp);
+      if (to_dp) {
+        DpTask t;
+        t.s1_off = u.off[C.ci];
+        t.s2_off = u.off[Q.qq];
+        t.out_idx = out_idx;
+        t.bin = (C.len - (uint32_t)a.bins.lmin) * (uint32_t)a.bins.range + (Q.len - (uint32_t)a.bins.lmin);
+        t.u_idx = Q.qq;
+        t.c_idx = C.ci;
+        dp_region[dpos] = t;
+        atomicAdd(&a.bin_count[t.bin], 1u);
+      }
     }
   }
   // block totals of skipped / shrouded pairs, summed by the plan kernel
   // (no device-scope atomics on shared counter words)
-  __shared__ uint32_t s_cnt[2][QB * SPLIT / 64];
+  __shared__ uint32_t s_cnt[2][GT * SPLIT / 64];
   uint32_t s1 = n_skip, s2 = n_shroud;
   for (int off = 32; off > 0; off >>= 1) {
     s1 += __shfl_xor(s1, off);
@@ -273,7 +304,7 @@ __global__ __launch_bounds__(QB* SPLIT) void screen_kernel(ScreenDev a) {
   __syncthreads();
   if (threadIdx.x == 0) {
     uint32_t t1 = 0, t2 = 0;
-    for (int w = 0; w < QB * SPLIT / 64; ++w) {
+    for (int w = 0; w < GT * SPLIT / 64; ++w) {
       t1 += s_cnt[0][w];
       t2 += s_cnt[1][w];
     }
@@ -284,16 +315,31 @@ __global__ __launch_bounds__(QB* SPLIT) void screen_kernel(ScreenDev a) {
   }
 }
 
+void screen_thresholds(double cutoff, uint32_t nmax, uint32_t* out) {
+  out[0] = 0;
+  for (uint32_t n = 1; n <= nmax; ++n) {
+    // the predicate 1 - d/n <= cutoff is monotone in d: bisect for its
+    // first true d in 0 .. n + 1 (n + 1: never true)
+    uint32_t lo = 0, hi = n + 1;
+    while (lo < hi) {
+      const uint32_t mid = (lo + hi) / 2;
+      const double kdist = 1.0 - (double)mid / (double)n;
+      if (!(kdist > cutoff)) hi = mid; else lo = mid + 1;
+    }
+    out[n] = lo;
+  }
+}
+
 uint32_t screen_blocks(uint32_t n_uniques, uint32_t n_centers, uint32_t qblock) {
   return ((n_uniques + qblock - 1) / qblock) * ((n_centers + kScreenCenters - 1) / kScreenCenters);
 }
 
-template <int QB>
+template <int QB, int QL>
 static void launch_qb(const ScreenDev& d, dim3 grid, int split, hipStream_t st) {
   switch (split) {
-    case 1: hipLaunchKernelGGL((screen_kernel<QB, 1>), grid, dim3(QB), 0, st, d); break;
-    case 2: hipLaunchKernelGGL((screen_kernel<QB, 2>), grid, dim3(QB * 2), 0, st, d); break;
-    default: hipLaunchKernelGGL((screen_kernel<QB, 4>), grid, dim3(QB * 4), 0, st, d); break;
+    case 1: hipLaunchKernelGGL((screen_kernel<QB, 1, QL>), grid, dim3(QB / QL), 0, st, d); break;
+    case 2: hipLaunchKernelGGL((screen_kernel<QB, 2, QL>), grid, dim3(QB / QL * 2), 0, st, d); break;
+    default: hipLaunchKernelGGL((screen_kernel<QB, 4, QL>), grid, dim3(QB / QL * 4), 0, st, d); break;
   }
 }
 
@@ -317,13 +363,18 @@ hipError_t launch_screen(const ScreenArgs& s, hipStream_t st) {
   d.bin_count = s.bin_count;
   d.partials = s.partials;
   d.ctr = s.ctr;
+  d.dstar = s.dstar;
   if (s.split != 1 && s.split != 2 && s.split != 4) return hipErrorInvalidValue;
+  if (s.use_kmers && s.dstar == nullptr) return hipErrorInvalidValue;
   const dim3 grid((d.u.n + s.qblock - 1) / s.qblock,
                   (s.n_centers + kScreenCenters - 1) / kScreenCenters);
-  switch (s.qblock) {
-    case 64: launch_qb<64>(d, grid, s.split, st); break;
-    case 128: launch_qb<128>(d, grid, s.split, st); break;
-    case 256: launch_qb<256>(d, grid, s.split, st); break;
+  const int key = (int)s.qblock * 4 + s.per_lane;
+  switch (key) {
+    case 64 * 4 + 1: launch_qb<64, 1>(d, grid, s.split, st); break;
+    case 128 * 4 + 1: launch_qb<128, 1>(d, grid, s.split, st); break;
+    case 256 * 4 + 1: launch_qb<256, 1>(d, grid, s.split, st); break;
+    case 128 * 4 + 2: launch_qb<128, 2>(d, grid, s.split, st); break;
+    case 256 * 4 + 2: launch_qb<256, 2>(d, grid, s.split, st); break;
     default: return hipErrorInvalidValue;
   }
   return hipGetLastError();

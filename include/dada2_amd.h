@@ -91,9 +91,10 @@ int d2g_ctx_set_timing(d2g_ctx* ctx, int enable);
 int d2g_ctx_set_graphs(d2g_ctx* ctx, int enable);
 /* Launch shape of the k-mer screen: a workgroup compares qblock = 64, 128
  * or 256 uniques with 16 centers, split over split = 1, 2 or 4 wavefront
- * groups (16/split centers each).  Results do not depend on it (default
- * 256, 2).  No counterpart in the reference. */
-int d2g_ctx_set_screen_shape(d2g_ctx* ctx, int qblock, int split);
+ * groups (16/split centers each), each lane taking per_lane = 1 or 2
+ * uniques (2 needs qblock >= 128).  Results do not depend on it (default
+ * 64, 4, 1).  No counterpart in the reference. */
+int d2g_ctx_set_screen_shape(d2g_ctx* ctx, int qblock, int split, int per_lane);
 /* The hipStream_t every kernel of this context is launched on. */
 void* d2g_ctx_stream(d2g_ctx* ctx);
 /* Device properties used for roofline figures. */

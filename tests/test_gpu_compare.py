@@ -121,16 +121,17 @@ def test_graph_replays_match_eager(pkg, oracle, batched_small):
         e.close()
 
 
-@pytest.mark.parametrize("qblock,split", [(256, 1), (256, 4), (128, 2), (64, 1), (64, 4)])
-def test_screen_shapes_same_results(ctx, oracle, pkg, batched_small, qblock, split):
+@pytest.mark.parametrize("qblock,split,per_lane", [(256, 1, 1), (256, 4, 1), (128, 2, 1), (64, 1, 1),
+                                                   (64, 4, 1), (256, 2, 2), (128, 4, 2), (128, 1, 2)])
+def test_screen_shapes_same_results(ctx, oracle, pkg, batched_small, qblock, split, per_lane):
     """Each launch shape of the screen (64, 128 or 256 uniques x 16 centers
-    per workgroup, over 1, 2 or 4 wavefront groups) gives the oracle's
-    results; 21 centers leave the last workgroup's groups partly or wholly
-    without centers."""
+    per workgroup, over 1, 2 or 4 wavefront groups, 1 or 2 uniques per lane)
+    gives the oracle's results; 21 centers leave the last workgroup's groups
+    partly or wholly without centers."""
     w = batched_small
     ctx.load_uniques(w.seqs, w.quals, w.reads, k=5)
     cs = w.centers[:21]
-    ctx.set_screen_shape(qblock, split)
+    ctx.set_screen_shape(qblock, split, per_lane)
     try:
         got = ctx.compare_batch(cs, w.err)
     finally:
