@@ -123,6 +123,7 @@ def _lib():
                                               C.POINTER(C.c_float)]),
             "d2g_ctx_set_graphs": (C.c_int, [vp, C.c_int]),
             "d2g_ctx_set_screen_shape": (C.c_int, [vp, C.c_int, C.c_int, C.c_int]),
+            "d2g_ctx_set_autotune": (C.c_int, [vp, C.c_int]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -141,7 +142,7 @@ def exported_symbols() -> list[str]:
             "d2g_align_endsfree", "d2g_band_supported", "d2g_pairs_load", "d2g_pairs_lambda",
             "d2g_pairs_fetch", "d2g_bimera_pairs", "d2g_is_bimera_batch", "d2g_table_bimera2",
             "d2g_loglik_forward", "d2g_loglik_backward", "d2g_ctx_set_graphs",
-            "d2g_ctx_set_screen_shape"]
+            "d2g_ctx_set_screen_shape", "d2g_ctx_set_autotune"]
 
 
 def _p(a: np.ndarray, t):
@@ -236,6 +237,11 @@ class Context:
     def set_graphs(self, on: bool):
         """Replay the compare's launches as a captured HIP graph (default on)."""
         _check(_lib().d2g_ctx_set_graphs(self.h, int(on)), "d2g_ctx_set_graphs")
+
+    def set_autotune(self, on: bool):
+        """Time the screen's launch shapes on the first compare of each
+        workload class and keep the fastest (default off)."""
+        _check(_lib().d2g_ctx_set_autotune(self.h, int(on)), "d2g_ctx_set_autotune")
 
     def set_screen_shape(self, qblock: int = 64, split: int = 4, per_lane: int = 1):
         """Screen launch shape: uniques per workgroup (64, 128, 256), the

@@ -6,7 +6,9 @@
 // checks, uploading the center list / lock flags / error matrix, and the
 // final copy of results (skipped by the device-resident entry point).
 #include <algorithm>
+#include <array>
 #include <cstring>
+#include <map>
 #include <numeric>
 #include <vector>
 
@@ -103,6 +105,9 @@ struct d2g_ctx {
   uint32_t screen_qblock = 64;
   int screen_split = 4;
   int screen_per_lane = 1;
+  // autotuned screen shape per workload class (qblock, split, per_lane)
+  bool autotune = false;
+  std::map<uint64_t, std::array<int, 3>> tuned;
   // the screen's shroud thresholds, for one (cutoff, k, longest unique)
   DevBuf<uint32_t> dstar;
   uint64_t dstar_cut = 0;
@@ -234,6 +239,12 @@ int d2g_ctx_set_timing(d2g_ctx* c, int enable) {
 int d2g_ctx_set_graphs(d2g_ctx* c, int enable) {
   if (!c) return D2G_ERR_ARG;
   c->graphs = enable != 0;
+  return D2G_OK;
+}
+
+int d2g_ctx_set_autotune(d2g_ctx* c, int enable) {
+  if (!c) return D2G_ERR_ARG;
+  c->autotune = enable != 0;
   return D2G_OK;
 }
 
@@ -473,6 +484,14 @@ static int dp_kind(const d2g_align_params* p, uint32_t minlen, uint32_t maxlen, 
   return kDpRowBand;
 }
 
+// Workload class of the autotuner: uniques and centers rounded to powers of
+// two, k, and the DP bin space's band and kind.
+static uint64_t tune_class(uint32_t n, uint32_t nc, uint32_t k, const BinSpace& bs) {
+  auto lg = [](uint32_t v) { uint64_t b = 0; while ((1u << b) < v && b < 31) ++b; return b; };
+  return lg(n) | lg(nc) << 6 | (uint64_t)k << 12 | (uint64_t)(uint32_t)bs.band << 16 |
+         (uint64_t)bs.kind << 40 | (uint64_t)bs.homo << 44;
+}
+
 static float ev_ms(d2g_ctx* c, int a, int b) {
   float ms = 0.f;
   if (hipEventElapsedTime(&ms, c->ev[a], c->ev[b]) != hipSuccess) return 0.f;
@@ -542,20 +561,24 @@ static int compare_device(d2g_ctx* c, const uint32_t* centers, uint32_t nc, cons
   HIP_TRY(c->out_lambda.ensure(npairs));
   HIP_TRY(c->out_hamming.ensure(npairs));
   HIP_TRY(c->out_skipped.ensure(npairs));
-  // per screen block: a region of `region` task slots in each list
-  const uint32_t n_part = screen_blocks(n, nc, c->screen_qblock);
-  const uint32_t region = screen_region(c->screen_qblock);
+  // per screen block: a region of screen_region(qblock) task slots in each
+  // list; sized for every screen shape (the autotuner tries several)
+  size_t max_part = 0, max_slots = 0;
+  for (uint32_t qb : {64u, 128u, 256u}) {
+    max_part = std::max<size_t>(max_part, screen_blocks(n, nc, qb));
+    max_slots = std::max<size_t>(max_slots, (size_t)screen_blocks(n, nc, qb) * screen_region(qb));
+  }
   HIP_TRY(c->gl.ensure(npairs));
-  HIP_TRY(c->gl_unsorted.ensure((size_t)n_part * region));
-  HIP_TRY(c->gl_off.ensure(n_part));
-  HIP_TRY(c->dp_unsorted.ensure((size_t)n_part * region));
+  HIP_TRY(c->gl_unsorted.ensure(max_slots));
+  HIP_TRY(c->gl_off.ensure(max_part));
+  HIP_TRY(c->dp_unsorted.ensure(max_slots));
   HIP_TRY(c->dp_sorted.ensure(npairs));
   HIP_TRY(c->sub_cnt.ensure(npairs));
   HIP_TRY(c->bin_count.ensure(bs.nbins + 1));
   HIP_TRY(c->bin_start.ensure(bs.nbins + 1));
   HIP_TRY(c->wave_start.ensure(bs.nbins + 1));
   HIP_TRY(c->bin_fill.ensure(bs.nbins + 1));
-  HIP_TRY(c->screen_partials.ensure((size_t)4 * n_part));
+  HIP_TRY(c->screen_partials.ensure((size_t)4 * max_part));
   const uint32_t sub_stride = (uint32_t)(bs.lmin + bs.range);  // longest length + 1
   if (c->sub_capacity < (size_t)1 << 20) c->sub_capacity = (size_t)1 << 20;
   HIP_TRY(c->sub_pool.ensure(c->sub_capacity));
@@ -610,6 +633,8 @@ static int compare_device(d2g_ctx* c, const uint32_t* centers, uint32_t nc, cons
   // lambda).  Timed runs bracket the kernels with events; otherwise the
   // sequence is captured once per shape into a HIP graph and replayed.
   auto enqueue = [&](bool timed) -> int {
+    const uint32_t n_part = screen_blocks(n, nc, c->screen_qblock);
+    const uint32_t region = screen_region(c->screen_qblock);
     if (timed) HIP_TRY(hipEventRecord(c->ev[0], s));
     HIP_TRY(launch_zero(reinterpret_cast<uint32_t*>(c->ctr.p), sizeof(Counters) / 4, s));
     HIP_TRY(launch_zero(c->bin_count.p, bs.nbins + 1, s));
@@ -693,6 +718,40 @@ static int compare_device(d2g_ctx* c, const uint32_t* centers, uint32_t nc, cons
     if (timed) HIP_TRY(hipEventRecord(c->ev[4], s));
     return D2G_OK;
   };
+
+  // Autotuning: the first compare of a workload class times each screen
+  // shape on this call's inputs (results do not depend on the shape) and
+  // keeps the fastest for the class.
+  if (c->autotune) {
+    const uint64_t cls = tune_class(n, nc, u.k, bs);
+    auto it = c->tuned.find(cls);
+    if (it == c->tuned.end()) {
+      static const int kShapes[][3] = {{64, 4, 1}, {64, 2, 1}, {128, 2, 1}, {256, 2, 1},
+                                       {128, 4, 2}, {256, 4, 2}};
+      float best = 0.f;
+      int best_i = 0;
+      for (int i = 0; i < (int)(sizeof(kShapes) / sizeof(kShapes[0])); ++i) {
+        c->screen_qblock = (uint32_t)kShapes[i][0];
+        c->screen_split = kShapes[i][1];
+        c->screen_per_lane = kShapes[i][2];
+        float ms = 0.f;
+        for (int rep = 0; rep < 2; ++rep) {  // the second run is timed
+          HIP_TRY_INT(enqueue(true));
+          HIP_TRY(hipEventSynchronize(c->ev[4]));
+          ms = ev_ms(c, 0, 1);
+        }
+        if (i == 0 || ms < best) {
+          best = ms;
+          best_i = i;
+        }
+      }
+      it = c->tuned.emplace(cls, std::array<int, 3>{kShapes[best_i][0], kShapes[best_i][1],
+                                                     kShapes[best_i][2]}).first;
+    }
+    c->screen_qblock = (uint32_t)it->second[0];
+    c->screen_split = it->second[1];
+    c->screen_per_lane = it->second[2];
+  }
 
   for (int attempt = 0; attempt < 6; ++attempt) {
     const bool graph = c->graphs && !c->timing;

@@ -95,6 +95,12 @@ int d2g_ctx_set_graphs(d2g_ctx* ctx, int enable);
  * uniques (2 needs qblock >= 128).  Results do not depend on it (default
  * 64, 4, 1).  No counterpart in the reference. */
 int d2g_ctx_set_screen_shape(d2g_ctx* ctx, int qblock, int split, int per_lane);
+/* Autotuning of the screen's launch shape (default off): the first compare
+ * of each workload class (uniques and centers by power of two, k, band and
+ * aligner) runs once per candidate shape, timed, and the class keeps the
+ * fastest.  Results do not depend on it.  No counterpart in the
+ * reference. */
+int d2g_ctx_set_autotune(d2g_ctx* ctx, int enable);
 /* The hipStream_t every kernel of this context is launched on. */
 void* d2g_ctx_stream(d2g_ctx* ctx);
 /* Device properties used for roofline figures. */

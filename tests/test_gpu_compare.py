@@ -155,3 +155,32 @@ def test_first_call_outgrows_substitution_pool(pkg, oracle):
         _assert_same(got, *_oracle_rows(oracle, w, w.centers, params=oracle.make_params(kdist_cutoff=1.0)))
     finally:
         c.close()
+
+
+@pytest.mark.parametrize("cutoff", [0.05, 0.2, 0.6, 0.0])
+def test_shroud_thresholds_at_other_cutoffs(ctx, oracle, pkg, batched_small, cutoff):
+    """The screen decides kdist > cutoff from a tabulated dot threshold per
+    overlap length; at other cutoffs (0.0: every pair with kdist > 0 is
+    shrouded) the shroud and dispatch decisions stay the oracle's."""
+    w = batched_small
+    ctx.load_uniques(w.seqs, w.quals, w.reads, k=5)
+    cs = w.centers[:8]
+    got = ctx.compare_batch(cs, w.err, params=pkg.align_params(kdist_cutoff=cutoff))
+    _assert_same(got, *_oracle_rows(oracle, w, cs, params=oracle.make_params(kdist_cutoff=cutoff)))
+
+
+def test_autotuned_shape_same_results(pkg, oracle, batched_small):
+    """With autotuning the first call times every screen shape on its own
+    inputs; that call and the next (on the kept shape) give the oracle's
+    results."""
+    w = batched_small
+    c = pkg.Context(0)
+    try:
+        c.set_autotune(True)
+        c.load_uniques(w.seqs, w.quals, w.reads, k=5)
+        cs = w.centers[:10]
+        want = _oracle_rows(oracle, w, cs)
+        _assert_same(c.compare_batch(cs, w.err), *want)
+        _assert_same(c.compare_batch(cs, w.err), *want)
+    finally:
+        c.close()
