@@ -6,6 +6,7 @@
 // checks, uploading the center list / lock flags / error matrix, and the
 // final copy of results (skipped by the device-resident entry point).
 #include <algorithm>
+#include <cmath>
 #include <array>
 #include <cstring>
 #include <map>
@@ -95,7 +96,8 @@ struct d2g_ctx {
   bool p_has_qual = false;
   DevBuf<uint8_t> p_seq, p_qual, p_flag;
   DevBuf<uint64_t> p_off, p_suboff;
-  DevBuf<uint32_t> p_len, p_subs, p_reads, p_creads, p_nrec;
+  DevBuf<uint32_t> p_len, p_subs, p_reads, p_creads, p_nrec, p_list;
+  DevBuf<double> p_lkf;  // sum_{i=1..r} log(i) for r = 0 .. max reads
   DevBuf<float> p_loglam;
   DevBuf<uint32_t> b_out;  // bimera: 5 words per pair
   // the compare's launch sequence as a replayable HIP graph, per shape key
@@ -214,8 +216,9 @@ int d2g_ctx_destroy(d2g_ctx* c) {
   c->sched_info.release();
   for (auto* b : {&c->p_seq, &c->p_qual, &c->p_flag}) b->release();
   for (auto* b : {&c->p_off, &c->p_suboff}) b->release();
-  for (auto* b : {&c->p_len, &c->p_subs, &c->p_reads, &c->p_creads, &c->p_nrec}) b->release();
+  for (auto* b : {&c->p_len, &c->p_subs, &c->p_reads, &c->p_creads, &c->p_nrec, &c->p_list}) b->release();
   c->p_loglam.release();
+  c->p_lkf.release();
   c->b_out.release();
   c->last_ctr.release();
   if (c->h_pin) (void)hipHostFree(c->h_pin);
@@ -1333,6 +1336,8 @@ int d2g_pairs_load(d2g_ctx* c, uint32_t n, const uint8_t* seq_concat, const uint
     if (qual_concat) std::memcpy(ql.data() + off[i], qual_concat + offsets[i], len[i]);
   }
   const uint64_t ns = sub_offsets[n] - sub_offsets[0];
+  // the kernels address both pools with 32-bit offsets
+  if (off[n] + 16 >= ((uint64_t)1 << 32) || ns >= ((uint64_t)1 << 30)) return D2G_ERR_UNSUPPORTED;
   std::vector<uint64_t> so(n + 1);
   for (uint32_t i = 0; i <= n; ++i) so[i] = sub_offsets[i] - sub_offsets[0];
   hipStream_t s = c->stream;
@@ -1349,6 +1354,17 @@ int d2g_pairs_load(d2g_ctx* c, uint32_t n, const uint8_t* seq_concat, const uint
   HIP_TRY(c->p_pval.ensure(n));
   HIP_TRY(c->p_flag.ensure(n));
   HIP_TRY(c->p_nrec.ensure(1));
+  HIP_TRY(c->p_list.ensure(n));
+  // poisson_upper_tail_direct's log(reads!) as the reference's running sum
+  // of logs (pval.rs:335), tabulated once in the same order
+  uint32_t max_reads = 0;
+  for (uint32_t i = 0; i < n; ++i) max_reads = std::max(max_reads, reads[i]);
+  if (max_reads >= (1u << 24)) return D2G_ERR_UNSUPPORTED;
+  std::vector<double> lkf((size_t)max_reads + 1);
+  lkf[0] = 0.0;
+  for (uint32_t i = 1; i <= max_reads; ++i) lkf[i] = lkf[i - 1] + std::log((double)i);
+  HIP_TRY(c->p_lkf.ensure(lkf.size()));
+  HIP_TRY(hipMemcpyAsync(c->p_lkf.p, lkf.data(), lkf.size() * sizeof(double), hipMemcpyHostToDevice, s));
   HIP_TRY(hipMemcpyAsync(c->p_seq.p, sq.data(), off[n], hipMemcpyHostToDevice, s));
   if (qual_concat) HIP_TRY(hipMemcpyAsync(c->p_qual.p, ql.data(), off[n], hipMemcpyHostToDevice, s));
   HIP_TRY(hipMemcpyAsync(c->p_off.p, off.data(), (n + 1) * 8, hipMemcpyHostToDevice, s));
@@ -1394,6 +1410,8 @@ int d2g_pairs_lambda(d2g_ctx* c, const double* err, uint32_t ncol, int use_quals
   pa.out_p = c->p_pval.p;
   pa.out_flag = c->p_flag.p;
   pa.n_recomputed = c->p_nrec.p;
+  pa.list = c->p_list.p;
+  pa.log_kfact = c->p_lkf.p;
   pa.max_blocks = (uint32_t)c->cu_count * 8;
   pa.ev_mid = c->timing ? c->ev[1] : nullptr;
   HIP_TRY(launch_pairs_lambda(pa, s));

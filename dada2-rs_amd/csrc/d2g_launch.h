@@ -203,7 +203,9 @@ struct PairArgs {
   double* out_lambda;
   double* out_p;
   uint8_t* out_flag;
-  uint32_t* n_recomputed;
+  uint32_t* n_recomputed;   // zeroed by the caller; counts `list`
+  uint32_t* list;           // n: pairs whose first p-value fell near omega
+  const double* log_kfact;  // [max reads + 1]: sum_{i=1..r} log(i) in running order
   uint32_t max_blocks;
   hipEvent_t ev_mid;        // nullable: recorded between the two kernels
 };

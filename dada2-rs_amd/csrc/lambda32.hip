@@ -7,21 +7,30 @@
 // product over query positions of err[t][q], t = 5 (nt-1) on the diagonal and
 // 4 (nt0-1) + (nt-1) at a substitution.
 //
-// loglam32_kernel: log lambda as an FP32 Kahan (compensated) sum, in the same
-// position order, of a table log_err[t][q] = (float)log(err[t][q]) computed in
-// FP64.  Each table entry carries one FP32 rounding (<= 2^-24 relative) and
-// every term has the same sign, so the sum's relative error stays near
-// 2^-24 plus the compensated-summation bound.
-//
-// pval_kernel: get_pA / calc_pA (pval.rs:376-398, 208-241) in FP64 from
-// lambda = exp(log lambda):  sentinels (singleton, hamming 0, lambda 0), the
+// Pass 1 (pairs_loglam_kernel): log lambda as an FP32 Kahan (compensated) sum,
+// in the same position order, of a table log_err[t][q] = (float)log(err[t][q])
+// computed in FP64.  Each table entry carries one FP32 rounding (<= 2^-24
+// relative) and every term has the same sign, so the sum's relative error
+// stays near 2^-24 plus the compensated-summation bound.  pairs_pval_kernel
+// (one pair per lane) then takes get_pA / calc_pA (pval.rs:376-398, 208-241)
+// in FP64 from lambda =
+// exp(log lambda): the sentinels (singleton, hamming 0, lambda 0), the
 // Poisson upper tail P(X >= reads) = P(reads, E) (regularised lower
 // incomplete gamma, E = lambda * center reads), the reference's direct
 // series when that underflows (pval.rs:329-353), and the 1 - e^{-E}
 // conditioning with its Taylor replacement below 1e-7.  Pairs whose p-value
-// lands within a factor 10 of omega_a get lambda recomputed as the exact
-// ordered FP64 product and their p-value recomputed, so the p < omega_a call
+// lands within a factor 10 of omega_a are appended to a list instead.
+//
+// Pass 2 (pairs_recompute_kernel): for the listed pairs only, lambda as the
+// exact ordered FP64 product and the p-value again, so the p < omega_a call
 // is taken on FP64 lambda wherever it is close.
+//
+// Both passes feed their per-pair chains from LDS, as the compare's lambda
+// kernel does: a workgroup of four wavefronts takes 64 pairs, each wavefront
+// stages its 16 pairs' (transition row << 2 | nt - 1, quality) codes with
+// coalesced loads (lane l: positions 4l .. 4l + 3 of a pair's rows), the
+// owning lanes (0..15) patch their substitution positions, and each owning
+// lane runs its chain from its row.
 //
 // The incomplete gamma is evaluated with the series (x < a + 1) or the
 // modified-Lentz continued fraction (Numerical Recipes, 3rd ed., 6.2), using
@@ -32,63 +41,124 @@
 
 namespace d2g {
 
+constexpr int kPSeg = 256;              // query positions per staged segment (4 per lane)
+constexpr int kPRow = 2 * kPSeg + 4;    // bytes of one pair's LDS row (odd word stride)
+constexpr int kPWaves = 4;              // wavefronts per workgroup
+constexpr int kPPairsW = 16;            // pairs per wavefront
+constexpr int kPTile = kPWaves * kPPairsW;
+constexpr int kPUnroll = 8;             // pairs staged per round of loads
+constexpr int kPSubBatch = 8;           // substitution records requested together per lane
+constexpr uint32_t kPOneRow = 16;       // transition row of the past-the-end entry
+
 struct PairDev {
   uint32_t n;
-  const uint8_t* seq;       // rows padded to 16 bytes
+  const uint8_t* seq;       // rows padded to 16 bytes (the pool stays under 2^32 bytes)
   const uint8_t* qual;      // nullable, same layout
   const uint64_t* off;      // n + 1
   const uint32_t* len;      // n
-  const uint32_t* subs;     // (query pos << 16 | center base)
+  const uint32_t* subs;     // (query pos << 16 | center base), under 2^32 records
   const uint64_t* sub_off;  // n + 1
   const uint32_t* reads;
   const uint32_t* center_reads;
 };
 
-__global__ __launch_bounds__(256) void loglam32_kernel(PairDev a, const double* err, uint32_t ncol,
-                                                       int use_quals, float* out_loglam) {
-  extern __shared__ float logerr[];
-  const uint32_t ne = 16u * ncol;
-  for (uint32_t i = threadIdx.x; i < ne; i += blockDim.x) logerr[i] = (float)log(err[i]);
-  __syncthreads();
-  const bool quals = use_quals && a.qual != nullptr;
-  for (uint32_t t = blockIdx.x * blockDim.x + threadIdx.x; t < a.n; t += gridDim.x * blockDim.x) {
-    const uint4* q16 = reinterpret_cast<const uint4*>(a.seq + a.off[t]);
-    const uint4* u16 = quals ? reinterpret_cast<const uint4*>(a.qual + a.off[t]) : nullptr;
-    const uint32_t lq = a.len[t];
-    const uint32_t* subs = a.subs + a.sub_off[t];
-    const uint32_t ns = (uint32_t)(a.sub_off[t + 1] - a.sub_off[t]);
-    uint32_t si = 0;
-    uint32_t next = ns ? subs[0] : 0xFFFFFFFFu;
-    float sum = 0.0f, comp = 0.0f;
-    bool zero = false;
-    for (uint32_t base = 0; base < lq; base += 16) {
-      const uint4 qv = q16[base >> 4];
-      const uint4 uv = u16 ? u16[base >> 4] : make_uint4(0, 0, 0, 0);
-      const uint32_t qw[4] = {qv.x, qv.y, qv.z, qv.w};
-      const uint32_t uw[4] = {uv.x, uv.y, uv.z, uv.w};
+template <class T>
+__device__ __forceinline__ T pld(const void* base, uint32_t byte_off) {
+  return *reinterpret_cast<const T*>(reinterpret_cast<const uint8_t*>(base) + byte_off);
+}
+__device__ __forceinline__ uint32_t prl(uint32_t v, uint32_t lane) {
+  return (uint32_t)__builtin_amdgcn_readlane((int)v, (int)lane);
+}
+__device__ __forceinline__ void pwave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+}
+
+// One owning lane's pair (lanes 0..15 of each wavefront; others hold none).
+struct PairRow {
+  bool live;
+  uint32_t t;        // pair index
+  uint32_t lq;       // query length
+  uint32_t qoff;     // row offset in the pools
+  uint32_t s0, ns;   // substitution records
+};
+
+__device__ __forceinline__ PairRow pair_row(const PairDev& a, bool live, uint32_t t) {
+  PairRow r;
+  r.live = live;
+  r.t = live ? t : 0u;
+  r.lq = live ? a.len[r.t] : 0u;
+  r.qoff = (uint32_t)a.off[r.t];
+  const uint64_t so = a.sub_off[r.t];
+  r.s0 = (uint32_t)so;
+  r.ns = live ? (uint32_t)(a.sub_off[r.t + 1] - so) : 0u;
+  return r;
+}
+
+// Stages the segment at s0 of the wavefront's 16 pairs into their rows
+// (rows + p * kPRow): two rounds of kPUnroll pairs, the second round's words
+// requested before the first round's bytes are converted.  Positions past a
+// pair's end get transition row kPOneRow.
+__device__ __forceinline__ void stage_pairs(const PairDev& a, const PairRow& me, uint32_t s0,
+                                            uint32_t lane, bool quals, uint8_t* rows) {
+  const uint8_t* qual = quals ? a.qual : a.seq;
+  uint32_t qw[2][kPUnroll], uw[2][kPUnroll], lq[2][kPUnroll];
 #pragma unroll
-      for (int b = 0; b < 16; ++b) {
-        const uint32_t pos = base + (uint32_t)b;
-        if (pos < lq) {
-          const uint32_t nt = (qw[b >> 2] >> (8 * (b & 3))) & 0xFFu;
-          const uint32_t qi = (uw[b >> 2] >> (8 * (b & 3))) & 0xFFu;
-          uint32_t tr = 5u * (nt - 1u);
-          if ((next >> 16) == pos) {
-            tr = 4u * ((next & 0xFFFFu) - 1u) + (nt - 1u);
-            ++si;
-            next = si < ns ? subs[si] : 0xFFFFFFFFu;
-          }
-          const float term = logerr[tr * ncol + qi];
-          zero = zero || term == -INFINITY;
-          // Kahan: compensated sum in FP32 (no contraction: no products)
-          const float y = term - comp;
-          const float s = sum + y;
-          comp = (s - sum) - y;
-          sum = s;
-        }
-      }
+  for (int h = 0; h < 2; ++h)
+#pragma unroll
+    for (int r = 0; r < kPUnroll; ++r) {
+      const uint32_t pr = (uint32_t)(h * kPUnroll + r);
+      lq[h][r] = prl(me.lq, pr);
+      const uint32_t pq = prl(me.qoff, pr);
+      const uint32_t pos = s0 + 4u * lane;
+      const uint32_t o = pos < lq[h][r] ? pq + pos : pq;  // past the end: the row's first word
+      qw[h][r] = pld<uint32_t>(a.seq, o);
+      uw[h][r] = pld<uint32_t>(qual, o);
     }
-    out_loglam[t] = zero ? -INFINITY : sum;
+#pragma unroll
+  for (int h = 0; h < 2; ++h) {
+    __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+    for (int r = 0; r < kPUnroll; ++r) {
+      const uint32_t pr = (uint32_t)(h * kPUnroll + r);
+      uint32_t packed[2] = {0u, 0u};
+#pragma unroll
+      for (int b = 0; b < 4; ++b) {
+        const uint32_t pos = s0 + 4u * lane + (uint32_t)b;
+        const bool in = pos < lq[h][r];
+        const uint32_t nt1 = ((qw[h][r] >> (8 * b)) & 0xFFu) - 1u;
+        const uint32_t tr = in ? 5u * nt1 : kPOneRow;
+        const uint32_t qi = (quals && in) ? (uw[h][r] >> (8 * b)) & 0xFFu : 0u;
+        const uint32_t code = (((tr << 2) | (nt1 & 3u)) & 0xFFu) | (qi << 8);
+        packed[b >> 1] |= code << (16 * (b & 1));
+      }
+      uint32_t* row = reinterpret_cast<uint32_t*>(rows + pr * kPRow) + 2 * lane;
+      row[0] = packed[0];
+      row[1] = packed[1];
+    }
+  }
+}
+
+// The owning lanes rewrite the transition row at their substitution
+// positions of the segment (records carry the center base).
+__device__ __forceinline__ void patch_pairs(const PairDev& a, const PairRow& me, uint32_t s0,
+                                            uint8_t* myb) {
+  for (uint32_t k0 = 0; __builtin_amdgcn_ballot_w64(k0 < me.ns); k0 += kPSubBatch) {
+    uint32_t rec[kPSubBatch];
+#pragma unroll
+    for (int j = 0; j < kPSubBatch; ++j) {
+      const uint32_t k = k0 + (uint32_t)j;
+      rec[j] = pld<uint32_t>(a.subs, (k < me.ns ? me.s0 + k : 0u) << 2);
+    }
+#pragma unroll
+    for (int j = 0; j < kPSubBatch; ++j) {
+      const uint32_t k = k0 + (uint32_t)j;
+      const uint32_t qp = rec[j] >> 16;
+      const bool here = k < me.ns && qp >= s0 && qp - s0 < (uint32_t)kPSeg;
+      uint8_t* c = myb + (here ? 2 * (qp - s0) : 2 * kPSeg);  // else the row's pad byte
+      const uint32_t nt1 = (uint32_t)c[0] & 3u;
+      c[0] = (uint8_t)(((4u * ((rec[j] & 0xFFFFu) - 1u) + nt1) << 2) | nt1);
+    }
   }
 }
 
@@ -123,13 +193,13 @@ __device__ double gamma_p(double a, double x) {
   return 1.0 - exp(-x + a * log(x) - gln) * h;
 }
 
-// poisson_upper_tail_direct, pval.rs:329-353
-__device__ double poisson_tail_direct(uint32_t reads, double lambda) {
+// poisson_upper_tail_direct, pval.rs:329-353.  log_kfact[r] = sum_{i=1..r}
+// log(i), the reference's running sum, tabulated on the host in the same
+// order.
+__device__ double poisson_tail_direct(uint32_t reads, double lambda, const double* log_kfact) {
   const double k = (double)reads;
   const double log_lambda = log(lambda);
-  double log_kfact = 0.0;
-  for (uint32_t i = 1; i <= reads; ++i) log_kfact += log((double)i);
-  const double log_leading = -lambda + k * log_lambda - log_kfact;
+  const double log_leading = -lambda + k * log_lambda - log_kfact[reads];
   double series = 1.0, term = 1.0;
   for (uint32_t j = 1; j < 10000u; ++j) {
     term *= lambda / (k + (double)j);
@@ -142,68 +212,153 @@ __device__ double poisson_tail_direct(uint32_t reads, double lambda) {
 }
 
 // get_pA with prior = detect_singletons = false (b_p_update's per-raw call)
-__device__ double get_pA_dev(uint32_t reads, double lambda, uint32_t hamming, uint32_t bi_reads) {
+__device__ double get_pA_dev(uint32_t reads, double lambda, uint32_t hamming, uint32_t bi_reads,
+                             const double* log_kfact) {
   if (reads == 1) return 1.0;
   if (hamming == 0) return 1.0;
   if (lambda == 0.0) return 0.0;
   const double e = lambda * (double)bi_reads;
   if (!(e > 0.0)) return 1.0;
   double p = gamma_p((double)reads, e);
-  if (p == 0.0) p = poisson_tail_direct(reads, e);
+  if (p == 0.0) p = poisson_tail_direct(reads, e, log_kfact);
   double norm = 1.0 - exp(-e);
   if (norm < 1e-7) norm = e - 0.5 * e * e;
   return p / norm;
 }
 
-// The exact ordered FP64 product of compute_lambda (pval.rs:311-313).
-__device__ double lambda_fp64(const PairDev& a, uint32_t t, const double* err, uint32_t ncol,
-                              bool quals) {
-  const uint8_t* q = a.seq + a.off[t];
-  const uint8_t* u = quals ? a.qual + a.off[t] : nullptr;
-  const uint32_t lq = a.len[t];
-  const uint32_t* subs = a.subs + a.sub_off[t];
-  const uint32_t ns = (uint32_t)(a.sub_off[t + 1] - a.sub_off[t]);
-  uint32_t si = 0;
-  uint32_t next = ns ? subs[0] : 0xFFFFFFFFu;
-  double lam = 1.0;
-  for (uint32_t pos = 0; pos < lq; ++pos) {
-    const uint32_t nt = q[pos];
-    uint32_t tr = 5u * (nt - 1u);
-    if ((next >> 16) == pos) {
-      tr = 4u * ((next & 0xFFFFu) - 1u) + (nt - 1u);
-      ++si;
-      next = si < ns ? subs[si] : 0xFFFFFFFFu;
+// Pass 1: FP32 Kahan log-lambda of every pair.
+__global__ __launch_bounds__(64 * kPWaves) void pairs_loglam_kernel(PairDev a, const double* err,
+                                                                     uint32_t ncol, int use_quals,
+                                                                     float* out_loglam) {
+  extern __shared__ float logerr[];  // 16 ncol entries and a 0 after them
+  const uint32_t lane = __lane_id(), wave = threadIdx.x / 64u;
+  const uint32_t ne = 16u * ncol;
+  for (uint32_t i = threadIdx.x; i <= ne; i += 64u * kPWaves)
+    logerr[i] = i < ne ? (float)log(err[i]) : 0.0f;
+  uint8_t* rows = reinterpret_cast<uint8_t*>(logerr + ((ne + 2) & ~1u)) + wave * kPPairsW * kPRow;
+  uint8_t* myb = rows + min(lane, (uint32_t)kPPairsW - 1u) * kPRow;
+  const bool quals = use_quals && a.qual != nullptr;
+  __syncthreads();
+  for (uint32_t base = blockIdx.x * (uint32_t)kPTile; base < a.n; base += gridDim.x * (uint32_t)kPTile) {
+    const uint32_t t = base + wave * kPPairsW + lane;
+    const PairRow me = pair_row(a, lane < (uint32_t)kPPairsW && t < a.n, t);
+    float sum = 0.0f, comp = 0.0f;
+    bool zero = false;
+    for (uint32_t s0 = 0; __builtin_amdgcn_ballot_w64(s0 < me.lq); s0 += kPSeg) {
+      stage_pairs(a, me, s0, lane, quals, rows);
+      patch_pairs(a, me, s0, myb);
+      pwave_sync();
+      const uint32_t seg_n = me.lq > s0 ? min(me.lq - s0, (uint32_t)kPSeg) : 0u;
+      if (lane < (uint32_t)kPPairsW) {
+        const uint32_t* w = reinterpret_cast<const uint32_t*>(myb);
+        for (uint32_t i = 0; __builtin_amdgcn_ballot_w64(i < seg_n); i += 16) {
+          uint32_t w8[8];
+#pragma unroll
+          for (int h = 0; h < 8; ++h) w8[h] = w[(i >> 1) + (uint32_t)h];
+#pragma unroll
+          for (int e = 0; e < 16; ++e) {
+            const uint32_t code = (w8[e >> 1] >> (16 * (e & 1))) & 0xFFFFu;
+            const float term = logerr[((code & 0xFFu) >> 2) * ncol + (code >> 8)];
+            if (i + (uint32_t)e < seg_n) {
+              zero = zero || term == -INFINITY;
+              // Kahan: compensated sum in FP32 (no contraction: no products)
+              const float y = term - comp;
+              const float s1 = sum + y;
+              comp = (s1 - sum) - y;
+              sum = s1;
+            }
+          }
+        }
+      }
+      pwave_sync();  // rows are restaged for the next segment
     }
-    lam *= err[tr * ncol + (u ? u[pos] : 0u)];
+    if (me.live) out_loglam[me.t] = zero ? -INFINITY : sum;
   }
-  return lam;
 }
 
-__global__ __launch_bounds__(256) void pval_kernel(PairDev a, const double* err, uint32_t ncol,
-                                                   int use_quals, const float* loglam,
-                                                   double omega, double* out_lambda,
-                                                   double* out_p, uint8_t* out_flag,
-                                                   uint32_t* n_recomputed) {
-  const bool quals = use_quals && a.qual != nullptr;
-  uint32_t recomputed = 0;
-  for (uint32_t t = blockIdx.x * blockDim.x + threadIdx.x; t < a.n; t += gridDim.x * blockDim.x) {
-    const uint32_t ham = (uint32_t)(a.sub_off[t + 1] - a.sub_off[t]);
-    const float ll = loglam[t];
-    double lam = ll == -INFINITY ? 0.0 : exp((double)ll);
-    double p = get_pA_dev(a.reads[t], lam, ham, a.center_reads[t]);
-    uint8_t flag = 0;
-    if (p >= omega * 0.1 && p <= omega * 10.0) {
-      lam = lambda_fp64(a, t, err, ncol, quals);
-      p = get_pA_dev(a.reads[t], lam, ham, a.center_reads[t]);
-      flag = 1;
-      ++recomputed;
+// The first p-values, one pair per lane: lambda = exp(log lambda); pairs
+// whose p-value lands within a factor 10 of omega go to `list` instead.
+__global__ __launch_bounds__(256) void pairs_pval_kernel(PairDev a, const float* loglam,
+                                                         const double* log_kfact, double omega,
+                                                         double* out_lambda, double* out_p,
+                                                         uint8_t* out_flag, uint32_t* list,
+                                                         uint32_t* n_list) {
+  const uint32_t lane = __lane_id();
+  for (uint32_t b = blockIdx.x * blockDim.x; b < a.n; b += gridDim.x * blockDim.x) {
+    const uint32_t t = b + threadIdx.x;
+    bool near = false;
+    if (t < a.n) {
+      const float ll = loglam[t];
+      const double lam = ll == -INFINITY ? 0.0 : exp((double)ll);
+      const uint32_t ham = (uint32_t)(a.sub_off[t + 1] - a.sub_off[t]);
+      const double p = get_pA_dev(a.reads[t], lam, ham, a.center_reads[t], log_kfact);
+      near = p >= omega * 0.1 && p <= omega * 10.0;
+      if (!near) {
+        out_lambda[t] = lam;
+        out_p[t] = p;
+        out_flag[t] = 0;
+      }
     }
-    out_lambda[t] = lam;
-    out_p[t] = p;
-    out_flag[t] = flag;
+    // append the near pairs: one atomic per wavefront, ranks from the ballot
+    const uint64_t m = __builtin_amdgcn_ballot_w64(near);
+    if (m) {
+      uint32_t b0 = 0;
+      if (lane == 0) b0 = atomicAdd(n_list, (uint32_t)__builtin_popcountll(m));
+      b0 = (uint32_t)__builtin_amdgcn_readfirstlane((int)b0);
+      if (near) list[b0 + (uint32_t)__builtin_popcountll(m & ((1ull << lane) - 1ull))] = t;
+    }
   }
-  for (int off = 32; off > 0; off >>= 1) recomputed += __shfl_xor(recomputed, off);
-  if ((threadIdx.x & 63) == 0 && recomputed) atomicAdd(n_recomputed, recomputed);
+}
+
+// Pass 2: the listed pairs' exact ordered FP64 product and p-value.
+__global__ __launch_bounds__(64 * kPWaves) void pairs_recompute_kernel(
+    PairDev a, const double* err, uint32_t ncol, int use_quals, const uint32_t* list,
+    const uint32_t* n_list, const double* log_kfact, double* out_lambda, double* out_p,
+    uint8_t* out_flag) {
+  extern __shared__ double errd[];  // 16 ncol entries and 1.0 after them
+  const uint32_t lane = __lane_id(), wave = threadIdx.x / 64u;
+  const uint32_t ne = 16u * ncol;
+  const uint32_t cnt = *n_list;
+  if (blockIdx.x * (uint32_t)kPTile >= cnt) return;  // uniform over the workgroup
+  for (uint32_t i = threadIdx.x; i <= ne; i += 64u * kPWaves) errd[i] = i < ne ? err[i] : 1.0;
+  uint8_t* rows = reinterpret_cast<uint8_t*>(errd + ne + 1) + wave * kPPairsW * kPRow;
+  uint8_t* myb = rows + min(lane, (uint32_t)kPPairsW - 1u) * kPRow;
+  const bool quals = use_quals && a.qual != nullptr;
+  __syncthreads();
+  for (uint32_t base = blockIdx.x * (uint32_t)kPTile; base < cnt; base += gridDim.x * (uint32_t)kPTile) {
+    const uint32_t k = base + wave * kPPairsW + lane;
+    const bool live = lane < (uint32_t)kPPairsW && k < cnt;
+    const PairRow me = pair_row(a, live, live ? list[k] : 0u);
+    double lam = 1.0;
+    for (uint32_t s0 = 0; __builtin_amdgcn_ballot_w64(s0 < me.lq); s0 += kPSeg) {
+      stage_pairs(a, me, s0, lane, quals, rows);
+      patch_pairs(a, me, s0, myb);
+      pwave_sync();
+      const uint32_t seg_n = me.lq > s0 ? min(me.lq - s0, (uint32_t)kPSeg) : 0u;
+      if (lane < (uint32_t)kPPairsW) {
+        const uint32_t* w = reinterpret_cast<const uint32_t*>(myb);
+        for (uint32_t i = 0; __builtin_amdgcn_ballot_w64(i < seg_n); i += 32) {
+          uint32_t w16[16];
+#pragma unroll
+          for (int h = 0; h < 16; ++h) w16[h] = w[(i >> 1) + (uint32_t)h];
+          double f[32];
+#pragma unroll
+          for (int e = 0; e < 32; ++e) {
+            const uint32_t code = (w16[e >> 1] >> (16 * (e & 1))) & 0xFFFFu;
+            f[e] = errd[((code & 0xFFu) >> 2) * ncol + (code >> 8)];  // past the end: 1.0
+          }
+#pragma unroll
+          for (int e = 0; e < 32; ++e) lam *= f[e];
+        }
+      }
+      pwave_sync();
+    }
+    if (me.live) {
+      out_lambda[me.t] = lam;
+      out_p[me.t] = get_pA_dev(a.reads[me.t], lam, me.ns, a.center_reads[me.t], log_kfact);
+      out_flag[me.t] = 1;
+    }
+  }
 }
 
 hipError_t launch_pairs_lambda(const PairArgs& s, hipStream_t st) {
@@ -218,19 +373,38 @@ hipError_t launch_pairs_lambda(const PairArgs& s, hipStream_t st) {
   d.reads = s.reads;
   d.center_reads = s.center_reads;
   if (s.n == 0) return hipSuccess;
-  const size_t lds = (size_t)16 * s.ncol * sizeof(float);
-  uint32_t blocks = (s.n + 255) / 256;
-  if (blocks > s.max_blocks) blocks = s.max_blocks;
-  hipLaunchKernelGGL(loglam32_kernel, dim3(blocks), dim3(256), lds, st, d, s.err, s.ncol,
-                     s.use_quals, s.out_loglam);
+  const uint32_t tiles = (s.n + kPTile - 1) / kPTile;
+  const uint32_t blocks = min(tiles, s.max_blocks);
+  const size_t rows = (size_t)kPTile * kPRow;
+  const size_t lds1 = (((size_t)16 * s.ncol + 2) & ~(size_t)1) * sizeof(float) + rows;
+  const size_t lds2 = ((size_t)16 * s.ncol + 1) * sizeof(double) + rows;
+  if (lds1 > 48 * 1024) {
+    hipError_t e = hipFuncSetAttribute((const void*)pairs_loglam_kernel,
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds1);
+    if (e != hipSuccess) return e;
+  }
+  if (lds2 > 48 * 1024) {
+    hipError_t e = hipFuncSetAttribute((const void*)pairs_recompute_kernel,
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds2);
+    if (e != hipSuccess) return e;
+  }
+  hipLaunchKernelGGL(pairs_loglam_kernel, dim3(blocks), dim3(64 * kPWaves), lds1, st, d, s.err,
+                     s.ncol, s.use_quals, s.out_loglam);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
   if (s.ev_mid) {
     e = hipEventRecord(s.ev_mid, st);
     if (e != hipSuccess) return e;
   }
-  hipLaunchKernelGGL(pval_kernel, dim3(blocks), dim3(256), 0, st, d, s.err, s.ncol, s.use_quals,
-                     s.out_loglam, s.omega, s.out_lambda, s.out_p, s.out_flag, s.n_recomputed);
+  const uint32_t pblocks = min((s.n + 255) / 256, s.max_blocks);
+  hipLaunchKernelGGL(pairs_pval_kernel, dim3(pblocks), dim3(256), 0, st, d,
+                     (const float*)s.out_loglam, s.log_kfact, s.omega, s.out_lambda, s.out_p,
+                     s.out_flag, s.list, s.n_recomputed);
+  e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(pairs_recompute_kernel, dim3(blocks), dim3(64 * kPWaves), lds2, st, d, s.err,
+                     s.ncol, s.use_quals, (const uint32_t*)s.list, (const uint32_t*)s.n_recomputed,
+                     s.log_kfact, s.out_lambda, s.out_p, s.out_flag);
   return hipGetLastError();
 }
 

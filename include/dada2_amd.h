@@ -269,7 +269,7 @@ typedef struct d2g_pair_stats {
   uint64_t pairs;
   uint64_t recomputed;
   float ms_loglam;  /* FP32 log-lambda kernel */
-  float ms_pval;    /* p-value kernel (incl. recomputes) */
+  float ms_pval;    /* p-value kernel and the FP64 recompute of the pairs near omega_a */
   float ms_total;
   float _pad;
 } d2g_pair_stats;
