@@ -1038,7 +1038,9 @@ int d2g_band_supported(uint32_t len1, uint32_t len2, int32_t band, int homo) {
 static int loglik_args(d2g_ctx* c, uint32_t ncol, int use_quals, LoglikArgs* la) {
   if (!c->cmp_valid) return D2G_ERR_STATE;
   if (use_quals && c->has_qual && c->max_qual >= ncol) return D2G_ERR_ARG;
-  const uint32_t grid = (uint32_t)c->cu_count * 4;
+  // one workgroup per 64 aligned pairs, at most 4 per CU
+  const uint32_t grid = (uint32_t)std::max<size_t>(
+      1, std::min<size_t>((size_t)c->cu_count * 4, (c->cmp_pairs + 63) / 64));
   HIP_TRY(c->ll_w.ensure(c->cmp_pairs + 1));
   HIP_TRY(c->ll_part.ensure((size_t)grid * 16 * ncol));
   HIP_TRY(c->ll_out.ensure((size_t)16 * ncol + 1));
