@@ -143,12 +143,16 @@ D2G_HD BandGeom band_geom(int l1, int l2, int band) {
   return g;
 }
 
-// Per pair in LDS: both sequences, and with homopolymer gaps their run
-// masks (homopolymer_mask_into, nwalign.rs:211-227).
-D2G_HD size_t band_group_bytes(const BandGeom& g, bool homo) {
-  size_t b = (size_t)(g.len1 + g.len2) * (homo ? 2 : 1);
-  return (b + 15) & ~(size_t)15;
+// Per pair in LDS: one u16 per position of each sequence (the base in the
+// low byte; with homopolymer gaps, bit 8 flags a position inside a run of
+// >= 3 identical bytes, homopolymer_mask_into, nwalign.rs:211-227), the two
+// rows with 16 bytes of padding before, between (32) and after them.
+D2G_HD size_t band_group_bytes(const BandGeom& g, bool) {
+  return ((size_t)64 + 2 * (size_t)(g.len1 + g.len2) + 15) & ~(size_t)15;
 }
+// Byte offsets of the two rows within a pair's LDS.
+D2G_HD size_t band_q1_offset(const BandGeom&) { return 16; }
+D2G_HD size_t band_q2_offset(const BandGeom& g) { return 48 + 2 * (size_t)g.len1; }
 D2G_HD size_t band_wave_bytes(const BandGeom& g, int G, bool homo) {
   return (size_t)G * band_group_bytes(g, homo);
 }

@@ -107,8 +107,7 @@ struct BandLane {
 // lane's diagonals on this step.  Writes the K cells' pointer codes.
 template <bool HOMO, int PAR, int K>
 __device__ __forceinline__ void band_step(const int t, BandLane<K>& L, const BandK Kc,
-                                          const uint8_t* s1, const uint8_t* s2, const uint8_t* h1,
-                                          const uint8_t* h2, uint32_t* pout) {
+                                          const uint16_t* q1, const uint16_t* q2, uint32_t* pout) {
   // the one neighbour value from another lane: D+1 of cell K-1 (odd steps)
   // or D-1 of cell 0 (even steps)
   const int ext = PAR ? from_upper(L.prev[0], 0) : from_lower(L.prev[K - 1], 0);
@@ -121,15 +120,15 @@ __device__ __forceinline__ void band_step(const int t, BandLane<K>& L, const Ban
     const bool ok = (((PAR ? L.okO : L.okE) >> m) & 1u) != 0;
     const bool inm = ok && (unsigned)i <= (unsigned)Kc.len1 && (unsigned)j <= (unsigned)Kc.len2;
     const int ii = min(max(i - 1, 0), Kc.len1 - 1), jj = min(max(j - 1, 0), Kc.len2 - 1);
-    const int ca = s1[ii], cb = s2[jj];
-    const bool differ = ca != cb;
+    const uint32_t va = q1[ii], vb = q2[jj];
+    const bool differ = (va & 0xFFu) != (vb & 0xFFu);
     const int diag = L.prev2[m] + (differ ? Kc.mismatch : Kc.match);
     const int lraw = PAR ? L.prev[m] : (m >= 1 ? L.prev[m > 0 ? m - 1 : 0] : ext);
     const int uraw = PAR ? (m + 1 < K ? L.prev[m + 1 < K ? m + 1 : 0] : ext) : L.prev[m];
     int gl = Kc.gap, gu = Kc.gap;
     if (HOMO) {
-      gl = h2[jj] ? Kc.homo : Kc.gap;
-      gu = h1[ii] ? Kc.homo : Kc.gap;
+      gl = (vb >> 8) != 0 ? Kc.homo : Kc.gap;
+      gu = (va >> 8) != 0 ? Kc.homo : Kc.gap;
     }
     gl = (i == Kc.len1) ? 0 : gl;  // free end gaps on the last row / column
     gu = (j == Kc.len2) ? 0 : gu;
@@ -157,8 +156,7 @@ __device__ __forceinline__ void band_step(const int t, BandLane<K>& L, const Ban
 // History word m of the block holds cell m's pointers.
 template <bool HOMO, int P0, int K>
 __device__ __forceinline__ void band_block(const int blk, const int last, BandLane<K>& L,
-                                           const BandK Kc, const uint8_t* s1, const uint8_t* s2,
-                                           const uint8_t* h1, const uint8_t* h2,
+                                           const BandK Kc, const uint16_t* q1, const uint16_t* q2,
                                            uint32_t* hist_p, const int lane) {
   const int t0 = blk * 16;
   uint32_t hp[K];
@@ -169,21 +167,134 @@ __device__ __forceinline__ void band_block(const int blk, const int last, BandLa
 #pragma unroll
     for (int k = 0; k < 16; ++k) {
       if (k & 1)
-        band_step<HOMO, 1 - P0, K>(t0 + k, L, Kc, s1, s2, h1, h2, p);
+        band_step<HOMO, 1 - P0, K>(t0 + k, L, Kc, q1, q2, p);
       else
-        band_step<HOMO, P0, K>(t0 + k, L, Kc, s1, s2, h1, h2, p);
+        band_step<HOMO, P0, K>(t0 + k, L, Kc, q1, q2, p);
 #pragma unroll
-      for (int m = 0; m < K; ++m) hp[m] |= p[m] << (2 * k);
+      for (int m = 0; m < K; ++m) hp[m] = hp[m] * 4u + p[m];
     }
   } else {
-    for (int k = 0; t0 + k <= last; ++k) {
+    int k = 0;
+    for (; t0 + k <= last; ++k) {
       if (k & 1)
-        band_step<HOMO, 1 - P0, K>(t0 + k, L, Kc, s1, s2, h1, h2, p);
+        band_step<HOMO, 1 - P0, K>(t0 + k, L, Kc, q1, q2, p);
       else
-        band_step<HOMO, P0, K>(t0 + k, L, Kc, s1, s2, h1, h2, p);
+        band_step<HOMO, P0, K>(t0 + k, L, Kc, q1, q2, p);
 #pragma unroll
-      for (int m = 0; m < K; ++m) hp[m] |= p[m] << (2 * k);
+      for (int m = 0; m < K; ++m) hp[m] = hp[m] * 4u + p[m];
     }
+#pragma unroll
+    for (int m = 0; m < K; ++m) hp[m] <<= 2 * (16 - k);  // step t0 + k' at bits 2 (15 - k')
+  }
+#pragma unroll
+  for (int m = 0; m < K; ++m) hist_p[(blk * K + m) * 64 + lane] = hp[m];
+}
+
+// hp = 2 hp + (1 on the lanes of mask): one add with the mask as carry-in.
+__device__ __forceinline__ uint32_t band_shift_in(uint32_t hp, uint64_t mask) {
+  uint32_t r;
+  uint64_t carry_out;
+  asm("v_addc_co_u32_e64 %0, %1, %2, %2, %3" : "=v"(r), "=s"(carry_out) : "v"(hp), "s"(mask));
+  return r;
+}
+
+// Per-lane constants of the steady steps: for each step parity e (of k) and
+// cell m, the lanes whose cell is a sentinel or past dmax (special), and the
+// value such a cell holds (-9999 or 0).
+template <int K>
+struct BandSpecial {
+  uint64_t mask[2][K];
+  int cval[2][K];
+};
+
+// A steady step (band_step where every cell that is neither a sentinel nor
+// past dmax is interior and off the last row and column): the packed rows
+// give base and homopolymer mask in one read at a fixed offset from the
+// block's base, and the special cells are sent to their constant.  The
+// pointer's two bits are shifted into hp[m].
+// Scores of the steady steps, held in VGPRs (the selects take one
+// scalar operand at most).
+struct BandV {
+  int match, mismatch, gap, homo;
+};
+__device__ __forceinline__ int in_vgpr(int x) {
+  asm volatile("" : "+v"(x));
+  return x;
+}
+
+template <bool HOMO, int PAR, int E, int K>
+__device__ __forceinline__ void band_steady_step(const int k, BandLane<K>& L, const BandV Kc,
+                                                 const uint16_t* const (&q1b)[2][K],
+                                                 const uint16_t* const (&q2b)[2][K],
+                                                 const BandSpecial<K>& S, uint32_t (&hp)[K]) {
+  const int ext = PAR ? from_upper(L.prev[0], 0) : from_lower(L.prev[K - 1], 0);
+  int nv[K];
+#pragma unroll
+  for (int m = 0; m < K; ++m) {
+    const uint32_t va = q1b[E][m][k >> 1], vb = q2b[E][m][k >> 1];
+    const bool eq = (va & 0xFFu) == (vb & 0xFFu);
+    const int diag = L.prev2[m] + (eq ? Kc.match : Kc.mismatch);
+    const int lraw = PAR ? L.prev[m] : (m >= 1 ? L.prev[m > 0 ? m - 1 : 0] : ext);
+    const int uraw = PAR ? (m + 1 < K ? L.prev[m + 1 < K ? m + 1 : 0] : ext) : L.prev[m];
+    const int gl = (HOMO && (vb >> 8) != 0) ? Kc.homo : Kc.gap;
+    const int gu = (HOMO && (va >> 8) != 0) ? Kc.homo : Kc.gap;
+    const int left = lraw + gl, up = uraw + gu;
+    const bool upw = up >= diag && up >= left;
+    const bool lw = left >= diag;
+    const int e = upw ? up : (lw ? left : diag);
+    const bool sp = __builtin_amdgcn_inverse_ballot_w64(S.mask[E][m]);
+    nv[m] = sp ? S.cval[E][m] : e;
+    const uint64_t mup = __builtin_amdgcn_ballot_w64(upw);
+    const uint64_t mlw = __builtin_amdgcn_ballot_w64(lw);
+    const uint64_t meq = __builtin_amdgcn_ballot_w64(eq);
+    hp[m] = band_shift_in(hp[m], (mup | mlw) & ~S.mask[E][m]);
+    hp[m] = band_shift_in(hp[m], (mup | (~mlw & meq)) & ~S.mask[E][m]);
+  }
+#pragma unroll
+  for (int m = 0; m < K; ++m) {
+    L.prev2[m] = L.prev[m];
+    L.prev[m] = nv[m];
+  }
+}
+
+// Steps [16 blk, 16 blk + 16), all steady.  Cell m's row index on the
+// block's steps of parity e advances by one every two steps from the base
+// taken at the block's first step of that parity; bases of special cells
+// are clamped to the padded rows (their reads are discarded).
+// Cell m's bases are cell 0's shifted by (-m, +m) (diagonal d + 2m), so
+// the reads of neighbouring cells coincide; cell 0's bases are clamped to
+// the padded rows far enough that every cell's reads stay inside them.
+template <bool HOMO, int P0, int K>
+__device__ __forceinline__ void band_block_steady(const int blk, BandLane<K>& L, const BandV Kc,
+                                                  const int len1, const int len2,
+                                                  const uint16_t* q1, const uint16_t* q2,
+                                                  const BandSpecial<K>& S, uint32_t* hist_p,
+                                                  const int lane) {
+  static_assert(K <= 2, "steady steps hold K <= 2 cells per lane");
+  const int t0 = blk * 16;
+  const uint16_t* q1b[2][K];
+  const uint16_t* q2b[2][K];
+#pragma unroll
+  for (int e = 0; e < 2; ++e) {
+    const int d = L.d0 + (e ? 1 - P0 : P0);
+    const int ii = ((t0 + e - d) >> 1) - 1, jj = ii + d;
+    const uint16_t* b1 = q1 + min(max(ii, -7), len1 - 1);
+    const uint16_t* b2 = q2 + min(max(jj, -8), len2 - 2);
+#pragma unroll
+    for (int m = 0; m < K; ++m) {
+      q1b[e][m] = b1 - m;
+      q2b[e][m] = b2 + m;
+    }
+  }
+  uint32_t hp[K];
+#pragma unroll
+  for (int m = 0; m < K; ++m) hp[m] = 0;
+#pragma unroll
+  for (int k = 0; k < 16; ++k) {
+    if (k & 1)
+      band_steady_step<HOMO, 1 - P0, 1, K>(k, L, Kc, q1b, q2b, S, hp);
+    else
+      band_steady_step<HOMO, P0, 0, K>(k, L, Kc, q1b, q2b, S, hp);
   }
 #pragma unroll
   for (int m = 0; m < K; ++m) hist_p[(blk * K + m) * 64 + lane] = hp[m];
@@ -194,8 +305,8 @@ struct BandBw {
   int gpos, c, lband, len1, len2, dmax;
   bool sentinels, real, sub_ok;
   size_t my_sub;
-  const uint8_t* s1;
-  const uint8_t* s2;
+  const uint16_t* q1;
+  const uint16_t* q2;
   uint64_t ao;
 };
 struct BandPath {
@@ -218,7 +329,7 @@ __device__ __forceinline__ void band_bw_step(const int t, const int k, const uin
     const uint32_t x = (uint32_t)__builtin_amdgcn_ds_bpermute((B.gpos + owner) * 4, (int)hp[m]);
     w = mi == m ? x : w;
   }
-  int pp = (int)((w >> (2 * k)) & 3u);
+  int pp = (int)__builtin_amdgcn_ubfe(w, (uint32_t)(30 - 2 * k), 2u);
   const bool on = ps.rc == t;
   const int d = ps.D - B.lband - 1;
   const int i = (t - d) >> 1, j = i + d;
@@ -234,8 +345,8 @@ __device__ __forceinline__ void band_bw_step(const int t, const int k, const uin
   }
   if (EMIT) {
     if (on && pp != 0 && B.c == owner && B.real) {
-      a.al0[B.ao + ps.nt] = pp == 2 ? (uint8_t)'-' : B.s1[max(i - 1, 0)];
-      a.al1[B.ao + ps.nt] = pp == 3 ? (uint8_t)'-' : B.s2[max(j - 1, 0)];
+      a.al0[B.ao + ps.nt] = pp == 2 ? (uint8_t)'-' : (uint8_t)B.q1[max(i - 1, 0)];
+      a.al1[B.ao + ps.nt] = pp == 3 ? (uint8_t)'-' : (uint8_t)B.q2[max(j - 1, 0)];
     }
     ps.nt += (on && pp != 0) ? 1 : 0;
   }
@@ -281,8 +392,7 @@ template <bool HOMO, int K>
 __device__ __forceinline__ void band_pairs(const BandDev& a, const BandGeom& geo, const int c,
                                            const int lane, const int g_raw, const bool real,
                                            const DpTask& task, const uint32_t ti,
-                                           const uint8_t* s1, const uint8_t* s2,
-                                           const uint8_t* h1, const uint8_t* h2,
+                                           const uint16_t* q1, const uint16_t* q2,
                                            uint32_t* hist_p) {
   const int len1 = geo.len1, len2 = geo.len2, last = len1 + len2, lanes = geo.lanes;
   BandK Kc;
@@ -310,11 +420,40 @@ __device__ __forceinline__ void band_pairs(const BandDev& a, const BandGeom& geo
 
   // ---- forward: anti-diagonals 0..last ----
   const int P0 = (geo.lband + 1) & 1;  // parity of D on even steps
+  // steady steps: every band diagonal's cell has 1 <= i < len1, 1 <= j < len2
+  const int t_lo = imax(geo.lband, geo.rband) + 2;
+  const int t_hi = imin(2 * len1 - 2 - geo.lband, 2 * len2 - 2 - geo.rband);
+  BandSpecial<K> S;
+#pragma unroll
+  for (int e = 0; e < 2; ++e)
+#pragma unroll
+    for (int m = 0; m < K; ++m) {
+      const int par = e ? 1 - P0 : P0;
+      const bool ok = (((par ? L.okO : L.okE) >> m) & 1u) != 0;
+      const bool sent = (((par ? L.sentO : L.sentE) >> m) & 1u) != 0;
+      S.mask[e][m] = __builtin_amdgcn_ballot_w64(sent || !ok);
+      S.cval[e][m] = sent ? kBandSentinel : 0;
+    }
+  BandV V;
+  V.match = in_vgpr(a.match);
+  V.mismatch = in_vgpr(a.mismatch);
+  V.gap = in_vgpr(a.gap);
+  V.homo = in_vgpr(a.homo_gap);
   for (int blk = 0; blk * 16 <= last; ++blk) {
+    const bool steady = geo.sentinels && blk * 16 >= t_lo && blk * 16 + 15 <= t_hi;
+    if constexpr (K <= 2) {
+      if (steady) {
+        if (P0)
+          band_block_steady<HOMO, 1, K>(blk, L, V, len1, len2, q1, q2, S, hist_p, lane);
+        else
+          band_block_steady<HOMO, 0, K>(blk, L, V, len1, len2, q1, q2, S, hist_p, lane);
+        continue;
+      }
+    }
     if (P0)
-      band_block<HOMO, 1, K>(blk, last, L, Kc, s1, s2, h1, h2, hist_p, lane);
+      band_block<HOMO, 1, K>(blk, last, L, Kc, q1, q2, hist_p, lane);
     else
-      band_block<HOMO, 0, K>(blk, last, L, Kc, s1, s2, h1, h2, hist_p, lane);
+      band_block<HOMO, 0, K>(blk, last, L, Kc, q1, q2, hist_p, lane);
   }
   __syncthreads();
 
@@ -334,8 +473,8 @@ __device__ __forceinline__ void band_pairs(const BandDev& a, const BandGeom& geo
   B.real = real;
   B.sub_ok = sub_ok;
   B.my_sub = sub_ok ? sub0 : 0;
-  B.s1 = s1;
-  B.s2 = s2;
+  B.q1 = q1;
+  B.q2 = q2;
   B.ao = (a.emit && real) ? a.al_off[task.out_idx] : 0;
   BandPath ps;
   ps.rc = real ? last : -1;
@@ -402,31 +541,31 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4, 8))) void
     const DpTask task = a.tasks[first + (uint32_t)g];
     const int len1 = geo.len1, len2 = geo.len2;
 
-    // ---- stage s1 (center), s2 (query) and their homopolymer masks ----
+    // ---- stage s1 (center) and s2 (query) as packed rows ----
     uint8_t* gb = band_lds + (size_t)g * band_group_bytes(geo, HOMO);
-    uint8_t* s1 = gb;
-    uint8_t* s2 = gb + len1;
-    uint8_t* h1 = gb + len1 + len2;
-    uint8_t* h2 = h1 + len1;
+    uint16_t* q1 = reinterpret_cast<uint16_t*>(gb + band_q1_offset(geo));
+    uint16_t* q2 = reinterpret_cast<uint16_t*>(gb + band_q2_offset(geo));
     if (real) {
       const uint8_t* src1 = a.pool + task.s1_off;
       const uint8_t* src2 = a.pool + task.s2_off;
-      for (int x = c; x < len1; x += lanes) s1[x] = src1[x];
-      for (int x = c; x < len2; x += lanes) s2[x] = src2[x];
+      for (int x = c; x < len1; x += lanes) q1[x] = src1[x];
+      for (int x = c; x < len2; x += lanes) q2[x] = src2[x];
     }
     __syncthreads();
     if (HOMO && real) {
-      // position x lies in a run of >= 3 identical bytes
+      // bit 8: position y lies in a run of >= 3 identical bytes; written as
+      // the row entry's high byte, so the low bytes read here never change
       for (int x = c; x < len1 + len2; x += lanes) {
-        const uint8_t* s = x < len1 ? s1 : s2;
+        uint16_t* q = x < len1 ? q1 : q2;
         const int n = x < len1 ? len1 : len2;
         const int y = x < len1 ? x : x - len1;
-        const uint8_t v = s[y];
-        const bool m1 = y >= 1 && s[y - 1] == v;
-        const bool m2 = y >= 2 && m1 && s[y - 2] == v;
-        const bool p1 = y + 1 < n && s[y + 1] == v;
-        const bool p2 = y + 2 < n && p1 && s[y + 2] == v;
-        (x < len1 ? h1 : h2)[y] = (m2 || (m1 && p1) || p2) ? 1 : 0;
+        const uint8_t* qb = reinterpret_cast<const uint8_t*>(q);
+        const uint8_t v = qb[2 * y];
+        const bool m1 = y >= 1 && qb[2 * y - 2] == v;
+        const bool m2 = y >= 2 && m1 && qb[2 * y - 4] == v;
+        const bool p1 = y + 1 < n && qb[2 * y + 2] == v;
+        const bool p2 = y + 2 < n && p1 && qb[2 * y + 4] == v;
+        reinterpret_cast<uint8_t*>(q)[2 * y + 1] = (m2 || (m1 && p1) || p2) ? 1 : 0;
       }
     }
     __syncthreads();
@@ -434,11 +573,11 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4, 8))) void
     uint32_t* hist_p = a.hist + (size_t)blockIdx.x * a.hist_stride;
     const uint32_t ti = first + (uint32_t)g;
     if (geo.k == 1)
-      band_pairs<HOMO, 1>(a, geo, c, lane, g_raw, real, task, ti, s1, s2, h1, h2, hist_p);
+      band_pairs<HOMO, 1>(a, geo, c, lane, g_raw, real, task, ti, q1, q2, hist_p);
     else if (geo.k == 2)
-      band_pairs<HOMO, 2>(a, geo, c, lane, g_raw, real, task, ti, s1, s2, h1, h2, hist_p);
+      band_pairs<HOMO, 2>(a, geo, c, lane, g_raw, real, task, ti, q1, q2, hist_p);
     else
-      band_pairs<HOMO, 4>(a, geo, c, lane, g_raw, real, task, ti, s1, s2, h1, h2, hist_p);
+      band_pairs<HOMO, 4>(a, geo, c, lane, g_raw, real, task, ti, q1, q2, hist_p);
   }
 }
 
