@@ -386,6 +386,72 @@ __device__ __forceinline__ void band_backward(const int last, const uint32_t* hi
   }
 }
 
+// The traceback of a wavefront that holds one pair: the path is
+// wavefront-uniform, so it advances one cell per iteration (a diagonal move
+// skips a step) with its state in scalar registers, and the path cell's
+// pointer word comes from its owner lane by v_readlane.  Same moves, records
+// and end conditions as band_bw_step.
+template <bool EMIT, int K>
+__device__ __forceinline__ void band_backward_solo(const int last, const uint32_t* hist_p,
+                                                   const int lane, BandPath& ps, const BandBw& B,
+                                                   const BandDev& a) {
+  int blk = last >> 4;
+  uint32_t hp[K], hn[K];
+#pragma unroll
+  for (int m = 0; m < K; ++m) {
+    hp[m] = hist_p[(blk * K + m) * 64 + lane];
+    hn[m] = blk >= 1 ? hist_p[((blk - 1) * K + m) * 64 + lane] : 0u;
+  }
+  int rc = uni(ps.rc), D = uni(ps.D), ns = 0, nt = 0;
+  bool bad = false;
+  const bool writer = lane == 0 && B.real;
+  while (rc >= 1) {
+    if ((rc >> 4) != blk) {  // moves take at most two steps: the block below
+      --blk;
+#pragma unroll
+      for (int m = 0; m < K; ++m) {
+        hp[m] = hn[m];
+        hn[m] = blk >= 1 ? hist_p[((blk - 1) * K + m) * 64 + lane] : 0u;
+      }
+    }
+    const int owner = (int)((unsigned)D / (unsigned)(2 * K));
+    const int mi = (D - owner * 2 * K) >> 1;
+    uint32_t src = hp[0];
+#pragma unroll
+    for (int m = 1; m < K; ++m) src = mi == m ? hp[m] : src;
+    const uint32_t w = (uint32_t)__builtin_amdgcn_readlane((int)src, owner);
+    int pp = (int)((w >> (30 - 2 * (rc & 15))) & 3u);
+    const int d = D - B.lband - 1;
+    const int i = (rc - d) >> 1, j = i + d;
+    if (pp == 0) {  // a mismatching diagonal on an interior cell, else unset
+      const bool sentd = B.sentinels && (D == 0 || D == B.dmax);
+      const bool interior = i >= 1 && j >= 1 && i <= B.len1 && j <= B.len2 && !sentd;
+      if (!interior) {
+        bad = true;
+        break;
+      }
+      if (!EMIT && writer && B.sub_ok)
+        a.sub_pool[B.my_sub + ns] = ((uint32_t)(j - 1) << 16) | (uint32_t)(i - 1);
+      ++ns;
+      pp = 1;
+    }
+    if (EMIT) {
+      if (writer) {
+        a.al0[B.ao + nt] = pp == 2 ? (uint8_t)'-' : (uint8_t)B.q1[max(i - 1, 0)];
+        a.al1[B.ao + nt] = pp == 3 ? (uint8_t)'-' : (uint8_t)B.q2[max(j - 1, 0)];
+      }
+      ++nt;
+    }
+    D += pp == 2 ? -1 : (pp == 3 ? 1 : 0);
+    rc = pp == 1 ? rc - 2 : rc - 1;
+  }
+  ps.rc = bad ? -1 : rc;
+  ps.D = D;
+  ps.ns = ns;
+  ps.nt = nt;
+  ps.bad = bad;
+}
+
 // Forward, traceback and outputs of one wavefront whose bin has K cells
 // per lane.
 template <bool HOMO, int K>
@@ -393,7 +459,7 @@ __device__ __forceinline__ void band_pairs(const BandDev& a, const BandGeom& geo
                                            const int lane, const int g_raw, const bool real,
                                            const DpTask& task, const uint32_t ti,
                                            const uint16_t* q1, const uint16_t* q2,
-                                           uint32_t* hist_p) {
+                                           uint32_t* hist_p, const bool solo) {
   const int len1 = geo.len1, len2 = geo.len2, last = len1 + len2, lanes = geo.lanes;
   BandK Kc;
   Kc.len1 = len1;
@@ -482,10 +548,16 @@ __device__ __forceinline__ void band_pairs(const BandDev& a, const BandGeom& geo
   ps.ns = 0;
   ps.nt = 0;
   ps.bad = false;
-  if (a.emit)
+  if (solo) {
+    if (a.emit)
+      band_backward_solo<true, K>(last, hist_p, lane, ps, B, a);
+    else
+      band_backward_solo<false, K>(last, hist_p, lane, ps, B, a);
+  } else if (a.emit) {
     band_backward<true, K>(last, hist_p, lane, ps, B, a);
-  else
+  } else {
     band_backward<false, K>(last, hist_p, lane, ps, B, a);
+  }
   if (__builtin_amdgcn_ballot_w64(ps.bad && real) && lane == 0)
     atomicOr(&a.ctr->err_flags, kErrTraceback);
   __syncthreads();
@@ -573,11 +645,11 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4, 8))) void
     uint32_t* hist_p = a.hist + (size_t)blockIdx.x * a.hist_stride;
     const uint32_t ti = first + (uint32_t)g;
     if (geo.k == 1)
-      band_pairs<HOMO, 1>(a, geo, c, lane, g_raw, real, task, ti, q1, q2, hist_p);
+      band_pairs<HOMO, 1>(a, geo, c, lane, g_raw, real, task, ti, q1, q2, hist_p, G == 1);
     else if (geo.k == 2)
-      band_pairs<HOMO, 2>(a, geo, c, lane, g_raw, real, task, ti, q1, q2, hist_p);
+      band_pairs<HOMO, 2>(a, geo, c, lane, g_raw, real, task, ti, q1, q2, hist_p, G == 1);
     else
-      band_pairs<HOMO, 4>(a, geo, c, lane, g_raw, real, task, ti, q1, q2, hist_p);
+      band_pairs<HOMO, 4>(a, geo, c, lane, g_raw, real, task, ti, q1, q2, hist_p, G == 1);
   }
 }
 
