@@ -386,70 +386,151 @@ __device__ __forceinline__ void band_backward(const int last, const uint32_t* hi
   }
 }
 
+// Path state of the one-pair traceback (wavefront-uniform).  Substitution
+// records wait in two 64-entry registers (record n in lane n & 63 of
+// sbuf[(n >> 6) & 1]) and are written a full register at a time between
+// blocks; `flushed` counts the registers written.
+struct BandSolo {
+  int rc, D, ns, nt, flushed;
+  bool bad;
+  uint32_t sbuf[2];
+};
+
+// s_waitcnt vmcnt(0): a store written here is complete before the next
+// history block is used, so the loads ahead never wait behind stores.
+__device__ __forceinline__ void wait_vm_all() { __builtin_amdgcn_s_waitcnt(0xF70); }
+
+// The path's moves inside history block blk (steps [16 blk, 16 blk + 16)),
+// whose words are hb.  Same moves, records and end conditions as
+// band_bw_step; returns false once the path has ended.
+template <bool EMIT, int K>
+__device__ __forceinline__ bool band_solo_block(const int blk, const uint32_t (&hb)[K],
+                                                BandSolo& st, const bool writer, const int lane,
+                                                const BandBw& B, const BandDev& a) {
+  const int lo = imax(blk * 16, 1);
+  while (st.rc >= lo) {
+    const int owner = (int)((unsigned)st.D / (unsigned)(2 * K));
+    const int mi = (st.D - owner * 2 * K) >> 1;
+    uint32_t w = (uint32_t)__builtin_amdgcn_readlane((int)hb[0], owner);
+#pragma unroll
+    for (int m = 1; m < K; ++m) {
+      const uint32_t x = (uint32_t)__builtin_amdgcn_readlane((int)hb[m], owner);
+      w = mi == m ? x : w;
+    }
+    int pp = (int)((w >> (30 - 2 * (st.rc & 15))) & 3u);
+    const int d = st.D - B.lband - 1;
+    const int i = (st.rc - d) >> 1, j = i + d;
+    if (pp == 0) {  // a mismatching diagonal on an interior cell, else unset
+      const bool sentd = B.sentinels && (st.D == 0 || st.D == B.dmax);
+      const bool interior = i >= 1 && j >= 1 && i <= B.len1 && j <= B.len2 && !sentd;
+      if (!EMIT && interior) {
+        const uint32_t rec = ((uint32_t)(j - 1) << 16) | (uint32_t)(i - 1);
+        const bool mine = lane == (st.ns & 63);
+        if ((st.ns >> 6) & 1)
+          st.sbuf[1] = mine ? rec : st.sbuf[1];
+        else
+          st.sbuf[0] = mine ? rec : st.sbuf[0];
+      }
+      st.ns += interior ? 1 : 0;
+      st.bad = !interior;
+      pp = interior ? 1 : 0;
+    }
+    if (EMIT && pp != 0) {
+      if (writer) {
+        a.al0[B.ao + st.nt] = pp == 2 ? (uint8_t)'-' : (uint8_t)B.q1[max(i - 1, 0)];
+        a.al1[B.ao + st.nt] = pp == 3 ? (uint8_t)'-' : (uint8_t)B.q2[max(j - 1, 0)];
+      }
+      ++st.nt;
+    }
+    // diag: D, two steps back; left (2): D - 1; up (3): D + 1; unset: end
+    st.D += (pp >> 1) * (2 * (pp & 1) - 1);
+    st.rc = pp == 0 ? -1 : st.rc - 1 - (pp == 1 ? 1 : 0);
+  }
+  return st.rc >= 1;
+}
+
+// Writes the register of 64 substitution records filled by the last block
+// (a block adds at most 16 records, so at most one).
+template <bool EMIT>
+__device__ __forceinline__ void band_solo_flush(BandSolo& st, const bool store, const int lane,
+                                                const BandBw& B, const BandDev& a) {
+  if (!EMIT && (st.ns >> 6) > st.flushed) {
+    if (store)
+      a.sub_pool[B.my_sub + st.flushed * 64 + lane] = (st.flushed & 1) ? st.sbuf[1] : st.sbuf[0];
+    ++st.flushed;
+    wait_vm_all();
+  }
+}
+
 // The traceback of a wavefront that holds one pair: the path is
 // wavefront-uniform, so it advances one cell per iteration (a diagonal move
 // skips a step) with its state in scalar registers, and the path cell's
-// pointer word comes from its owner lane by v_readlane.  Same moves, records
-// and end conditions as band_bw_step.
+// pointer word comes from its owner lane by v_readlane.  A move takes at
+// most two steps, so the path visits every block from the top one down.
+// History words come in two groups of kG blocks: group B's load is issued
+// after group A's first block and A's next load after A's last block, so
+// each group's load is issued kG - 1 or more blocks before its first use and
+// no load is younger than that where the loop starts over.
 template <bool EMIT, int K>
 __device__ __forceinline__ void band_backward_solo(const int last, const uint32_t* hist_p,
                                                    const int lane, BandPath& ps, const BandBw& B,
                                                    const BandDev& a) {
-  int blk = last >> 4;
-  uint32_t hp[K], hn[K];
+  constexpr int kG = K == 1 ? 8 : (K == 2 ? 4 : 2);
+  int b = last >> 4;
+  uint32_t ga[kG][K], gb[kG][K];
 #pragma unroll
-  for (int m = 0; m < K; ++m) {
-    hp[m] = hist_p[(blk * K + m) * 64 + lane];
-    hn[m] = blk >= 1 ? hist_p[((blk - 1) * K + m) * 64 + lane] : 0u;
-  }
-  int rc = uni(ps.rc), D = uni(ps.D), ns = 0, nt = 0;
-  bool bad = false;
+  for (int s = 0; s < kG; ++s)
+#pragma unroll
+    for (int m = 0; m < K; ++m) ga[s][m] = hist_p[(imax(b - s, 0) * K + m) * 64 + lane];
+  BandSolo st;
+  st.rc = uni(ps.rc);
+  st.D = uni(ps.D);
+  st.ns = 0;
+  st.nt = 0;
+  st.flushed = 0;
+  st.bad = false;
+  st.sbuf[0] = st.sbuf[1] = 0;
   const bool writer = lane == 0 && B.real;
-  while (rc >= 1) {
-    if ((rc >> 4) != blk) {  // moves take at most two steps: the block below
-      --blk;
+  const bool store = !EMIT && B.sub_ok && B.real;
+  bool more = st.rc >= 1;
+  while (more) {
 #pragma unroll
-      for (int m = 0; m < K; ++m) {
-        hp[m] = hn[m];
-        hn[m] = blk >= 1 ? hist_p[((blk - 1) * K + m) * 64 + lane] : 0u;
-      }
-    }
-    const int owner = (int)((unsigned)D / (unsigned)(2 * K));
-    const int mi = (D - owner * 2 * K) >> 1;
-    uint32_t src = hp[0];
+    for (int s = 0; s < kG; ++s) {
+      more = band_solo_block<EMIT, K>(b - s, ga[s], st, writer, lane, B, a);
+      if (s == 0) {
 #pragma unroll
-    for (int m = 1; m < K; ++m) src = mi == m ? hp[m] : src;
-    const uint32_t w = (uint32_t)__builtin_amdgcn_readlane((int)src, owner);
-    int pp = (int)((w >> (30 - 2 * (rc & 15))) & 3u);
-    const int d = D - B.lband - 1;
-    const int i = (rc - d) >> 1, j = i + d;
-    if (pp == 0) {  // a mismatching diagonal on an interior cell, else unset
-      const bool sentd = B.sentinels && (D == 0 || D == B.dmax);
-      const bool interior = i >= 1 && j >= 1 && i <= B.len1 && j <= B.len2 && !sentd;
-      if (!interior) {
-        bad = true;
-        break;
+        for (int u = 0; u < kG; ++u)
+#pragma unroll
+          for (int m = 0; m < K; ++m)
+            gb[u][m] = hist_p[(imax(b - kG - u, 0) * K + m) * 64 + lane];
       }
-      if (!EMIT && writer && B.sub_ok)
-        a.sub_pool[B.my_sub + ns] = ((uint32_t)(j - 1) << 16) | (uint32_t)(i - 1);
-      ++ns;
-      pp = 1;
+      band_solo_flush<EMIT>(st, store, lane, B, a);
+      if (!more) break;
     }
-    if (EMIT) {
-      if (writer) {
-        a.al0[B.ao + nt] = pp == 2 ? (uint8_t)'-' : (uint8_t)B.q1[max(i - 1, 0)];
-        a.al1[B.ao + nt] = pp == 3 ? (uint8_t)'-' : (uint8_t)B.q2[max(j - 1, 0)];
-      }
-      ++nt;
+    if (!more) break;
+#pragma unroll
+    for (int u = 0; u < kG; ++u)
+#pragma unroll
+      for (int m = 0; m < K; ++m)
+        ga[u][m] = hist_p[(imax(b - 2 * kG - u, 0) * K + m) * 64 + lane];
+#pragma unroll
+    for (int s = 0; s < kG; ++s) {
+      more = band_solo_block<EMIT, K>(b - kG - s, gb[s], st, writer, lane, B, a);
+      band_solo_flush<EMIT>(st, store, lane, B, a);
+      if (!more) break;
     }
-    D += pp == 2 ? -1 : (pp == 3 ? 1 : 0);
-    rc = pp == 1 ? rc - 2 : rc - 1;
+    b -= 2 * kG;
   }
-  ps.rc = bad ? -1 : rc;
-  ps.D = D;
-  ps.ns = ns;
-  ps.nt = nt;
-  ps.bad = bad;
+  if (!EMIT) {
+    for (; st.flushed * 64 < st.ns; ++st.flushed)
+      if (store && lane < st.ns - st.flushed * 64)
+        a.sub_pool[B.my_sub + st.flushed * 64 + lane] = (st.flushed & 1) ? st.sbuf[1] : st.sbuf[0];
+  }
+  ps.rc = st.rc;
+  ps.D = st.D;
+  ps.ns = st.ns;
+  ps.nt = st.nt;
+  ps.bad = st.bad;
 }
 
 // Forward, traceback and outputs of one wavefront whose bin has K cells
