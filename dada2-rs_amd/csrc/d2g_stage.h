@@ -164,14 +164,33 @@ __device__ __forceinline__ void stage_round(const StageRound& R, uint32_t p0, ui
   }
 }
 
+// Substitution cursor of one DP pair.  A traceback writes a pair's records
+// in descending query position (the path runs from the matrix's end, and a
+// substitution is a diagonal move), so the segments, staged in ascending
+// position, consume them from the end of the list: records [0, left) are
+// still to patch, and nqp is the query position of record left - 1 once a
+// batch has read it beyond the current segment (kQpUnknown before).
+constexpr uint32_t kQpUnknown = 0xFFFFFFFFu;
+struct SubCursor {
+  uint32_t left, nqp;
+};
+__device__ __forceinline__ SubCursor sub_cursor(const TaskRow& me) {
+  SubCursor c;
+  c.left = me.cnt;
+  c.nqp = kQpUnknown;
+  return c;
+}
+
 // Stages the segment at s0 of the wavefront's pairs (rows: the wavefront's
 // kStPairsW rows; myb: the lane's own row, row kStPairsW - 1 for lanes that
 // own none): two rounds of codes, the second round's words requested before
-// the first round's bytes are converted, then the DP substitution patches.
-// The caller runs wave_lds_sync() before reading other lanes' writes.
+// the first round's bytes are converted, then the DP substitution patches of
+// the segment, kStSubBatch records per round from the cursor.  The caller
+// runs wave_lds_sync() before reading other lanes' writes.
 __device__ __forceinline__ void stage_segment(const TaskSet& a, const TaskRow& me, uint32_t s0,
                                               uint32_t lane, bool quals, bool any_gl,
-                                              uint8_t* rows, uint8_t* myb, uint32_t& ham) {
+                                              uint8_t* rows, uint8_t* myb, uint32_t& ham,
+                                              SubCursor& cur) {
   const uint8_t* seq = a.u.seq;
   const uint8_t* qual = quals ? a.u.qual : a.u.seq;
   StageRound ra, rb;
@@ -181,31 +200,47 @@ __device__ __forceinline__ void stage_segment(const TaskSet& a, const TaskRow& m
   stage_round(ra, 0, s0, lane, quals, any_gl, rows, ham);
   __builtin_amdgcn_sched_barrier(0);
   stage_round(rb, kStUnroll, s0, lane, quals, any_gl, rows, ham);
-  // DP substitutions: each owning lane patches its pair's row (lanes that
-  // own none have cnt = 0 and write their row's pad byte, which nothing reads)
-  for (uint32_t k0 = 0; __builtin_amdgcn_ballot_w64(k0 < me.cnt); k0 += kStSubBatch) {
+  // DP substitutions of this segment (lanes that own none have left = 0;
+  // lanes without a record here write their row's pad byte, which nothing
+  // reads)
+  const uint32_t end = s0 + (uint32_t)kStSeg;
+  bool want = cur.left > 0 && (cur.nqp == kQpUnknown || cur.nqp < end);
+  while (__builtin_amdgcn_ballot_w64(want)) {
     uint32_t rec[kStSubBatch], cb[kStSubBatch];
 #pragma unroll
     for (int j = 0; j < kStSubBatch; ++j) {
-      const uint32_t k = k0 + (uint32_t)j;
-      // empty slots read the pool's first word: a pair whose list
+      // slots past the list read the pool's first word: a pair whose list
       // overflowed the pool has cnt = 0 and a slot past the pool's end
-      rec[j] = ld_off<uint32_t>(a.sub_pool, (k < me.cnt ? me.sub0 + k : 0u) << 2);
+      const bool v = want && (uint32_t)j < cur.left;
+      rec[j] = ld_off<uint32_t>(a.sub_pool, (v ? me.sub0 + cur.left - 1u - (uint32_t)j : 0u) << 2);
     }
 #pragma unroll
     for (int j = 0; j < kStSubBatch; ++j) {
-      const uint32_t k = k0 + (uint32_t)j;
-      cb[j] = ld_off<uint8_t>(seq, me.coff + (k < me.cnt ? rec[j] & 0xFFFFu : 0u));
+      const bool v = want && (uint32_t)j < cur.left;
+      cb[j] = ld_off<uint8_t>(seq, me.coff + (v ? rec[j] & 0xFFFFu : 0u));
     }
+    // records ascend in query position along j: the ones in this segment
+    // are a prefix
+    uint32_t take = 0, nq = kQpUnknown;
+    bool beyond = false;
 #pragma unroll
     for (int j = 0; j < kStSubBatch; ++j) {
-      const uint32_t k = k0 + (uint32_t)j;
+      const bool v = want && (uint32_t)j < cur.left;
       const uint32_t qp = rec[j] >> 16;
-      const bool here = k < me.cnt && qp >= s0 && qp - s0 < (uint32_t)kStSeg;
+      const bool in = v && !beyond && qp < end;
+      nq = (v && !beyond && !in) ? qp : nq;
+      beyond = beyond || (v && !in);
+      take += in ? 1u : 0u;
+      const bool here = in && qp >= s0;
       uint8_t* c = myb + (here ? 2 * (qp - s0) : 2 * kStSeg);
       const uint32_t nt1 = (uint32_t)c[0] & 3u;
       c[0] = (uint8_t)(((4u * (cb[j] - 1u) + nt1) << 2) | nt1);
     }
+    const uint32_t avail = want ? min(cur.left, (uint32_t)kStSubBatch) : 0u;
+    cur.left -= take;
+    cur.nqp = beyond ? nq : kQpUnknown;
+    // a full batch in this segment may be followed by more of it
+    want = want && !beyond && take == avail && cur.left > 0;
   }
 }
 

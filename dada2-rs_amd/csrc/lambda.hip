@@ -62,8 +62,9 @@ __global__ __launch_bounds__(64 * kStWaves) void lambda_kernel(LambdaDev a) {
     const bool any_gl = __builtin_amdgcn_ballot_w64(me.gl) != 0;
     double lam = 1.0;
     uint32_t ham = 0;
+    SubCursor cur = sub_cursor(me);
     for (uint32_t s0 = 0; __builtin_amdgcn_ballot_w64(s0 < me.lq); s0 += kStSeg) {
-      stage_segment(a.ts, me, s0, lane, quals, any_gl, rows, myb, ham);
+      stage_segment(a.ts, me, s0, lane, quals, any_gl, rows, myb, ham, cur);
       // the wavefront's staged rows are read by other lanes of the same
       // wavefront (segment counts differ between wavefronts, so no
       // workgroup barrier here)

@@ -50,8 +50,9 @@ __global__ __launch_bounds__(64 * kStWaves) void loglik_fwd_kernel(LoglikDev a, 
     const bool any_gl = __builtin_amdgcn_ballot_w64(me.gl) != 0;
     double ll_lam = 0.0;
     uint32_t ham = 0;
+    SubCursor cur = sub_cursor(me);
     for (uint32_t s0 = 0; __builtin_amdgcn_ballot_w64(s0 < me.lq); s0 += kStSeg) {
-      stage_segment(a.ts, me, s0, lane, quals, any_gl, rows, myb, ham);
+      stage_segment(a.ts, me, s0, lane, quals, any_gl, rows, myb, ham, cur);
       wave_lds_sync();
       const uint32_t seg_n = me.lq > s0 ? min(me.lq - s0, (uint32_t)kStSeg) : 0u;
       if (lane < (uint32_t)kStPairsW) {
@@ -102,8 +103,9 @@ __global__ __launch_bounds__(64 * kStWaves) void loglik_bwd_kernel(LoglikDev a) 
     const bool any_gl = __builtin_amdgcn_ballot_w64(me.gl) != 0;
     const double wt = me.live ? a.w[me.t] : 0.0;
     uint32_t ham = 0;
+    SubCursor cur = sub_cursor(me);
     for (uint32_t s0 = 0; __builtin_amdgcn_ballot_w64(s0 < me.lq); s0 += kStSeg) {
-      stage_segment(a.ts, me, s0, lane, quals, any_gl, rows, myb, ham);
+      stage_segment(a.ts, me, s0, lane, quals, any_gl, rows, myb, ham, cur);
       wave_lds_sync();
       const uint32_t seg_n = me.lq > s0 ? min(me.lq - s0, (uint32_t)kStSeg) : 0u;
       if (lane < (uint32_t)kStPairsW && me.live) {
