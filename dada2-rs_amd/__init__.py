@@ -124,6 +124,10 @@ def _lib():
             "d2g_ctx_set_graphs": (C.c_int, [vp, C.c_int]),
             "d2g_ctx_set_screen_shape": (C.c_int, [vp, C.c_int, C.c_int, C.c_int]),
             "d2g_ctx_set_autotune": (C.c_int, [vp, C.c_int]),
+            "d2g_run_dada": (C.c_int, [vp, f64p, C.c_uint32, C.POINTER(AlignParams),
+                                       C.POINTER(DadaParams), u8p, C.c_uint32,
+                                       C.POINTER(C.c_uint32), C.POINTER(ClusterInfo), u32p, f64p,
+                                       C.POINTER(C.c_int32), f64p, u32p, C.POINTER(DadaStats)]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -142,7 +146,7 @@ def exported_symbols() -> list[str]:
             "d2g_align_endsfree", "d2g_band_supported", "d2g_pairs_load", "d2g_pairs_lambda",
             "d2g_pairs_fetch", "d2g_bimera_pairs", "d2g_is_bimera_batch", "d2g_table_bimera2",
             "d2g_loglik_forward", "d2g_loglik_backward", "d2g_ctx_set_graphs",
-            "d2g_ctx_set_screen_shape", "d2g_ctx_set_autotune"]
+            "d2g_ctx_set_screen_shape", "d2g_ctx_set_autotune", "d2g_run_dada"]
 
 
 def _p(a: np.ndarray, t):
@@ -194,6 +198,56 @@ class PairStats(C.Structure):
 
     def as_dict(self) -> dict:
         return {name: getattr(self, name) for name, _ in self._fields_ if name != "_pad"}
+
+
+class DadaParams(C.Structure):
+    """DadaParams fields of the partition loop (dada.rs:54-90)."""
+    _fields_ = [("omega_a", C.c_double), ("omega_p", C.c_double), ("omega_c", C.c_double),
+                ("min_fold", C.c_double), ("min_hamming", C.c_uint32), ("min_abund", C.c_uint32),
+                ("max_clust", C.c_uint32), ("max_shuffle", C.c_uint32), ("greedy", C.c_int32),
+                ("detect_singletons", C.c_int32), ("use_quals", C.c_int32),
+                ("_reserved", C.c_int32)]
+
+
+def dada_params(omega_a=1e-40, omega_p=1e-4, omega_c=1e-40, min_fold=1.0, min_hamming=1,
+                min_abund=1, max_clust=0, max_shuffle=10, greedy=True, detect_singletons=False,
+                use_quals=True) -> DadaParams:
+    """Defaults of resolve_dada_params (main.rs:4393-4433); MAX_SHUFFLE = 10
+    (dada.rs:39)."""
+    return DadaParams(omega_a, omega_p, omega_c, min_fold, min_hamming, min_abund, max_clust,
+                      max_shuffle, int(greedy), int(detect_singletons), int(use_quals), 0)
+
+
+class ClusterInfo(C.Structure):
+    _fields_ = [("center", C.c_uint32), ("reads", C.c_uint32), ("nraw", C.c_uint32),
+                ("birth_type", C.c_int32), ("birth_from", C.c_uint32),
+                ("birth_hamming", C.c_uint32), ("birth_pval", C.c_double),
+                ("birth_fold", C.c_double), ("birth_e", C.c_double),
+                ("birth_lambda", C.c_double), ("self_lambda", C.c_double)]
+
+
+class DadaStats(C.Structure):
+    _fields_ = [("nalign", C.c_uint64), ("nshroud", C.c_uint64), ("compares", C.c_uint32),
+                ("shuffle_passes", C.c_uint32), ("ms_compare", C.c_double),
+                ("ms_store", C.c_double), ("ms_shuffle", C.c_double), ("ms_bud", C.c_double),
+                ("ms_pupdate", C.c_double), ("ms_total", C.c_double)]
+
+    def as_dict(self):
+        return {name: getattr(self, name) for name, _ in self._fields_}
+
+
+@dataclass
+class DadaResult:
+    """The partition after run_dada and the final p pass (dada.rs:502-533):
+    per cluster (center, reads, nraw, birth_*, self_lambda) and per unique
+    (cluster_of, p, map with -1 for uncorrected, comp_lambda, comp_hamming)."""
+    clusters: list
+    cluster_of: np.ndarray
+    p: np.ndarray
+    map: np.ndarray
+    comp_lambda: np.ndarray
+    comp_hamming: np.ndarray
+    stats: dict
 
 
 @dataclass
@@ -327,6 +381,36 @@ class Context:
                                          _p(ham, u32p), _p(skp, u8p), C.byref(st)),
                "d2g_compare_center")
         return CompareResult(lam, ham, skp, st.as_dict())
+
+    def run_dada(self, err, params: AlignParams | None = None, dada: DadaParams | None = None,
+                 prior=None) -> DadaResult:
+        """run_dada (dada.rs:749) over the loaded uniques, then the final p
+        pass and read map (dada.rs:502-533).  Every cluster's compare runs on
+        the device; the partition state lives on the host."""
+        params = params or align_params()
+        dada = dada or dada_params()
+        e = np.ascontiguousarray(err, np.float64)
+        n = self.n
+        pr = None if prior is None else np.ascontiguousarray(prior, np.uint8)
+        if pr is not None and pr.shape != (n,):
+            raise ValueError("prior needs one flag per unique")
+        cap = max(n, 1)
+        infos = (ClusterInfo * cap)()
+        ncl = C.c_uint32()
+        cof = np.zeros(n, np.uint32)
+        pv = np.zeros(n, np.float64)
+        mp = np.zeros(n, np.int32)
+        cl = np.zeros(n, np.float64)
+        ch = np.zeros(n, np.uint32)
+        st = DadaStats()
+        _check(_lib().d2g_run_dada(self.h, _p(e, f64p), e.shape[1], C.byref(params),
+                                   C.byref(dada), _p(pr, u8p) if pr is not None else None, cap,
+                                   C.byref(ncl), infos, _p(cof, u32p), _p(pv, f64p),
+                                   mp.ctypes.data_as(C.POINTER(C.c_int32)), _p(cl, f64p),
+                                   _p(ch, u32p), C.byref(st)), "d2g_run_dada")
+        clusters = [{name: getattr(infos[i], name) for name, _ in ClusterInfo._fields_}
+                    for i in range(ncl.value)]
+        return DadaResult(clusters, cof, pv, mp, cl, ch, st.as_dict())
 
     def pairs_load(self, seqs, quals, subs, reads, center_reads):
         """Resident aligned pairs: query i with its substitution list subs[i]

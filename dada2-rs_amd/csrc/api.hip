@@ -57,6 +57,7 @@ struct d2g_ctx {
   std::vector<uint32_t> h_len;   // sorted order
   std::vector<uint32_t> h_orig;  // sorted -> original
   std::vector<uint32_t> h_inv;   // original -> sorted
+  std::vector<uint32_t> h_reads; // read counts, caller order
   uint32_t max_qual = 0;
   bool has_qual = false;
   DevBuf<uint8_t> seq, qual, has255;
@@ -361,6 +362,7 @@ int d2g_load_uniques(d2g_ctx* c, uint32_t n, const uint8_t* seq_concat, const ui
     std::memcpy(&sseq[soff[s]], seq_concat + offsets[order[s]], slen[s]);
     if (qual_concat) std::memcpy(&squal[soff[s]], qual_concat + offsets[order[s]], slen[s]);
   }
+  c->h_reads.assign(reads, reads + n);
   UniqueDev& u = c->u;
   u.n = n;
   u.npad = (n + 63) & ~63u;
@@ -1450,3 +1452,11 @@ int d2g_pairs_fetch(d2g_ctx* c, float* out_loglam32, double* out_lambda, double*
 }
 
 }  // extern "C"
+
+// Read counts of the loaded uniques in the caller's order (empty before a
+// load); the partition loop's host state starts from them.
+namespace d2g {
+const std::vector<uint32_t>* ctx_host_reads(const d2g_ctx* c) {
+  return (c && c->loaded) ? &c->h_reads : nullptr;
+}
+}  // namespace d2g

@@ -1,0 +1,496 @@
+// partition.hip — run_dada's partition loop (dada.rs:749-1150) and the final
+// per-raw p-value pass and read map (dada.rs:502-533), on the host, with the
+// compare of each new cluster center on the device (d2g_compare_center).
+// Host code only; it links against the rest of the library through the C
+// entry points.
+//
+// Semantics restated from the reference:
+//   * B::new / init (containers.rs:337-391): one cluster holding every raw in
+//     index order; e_minmax starts at -999.0 per raw and is never reset;
+//     the center is the raw with the most reads, the last one among equals
+//     (assign_center, containers.rs:450-463: Rust's max_by_key keeps the
+//     last maximum).
+//   * b_compare (cluster.rs:106-164): the device returns each raw's lambda
+//     and hamming; the store is serial: a comparison is kept when
+//     lambda * B.reads > e_minmax[raw], e_minmax rises to lambda *
+//     center_reads, and Raw.comp follows it for cluster 0 and for the
+//     center itself.  Cluster 0's compare uses kdist_cutoff 1.0
+//     (dada.rs:835-838).
+//   * b_p_update (pval.rs:45-130): clusters flagged update_e reprice their
+//     members with get_pA (pval.rs:376-398); with greedy, a cluster's first
+//     p-update locks the members its center explains and the center.
+//   * b_bud (cluster.rs:1331-1480): the smallest p among non-first members
+//     (reads as tie-break) over all clusters, seeded with cluster 0's
+//     center; a Bonferroni test against omega_a, then the prior test against
+//     omega_p.  b_bud_incremental (cluster.rs:1501) is its cached equivalent.
+//   * b_shuffle_converge (cluster.rs:908) is byte-identical to repeating
+//     b_shuffle2 (cluster.rs:615-700) until a pass moves nothing, with one
+//     difference in counting: with the carried map (on by default,
+//     cluster.rs:79-82, 957) every call after the first spends its first
+//     MAX_SHUFFLE iteration on a reconcile (cluster.rs:1037, 1066-1073), so
+//     the first call makes up to max_shuffle move passes and later calls up
+//     to max_shuffle - 1.
+//   * calc_pA (pval.rs:208-241): statrs's Poisson survival function is the
+//     regularised lower incomplete gamma P(reads, e), evaluated by the
+//     series / Lentz continued fraction of Numerical Recipes (3rd ed.,
+//     section 6.2), with the reference's own small-lambda series when it
+//     returns 0 (pval.rs:329-353).
+#include <algorithm>
+#include <cfloat>
+#include <chrono>
+#include <cmath>
+#include <cstdint>
+#include <vector>
+
+#include "../../include/dada2_amd.h"
+
+namespace d2g {
+const std::vector<uint32_t>* ctx_host_reads(const d2g_ctx* c);
+}
+
+namespace {
+
+// Comparison, containers.rs:15-20.
+struct Comp {
+  uint32_t i = 0, index = 0;
+  double lambda = 0.0;
+  uint32_t hamming = 0;
+};
+
+// The Raw fields the loop reads and writes (containers.rs:78-108).
+struct Raw {
+  uint32_t reads = 0;
+  bool prior = false, lock = false, correct = false;
+  double p = 0.0;
+  Comp comp;
+};
+
+enum : int32_t { kBirthInitial = 0, kBirthAbundance = 1, kBirthPrior = 2 };
+
+// Bi, containers.rs:200-265.
+struct Cluster {
+  std::vector<uint32_t> raws;
+  uint32_t reads = 0;
+  int64_t center = -1;
+  bool update_e = true, check_locks = true;
+  double self = 0.0;
+  int32_t birth_type = kBirthInitial;
+  uint32_t birth_from = 0;
+  double birth_pval = 0.0, birth_fold = 1.0, birth_e = 0.0;
+  Comp birth_comp;
+  std::vector<Comp> comp;
+};
+
+// B, containers.rs:268-337.
+struct Part {
+  std::vector<Raw> raws;
+  std::vector<Cluster> cl;
+  std::vector<double> e_minmax;
+  uint32_t reads = 0;
+  uint64_t nalign = 0, nshroud = 0;
+
+  // bi_add_raw, containers.rs:403-417
+  void add_raw(size_t ci, uint32_t r) {
+    cl[ci].raws.push_back(r);
+    cl[ci].reads += raws[r].reads;
+    cl[ci].update_e = true;
+  }
+  // bi_pop_raw, containers.rs:428-439: swap-remove
+  uint32_t pop_raw(size_t ci, size_t pos) {
+    std::vector<uint32_t>& v = cl[ci].raws;
+    const uint32_t r = v[pos];
+    v[pos] = v.back();
+    v.pop_back();
+    cl[ci].reads -= raws[r].reads;
+    cl[ci].update_e = true;
+    return r;
+  }
+  // assign_center, containers.rs:450-463 (last raw among the most reads)
+  void assign_center(size_t ci) {
+    int64_t best = -1;
+    for (uint32_t r : cl[ci].raws)
+      if (best < 0 || raws[r].reads >= raws[(size_t)best].reads) best = r;
+    if (best >= 0) cl[ci].center = best;
+  }
+};
+
+// ---- p-values (pval.rs) ----------------------------------------------------
+
+// Regularised lower incomplete gamma P(a, x).
+double gamma_p(double a, double x) {
+  if (x <= 0.0) return 0.0;
+  const double gln = std::lgamma(a);
+  if (x < a + 1.0) {
+    double ap = a, del = 1.0 / a, sum = del;
+    for (int n = 0; n < 100000; ++n) {
+      ap += 1.0;
+      del *= x / ap;
+      sum += del;
+      if (std::fabs(del) < std::fabs(sum) * DBL_EPSILON) break;
+    }
+    return sum * std::exp(-x + a * std::log(x) - gln);
+  }
+  const double fpmin = DBL_MIN / DBL_EPSILON;
+  double bb = x + 1.0 - a, c = 1.0 / fpmin, d = 1.0 / bb, h = d;
+  for (int i = 1; i < 100000; ++i) {
+    const double an = -i * (i - a);
+    bb += 2.0;
+    d = an * d + bb;
+    if (std::fabs(d) < fpmin) d = fpmin;
+    c = bb + an / c;
+    if (std::fabs(c) < fpmin) c = fpmin;
+    d = 1.0 / d;
+    const double del = d * c;
+    h *= del;
+    if (std::fabs(del - 1.0) <= DBL_EPSILON) break;
+  }
+  return 1.0 - std::exp(-x + a * std::log(x) - gln) * h;
+}
+
+// poisson_upper_tail_direct, pval.rs:329-353
+double poisson_tail_direct(uint32_t reads, double lambda) {
+  const double k = (double)reads;
+  const double log_lambda = std::log(lambda);
+  double log_kfact = 0.0;
+  for (uint32_t i = 1; i <= reads; ++i) log_kfact += std::log((double)i);
+  const double log_leading = -lambda + k * log_lambda - log_kfact;
+  double series = 1.0, term = 1.0;
+  for (uint32_t j = 1; j < 10000u; ++j) {
+    term *= lambda / (k + (double)j);
+    if (!std::isfinite(term) || term <= 0.0) break;
+    const double ns = series + term;
+    if (ns == series) break;
+    series = ns;
+  }
+  return std::exp(log_leading + std::log(series));
+}
+
+// calc_pA, pval.rs:208-241
+double calc_pA(uint32_t reads, double e_reads, bool prior) {
+  if (!(e_reads > 0.0)) return 1.0;  // Poisson::new rejects e_reads <= 0 and NaN
+  double pval = gamma_p((double)reads, e_reads);  // sf(reads - 1)
+  if (pval == 0.0) pval = poisson_tail_direct(reads, e_reads);
+  if (prior) return pval;
+  double norm = 1.0 - std::exp(-e_reads);
+  if (norm < 1e-7) norm = e_reads - 0.5 * e_reads * e_reads;  // TAIL_APPROX_CUTOFF, pval.rs:18
+  return pval / norm;
+}
+
+// get_pA, pval.rs:376-398
+double get_pA(uint32_t reads, bool prior, double lambda, uint32_t hamming, uint32_t bi_reads,
+              bool detect_singletons) {
+  if (reads == 1 && !prior && !detect_singletons) return 1.0;
+  if (hamming == 0) return 1.0;
+  if (lambda == 0.0) return 0.0;
+  return calc_pA(reads, lambda * (double)bi_reads, prior || detect_singletons);
+}
+
+// ---- the loop's steps ------------------------------------------------------
+
+struct Run {
+  d2g_ctx* ctx;
+  const double* err;
+  uint32_t ncol;
+  d2g_align_params align;
+  d2g_dada_params dp;
+  Part B;
+  std::vector<double> lam;
+  std::vector<uint32_t> ham;
+  std::vector<uint8_t> skp, lock;
+  d2g_dada_stats st{};
+
+  static double ms_since(std::chrono::steady_clock::time_point t0) {
+    return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+  }
+
+  // b_compare, cluster.rs:106-164
+  int compare(size_t i) {
+    const uint32_t n = (uint32_t)B.raws.size();
+    const uint32_t center = (uint32_t)B.cl[i].center;
+    const uint32_t center_reads = B.raws[center].reads;
+    for (uint32_t r = 0; r < n; ++r) lock[r] = B.raws[r].lock ? 1 : 0;
+    d2g_align_params p = align;
+    if (i == 0) p.kdist_cutoff = 1.0;  // dada.rs:835-838
+    auto t0 = std::chrono::steady_clock::now();
+    const int rc = d2g_compare_center(ctx, center, err, ncol, &p, dp.use_quals, dp.greedy,
+                                      lock.data(), lam.data(), ham.data(), skp.data(), nullptr);
+    st.ms_compare += ms_since(t0);
+    ++st.compares;
+    if (rc != D2G_OK) return rc;
+    t0 = std::chrono::steady_clock::now();
+    const double total = (double)B.reads;
+    Cluster& C = B.cl[i];
+    for (uint32_t r = 0; r < n; ++r) {
+      if (!skp[r]) {
+        ++B.nalign;
+        if (ham[r] == D2G_HAMMING_NONE) ++B.nshroud;
+      }
+      const double lambda = lam[r];
+      if (r == center) C.self = lambda;
+      if (lambda * total > B.e_minmax[r]) {
+        const double e = lambda * (double)center_reads;
+        if (e > B.e_minmax[r]) B.e_minmax[r] = e;
+        Comp c;
+        c.i = (uint32_t)i;
+        c.index = r;
+        c.lambda = lambda;
+        c.hamming = ham[r] == D2G_HAMMING_NONE ? 0u : ham[r];
+        C.comp.push_back(c);
+        if (i == 0 || r == center) B.raws[r].comp = c;
+      }
+    }
+    st.ms_store += ms_since(t0);
+    return D2G_OK;
+  }
+
+  // b_p_update, pval.rs:45-130
+  void p_update() {
+    const auto t0 = std::chrono::steady_clock::now();
+    for (Cluster& C : B.cl) {
+      if (C.update_e) {
+        for (uint32_t r : C.raws) {
+          Raw& R = B.raws[r];
+          R.p = get_pA(R.reads, R.prior, R.comp.lambda, R.comp.hamming, C.reads,
+                       dp.detect_singletons != 0);
+        }
+        C.update_e = false;
+      }
+      if (dp.greedy && C.check_locks) {
+        const uint32_t center_reads = C.center >= 0 ? B.raws[(size_t)C.center].reads : 0u;
+        for (uint32_t r : C.raws) {
+          Raw& R = B.raws[r];
+          if ((double)center_reads * R.comp.lambda > (double)R.reads) R.lock = true;
+          if ((int64_t)r == C.center) R.lock = true;
+        }
+        C.check_locks = false;
+      }
+    }
+    st.ms_pupdate += ms_since(t0);
+  }
+
+  // The tail of b_bud (cluster.rs:1419-1449): pop raw r of cluster ci into a
+  // new cluster born with `type`.
+  size_t bud_into(size_t ci, size_t pos, uint32_t r, int32_t type, double pval) {
+    const double expected = B.raws[r].comp.lambda * (double)B.cl[ci].reads;
+    Cluster nb;
+    nb.birth_type = type;
+    nb.birth_from = type == kBirthAbundance ? (uint32_t)ci : 0u;
+    nb.birth_pval = pval;
+    nb.birth_fold = (double)B.raws[r].reads / std::max(expected, DBL_MIN);
+    nb.birth_e = expected;
+    nb.birth_comp = B.raws[r].comp;
+    B.pop_raw(ci, pos);
+    B.cl.push_back(std::move(nb));
+    const size_t ni = B.cl.size() - 1;
+    B.add_raw(ni, r);
+    B.assign_center(ni);
+    return ni;
+  }
+
+  // b_bud, cluster.rs:1331-1480; returns the new cluster or -1
+  int64_t bud() {
+    const auto t0 = std::chrono::steady_clock::now();
+    const double nraw = (double)B.raws.size();
+    const uint32_t c0 = (uint32_t)B.cl[0].center;
+    bool have = false, have_prior = false;
+    size_t mci = 0, mpos = 0, pci = 0, ppos = 0;
+    double min_p = B.raws[c0].p, min_p_prior = min_p;
+    uint32_t min_reads = B.raws[c0].reads, min_reads_prior = min_reads;
+    for (size_t ci = 0; ci < B.cl.size(); ++ci) {
+      const Cluster& C = B.cl[ci];
+      for (size_t pos = 1; pos < C.raws.size(); ++pos) {
+        const Raw& R = B.raws[C.raws[pos]];
+        if (R.reads < dp.min_abund) continue;
+        if (R.comp.hamming < dp.min_hamming) continue;
+        const bool fold_ok =
+            dp.min_fold <= 1.0 || (double)R.reads >= dp.min_fold * R.comp.lambda * (double)C.reads;
+        if (!fold_ok) continue;
+        if (R.p < min_p || (R.p == min_p && R.reads > min_reads)) {
+          have = true;
+          mci = ci;
+          mpos = pos;
+          min_p = R.p;
+          min_reads = R.reads;
+        }
+        if (R.prior && (R.p < min_p_prior || (R.p == min_p_prior && R.reads > min_reads_prior))) {
+          have_prior = true;
+          pci = ci;
+          ppos = pos;
+          min_p_prior = R.p;
+          min_reads_prior = R.reads;
+        }
+      }
+    }
+    const double p_a = min_p * nraw;
+    int64_t out = -1;
+    if (p_a < dp.omega_a && have)
+      out = (int64_t)bud_into(mci, mpos, B.cl[mci].raws[mpos], kBirthAbundance, p_a);
+    else if (min_p_prior < dp.omega_p && have_prior)
+      out = (int64_t)bud_into(pci, ppos, B.cl[pci].raws[ppos], kBirthPrior, min_p_prior);
+    st.ms_bud += ms_since(t0);
+    return out;
+  }
+
+  // b_shuffle2, cluster.rs:615-700: one move pass; returns the moves
+  size_t shuffle_pass(std::vector<double>& emax, std::vector<Comp>& compmax) {
+    const size_t n = B.raws.size();
+    emax.assign(n, -INFINITY);
+    compmax.assign(n, Comp());
+    const double c0 = (double)B.cl[0].reads;
+    for (const Comp& c : B.cl[0].comp) {
+      emax[c.index] = c.lambda * c0;
+      compmax[c.index] = c;
+    }
+    for (size_t ci = 1; ci < B.cl.size(); ++ci) {
+      const double cr = (double)B.cl[ci].reads;
+      for (const Comp& c : B.cl[ci].comp) {
+        const double e = c.lambda * cr;
+        if (e > emax[c.index]) {
+          emax[c.index] = e;
+          compmax[c.index] = c;
+        }
+      }
+    }
+    size_t moves = 0;
+    for (size_t ci = 0; ci < B.cl.size(); ++ci) {
+      size_t pos = B.cl[ci].raws.size();
+      while (pos > 0) {
+        --pos;
+        const uint32_t r = B.cl[ci].raws[pos];
+        const size_t best = compmax[r].i;
+        if (best == ci) continue;
+        if (B.cl[ci].center == (int64_t)r) continue;
+        B.pop_raw(ci, pos);
+        B.add_raw(best, r);
+        B.raws[r].comp = compmax[r];
+        ++moves;
+      }
+    }
+    return moves;
+  }
+
+  // b_shuffle_converge, cluster.rs:908-1320 (see the counting note above)
+  void shuffle(bool first) {
+    const auto t0 = std::chrono::steady_clock::now();
+    const uint32_t passes = first ? dp.max_shuffle : std::max(dp.max_shuffle, 2u) - 1u;
+    std::vector<double> emax;
+    std::vector<Comp> compmax;
+    for (uint32_t k = 0; k < passes; ++k) {
+      ++st.shuffle_passes;
+      if (shuffle_pass(emax, compmax) == 0) break;
+    }
+    st.ms_shuffle += ms_since(t0);
+  }
+};
+
+}  // namespace
+
+extern "C" int d2g_run_dada(d2g_ctx* ctx, const double* err, uint32_t ncol,
+                            const d2g_align_params* align, const d2g_dada_params* dada,
+                            const uint8_t* prior, uint32_t cluster_capacity,
+                            uint32_t* out_nclusters, d2g_cluster_info* out_clusters,
+                            uint32_t* out_cluster_of, double* out_p, int32_t* out_map,
+                            double* out_comp_lambda, uint32_t* out_comp_hamming,
+                            d2g_dada_stats* stats) {
+  if (!ctx || !err || !align || !dada || !out_nclusters || !out_clusters || !out_cluster_of ||
+      !out_p || !out_map)
+    return D2G_ERR_ARG;
+  if (dada->max_shuffle < 1 || ncol == 0) return D2G_ERR_ARG;
+  const std::vector<uint32_t>* reads = d2g::ctx_host_reads(ctx);
+  if (!reads) return D2G_ERR_STATE;
+  const auto t_start = std::chrono::steady_clock::now();
+  Run run;
+  run.ctx = ctx;
+  run.err = err;
+  run.ncol = ncol;
+  run.align = *align;
+  run.dp = *dada;
+  Part& B = run.B;
+  const uint32_t n = (uint32_t)reads->size();
+  B.raws.resize(n);
+  for (uint32_t r = 0; r < n; ++r) {
+    B.raws[r].reads = (*reads)[r];
+    B.raws[r].prior = prior ? prior[r] != 0 : false;
+    B.reads += (*reads)[r];
+  }
+  B.e_minmax.assign(n, -999.0);
+  run.lam.resize(n);
+  run.ham.resize(n);
+  run.skp.resize(n);
+  run.lock.resize(n);
+  // init, containers.rs:363-391
+  {
+    Cluster c0;
+    c0.birth_type = kBirthInitial;
+    c0.birth_e = (double)B.reads;
+    c0.raws.reserve(n);
+    B.cl.push_back(std::move(c0));
+    for (uint32_t r = 0; r < n; ++r) {
+      B.cl[0].raws.push_back(r);
+      B.cl[0].reads += B.raws[r].reads;
+    }
+    B.cl[0].update_e = true;
+    B.assign_center(0);
+  }
+  // run_dada, dada.rs:822-1000
+  int rc = run.compare(0);
+  if (rc != D2G_OK) return rc;
+  run.p_update();
+  const size_t max_clust = dada->max_clust == 0 ? (size_t)n : (size_t)dada->max_clust;
+  bool first_shuffle = true;
+  while (B.cl.size() < max_clust) {
+    const int64_t ni = run.bud();
+    if (ni < 0) break;
+    if ((rc = run.compare((size_t)ni)) != D2G_OK) return rc;
+    run.shuffle(first_shuffle);
+    first_shuffle = false;
+    run.p_update();
+  }
+  // final p pass and map, dada.rs:502-533
+  for (size_t ci = 0; ci < B.cl.size(); ++ci) {
+    const Cluster& C = B.cl[ci];
+    for (uint32_t r : C.raws) {
+      Raw& R = B.raws[r];
+      if ((int64_t)r == C.center) {
+        R.p = 1.0;
+        R.correct = true;
+      } else {
+        R.p = calc_pA(R.reads, R.comp.lambda * (double)C.reads, true);
+        R.correct = R.p >= dada->omega_c;
+      }
+    }
+  }
+  *out_nclusters = (uint32_t)B.cl.size();
+  if (B.cl.size() > cluster_capacity) return D2G_ERR_ARG;
+  for (size_t ci = 0; ci < B.cl.size(); ++ci) {
+    const Cluster& C = B.cl[ci];
+    d2g_cluster_info& o = out_clusters[ci];
+    o.center = (uint32_t)C.center;
+    o.reads = C.reads;
+    o.nraw = (uint32_t)C.raws.size();
+    o.birth_type = C.birth_type;
+    o.birth_from = C.birth_from;
+    o.birth_hamming = C.birth_comp.hamming;
+    o.birth_pval = C.birth_pval;
+    o.birth_fold = C.birth_fold;
+    o.birth_e = C.birth_e;
+    o.birth_lambda = C.birth_comp.lambda;
+    o.self_lambda = C.self;
+    for (uint32_t r : C.raws) {
+      out_cluster_of[r] = (uint32_t)ci;
+      out_map[r] = B.raws[r].correct ? (int32_t)ci : -1;
+    }
+  }
+  for (uint32_t r = 0; r < n; ++r) {
+    out_p[r] = B.raws[r].p;
+    if (out_comp_lambda) out_comp_lambda[r] = B.raws[r].comp.lambda;
+    if (out_comp_hamming) out_comp_hamming[r] = B.raws[r].comp.hamming;
+  }
+  if (stats) {
+    run.st.nalign = B.nalign;
+    run.st.nshroud = B.nshroud;
+    run.st.ms_total = Run::ms_since(t_start);
+    *stats = run.st;
+  }
+  return D2G_OK;
+}

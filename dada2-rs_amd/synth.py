@@ -188,6 +188,65 @@ def workload_batched(n_variants=200, L=260, n_uniques=20000, lmin=240, lmax=260,
     return _finish("batched", seqs, quals, reads, err, centers, extra={"origin": origin})
 
 
+def workload_partition(n_true=8, n_near=2, n_prior=0, L=150, n_uniques=300,
+                       seed=SEED_BASE + 5) -> Workload:
+    """Partition-loop workload.  Planted sequences:
+      * n_true random variants with 300-3000 reads;
+      * n_near variants two substitutions from a true one with 60-150 reads
+        (their abundance p-value is far below omega_a: each should bud);
+      * n_prior variants two substitutions from a true one with 3 reads
+        (no abundance bud; flagged prior, they should bud by omega_p);
+    and error copies of the true variants (1-3 random substitutions, 1-4
+    reads) up to n_uniques.  extra: "origin" (planted index of each unique's
+    source in sorted order, itself for planted ones), "planted" (sorted
+    indices of the planted sequences), "prior" (flags, sorted order)."""
+    rng = np.random.default_rng(seed)
+    err = fixed_err(41)
+    seen: set = set()
+    truth = _unique_set(rng, lambda: rng.integers(1, 5, size=L).astype(np.uint8), n_true, seen)
+
+    def mutant(src, k):
+        def make():
+            s = src.copy()
+            pos = rng.choice(L, size=k, replace=False)
+            s[pos] = ((s[pos] - 1 + rng.integers(1, 4, size=k)) % 4) + 1
+            return s
+        return make
+
+    seqs, reads, origin, prior = [], [], [], []
+    for t in truth:
+        seqs.append(t)
+        reads.append(int(rng.integers(300, 3001)))
+        origin.append(len(seqs) - 1)
+        prior.append(0)
+    for k in range(n_near + n_prior):
+        src = int(rng.integers(0, n_true))
+        (s,) = _unique_set(rng, mutant(truth[src], 2), 1, seen)
+        seqs.append(s)
+        reads.append(int(rng.integers(60, 151)) if k < n_near else 3)
+        origin.append(len(seqs) - 1)
+        prior.append(0 if k < n_near else 1)
+    while len(seqs) < n_uniques:
+        src = int(rng.integers(0, n_true))
+        (s,) = _unique_set(rng, mutant(truth[src], int(rng.integers(1, 4))), 1, seen)
+        seqs.append(s)
+        reads.append(int(rng.integers(1, 5)))
+        origin.append(src)
+        prior.append(0)
+    quals = [quality_profile(rng, L) for _ in seqs]
+    reads = np.asarray(reads, np.uint32)
+    order = _derep_order(seqs, reads)
+    inv = np.empty_like(order)
+    inv[order] = np.arange(len(order))
+    n_planted = n_true + n_near + n_prior
+    origin_sorted = np.asarray([inv[origin[i]] for i in order], np.int64)
+    extra = {"origin": origin_sorted,
+             "planted": np.sort(inv[:n_planted]),
+             "prior": np.asarray(prior, np.uint8)[order]}
+    return _finish("partition", seqs, quals, reads, err,
+                   lambda inv_, r: [plain_center(r)], extra=extra)
+
+
 def subsample(w: Workload, n_uniques: int, n_centers: int | None = None) -> Workload:
     """The first n_uniques uniques (abundance order) and the centers among them."""
     keep = w.centers[w.centers < n_uniques]

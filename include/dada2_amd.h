@@ -153,6 +153,71 @@ int d2g_compare_batch_device(d2g_ctx* ctx, const uint32_t* centers, uint32_t n_c
 int d2g_fetch_results(d2g_ctx* ctx, double* out_lambda, uint32_t* out_hamming,
                       uint8_t* out_skipped);
 
+/* ---- the partition loop --------------------------------------------------
+ * Replaces run_dada (dada.rs:749-1150) plus the final per-raw p-value pass
+ * and read map (dada.rs:502-533), over the loaded uniques (raw index = load
+ * order).  Each cluster's compare runs on the device (d2g_compare_center).
+ * B, the serial store, b_p_update, b_bud and the shuffle run on the host.
+ * DadaParams fields (dada.rs:54-90; defaults main.rs:4393-4433): */
+typedef struct {
+  double omega_a;        /* 1e-40: bud when min p * nraw < omega_a */
+  double omega_p;        /* 1e-4: prior bud when min prior p < omega_p */
+  double omega_c;        /* 1e-40: a raw is corrected when its final p >= omega_c */
+  double min_fold;       /* 1.0 */
+  uint32_t min_hamming;  /* 1 */
+  uint32_t min_abund;    /* 1 */
+  uint32_t max_clust;    /* 0: up to the number of uniques */
+  uint32_t max_shuffle;  /* MAX_SHUFFLE = 10 (dada.rs:39); >= 1 */
+  int32_t greedy;        /* 1 */
+  int32_t detect_singletons; /* 0 */
+  int32_t use_quals;     /* 1 */
+  int32_t _reserved;
+} d2g_dada_params;
+
+/* One cluster of the final partition (Bi, containers.rs:200-238). */
+typedef struct {
+  uint32_t center;       /* unique index of the center */
+  uint32_t reads;        /* reads of the members */
+  uint32_t nraw;         /* members */
+  int32_t birth_type;    /* BirthType (containers.rs:59): 0 initial, 1 abundance, 2 prior */
+  uint32_t birth_from;   /* cluster it budded from */
+  uint32_t birth_hamming;
+  double birth_pval;
+  double birth_fold;
+  double birth_e;
+  double birth_lambda;
+  double self_lambda;    /* Bi.self_: the center's lambda against itself */
+} d2g_cluster_info;
+
+/* Counters and host-clock phase times of one run (the phase timers of
+ * run_dada, dada.rs:752-760). */
+typedef struct {
+  uint64_t nalign;       /* B.nalign */
+  uint64_t nshroud;      /* B.nshroud */
+  uint32_t compares;     /* device compares (one per cluster) */
+  uint32_t shuffle_passes;
+  double ms_compare;     /* device compare calls, results on the host */
+  double ms_store;       /* serial store */
+  double ms_shuffle;
+  double ms_bud;
+  double ms_pupdate;
+  double ms_total;
+} d2g_dada_stats;
+
+/* prior: nullable per-unique flags (Raw.prior).  cluster_capacity bounds
+ * out_clusters; when the partition has more clusters the call returns
+ * D2G_ERR_ARG with *out_nclusters set.  Per unique (n = loaded uniques):
+ *   out_cluster_of[i]  final cluster
+ *   out_p[i]           final p (1.0 for centers, calc_pA otherwise)
+ *   out_map[i]         cluster when corrected (p >= omega_c), else -1
+ *   out_comp_lambda/hamming[i]  Raw.comp at the end (nullable)
+ * stats is nullable. */
+int d2g_run_dada(d2g_ctx* ctx, const double* err, uint32_t ncol, const d2g_align_params* align,
+                 const d2g_dada_params* dada, const uint8_t* prior, uint32_t cluster_capacity,
+                 uint32_t* out_nclusters, d2g_cluster_info* out_clusters,
+                 uint32_t* out_cluster_of, double* out_p, int32_t* out_map,
+                 double* out_comp_lambda, uint32_t* out_comp_hamming, d2g_dada_stats* stats);
+
 /* ---- pairwise alignment (parity / aux mode) -------------------------------
  * Backs align_vectorized_with_buf (nwalign.rs:761-1140) for n arbitrary byte
  * pairs (any alphabet: 1..4 or ASCII).  s1/s2 are concatenated with offsets
