@@ -42,10 +42,11 @@ def parse():
                     help="minimum CPU time of the baseline sample (whole passes)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--workload", default="batched",
-                    choices=["batched", "plain", "precision", "gradient", "longread"],
+                    choices=["batched", "plain", "precision", "gradient", "longread", "dada"],
                     help="batched: the headline line (default); the others measure the "
                          "plain single-center compare, the mixed-precision lambda, the "
-                         "log-likelihood gradient and the long-read compare + bimera search")
+                         "log-likelihood gradient, the long-read compare + bimera search and "
+                         "the partition loop (run_dada)")
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
                     help="after the run, one device-to-device copy of this size (a plain copy "
                          "kernel of known bytes that scales the HBM byte counters)")
@@ -272,6 +273,74 @@ def bench_gradient(args, pkg):
                            "kernel": "loglik_fwd_kernel + loglik_bwd_kernel",
                            "algorithmic_bytes_per_launch": hbm},
                  cpu_baseline=None)
+
+
+def bench_dada(args, pkg):
+    """The partition loop: run_dada over 50,000 uniques (250 nt; 300 planted
+    variants, 60 near variants that bud, error copies), every cluster's
+    compare on the device.  A step is one whole run_dada."""
+    from dada2_rs_amd import synth
+    w = synth.workload_partition(n_true=300, n_near=60, n_prior=0, L=250, n_uniques=50000,
+                                 seed=synth.SEED_BASE + 20)
+    ap, dp = pkg.align_params(band=16), pkg.dada_params()
+    ctx = pkg.Context(0)
+    info = ctx.device_info()
+    ctx.load_uniques(w.seqs, w.quals, w.reads, k=w.kmer_size)
+    for _ in range(args.warmup):
+        ctx.run_dada(w.err, ap, dp)
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        r = ctx.run_dada(w.err, ap, dp)
+    wall = time.perf_counter() - t0
+    ctx.set_timing(True)
+    rt = ctx.run_dada(w.err, ap, dp)
+    ctx.close()
+    st, stt = r.stats, rt.stats
+    ms = wall / args.steps * 1e3
+    # the dominant kernel by device time is the DP (VALU); the screen's HBM
+    # rate is reported beside it
+    peak_ops = info["cu_count"] * 64 * info["clock_khz"] * 1e3 / 1e12  # int32 TOP/s
+    dp_tops = stt["dp_cells"] * 8 / (stt["ms_dev_dp"] * 1e-3) / 1e12
+    screen_bytes = stt["nalign"] * 1024.0  # one 4^5-byte k-mer profile per comparison
+    screen_gbs = screen_bytes / (stt["ms_dev_screen"] * 1e-3) / 1e9
+    cpu = None
+    if not args.no_cpu:
+        import oracle as O
+        from oracle import partition as OP
+        ws = synth.workload_partition(n_true=30, n_near=6, n_prior=0, L=250, n_uniques=5000,
+                                      seed=synth.SEED_BASE + 21)
+        th = O.cpu_count_allowed()
+        box = {}
+
+        def one():
+            rs = O.RawSet(ws.seqs, ws.quals, ws.reads, k=ws.kmer_size)
+            box["out"] = OP.run_dada(rs, ws.err, O.make_params(band=16), nthreads=th)
+        calls, dt = _timed_cpu(one, args.cpu_seconds)
+        cpu = {"value": calls * box["out"].nalign / dt, "unit": "comparisons/s", "cores": th,
+               "kind": "port",
+               "sample": f"{calls} whole runs of 5,000 uniques (30 planted, "
+                         f"{len(box['out'].clusters)} clusters), {dt:.1f} s: the oracle's "
+                         f"Python partition loop around its C compare on {th} threads"}
+    return _line("raw-cluster comparisons/sec through run_dada (partition loop)",
+                 st["nalign"] / (ms * 1e-3), "comparisons/s", args, ms, "i16 DP / f64 lambda",
+                 {"workload": "dada: run_dada over 50000 uniques, 250 nt, 300 planted + 60 near "
+                              "variants, band 16, (5,-4,-8), k=5 cutoff 0.42, greedy, "
+                              "omega_a 1e-40"},
+                 clusters=len(r.clusters),
+                 counts={"nalign": st["nalign"], "nshroud": st["nshroud"],
+                         "compares": st["compares"], "shuffle_passes": st["shuffle_passes"],
+                         "dp_cells": st["dp_cells"]},
+                 phase_ms={k: st[k] for k in ("ms_compare", "ms_store", "ms_shuffle", "ms_bud",
+                                              "ms_pupdate", "ms_total")},
+                 kernel_ms={k: stt[k] for k in ("ms_dev_screen", "ms_dev_plan", "ms_dev_dp",
+                                                "ms_dev_lambda")},
+                 roofline={"bound": "valu", "achieved": dp_tops, "peak": peak_ops,
+                           "unit": "TOP/s", "frac": dp_tops / peak_ops, "traffic": None,
+                           "kernel": "nw_vec_kernel (summed over the run's compares)",
+                           "cells_per_run": stt["dp_cells"]},
+                 screen_hbm={"achieved_gbs": screen_gbs, "frac": screen_gbs / HBM_PEAK_GBS,
+                             "algorithmic_bytes_per_run": screen_bytes},
+                 cpu_baseline=cpu)
 
 
 def bench_longread(args, pkg):

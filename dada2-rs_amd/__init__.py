@@ -230,7 +230,9 @@ class DadaStats(C.Structure):
     _fields_ = [("nalign", C.c_uint64), ("nshroud", C.c_uint64), ("compares", C.c_uint32),
                 ("shuffle_passes", C.c_uint32), ("ms_compare", C.c_double),
                 ("ms_store", C.c_double), ("ms_shuffle", C.c_double), ("ms_bud", C.c_double),
-                ("ms_pupdate", C.c_double), ("ms_total", C.c_double)]
+                ("ms_pupdate", C.c_double), ("ms_total", C.c_double), ("dp_cells", C.c_uint64),
+                ("ms_dev_screen", C.c_double), ("ms_dev_plan", C.c_double),
+                ("ms_dev_dp", C.c_double), ("ms_dev_lambda", C.c_double)]
 
     def as_dict(self):
         return {name: getattr(self, name) for name, _ in self._fields_}

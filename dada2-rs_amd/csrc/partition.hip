@@ -35,11 +35,18 @@
 //     series / Lentz continued fraction of Numerical Recipes (3rd ed.,
 //     section 6.2), with the reference's own small-lambda series when it
 //     returns 0 (pval.rs:329-353).
+#include <sched.h>
+
 #include <algorithm>
+#include <atomic>
 #include <cfloat>
 #include <chrono>
 #include <cmath>
+#include <condition_variable>
 #include <cstdint>
+#include <functional>
+#include <mutex>
+#include <thread>
 #include <vector>
 
 #include "../../include/dada2_amd.h"
@@ -50,11 +57,104 @@ const std::vector<uint32_t>* ctx_host_reads(const d2g_ctx* c);
 
 namespace {
 
+// A persistent pool for the loop's per-raw work (p-values, the shuffle's
+// best-cluster scan): run() hands out chunks of [0, n) to the workers and
+// the caller and returns when every chunk is done.  Results are written by
+// index, so they do not depend on the thread count.
+class Pool {
+ public:
+  explicit Pool(int nthreads) {
+    for (int t = 1; t < nthreads; ++t) th_.emplace_back([this] { work(); });
+  }
+  ~Pool() {
+    {
+      std::lock_guard<std::mutex> g(m_);
+      stop_ = true;
+    }
+    cv_.notify_all();
+    for (auto& t : th_) t.join();
+  }
+  int size() const { return (int)th_.size() + 1; }
+  void run(size_t n, size_t chunk, const std::function<void(size_t, size_t)>& fn) {
+    if (th_.empty() || n <= chunk) {
+      if (n) fn(0, n);
+      return;
+    }
+    {
+      std::lock_guard<std::mutex> g(m_);
+      job_ = &fn;
+      n_ = n;
+      chunk_ = chunk;
+      next_.store(0);
+      busy_ = (int)th_.size();
+      ++gen_;
+    }
+    cv_.notify_all();
+    drain(fn);
+    std::unique_lock<std::mutex> g(m_);
+    done_.wait(g, [this] { return busy_ == 0; });
+    job_ = nullptr;
+  }
+
+ private:
+  void drain(const std::function<void(size_t, size_t)>& fn) {
+    for (;;) {
+      const size_t b = next_.fetch_add(chunk_);
+      if (b >= n_) break;
+      fn(b, std::min(n_, b + chunk_));
+    }
+  }
+  void work() {
+    uint64_t seen = 0;
+    for (;;) {
+      const std::function<void(size_t, size_t)>* job;
+      {
+        std::unique_lock<std::mutex> g(m_);
+        cv_.wait(g, [&] { return stop_ || gen_ != seen; });
+        if (stop_) return;
+        seen = gen_;
+        job = job_;
+      }
+      drain(*job);
+      {
+        std::lock_guard<std::mutex> g(m_);
+        if (--busy_ == 0) done_.notify_one();
+      }
+    }
+  }
+  std::vector<std::thread> th_;
+  std::mutex m_;
+  std::condition_variable cv_, done_;
+  const std::function<void(size_t, size_t)>* job_ = nullptr;
+  size_t n_ = 0, chunk_ = 1;
+  std::atomic<size_t> next_{0};
+  int busy_ = 0;
+  uint64_t gen_ = 0;
+  bool stop_ = false;
+};
+
+// Host threads for the loop: the CPUs this process may run on, at most 16.
+int pool_threads() {
+  cpu_set_t set;
+  int n = 1;
+  if (sched_getaffinity(0, sizeof(set), &set) == 0) n = CPU_COUNT(&set);
+  return std::max(1, std::min(n, 16));
+}
+
 // Comparison, containers.rs:15-20.
 struct Comp {
   uint32_t i = 0, index = 0;
   double lambda = 0.0;
   uint32_t hamming = 0;
+};
+
+// One stored comparison of a raw, in the raw's candidate list (the
+// CandIndex of dada.rs:895-897): clusters append in creation order, so a
+// list is in ascending cluster order.
+struct Cand {
+  uint32_t i;
+  uint32_t hamming;
+  double lambda;
 };
 
 // The Raw fields the loop reads and writes (containers.rs:78-108).
@@ -88,9 +188,14 @@ struct Part {
   std::vector<double> e_minmax;
   uint32_t reads = 0;
   uint64_t nalign = 0, nshroud = 0;
+  // each raw's cluster and position in it (the raw_cluster map of
+  // containers.rs:302-316, plus the position)
+  std::vector<uint32_t> where, at;
 
   // bi_add_raw, containers.rs:403-417
   void add_raw(size_t ci, uint32_t r) {
+    where[r] = (uint32_t)ci;
+    at[r] = (uint32_t)cl[ci].raws.size();
     cl[ci].raws.push_back(r);
     cl[ci].reads += raws[r].reads;
     cl[ci].update_e = true;
@@ -100,6 +205,7 @@ struct Part {
     std::vector<uint32_t>& v = cl[ci].raws;
     const uint32_t r = v[pos];
     v[pos] = v.back();
+    at[v[pos]] = (uint32_t)pos;
     v.pop_back();
     cl[ci].reads -= raws[r].reads;
     cl[ci].update_e = true;
@@ -197,6 +303,8 @@ struct Run {
   std::vector<double> lam;
   std::vector<uint32_t> ham;
   std::vector<uint8_t> skp, lock;
+  std::vector<std::vector<Cand>> cand;  // per raw
+  Pool* pool = nullptr;
   d2g_dada_stats st{};
 
   static double ms_since(std::chrono::steady_clock::time_point t0) {
@@ -212,11 +320,17 @@ struct Run {
     d2g_align_params p = align;
     if (i == 0) p.kdist_cutoff = 1.0;  // dada.rs:835-838
     auto t0 = std::chrono::steady_clock::now();
+    d2g_stats cs{};
     const int rc = d2g_compare_center(ctx, center, err, ncol, &p, dp.use_quals, dp.greedy,
-                                      lock.data(), lam.data(), ham.data(), skp.data(), nullptr);
+                                      lock.data(), lam.data(), ham.data(), skp.data(), &cs);
     st.ms_compare += ms_since(t0);
     ++st.compares;
     if (rc != D2G_OK) return rc;
+    st.dp_cells += cs.dp_cells;
+    st.ms_dev_screen += cs.ms_screen;
+    st.ms_dev_plan += cs.ms_plan;
+    st.ms_dev_dp += cs.ms_dp;
+    st.ms_dev_lambda += cs.ms_lambda;
     t0 = std::chrono::steady_clock::now();
     const double total = (double)B.reads;
     Cluster& C = B.cl[i];
@@ -236,6 +350,7 @@ struct Run {
         c.lambda = lambda;
         c.hamming = ham[r] == D2G_HAMMING_NONE ? 0u : ham[r];
         C.comp.push_back(c);
+        cand[r].push_back(Cand{c.i, c.hamming, c.lambda});
         if (i == 0 || r == center) B.raws[r].comp = c;
       }
     }
@@ -248,11 +363,14 @@ struct Run {
     const auto t0 = std::chrono::steady_clock::now();
     for (Cluster& C : B.cl) {
       if (C.update_e) {
-        for (uint32_t r : C.raws) {
-          Raw& R = B.raws[r];
-          R.p = get_pA(R.reads, R.prior, R.comp.lambda, R.comp.hamming, C.reads,
-                       dp.detect_singletons != 0);
-        }
+        const uint32_t bi_reads = C.reads;
+        const bool ds = dp.detect_singletons != 0;
+        pool->run(C.raws.size(), 1024, [&](size_t b, size_t e) {
+          for (size_t k = b; k < e; ++k) {
+            Raw& R = B.raws[C.raws[k]];
+            R.p = get_pA(R.reads, R.prior, R.comp.lambda, R.comp.hamming, bi_reads, ds);
+          }
+        });
         C.update_e = false;
       }
       if (dp.greedy && C.check_locks) {
@@ -331,53 +449,70 @@ struct Run {
     return out;
   }
 
-  // b_shuffle2, cluster.rs:615-700: one move pass; returns the moves
-  size_t shuffle_pass(std::vector<double>& emax, std::vector<Comp>& compmax) {
+  // b_shuffle2, cluster.rs:615-700: one move pass; returns the moves.
+  //   * The best-cluster scan runs per raw over its candidate list
+  //     (ascending cluster order, strict >: the lowest cluster keeps ties),
+  //     which is b_shuffle2's cluster-major scan regrouped by raw.
+  //   * The move pass walks every cluster from its end and swap-removes the
+  //     raws whose best cluster is another (centers stay).  Raws appended
+  //     during the pass never move again, and a swap-remove only disturbs
+  //     positions already walked, so visiting just the movers, by cluster
+  //     and then by descending position, makes the same moves in the same
+  //     order.
+  size_t shuffle_pass(std::vector<Comp>& compmax, std::vector<double>& creads,
+                      std::vector<std::vector<uint64_t>>& found) {
+    constexpr size_t kChunk = 2048;
     const size_t n = B.raws.size();
-    emax.assign(n, -INFINITY);
-    compmax.assign(n, Comp());
-    const double c0 = (double)B.cl[0].reads;
-    for (const Comp& c : B.cl[0].comp) {
-      emax[c.index] = c.lambda * c0;
-      compmax[c.index] = c;
-    }
-    for (size_t ci = 1; ci < B.cl.size(); ++ci) {
-      const double cr = (double)B.cl[ci].reads;
-      for (const Comp& c : B.cl[ci].comp) {
-        const double e = c.lambda * cr;
-        if (e > emax[c.index]) {
-          emax[c.index] = e;
-          compmax[c.index] = c;
+    compmax.resize(n);
+    creads.resize(B.cl.size());
+    for (size_t ci = 0; ci < B.cl.size(); ++ci) creads[ci] = (double)B.cl[ci].reads;
+    found.resize((n + kChunk - 1) / kChunk);
+    pool->run(n, kChunk, [&](size_t b, size_t e) {
+      std::vector<uint64_t>& mv = found[b / kChunk];
+      mv.clear();
+      for (size_t r = b; r < e; ++r) {
+        double emax = -INFINITY;
+        Comp best;
+        for (const Cand& c : cand[r]) {
+          const double x = c.lambda * creads[c.i];
+          if (x > emax) {
+            emax = x;
+            best.i = c.i;
+            best.index = (uint32_t)r;
+            best.lambda = c.lambda;
+            best.hamming = c.hamming;
+          }
         }
+        compmax[r] = best;
+        const uint32_t ci = B.where[r];
+        if (best.i != ci && B.cl[ci].center != (int64_t)r)
+          mv.push_back((uint64_t)ci << 32 | (uint64_t)(0xFFFFFFFFu - B.at[r]));
       }
+    });
+    std::vector<uint64_t> movers;
+    for (const auto& f : found) movers.insert(movers.end(), f.begin(), f.end());
+    std::sort(movers.begin(), movers.end());  // cluster ascending, position descending
+    for (uint64_t key : movers) {
+      const size_t ci = (size_t)(key >> 32);
+      const size_t pos = 0xFFFFFFFFu - (uint32_t)key;
+      const uint32_t r = B.cl[ci].raws[pos];
+      B.pop_raw(ci, pos);
+      B.add_raw(compmax[r].i, r);
+      B.raws[r].comp = compmax[r];
     }
-    size_t moves = 0;
-    for (size_t ci = 0; ci < B.cl.size(); ++ci) {
-      size_t pos = B.cl[ci].raws.size();
-      while (pos > 0) {
-        --pos;
-        const uint32_t r = B.cl[ci].raws[pos];
-        const size_t best = compmax[r].i;
-        if (best == ci) continue;
-        if (B.cl[ci].center == (int64_t)r) continue;
-        B.pop_raw(ci, pos);
-        B.add_raw(best, r);
-        B.raws[r].comp = compmax[r];
-        ++moves;
-      }
-    }
-    return moves;
+    return movers.size();
   }
 
   // b_shuffle_converge, cluster.rs:908-1320 (see the counting note above)
   void shuffle(bool first) {
     const auto t0 = std::chrono::steady_clock::now();
     const uint32_t passes = first ? dp.max_shuffle : std::max(dp.max_shuffle, 2u) - 1u;
-    std::vector<double> emax;
     std::vector<Comp> compmax;
+    std::vector<double> creads;
+    std::vector<std::vector<uint64_t>> found;
     for (uint32_t k = 0; k < passes; ++k) {
       ++st.shuffle_passes;
-      if (shuffle_pass(emax, compmax) == 0) break;
+      if (shuffle_pass(compmax, creads, found) == 0) break;
     }
     st.ms_shuffle += ms_since(t0);
   }
@@ -414,10 +549,16 @@ extern "C" int d2g_run_dada(d2g_ctx* ctx, const double* err, uint32_t ncol,
     B.reads += (*reads)[r];
   }
   B.e_minmax.assign(n, -999.0);
+  B.where.assign(n, 0);
+  B.at.resize(n);
+  for (uint32_t r = 0; r < n; ++r) B.at[r] = r;
   run.lam.resize(n);
   run.ham.resize(n);
   run.skp.resize(n);
   run.lock.resize(n);
+  run.cand.resize(n);
+  Pool pool(pool_threads());
+  run.pool = &pool;
   // init, containers.rs:363-391
   {
     Cluster c0;

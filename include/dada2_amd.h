@@ -202,6 +202,11 @@ typedef struct {
   double ms_bud;
   double ms_pupdate;
   double ms_total;
+  uint64_t dp_cells;     /* DP cells of the device compares */
+  double ms_dev_screen;  /* summed device kernel times of the compares, */
+  double ms_dev_plan;    /* filled when the context's timing is on */
+  double ms_dev_dp;
+  double ms_dev_lambda;
 } d2g_dada_stats;
 
 /* prior: nullable per-unique flags (Raw.prior).  cluster_capacity bounds

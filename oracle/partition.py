@@ -112,8 +112,8 @@ class Partition:
             self.cl[ci].center = best
 
 
-def _compare(B: Partition, i: int, err, align, greedy, use_quals):
-    """b_compare, cluster.rs:106-164."""
+def _compare(B: Partition, i: int, err, align, greedy, use_quals, nthreads=1):
+    """b_compare, cluster.rs:106-164 (the map over nthreads, cluster.rs:279-330)."""
     rs = B.rs
     for r in range(len(rs)):
         rs.arr[r].lock = int(B.lock[r])
@@ -122,7 +122,7 @@ def _compare(B: Partition, i: int, err, align, greedy, use_quals):
         params.kdist_cutoff = 1.0  # dada.rs:835-838
     center = B.cl[i].center
     lam, ham, skp, _, _ = compare_center(rs, center, err, err.shape[1], params,
-                                         use_quals=use_quals, greedy=greedy)
+                                         use_quals=use_quals, greedy=greedy, nthreads=nthreads)
     center_reads = B.reads_of[center]
     total = float(B.total)
     for r in range(len(rs)):
@@ -255,13 +255,13 @@ class DadaOut:
 
 def run_dada(rs: RawSet, err: np.ndarray, align, prior=None, omega_a=1e-40, omega_p=1e-4,
              omega_c=1e-40, min_fold=1.0, min_hamming=1, min_abund=1, max_clust=0,
-             max_shuffle=10, greedy=True, detect_singletons=False,
-             use_quals=True) -> DadaOut:
+             max_shuffle=10, greedy=True, detect_singletons=False, use_quals=True,
+             nthreads=1) -> DadaOut:
     """run_dada (dada.rs:749-1150) then the final p pass and map (dada.rs:502-533)."""
     err = np.ascontiguousarray(err, np.float64)
     reads = [rs.arr[i].reads for i in range(len(rs))]
     B = Partition(rs, reads, prior)
-    _compare(B, 0, err, align, greedy, use_quals)
+    _compare(B, 0, err, align, greedy, use_quals, nthreads)
     _p_update(B, greedy, detect_singletons)
     limit = len(rs) if max_clust == 0 else max_clust
     passes = 0
@@ -270,7 +270,7 @@ def run_dada(rs: RawSet, err: np.ndarray, align, prior=None, omega_a=1e-40, omeg
         ni = _bud(B, omega_a, omega_p, min_fold, min_hamming, min_abund)
         if ni is None:
             break
-        _compare(B, ni, err, align, greedy, use_quals)
+        _compare(B, ni, err, align, greedy, use_quals, nthreads)
         cap = max_shuffle if first else max(max_shuffle - 1, 1)
         first = False
         for _ in range(cap):
