@@ -305,6 +305,10 @@ struct Run {
   std::vector<uint8_t> skp, lock;
   std::vector<std::vector<Cand>> cand;  // per raw
   Pool* pool = nullptr;
+  // per-chunk scratch of the store and the bud scan
+  std::vector<std::vector<Comp>> store_comps;
+  std::vector<uint64_t> store_counts;
+  std::vector<size_t> bud_off;
   d2g_dada_stats st{};
 
   static double ms_since(std::chrono::steady_clock::time_point t0) {
@@ -332,27 +336,45 @@ struct Run {
     st.ms_dev_dp += cs.ms_dp;
     st.ms_dev_lambda += cs.ms_lambda;
     t0 = std::chrono::steady_clock::now();
+    // every raw's step is its own; the cluster's comp vector is the chunks'
+    // lists concatenated in raw order
+    constexpr size_t kChunk = 4096;
     const double total = (double)B.reads;
     Cluster& C = B.cl[i];
-    for (uint32_t r = 0; r < n; ++r) {
-      if (!skp[r]) {
-        ++B.nalign;
-        if (ham[r] == D2G_HAMMING_NONE) ++B.nshroud;
+    const size_t nch = (n + kChunk - 1) / kChunk;
+    store_comps.resize(nch);
+    store_counts.assign(2 * nch, 0);
+    pool->run(n, kChunk, [&](size_t b, size_t e) {
+      std::vector<Comp>& out = store_comps[b / kChunk];
+      out.clear();
+      uint64_t na = 0, ns = 0;
+      for (size_t r = b; r < e; ++r) {
+        if (!skp[r]) {
+          ++na;
+          if (ham[r] == D2G_HAMMING_NONE) ++ns;
+        }
+        const double lambda = lam[r];
+        if (lambda * total > B.e_minmax[r]) {
+          const double ex = lambda * (double)center_reads;
+          if (ex > B.e_minmax[r]) B.e_minmax[r] = ex;
+          Comp c;
+          c.i = (uint32_t)i;
+          c.index = (uint32_t)r;
+          c.lambda = lambda;
+          c.hamming = ham[r] == D2G_HAMMING_NONE ? 0u : ham[r];
+          out.push_back(c);
+          cand[r].push_back(Cand{c.i, c.hamming, c.lambda});
+          if (i == 0 || r == center) B.raws[r].comp = c;
+        }
       }
-      const double lambda = lam[r];
-      if (r == center) C.self = lambda;
-      if (lambda * total > B.e_minmax[r]) {
-        const double e = lambda * (double)center_reads;
-        if (e > B.e_minmax[r]) B.e_minmax[r] = e;
-        Comp c;
-        c.i = (uint32_t)i;
-        c.index = r;
-        c.lambda = lambda;
-        c.hamming = ham[r] == D2G_HAMMING_NONE ? 0u : ham[r];
-        C.comp.push_back(c);
-        cand[r].push_back(Cand{c.i, c.hamming, c.lambda});
-        if (i == 0 || r == center) B.raws[r].comp = c;
-      }
+      store_counts[2 * (b / kChunk)] = na;
+      store_counts[2 * (b / kChunk) + 1] = ns;
+    });
+    C.self = lam[center];
+    for (size_t k = 0; k < nch; ++k) {
+      B.nalign += store_counts[2 * k];
+      B.nshroud += store_counts[2 * k + 1];
+      C.comp.insert(C.comp.end(), store_comps[k].begin(), store_comps[k].end());
     }
     st.ms_store += ms_since(t0);
     return D2G_OK;
@@ -406,45 +428,66 @@ struct Run {
   }
 
   // b_bud, cluster.rs:1331-1480; returns the new cluster or -1
+  // A bud candidate; `better` is the scan's replacement test (smaller p,
+  // then more reads).  Keeping the first candidate of the smallest key is an
+  // associative fold, so chunks of the scan combine in scan order.
+  struct Best {
+    double p;
+    uint32_t reads;
+    size_t ci, pos;
+    bool have;
+  };
+  static bool better(double p, uint32_t reads, const Best& b) {
+    return p < b.p || (p == b.p && reads > b.reads);
+  }
+  std::vector<Best> bud_best;
+
   int64_t bud() {
+    constexpr size_t kChunk = 4096;
     const auto t0 = std::chrono::steady_clock::now();
     const double nraw = (double)B.raws.size();
     const uint32_t c0 = (uint32_t)B.cl[0].center;
-    bool have = false, have_prior = false;
-    size_t mci = 0, mpos = 0, pci = 0, ppos = 0;
-    double min_p = B.raws[c0].p, min_p_prior = min_p;
-    uint32_t min_reads = B.raws[c0].reads, min_reads_prior = min_reads;
-    for (size_t ci = 0; ci < B.cl.size(); ++ci) {
-      const Cluster& C = B.cl[ci];
-      for (size_t pos = 1; pos < C.raws.size(); ++pos) {
+    // the scan visits (cluster, position >= 1) in order: flatten it
+    bud_off.resize(B.cl.size() + 1);
+    bud_off[0] = 0;
+    for (size_t ci = 0; ci < B.cl.size(); ++ci) bud_off[ci + 1] = bud_off[ci] + B.cl[ci].raws.size();
+    const size_t m = bud_off.back();
+    const size_t nch = (m + kChunk - 1) / kChunk;
+    bud_best.assign(2 * nch, Best{0.0, 0u, 0, 0, false});
+    pool->run(m, kChunk, [&](size_t b, size_t e) {
+      Best ba{0.0, 0u, 0, 0, false}, bp{0.0, 0u, 0, 0, false};
+      size_t ci = (size_t)(std::upper_bound(bud_off.begin(), bud_off.end(), b) - bud_off.begin()) - 1;
+      for (size_t x = b; x < e; ++x) {
+        while (x >= bud_off[ci + 1]) ++ci;
+        const size_t pos = x - bud_off[ci];
+        if (pos == 0) continue;  // the first member is never a candidate
+        const Cluster& C = B.cl[ci];
         const Raw& R = B.raws[C.raws[pos]];
         if (R.reads < dp.min_abund) continue;
         if (R.comp.hamming < dp.min_hamming) continue;
         const bool fold_ok =
             dp.min_fold <= 1.0 || (double)R.reads >= dp.min_fold * R.comp.lambda * (double)C.reads;
         if (!fold_ok) continue;
-        if (R.p < min_p || (R.p == min_p && R.reads > min_reads)) {
-          have = true;
-          mci = ci;
-          mpos = pos;
-          min_p = R.p;
-          min_reads = R.reads;
-        }
-        if (R.prior && (R.p < min_p_prior || (R.p == min_p_prior && R.reads > min_reads_prior))) {
-          have_prior = true;
-          pci = ci;
-          ppos = pos;
-          min_p_prior = R.p;
-          min_reads_prior = R.reads;
-        }
+        if (!ba.have || better(R.p, R.reads, ba)) ba = Best{R.p, R.reads, ci, pos, true};
+        if (R.prior && (!bp.have || better(R.p, R.reads, bp))) bp = Best{R.p, R.reads, ci, pos, true};
       }
+      bud_best[2 * (b / kChunk)] = ba;
+      bud_best[2 * (b / kChunk) + 1] = bp;
+    });
+    // seeded with cluster 0's center, which is never chosen itself
+    Best ma{B.raws[c0].p, B.raws[c0].reads, 0, 0, false}, mp = ma;
+    for (size_t k = 0; k < nch; ++k) {
+      const Best& a = bud_best[2 * k];
+      const Best& q = bud_best[2 * k + 1];
+      if (a.have && better(a.p, a.reads, ma)) ma = a;
+      if (q.have && better(q.p, q.reads, mp)) mp = q;
     }
-    const double p_a = min_p * nraw;
+    const double p_a = ma.p * nraw;
     int64_t out = -1;
-    if (p_a < dp.omega_a && have)
-      out = (int64_t)bud_into(mci, mpos, B.cl[mci].raws[mpos], kBirthAbundance, p_a);
-    else if (min_p_prior < dp.omega_p && have_prior)
-      out = (int64_t)bud_into(pci, ppos, B.cl[pci].raws[ppos], kBirthPrior, min_p_prior);
+    if (p_a < dp.omega_a && ma.have)
+      out = (int64_t)bud_into(ma.ci, ma.pos, B.cl[ma.ci].raws[ma.pos], kBirthAbundance, p_a);
+    else if (mp.p < dp.omega_p && mp.have)
+      out = (int64_t)bud_into(mp.ci, mp.pos, B.cl[mp.ci].raws[mp.pos], kBirthPrior, mp.p);
     st.ms_bud += ms_since(t0);
     return out;
   }
