@@ -63,6 +63,65 @@ def test_run_dada_matches_oracle(pkg, oracle, name, wargs, opts):
     assert got.stats["compares"] == len(want.clusters)
 
 
+def _run_pair(pkg, oracle, seqs, quals, reads, align_kw, prior=None, **kw):
+    from oracle import partition as P
+    reads = np.asarray(reads, np.uint32)
+    rs = oracle.RawSet(seqs, quals, reads, k=5)
+    want = P.run_dada(rs, synth.fixed_err(41), oracle.make_params(**align_kw), prior=prior, **kw)
+    ctx = pkg.Context(0)
+    try:
+        ctx.load_uniques(seqs, quals, reads, k=5)
+        got = ctx.run_dada(synth.fixed_err(41), pkg.align_params(**align_kw),
+                           pkg.dada_params(**kw), prior=prior)
+    finally:
+        ctx.close()
+    return got, want
+
+
+def _same(got, want):
+    assert [c["center"] for c in got.clusters] == [c["center"] for c in want.clusters]
+    for g, o in zip(got.clusters, want.clusters):
+        assert g == o or all(g[k] == o[k] or (g[k] != g[k] and o[k] != o[k]) for k in o)
+    assert np.array_equal(got.cluster_of, want.cluster_of)
+    assert np.array_equal(got.p, want.p)
+    assert np.array_equal(got.map, want.map)
+    assert got.stats["nalign"] == want.nalign and got.stats["nshroud"] == want.nshroud
+
+
+def test_run_dada_edge_cases(pkg, oracle):
+    rng = np.random.default_rng(synth.SEED_BASE + 30)
+    base = rng.integers(1, 5, size=120).astype(np.uint8)
+    q = np.full(120, 35, np.uint8)
+    # one unique: a single cluster, its center corrected with p = 1
+    got, want = _run_pair(pkg, oracle, [base], [q], [7], dict(band=16))
+    _same(got, want)
+    assert len(got.clusters) == 1 and got.p[0] == 1.0 and got.map[0] == 0
+    # singletons only: no abundance p-value applies, no bud
+    seqs = [base]
+    for k in range(1, 40):
+        s = base.copy()
+        s[k] = s[k] % 4 + 1
+        seqs.append(s)
+    got, want = _run_pair(pkg, oracle, seqs, [q] * 40, [1] * 40, dict(band=16))
+    _same(got, want)
+    assert len(got.clusters) == 1
+    # equal reads everywhere: the center is the last unique; with singleton
+    # detection the prior/abundance tests still see every raw
+    got, want = _run_pair(pkg, oracle, seqs, [q] * 40, [5] * 40, dict(band=16),
+                          detect_singletons=True)
+    _same(got, want)
+    assert got.clusters[0]["center"] == 39
+
+
+def test_run_dada_long_reads_homopolymer(pkg, oracle):
+    w = synth.workload_partition(n_true=3, n_near=1, n_prior=0, L=1450, n_uniques=50,
+                                 seed=synth.SEED_BASE + 31)
+    print(f"long-read partition: {w.n} uniques of {len(w.seqs[0])} nt", flush=True)
+    got, want = _run_pair(pkg, oracle, w.seqs, w.quals, w.reads, dict(band=32, homo_gap_p=-1))
+    _same(got, want)
+    assert sorted(c["center"] for c in got.clusters) == sorted(int(x) for x in w.extra["planted"])
+
+
 def test_run_dada_errors(pkg):
     w = synth.workload_partition(4, 1, 0, 120, 60, seed=synth.SEED_BASE + 16)
     ctx = pkg.Context(0)
