@@ -366,10 +366,14 @@ def bench_longread(args, pkg):
     st = [ctx.compare_batch(centers, w.err, params, device_only=True)
           for _ in range(max(3, args.steps // 2))]
     bs = synth.workload_bimera()
-    ctx.is_bimera_batch(bs.queries[:50], bs.parent_lists[:50])
+    ctx.is_bimera_pool(bs.queries[:50], bs.pool, bs.parent_idx[:50])
     t1 = time.perf_counter()
-    flags = ctx.is_bimera_batch(bs.queries, bs.parent_lists)
+    flags = ctx.is_bimera_pool(bs.queries, bs.pool, bs.parent_idx)
     bim_s = time.perf_counter() - t1
+    t2 = time.perf_counter()
+    flags_listed = ctx.is_bimera_batch(bs.queries, bs.parent_lists)
+    bim_listed_s = time.perf_counter() - t2
+    assert np.array_equal(flags, flags_listed)
     ctx.close()
     n_pairs = len(centers) * w.n
     ms = wall / args.steps * 1e3
@@ -405,7 +409,9 @@ def bench_longread(args, pkg):
                                                 "aligned_dp", "dp_cells")},
                  bimera={"queries": len(bs.queries), "parents_per_query": 50,
                          "alignments_per_s": len(bs.queries) * 50 / bim_s,
-                         "seconds": bim_s, "flagged": int(flags.sum())},
+                         "seconds": bim_s, "flagged": int(flags.sum()),
+                         "entry": "d2g_is_bimera_pool (parents as pool indices)",
+                         "listed_parents_seconds": bim_listed_s},
                  cpu_baseline=cpu)
 
 

@@ -114,6 +114,8 @@ def _lib():
                                            C.POINTER(BimeraParams), u32p, u32p]),
             "d2g_is_bimera_batch": (C.c_int, [vp, C.c_uint32, u8p, u64p, u8p, u64p, u32p,
                                               C.POINTER(BimeraParams), u8p]),
+            "d2g_is_bimera_pool": (C.c_int, [vp, C.c_uint32, u8p, u64p, C.c_uint32, u8p, u64p,
+                                             u32p, u32p, C.POINTER(BimeraParams), u8p]),
             "d2g_table_bimera2": (C.c_int, [vp, u32p, C.c_uint32, C.c_uint32, u8p, u64p,
                                             C.c_double, C.c_uint32, C.POINTER(BimeraParams),
                                             u32p, u32p]),
@@ -144,7 +146,7 @@ def exported_symbols() -> list[str]:
             "d2g_compare_batch", "d2g_compare_batch_device", "d2g_fetch_results",
             "d2g_align_vectorized", "d2g_vectorized_cells", "d2g_vectorized_supported",
             "d2g_align_endsfree", "d2g_band_supported", "d2g_pairs_load", "d2g_pairs_lambda",
-            "d2g_pairs_fetch", "d2g_bimera_pairs", "d2g_is_bimera_batch", "d2g_table_bimera2",
+            "d2g_pairs_fetch", "d2g_bimera_pairs", "d2g_is_bimera_batch", "d2g_is_bimera_pool", "d2g_table_bimera2",
             "d2g_loglik_forward", "d2g_loglik_backward", "d2g_ctx_set_graphs",
             "d2g_ctx_set_screen_shape", "d2g_ctx_set_autotune", "d2g_run_dada"]
 
@@ -503,6 +505,23 @@ class Context:
         _check(_lib().d2g_is_bimera_batch(self.h, nq, _p(qc, u8p), _p(qo, u64p), _p(pc, u8p),
                                           _p(po, u64p), _p(ps_, u32p), C.byref(params),
                                           _p(out, u8p)), "d2g_is_bimera_batch")
+        return out.astype(bool)
+
+    def is_bimera_pool(self, queries, pool, parent_idx, params: BimeraParams | None = None):
+        """is_bimera_with_buf (chimera.rs:218) for each query against the
+        pool sequences parent_idx[i] (one index list per query)."""
+        params = params or bimera_params()
+        nq = len(queries)
+        qc, qo = _concat(queries)
+        pc, po = _concat(pool)
+        idx = [int(x) for ps in parent_idx for x in ps]
+        pi = np.asarray(idx if idx else [0], np.uint32)
+        ps_ = np.zeros(nq + 1, np.uint32)
+        ps_[1:] = np.cumsum([len(ps) for ps in parent_idx])
+        out = np.zeros(nq, np.uint8)
+        _check(_lib().d2g_is_bimera_pool(self.h, nq, _p(qc, u8p), _p(qo, u64p), len(pool),
+                                         _p(pc, u8p), _p(po, u64p), _p(pi, u32p), _p(ps_, u32p),
+                                         C.byref(params), _p(out, u8p)), "d2g_is_bimera_pool")
         return out.astype(bool)
 
     def table_bimera2(self, mat, seqs, min_fold=1.5, min_abund=2,

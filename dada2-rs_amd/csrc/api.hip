@@ -882,22 +882,44 @@ int d2g_compare_center(d2g_ctx* c, uint32_t center, const double* err, uint32_t 
 // in the context's device buffers (a_al0/a_al1 at a_aloff, lengths a_allen);
 // *al_total receives the bytes spanned.  Shared by the align and bimera
 // entry points.
-static int align_pairs_device(d2g_ctx* c, uint32_t n, const uint8_t* s1, const uint64_t* s1_off,
-                              const uint8_t* s2, const uint64_t* s2_off, int kind, int homo,
+// The sequences of n pairs: pair i is sequence i1(i) of the first set and
+// i2(i) of the second (each set concatenated, offsets with n1 + 1 / n2 + 1
+// entries; a null index is the identity).  Pairs that share a sequence
+// share its copy on the device.
+struct PairSeqs {
+  const uint8_t* s1;
+  const uint64_t* s1_off;
+  uint32_t n1;
+  const uint32_t* idx1;
+  const uint8_t* s2;
+  const uint64_t* s2_off;
+  uint32_t n2;
+  const uint32_t* idx2;
+  uint32_t i1(uint32_t i) const { return idx1 ? idx1[i] : i; }
+  uint32_t i2(uint32_t i) const { return idx2 ? idx2[i] : i; }
+  uint64_t len1(uint32_t i) const { return s1_off[i1(i) + 1] - s1_off[i1(i)]; }
+  uint64_t len2(uint32_t i) const { return s2_off[i2(i) + 1] - s2_off[i2(i)]; }
+};
+
+static int align_pairs_device(d2g_ctx* c, uint32_t n, const PairSeqs& ps, int kind, int homo,
                               int band, int match, int mismatch, int gap, int end_gap,
                               int homo_gap, const uint64_t* al_offsets, uint64_t* al_total_out) {
-  if (!c || !s1 || !s1_off || !s2 || !s2_off) return D2G_ERR_ARG;
+  if (!c || !ps.s1 || !ps.s1_off || !ps.s2 || !ps.s2_off) return D2G_ERR_ARG;
   *al_total_out = 0;
   c->cmp_valid = false;  // the task buffers are reused below
   if (n == 0) return D2G_OK;
   (void)hipSetDevice(c->device);
-  // pool: all s1 then all s2
+  for (uint32_t k = 0; k < ps.n1; ++k)
+    if (ps.s1_off[k + 1] < ps.s1_off[k]) return D2G_ERR_ARG;
+  for (uint32_t k = 0; k < ps.n2; ++k)
+    if (ps.s2_off[k + 1] < ps.s2_off[k]) return D2G_ERR_ARG;
+  // pool: the first set, then the second
   std::vector<uint32_t> l1(n), l2(n);
   uint32_t lmin = 0xFFFFFFFFu, lmax = 0;
   for (uint32_t i = 0; i < n; ++i) {
-    if (s1_off[i + 1] < s1_off[i] || s2_off[i + 1] < s2_off[i]) return D2G_ERR_ARG;
-    l1[i] = (uint32_t)(s1_off[i + 1] - s1_off[i]);
-    l2[i] = (uint32_t)(s2_off[i + 1] - s2_off[i]);
+    if (ps.i1(i) >= ps.n1 || ps.i2(i) >= ps.n2) return D2G_ERR_ARG;
+    l1[i] = (uint32_t)ps.len1(i);
+    l2[i] = (uint32_t)ps.len2(i);
     if (l1[i] == 0 || l2[i] == 0) return D2G_ERR_ARG;
     if (kind == kDpVectorized && (l1[i] > kVectorizedMaxLen || l2[i] > kVectorizedMaxLen))
       return D2G_ERR_UNSUPPORTED;
@@ -905,7 +927,7 @@ static int align_pairs_device(d2g_ctx* c, uint32_t n, const uint8_t* s1, const u
     lmin = std::min({lmin, l1[i], l2[i]});
     lmax = std::max({lmax, l1[i], l2[i]});
   }
-  const uint64_t n1 = s1_off[n] - s1_off[0], n2 = s2_off[n] - s2_off[0];
+  const uint64_t n1 = ps.s1_off[ps.n1] - ps.s1_off[0], n2 = ps.s2_off[ps.n2] - ps.s2_off[0];
   std::vector<uint64_t> aoff(n + 1, 0);
   for (uint32_t i = 0; i < n; ++i) aoff[i + 1] = aoff[i] + l1[i] + l2[i];
   if (al_offsets)
@@ -924,8 +946,8 @@ static int align_pairs_device(d2g_ctx* c, uint32_t n, const uint8_t* s1, const u
   std::vector<uint32_t> counts(bs.nbins + 1, 0);
   for (uint32_t i = 0; i < n; ++i) {
     DpTask& t = tasks[i];
-    t.s1_off = s1_off[i] - s1_off[0];
-    t.s2_off = n1 + (s2_off[i] - s2_off[0]);
+    t.s1_off = ps.s1_off[ps.i1(i)] - ps.s1_off[0];
+    t.s2_off = n1 + (ps.s2_off[ps.i2(i)] - ps.s2_off[0]);
     t.out_idx = i;
     t.bin = (l1[i] - lmin) * (uint32_t)bs.range + (l2[i] - lmin);
     t.u_idx = 0;
@@ -946,8 +968,8 @@ static int align_pairs_device(d2g_ctx* c, uint32_t n, const uint8_t* s1, const u
   HIP_TRY(c->a_al1.ensure(al_total + 1));
   HIP_TRY(c->a_aloff.ensure(n));
   HIP_TRY(c->a_allen.ensure(n));
-  HIP_TRY(hipMemcpyAsync(c->a_pool.p, s1 + s1_off[0], n1, hipMemcpyHostToDevice, s));
-  HIP_TRY(hipMemcpyAsync(c->a_pool.p + n1, s2 + s2_off[0], n2, hipMemcpyHostToDevice, s));
+  HIP_TRY(hipMemcpyAsync(c->a_pool.p, ps.s1 + ps.s1_off[0], n1, hipMemcpyHostToDevice, s));
+  HIP_TRY(hipMemcpyAsync(c->a_pool.p + n1, ps.s2 + ps.s2_off[0], n2, hipMemcpyHostToDevice, s));
   HIP_TRY(hipMemcpyAsync(c->dp_unsorted.p, tasks.data(), n * sizeof(DpTask), hipMemcpyHostToDevice, s));
   HIP_TRY(hipMemcpyAsync(c->bin_count.p, counts.data(), (bs.nbins + 1) * sizeof(uint32_t), hipMemcpyHostToDevice, s));
   HIP_TRY(hipMemcpyAsync(c->a_aloff.p, aoff.data(), n * sizeof(uint64_t), hipMemcpyHostToDevice, s));
@@ -1003,8 +1025,10 @@ static int align_pairs(d2g_ctx* c, uint32_t n, const uint8_t* s1, const uint64_t
                        uint8_t* al1, const uint64_t* al_offsets, uint32_t* al_len) {
   if (!al0 || !al1 || !al_len) return D2G_ERR_ARG;
   uint64_t al_total = 0;
-  const int rc = align_pairs_device(c, n, s1, s1_off, s2, s2_off, kind, homo, band, match,
-                                    mismatch, gap, end_gap, homo_gap, al_offsets, &al_total);
+  if (!s1 || !s1_off || !s2 || !s2_off) return D2G_ERR_ARG;
+  const PairSeqs ps{s1, s1_off, n, nullptr, s2, s2_off, n, nullptr};
+  const int rc = align_pairs_device(c, n, ps, kind, homo, band, match, mismatch, gap, end_gap,
+                                    homo_gap, al_offsets, &al_total);
   if (rc != D2G_OK || n == 0) return rc;
   hipStream_t s = c->stream;
   HIP_TRY(hipMemcpyAsync(al_len, c->a_allen.p, n * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
@@ -1106,15 +1130,13 @@ int d2g_loglik_backward(d2g_ctx* c, float* out_grad, double* out_grad64, float* 
 
 // ---- bimera ---------------------------------------------------------------
 // (query, parent) pairs -> get_lr + get_ham_endsfree, 5 words per pair.
-static int bimera_lr_pairs(d2g_ctx* c, uint32_t n, const uint8_t* q, const uint64_t* q_off,
-                           const uint8_t* par, const uint64_t* p_off, const d2g_bimera_params* p,
-                           uint32_t* out5) {
+static int bimera_lr_pairs(d2g_ctx* c, uint32_t n, const PairSeqs& ps,
+                           const d2g_bimera_params* p, uint32_t* out5) {
   if (n == 0) return D2G_OK;
   uint64_t al_total = 0;
   // the vectorized aligner with end_gap 0 and band = max_shift (chimera.rs:234-240)
-  const int rc = align_pairs_device(c, n, q, q_off, par, p_off, kDpVectorized, 0, p->max_shift,
-                                    p->match_score, p->mismatch, p->gap_p, 0, p->gap_p, nullptr,
-                                    &al_total);
+  const int rc = align_pairs_device(c, n, ps, kDpVectorized, 0, p->max_shift, p->match_score,
+                                    p->mismatch, p->gap_p, 0, p->gap_p, nullptr, &al_total);
   if (rc != D2G_OK) return rc;
   hipStream_t s = c->stream;
   HIP_TRY(c->b_out.ensure((size_t)5 * n));
@@ -1125,33 +1147,40 @@ static int bimera_lr_pairs(d2g_ctx* c, uint32_t n, const uint8_t* q, const uint6
   return D2G_OK;
 }
 
-// Pairs per device batch: bounds the alignment-string buffers.
-static uint32_t bimera_chunk(const uint64_t* q_off, const uint64_t* p_off, uint32_t i0,
-                             uint32_t n) {
-  uint64_t bytes = 0;
-  uint32_t k = 0;
-  while (i0 + k < n && k < (1u << 18)) {
-    bytes += (q_off[i0 + k + 1] - q_off[i0 + k]) + (p_off[i0 + k + 1] - p_off[i0 + k]);
-    if (bytes > ((uint64_t)1 << 30) && k > 0) break;
-    ++k;
-  }
-  return k;
-}
-
-int d2g_bimera_pairs(d2g_ctx* c, uint32_t n, const uint8_t* q, const uint64_t* q_off,
-                     const uint8_t* par, const uint64_t* p_off, const d2g_bimera_params* p,
-                     uint32_t* out_lr, uint32_t* out_ham) {
-  if (!c || !q || !q_off || !par || !p_off || !p || !out_lr || !out_ham) return D2G_ERR_ARG;
+// get_lr + get_ham_endsfree for n (query, parent) pairs in device batches
+// that bound the alignment-string buffers (1 GiB of pair bytes, 2^18 pairs).
+// Pair t is query qidx[t] and parent pidx[t] of the two sets; a null index
+// means one sequence per pair.
+static int bimera_run(d2g_ctx* c, uint32_t n, const uint8_t* q, const uint64_t* q_off,
+                      uint32_t nq, const uint32_t* qidx, const uint8_t* par,
+                      const uint64_t* p_off, uint32_t np, const uint32_t* pidx,
+                      const d2g_bimera_params* p, uint32_t* out_lr, uint32_t* out_ham) {
+  if (qidx)
+    for (uint32_t t = 0; t < n; ++t)
+      if (qidx[t] >= nq) return D2G_ERR_ARG;
+  if (pidx)
+    for (uint32_t t = 0; t < n; ++t)
+      if (pidx[t] >= np) return D2G_ERR_ARG;
   std::vector<uint32_t> o5;
   for (uint32_t i0 = 0; i0 < n;) {
-    const uint32_t k = bimera_chunk(q_off, p_off, i0, n);
-    std::vector<uint64_t> qo(k + 1), po(k + 1);
-    for (uint32_t t = 0; t <= k; ++t) {
-      qo[t] = q_off[i0 + t];
-      po[t] = p_off[i0 + t];
+    PairSeqs ps;
+    ps.s1 = q;
+    ps.s1_off = qidx ? q_off : q_off + i0;
+    ps.idx1 = qidx ? qidx + i0 : nullptr;
+    ps.s2 = par;
+    ps.s2_off = pidx ? p_off : p_off + i0;
+    ps.idx2 = pidx ? pidx + i0 : nullptr;
+    uint64_t bytes = 0;
+    uint32_t k = 0;
+    while (i0 + k < n && k < (1u << 18)) {
+      bytes += ps.len1(k) + ps.len2(k);
+      if (bytes > ((uint64_t)1 << 30) && k > 0) break;
+      ++k;
     }
+    ps.n1 = qidx ? nq : k;
+    ps.n2 = pidx ? np : k;
     o5.resize((size_t)5 * k);
-    const int rc = bimera_lr_pairs(c, k, q, qo.data(), par, po.data(), p, o5.data());
+    const int rc = bimera_lr_pairs(c, k, ps, p, o5.data());
     if (rc != D2G_OK) return rc;
     for (uint32_t t = 0; t < k; ++t) {
       for (int f = 0; f < 4; ++f) out_lr[4 * (size_t)(i0 + t) + f] = o5[5 * (size_t)t + f];
@@ -1162,36 +1191,32 @@ int d2g_bimera_pairs(d2g_ctx* c, uint32_t n, const uint8_t* q, const uint64_t* q
   return D2G_OK;
 }
 
-int d2g_is_bimera_batch(d2g_ctx* c, uint32_t nq, const uint8_t* q, const uint64_t* q_off,
-                        const uint8_t* par, const uint64_t* p_off, const uint32_t* par_start,
-                        const d2g_bimera_params* p, uint8_t* out_flag) {
-  if (!c || !q || !q_off || !par || !p_off || !par_start || !p || !out_flag) return D2G_ERR_ARG;
+int d2g_bimera_pairs(d2g_ctx* c, uint32_t n, const uint8_t* q, const uint64_t* q_off,
+                     const uint8_t* par, const uint64_t* p_off, const d2g_bimera_params* p,
+                     uint32_t* out_lr, uint32_t* out_ham) {
+  if (!c || !q || !q_off || !par || !p_off || !p || !out_lr || !out_ham) return D2G_ERR_ARG;
+  return bimera_run(c, n, q, q_off, n, nullptr, par, p_off, n, nullptr, p, out_lr, out_ham);
+}
+
+// is_bimera_with_buf (chimera.rs:218-292) over each query's parents: pair
+// t = (query i, parent t) for t in [par_start[i], par_start[i+1]); pidx maps
+// pairs to the parent set (null: parent t is sequence t).
+static int is_bimera_run(d2g_ctx* c, uint32_t nq, const uint8_t* q, const uint64_t* q_off,
+                         const uint8_t* par, const uint64_t* p_off, uint32_t np,
+                         const uint32_t* pidx, const uint32_t* par_start,
+                         const d2g_bimera_params* p, uint8_t* out_flag) {
   const uint32_t npairs = par_start[nq];
-  // pair t = (query i, parent t) for t in [par_start[i], par_start[i+1])
-  std::vector<uint64_t> qo(npairs + 1), po(npairs + 1);
-  std::vector<uint8_t> qcat;
-  uint64_t tot = 0;
+  std::vector<uint32_t> qidx(npairs);
   for (uint32_t i = 0; i < nq; ++i) {
-    if (par_start[i + 1] < par_start[i]) return D2G_ERR_ARG;
-    tot += (uint64_t)(par_start[i + 1] - par_start[i]) * (q_off[i + 1] - q_off[i]);
+    if (par_start[i + 1] < par_start[i] || q_off[i + 1] < q_off[i]) return D2G_ERR_ARG;
+    for (uint32_t t = par_start[i]; t < par_start[i + 1]; ++t) qidx[t] = i;
   }
-  qcat.resize(tot + 1);
-  uint64_t w = 0;
-  for (uint32_t i = 0; i < nq; ++i)
-    for (uint32_t t = par_start[i]; t < par_start[i + 1]; ++t) {
-      qo[t] = w;
-      const uint64_t l = q_off[i + 1] - q_off[i];
-      std::memcpy(qcat.data() + w, q + q_off[i], l);
-      w += l;
-    }
-  qo[npairs] = w;
-  for (uint32_t t = 0; t <= npairs; ++t) po[t] = p_off[t];
   std::vector<uint32_t> lr((size_t)4 * npairs + 4), ham(npairs + 1);
-  int rc = d2g_bimera_pairs(c, npairs, qcat.data(), qo.data(), par, po.data(), p, lr.data(),
-                            ham.data());
+  const int rc = bimera_run(c, npairs, q, q_off, nq, qidx.data(), par, p_off,
+                            pidx ? np : npairs, pidx, p, lr.data(), ham.data());
   if (rc != D2G_OK) return rc;
-  // is_bimera_with_buf (chimera.rs:218-292): maxima over the parents; the
-  // early exits of the reference only stop a search already flagged
+  // maxima over the parents; the early exits of the reference only stop a
+  // search already flagged
   for (uint32_t i = 0; i < nq; ++i) {
     const uint32_t sqlen = (uint32_t)(q_off[i + 1] - q_off[i]);
     uint32_t ml = 0, mr = 0, oml = 0, omr = 0, omloo = 0, omroo = 0;
@@ -1213,6 +1238,23 @@ int d2g_is_bimera_batch(d2g_ctx* c, uint32_t nq, const uint8_t* q, const uint64_
     out_flag[i] = flag ? 1 : 0;
   }
   return D2G_OK;
+}
+
+int d2g_is_bimera_batch(d2g_ctx* c, uint32_t nq, const uint8_t* q, const uint64_t* q_off,
+                        const uint8_t* par, const uint64_t* p_off, const uint32_t* par_start,
+                        const d2g_bimera_params* p, uint8_t* out_flag) {
+  if (!c || !q || !q_off || !par || !p_off || !par_start || !p || !out_flag) return D2G_ERR_ARG;
+  return is_bimera_run(c, nq, q, q_off, par, p_off, 0, nullptr, par_start, p, out_flag);
+}
+
+int d2g_is_bimera_pool(d2g_ctx* c, uint32_t nq, const uint8_t* q, const uint64_t* q_off,
+                       uint32_t npool, const uint8_t* pool, const uint64_t* pool_off,
+                       const uint32_t* par_idx, const uint32_t* par_start,
+                       const d2g_bimera_params* p, uint8_t* out_flag) {
+  if (!c || !q || !q_off || !pool || !pool_off || !par_idx || !par_start || !p || !out_flag ||
+      npool == 0)
+    return D2G_ERR_ARG;
+  return is_bimera_run(c, nq, q, q_off, pool, pool_off, npool, par_idx, par_start, p, out_flag);
 }
 
 int d2g_table_bimera2(d2g_ctx* c, const uint32_t* mat, uint32_t nrow, uint32_t ncol,
@@ -1246,21 +1288,12 @@ int d2g_table_bimera2(d2g_ctx* c, const uint32_t* mat, uint32_t nrow, uint32_t n
       pk.push_back(k);
     }
   const uint32_t npairs = (uint32_t)pj.size();
-  std::vector<uint64_t> qo(npairs + 1), po(npairs + 1);
-  std::vector<uint8_t> qcat, pcat;
-  for (uint32_t t = 0; t < npairs; ++t) {
-    qo[t] = qcat.size();
-    po[t] = pcat.size();
-    qcat.insert(qcat.end(), seqs + off[pj[t]], seqs + off[pj[t] + 1]);
-    pcat.insert(pcat.end(), seqs + off[pk[t]], seqs + off[pk[t] + 1]);
-  }
-  qo[npairs] = qcat.size();
-  po[npairs] = pcat.size();
-  qcat.push_back(0);
-  pcat.push_back(0);
+  for (uint32_t j = 0; j < ncol; ++j)
+    if (off[j + 1] < off[j]) return D2G_ERR_ARG;
+  // both sides index the one sequence table
   std::vector<uint32_t> lr((size_t)4 * npairs + 4), ham(npairs + 1);
-  int rc = d2g_bimera_pairs(c, npairs, qcat.data(), qo.data(), pcat.data(), po.data(), p,
-                            lr.data(), ham.data());
+  int rc = bimera_run(c, npairs, seqs, off, ncol, pj.data(), seqs, off, ncol, pk.data(), p,
+                      lr.data(), ham.data());
   if (rc != D2G_OK) return rc;
   // per column: the sample loop of chimera.rs:351-431 over the cached pairs
   std::vector<int64_t> slot(ncol, -1);

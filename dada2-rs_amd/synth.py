@@ -368,6 +368,8 @@ class BimeraSet:
     queries: list[bytes]
     parent_lists: list[list[bytes]]
     chimeric: np.ndarray  # construction truth (not the reference's verdict)
+    pool: list[bytes] = field(default_factory=list)  # the parents, ASCII
+    parent_idx: list[list[int]] = field(default_factory=list)  # pool index per parent
 
 
 def workload_bimera(n_pool=200, n_queries=2000, parents_per_query=50, lmin=1420, lmax=1480,
@@ -384,7 +386,7 @@ def workload_bimera(n_pool=200, n_queries=2000, parents_per_query=50, lmin=1420,
     for _ in range(n_pool):
         L = int(rng.integers(lmin, lmax + 1))
         pool.append(_substitute(rng, anc[:L], np.full(L, div)))
-    queries, lists, truth = [], [], []
+    queries, lists, truth, idx = [], [], [], []
     for _ in range(n_queries):
         chim = rng.random() < chimera_frac
         if chim:
@@ -402,5 +404,7 @@ def workload_bimera(n_pool=200, n_queries=2000, parents_per_query=50, lmin=1420,
         par = [par[i] for i in rng.permutation(len(par))]
         queries.append(ascii_[q - 1].tobytes())
         lists.append([ascii_[pool[k] - 1].tobytes() for k in par])
+        idx.append(par)
         truth.append(chim)
-    return BimeraSet(queries, lists, np.asarray(truth))
+    return BimeraSet(queries, lists, np.asarray(truth), [ascii_[s - 1].tobytes() for s in pool],
+                     idx)

@@ -307,6 +307,16 @@ int d2g_is_bimera_batch(d2g_ctx* ctx, uint32_t nq, const uint8_t* q, const uint6
                         const uint8_t* p, const uint64_t* p_off, const uint32_t* par_start,
                         const d2g_bimera_params* params, uint8_t* out_flag);
 
+/* The same search with the parents given as indices into one pool of npool
+ * sequences (pool_off has npool+1 entries): query i's parents are
+ * pool[par_idx[t]] for t in par_start[i] .. par_start[i+1]-1.  A parent
+ * shared by many queries is copied to the device once, as the reference's
+ * is_bimera borrows its parents' slices (chimera.rs:218-222). */
+int d2g_is_bimera_pool(d2g_ctx* ctx, uint32_t nq, const uint8_t* q, const uint64_t* q_off,
+                       uint32_t npool, const uint8_t* pool, const uint64_t* pool_off,
+                       const uint32_t* par_idx, const uint32_t* par_start,
+                       const d2g_bimera_params* params, uint8_t* out_flag);
+
 /* table_bimera2 (chimera.rs:317-433): mat is column-major nrow (samples) x
  * ncol (sequences); seqs holds the ncol sequences with off[ncol+1].  Per
  * column j: out_nsam[j] samples where j is present, out_nflag[j] of them

@@ -52,6 +52,25 @@ def test_is_bimera_batch(ctx, oracle, pkg, one_off):
     assert want.any() and not want.all()
 
 
+def test_is_bimera_pool_matches_batch(ctx, oracle, pkg):
+    """Parents as indices into one pool: the same flags as per-parent bytes
+    and as the oracle."""
+    bs = synth.workload_bimera(n_pool=30, n_queries=60, parents_per_query=10, lmin=240,
+                               lmax=260, div=0.02, seed=synth.SEED_BASE + 54)
+    for one_off in (False, True):
+        params = pkg.bimera_params(allow_one_off=one_off)
+        pooled = ctx.is_bimera_pool(bs.queries, bs.pool, bs.parent_idx, params)
+        listed = ctx.is_bimera_batch(bs.queries, bs.parent_lists, params)
+        op = oracle.bimera_params(allow_one_off=one_off)
+        want = np.array([oracle.is_bimera(q, ps, op) for q, ps in zip(bs.queries, bs.parent_lists)])
+        assert np.array_equal(pooled, want) and np.array_equal(listed, want)
+    assert want.any() and not want.all()
+    # queries without parents, and an empty search
+    assert not ctx.is_bimera_pool(bs.queries[:3], bs.pool, [[], [], []]).any()
+    with pytest.raises(pkg.D2GError):  # a parent index past the pool
+        ctx.is_bimera_pool(bs.queries[:1], bs.pool, [[len(bs.pool)]])
+
+
 def test_is_bimera_long_reads(ctx, oracle, pkg):
     """The long-read shape: ~1,450 nt ASCII, band max_shift 16."""
     bs = synth.workload_bimera(n_pool=20, n_queries=12, parents_per_query=5,
