@@ -42,7 +42,8 @@ def parse():
                     help="minimum CPU time of the baseline sample (whole passes)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--workload", default="batched",
-                    choices=["batched", "plain", "precision", "gradient", "longread", "dada"],
+                    choices=["batched", "plain", "precision", "gradient", "longread", "dada",
+                             "transitions"],
                     help="batched: the headline line (default); the others measure the "
                          "plain single-center compare, the mixed-precision lambda, the "
                          "log-likelihood gradient, the long-read compare + bimera search and "
@@ -340,6 +341,80 @@ def bench_dada(args, pkg):
                            "cells_per_run": stt["dp_cells"]},
                  screen_hbm={"achieved_gbs": screen_gbs, "frac": screen_gbs / HBM_PEAK_GBS,
                              "algorithmic_bytes_per_run": screen_bytes},
+                 cpu_baseline=cpu)
+
+
+def bench_transitions(args, pkg):
+    """learn-errors' per-sample transition counts (build_trans_mat): the
+    dada workload's 50,000 uniques partitioned once (untimed), then each
+    corrected unique aligned to its cluster center and counted into the
+    16 x 41 (transition, quality) matrix.  A step is one whole
+    d2g_build_trans_mat call; transition_counts (the aux output) is timed
+    beside it."""
+    from dada2_rs_amd import synth
+    w = synth.workload_partition(n_true=300, n_near=60, n_prior=0, L=250, n_uniques=50000,
+                                 seed=synth.SEED_BASE + 20)
+    ap, dp = pkg.align_params(band=16), pkg.dada_params()
+    ctx = pkg.Context(0)
+    info = ctx.device_info()
+    ctx.load_uniques(w.seqs, w.quals, w.reads, k=w.kmer_size)
+    r = ctx.run_dada(w.err, ap, dp)
+    centers = np.asarray([c["center"] for c in r.clusters], np.int64)
+    center_of = centers[r.cluster_of.astype(np.int64)]
+    weight = np.where(r.map >= 0, np.asarray(w.reads, np.int64), 0)
+    for _ in range(args.warmup):
+        ctx.build_trans_mat(center_of, weight, ap, nq=41)
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        mat = ctx.build_trans_mat(center_of, weight, ap, nq=41)
+    wall = time.perf_counter() - t0
+    t1 = time.perf_counter()
+    for _ in range(args.steps):
+        tc = ctx.transition_counts(center_of, weight, ap, ncol=41)
+    wall_tc = time.perf_counter() - t1
+    ctx.set_timing(True)
+    st = []
+    for _ in range(max(3, args.steps // 2)):
+        ctx.build_trans_mat(center_of, weight, ap, nq=41)
+        st.append(ctx.transition_stats())
+    ctx.close()
+    counted = int((weight > 0).sum())
+    ms = wall / args.steps * 1e3
+    ms_dp = float(np.mean([x["ms_dp"] for x in st]))
+    peak_ops = info["cu_count"] * 64 * info["clock_khz"] * 1e3 / 1e12  # int32 TOP/s
+    dp_tops = st[-1]["dp_cells"] * 8 / (ms_dp * 1e-3) / 1e12
+    cpu = None
+    if not args.no_cpu:
+        import oracle as O
+        from oracle import learn
+        n_s = 3000  # the first corrected uniques of the same partition
+        idx = np.nonzero(weight > 0)[0][:n_s]
+        wsub = np.zeros_like(weight)
+        wsub[idx] = weight[idx]
+        box = {}
+
+        def one():
+            box["m"] = learn.build_trans_mat(w.seqs, w.quals, center_of, wsub, O.make_params(band=16),
+                                             41)
+        calls, dt = _timed_cpu(one, args.cpu_seconds)
+        cpu = {"value": calls * len(idx) / dt, "unit": "alignments/s", "cores": 1, "kind": "port",
+               "sample": f"{calls} x {len(idx)} corrected uniques of the same partition, "
+                         f"{dt:.1f} s: the oracle's build_trans_mat (C alignment, Python count)"}
+    return _line("raw-center alignments/sec through build_trans_mat (learn-errors counts)",
+                 counted / (ms * 1e-3), "alignments/s", args, ms, "i16 DP / u64 counts",
+                 {"workload": "transitions: build_trans_mat over the dada workload's partition "
+                              "(50000 uniques, 250 nt, 360 clusters), band 16, (5,-4,-8), nq 41"},
+                 counts={"counted": counted, "clusters": len(r.clusters),
+                         "gapless": st[-1]["gapless"], "aligned_dp": st[-1]["aligned_dp"],
+                         "dp_cells": st[-1]["dp_cells"], "matrix_sum": int(mat.sum())},
+                 transition_counts_ms=wall_tc / args.steps * 1e3,
+                 transition_counts_equal=bool(np.array_equal(tc, mat)),
+                 kernel_ms={k: float(np.mean([x[k] for x in st]))
+                            for k in ("ms_screen", "ms_plan", "ms_dp", "ms_lambda", "ms_total")},
+                 roofline={"bound": "valu", "achieved": dp_tops, "peak": peak_ops, "unit": "TOP/s",
+                           "frac": dp_tops / peak_ops, "traffic": None,
+                           "kernel": "nw_vec_kernel (alignment-output mode)",
+                           "cells_per_launch": st[-1]["dp_cells"], "avg_ms": ms_dp},
                  cpu_baseline=cpu)
 
 

@@ -22,6 +22,7 @@ LIB_PATH = PKG_DIR / "lib" / "libdada2_amd.so"
 
 D2G_OK = 0
 HAMMING_NONE = 0xFFFFFFFF
+NO_CENTER = 0xFFFFFFFF  # d2g_transition_counts: a unique outside every cluster
 
 u8p = C.POINTER(C.c_uint8)
 u32p = C.POINTER(C.c_uint32)
@@ -130,6 +131,11 @@ def _lib():
                                        C.POINTER(DadaParams), u8p, C.c_uint32,
                                        C.POINTER(C.c_uint32), C.POINTER(ClusterInfo), u32p, f64p,
                                        C.POINTER(C.c_int32), f64p, u32p, C.POINTER(DadaStats)]),
+            "d2g_transition_counts": (C.c_int, [vp, u32p, u32p, C.POINTER(AlignParams), C.c_int,
+                                                C.c_uint32, u32p]),
+            "d2g_build_trans_mat": (C.c_int, [vp, u32p, u32p, C.POINTER(AlignParams), C.c_uint32,
+                                              u32p]),
+            "d2g_transition_stats": (C.c_int, [vp, C.POINTER(Stats)]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -148,7 +154,8 @@ def exported_symbols() -> list[str]:
             "d2g_align_endsfree", "d2g_band_supported", "d2g_pairs_load", "d2g_pairs_lambda",
             "d2g_pairs_fetch", "d2g_bimera_pairs", "d2g_is_bimera_batch", "d2g_is_bimera_pool", "d2g_table_bimera2",
             "d2g_loglik_forward", "d2g_loglik_backward", "d2g_ctx_set_graphs",
-            "d2g_ctx_set_screen_shape", "d2g_ctx_set_autotune", "d2g_run_dada"]
+            "d2g_ctx_set_screen_shape", "d2g_ctx_set_autotune", "d2g_run_dada",
+            "d2g_transition_counts", "d2g_build_trans_mat", "d2g_transition_stats"]
 
 
 def _p(a: np.ndarray, t):
@@ -272,6 +279,7 @@ class Context:
         _check(L.d2g_ctx_create(device, C.byref(h)), "d2g_ctx_create")
         self.h = h
         self.n = 0
+        self.has_quals = False
         if timing:
             _check(L.d2g_ctx_set_timing(self.h, 1), "d2g_ctx_set_timing")
 
@@ -328,6 +336,7 @@ class Context:
                                        _p(q, u8p) if q is not None else None, _p(r, u32p), k),
                "d2g_load_uniques")
         self.n = len(seqs)
+        self.has_quals = q is not None
 
     def _common(self, err, params, lock):
         e = np.ascontiguousarray(err, np.float64)
@@ -415,6 +424,48 @@ class Context:
         clusters = [{name: getattr(infos[i], name) for name, _ in ClusterInfo._fields_}
                     for i in range(ncl.value)]
         return DadaResult(clusters, cof, pv, mp, cl, ch, st.as_dict())
+
+    def transition_counts(self, center_of, weight, params: AlignParams | None = None,
+                          use_quals: bool = True, ncol: int = 41) -> np.ndarray:
+        """transition_counts (error.rs:194-236) over the final alignments
+        (compute_final_subs, dada.rs:646-673) of the loaded uniques:
+        center_of[i] is the center unique of i's cluster (-1 or NO_CENTER for
+        none), weight[i] the reads it adds (0 skips it: Raw.correct false).
+        Returns a 16 x ncol u32 matrix (16 x 1 without qualities)."""
+        params = params or align_params()
+        co, w = self._center_weight(center_of, weight)
+        nout = ncol if (use_quals and self.has_quals) else 1
+        out = np.zeros(16 * nout, np.uint32)
+        _check(_lib().d2g_transition_counts(self.h, _p(co, u32p), _p(w, u32p), C.byref(params),
+                                            int(use_quals), ncol, _p(out, u32p)),
+               "d2g_transition_counts")
+        return out.reshape(16, nout)
+
+    def build_trans_mat(self, center_of, weight, params: AlignParams | None = None,
+                        nq: int = 41) -> np.ndarray:
+        """build_trans_mat (learn_errors.rs:618-690): center_of as in
+        transition_counts, weight[i] the abundance of input i when the
+        result maps it (DadaResult.map[i] >= 0), else 0.  Returns 16 x nq."""
+        params = params or align_params()
+        co, w = self._center_weight(center_of, weight)
+        out = np.zeros(16 * nq, np.uint32)
+        _check(_lib().d2g_build_trans_mat(self.h, _p(co, u32p), _p(w, u32p), C.byref(params),
+                                          nq, _p(out, u32p)), "d2g_build_trans_mat")
+        return out.reshape(16, nq)
+
+    def transition_stats(self) -> dict:
+        """Counters (and, with timing on, kernel times) of the last
+        transition_counts / build_trans_mat call."""
+        st = Stats()
+        _check(_lib().d2g_transition_stats(self.h, C.byref(st)), "d2g_transition_stats")
+        return st.as_dict()
+
+    def _center_weight(self, center_of, weight):
+        co = np.asarray(center_of)
+        if co.shape != (self.n,) or np.shape(weight) != (self.n,):
+            raise ValueError("center_of and weight need one entry per unique")
+        co = np.ascontiguousarray(np.where(co < 0, NO_CENTER, co), np.uint32)
+        return co, np.ascontiguousarray(weight, np.uint32)
 
     def pairs_load(self, seqs, quals, subs, reads, center_reads):
         """Resident aligned pairs: query i with its substitution list subs[i]

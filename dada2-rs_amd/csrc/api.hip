@@ -134,6 +134,10 @@ struct d2g_ctx {
   int ll_use_quals = 0;
   bool ll_have_w = false;
   DevBuf<double> p_lambda, p_pval, p_err;
+  // transition counts: per-unique filter and weights, the 16 x ncol matrix
+  DevBuf<uint32_t> tr_want, tr_weight;
+  DevBuf<unsigned long long> tr_mat;
+  d2g_stats tr_stats{};  // the last transition pass's compare
   // align mode buffers
   DevBuf<uint8_t> a_pool, a_al0, a_al1;
   DevBuf<uint64_t> a_aloff;
@@ -529,13 +533,30 @@ static hipError_t enqueue_fetch(d2g_ctx* c, uint32_t nc, double* out_lambda,
   return hipSuccess;
 }
 
+// The transition pass's variant of the compare (transition_counts,
+// error.rs:194-236): the screen keeps each unique's pair with its own center
+// (want, sorted order), the DP writes alignment strings per unique, and the
+// transition kernel replaces lambda.  Device pointers.
+struct TransMode {
+  const uint32_t* want;
+  const uint32_t* weight;
+  uint8_t* al0;
+  uint8_t* al1;
+  const uint64_t* al_off;
+  uint32_t* al_len;
+  int use_quals;
+  uint32_t ncol;
+  int clamp;
+  unsigned long long* mat;
+};
+
 // The compare pipeline; with fetch, the results are copied to the host
 // before the call's single synchronisation.
 static int compare_device(d2g_ctx* c, const uint32_t* centers, uint32_t nc, const double* err,
                           uint32_t ncol, const d2g_align_params* p, int use_quals, int greedy,
                           const uint8_t* lock, d2g_stats* stats, bool fetch = false,
                           double* out_lambda = nullptr, uint32_t* out_hamming = nullptr,
-                          uint8_t* out_skipped = nullptr) {
+                          uint8_t* out_skipped = nullptr, const TransMode* tm = nullptr) {
   if (!c || !centers || !err || !p || nc == 0 || ncol == 0) return D2G_ERR_ARG;
   if (!c->loaded) return D2G_ERR_STATE;
   (void)hipSetDevice(c->device);
@@ -563,22 +584,31 @@ static int compare_device(d2g_ctx* c, const uint32_t* centers, uint32_t nc, cons
   HIP_TRY(c->centers.ensure(nc));
   if (c->err.cap < (size_t)16 * ncol) c->h_err.clear();  // reallocation loses the upload
   HIP_TRY(c->err.ensure((size_t)16 * ncol));
-  HIP_TRY(c->out_lambda.ensure(npairs));
-  HIP_TRY(c->out_hamming.ensure(npairs));
-  HIP_TRY(c->out_skipped.ensure(npairs));
+  // results and task lists: one per pair, or one per unique with the
+  // transition filter (which writes no results)
+  const size_t ntask = tm ? n : npairs;
+  HIP_TRY(c->out_lambda.ensure(tm ? 1 : npairs));
+  HIP_TRY(c->out_hamming.ensure(tm ? 1 : npairs));
+  HIP_TRY(c->out_skipped.ensure(tm ? 1 : npairs));
   // per screen block: a region of screen_region(qblock) task slots in each
-  // list; sized for every screen shape (the autotuner tries several)
+  // list, sized for every screen shape (the autotuner tries several); the
+  // transition pass's pairing: kPairBlock slots per block
   size_t max_part = 0, max_slots = 0;
+  if (tm) {
+    max_part = pair_blocks(n);
+    max_slots = max_part * kPairBlock;
+  }
   for (uint32_t qb : {64u, 128u, 256u}) {
+    if (tm) break;
     max_part = std::max<size_t>(max_part, screen_blocks(n, nc, qb));
     max_slots = std::max<size_t>(max_slots, (size_t)screen_blocks(n, nc, qb) * screen_region(qb));
   }
-  HIP_TRY(c->gl.ensure(npairs));
+  HIP_TRY(c->gl.ensure(ntask));
   HIP_TRY(c->gl_unsorted.ensure(max_slots));
   HIP_TRY(c->gl_off.ensure(max_part));
   HIP_TRY(c->dp_unsorted.ensure(max_slots));
-  HIP_TRY(c->dp_sorted.ensure(npairs));
-  HIP_TRY(c->sub_cnt.ensure(npairs));
+  HIP_TRY(c->dp_sorted.ensure(ntask));
+  HIP_TRY(c->sub_cnt.ensure(ntask));
   HIP_TRY(c->bin_count.ensure(bs.nbins + 1));
   HIP_TRY(c->bin_start.ensure(bs.nbins + 1));
   HIP_TRY(c->wave_start.ensure(bs.nbins + 1));
@@ -592,8 +622,7 @@ static int compare_device(d2g_ctx* c, const uint32_t* centers, uint32_t nc, cons
   // synchronisation, so the staging area is free)
   const size_t pin_need = (size_t)nc * 4 + (lock ? n : 0) + 16;
   if (c->h_pin_cap < pin_need) {
-    if (c->h_pin) (void)hipHostFree(c->h_pin);
-  if (c->graph_exec) (void)hipGraphExecDestroy(c->graph_exec);
+    if (c->h_pin) (void)hipHostFree(c->h_pin);  // the graphs never read the staging area
     c->h_pin = nullptr;
     c->h_pin_cap = 0;
     HIP_TRY(hipHostMalloc((void**)&c->h_pin, pin_need, hipHostMallocDefault));
@@ -638,8 +667,8 @@ static int compare_device(d2g_ctx* c, const uint32_t* centers, uint32_t nc, cons
   // lambda).  Timed runs bracket the kernels with events; otherwise the
   // sequence is captured once per shape into a HIP graph and replayed.
   auto enqueue = [&](bool timed) -> int {
-    const uint32_t n_part = screen_blocks(n, nc, c->screen_qblock);
-    const uint32_t region = screen_region(c->screen_qblock);
+    const uint32_t n_part = tm ? pair_blocks(n) : screen_blocks(n, nc, c->screen_qblock);
+    const uint32_t region = tm ? kPairBlock : screen_region(c->screen_qblock);
     if (timed) HIP_TRY(hipEventRecord(c->ev[0], s));
     HIP_TRY(launch_zero(reinterpret_cast<uint32_t*>(c->ctr.p), sizeof(Counters) / 4, s));
     HIP_TRY(launch_zero(c->bin_count.p, bs.nbins + 1, s));
@@ -666,7 +695,7 @@ static int compare_device(d2g_ctx* c, const uint32_t* centers, uint32_t nc, cons
     sa.split = c->screen_split;
     sa.per_lane = c->screen_per_lane;
     sa.dstar = c->dstar.p;
-    HIP_TRY(launch_screen(sa, s));
+    HIP_TRY(tm ? launch_pair_screen(sa, tm->want, s) : launch_screen(sa, s));
     if (timed) HIP_TRY(hipEventRecord(c->ev[1], s));
     HIP_TRY(launch_plan(bs, c->bin_count.p, c->bin_start.p, c->wave_start.p, c->bin_fill.p, c->ctr.p,
                         c->sched_info.p, c->screen_partials.p, n_part, c->gl_off.p, s));
@@ -702,8 +731,37 @@ static int compare_device(d2g_ctx* c, const uint32_t* centers, uint32_t nc, cons
     na.sched = c->sched.p;
     na.info = c->sched_info.p;
     na.sched_rows = c->sched_rows;
+    if (tm) {
+      na.emit_alignment = 1;
+      na.al0 = tm->al0;
+      na.al1 = tm->al1;
+      na.al_off = tm->al_off;
+      na.al_len = tm->al_len;
+    }
     HIP_TRY(vec ? launch_nw_vec(na, s) : launch_nw_band(na, s));
     if (timed) HIP_TRY(hipEventRecord(c->ev[3], s));
+    if (tm) {
+      TransArgs ta;
+      ta.u = &u;
+      ta.gapless_tasks = c->gl.p;
+      ta.dp_tasks = c->dp_sorted.p;
+      ta.ctr = c->ctr.p;
+      ta.al0 = tm->al0;
+      ta.al1 = tm->al1;
+      ta.al_off = tm->al_off;
+      ta.al_len = tm->al_len;
+      ta.weight = tm->weight;
+      ta.use_quals = tm->use_quals;
+      ta.ncol = tm->ncol;
+      ta.clamp = tm->clamp;
+      ta.mat = tm->mat;
+      // eight waves per block, one pair per wave at a time; about 16 waves
+      // per SIMD keep the loads of the chunks in flight
+      ta.grid = (uint32_t)std::min<size_t>((size_t)c->cu_count * 8, (n + 7) / 8);
+      HIP_TRY(launch_transition(ta, s));
+      if (timed) HIP_TRY(hipEventRecord(c->ev[4], s));
+      return D2G_OK;
+    }
     LambdaArgs la;
     la.u = &u;
     la.gapless_tasks = c->gl.p;
@@ -727,7 +785,7 @@ static int compare_device(d2g_ctx* c, const uint32_t* centers, uint32_t nc, cons
   // Autotuning: the first compare of a workload class times each screen
   // shape on this call's inputs (results do not depend on the shape) and
   // keeps the fastest for the class.
-  if (c->autotune) {
+  if (c->autotune && !tm) {
     const uint64_t cls = tune_class(n, nc, u.k, bs);
     auto it = c->tuned.find(cls);
     if (it == c->tuned.end()) {
@@ -759,7 +817,7 @@ static int compare_device(d2g_ctx* c, const uint32_t* centers, uint32_t nc, cons
   }
 
   for (int attempt = 0; attempt < 6; ++attempt) {
-    const bool graph = c->graphs && !c->timing;
+    const bool graph = c->graphs && !c->timing && !tm;
     if (graph) {
       // everything a kernel argument depends on
       std::vector<uint64_t> key = {
@@ -817,6 +875,26 @@ static int compare_device(d2g_ctx* c, const uint32_t* centers, uint32_t nc, cons
       c->sub_capacity = std::min(c->sub_capacity, ((size_t)1 << 32) - 1);
       HIP_TRY(c->sub_pool.ensure(c->sub_capacity));
       continue;
+    }
+    if (tm) {  // the task lists now hold the filtered pairs only
+      c->cmp_valid = false;
+      if (stats) {
+        std::memset(stats, 0, sizeof(*stats));
+        stats->comparisons = (uint64_t)h.n_gapless + h.n_dp;  // one pair per counted unique
+        stats->gapless = h.n_gapless;
+        stats->aligned_dp = h.n_dp;
+        stats->dp_cells = h.dp_cells;
+        if (c->timing) {
+          stats->ms_screen = ev_ms(c, 0, 1);
+          stats->ms_plan = ev_ms(c, 1, 2);
+          stats->ms_dp = ev_ms(c, 2, 3);
+          stats->ms_lambda = ev_ms(c, 3, 4);  // the counting kernel
+          stats->ms_total = ev_ms(c, 0, 4);
+        }
+      }
+      if (h.err_flags & kErrTraceback) return D2G_ERR_TRACEBACK;
+      if (h.err_flags & kErrGeometry) return D2G_ERR_UNSUPPORTED;
+      return D2G_OK;
     }
     c->last_nc = nc;
     // the aligned pairs of this call stay resident for d2g_loglik_*
@@ -876,6 +954,129 @@ int d2g_compare_center(d2g_ctx* c, uint32_t center, const double* err, uint32_t 
                        d2g_stats* stats) {
   return d2g_compare_batch(c, &center, 1, err, ncol, p, use_quals, greedy, lock, out_lambda,
                            out_hamming, out_skipped, stats);
+}
+
+// The transition-by-quality counts of a finished partition: each counted
+// unique aligned to its center with *fp, counts per (transition, quality)
+// summed exactly on the device; ncol_out columns, qualities >= ncol_out
+// clamped (clamp) or skipped; u32 results saturated (saturate) or wrapped.
+static int transitions_run(d2g_ctx* c, const uint32_t* center_of, const uint32_t* weight,
+                           const d2g_align_params& fp, bool quals, uint32_t ncol_out, bool clamp,
+                           bool saturate, uint32_t* out_mat) {
+  (void)hipSetDevice(c->device);
+  const uint32_t n = c->u.n;
+  // qualities are bytes: with skip, columns from 256 on never count; with
+  // clamp, min(q, ncol - 1) = q there too
+  const uint32_t ncol_dev = std::min(ncol_out, 256u);
+  std::vector<uint32_t> centers, slot(n, D2G_NO_CENTER);
+  for (uint32_t i = 0; i < n; ++i) {
+    const uint32_t ci = center_of[i];
+    if (ci == D2G_NO_CENTER || weight[i] == 0) continue;
+    if (ci >= n) return D2G_ERR_ARG;
+    if (slot[ci] == D2G_NO_CENTER) {
+      slot[ci] = (uint32_t)centers.size();
+      centers.push_back(ci);
+    }
+  }
+  std::fill(out_mat, out_mat + (size_t)16 * ncol_out, 0u);
+  c->tr_stats = d2g_stats{};
+  if (centers.empty()) return D2G_OK;
+  // per sorted unique: its center (sorted index) or none, and its weight
+  std::vector<uint32_t> want(n), w(n);
+  for (uint32_t s = 0; s < n; ++s) {
+    const uint32_t i = c->h_orig[s];
+    const bool counted = center_of[i] != D2G_NO_CENTER && weight[i] != 0;
+    want[s] = counted ? c->h_inv[center_of[i]] : D2G_NO_CENTER;
+    w[s] = counted ? weight[i] : 0u;
+  }
+  // one alignment slot of 2 * longest length per unique (the strings of an
+  // ends-free alignment are at most len1 + len2 long)
+  const uint64_t stride = 2ull * c->u.maxlen;
+  std::vector<uint64_t> aoff(n);
+  for (uint32_t s = 0; s < n; ++s) aoff[s] = s * stride;
+  hipStream_t st = c->stream;
+  HIP_TRY(c->tr_want.ensure(n));
+  HIP_TRY(c->tr_weight.ensure(n));
+  HIP_TRY(c->tr_mat.ensure((size_t)16 * ncol_dev));
+  HIP_TRY(c->a_al0.ensure(n * stride));
+  HIP_TRY(c->a_al1.ensure(n * stride));
+  HIP_TRY(c->a_aloff.ensure(n));
+  HIP_TRY(c->a_allen.ensure(n));
+  HIP_TRY(hipMemcpyAsync(c->tr_want.p, want.data(), n * sizeof(uint32_t), hipMemcpyHostToDevice, st));
+  HIP_TRY(hipMemcpyAsync(c->tr_weight.p, w.data(), n * sizeof(uint32_t), hipMemcpyHostToDevice, st));
+  HIP_TRY(hipMemcpyAsync(c->a_aloff.p, aoff.data(), n * sizeof(uint64_t), hipMemcpyHostToDevice, st));
+  HIP_TRY(launch_zero(reinterpret_cast<uint32_t*>(c->tr_mat.p), (size_t)32 * ncol_dev, st));
+  HIP_TRY(hipStreamSynchronize(st));  // the host vectors are pageable
+  TransMode tm;
+  tm.want = c->tr_want.p;
+  tm.weight = c->tr_weight.p;
+  tm.al0 = c->a_al0.p;
+  tm.al1 = c->a_al1.p;
+  tm.al_off = c->a_aloff.p;
+  tm.al_len = c->a_allen.p;
+  tm.use_quals = quals ? 1 : 0;
+  tm.ncol = ncol_dev;
+  tm.clamp = clamp ? 1 : 0;
+  tm.mat = c->tr_mat.p;
+  // the lambda stage does not run: a zero one-column error matrix stands in
+  const double zero_err[16] = {0};
+  HIP_TRY_INT(compare_device(c, centers.data(), (uint32_t)centers.size(), zero_err, 1, &fp, 0, 0,
+                             nullptr, &c->tr_stats, false, nullptr, nullptr, nullptr, &tm));
+  std::vector<unsigned long long> m((size_t)16 * ncol_dev);
+  HIP_TRY(hipMemcpy(m.data(), c->tr_mat.p, m.size() * sizeof(m[0]), hipMemcpyDeviceToHost));
+  for (uint32_t t = 0; t < 16; ++t)
+    for (uint32_t q = 0; q < ncol_dev; ++q) {
+      const unsigned long long v = m[(size_t)t * ncol_dev + q];
+      out_mat[(size_t)t * ncol_out + q] =
+          saturate ? (uint32_t)std::min<unsigned long long>(v, 0xFFFFFFFFull) : (uint32_t)v;
+    }
+  return D2G_OK;
+}
+
+// learn-errors' aux transition counts (compute_aux, dada.rs:610-631:
+// compute_final_subs dada.rs:646-673 with the k-mer screen off, then
+// transition_counts error.rs:194-236).
+int d2g_transition_counts(d2g_ctx* c, const uint32_t* center_of, const uint32_t* weight,
+                          const d2g_align_params* p, int use_quals, uint32_t ncol,
+                          uint32_t* out_mat) {
+  if (!c || !center_of || !weight || !p || !out_mat || ncol == 0) return D2G_ERR_ARG;
+  if (!c->loaded) return D2G_ERR_STATE;
+  const bool quals = use_quals && c->has_qual;
+  d2g_align_params fp = *p;  // dada.rs:612-616
+  fp.use_kmers = 0;
+  fp.kdist_cutoff = 1.0;
+  return transitions_run(c, center_of, weight, fp, quals, quals ? ncol : 1u, false, false,
+                         out_mat);
+}
+
+int d2g_transition_stats(d2g_ctx* c, d2g_stats* out) {
+  if (!c || !out) return D2G_ERR_ARG;
+  *out = c->tr_stats;
+  return D2G_OK;
+}
+
+// build_trans_mat (learn_errors.rs:618-690): the counts of one sample's
+// result that each learn-errors iteration sums.
+int d2g_build_trans_mat(d2g_ctx* c, const uint32_t* center_of, const uint32_t* weight,
+                        const d2g_align_params* p, uint32_t nq, uint32_t* out_mat) {
+  if (!c || !center_of || !weight || !p || !out_mat || nq == 0) return D2G_ERR_ARG;
+  if (!c->loaded) return D2G_ERR_STATE;
+  // inputs without qualities add nothing (learn_errors.rs:648-651)
+  if (!c->has_qual) {
+    std::fill(out_mat, out_mat + (size_t)16 * nq, 0u);
+    return D2G_OK;
+  }
+  // the center and query Raws carry no k-mer vectors, so kdist is 0.0 and
+  // kodist -1.0 (nwalign.rs:1262-1292): the screen passes every pair unless
+  // the cutoff is below 0, and only band 0 aligns gaplessly
+  if (p->use_kmers && 0.0 > p->kdist_cutoff) {
+    std::fill(out_mat, out_mat + (size_t)16 * nq, 0u);
+    return D2G_OK;
+  }
+  d2g_align_params fp = *p;
+  fp.use_kmers = 0;
+  fp.kdist_cutoff = 1.0;
+  return transitions_run(c, center_of, weight, fp, true, nq, true, true, out_mat);
 }
 
 // n independent pairs through one DP kernel with the alignment strings left

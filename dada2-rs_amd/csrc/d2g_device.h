@@ -25,4 +25,27 @@ __device__ __forceinline__ int shift_from_upper(int v) {
 
 __device__ __forceinline__ int uni(int v) { return __builtin_amdgcn_readfirstlane(v); }
 
+// Adds one per lane with `pred` to counter[key]; the lanes of a wavefront
+// with equal keys share one atomic (device-scope atomics on one word
+// serialise across the chip, and equal-length pairs all share one bin).
+// Returns each such lane's slot: the counter's old value plus the lane's
+// rank among its key's lanes.  Every lane of the wavefront must call this.
+__device__ __forceinline__ uint32_t wave_add_by_key(uint32_t* counter, uint32_t key, bool pred) {
+  const uint32_t lane = __lane_id();
+  uint64_t pending = __builtin_amdgcn_ballot_w64(pred);
+  uint32_t slot = 0;
+  while (pending) {
+    const int leader = __builtin_ffsll((long long)pending) - 1;
+    const uint32_t k = (uint32_t)__shfl((int)key, leader);
+    const uint64_t grp = __builtin_amdgcn_ballot_w64(pred && key == k) & pending;
+    uint32_t base = 0;
+    if ((int)lane == leader) base = atomicAdd(&counter[k], (uint32_t)__builtin_popcountll(grp));
+    base = (uint32_t)__shfl((int)base, leader);
+    if ((grp >> lane) & 1ull)
+      slot = base + (uint32_t)__builtin_popcountll(grp & ((1ull << lane) - 1ull));
+    pending &= ~grp;
+  }
+  return slot;
+}
+
 }  // namespace d2g

@@ -71,6 +71,16 @@ uint32_t screen_blocks(uint32_t n_uniques, uint32_t n_centers, uint32_t qblock);
 constexpr uint32_t kScreenCenters = 16;  // centers per screen workgroup
 inline uint32_t screen_region(uint32_t qblock) { return qblock * kScreenCenters; }
 
+// The transition pass's pairing in place of the screen: unique q (sorted
+// index) with the one center want[q] (0xFFFFFFFF: none), tasks with
+// out_idx = q, no k-mer screen (use_kmers off, so gapless exactly when
+// band == 0) and no results.  Same outputs as the screen for the plan and
+// scatter: kPairBlock uniques per workgroup, each owning kPairBlock task
+// slots in each list.
+constexpr uint32_t kPairBlock = 256;
+inline uint32_t pair_blocks(uint32_t n_uniques) { return (n_uniques + kPairBlock - 1) / kPairBlock; }
+hipError_t launch_pair_screen(const ScreenArgs& a, const uint32_t* want, hipStream_t s);
+
 // ---- plan.hip ----------------------------------------------------------
 hipError_t launch_zero(uint32_t* p, size_t n_words, hipStream_t s);
 // Also totals the DP cells (info: per-bin cells, nullable) and, for a
@@ -154,6 +164,27 @@ struct LambdaArgs {
   uint32_t grid;
 };
 hipError_t launch_lambda(const LambdaArgs& a, hipStream_t s);
+
+// ---- transition.hip ----------------------------------------------------
+// transition_counts (error.rs:194-236) over the tasks of a filtered compare
+// whose DP pairs wrote their alignment strings at al_off[unique].
+struct TransArgs {
+  const UniqueDev* u;
+  const GaplessTask* gapless_tasks;
+  const DpTask* dp_tasks;  // sorted
+  const Counters* ctr;
+  const uint8_t* al0;
+  const uint8_t* al1;
+  const uint64_t* al_off;
+  const uint32_t* al_len;
+  const uint32_t* weight;  // per sorted unique
+  int use_quals;
+  uint32_t ncol;            // <= 256 (qualities are bytes)
+  int clamp;                // qualities >= ncol: clamped to ncol - 1, else skipped
+  unsigned long long* mat;  // [16][ncol], exact sums (zeroed by the caller)
+  uint32_t grid;
+};
+hipError_t launch_transition(const TransArgs& a, hipStream_t s);
 
 // ---- bimera.hip --------------------------------------------------------
 // get_lr + get_ham_endsfree of n device-resident alignments (5 words each).

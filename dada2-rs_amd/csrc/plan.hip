@@ -7,6 +7,7 @@
 // (scalar registers) while lanes compute cells.  plan_kernel turns per-bin
 // pair counts into bin offsets and wavefront offsets (G pairs per
 // wavefront, bin_groups); scatter_kernel places each pair into its bin.
+#include "d2g_device.h"
 #include "d2g_launch.h"
 
 namespace d2g {
@@ -155,10 +156,14 @@ __global__ __launch_bounds__(256) void scatter_kernel(const DpTask* unsorted, Dp
   const uint32_t r = blockIdx.x;
   const uint32_t ndp = partials ? partials[4 * r + 3] : min(region, count - r * region);
   const DpTask* src = unsorted + (size_t)r * region;
-  for (uint32_t i = threadIdx.x; i < ndp; i += blockDim.x) {
-    const DpTask t = src[i];
-    const uint32_t pos = bin_start[t.bin] + atomicAdd(&bin_fill[t.bin], 1u);
-    sorted[pos] = t;
+  // (a block-uniform trip count: every lane takes part in the keyed add)
+  for (uint32_t i0 = 0; i0 < ndp; i0 += blockDim.x) {
+    const uint32_t i = i0 + threadIdx.x;
+    const bool ok = i < ndp;
+    DpTask t;
+    if (ok) t = src[i];
+    const uint32_t rank = wave_add_by_key(bin_fill, ok ? t.bin : 0u, ok);
+    if (ok) sorted[bin_start[t.bin] + rank] = t;
   }
   if (partials) {
     const uint32_t ngl = partials[4 * r + 2];

@@ -32,6 +32,8 @@
 #include <float.h>
 
 #include "d2g_device.h"
+#include <cstring>
+
 #include "d2g_launch.h"
 
 namespace d2g {
@@ -332,6 +334,72 @@ void screen_thresholds(double cutoff, uint32_t nmax, uint32_t* out) {
 
 uint32_t screen_blocks(uint32_t n_uniques, uint32_t n_centers, uint32_t qblock) {
   return ((n_uniques + qblock - 1) / qblock) * ((n_centers + kScreenCenters - 1) / kScreenCenters);
+}
+
+__global__ __launch_bounds__(kPairBlock) void pair_kernel(ScreenDev a, const uint32_t* want) {
+  __shared__ uint32_t s_ngl, s_ndp;
+  if (threadIdx.x == 0) {
+    s_ngl = 0;
+    s_ndp = 0;
+  }
+  __syncthreads();
+  const UniqueDev& u = a.u;
+  const uint32_t q = blockIdx.x * kPairBlock + threadIdx.x;
+  const uint32_t ci = q < u.n ? want[q] : 0xFFFFFFFFu;
+  const bool live = ci < u.n;
+  // raw_align_dp (nwalign.rs:1324): kdist 0.0 and kodist -1.0 without the
+  // k-mer screen, so only band 0 takes the gapless alignment
+  const bool to_gl = live && a.band == 0;
+  const bool to_dp = live && a.band != 0;
+  GaplessTask* gl_region = a.gapless_tasks + (size_t)blockIdx.x * kPairBlock;
+  DpTask* dp_region = a.dp_unsorted + (size_t)blockIdx.x * kPairBlock;
+  const uint32_t gpos = wave_append(&s_ngl, to_gl);
+  if (to_gl) {
+    GaplessTask t;
+    t.c_idx = ci;
+    t.u_idx = q;
+    t.out_idx = q;
+    t._pad = 0;
+    gl_region[gpos] = t;
+  }
+  const uint32_t dpos = wave_append(&s_ndp, to_dp);
+  uint32_t bin = 0;
+  if (to_dp) {
+    const uint32_t lc = u.len[ci], lq = u.len[q];
+    bin = (lc - (uint32_t)a.bins.lmin) * (uint32_t)a.bins.range + (lq - (uint32_t)a.bins.lmin);
+    DpTask t;
+    t.s1_off = u.off[ci];
+    t.s2_off = u.off[q];
+    t.out_idx = q;
+    t.bin = bin;
+    t.u_idx = q;
+    t.c_idx = ci;
+    dp_region[dpos] = t;
+  }
+  (void)wave_add_by_key(a.bin_count, bin, to_dp);
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    a.partials[4 * blockIdx.x] = 0;
+    a.partials[4 * blockIdx.x + 1] = 0;
+    a.partials[4 * blockIdx.x + 2] = s_ngl;
+    a.partials[4 * blockIdx.x + 3] = s_ndp;
+  }
+}
+
+hipError_t launch_pair_screen(const ScreenArgs& s, const uint32_t* want, hipStream_t st) {
+  if (s.use_kmers || want == nullptr) return hipErrorInvalidValue;
+  ScreenDev d;
+  std::memset(&d, 0, sizeof(d));
+  d.u = *s.u;
+  d.band = s.band;
+  d.bins = s.bins;
+  d.gapless_tasks = s.gapless_tasks;
+  d.dp_unsorted = s.dp_unsorted;
+  d.bin_count = s.bin_count;
+  d.partials = s.partials;
+  d.ctr = s.ctr;
+  hipLaunchKernelGGL(pair_kernel, dim3(pair_blocks(d.u.n)), dim3(kPairBlock), 0, st, d, want);
+  return hipGetLastError();
 }
 
 template <int QB, int QL>

@@ -223,6 +223,41 @@ int d2g_run_dada(d2g_ctx* ctx, const double* err, uint32_t ncol, const d2g_align
                  uint32_t* out_cluster_of, double* out_p, int32_t* out_map,
                  double* out_comp_lambda, uint32_t* out_comp_hamming, d2g_dada_stats* stats);
 
+/* Backs transition_counts (error.rs:194-236) over compute_final_subs
+ * (dada.rs:646-673): the learn-errors transition-by-quality matrix of a
+ * finished partition.  Per loaded unique i (caller order):
+ *   center_of[i]  the unique at the center of i's cluster, or D2G_NO_CENTER
+ *   weight[i]     the reads i adds (Raw.reads when Raw.correct), 0 = skip
+ * Each counted unique is aligned to its center as compute_aux does
+ * (dada.rs:612-616: align with use_kmers = 0 and kdist_cutoff = 1.0, the
+ * rest of *align as given).  For each center position aligned to a base,
+ * out_mat[(4 (center base - 1) + (base - 1)) * ncol_out + quality] += weight;
+ * qualities >= ncol_out are not counted (error.rs:229).  ncol_out = ncol
+ * when use_quals is set and the uniques were loaded with qualities, else 1
+ * (dada.rs:630); out_mat holds 16 * ncol_out u32 (the sums wrap, as the
+ * reference's u32 additions do in a release build). */
+#define D2G_NO_CENTER 0xFFFFFFFFu
+int d2g_transition_counts(d2g_ctx* ctx, const uint32_t* center_of, const uint32_t* weight,
+                          const d2g_align_params* align, int use_quals, uint32_t ncol,
+                          uint32_t* out_mat);
+
+/* Backs build_trans_mat (learn_errors.rs:618-690), the per-sample counts
+ * each learn-errors iteration sums.  center_of as above; weight[i] the
+ * input's abundance when DadaResult.map[i] is set, else 0.  The pairs align
+ * as in the reference, whose Raws there carry no k-mer vectors: no k-mer
+ * screen (nothing counts when align->use_kmers and kdist_cutoff < 0), gapless
+ * only at band 0.  Qualities >= nq count in column nq - 1, the u32 sums
+ * saturate, and uniques loaded without qualities add nothing (the matrix is
+ * all zero).  out_mat holds 16 * nq u32. */
+int d2g_build_trans_mat(d2g_ctx* ctx, const uint32_t* center_of, const uint32_t* weight,
+                        const d2g_align_params* align, uint32_t nq, uint32_t* out_mat);
+
+/* Counters of the last d2g_transition_counts / d2g_build_trans_mat call's
+ * alignments (comparisons = centers x uniques screened, gapless, aligned_dp,
+ * dp_cells) and, with the context's timing on, its kernel times (ms_lambda
+ * holds the counting kernel's).  No reference counterpart: diagnostics. */
+int d2g_transition_stats(d2g_ctx* ctx, d2g_stats* out);
+
 /* ---- pairwise alignment (parity / aux mode) -------------------------------
  * Backs align_vectorized_with_buf (nwalign.rs:761-1140) for n arbitrary byte
  * pairs (any alphabet: 1..4 or ASCII).  s1/s2 are concatenated with offsets

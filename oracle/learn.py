@@ -1,0 +1,107 @@
+"""learn.py — restatement of the transition-by-quality counts of a finished
+partition.  TEST INFRASTRUCTURE ONLY: the checker of the device library's
+d2g_transition_counts and d2g_build_trans_mat; only tests/ may import it.
+
+  * transition_counts (src/error.rs:194-236) over compute_final_subs
+    (src/dada.rs:646-673), aligned with compute_aux's final parameters
+    (src/dada.rs:612-616: use_kmers false, kdist_cutoff 1.0): each counted
+    raw's Sub against its center (the oracle's C sub_new, nwalign.rs:1429),
+    then for every center position whose map is not GAP_GLYPH,
+    mat[4 (nt0 - 1) + (nt1 - 1)][qual] += reads, qualities >= ncol skipped,
+    the u32 sums wrapping;
+  * build_trans_mat (src/learn_errors.rs:618-690): the alignment strings of
+    raw_align_with_buf (the oracle's C raw_align) over Raws without k-mer
+    vectors, walked position by position; qualities clamped to nq - 1, the
+    u32 sums saturating, inputs without qualities skipped.
+
+Parity: pinned by a hand-worked case (an insertion, a deletion, a
+substitution and an end overhang, counted by hand), by the closed form of
+gapless pairs (band 0: a histogram of (center base, query base, quality) at
+equal positions), by the two restatements agreeing wherever their quality
+and overflow rules coincide, and by the row sums (every aligned center
+position counts once).
+"""
+from __future__ import annotations
+
+import numpy as np
+
+from . import AlignParams, Buf, RawSet, raw_align, sub_new
+
+GAP_GLYPH = 9999  # nwalign.rs:27
+
+
+def final_align(align: AlignParams) -> AlignParams:
+    """compute_aux's final-subs parameters (dada.rs:612-616)."""
+    p = AlignParams.from_buffer_copy(align)
+    p.use_kmers = 0
+    p.kdist_cutoff = 1.0
+    return p
+
+
+def transition_counts(rs: RawSet, center_of, weight, align: AlignParams, has_quals: bool,
+                      ncol: int) -> np.ndarray:
+    """transition_counts (error.rs:194-236) over compute_final_subs
+    (dada.rs:646-673); center_of[i] < 0 for none, weight[i] = reads when
+    Raw.correct, else 0.  16 x ncol (x 1 without qualities), u32."""
+    ncol = ncol if has_quals else 1
+    fp = final_align(align)
+    buf = Buf()
+    mat = [0] * (16 * ncol)
+    for i in range(len(rs)):
+        c, w = int(center_of[i]), int(weight[i])
+        if c < 0 or w == 0:
+            continue
+        r = sub_new(rs, c, i, fp, buf)
+        if r is None:  # no Sub (dada.rs:666-668): nothing counted
+            continue
+        s, _buf = r
+        cseq, qseq = rs.seqs[c], rs.seqs[i]
+        for pos0 in range(s.len0):
+            pos1 = s.map[pos0]
+            if pos1 == GAP_GLYPH:
+                continue
+            nti0 = max(int(cseq[pos0]) - 1, 0)
+            nti1 = max(int(qseq[pos1]) - 1, 0)
+            qual = int(rs.quals[i][pos1]) if (has_quals and rs.quals is not None) else 0
+            t = 4 * nti0 + nti1
+            if t < 16 and qual < ncol:
+                mat[t * ncol + qual] += w
+    return (np.asarray(mat, dtype=object) % (1 << 32)).astype(np.uint32).reshape(16, ncol)
+
+
+def build_trans_mat(seqs, quals, center_of, weight, align: AlignParams, nq: int) -> np.ndarray:
+    """build_trans_mat (learn_errors.rs:618-690): seqs / quals the inputs
+    (rounded qualities, or None), center_of[i] the unique at the center of
+    i's cluster (< 0 when the result maps no cluster), weight[i] the input's
+    abundance.  16 x nq, u32, saturating."""
+    mat = [0] * (16 * nq)
+    if quals is None:
+        return np.zeros((16, nq), np.uint32)
+    rs = RawSet(seqs, quals, weight, with_kmers=False)  # no k-mer vectors here
+    buf = Buf()
+    for i in range(len(rs)):
+        c, w = int(center_of[i]), int(weight[i])
+        if c < 0 or w == 0:
+            continue
+        ok, _kd, _ko, al, _m = raw_align(rs, c, i, align, buf)
+        if not ok:
+            continue
+        al0, al1 = al
+        qpos = 0
+        for a0, a1 in zip(al0, al1):
+            if 1 <= a0 <= 4 and 1 <= a1 <= 4:
+                q = min(int(rs.quals[i][qpos]), nq - 1)
+                mat[((a0 - 1) * 4 + (a1 - 1)) * nq + q] += w
+            if a1 != ord("-"):
+                qpos += 1
+    return np.minimum(np.asarray(mat, dtype=object), 0xFFFFFFFF).astype(np.uint32).reshape(16, nq)
+
+
+def center_weight(out, reads):
+    """(center_of, weight) of a run_dada result: center_of[i] the center of
+    i's cluster, weight[i] its reads when the read map keeps it (Raw.correct,
+    dada.rs:502-533), else 0."""
+    centers = np.asarray([c["center"] for c in out.clusters], np.int64)
+    center_of = centers[np.asarray(out.cluster_of, np.int64)]
+    weight = np.where(np.asarray(out.map) >= 0, np.asarray(reads, np.int64), 0)
+    return center_of, weight

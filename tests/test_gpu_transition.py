@@ -1,0 +1,151 @@
+"""d2g_transition_counts and d2g_build_trans_mat (the device alignment of
+each raw with its center, counted into a 16 x ncol matrix) against the
+oracle's restatements (oracle/learn.py), bit-exact."""
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+from dada2_rs_amd import formats, synth
+
+pytestmark = pytest.mark.gpu
+GOLDEN = Path(__file__).resolve().parent / "golden"
+
+
+def _with_indels(seqs, quals, rng, frac=0.3):
+    """A deletion or an insertion of one base in a fraction of the uniques
+    (their quality string edited the same way), so the final alignments have
+    gaps and unequal lengths."""
+    seqs = [s.copy() for s in seqs]
+    quals = [q.copy() for q in quals]
+    for i in range(len(seqs)):
+        if rng.random() >= frac:
+            continue
+        p = int(rng.integers(5, len(seqs[i]) - 5))
+        if rng.random() < 0.5:
+            seqs[i] = np.delete(seqs[i], p)
+            quals[i] = np.delete(quals[i], p)
+        else:
+            seqs[i] = np.insert(seqs[i], p, np.uint8(rng.integers(1, 5)))
+            quals[i] = np.insert(quals[i], p, quals[i][p])
+    return seqs, quals
+
+
+def _partition(oracle, seqs, quals, reads, err, params):
+    from oracle import learn
+    from oracle import partition as P
+    rs = oracle.RawSet(seqs, quals, reads, k=5)
+    out = P.run_dada(rs, err, params)
+    center_of, weight = learn.center_weight(out, reads)
+    return rs, center_of, weight
+
+
+CASES = [
+    # (name, workload args, align args, ncol, big weights)
+    ("vectorized", dict(n_true=8, n_near=2, L=150, n_uniques=400, seed_off=50), dict(band=16), 41,
+     False),
+    ("gapless_band0", dict(n_true=6, n_near=2, L=140, n_uniques=300, seed_off=51), dict(band=0), 41,
+     False),
+    ("scalar_endsfree", dict(n_true=6, n_near=2, L=150, n_uniques=300, seed_off=52),
+     dict(band=16, vectorized=False), 41, False),
+    ("narrow_ncol", dict(n_true=6, n_near=2, L=150, n_uniques=300, seed_off=53), dict(band=16), 20,
+     False),
+    ("big_weights", dict(n_true=5, n_near=1, L=150, n_uniques=250, seed_off=54), dict(band=16), 41,
+     True),
+    ("long_homopolymer", dict(n_true=3, n_near=1, L=1450, n_uniques=40, seed_off=55),
+     dict(band=32, homo_gap_p=-1), 41, False),
+]
+
+
+@pytest.mark.parametrize("name,wargs,akw,ncol,big", CASES, ids=[c[0] for c in CASES])
+def test_transitions_match_oracle(pkg, oracle, name, wargs, akw, ncol, big):
+    from oracle import learn
+    wargs = dict(wargs)
+    seed = synth.SEED_BASE + wargs.pop("seed_off")
+    w = synth.workload_partition(seed=seed, **wargs)
+    rng = np.random.default_rng(seed)
+    seqs, quals = _with_indels(w.seqs, w.quals, rng)
+    params = oracle.make_params(**akw)
+    rs, center_of, weight = _partition(oracle, seqs, quals, w.reads, w.err, params)
+    if big:  # u32 sums that wrap (transition_counts) or saturate (build_trans_mat)
+        weight = np.where(weight > 0, rng.integers(1 << 28, 1 << 32, size=len(weight)), 0)
+        rs = oracle.RawSet(seqs, quals, np.maximum(weight, 1), k=5)
+    print(f"case {name}: {len(seqs)} uniques, {len(set(center_of.tolist()))} centers, "
+          f"{int((weight > 0).sum())} counted", flush=True)
+    want_tc = learn.transition_counts(rs, center_of, weight, params, True, ncol)
+    want_bt = learn.build_trans_mat(seqs, quals, center_of, weight, params, ncol)
+    ctx = pkg.Context(0)
+    try:
+        ctx.load_uniques(seqs, quals, w.reads, k=5)
+        ap = pkg.align_params(**akw)
+        got_tc = ctx.transition_counts(center_of, weight, ap, ncol=ncol)
+        got_bt = ctx.build_trans_mat(center_of, weight, ap, nq=ncol)
+    finally:
+        ctx.close()
+    assert np.array_equal(got_tc, want_tc)
+    assert np.array_equal(got_bt, want_bt)
+    assert want_tc.sum() > 0
+    if big:
+        assert (want_bt == 0xFFFFFFFF).any()
+
+
+def test_transitions_fixture_reads(pkg, oracle):
+    """The reference's fixture reads: dereplicated, partitioned, counted."""
+    from oracle import learn
+    seqs, quals, reads = formats.raws_from_derep(formats.derep_fastq(GOLDEN / "sam1F.fastq.gz"))
+    err = synth.fixed_err(41)
+    params = oracle.make_params()
+    rs, center_of, weight = _partition(oracle, seqs, quals, reads, err, params)
+    want_tc = learn.transition_counts(rs, center_of, weight, params, True, 41)
+    want_bt = learn.build_trans_mat(seqs, quals, center_of, weight, params, 41)
+    ctx = pkg.Context(0)
+    try:
+        ctx.load_uniques(seqs, quals, reads, k=5)
+        got_tc = ctx.transition_counts(center_of, weight, ncol=41)
+        got_bt = ctx.build_trans_mat(center_of, weight, nq=41)
+    finally:
+        ctx.close()
+    print(f"fixture: {len(seqs)} uniques, {int(want_tc.sum())} counted bases", flush=True)
+    assert np.array_equal(got_tc, want_tc)
+    assert np.array_equal(got_bt, want_bt)
+
+
+def test_transitions_edges_and_errors(pkg, oracle):
+    from oracle import learn
+    w = synth.workload_partition(4, 1, 0, 120, 60, seed=synth.SEED_BASE + 56)
+    n = w.n
+    params = oracle.make_params()
+    ctx = pkg.Context(0)
+    try:
+        with pytest.raises(pkg.D2GError):  # nothing loaded
+            ctx.n = n
+            ctx.transition_counts(np.zeros(n, np.int64), np.ones(n))
+        # no qualities: one column, and build_trans_mat counts nothing
+        ctx.load_uniques(w.seqs, None, w.reads, k=5)
+        center_of = np.zeros(n, np.int64)
+        weight = np.asarray(w.reads, np.int64)
+        rs0 = oracle.RawSet(w.seqs, None, w.reads, k=5)
+        got = ctx.transition_counts(center_of, weight)
+        assert got.shape == (16, 1)
+        assert np.array_equal(got, learn.transition_counts(rs0, center_of, weight, params, False, 41))
+        assert not ctx.build_trans_mat(center_of, weight).any()
+        ctx.load_uniques(w.seqs, w.quals, w.reads, k=5)
+        # nothing counted: every unique outside a cluster, or weight 0
+        assert not ctx.transition_counts(np.full(n, -1), weight).any()
+        assert not ctx.build_trans_mat(center_of, np.zeros(n)).any()
+        # use_quals off: one column even with qualities loaded
+        one = ctx.transition_counts(center_of, weight, use_quals=False)
+        assert one.shape == (16, 1) and one.sum() == ctx.transition_counts(center_of, weight).sum()
+        bad = center_of.copy()
+        bad[3] = n
+        with pytest.raises(pkg.D2GError):
+            ctx.transition_counts(bad, weight)
+        with pytest.raises(ValueError):
+            ctx.transition_counts(center_of[:-1], weight)
+        # the plain compare still works after a transition pass
+        lam = ctx.compare_center(0, w.err).lam
+        lam2, _, _, _, _ = oracle.compare_center(oracle.RawSet(w.seqs, w.quals, w.reads, k=5), 0,
+                                                 w.err, 41, params)
+        assert np.array_equal(lam, lam2)
+    finally:
+        ctx.close()
