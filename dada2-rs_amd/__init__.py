@@ -136,6 +136,11 @@ def _lib():
             "d2g_build_trans_mat": (C.c_int, [vp, u32p, u32p, C.POINTER(AlignParams), C.c_uint32,
                                               u32p]),
             "d2g_transition_stats": (C.c_int, [vp, C.POINTER(Stats)]),
+            "d2g_dada_aux": (C.c_int, [vp, C.c_uint32, u32p, u32p, u32p, C.POINTER(AlignParams),
+                                       C.c_int, C.c_uint32, C.c_uint32, u32p, u32p, f64p]),
+            "d2g_birth_subs": (C.c_int, [vp, C.c_uint32, u32p, u32p, C.POINTER(AlignParams),
+                                         C.c_int, C.c_uint32, u32p, f64p, u32p, u32p, u8p, u8p,
+                                         u8p, u32p]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -155,7 +160,8 @@ def exported_symbols() -> list[str]:
             "d2g_pairs_fetch", "d2g_bimera_pairs", "d2g_is_bimera_batch", "d2g_is_bimera_pool", "d2g_table_bimera2",
             "d2g_loglik_forward", "d2g_loglik_backward", "d2g_ctx_set_graphs",
             "d2g_ctx_set_screen_shape", "d2g_ctx_set_autotune", "d2g_run_dada",
-            "d2g_transition_counts", "d2g_build_trans_mat", "d2g_transition_stats"]
+            "d2g_transition_counts", "d2g_build_trans_mat", "d2g_transition_stats",
+            "d2g_dada_aux", "d2g_birth_subs"]
 
 
 def _p(a: np.ndarray, t):
@@ -232,7 +238,8 @@ class ClusterInfo(C.Structure):
                 ("birth_type", C.c_int32), ("birth_from", C.c_uint32),
                 ("birth_hamming", C.c_uint32), ("birth_pval", C.c_double),
                 ("birth_fold", C.c_double), ("birth_e", C.c_double),
-                ("birth_lambda", C.c_double), ("self_lambda", C.c_double)]
+                ("birth_lambda", C.c_double), ("self_lambda", C.c_double),
+                ("posthoc_pval", C.c_double)]
 
 
 class DadaStats(C.Structure):
@@ -337,6 +344,8 @@ class Context:
                "d2g_load_uniques")
         self.n = len(seqs)
         self.has_quals = q is not None
+        self.reads = r
+        self.maxlen = int(np.diff(off).max())
 
     def _common(self, err, params, lock):
         e = np.ascontiguousarray(err, np.float64)
@@ -452,6 +461,97 @@ class Context:
         _check(_lib().d2g_build_trans_mat(self.h, _p(co, u32p), _p(w, u32p), C.byref(params),
                                           nq, _p(out, u32p)), "d2g_build_trans_mat")
         return out.reshape(16, nq)
+
+    def dada_aux(self, result: "DadaResult", params: AlignParams | None = None,
+                 use_quals: bool = True, ncol: int = 41, maxlen: int | None = None) -> dict:
+        """compute_aux's per-raw part (dada.rs:610-631) for a run_dada
+        result over the loaded uniques: the transitions, each raw's Sub.nsubs
+        against its center (-1 when not counted) and cluster_quality
+        (clusters x maxlen, NaN where uncovered; maxlen defaults to the
+        longest unique, as the reference's)."""
+        params = params or align_params()
+        n = self.n
+        ncl = len(result.clusters)
+        centers = np.ascontiguousarray([c["center"] for c in result.clusters], np.uint32)
+        cof = np.ascontiguousarray(result.cluster_of, np.uint32)
+        reads = np.ascontiguousarray(self.reads, np.uint32)
+        w = np.ascontiguousarray(np.where(np.asarray(result.map) >= 0, reads, 0), np.uint32)
+        maxlen = int(self.maxlen if maxlen is None else maxlen)
+        nout = ncol if (use_quals and self.has_quals) else 1
+        trans = np.zeros(16 * nout, np.uint32)
+        nsubs = np.zeros(n, np.uint32)
+        qual = np.zeros(ncl * maxlen, np.float64)
+        _check(_lib().d2g_dada_aux(self.h, ncl, _p(centers, u32p), _p(cof, u32p), _p(w, u32p),
+                                   C.byref(params), int(use_quals), ncol, maxlen,
+                                   _p(trans, u32p), _p(nsubs, u32p), _p(qual, f64p)),
+               "d2g_dada_aux")
+        return {"transitions": trans.reshape(16, nout),
+                "nsubs": np.where(nsubs == NO_CENTER, -1, nsubs.astype(np.int64)),
+                "cluster_quality": qual.reshape(ncl, maxlen)}
+
+    def birth_subs(self, result: "DadaResult", params: AlignParams | None = None,
+                   use_quals: bool = True):
+        """compute_birth_subs (dada.rs:679-693) for a run_dada result:
+        (records [(cluster, pos, nt0, nt1, qual or None)], nsubs per cluster
+        (-1 without a Sub), birth_qave per cluster (NaN when None))."""
+        params = params or align_params()
+        ncl = len(result.clusters)
+        centers = np.ascontiguousarray([c["center"] for c in result.clusters], np.uint32)
+        bfrom = np.ascontiguousarray([c["birth_from"] for c in result.clusters], np.uint32)
+        nsubs = np.zeros(ncl, np.uint32)
+        qave = np.zeros(ncl, np.float64)
+        nrec = C.c_uint32()
+        cap = 0
+        while True:
+            k = max(cap, 1)
+            cl, pos = np.zeros(k, np.uint32), np.zeros(k, np.uint32)
+            nt0, nt1, q = np.zeros(k, np.uint8), np.zeros(k, np.uint8), np.zeros(k, np.uint8)
+            rc = _lib().d2g_birth_subs(self.h, ncl, _p(centers, u32p), _p(bfrom, u32p),
+                                       C.byref(params), int(use_quals), cap, _p(nsubs, u32p),
+                                       _p(qave, f64p), _p(cl, u32p), _p(pos, u32p), _p(nt0, u8p),
+                                       _p(nt1, u8p), _p(q, u8p), C.byref(nrec))
+            if rc == -1 and nrec.value > cap:  # more records than room: size and rerun
+                cap = nrec.value
+                continue
+            _check(rc, "d2g_birth_subs")
+            break
+        quals = use_quals and self.has_quals
+        recs = [(int(cl[i]), int(pos[i]), int(nt0[i]), int(nt1[i]), int(q[i]) if quals else None)
+                for i in range(nrec.value)]
+        return recs, np.where(nsubs == NO_CENTER, -1, nsubs.astype(np.int64)), qave
+
+    def compute_aux(self, result: "DadaResult", params: AlignParams | None = None,
+                    use_quals: bool = True, ncol: int = 41) -> dict:
+        """compute_aux (dada.rs:610-640) for a run_dada result: DadaAux's
+        cluster_stats (error.rs:55-140), cluster_quality, birth_subs records
+        and transitions."""
+        params = params or align_params()
+        a = self.dada_aux(result, params, use_quals, ncol)
+        recs, _bn, qave = self.birth_subs(result, params, use_quals)
+        reads = np.asarray(self.reads, np.int64)
+        correct = np.asarray(result.map) >= 0
+        cof = np.asarray(result.cluster_of, np.int64)
+        stats = []
+        for k, c in enumerate(result.clusters):
+            m = correct & (cof == k)
+            ns = a["nsubs"]
+            stats.append({
+                "center": int(c["center"]),
+                "abundance": int(reads[m].sum()) & 0xFFFFFFFF,
+                "n0": int(reads[m & (ns == 0)].sum()) & 0xFFFFFFFF,
+                "n1": int(reads[m & (ns == 1)].sum()) & 0xFFFFFFFF,
+                "nunq": int(m.sum()),
+                "pval": c["posthoc_pval"],
+                "birth_from": None if k == 0 else int(c["birth_from"]),
+                "birth_pval": 0.0 if k == 0 else c["birth_pval"],
+                "birth_fold": 0.0 if k == 0 else c["birth_fold"],
+                "birth_ham": 0 if k == 0 else int(c["birth_hamming"]),
+                "birth_e": 0.0 if k == 0 else c["birth_e"],
+                "birth_qave": None if k == 0 or qave[k] != qave[k] else float(qave[k]),
+            })
+        return {"cluster_stats": stats, "cluster_quality": a["cluster_quality"],
+                "birth_subs": recs, "transitions": a["transitions"],
+                "transitions_ncol": a["transitions"].shape[1], "nsubs": a["nsubs"]}
 
     def transition_stats(self) -> dict:
         """Counters (and, with timing on, kernel times) of the last

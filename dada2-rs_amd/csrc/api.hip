@@ -11,6 +11,7 @@
 #include <cstring>
 #include <map>
 #include <numeric>
+#include <limits>
 #include <vector>
 
 #include "../../include/dada2_amd.h"
@@ -135,7 +136,11 @@ struct d2g_ctx {
   bool ll_have_w = false;
   DevBuf<double> p_lambda, p_pval, p_err;
   // transition counts: per-unique filter and weights, the 16 x ncol matrix
-  DevBuf<uint32_t> tr_want, tr_weight;
+  // host copies of the packed uniques (sorted order, 16-byte rows)
+  std::vector<uint8_t> h_seq, h_qual;
+  std::vector<uint64_t> h_soff;
+  DevBuf<uint32_t> tr_want, tr_weight, tr_order, tr_cluster, tr_nsubs, tr_qreads;
+  DevBuf<unsigned long long> tr_qsum;
   DevBuf<unsigned long long> tr_mat;
   d2g_stats tr_stats{};  // the last transition pass's compare
   // align mode buffers
@@ -412,6 +417,9 @@ int d2g_load_uniques(d2g_ctx* c, uint32_t n, const uint8_t* seq_concat, const ui
   HIP_TRY(launch_prep(u, s));
   HIP_TRY(hipStreamSynchronize(s));
   c->h_len = std::move(slen);
+  c->h_seq = std::move(sseq);
+  c->h_qual = std::move(squal);
+  c->h_soff = std::move(soff);
   c->h_orig = std::move(order);
   c->h_inv = std::move(inv);
   c->max_qual = maxq;
@@ -548,6 +556,16 @@ struct TransMode {
   uint32_t ncol;
   int clamp;
   unsigned long long* mat;
+  bool count = true;  // the transition kernel (off: alignments only)
+  // compute_aux's per-raw sums (optional): order / cluster per sorted unique
+  bool aux = false;
+  const uint32_t* order = nullptr;
+  uint32_t n_order = 0;
+  const uint32_t* cluster = nullptr;
+  uint32_t maxlen = 0;
+  uint32_t* nsubs = nullptr;
+  uint32_t* qual_reads = nullptr;
+  unsigned long long* qual_sum = nullptr;
 };
 
 // The compare pipeline; with fetch, the results are copied to the host
@@ -570,7 +588,8 @@ static int compare_device(d2g_ctx* c, const uint32_t* centers, uint32_t nc, cons
   for (uint32_t i = 0; i < nc; ++i)
     if (centers[i] >= n) return D2G_ERR_ARG;
   const size_t npairs = (size_t)nc * n;
-  if (npairs > 0xFFFFFFFFull) return D2G_ERR_UNSUPPORTED;
+  // (the pairing passes form at most one pair per unique)
+  if (!tm && npairs > 0xFFFFFFFFull) return D2G_ERR_UNSUPPORTED;
   BinSpace bs;
   bs.lmin = (int)u.minlen;
   bs.range = (int)(u.maxlen - u.minlen + 1);
@@ -758,7 +777,28 @@ static int compare_device(d2g_ctx* c, const uint32_t* centers, uint32_t nc, cons
       // eight waves per block, one pair per wave at a time; about 16 waves
       // per SIMD keep the loads of the chunks in flight
       ta.grid = (uint32_t)std::min<size_t>((size_t)c->cu_count * 8, (n + 7) / 8);
-      HIP_TRY(launch_transition(ta, s));
+      if (tm->count) HIP_TRY(launch_transition(ta, s));
+      if (tm->aux) {
+        AuxArgs xa;
+        xa.u = &u;
+        xa.al0 = tm->al0;
+        xa.al1 = tm->al1;
+        xa.al_off = tm->al_off;
+        xa.al_len = tm->al_len;
+        xa.order = tm->order;
+        xa.n_order = tm->n_order;
+        xa.cluster = tm->cluster;
+        xa.want = tm->want;
+        xa.weight = tm->weight;
+        xa.gapless = p->band == 0;
+        xa.use_quals = tm->use_quals;
+        xa.maxlen = tm->maxlen;
+        xa.nsubs = tm->nsubs;
+        xa.qual_reads = tm->qual_reads;
+        xa.qual_sum = tm->qual_sum;
+        xa.grid = (uint32_t)c->cu_count * 8;
+        HIP_TRY(launch_aux(xa, s));
+      }
       if (timed) HIP_TRY(hipEventRecord(c->ev[4], s));
       return D2G_OK;
     }
@@ -960,9 +1000,20 @@ int d2g_compare_center(d2g_ctx* c, uint32_t center, const double* err, uint32_t 
 // unique aligned to its center with *fp, counts per (transition, quality)
 // summed exactly on the device; ncol_out columns, qualities >= ncol_out
 // clamped (clamp) or skipped; u32 results saturated (saturate) or wrapped.
+// compute_aux's per-raw outputs besides the transitions (caller order):
+// Sub.nsubs per counted unique (D2G_NO_CENTER otherwise) and cluster_quality
+// (error.rs:251-305) as read-weighted means per (cluster, center position).
+struct AuxSpec {
+  uint32_t ncl;
+  const uint32_t* cluster_of;
+  uint32_t maxlen;
+  uint32_t* out_nsubs;
+  double* out_quality;
+};
+
 static int transitions_run(d2g_ctx* c, const uint32_t* center_of, const uint32_t* weight,
                            const d2g_align_params& fp, bool quals, uint32_t ncol_out, bool clamp,
-                           bool saturate, uint32_t* out_mat) {
+                           bool saturate, uint32_t* out_mat, const AuxSpec* aux = nullptr) {
   (void)hipSetDevice(c->device);
   const uint32_t n = c->u.n;
   // qualities are bytes: with skip, columns from 256 on never count; with
@@ -980,6 +1031,11 @@ static int transitions_run(d2g_ctx* c, const uint32_t* center_of, const uint32_t
   }
   std::fill(out_mat, out_mat + (size_t)16 * ncol_out, 0u);
   c->tr_stats = d2g_stats{};
+  if (aux) {
+    std::fill(aux->out_nsubs, aux->out_nsubs + n, D2G_NO_CENTER);
+    std::fill(aux->out_quality, aux->out_quality + (size_t)aux->ncl * aux->maxlen,
+              std::numeric_limits<double>::quiet_NaN());
+  }
   if (centers.empty()) return D2G_OK;
   // per sorted unique: its center (sorted index) or none, and its weight
   std::vector<uint32_t> want(n), w(n);
@@ -1018,6 +1074,43 @@ static int transitions_run(d2g_ctx* c, const uint32_t* center_of, const uint32_t
   tm.ncol = ncol_dev;
   tm.clamp = clamp ? 1 : 0;
   tm.mat = c->tr_mat.p;
+  std::vector<uint32_t> order, cl_s;
+  if (aux) {
+    // the counted uniques grouped by cluster (a counting sort by cluster)
+    cl_s.assign(n, 0u);
+    std::vector<uint32_t> cnt(aux->ncl + 1, 0u);
+    for (uint32_t s = 0; s < n; ++s)
+      if (w[s]) {
+        cl_s[s] = aux->cluster_of[c->h_orig[s]];
+        ++cnt[cl_s[s] + 1];
+      }
+    for (uint32_t k = 0; k < aux->ncl; ++k) cnt[k + 1] += cnt[k];
+    order.assign(cnt[aux->ncl], 0u);
+    for (uint32_t s = 0; s < n; ++s)
+      if (w[s]) order[cnt[cl_s[s]]++] = s;
+    const size_t cells = (size_t)aux->ncl * aux->maxlen;
+    HIP_TRY(c->tr_order.ensure(std::max<size_t>(order.size(), 1)));
+    HIP_TRY(c->tr_cluster.ensure(n));
+    HIP_TRY(c->tr_nsubs.ensure(n));
+    HIP_TRY(c->tr_qreads.ensure(std::max<size_t>(cells, 1)));
+    HIP_TRY(c->tr_qsum.ensure(std::max<size_t>(cells, 1)));
+    if (!order.empty())
+      HIP_TRY(hipMemcpyAsync(c->tr_order.p, order.data(), order.size() * sizeof(uint32_t),
+                             hipMemcpyHostToDevice, st));
+    HIP_TRY(hipMemcpyAsync(c->tr_cluster.p, cl_s.data(), n * sizeof(uint32_t),
+                           hipMemcpyHostToDevice, st));
+    HIP_TRY(launch_zero(c->tr_qreads.p, std::max<size_t>(cells, 1), st));
+    HIP_TRY(launch_zero(reinterpret_cast<uint32_t*>(c->tr_qsum.p), 2 * std::max<size_t>(cells, 1), st));
+    HIP_TRY(hipStreamSynchronize(st));  // pageable host vectors
+    tm.aux = true;
+    tm.order = c->tr_order.p;
+    tm.n_order = (uint32_t)order.size();
+    tm.cluster = c->tr_cluster.p;
+    tm.maxlen = aux->maxlen;
+    tm.nsubs = c->tr_nsubs.p;
+    tm.qual_reads = c->tr_qreads.p;
+    tm.qual_sum = c->tr_qsum.p;
+  }
   // the lambda stage does not run: a zero one-column error matrix stands in
   const double zero_err[16] = {0};
   HIP_TRY_INT(compare_device(c, centers.data(), (uint32_t)centers.size(), zero_err, 1, &fp, 0, 0,
@@ -1030,6 +1123,23 @@ static int transitions_run(d2g_ctx* c, const uint32_t* center_of, const uint32_t
       out_mat[(size_t)t * ncol_out + q] =
           saturate ? (uint32_t)std::min<unsigned long long>(v, 0xFFFFFFFFull) : (uint32_t)v;
     }
+  if (aux) {
+    std::vector<uint32_t> ns(n);
+    HIP_TRY(hipMemcpy(ns.data(), c->tr_nsubs.p, n * sizeof(uint32_t), hipMemcpyDeviceToHost));
+    for (uint32_t k : order) aux->out_nsubs[c->h_orig[k]] = ns[k];
+    if (quals) {
+      // qual_sum / nreads per position with reads (error.rs:296-302); the
+      // f64 sum of the integer terms is exact, so its order does not matter
+      const size_t cells = (size_t)aux->ncl * aux->maxlen;
+      std::vector<uint32_t> qr(cells);
+      std::vector<unsigned long long> qsum(cells);
+      HIP_TRY(hipMemcpy(qr.data(), c->tr_qreads.p, cells * sizeof(uint32_t), hipMemcpyDeviceToHost));
+      HIP_TRY(hipMemcpy(qsum.data(), c->tr_qsum.p, cells * sizeof(unsigned long long),
+                        hipMemcpyDeviceToHost));
+      for (size_t i = 0; i < cells; ++i)
+        if (qr[i] > 0) aux->out_quality[i] = (double)qsum[i] / (double)qr[i];
+    }
+  }
   return D2G_OK;
 }
 
@@ -1047,6 +1157,176 @@ int d2g_transition_counts(d2g_ctx* c, const uint32_t* center_of, const uint32_t*
   fp.kdist_cutoff = 1.0;
   return transitions_run(c, center_of, weight, fp, quals, quals ? ncol : 1u, false, false,
                          out_mat);
+}
+
+// compute_aux's per-raw part (dada.rs:610-631): one final-subs pass over a
+// finished partition giving the transitions, each raw's Sub.nsubs and
+// cluster_quality.
+int d2g_dada_aux(d2g_ctx* c, uint32_t ncl, const uint32_t* centers, const uint32_t* cluster_of,
+                 const uint32_t* weight, const d2g_align_params* p, int use_quals, uint32_t ncol,
+                 uint32_t maxlen, uint32_t* out_trans, uint32_t* out_nsubs, double* out_quality) {
+  if (!c || !centers || !cluster_of || !weight || !p || !out_trans || !out_nsubs || !out_quality ||
+      ncol == 0 || ncl == 0 || maxlen == 0)
+    return D2G_ERR_ARG;
+  if (!c->loaded) return D2G_ERR_STATE;
+  const uint32_t n = c->u.n;
+  std::vector<uint32_t> center_of(n, D2G_NO_CENTER);
+  for (uint32_t k = 0; k < ncl; ++k)
+    if (centers[k] != D2G_NO_CENTER && centers[k] >= n) return D2G_ERR_ARG;
+  for (uint32_t i = 0; i < n; ++i) {
+    if (cluster_of[i] == D2G_NO_CENTER) continue;
+    if (cluster_of[i] >= ncl) return D2G_ERR_ARG;
+    center_of[i] = centers[cluster_of[i]];
+  }
+  const bool quals = use_quals && c->has_qual;
+  d2g_align_params fp = *p;  // dada.rs:612-616
+  fp.use_kmers = 0;
+  fp.kdist_cutoff = 1.0;
+  AuxSpec aux{ncl, cluster_of, maxlen, out_nsubs, out_quality};
+  return transitions_run(c, center_of.data(), weight, fp, quals, quals ? ncol : 1u, false, false,
+                         out_trans, &aux);
+}
+
+// compute_birth_subs (dada.rs:679-693) and the Sub-derived outputs of
+// birth_sub_records (error.rs:328-352) and cluster_stats' birth_qave
+// (error.rs:90-100).  Each child center aligned to its parent's center on
+// the device (one task per child unique, alignment strings in a slot per
+// cluster), the Subs read from the alignments on the host (al2subs,
+// nwalign.rs:1169-1224).
+int d2g_birth_subs(d2g_ctx* c, uint32_t ncl, const uint32_t* centers, const uint32_t* birth_from,
+                   const d2g_align_params* p, int use_quals, uint32_t capacity, uint32_t* out_nsubs,
+                   double* out_qave, uint32_t* out_cluster, uint32_t* out_pos, uint8_t* out_nt0,
+                   uint8_t* out_nt1, uint8_t* out_qual, uint32_t* out_nrec) {
+  if (!c || !centers || !birth_from || !p || !out_nsubs || !out_qave || !out_nrec || ncl == 0)
+    return D2G_ERR_ARG;
+  if (capacity && (!out_cluster || !out_pos || !out_nt0 || !out_nt1 || !out_qual)) return D2G_ERR_ARG;
+  if (!c->loaded) return D2G_ERR_STATE;
+  (void)hipSetDevice(c->device);
+  const uint32_t n = c->u.n;
+  const bool quals = use_quals && c->has_qual;
+  std::fill(out_nsubs, out_nsubs + ncl, D2G_NO_CENTER);
+  std::fill(out_qave, out_qave + ncl, std::numeric_limits<double>::quiet_NaN());
+  *out_nrec = 0;
+  // the child unique of each birth pair (sorted index) and its cluster
+  std::vector<uint32_t> want(n, D2G_NO_CENTER), one(n, 0u), slot_of(n, D2G_NO_CENTER);
+  std::vector<uint32_t> child_of(ncl, D2G_NO_CENTER);
+  uint32_t npair = 0;
+  for (uint32_t ci = 0; ci < ncl; ++ci) {
+    if (centers[ci] != D2G_NO_CENTER && centers[ci] >= n) return D2G_ERR_ARG;
+    if (ci == 0) continue;  // cluster 0 has no parent (dada.rs:684)
+    if (birth_from[ci] >= ncl) return D2G_ERR_ARG;
+    const uint32_t pc = centers[birth_from[ci]], cc = centers[ci];
+    if (pc == D2G_NO_CENTER || cc == D2G_NO_CENTER) continue;
+    const uint32_t s = c->h_inv[cc];
+    if (want[s] != D2G_NO_CENTER) return D2G_ERR_ARG;  // one cluster per center
+    want[s] = c->h_inv[pc];
+    one[s] = 1;
+    slot_of[s] = npair++;
+    child_of[ci] = s;
+  }
+  if (npair == 0) return D2G_OK;
+  // birth alignments: the k-mer screen as given, cutoff 1.0 (dada.rs:617-621)
+  d2g_align_params bp = *p;
+  bp.kdist_cutoff = 1.0;
+  const uint64_t stride = 2ull * c->u.maxlen;
+  std::vector<uint64_t> aoff(n, 0);
+  for (uint32_t s = 0; s < n; ++s)
+    if (slot_of[s] != D2G_NO_CENTER) aoff[s] = slot_of[s] * stride;
+  hipStream_t st = c->stream;
+  HIP_TRY(c->tr_want.ensure(n));
+  HIP_TRY(c->tr_weight.ensure(n));
+  HIP_TRY(c->tr_mat.ensure(16));
+  HIP_TRY(c->a_al0.ensure(npair * stride));
+  HIP_TRY(c->a_al1.ensure(npair * stride));
+  HIP_TRY(c->a_aloff.ensure(n));
+  HIP_TRY(c->a_allen.ensure(n));
+  HIP_TRY(hipMemcpyAsync(c->tr_want.p, want.data(), n * sizeof(uint32_t), hipMemcpyHostToDevice, st));
+  HIP_TRY(hipMemcpyAsync(c->tr_weight.p, one.data(), n * sizeof(uint32_t), hipMemcpyHostToDevice, st));
+  HIP_TRY(hipMemcpyAsync(c->a_aloff.p, aoff.data(), n * sizeof(uint64_t), hipMemcpyHostToDevice, st));
+  HIP_TRY(hipStreamSynchronize(st));  // pageable host vectors
+  TransMode tm;
+  tm.want = c->tr_want.p;
+  tm.weight = c->tr_weight.p;
+  tm.al0 = c->a_al0.p;
+  tm.al1 = c->a_al1.p;
+  tm.al_off = c->a_aloff.p;
+  tm.al_len = c->a_allen.p;
+  tm.use_quals = 0;
+  tm.ncol = 1;
+  tm.clamp = 0;
+  tm.mat = c->tr_mat.p;
+  tm.count = false;
+  std::vector<uint32_t> parents;
+  for (uint32_t s = 0; s < n; ++s)
+    if (want[s] != D2G_NO_CENTER) parents.push_back(c->h_orig[want[s]]);
+  std::sort(parents.begin(), parents.end());
+  parents.erase(std::unique(parents.begin(), parents.end()), parents.end());
+  const double zero_err[16] = {0};
+  HIP_TRY_INT(compare_device(c, parents.data(), (uint32_t)parents.size(), zero_err, 1, &bp, 0, 0,
+                             nullptr, &c->tr_stats, false, nullptr, nullptr, nullptr, &tm));
+  // which pairs took the gapless alignment, and the DP alignments
+  Counters h;
+  HIP_TRY(hipMemcpy(&h, c->ctr.p, sizeof(Counters), hipMemcpyDeviceToHost));
+  std::vector<GaplessTask> glt(h.n_gapless);
+  if (h.n_gapless)
+    HIP_TRY(hipMemcpy(glt.data(), c->gl.p, h.n_gapless * sizeof(GaplessTask), hipMemcpyDeviceToHost));
+  std::vector<uint8_t> gl_flag(n, 0);
+  for (const GaplessTask& t : glt) gl_flag[t.u_idx] = 1;
+  std::vector<uint32_t> alen(n);
+  std::vector<uint8_t> a0(npair * stride), a1(npair * stride);
+  HIP_TRY(hipMemcpy(alen.data(), c->a_allen.p, n * sizeof(uint32_t), hipMemcpyDeviceToHost));
+  HIP_TRY(hipMemcpy(a0.data(), c->a_al0.p, a0.size(), hipMemcpyDeviceToHost));
+  HIP_TRY(hipMemcpy(a1.data(), c->a_al1.p, a1.size(), hipMemcpyDeviceToHost));
+  std::vector<uint8_t> g0, g1;
+  uint32_t nrec = 0;
+  for (uint32_t ci = 1; ci < ncl; ++ci) {
+    const uint32_t s = child_of[ci];
+    if (s == D2G_NO_CENTER) continue;
+    const uint32_t ps = want[s];
+    const uint8_t *x0, *x1;
+    uint32_t L;
+    if (gl_flag[s]) {  // align_gapless_with_buf (nwalign.rs:645-655)
+      const uint32_t l0 = c->h_len[ps], l1 = c->h_len[s];
+      L = std::max(l0, l1);
+      g0.assign(L, (uint8_t)'-');
+      g1.assign(L, (uint8_t)'-');
+      std::memcpy(g0.data(), &c->h_seq[c->h_soff[ps]], l0);
+      std::memcpy(g1.data(), &c->h_seq[c->h_soff[s]], l1);
+      x0 = g0.data();
+      x1 = g1.data();
+    } else {
+      x0 = &a0[slot_of[s] * stride];
+      x1 = &a1[slot_of[s] * stride];
+      L = alen[s];
+    }
+    const uint8_t* q1 = quals ? &c->h_qual[c->h_soff[s]] : nullptr;
+    static const uint8_t kDecode[6] = {'?', 'A', 'C', 'G', 'T', 'N'};  // nt_decode, misc.rs:360-370
+    uint32_t ns = 0;
+    double qsum = 0.0;
+    int64_t i0 = -1, i1 = -1;
+    for (uint32_t k = 0; k < L; ++k) {
+      const bool nt0 = x0[k] >= 1 && x0[k] <= 5, nt1 = x1[k] >= 1 && x1[k] <= 5;
+      i0 += nt0;
+      i1 += nt1;
+      if (nt0 && nt1 && x0[k] != x1[k] && x0[k] != 5 && x1[k] != 5) {
+        const uint8_t qv = q1 ? q1[i1] : 0;
+        if (nrec < capacity) {
+          out_cluster[nrec] = ci;
+          out_pos[nrec] = (uint32_t)i0;
+          out_nt0[nrec] = kDecode[x0[k]];
+          out_nt1[nrec] = kDecode[x1[k]];
+          out_qual[nrec] = qv;
+        }
+        ++nrec;
+        ++ns;
+        qsum += (double)qv;
+      }
+    }
+    out_nsubs[ci] = ns;
+    if (quals && ns > 0) out_qave[ci] = qsum / (double)ns;  // error.rs:90-100
+  }
+  *out_nrec = nrec;
+  return nrec > capacity ? D2G_ERR_ARG : D2G_OK;
 }
 
 int d2g_transition_stats(d2g_ctx* c, d2g_stats* out) {

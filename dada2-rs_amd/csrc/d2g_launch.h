@@ -71,12 +71,12 @@ uint32_t screen_blocks(uint32_t n_uniques, uint32_t n_centers, uint32_t qblock);
 constexpr uint32_t kScreenCenters = 16;  // centers per screen workgroup
 inline uint32_t screen_region(uint32_t qblock) { return qblock * kScreenCenters; }
 
-// The transition pass's pairing in place of the screen: unique q (sorted
-// index) with the one center want[q] (0xFFFFFFFF: none), tasks with
-// out_idx = q, no k-mer screen (use_kmers off, so gapless exactly when
-// band == 0) and no results.  Same outputs as the screen for the plan and
-// scatter: kPairBlock uniques per workgroup, each owning kPairBlock task
-// slots in each list.
+// The pairing of the final-subs and birth-subs passes in place of the
+// screen: unique q (sorted index) with the one center want[q] (0xFFFFFFFF:
+// none), tasks with out_idx = q and no results.  The pair's k-mer distances
+// (with use_kmers) decide the shroud and the gapless route as the screen's
+// do.  Same outputs as the screen for the plan and scatter: kPairBlock
+// uniques per workgroup, each owning kPairBlock task slots in each list.
 constexpr uint32_t kPairBlock = 256;
 inline uint32_t pair_blocks(uint32_t n_uniques) { return (n_uniques + kPairBlock - 1) / kPairBlock; }
 hipError_t launch_pair_screen(const ScreenArgs& a, const uint32_t* want, hipStream_t s);
@@ -185,6 +185,29 @@ struct TransArgs {
   uint32_t grid;
 };
 hipError_t launch_transition(const TransArgs& a, hipStream_t s);
+
+// compute_aux's per-raw sums over the same alignments: Sub.nsubs per
+// counted unique and cluster_quality's sums per (cluster, center position).
+struct AuxArgs {
+  const UniqueDev* u;
+  const uint8_t* al0;
+  const uint8_t* al1;
+  const uint64_t* al_off;
+  const uint32_t* al_len;
+  const uint32_t* order;    // counted sorted uniques, grouped by cluster
+  uint32_t n_order;
+  const uint32_t* cluster;  // per sorted unique
+  const uint32_t* want;     // per sorted unique: its center
+  const uint32_t* weight;   // per sorted unique
+  int gapless;              // band 0
+  int use_quals;
+  uint32_t maxlen;          // row length of qual_reads / qual_sum (>= every center length)
+  uint32_t* nsubs;          // per sorted unique
+  uint32_t* qual_reads;     // [clusters][maxlen], zeroed by the caller
+  unsigned long long* qual_sum;
+  uint32_t grid;
+};
+hipError_t launch_aux(const AuxArgs& a, hipStream_t s);
 
 // ---- bimera.hip --------------------------------------------------------
 // get_lr + get_ham_endsfree of n device-resident alignments (5 words each).

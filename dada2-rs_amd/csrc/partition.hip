@@ -644,6 +644,18 @@ extern "C" int d2g_run_dada(d2g_ctx* ctx, const double* err, uint32_t ncol,
       }
     }
   }
+  // post_hoc_pvals (error.rs:145-180): per center, the reads the other
+  // clusters' stored comparisons expect of it, summed over clusters in
+  // order and each cluster's comparisons in stored order
+  std::vector<int64_t> cluster_at(B.raws.size(), -1);
+  for (size_t ci = 0; ci < B.cl.size(); ++ci)
+    if (B.cl[ci].center >= 0) cluster_at[(size_t)B.cl[ci].center] = (int64_t)ci;
+  std::vector<double> tot_e(B.cl.size(), 0.0);
+  for (size_t ci = 0; ci < B.cl.size(); ++ci)
+    for (const Comp& cp : B.cl[ci].comp) {
+      const int64_t j = cluster_at[cp.index];
+      if (j >= 0 && (size_t)j != ci) tot_e[(size_t)j] += cp.lambda * (double)B.cl[ci].reads;
+    }
   *out_nclusters = (uint32_t)B.cl.size();
   if (B.cl.size() > cluster_capacity) return D2G_ERR_ARG;
   for (size_t ci = 0; ci < B.cl.size(); ++ci) {
@@ -660,6 +672,7 @@ extern "C" int d2g_run_dada(d2g_ctx* ctx, const double* err, uint32_t ncol,
     o.birth_e = C.birth_e;
     o.birth_lambda = C.birth_comp.lambda;
     o.self_lambda = C.self;
+    o.posthoc_pval = C.center >= 0 ? calc_pA(B.raws[(size_t)C.center].reads, tot_e[ci], true) : 1.0;
     for (uint32_t r : C.raws) {
       out_cluster_of[r] = (uint32_t)ci;
       out_map[r] = B.raws[r].correct ? (int32_t)ci : -1;

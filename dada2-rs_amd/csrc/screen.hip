@@ -346,11 +346,38 @@ __global__ __launch_bounds__(kPairBlock) void pair_kernel(ScreenDev a, const uin
   const UniqueDev& u = a.u;
   const uint32_t q = blockIdx.x * kPairBlock + threadIdx.x;
   const uint32_t ci = q < u.n ? want[q] : 0xFFFFFFFFu;
-  const bool live = ci < u.n;
-  // raw_align_dp (nwalign.rs:1324): kdist 0.0 and kodist -1.0 without the
-  // k-mer screen, so only band 0 takes the gapless alignment
-  const bool to_gl = live && a.band == 0;
-  const bool to_dp = live && a.band != 0;
+  bool live = ci < u.n;
+  // raw_align_with_buf (nwalign.rs:1244-1310): without the k-mer screen
+  // kdist is 0.0 and kodist -1.0, so only band 0 takes the gapless
+  // alignment (nwalign.rs:1324); with it, the pair's own distances decide
+  double kdist = 0.0, kodist = -1.0;
+  if (a.use_kmers && live) {
+    const uint32_t k = u.k, lq = u.len[q], lc = u.len[ci];
+    bool ovf = false;
+    uint32_t dot = 0;
+    for (uint32_t w = 0; w < u.kwords; ++w) {  // kmer_dist8, kmers.rs:254-269
+      const uint32_t x = u.kmer_il[(size_t)w * u.npad + q];
+      const uint32_t y = u.kmer_row[(size_t)ci * u.kwords + w];
+      for (int b = 0; b < 4; ++b) {
+        const uint32_t m = min((x >> (8 * b)) & 255u, (y >> (8 * b)) & 255u);
+        ovf |= m == 255;
+        dot += m;
+      }
+    }
+    if (ovf) dot = u16_min_dot(u, q, ci);  // nwalign.rs:1270-1278
+    const uint32_t mn = lq < lc ? lq : lc;
+    kdist = 1.0 - (double)dot / (double)(mn - k + 1);
+    if (a.gapless && lq == lc) {  // kord_dist, kmers.rs:314-328
+      const uint32_t klen = lq - (k - 1);
+      uint32_t eq = 0;
+      for (uint32_t s2 = 0; s2 < klen; ++s2) eq += kord_il_at(u, q, s2) == kord_row_at(u, ci, s2);
+      kodist = 1.0 - (double)eq / (double)klen;
+    }
+    live = !(kdist > a.kdist_cutoff);  // shrouded: no alignment
+  }
+  const bool gapless_route = a.band == 0 || (a.gapless && fabs(kodist - kdist) < DBL_EPSILON);
+  const bool to_gl = live && gapless_route;
+  const bool to_dp = live && !gapless_route;
   GaplessTask* gl_region = a.gapless_tasks + (size_t)blockIdx.x * kPairBlock;
   DpTask* dp_region = a.dp_unsorted + (size_t)blockIdx.x * kPairBlock;
   const uint32_t gpos = wave_append(&s_ngl, to_gl);
@@ -387,10 +414,13 @@ __global__ __launch_bounds__(kPairBlock) void pair_kernel(ScreenDev a, const uin
 }
 
 hipError_t launch_pair_screen(const ScreenArgs& s, const uint32_t* want, hipStream_t st) {
-  if (s.use_kmers || want == nullptr) return hipErrorInvalidValue;
+  if (want == nullptr) return hipErrorInvalidValue;
   ScreenDev d;
   std::memset(&d, 0, sizeof(d));
   d.u = *s.u;
+  d.use_kmers = s.use_kmers;
+  d.gapless = s.gapless;
+  d.kdist_cutoff = s.kdist_cutoff;
   d.band = s.band;
   d.bins = s.bins;
   d.gapless_tasks = s.gapless_tasks;

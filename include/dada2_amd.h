@@ -187,6 +187,9 @@ typedef struct {
   double birth_e;
   double birth_lambda;
   double self_lambda;    /* Bi.self_: the center's lambda against itself */
+  double posthoc_pval;   /* post_hoc_pvals (error.rs:145-180): calc_pA of the center's
+                          * reads against the expected reads the other clusters'
+                          * stored comparisons put on it (1.0 without a center) */
 } d2g_cluster_info;
 
 /* Counters and host-clock phase times of one run (the phase timers of
@@ -252,7 +255,48 @@ int d2g_transition_counts(d2g_ctx* ctx, const uint32_t* center_of, const uint32_
 int d2g_build_trans_mat(d2g_ctx* ctx, const uint32_t* center_of, const uint32_t* weight,
                         const d2g_align_params* align, uint32_t nq, uint32_t* out_mat);
 
-/* Counters of the last d2g_transition_counts / d2g_build_trans_mat call's
+/* Backs compute_aux's per-raw part (dada.rs:610-631): one pass of final
+ * Subs (compute_final_subs, dada.rs:646-673, aligned as d2g_transition_counts
+ * aligns) over a finished partition of ncl clusters:
+ *   centers[k]     the unique at cluster k's center (D2G_NO_CENTER: none)
+ *   cluster_of[i]  unique i's cluster (D2G_NO_CENTER: none)
+ *   weight[i]      Raw.reads when Raw.correct, else 0 (not counted)
+ * Outputs:
+ *   out_trans      transition_counts (error.rs:194-236), 16 * ncol_out u32
+ *                  as d2g_transition_counts
+ *   out_nsubs[i]   Sub.nsubs of each counted unique against its center (the
+ *                  n0 / n1 of cluster_stats, error.rs:55-140), else
+ *                  D2G_NO_CENTER
+ *   out_quality    cluster_quality (error.rs:251-305), ncl * maxlen f64: the
+ *                  read-weighted mean quality at each center position below
+ *                  min(center length, maxlen), NaN where no counted read
+ *                  covers it and everywhere without qualities. */
+int d2g_dada_aux(d2g_ctx* ctx, uint32_t ncl, const uint32_t* centers, const uint32_t* cluster_of,
+                 const uint32_t* weight, const d2g_align_params* align, int use_quals,
+                 uint32_t ncol, uint32_t maxlen, uint32_t* out_trans, uint32_t* out_nsubs,
+                 double* out_quality);
+
+/* Backs compute_birth_subs (dada.rs:679-693): each cluster k >= 1 whose
+ * center and parent center (centers[birth_from[k]]) exist has its center
+ * aligned to the parent's (the k-mer settings of *align, kdist_cutoff 1.0:
+ * dada.rs:617-621), and the Sub gives
+ *   out_nsubs[k]   its substitutions (D2G_NO_CENTER for cluster 0 and
+ *                  clusters without a Sub)
+ *   out_qave[k]    cluster_stats' birth_qave (error.rs:90-100): the mean
+ *                  query quality of the substitutions, NaN when there are
+ *                  none or no qualities are used
+ *   the records of birth_sub_records (error.rs:328-352), cluster by
+ *   cluster: out_cluster, out_pos (parent position), out_nt0 / out_nt1
+ *   (ASCII by nt_decode), out_qual (the child's quality; 0 without
+ *   qualities).
+ * *out_nrec receives the record count; when it exceeds capacity the call
+ * returns D2G_ERR_ARG and the records past capacity are not written. */
+int d2g_birth_subs(d2g_ctx* ctx, uint32_t ncl, const uint32_t* centers, const uint32_t* birth_from,
+                   const d2g_align_params* align, int use_quals, uint32_t capacity,
+                   uint32_t* out_nsubs, double* out_qave, uint32_t* out_cluster, uint32_t* out_pos,
+                   uint8_t* out_nt0, uint8_t* out_nt1, uint8_t* out_qual, uint32_t* out_nrec);
+
+/* Counters of the last d2g_transition_counts / d2g_build_trans_mat / d2g_dada_aux call's
  * alignments (comparisons = centers x uniques screened, gapless, aligned_dp,
  * dp_cells) and, with the context's timing on, its kernel times (ms_lambda
  * holds the counting kernel's).  No reference counterpart: diagnostics. */

@@ -105,3 +105,77 @@ def center_weight(out, reads):
     center_of = centers[np.asarray(out.cluster_of, np.int64)]
     weight = np.where(np.asarray(out.map) >= 0, np.asarray(reads, np.int64), 0)
     return center_of, weight
+
+
+def dada_aux(rs: RawSet, centers, cluster_of, weight, align: AlignParams, has_quals: bool,
+             ncol: int, maxlen: int):
+    """compute_aux's per-raw part (dada.rs:610-631) over a finished
+    partition: transition_counts, each counted raw's Sub.nsubs against its
+    center (-1 otherwise) and cluster_quality (error.rs:251-305: per cluster
+    and center position below min(center length, maxlen), the read-weighted
+    mean quality, NaN where no counted read covers it or without qualities;
+    the u32 read sums wrap)."""
+    ncl = len(centers)
+    center_of = np.asarray([centers[k] if k >= 0 else -1 for k in cluster_of], np.int64)
+    trans = transition_counts(rs, center_of, weight, align, has_quals, ncol)
+    fp = final_align(align)
+    buf = Buf()
+    nsubs = np.full(len(rs), -1, np.int64)
+    nreads = [[0] * maxlen for _ in range(ncl)]
+    qsum = [[0.0] * maxlen for _ in range(ncl)]
+    for i in range(len(rs)):
+        k, w = int(cluster_of[i]), int(weight[i])
+        if k < 0 or w == 0:
+            continue
+        c = int(centers[k])
+        s, _buf = sub_new(rs, c, i, fp, buf)
+        nsubs[i] = s.nsubs
+        if not has_quals or rs.quals is None:
+            continue
+        clen = min(len(rs.seqs[c]), maxlen)
+        for pos0 in range(clen):
+            pos1 = s.map[pos0]
+            if pos1 == GAP_GLYPH:
+                continue
+            nreads[k][pos0] += w
+            qsum[k][pos0] += float(rs.quals[i][pos1]) * float(w)
+    qual = np.full((ncl, maxlen), np.nan)
+    if has_quals and rs.quals is not None:
+        for k in range(ncl):
+            for p in range(maxlen):
+                r = nreads[k][p] % (1 << 32)
+                if r > 0:
+                    qual[k, p] = qsum[k][p] / float(r)
+    return {"transitions": trans, "nsubs": nsubs, "cluster_quality": qual}
+
+
+_DECODE = b"?ACGTN"  # nt_decode, misc.rs:360-370
+
+
+def birth_subs(rs: RawSet, centers, birth_from, align: AlignParams, has_quals: bool):
+    """compute_birth_subs (dada.rs:679-693) with birth_align (dada.rs:617-621:
+    the k-mer settings as given, kdist_cutoff 1.0), then birth_sub_records
+    (error.rs:328-352) and cluster_stats' birth_qave (error.rs:90-100).
+    Returns (records [(cluster, pos, nt0, nt1, qual or None)], nsubs per
+    cluster (-1 without a Sub), qave per cluster (NaN when None))."""
+    bp = AlignParams.from_buffer_copy(align)
+    bp.kdist_cutoff = 1.0
+    buf = Buf()
+    ncl = len(centers)
+    recs, nsubs, qave = [], [-1] * ncl, [float("nan")] * ncl
+    for ci in range(1, ncl):
+        pc, cc = centers[birth_from[ci]], centers[ci]
+        if pc is None or cc is None or pc < 0 or cc < 0:
+            continue
+        r = sub_new(rs, int(pc), int(cc), bp, buf)
+        if r is None:
+            continue
+        s, _buf = r
+        nsubs[ci] = s.nsubs
+        q1 = [int(s.q1[k]) for k in range(s.nsubs)] if (has_quals and rs.quals is not None) else []
+        for k in range(s.nsubs):
+            recs.append((ci, int(s.pos[k]), _DECODE[s.nt0[k]], _DECODE[s.nt1[k]],
+                         q1[k] if q1 else None))
+        if q1:
+            qave[ci] = sum(float(q) for q in q1) / s.nsubs
+    return recs, nsubs, qave

@@ -239,6 +239,21 @@ def shuffle_pass(B: Partition) -> int:
     return moves
 
 
+def post_hoc_pvals(B: Partition) -> list:
+    """post_hoc_pvals (error.rs:145-180): for each center, calc_pA of its
+    reads against the reads the other clusters' stored comparisons expect
+    of it (sum over clusters in order, comparisons in stored order)."""
+    cluster_at = {c.center: ci for ci, c in enumerate(B.cl) if c.center is not None}
+    tot = [0.0] * len(B.cl)
+    for ci, c in enumerate(B.cl):
+        for cp in c.comp:
+            j = cluster_at.get(cp.index)
+            if j is not None and j != ci:
+                tot[j] += cp.lam * float(c.reads)
+    return [calc_pA(B.reads_of[c.center], tot[ci], True) if c.center is not None else 1.0
+            for ci, c in enumerate(B.cl)]
+
+
 @dataclass
 class DadaOut:
     clusters: list
@@ -293,11 +308,13 @@ def run_dada(rs: RawSet, err: np.ndarray, align, prior=None, omega_a=1e-40, omeg
             pv[r] = p
             if ok:
                 mp[r] = ci
+    posthoc = post_hoc_pvals(B)
     clusters = [{"center": c.center, "reads": c.reads, "nraw": len(c.raws),
                  "birth_type": c.birth_type, "birth_from": c.birth_from,
                  "birth_hamming": c.birth_comp.hamming, "birth_pval": c.birth_pval,
                  "birth_fold": c.birth_fold, "birth_e": c.birth_e,
-                 "birth_lambda": c.birth_comp.lam, "self_lambda": c.self_lam} for c in B.cl]
+                 "birth_lambda": c.birth_comp.lam, "self_lambda": c.self_lam,
+                 "posthoc_pval": posthoc[ci]} for ci, c in enumerate(B.cl)]
     return DadaOut(clusters, cluster_of, pv, mp,
                    np.array([B.comp[r].lam for r in range(n)], np.float64),
                    np.array([B.comp[r].hamming for r in range(n)], np.uint32),

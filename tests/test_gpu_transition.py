@@ -149,3 +149,82 @@ def test_transitions_edges_and_errors(pkg, oracle):
         assert np.array_equal(lam, lam2)
     finally:
         ctx.close()
+
+
+AUX_CASES = [
+    # (name, workload args, align args, use quals)
+    ("vectorized", dict(n_true=8, n_near=3, L=150, n_uniques=400, seed_off=60), dict(band=16), True),
+    ("gapless_band0", dict(n_true=6, n_near=2, L=140, n_uniques=300, seed_off=61), dict(band=0),
+     True),
+    ("no_quals", dict(n_true=6, n_near=2, L=150, n_uniques=300, seed_off=62), dict(band=16), False),
+    ("long_homopolymer", dict(n_true=3, n_near=2, L=1450, n_uniques=40, seed_off=63),
+     dict(band=32, homo_gap_p=-1), True),
+]
+
+
+def _aux_both(pkg, oracle, seqs, quals, reads, err, akw, use_quals):
+    from oracle import learn
+    from oracle import partition as P
+    params = oracle.make_params(**akw)
+    rs = oracle.RawSet(seqs, quals if use_quals else None, reads, k=5)
+    out = P.run_dada(rs, err, params, use_quals=use_quals)
+    centers = [c["center"] for c in out.clusters]
+    bfrom = [c["birth_from"] for c in out.clusters]
+    weight = np.where(out.map >= 0, np.asarray(reads, np.int64), 0)
+    maxlen = max(len(s) for s in seqs)
+    want = learn.dada_aux(rs, centers, out.cluster_of.astype(np.int64), weight, params, use_quals,
+                          41, maxlen)
+    want_b = learn.birth_subs(rs, centers, bfrom, params, use_quals)
+    ctx = pkg.Context(0)
+    try:
+        ctx.load_uniques(seqs, quals if use_quals else None, reads, k=5)
+        ap = pkg.align_params(**akw)
+        res = ctx.run_dada(err, ap, pkg.dada_params(use_quals=use_quals))
+        got = ctx.compute_aux(res, ap, use_quals=use_quals, ncol=41)
+        got_b = ctx.birth_subs(res, ap, use_quals=use_quals)
+    finally:
+        ctx.close()
+    return out, res, want, want_b, got, got_b
+
+
+@pytest.mark.parametrize("name,wargs,akw,use_quals", AUX_CASES, ids=[c[0] for c in AUX_CASES])
+def test_dada_aux_matches_oracle(pkg, oracle, name, wargs, akw, use_quals):
+    wargs = dict(wargs)
+    seed = synth.SEED_BASE + wargs.pop("seed_off")
+    w = synth.workload_partition(seed=seed, **wargs)
+    seqs, quals = _with_indels(w.seqs, w.quals, np.random.default_rng(seed))
+    out, res, want, want_b, got, got_b = _aux_both(pkg, oracle, seqs, quals, w.reads, w.err, akw,
+                                                   use_quals)
+    print(f"case {name}: {len(seqs)} uniques, {len(out.clusters)} clusters, "
+          f"{len(want_b[0])} birth subs", flush=True)
+    assert [c["center"] for c in res.clusters] == [c["center"] for c in out.clusters]
+    assert np.array_equal(got["transitions"], want["transitions"])
+    assert np.array_equal(got["nsubs"], want["nsubs"])
+    gq, wq = got["cluster_quality"], want["cluster_quality"]
+    assert np.array_equal(np.isnan(gq), np.isnan(wq))
+    assert np.array_equal(gq[~np.isnan(gq)], wq[~np.isnan(wq)])  # bit-exact
+    recs, nsubs, qave = got_b
+    assert recs == want_b[0]
+    assert nsubs.tolist() == want_b[1]
+    assert np.array_equal(np.isnan(qave), np.isnan(want_b[2]))
+    assert np.array_equal(qave[~np.isnan(qave)], np.asarray(want_b[2])[~np.isnan(want_b[2])])
+    # cluster_stats from the same pieces
+    for k, st in enumerate(got["cluster_stats"]):
+        assert st["pval"] == out.clusters[k]["posthoc_pval"]
+        m = (out.map == k)
+        assert st["nunq"] == int(m.sum())
+        assert st["n0"] == int(np.asarray(w.reads)[m & (want["nsubs"] == 0)].sum())
+    assert len(out.clusters) > 1
+
+
+def test_dada_aux_fixture_reads(pkg, oracle):
+    seqs, quals, reads = formats.raws_from_derep(formats.derep_fastq(GOLDEN / "sam1F.fastq.gz"))
+    out, res, want, want_b, got, got_b = _aux_both(pkg, oracle, seqs, quals, reads,
+                                                   synth.fixed_err(41), {}, True)
+    print(f"fixture: {len(out.clusters)} clusters, {len(want_b[0])} birth subs", flush=True)
+    assert np.array_equal(got["transitions"], want["transitions"])
+    assert np.array_equal(got["nsubs"], want["nsubs"])
+    gq, wq = got["cluster_quality"], want["cluster_quality"]
+    assert np.array_equal(np.isnan(gq), np.isnan(wq))
+    assert np.array_equal(gq[~np.isnan(gq)], wq[~np.isnan(wq)])
+    assert got_b[0] == want_b[0] and got_b[1].tolist() == want_b[1]

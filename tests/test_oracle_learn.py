@@ -158,3 +158,74 @@ def test_gapless_unequal_lengths():
         np.add.at(want, (t, quals[i][:m].astype(np.int64)), w)
     assert np.array_equal(tc, want)
     assert tc.sum() == 2 * 43 + 3 * 50
+
+
+def test_dada_aux_hand_worked():
+    """cluster_quality and Sub.nsubs by hand: a center (5 reads, quality 20)
+    and a one-substitution member (2 reads, quality p at position p); the
+    read-weighted mean at p is (20 * 5 + p * 2) / 7."""
+    c = np.array([1, 2, 3, 4, 1, 2, 3, 4, 2, 2, 1, 3], np.uint8)
+    q = c.copy()
+    q[3] = 1
+    quals = [np.full(12, 20, np.uint8), np.arange(12, dtype=np.uint8)]
+    rs = oracle.RawSet([c, q], quals, [5, 2], k=5)
+    out = learn.dada_aux(rs, [0], [0, 0], [5, 2], oracle.make_params(band=16), True, 41, 14)
+    assert out["nsubs"].tolist() == [0, 1]
+    want = np.full((1, 14), np.nan)
+    want[0, :12] = (20 * 5 + np.arange(12) * 2) / 7
+    assert np.allclose(out["cluster_quality"], want, equal_nan=True, rtol=0, atol=1e-12)
+    # maxlen below the center length truncates the rows; a raw outside the
+    # read map (weight 0) adds nothing
+    out = learn.dada_aux(rs, [0], [0, 0], [5, 0], oracle.make_params(band=16), True, 41, 10)
+    assert np.allclose(out["cluster_quality"], np.full((1, 10), 20.0))
+    assert out["nsubs"].tolist() == [0, -1]
+    # without qualities: all NaN
+    rs0 = oracle.RawSet([c, q], None, [5, 2], k=5)
+    out = learn.dada_aux(rs0, [0], [0, 0], [5, 2], oracle.make_params(band=16), False, 41, 12)
+    assert np.isnan(out["cluster_quality"]).all() and out["transitions"].shape == (16, 1)
+
+
+def test_birth_subs_hand_worked():
+    """Cluster 1 = two substitutions of cluster 0's center (positions 10 and
+    40), cluster 2 = cluster 0's center with position 30 deleted, born from
+    cluster 1: its Sub against cluster 1's center has the same two
+    substitutions, the second one's query quality shifted by the deletion."""
+    rng = np.random.default_rng(synth.SEED_BASE + 45)
+    while True:
+        a = rng.integers(1, 5, size=60).astype(np.uint8)
+        if a[29] != a[30] and a[30] != a[31]:
+            break
+    b = a.copy()
+    b[10] = b[10] % 4 + 1
+    b[40] = b[40] % 4 + 1
+    c = np.delete(a, 30)
+    quals = [np.full(60, 30, np.uint8), np.arange(60, dtype=np.uint8),
+             np.arange(59, dtype=np.uint8)]
+    rs = oracle.RawSet([a, b, c], quals, [100, 20, 10], k=5)
+    recs, nsubs, qave = learn.birth_subs(rs, [0, 1, 2], [0, 0, 1], oracle.make_params(), True)
+    dec = b"?ACGT"
+    assert recs == [(1, 10, dec[a[10]], dec[b[10]], 10), (1, 40, dec[a[40]], dec[b[40]], 40),
+                    (2, 10, dec[b[10]], dec[a[10]], 10), (2, 40, dec[b[40]], dec[a[40]], 39)]
+    assert nsubs == [-1, 2, 2]
+    assert np.isnan(qave[0]) and qave[1] == 25.0 and qave[2] == 24.5
+    recs, _, qave = learn.birth_subs(rs, [0, 1, 2], [0, 0, 1], oracle.make_params(), False)
+    assert all(r[4] is None for r in recs) and all(x != x for x in qave)
+
+
+def test_post_hoc_pval_two_clusters():
+    """post_hoc_pvals with two clusters: cluster 0's compare stores every
+    raw, so cluster 1's expected reads are lambda(center 0 -> center 1) x
+    cluster 0's reads, from the (pinned) compare_center and calc_pA."""
+    from oracle import partition as P
+    w = synth.workload_partition(n_true=1, n_near=1, n_prior=0, L=150, n_uniques=60,
+                                 seed=synth.SEED_BASE + 46)
+    params = oracle.make_params(band=16)
+    rs = oracle.RawSet(w.seqs, w.quals, w.reads, k=5)
+    out = P.run_dada(rs, w.err, params)
+    assert len(out.clusters) == 2
+    c0, c1 = out.clusters[0]["center"], out.clusters[1]["center"]
+    lam0, _, _, _, _ = oracle.compare_center(rs, c0, w.err, 41, params)
+    want = oracle.calc_pA(int(w.reads[c1]), float(lam0[c1]) * float(out.clusters[0]["reads"]),
+                          True)
+    assert out.clusters[1]["posthoc_pval"] == want
+    assert 0.0 <= out.clusters[0]["posthoc_pval"] <= 1.0
