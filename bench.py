@@ -372,11 +372,24 @@ def bench_transitions(args, pkg):
     for _ in range(args.steps):
         tc = ctx.transition_counts(center_of, weight, ap, ncol=41)
     wall_tc = time.perf_counter() - t1
+    # compute_aux's device passes on the same partition
+    ctx.dada_aux(r, ap, ncol=41)
+    ctx.birth_subs(r, ap)
+    t2 = time.perf_counter()
+    for _ in range(args.steps):
+        aux = ctx.dada_aux(r, ap, ncol=41)
+    wall_aux = time.perf_counter() - t2
+    t3 = time.perf_counter()
+    for _ in range(args.steps):
+        recs, _, _ = ctx.birth_subs(r, ap)
+    wall_birth = time.perf_counter() - t3
     ctx.set_timing(True)
     st = []
     for _ in range(max(3, args.steps // 2)):
         ctx.build_trans_mat(center_of, weight, ap, nq=41)
         st.append(ctx.transition_stats())
+    ctx.dada_aux(r, ap, ncol=41)
+    st_aux = ctx.transition_stats()
     ctx.close()
     counted = int((weight > 0).sum())
     ms = wall / args.steps * 1e3
@@ -409,6 +422,11 @@ def bench_transitions(args, pkg):
                          "dp_cells": st[-1]["dp_cells"], "matrix_sum": int(mat.sum())},
                  transition_counts_ms=wall_tc / args.steps * 1e3,
                  transition_counts_equal=bool(np.array_equal(tc, mat)),
+                 compute_aux={"dada_aux_ms": wall_aux / args.steps * 1e3,
+                              "birth_subs_ms": wall_birth / args.steps * 1e3,
+                              "birth_records": len(recs),
+                              "aux_kernel_ms": st_aux["ms_lambda"],
+                              "covered_positions": int(np.isfinite(aux["cluster_quality"]).sum())},
                  kernel_ms={k: float(np.mean([x[k] for x in st]))
                             for k in ("ms_screen", "ms_plan", "ms_dp", "ms_lambda", "ms_total")},
                  roofline={"bound": "valu", "achieved": dp_tops, "peak": peak_ops, "unit": "TOP/s",

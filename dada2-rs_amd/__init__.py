@@ -501,23 +501,19 @@ class Context:
         nsubs = np.zeros(ncl, np.uint32)
         qave = np.zeros(ncl, np.float64)
         nrec = C.c_uint32()
-        cap = 0
-        while True:
-            k = max(cap, 1)
-            cl, pos = np.zeros(k, np.uint32), np.zeros(k, np.uint32)
-            nt0, nt1, q = np.zeros(k, np.uint8), np.zeros(k, np.uint8), np.zeros(k, np.uint8)
-            rc = _lib().d2g_birth_subs(self.h, ncl, _p(centers, u32p), _p(bfrom, u32p),
-                                       C.byref(params), int(use_quals), cap, _p(nsubs, u32p),
-                                       _p(qave, f64p), _p(cl, u32p), _p(pos, u32p), _p(nt0, u8p),
-                                       _p(nt1, u8p), _p(q, u8p), C.byref(nrec))
-            if rc == -1 and nrec.value > cap:  # more records than room: size and rerun
-                cap = nrec.value
-                continue
-            _check(rc, "d2g_birth_subs")
-            break
+        # a substitution sits at a parent position: at most the longest
+        # length per birth pair
+        cap = max(1, (ncl - 1) * int(self.maxlen))
+        cl, pos = np.zeros(cap, np.uint32), np.zeros(cap, np.uint32)
+        nt0, nt1, q = np.zeros(cap, np.uint8), np.zeros(cap, np.uint8), np.zeros(cap, np.uint8)
+        _check(_lib().d2g_birth_subs(self.h, ncl, _p(centers, u32p), _p(bfrom, u32p),
+                                     C.byref(params), int(use_quals), cap, _p(nsubs, u32p),
+                                     _p(qave, f64p), _p(cl, u32p), _p(pos, u32p), _p(nt0, u8p),
+                                     _p(nt1, u8p), _p(q, u8p), C.byref(nrec)), "d2g_birth_subs")
+        k = nrec.value
         quals = use_quals and self.has_quals
-        recs = [(int(cl[i]), int(pos[i]), int(nt0[i]), int(nt1[i]), int(q[i]) if quals else None)
-                for i in range(nrec.value)]
+        qs = q[:k].tolist() if quals else [None] * k
+        recs = list(zip(cl[:k].tolist(), pos[:k].tolist(), nt0[:k].tolist(), nt1[:k].tolist(), qs))
         return recs, np.where(nsubs == NO_CENTER, -1, nsubs.astype(np.int64)), qave
 
     def compute_aux(self, result: "DadaResult", params: AlignParams | None = None,

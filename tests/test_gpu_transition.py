@@ -228,3 +228,35 @@ def test_dada_aux_fixture_reads(pkg, oracle):
     assert np.array_equal(np.isnan(gq), np.isnan(wq))
     assert np.array_equal(gq[~np.isnan(gq)], wq[~np.isnan(wq)])
     assert got_b[0] == want_b[0] and got_b[1].tolist() == want_b[1]
+
+
+def test_birth_subs_capacity(pkg):
+    """More birth records than the capacity: D2G_ERR_ARG, the count set and
+    only `capacity` records written; the full call then returns them all."""
+    import ctypes as C
+    w = synth.workload_partition(n_true=5, n_near=2, L=150, n_uniques=200,
+                                 seed=synth.SEED_BASE + 64)
+    ctx = pkg.Context(0)
+    try:
+        ctx.load_uniques(w.seqs, w.quals, w.reads, k=5)
+        res = ctx.run_dada(w.err)
+        recs, _, _ = ctx.birth_subs(res)
+        assert len(recs) > 2
+        ncl = len(res.clusters)
+        centers = np.ascontiguousarray([c["center"] for c in res.clusters], np.uint32)
+        bfrom = np.ascontiguousarray([c["birth_from"] for c in res.clusters], np.uint32)
+        ns, qa = np.zeros(ncl, np.uint32), np.zeros(ncl, np.float64)
+        cl, pos = np.zeros(2, np.uint32), np.zeros(2, np.uint32)
+        nt0, nt1, q = np.zeros(2, np.uint8), np.zeros(2, np.uint8), np.zeros(2, np.uint8)
+        nrec = C.c_uint32()
+        u32p, u8p = C.POINTER(C.c_uint32), C.POINTER(C.c_uint8)
+        f64p = C.POINTER(C.c_double)
+        rc = pkg._lib().d2g_birth_subs(
+            ctx.h, ncl, centers.ctypes.data_as(u32p), bfrom.ctypes.data_as(u32p),
+            C.byref(pkg.align_params()), 1, 2, ns.ctypes.data_as(u32p), qa.ctypes.data_as(f64p),
+            cl.ctypes.data_as(u32p), pos.ctypes.data_as(u32p), nt0.ctypes.data_as(u8p),
+            nt1.ctypes.data_as(u8p), q.ctypes.data_as(u8p), C.byref(nrec))
+        assert rc == -1 and nrec.value == len(recs)
+        assert [(int(cl[i]), int(pos[i])) for i in range(2)] == [r[:2] for r in recs[:2]]
+    finally:
+        ctx.close()
