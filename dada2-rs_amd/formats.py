@@ -14,12 +14,25 @@ Restated from dada2-rs v0.2.0:
     their sequence;
   * nt_encode (misc.rs:346-356): A=1 C=2 G=3 T/U=4;
   * read_tagged_json (misc.rs:116-147) and the error-model fields nq,
-    err_in, err_out (main.rs:4361-4374), gzip by the .gz extension.
+    err_in, err_out (main.rs:4361-4374), gzip by the .gz extension;
+  * the tagged writer (Tagged, misc.rs:116-135: the command and version
+    first, then the payload's fields) and the dada output document
+    (DadaOutput, main.rs:1139-1250; AsvEntry and birth_type_str,
+    main.rs:4776-4817; non-finite floats as null, as serde_json writes them);
+  * make_sequence_table (sequence_table.rs:69-330): dada / dada-pooled /
+    dada-pseudo and merge-pairs documents to a samples x sequences count
+    table, columns by decreasing total abundance (or samples present, or
+    first-seen order), md5 or sha1 sequence ids.  The reference's first-seen
+    order comes from HashMap iteration, which is unspecified; here it is
+    the order sequences first appear in the documents, so ties in the sort
+    keep that order.
 """
 from __future__ import annotations
 
 import gzip
+import hashlib
 import json
+import math
 from dataclasses import dataclass
 from pathlib import Path
 
@@ -163,3 +176,165 @@ def read_error_model(path, use_err_in: bool = False) -> np.ndarray:
     if len(rows) != 16 or any(len(r) != nq for r in rows):
         raise ValueError(f"Error model matrix must be 16 x {nq}, got {len(rows)} rows")
     return np.asarray(rows, np.float64)
+
+
+# ---- writers ---------------------------------------------------------------
+
+FORMAT_VERSION = "0.2.0"  # the dada2-rs release whose formats these mirror
+BIRTH_TYPES = ("Initial", "Abundance", "Prior", "Singleton")  # main.rs:4792-4799
+_DEC = np.frombuffer(b"?ACGTN", np.uint8)  # nt_decode, misc.rs:360-370
+
+
+def nt_decode(enc) -> str:
+    return _DEC[np.asarray(enc, np.uint8)].tobytes().decode()
+
+
+def _finite(x):
+    """serde_json writes NaN and infinities as null."""
+    if isinstance(x, float) and not math.isfinite(x):
+        return None
+    if isinstance(x, list):
+        return [_finite(v) for v in x]
+    return x
+
+
+def tagged(command: str, doc: dict) -> dict:
+    """Tagged (misc.rs:116-135): the command and version, then the payload."""
+    out = {"dada2_rs_command": command, "dada2_rs_version": FORMAT_VERSION}
+    out.update(doc)
+    return out
+
+
+def write_tagged_json(path, command: str, doc: dict, compact: bool = False):
+    text = json.dumps(_finite_doc(tagged(command, doc)), indent=None if compact else 2,
+                      allow_nan=False)
+    path = Path(path)
+    data = text.encode()
+    if path.suffix == ".gz":
+        with gzip.open(path, "wb") as f:
+            f.write(data)
+    else:
+        path.write_bytes(data)
+
+
+def _finite_doc(x):
+    if isinstance(x, dict):
+        return {k: _finite_doc(v) for k, v in x.items()}
+    if isinstance(x, (list, tuple)):
+        return [_finite_doc(v) for v in x]
+    if isinstance(x, (np.floating, float)):
+        v = float(x)
+        return v if math.isfinite(v) else None
+    if isinstance(x, np.integer):
+        return int(x)
+    if isinstance(x, np.bool_):
+        return bool(x)
+    return x
+
+
+def dada_output(result, seqs, sample: str, input_file: str, run_params: dict,
+                aux: dict | None = None) -> dict:
+    """The dada command's payload (DadaOutput, main.rs:1139-1250) for a
+    run_dada result over encoded uniques `seqs`: ASVs in cluster order, the
+    read map, the run's counters and parameters, and the aux outputs when
+    given (Context.compute_aux's dict)."""
+    asvs = [{"sequence": nt_decode(seqs[c["center"]]), "abundance": int(c["reads"]),
+             "birth_type": BIRTH_TYPES[c["birth_type"]], "birth_pval": c["birth_pval"],
+             "birth_fold": c["birth_fold"], "birth_e": c["birth_e"]} for c in result.clusters]
+    total = sum(int(c["reads"]) for c in result.clusters) & 0xFFFFFFFF
+    doc = {"sample": sample, "input_file": input_file, "num_asvs": len(asvs),
+           "total_reads": total, "asvs": asvs,
+           "stats": {"nalign": int(result.stats["nalign"]), "nshroud": int(result.stats["nshroud"])},
+           "params": dict(run_params),
+           "map": [None if m < 0 else int(m) for m in np.asarray(result.map)]}
+    if aux is not None:
+        doc["aux"] = {
+            "cluster_stats": [dict({k: v for k, v in st.items() if k != "center"},
+                                   sequence=nt_decode(seqs[st["center"]]))
+                              for st in aux["cluster_stats"]],
+            "cluster_quality": np.asarray(aux["cluster_quality"]).tolist(),
+            "cluster_quality_maxlen": int(np.asarray(aux["cluster_quality"]).shape[1]),
+            "birth_subs": [{"cluster": c, "pos": p, "nt0": chr(a), "nt1": chr(b), "qual": q}
+                           for c, p, a, b, q in aux["birth_subs"]],
+            "transitions": np.asarray(aux["transitions"]).reshape(-1).tolist(),
+            "transitions_ncol": int(aux["transitions_ncol"]),
+        }
+    return doc
+
+
+# ---- the sequence table ----------------------------------------------------
+
+def _load(doc_or_path):
+    if isinstance(doc_or_path, dict):
+        return doc_or_path, None
+    with _open(doc_or_path) as f:
+        return json.loads(f.read().decode()), Path(doc_or_path)
+
+
+def _file_stem(path: Path) -> str:
+    return Path(path.name).stem if path is not None else "unknown"
+
+
+def parse_counts(doc_or_path, sample_name: str | None = None):
+    """parse_file (sequence_table.rs:69-134): (sample name, {sequence:
+    count}) per sample of one dada-like or merge-pairs document."""
+    doc, path = _load(doc_or_path)
+    cmd = doc.get("dada2_rs_command")
+    if cmd is None:
+        raise ValueError(f"{path}: missing dada2_rs_command tag (expected 'dada', 'dada-pooled', "
+                         "'dada-pseudo', or 'merge-pairs')")
+    if cmd in ("dada", "dada-pooled", "dada-pseudo"):
+        name = sample_name or doc.get("sample") or _file_stem(path)
+        counts = {}
+        for a in doc["asvs"]:  # a later duplicate replaces an earlier one
+            counts[a["sequence"]] = int(a["abundance"])
+        return [(name, counts)]
+    if cmd == "merge-pairs":
+        out = []
+        for r in doc["samples"]:
+            counts = {}
+            for m in r["merged"]:
+                if m["accept"] and m["sequence"]:
+                    counts[m["sequence"]] = int(m["abundance"])
+            out.append((r["sample"], counts))
+        return out
+    raise ValueError(f"{path}: dada2_rs_command={cmd!r}, expected 'dada', 'dada-pooled', or "
+                     "'merge-pairs'")
+
+
+def make_sequence_table(inputs, sample_names=None, order_by: str = "abundance",
+                        hash_algo: str = "md5") -> dict:
+    """make_sequence_table (sequence_table.rs:233-330) over documents or
+    paths: {samples, sequences, sequence_ids, counts[sample][sequence]}."""
+    samples = []
+    for i, x in enumerate(inputs):
+        samples.extend(parse_counts(x, sample_names[i] if sample_names else None))
+    index: dict[str, int] = {}
+    for _, counts in samples:
+        for seq in counts:
+            index.setdefault(seq, len(index))
+    seqs = list(index)
+    mat = np.zeros((len(samples), len(seqs)), np.uint64)
+    for i, (_, counts) in enumerate(samples):
+        for seq, c in counts.items():
+            mat[i, index[seq]] += np.uint64(c)
+    if order_by == "abundance":
+        key = mat.sum(axis=0)
+    elif order_by == "nsamples":
+        key = (mat > 0).sum(axis=0)
+    else:
+        key = None
+    order = (sorted(range(len(seqs)), key=lambda j: -int(key[j])) if key is not None
+             else list(range(len(seqs))))
+    seqs = [seqs[j] for j in order]
+    digest = hashlib.sha1 if hash_algo == "sha1" else hashlib.md5
+    return {"samples": [n for n, _ in samples], "sequences": seqs,
+            "sequence_ids": [digest(s.encode()).hexdigest() for s in seqs],
+            "counts": mat[:, order].tolist()}
+
+
+def read_sequence_table(path) -> dict:
+    """A make-sequence-table or remove-bimera-denovo document
+    (main.rs:2752)."""
+    doc = read_tagged_json(path, ["make-sequence-table", "remove-bimera-denovo"])
+    return {k: doc[k] for k in ("samples", "sequences", "sequence_ids", "counts")}
