@@ -279,7 +279,8 @@ def bench_gradient(args, pkg):
 def bench_dada(args, pkg):
     """The partition loop: run_dada over 50,000 uniques (250 nt; 300 planted
     variants, 60 near variants that bud, error copies), every cluster's
-    compare on the device.  A step is one whole run_dada."""
+    compare and the shuffle's best-cluster scan on the device.  A step is
+    one whole run_dada; the host-scan variant is timed beside it."""
     from dada2_rs_amd import synth
     w = synth.workload_partition(n_true=300, n_near=60, n_prior=0, L=250, n_uniques=50000,
                                  seed=synth.SEED_BASE + 20)
@@ -293,6 +294,15 @@ def bench_dada(args, pkg):
     for _ in range(args.steps):
         r = ctx.run_dada(w.err, ap, dp)
     wall = time.perf_counter() - t0
+    # the same runs with the shuffle's best-cluster scan on the host's threads
+    dpd = pkg.dada_params(device_shuffle=False)
+    rd = ctx.run_dada(w.err, ap, dpd)
+    t1 = time.perf_counter()
+    for _ in range(args.steps):
+        rd = ctx.run_dada(w.err, ap, dpd)
+    wall_dev = time.perf_counter() - t1
+    same = (np.array_equal(rd.cluster_of, r.cluster_of) and np.array_equal(rd.p, r.p)
+            and [c["center"] for c in rd.clusters] == [c["center"] for c in r.clusters])
     ctx.set_timing(True)
     rt = ctx.run_dada(w.err, ap, dp)
     ctx.close()
@@ -333,6 +343,10 @@ def bench_dada(args, pkg):
                          "dp_cells": st["dp_cells"]},
                  phase_ms={k: st[k] for k in ("ms_compare", "ms_store", "ms_shuffle", "ms_bud",
                                               "ms_pupdate", "ms_total")},
+                 host_shuffle={"ms_per_run": wall_dev / args.steps * 1e3,
+                                 "ms_shuffle": rd.stats["ms_shuffle"],
+                                 "ms_store": rd.stats["ms_store"],
+                                 "identical_partition": bool(same)},
                  kernel_ms={k: stt[k] for k in ("ms_dev_screen", "ms_dev_plan", "ms_dev_dp",
                                                 "ms_dev_lambda")},
                  roofline={"bound": "valu", "achieved": dp_tops, "peak": peak_ops,

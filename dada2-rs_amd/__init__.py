@@ -221,16 +221,19 @@ class DadaParams(C.Structure):
                 ("min_fold", C.c_double), ("min_hamming", C.c_uint32), ("min_abund", C.c_uint32),
                 ("max_clust", C.c_uint32), ("max_shuffle", C.c_uint32), ("greedy", C.c_int32),
                 ("detect_singletons", C.c_int32), ("use_quals", C.c_int32),
-                ("_reserved", C.c_int32)]
+                ("device_shuffle", C.c_int32)]
 
 
 def dada_params(omega_a=1e-40, omega_p=1e-4, omega_c=1e-40, min_fold=1.0, min_hamming=1,
                 min_abund=1, max_clust=0, max_shuffle=10, greedy=True, detect_singletons=False,
-                use_quals=True) -> DadaParams:
+                use_quals=True, device_shuffle=True) -> DadaParams:
     """Defaults of resolve_dada_params (main.rs:4393-4433); MAX_SHUFFLE = 10
-    (dada.rs:39)."""
+    (dada.rs:39).  device_shuffle runs the shuffle's best-cluster scan on
+    the device (the same moves in the same order; off: the host's
+    threads)."""
     return DadaParams(omega_a, omega_p, omega_c, min_fold, min_hamming, min_abund, max_clust,
-                      max_shuffle, int(greedy), int(detect_singletons), int(use_quals), 0)
+                      max_shuffle, int(greedy), int(detect_singletons), int(use_quals),
+                      int(device_shuffle))
 
 
 class ClusterInfo(C.Structure):

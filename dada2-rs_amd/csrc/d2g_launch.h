@@ -209,6 +209,32 @@ struct AuxArgs {
 };
 hipError_t launch_aux(const AuxArgs& a, hipStream_t s);
 
+// ---- shuffle.hip -------------------------------------------------------
+// The best-cluster scan of a shuffle pass over the stored comparisons, the
+// movers compacted into `moves`.  One stored comparison, and one move:
+struct ShuffleComp {
+  uint32_t raw, cl, ham, _pad;
+  double lambda;
+};
+struct ShuffleMove {
+  uint32_t r, to, ham, _pad;
+  double lambda;
+};
+struct ShuffleArgs {
+  const ShuffleComp* comps;  // appended per compare (cluster-major)
+  uint32_t n_comp;
+  const double* creads;      // per cluster: its reads
+  const uint32_t* ccenter;   // per cluster: its center raw
+  uint32_t n_cl;
+  uint32_t n_raw;
+  unsigned long long* best_bits;  // per raw, scratch (reset by each pass)
+  unsigned long long* best_key;   // cluster << 32 | comparison index
+  uint32_t* where;           // per raw: its cluster (centers set by the pass)
+  ShuffleMove* moves;        // slot 0 holds the count in `r`; movers from slot 1
+};
+hipError_t launch_shuffle_init(const ShuffleArgs& a, hipStream_t s);  // where = 0, scratch reset
+hipError_t launch_shuffle_pass(const ShuffleArgs& a, hipStream_t s);
+
 // ---- bimera.hip --------------------------------------------------------
 // get_lr + get_ham_endsfree of n device-resident alignments (5 words each).
 hipError_t launch_bimera_lr(uint32_t n, const uint8_t* al0, const uint8_t* al1,

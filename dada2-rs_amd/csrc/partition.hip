@@ -50,6 +50,7 @@
 #include <vector>
 
 #include "../../include/dada2_amd.h"
+#include "d2g_launch.h"
 
 namespace d2g {
 const std::vector<uint32_t>* ctx_host_reads(const d2g_ctx* c);
@@ -293,6 +294,137 @@ double get_pA(uint32_t reads, bool prior, double lambda, uint32_t hamming, uint3
 
 // ---- the loop's steps ------------------------------------------------------
 
+// The shuffle scan's state on the device for one run: the stored
+// comparisons (appended after each compare), each raw's cluster, and the
+// scan's scratch; pinned staging for the per-compare and per-pass copies
+// (one copy each way per pass, one per compare).
+struct DevShuffle {
+  hipStream_t s = nullptr;
+  uint32_t n = 0, n_comp = 0, cap = 0, ncl_cap = 0;
+  d2g::ShuffleComp* comps = nullptr;
+  uint8_t* cl_info = nullptr;  // creads (f64 x ncl_cap), then ccenter (u32)
+  unsigned long long *best_bits = nullptr, *best_key = nullptr;
+  uint32_t* where = nullptr;
+  d2g::ShuffleMove* moves = nullptr;  // slot 0: the count
+  d2g::ShuffleComp* comp_pin = nullptr;
+  uint8_t* pass_pin = nullptr;
+  d2g::ShuffleMove* moves_pin = nullptr;
+  size_t comp_pin_cap = 0;
+  static constexpr uint32_t kFirst = 1024;  // moves read back with the count
+
+  ~DevShuffle() {
+    if (s) (void)hipStreamSynchronize(s);
+    for (void* q : {(void*)comps, (void*)cl_info, (void*)best_bits, (void*)best_key,
+                    (void*)where, (void*)moves})
+      if (q) (void)hipFree(q);
+    for (void* q : {(void*)comp_pin, (void*)pass_pin, (void*)moves_pin})
+      if (q) (void)hipHostFree(q);
+  }
+
+  static bool ok(hipError_t e) { return e == hipSuccess; }
+
+  bool init(hipStream_t stream, uint32_t nraw) {
+    s = stream;
+    n = nraw;
+    if (!ok(hipMalloc((void**)&best_bits, (size_t)n * 8)) ||
+        !ok(hipMalloc((void**)&best_key, (size_t)n * 8)) ||
+        !ok(hipMalloc((void**)&where, (size_t)n * 4)) ||
+        !ok(hipMalloc((void**)&moves, (size_t)(n + 1) * sizeof(d2g::ShuffleMove))) ||
+        !ok(hipHostMalloc((void**)&moves_pin, (kFirst + 1) * sizeof(d2g::ShuffleMove),
+                          hipHostMallocDefault)))
+      return false;
+    return ok(d2g::launch_shuffle_init(args(0), s));
+  }
+
+  d2g::ShuffleArgs args(uint32_t ncl) const {
+    d2g::ShuffleArgs a;
+    a.comps = comps;
+    a.n_comp = n_comp;
+    a.creads = reinterpret_cast<const double*>(cl_info);
+    a.ccenter = cl_info ? reinterpret_cast<const uint32_t*>(cl_info + (size_t)ncl_cap * 8) : nullptr;
+    a.n_cl = ncl;
+    a.n_raw = n;
+    a.best_bits = best_bits;
+    a.best_key = best_key;
+    a.where = where;
+    a.moves = moves;
+    return a;
+  }
+
+  // Cluster `cl`'s stored comparisons.  The previous copy from the staging
+  // area has finished: every pass and compare ends with a synchronisation.
+  bool append(const std::vector<Comp>& cs, uint32_t cl) {
+    const uint32_t k = (uint32_t)cs.size();
+    if (k == 0) return true;
+    if (n_comp + k > cap) {
+      const uint32_t nc = std::max<uint32_t>(n_comp + k, 2 * cap + 4096);
+      d2g::ShuffleComp* q = nullptr;
+      if (!ok(hipMalloc((void**)&q, (size_t)nc * sizeof(d2g::ShuffleComp)))) return false;
+      if (n_comp && (!ok(hipMemcpyAsync(q, comps, (size_t)n_comp * sizeof(d2g::ShuffleComp),
+                                        hipMemcpyDeviceToDevice, s)) ||
+                     !ok(hipStreamSynchronize(s))))
+        return false;
+      if (comps) (void)hipFree(comps);
+      comps = q;
+      cap = nc;
+    }
+    if (comp_pin_cap < k) {
+      if (comp_pin) (void)hipHostFree(comp_pin);
+      comp_pin = nullptr;
+      if (!ok(hipHostMalloc((void**)&comp_pin, 2 * (size_t)k * sizeof(d2g::ShuffleComp),
+                            hipHostMallocDefault)))
+        return false;
+      comp_pin_cap = 2 * (size_t)k;
+    }
+    for (uint32_t j = 0; j < k; ++j)
+      comp_pin[j] = d2g::ShuffleComp{cs[j].index, cl, cs[j].hamming, 0u, cs[j].lambda};
+    if (!ok(hipMemcpyAsync(comps + n_comp, comp_pin, (size_t)k * sizeof(d2g::ShuffleComp),
+                           hipMemcpyHostToDevice, s)))
+      return false;
+    n_comp += k;
+    return true;
+  }
+
+  // One pass: the scan on the device, the moves read back (unordered).
+  bool pass(const Part& B, std::vector<d2g::ShuffleMove>& out) {
+    const uint32_t ncl = (uint32_t)B.cl.size();
+    if (ncl > ncl_cap) {
+      const uint32_t nc = std::max(ncl, 2 * ncl_cap + 64);
+      if (cl_info) (void)hipFree(cl_info);
+      if (pass_pin) (void)hipHostFree(pass_pin);
+      cl_info = nullptr;
+      pass_pin = nullptr;
+      if (!ok(hipMalloc((void**)&cl_info, (size_t)nc * 12)) ||
+          !ok(hipHostMalloc((void**)&pass_pin, (size_t)nc * 12, hipHostMallocDefault)))
+        return false;
+      ncl_cap = nc;
+    }
+    double* hr = reinterpret_cast<double*>(pass_pin);
+    uint32_t* hc = reinterpret_cast<uint32_t*>(pass_pin + (size_t)ncl_cap * 8);
+    for (uint32_t ci = 0; ci < ncl; ++ci) {
+      hr[ci] = (double)B.cl[ci].reads;
+      hc[ci] = B.cl[ci].center >= 0 ? (uint32_t)B.cl[ci].center : 0xFFFFFFFFu;
+    }
+    const uint32_t first = std::min(n, kFirst) + 1;
+    if (!ok(hipMemcpyAsync(cl_info, pass_pin, (size_t)ncl_cap * 8 + (size_t)ncl * 4,
+                           hipMemcpyHostToDevice, s)) ||
+        !ok(d2g::launch_shuffle_pass(args(ncl), s)) ||
+        !ok(hipMemcpyAsync(moves_pin, moves, (size_t)first * sizeof(d2g::ShuffleMove),
+                           hipMemcpyDeviceToHost, s)) ||
+        !ok(hipStreamSynchronize(s)))
+      return false;
+    const uint32_t k = moves_pin[0].r;
+    out.assign(moves_pin + 1, moves_pin + 1 + std::min(k, kFirst));
+    if (k > kFirst) {
+      out.resize(k);
+      if (!ok(hipMemcpy(out.data() + kFirst, moves + 1 + kFirst,
+                        (size_t)(k - kFirst) * sizeof(d2g::ShuffleMove), hipMemcpyDeviceToHost)))
+        return false;
+    }
+    return true;
+  }
+};
+
 struct Run {
   d2g_ctx* ctx;
   const double* err;
@@ -310,6 +442,8 @@ struct Run {
   std::vector<uint64_t> store_counts;
   std::vector<size_t> bud_off;
   d2g_dada_stats st{};
+  DevShuffle* dev = nullptr;  // the device scan, when asked for
+  bool dev_failed = false;
 
   static double ms_since(std::chrono::steady_clock::time_point t0) {
     return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
@@ -363,7 +497,7 @@ struct Run {
           c.lambda = lambda;
           c.hamming = ham[r] == D2G_HAMMING_NONE ? 0u : ham[r];
           out.push_back(c);
-          cand[r].push_back(Cand{c.i, c.hamming, c.lambda});
+          if (!dev) cand[r].push_back(Cand{c.i, c.hamming, c.lambda});
           if (i == 0 || r == center) B.raws[r].comp = c;
         }
       }
@@ -376,6 +510,7 @@ struct Run {
       B.nshroud += store_counts[2 * k + 1];
       C.comp.insert(C.comp.end(), store_comps[k].begin(), store_comps[k].end());
     }
+    if (dev && !dev->append(C.comp, (uint32_t)i)) return D2G_ERR_HIP;
     st.ms_store += ms_since(t0);
     return D2G_OK;
   }
@@ -546,6 +681,38 @@ struct Run {
     return movers.size();
   }
 
+  // The same pass with the scan on the device: the movers come back
+  // unordered and are applied in the host pass's order.
+  size_t shuffle_pass_dev(std::vector<d2g::ShuffleMove>& mv, std::vector<uint64_t>& keys) {
+    if (!dev->pass(B, mv)) {
+      dev_failed = true;
+      return 0;
+    }
+    keys.resize(mv.size());
+    for (size_t j = 0; j < mv.size(); ++j) {
+      const uint32_t r = mv[j].r;
+      // cluster ascending, position descending; the index rides in the low bits
+      keys[j] = ((uint64_t)B.where[r] << 32 | (uint64_t)(0xFFFFFFFFu - B.at[r]));
+    }
+    std::vector<size_t> ord(mv.size());
+    for (size_t j = 0; j < ord.size(); ++j) ord[j] = j;
+    std::sort(ord.begin(), ord.end(), [&](size_t x, size_t y) { return keys[x] < keys[y]; });
+    for (size_t j : ord) {
+      const d2g::ShuffleMove& m = mv[j];
+      const size_t ci = (size_t)(keys[j] >> 32);
+      const size_t pos = 0xFFFFFFFFu - (uint32_t)keys[j];
+      B.pop_raw(ci, pos);
+      B.add_raw(m.to, m.r);
+      Comp c;
+      c.i = m.to;
+      c.index = m.r;
+      c.lambda = m.lambda;
+      c.hamming = m.ham;
+      B.raws[m.r].comp = c;
+    }
+    return mv.size();
+  }
+
   // b_shuffle_converge, cluster.rs:908-1320 (see the counting note above)
   void shuffle(bool first) {
     const auto t0 = std::chrono::steady_clock::now();
@@ -553,9 +720,12 @@ struct Run {
     std::vector<Comp> compmax;
     std::vector<double> creads;
     std::vector<std::vector<uint64_t>> found;
-    for (uint32_t k = 0; k < passes; ++k) {
+    std::vector<d2g::ShuffleMove> mv;
+    std::vector<uint64_t> keys;
+    for (uint32_t k = 0; k < passes && !dev_failed; ++k) {
       ++st.shuffle_passes;
-      if (shuffle_pass(compmax, creads, found) == 0) break;
+      const size_t moved = dev ? shuffle_pass_dev(mv, keys) : shuffle_pass(compmax, creads, found);
+      if (moved == 0) break;
     }
     st.ms_shuffle += ms_since(t0);
   }
@@ -602,6 +772,11 @@ extern "C" int d2g_run_dada(d2g_ctx* ctx, const double* err, uint32_t ncol,
   run.cand.resize(n);
   Pool pool(pool_threads());
   run.pool = &pool;
+  DevShuffle dev;
+  if (dada->device_shuffle) {
+    if (!dev.init((hipStream_t)d2g_ctx_stream(ctx), n)) return D2G_ERR_HIP;
+    run.dev = &dev;
+  }
   // init, containers.rs:363-391
   {
     Cluster c0;
@@ -627,6 +802,7 @@ extern "C" int d2g_run_dada(d2g_ctx* ctx, const double* err, uint32_t ncol,
     if (ni < 0) break;
     if ((rc = run.compare((size_t)ni)) != D2G_OK) return rc;
     run.shuffle(first_shuffle);
+    if (run.dev_failed) return D2G_ERR_HIP;
     first_shuffle = false;
     run.p_update();
   }

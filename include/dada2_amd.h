@@ -171,7 +171,8 @@ typedef struct {
   int32_t greedy;        /* 1 */
   int32_t detect_singletons; /* 0 */
   int32_t use_quals;     /* 1 */
-  int32_t _reserved;
+  int32_t device_shuffle; /* 0: the shuffle's best-cluster scan on the host's threads;
+                           * 1: on the device (same moves, same order) */
 } d2g_dada_params;
 
 /* One cluster of the final partition (Bi, containers.rs:200-238). */

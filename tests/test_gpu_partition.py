@@ -28,6 +28,8 @@ def run_both(pkg, oracle, w, opts):
     kw = dict(omega_a=1e-40, omega_p=1e-4, omega_c=1e-40, min_fold=1.0, min_hamming=1,
               min_abund=1, max_clust=0, max_shuffle=10, greedy=True, detect_singletons=False,
               use_quals=True)
+    opts = dict(opts)
+    dev = opts.pop("device_shuffle", False)
     kw.update(opts)
     prior = w.extra["prior"]
     rs = oracle.RawSet(w.seqs, w.quals, w.reads, k=5)
@@ -35,19 +37,21 @@ def run_both(pkg, oracle, w, opts):
     ctx = pkg.Context(0)
     try:
         ctx.load_uniques(w.seqs, w.quals, w.reads, k=5)
-        got = ctx.run_dada(w.err, pkg.align_params(band=16), pkg.dada_params(**kw), prior=prior)
+        got = ctx.run_dada(w.err, pkg.align_params(band=16),
+                           pkg.dada_params(device_shuffle=dev, **kw), prior=prior)
     finally:
         ctx.close()
     return got, want
 
 
+@pytest.mark.parametrize("device_shuffle", [False, True], ids=["host_shuffle", "device_shuffle"])
 @pytest.mark.parametrize("name,wargs,opts", CASES, ids=[c[0] for c in CASES])
-def test_run_dada_matches_oracle(pkg, oracle, name, wargs, opts):
+def test_run_dada_matches_oracle(pkg, oracle, name, wargs, opts, device_shuffle):
     wargs = dict(wargs)
     seed = synth.SEED_BASE + wargs.pop("seed_off")
     w = synth.workload_partition(seed=seed, **wargs)
-    print(f"case {name}: {w.n} uniques", flush=True)
-    got, want = run_both(pkg, oracle, w, opts)
+    print(f"case {name}, device_shuffle={device_shuffle}: {w.n} uniques", flush=True)
+    got, want = run_both(pkg, oracle, w, dict(opts, device_shuffle=device_shuffle))
     assert len(got.clusters) == len(want.clusters)
     for k, (g, o) in enumerate(zip(got.clusters, want.clusters)):
         for key in o:
@@ -119,6 +123,15 @@ def test_run_dada_long_reads_homopolymer(pkg, oracle):
     print(f"long-read partition: {w.n} uniques of {len(w.seqs[0])} nt", flush=True)
     got, want = _run_pair(pkg, oracle, w.seqs, w.quals, w.reads, dict(band=32, homo_gap_p=-1))
     _same(got, want)
+    # the same run with the shuffle's scan on the host's threads
+    ctx = pkg.Context(0)
+    try:
+        ctx.load_uniques(w.seqs, w.quals, w.reads, k=5)
+        dev = ctx.run_dada(synth.fixed_err(41), pkg.align_params(band=32, homo_gap_p=-1),
+                           pkg.dada_params(device_shuffle=False))
+    finally:
+        ctx.close()
+    _same(dev, want)
     assert sorted(c["center"] for c in got.clusters) == sorted(int(x) for x in w.extra["planted"])
 
 
