@@ -425,6 +425,30 @@ struct DevShuffle {
   }
 };
 
+// A pinned host array: the compare's results are copied straight into it
+// (a pageable destination goes through a staging copy).
+template <class T>
+struct Pinned {
+  T* p = nullptr;
+  size_t n = 0;
+  ~Pinned() {
+    if (p) (void)hipHostFree(p);
+  }
+  bool resize(size_t k) {
+    if (p) (void)hipHostFree(p);
+    p = nullptr;
+    n = 0;
+    if (hipHostMalloc((void**)&p, std::max<size_t>(k, 1) * sizeof(T), hipHostMallocDefault) !=
+        hipSuccess)
+      return false;
+    n = k;
+    return true;
+  }
+  T* data() { return p; }
+  T& operator[](size_t i) { return p[i]; }
+  const T& operator[](size_t i) const { return p[i]; }
+};
+
 struct Run {
   d2g_ctx* ctx;
   const double* err;
@@ -432,9 +456,10 @@ struct Run {
   d2g_align_params align;
   d2g_dada_params dp;
   Part B;
-  std::vector<double> lam;
-  std::vector<uint32_t> ham;
-  std::vector<uint8_t> skp, lock;
+  Pinned<double> lam;
+  Pinned<uint32_t> ham;
+  Pinned<uint8_t> skp;
+  std::vector<uint8_t> lock;
   std::vector<std::vector<Cand>> cand;  // per raw
   Pool* pool = nullptr;
   // per-chunk scratch of the store and the bud scan
@@ -765,9 +790,7 @@ extern "C" int d2g_run_dada(d2g_ctx* ctx, const double* err, uint32_t ncol,
   B.where.assign(n, 0);
   B.at.resize(n);
   for (uint32_t r = 0; r < n; ++r) B.at[r] = r;
-  run.lam.resize(n);
-  run.ham.resize(n);
-  run.skp.resize(n);
+  if (!run.lam.resize(n) || !run.ham.resize(n) || !run.skp.resize(n)) return D2G_ERR_HIP;
   run.lock.resize(n);
   run.cand.resize(n);
   Pool pool(pool_threads());
