@@ -575,10 +575,20 @@ def main():
     gcups = st["dp_cells"] / (ms_dp * 1e-3) / 1e9
     kernels = {"screen": ms_screen, "plan+scatter": ms_plan, "nw_vec": ms_dp, "lambda": ms_lam}
     dominant = max(kernels, key=kernels.get)
+    screen_traffic = traffic_from_profile("screen_kernel")
     roof_screen = {"bound": "hbm", "achieved": screen_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                   "frac": screen_gbs / HBM_PEAK_GBS, "traffic": traffic_from_profile("screen_kernel"),
+                   "frac": screen_gbs / HBM_PEAK_GBS, "traffic": screen_traffic,
                    "kernel": "screen_kernel", "algorithmic_bytes_per_launch": 1024 * screened,
-                   "avg_ms": ms_screen}
+                   "avg_ms": ms_screen,
+                   # the per-comparison unit counts a query's k-mer profile once per
+                   # center, but a workgroup reads it once for 16 centers: the measured
+                   # HBM traffic per launch is the rate the memory system sustains
+                   "note": "algorithmic bytes count 1 KB per comparison; profiles are "
+                           "reused across 16 centers in registers, so frac > 1 is possible",
+                   "traffic_rate_gbs": (screen_traffic / (ms_screen * 1e-3) / 1e9
+                                        if screen_traffic else None),
+                   "traffic_frac": (screen_traffic / (ms_screen * 1e-3) / 1e9 / HBM_PEAK_GBS
+                                    if screen_traffic else None)}
     roof_dp = {"bound": "valu", "achieved": gcups * 8 / 1e3, "peak": valu_cells_peak * 8 / 1e12,
                "unit": "TOP/s", "frac": gcups * 1e9 / valu_cells_peak,
                "traffic": traffic_from_profile("nw_vec_kernel"),
