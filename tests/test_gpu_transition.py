@@ -260,3 +260,47 @@ def test_birth_subs_capacity(pkg):
         assert [(int(cl[i]), int(pos[i])) for i in range(2)] == [r[:2] for r in recs[:2]]
     finally:
         ctx.close()
+
+
+def test_dada_aux_edges(pkg, oracle):
+    """cluster_quality truncated at a maxlen below the centers' length (as
+    the oracle's restatement, error.rs:268); build_trans_mat with the k-mer
+    screen on and a negative cutoff counts nothing (nwalign.rs:1296 with
+    kdist 0.0); argument errors."""
+    from oracle import learn
+    from oracle import partition as P
+    w = synth.workload_partition(n_true=4, n_near=2, L=150, n_uniques=200,
+                                 seed=synth.SEED_BASE + 65)
+    params = oracle.make_params()
+    rs = oracle.RawSet(w.seqs, w.quals, w.reads, k=5)
+    out = P.run_dada(rs, w.err, params)
+    centers = [c["center"] for c in out.clusters]
+    weight = np.where(out.map >= 0, np.asarray(w.reads, np.int64), 0)
+    want = learn.dada_aux(rs, centers, out.cluster_of.astype(np.int64), weight, params, True, 41,
+                          100)
+    ctx = pkg.Context(0)
+    try:
+        ctx.load_uniques(w.seqs, w.quals, w.reads, k=5)
+        res = ctx.run_dada(w.err)
+        got = ctx.dada_aux(res, maxlen=100)
+        gq, wq = got["cluster_quality"], want["cluster_quality"]
+        assert gq.shape == (len(centers), 100)
+        assert np.array_equal(np.isnan(gq), np.isnan(wq))
+        assert np.array_equal(gq[~np.isnan(gq)], wq[~np.isnan(wq)])
+        assert np.array_equal(got["transitions"], want["transitions"])
+        center_of, wt = learn.center_weight(out, w.reads)
+        neg = pkg.align_params(kdist_cutoff=-0.5)
+        assert not ctx.build_trans_mat(center_of, wt, neg).any()
+        assert np.array_equal(learn.build_trans_mat(w.seqs, w.quals, center_of, wt,
+                                                    oracle.make_params(kdist_cutoff=-0.5), 41),
+                              np.zeros((16, 41), np.uint32))
+        # a cluster index past the clusters: D2G_ERR_ARG
+        bad = res.cluster_of.copy()
+        bad[0] = len(res.clusters)
+        import copy
+        r2 = copy.copy(res)
+        r2.cluster_of = bad
+        with pytest.raises(pkg.D2GError):
+            ctx.dada_aux(r2)
+    finally:
+        ctx.close()
