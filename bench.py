@@ -414,19 +414,17 @@ def bench_transitions(args, pkg):
     if not args.no_cpu:
         import oracle as O
         from oracle import learn
-        n_s = 3000  # the first corrected uniques of the same partition
-        idx = np.nonzero(weight > 0)[0][:n_s]
-        wsub = np.zeros_like(weight)
-        wsub[idx] = weight[idx]
+        th = O.cpu_count_allowed()
         box = {}
 
         def one():
-            box["m"] = learn.build_trans_mat(w.seqs, w.quals, center_of, wsub, O.make_params(band=16),
-                                             41)
+            box["m"] = learn.build_trans_mat_c(w.seqs, w.quals, center_of, weight,
+                                               O.make_params(band=16), 41, th)
         calls, dt = _timed_cpu(one, args.cpu_seconds)
-        cpu = {"value": calls * len(idx) / dt, "unit": "alignments/s", "cores": 1, "kind": "port",
-               "sample": f"{calls} x {len(idx)} corrected uniques of the same partition, "
-                         f"{dt:.1f} s: the oracle's build_trans_mat (C alignment, Python count)"}
+        cpu = {"value": calls * counted / dt, "unit": "alignments/s", "cores": th, "kind": "port",
+               "sample": f"{calls} whole calls over the same {counted} corrected uniques, "
+                         f"{dt:.1f} s: the oracle's threaded C build_trans_mat",
+               "identical": bool(np.array_equal(box["m"], mat))}
     return _line("raw-center alignments/sec through build_trans_mat (learn-errors counts)",
                  counted / (ms * 1e-3), "alignments/s", args, ms, "i16 DP / u64 counts",
                  {"workload": "transitions: build_trans_mat over the dada workload's partition "

@@ -128,6 +128,9 @@ def lib():
             "orc_table_bimera2": (C.c_int, [u32p, C.c_uint32, C.c_uint32, u8p, u64p, C.c_double,
                                             C.c_uint32, C.POINTER(BimeraParams), u32p, u32p,
                                             C.c_int]),
+            "orc_build_trans_mat": (C.c_int, [C.POINTER(Raw), C.c_uint32,
+                                              C.POINTER(C.c_int64), u32p, C.POINTER(AlignParams),
+                                              C.c_uint32, C.c_int, u32p]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)

@@ -23,6 +23,8 @@ position counts once).
 """
 from __future__ import annotations
 
+import ctypes as C
+
 import numpy as np
 
 from . import AlignParams, Buf, RawSet, raw_align, sub_new
@@ -95,6 +97,25 @@ def build_trans_mat(seqs, quals, center_of, weight, align: AlignParams, nq: int)
             if a1 != ord("-"):
                 qpos += 1
     return np.minimum(np.asarray(mat, dtype=object), 0xFFFFFFFF).astype(np.uint32).reshape(16, nq)
+
+
+def build_trans_mat_c(seqs, quals, center_of, weight, align: AlignParams, nq: int,
+                      nthreads: int = 1) -> np.ndarray:
+    """build_trans_mat in the oracle's threaded C (oracle/learn.c): the same
+    counts as build_trans_mat above, computed as the reference's rayon fold."""
+    from . import lib
+    if quals is None:
+        return np.zeros((16, nq), np.uint32)
+    rs = RawSet(seqs, quals, weight, with_kmers=False)
+    co = np.ascontiguousarray(np.asarray(center_of, np.int64))
+    w = np.ascontiguousarray(np.asarray(weight, np.uint64) & np.uint64(0xFFFFFFFF), np.uint32)
+    out = np.zeros(16 * nq, np.uint32)
+    rc = lib().orc_build_trans_mat(rs.arr, len(rs), co.ctypes.data_as(C.POINTER(C.c_int64)),
+                                   w.ctypes.data_as(C.POINTER(C.c_uint32)), C.byref(align), nq,
+                                   nthreads, out.ctypes.data_as(C.POINTER(C.c_uint32)))
+    if rc < 0:
+        raise RuntimeError(f"oracle build_trans_mat failed with status {rc}")
+    return out.reshape(16, nq)
 
 
 def center_weight(out, reads):

@@ -197,6 +197,14 @@ int orc_table_bimera2(const uint32_t* mat, uint32_t nrow, uint32_t ncol, const u
                       const orc_bimera_params* p, uint32_t* out_nflag, uint32_t* out_nsam,
                       int nthreads);
 
+/* ---------------- learn_errors.rs ---------------- */
+/* build_trans_mat (learn_errors.rs:618-690): center_of[i] < 0 for none,
+ * weight[i] the abundance (0: skipped); raws without k-mer vectors and with
+ * qualities; out 16 * nq, saturating. */
+int orc_build_trans_mat(const orc_raw* raws, uint32_t nraw, const int64_t* center_of,
+                        const uint32_t* weight, const orc_align_params* p, uint32_t nq,
+                        int nthreads, uint32_t* out);
+
 #ifdef __cplusplus
 }
 #endif

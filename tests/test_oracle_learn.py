@@ -229,3 +229,20 @@ def test_post_hoc_pval_two_clusters():
                           True)
     assert out.clusters[1]["posthoc_pval"] == want
     assert 0.0 <= out.clusters[0]["posthoc_pval"] <= 1.0
+
+
+@pytest.mark.parametrize("nthreads", [1, 4])
+def test_build_trans_mat_c_matches_python(nthreads):
+    """The threaded C restatement (the CPU baseline's body) against the
+    Python one, pinned above: same counts, saturation included."""
+    from oracle import partition as P
+    w = synth.workload_partition(n_true=5, n_near=2, n_prior=0, L=150, n_uniques=200,
+                                 seed=synth.SEED_BASE + 47)
+    params = oracle.make_params(band=16)
+    rs = oracle.RawSet(w.seqs, w.quals, w.reads, k=5)
+    out = P.run_dada(rs, w.err, params)
+    center_of, weight = learn.center_weight(out, w.reads)
+    for wt, nq in ((weight, 41), (weight, 20), (np.where(weight > 0, (1 << 32) - 1, 0), 41)):
+        a = learn.build_trans_mat(w.seqs, w.quals, center_of, wt, params, nq)
+        b = learn.build_trans_mat_c(w.seqs, w.quals, center_of, wt, params, nq, nthreads)
+        assert np.array_equal(a, b)
