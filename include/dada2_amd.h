@@ -157,7 +157,9 @@ int d2g_fetch_results(d2g_ctx* ctx, double* out_lambda, uint32_t* out_hamming,
  * Replaces run_dada (dada.rs:749-1150) plus the final per-raw p-value pass
  * and read map (dada.rs:502-533), over the loaded uniques (raw index = load
  * order).  Each cluster's compare runs on the device (d2g_compare_center).
- * B, the serial store, b_p_update, b_bud and the shuffle run on the host.
+ * B, the store, b_p_update, b_bud and the shuffle's moves run on the host;
+ * the shuffle's best-cluster scan runs on the device with device_shuffle,
+ * else on the host's threads (the same moves either way).
  * DadaParams fields (dada.rs:54-90; defaults main.rs:4393-4433): */
 typedef struct {
   double omega_a;        /* 1e-40: bud when min p * nraw < omega_a */
