@@ -143,6 +143,12 @@ struct d2g_ctx {
   DevBuf<unsigned long long> tr_qsum;
   DevBuf<unsigned long long> tr_mat;
   d2g_stats tr_stats{};  // the last transition pass's compare
+  // pinned staging of the per-unique centers and weights (and the slot
+  // offsets, uploaded again only when another entry reused a_aloff)
+  uint8_t* tr_pin = nullptr;
+  size_t tr_pin_cap = 0;
+  const uint64_t* tr_aoff_for = nullptr;
+  uint64_t tr_aoff_n = 0, tr_aoff_stride = 0;
   // align mode buffers
   DevBuf<uint8_t> a_pool, a_al0, a_al1;
   DevBuf<uint64_t> a_aloff;
@@ -214,6 +220,8 @@ int d2g_ctx_destroy(d2g_ctx* c) {
     b->release();
   c->off.release();
   c->a_aloff.release();
+  if (c->tr_pin) (void)hipHostFree(c->tr_pin);
+  c->tr_pin = nullptr;
   c->err.release();
   c->out_lambda.release();
   c->gl.release();
@@ -1037,8 +1045,19 @@ static int transitions_run(d2g_ctx* c, const uint32_t* center_of, const uint32_t
               std::numeric_limits<double>::quiet_NaN());
   }
   if (centers.empty()) return D2G_OK;
+  // pinned staging (free: the previous call ended with a synchronisation)
+  const size_t pin_need = (size_t)n * 16;
+  if (c->tr_pin_cap < pin_need) {
+    if (c->tr_pin) (void)hipHostFree(c->tr_pin);
+    c->tr_pin = nullptr;
+    c->tr_pin_cap = 0;
+    HIP_TRY(hipHostMalloc((void**)&c->tr_pin, pin_need, hipHostMallocDefault));
+    c->tr_pin_cap = pin_need;
+  }
+  uint64_t* aoff = reinterpret_cast<uint64_t*>(c->tr_pin);
+  uint32_t* want = reinterpret_cast<uint32_t*>(c->tr_pin + (size_t)n * 8);
+  uint32_t* w = want + n;
   // per sorted unique: its center (sorted index) or none, and its weight
-  std::vector<uint32_t> want(n), w(n);
   for (uint32_t s = 0; s < n; ++s) {
     const uint32_t i = c->h_orig[s];
     const bool counted = center_of[i] != D2G_NO_CENTER && weight[i] != 0;
@@ -1048,8 +1067,6 @@ static int transitions_run(d2g_ctx* c, const uint32_t* center_of, const uint32_t
   // one alignment slot of 2 * longest length per unique (the strings of an
   // ends-free alignment are at most len1 + len2 long)
   const uint64_t stride = 2ull * c->u.maxlen;
-  std::vector<uint64_t> aoff(n);
-  for (uint32_t s = 0; s < n; ++s) aoff[s] = s * stride;
   hipStream_t st = c->stream;
   HIP_TRY(c->tr_want.ensure(n));
   HIP_TRY(c->tr_weight.ensure(n));
@@ -1058,11 +1075,16 @@ static int transitions_run(d2g_ctx* c, const uint32_t* center_of, const uint32_t
   HIP_TRY(c->a_al1.ensure(n * stride));
   HIP_TRY(c->a_aloff.ensure(n));
   HIP_TRY(c->a_allen.ensure(n));
-  HIP_TRY(hipMemcpyAsync(c->tr_want.p, want.data(), n * sizeof(uint32_t), hipMemcpyHostToDevice, st));
-  HIP_TRY(hipMemcpyAsync(c->tr_weight.p, w.data(), n * sizeof(uint32_t), hipMemcpyHostToDevice, st));
-  HIP_TRY(hipMemcpyAsync(c->a_aloff.p, aoff.data(), n * sizeof(uint64_t), hipMemcpyHostToDevice, st));
+  HIP_TRY(hipMemcpyAsync(c->tr_want.p, want, n * sizeof(uint32_t), hipMemcpyHostToDevice, st));
+  HIP_TRY(hipMemcpyAsync(c->tr_weight.p, w, n * sizeof(uint32_t), hipMemcpyHostToDevice, st));
+  if (c->tr_aoff_for != c->a_aloff.p || c->tr_aoff_n != n || c->tr_aoff_stride != stride) {
+    for (uint32_t s = 0; s < n; ++s) aoff[s] = s * stride;
+    HIP_TRY(hipMemcpyAsync(c->a_aloff.p, aoff, n * sizeof(uint64_t), hipMemcpyHostToDevice, st));
+    c->tr_aoff_for = c->a_aloff.p;
+    c->tr_aoff_n = n;
+    c->tr_aoff_stride = stride;
+  }
   HIP_TRY(launch_zero(reinterpret_cast<uint32_t*>(c->tr_mat.p), (size_t)32 * ncol_dev, st));
-  HIP_TRY(hipStreamSynchronize(st));  // the host vectors are pageable
   TransMode tm;
   tm.want = c->tr_want.p;
   tm.weight = c->tr_weight.p;
@@ -1240,6 +1262,7 @@ int d2g_birth_subs(d2g_ctx* c, uint32_t ncl, const uint32_t* centers, const uint
   HIP_TRY(c->a_al1.ensure(npair * stride));
   HIP_TRY(c->a_aloff.ensure(n));
   HIP_TRY(c->a_allen.ensure(n));
+  c->tr_aoff_for = nullptr;  // the offsets below are the birth pass's
   HIP_TRY(hipMemcpyAsync(c->tr_want.p, want.data(), n * sizeof(uint32_t), hipMemcpyHostToDevice, st));
   HIP_TRY(hipMemcpyAsync(c->tr_weight.p, one.data(), n * sizeof(uint32_t), hipMemcpyHostToDevice, st));
   HIP_TRY(hipMemcpyAsync(c->a_aloff.p, aoff.data(), n * sizeof(uint64_t), hipMemcpyHostToDevice, st));
@@ -1453,6 +1476,7 @@ static int align_pairs_device(d2g_ctx* c, uint32_t n, const PairSeqs& ps, int ki
   HIP_TRY(hipMemcpyAsync(c->a_pool.p + n1, ps.s2 + ps.s2_off[0], n2, hipMemcpyHostToDevice, s));
   HIP_TRY(hipMemcpyAsync(c->dp_unsorted.p, tasks.data(), n * sizeof(DpTask), hipMemcpyHostToDevice, s));
   HIP_TRY(hipMemcpyAsync(c->bin_count.p, counts.data(), (bs.nbins + 1) * sizeof(uint32_t), hipMemcpyHostToDevice, s));
+  c->tr_aoff_for = nullptr;  // the transition pass's offsets are overwritten
   HIP_TRY(hipMemcpyAsync(c->a_aloff.p, aoff.data(), n * sizeof(uint64_t), hipMemcpyHostToDevice, s));
   Counters h0;
   std::memset(&h0, 0, sizeof(h0));
