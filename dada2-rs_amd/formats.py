@@ -232,6 +232,20 @@ def _finite_doc(x):
     return x
 
 
+def run_params(align, dada, pool: bool = False, n_prior: int = 0) -> dict:
+    """DadaRunParams (main.rs:142-172) of a run, in the reference's field
+    order, from the package's AlignParams and DadaParams."""
+    return {"omega_a": dada.omega_a, "omega_c": dada.omega_c, "omega_p": dada.omega_p,
+            "min_fold": dada.min_fold, "min_hamming": int(dada.min_hamming),
+            "min_abund": int(dada.min_abund), "detect_singletons": bool(dada.detect_singletons),
+            "band": int(align.band), "homo_gap_p": int(align.homo_gap_p),
+            "gap_p": int(align.gap_p), "match_score": int(align.match_score),
+            "mismatch": int(align.mismatch), "max_clust": int(dada.max_clust),
+            "greedy": bool(dada.greedy), "use_quals": bool(dada.use_quals),
+            "kdist_cutoff": align.kdist_cutoff, "kmer_size": int(align.kmer_size),
+            "use_kmers": bool(align.use_kmers), "pool": bool(pool), "n_prior": int(n_prior)}
+
+
 def dada_output(result, seqs, sample: str, input_file: str, run_params: dict,
                 aux: dict | None = None) -> dict:
     """The dada command's payload (DadaOutput, main.rs:1139-1250) for a

@@ -67,7 +67,11 @@ def test_dada_output_round_trip(tmp_path):
            "cluster_quality": np.array([[30.0, np.nan], [20.0, 21.0]]),
            "birth_subs": [(1, 0, 84, 65, 33)], "transitions": np.arange(16).reshape(16, 1),
            "transitions_ncol": 1}
-    doc = formats.dada_output(R, seqs, "s1", "s1.fastq.gz", {"omega_a": 1e-40}, aux)
+    from dada2_rs_amd import align_params, dada_params
+    rp = formats.run_params(align_params(), dada_params(), n_prior=2)
+    assert list(rp)[:3] == ["omega_a", "omega_c", "omega_p"] and rp["band"] == 16
+    assert rp["kdist_cutoff"] == 0.42 and rp["n_prior"] == 2 and rp["greedy"] is True
+    doc = formats.dada_output(R, seqs, "s1", "s1.fastq.gz", rp, aux)
     path = tmp_path / "s1.json"
     formats.write_tagged_json(path, "dada", doc)
     back = json.loads(path.read_text())
@@ -76,6 +80,7 @@ def test_dada_output_round_trip(tmp_path):
     assert [a["sequence"] for a in back["asvs"]] == ["TTAA", "ACGT"]
     assert [a["birth_type"] for a in back["asvs"]] == ["Initial", "Abundance"]
     assert back["total_reads"] == 40 and back["map"] == [1, 0, None]
+    assert back["params"]["omega_a"] == 1e-40 and back["params"]["use_kmers"] is True
     assert back["aux"]["cluster_quality"] == [[30.0, None], [20.0, 21.0]]  # NaN -> null
     assert back["aux"]["birth_subs"][0]["nt0"] == "T"
     assert back["aux"]["cluster_stats"][0]["sequence"] == "TTAA"
