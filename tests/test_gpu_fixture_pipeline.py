@@ -37,8 +37,10 @@ def denoised(pkg, oracle):
             weight = np.where(want.map >= 0, np.asarray(reads, np.int64), 0)
             want_aux = learn.dada_aux(rs, centers, want.cluster_of.astype(np.int64), weight,
                                       oracle.make_params(), True, 41, 250)
+            rp = formats.run_params(pkg.align_params(), pkg.dada_params())
             doc = formats.tagged("dada", formats.dada_output(got, seqs, name, f"{name}.fastq.gz",
-                                                             {"omega_a": 1e-40}, aux))
+                                                             rp, aux))
+            assert doc["params"]["kdist_cutoff"] == 0.42 and doc["num_asvs"] == len(got.clusters)
             print(f"{name}: {len(seqs)} uniques, {len(got.clusters)} ASVs", flush=True)
             out[name] = (got, want, aux, want_aux, doc)
     finally:
