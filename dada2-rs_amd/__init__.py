@@ -111,6 +111,9 @@ def _lib():
             "d2g_pairs_lambda": (C.c_int, [vp, f64p, C.c_uint32, C.c_int, C.c_double,
                                            C.POINTER(PairStats)]),
             "d2g_pairs_fetch": (C.c_int, [vp, C.POINTER(C.c_float), f64p, f64p, u8p]),
+            "d2g_pairs_lambda_noalloc": (C.c_int, [vp, C.POINTER(PairsDev), f64p, C.c_uint32,
+                                                   C.c_int, C.c_double, vp, vp, vp, vp,
+                                                   C.POINTER(PairStats)]),
             "d2g_bimera_pairs": (C.c_int, [vp, C.c_uint32, u8p, u64p, u8p, u64p,
                                            C.POINTER(BimeraParams), u32p, u32p]),
             "d2g_is_bimera_batch": (C.c_int, [vp, C.c_uint32, u8p, u64p, u8p, u64p, u32p,
@@ -161,7 +164,7 @@ def exported_symbols() -> list[str]:
             "d2g_loglik_forward", "d2g_loglik_backward", "d2g_ctx_set_graphs",
             "d2g_ctx_set_screen_shape", "d2g_ctx_set_autotune", "d2g_run_dada",
             "d2g_transition_counts", "d2g_build_trans_mat", "d2g_transition_stats",
-            "d2g_dada_aux", "d2g_birth_subs"]
+            "d2g_dada_aux", "d2g_birth_subs", "d2g_pairs_lambda_noalloc"]
 
 
 def _p(a: np.ndarray, t):
@@ -213,6 +216,46 @@ class PairStats(C.Structure):
 
     def as_dict(self) -> dict:
         return {name: getattr(self, name) for name, _ in self._fields_ if name != "_pad"}
+
+
+class PairsDev(C.Structure):
+    """d2g_pairs_dev: a pair set in device memory (d2g_pairs_load's layout)."""
+    _fields_ = [("n", C.c_uint32), ("_pad", C.c_uint32), ("pool_bytes", C.c_uint64),
+                ("n_subs", C.c_uint64), ("seq", C.c_void_p), ("qual", C.c_void_p),
+                ("off", C.c_void_p), ("len", C.c_void_p), ("subs", C.c_void_p),
+                ("sub_off", C.c_void_p), ("reads", C.c_void_p), ("center_reads", C.c_void_p)]
+
+
+NOALLOC_MAX_PAIRS = 1 << 20
+NOALLOC_MAX_READS = (1 << 20) - 1
+NOALLOC_MAX_NCOL = 256
+
+
+def pairs_layout(seqs, quals, subs, reads, center_reads) -> dict:
+    """Host arrays of a pair set in d2g_pairs_load's resident layout (rows at
+    16-byte offsets, padded), ready to copy to device memory for
+    d2g_pairs_lambda_noalloc: seq, qual (or None), off (n + 1), len, subs,
+    sub_off (n + 1), reads, center_reads."""
+    n = len(seqs)
+    lens = np.array([len(x) for x in seqs], np.uint32)
+    off = np.zeros(n + 1, np.uint64)
+    off[1:] = np.cumsum((lens.astype(np.uint64) + 15) // 16 * 16)
+    total = int(off[-1]) + 16
+    seq = np.ones(total, np.uint8)
+    qual = None if quals is None else np.zeros(total, np.uint8)
+    for i in range(n):
+        o = int(off[i])
+        seq[o:o + lens[i]] = np.asarray(seqs[i], np.uint8)
+        if qual is not None:
+            qual[o:o + lens[i]] = np.asarray(quals[i], np.uint8)
+    so = np.zeros(n + 1, np.uint64)
+    so[1:] = np.cumsum([len(x) for x in subs])
+    sc = (np.concatenate([np.asarray(x, np.uint32) for x in subs]) if int(so[-1])
+          else np.zeros(1, np.uint32))
+    return {"seq": seq, "qual": qual, "off": off, "len": lens,
+            "subs": np.ascontiguousarray(sc, np.uint32), "sub_off": so,
+            "n_subs": int(so[-1]), "reads": np.ascontiguousarray(reads, np.uint32),
+            "center_reads": np.ascontiguousarray(center_reads, np.uint32)}
 
 
 class DadaParams(C.Structure):
@@ -603,6 +646,25 @@ class Context:
         _check(_lib().d2g_pairs_fetch(self.h, ll.ctypes.data_as(C.POINTER(C.c_float)),
                                       _p(lam, f64p), _p(pv, f64p), _p(fl, u8p)), "d2g_pairs_fetch")
         return ll, lam, pv, fl, st.as_dict()
+
+    def pairs_lambda_noalloc(self, n: int, pool_bytes: int, n_subs: int, ptrs: dict, err,
+                             out_ptrs: dict, use_quals=True, omega_a=1e-40) -> dict:
+        """d2g_pairs_lambda_noalloc over a pair set in device memory.  ptrs
+        maps seq, qual (0 for none), off, len, subs, sub_off, reads and
+        center_reads to device addresses (pairs_layout's arrays after the
+        copy); out_ptrs maps loglam32, lambda, pval and recomputed to device
+        addresses of n entries.  Returns the call's PairStats."""
+        e = np.ascontiguousarray(err, np.float64)
+        pd = PairsDev(n, 0, pool_bytes, n_subs, ptrs["seq"], ptrs.get("qual") or None,
+                      ptrs["off"], ptrs["len"], ptrs["subs"], ptrs["sub_off"], ptrs["reads"],
+                      ptrs["center_reads"])
+        st = PairStats()
+        _check(_lib().d2g_pairs_lambda_noalloc(self.h, C.byref(pd), _p(e, f64p), e.shape[1],
+                                               int(use_quals), float(omega_a),
+                                               out_ptrs["loglam32"], out_ptrs["lambda"],
+                                               out_ptrs["pval"], out_ptrs["recomputed"],
+                                               C.byref(st)), "d2g_pairs_lambda_noalloc")
+        return st.as_dict()
 
     def loglik_forward(self, logE, use_quals=True) -> float:
         """Poisson log-likelihood of the abundances over the last compare's

@@ -200,6 +200,11 @@ int d2g_ctx_create(int device, d2g_ctx** out) {
     d2g_ctx_destroy(c);
     return D2G_ERR_HIP;
   }
+  const int na = noalloc_setup(device);
+  if (na != D2G_OK) {
+    d2g_ctx_destroy(c);
+    return na;
+  }
   *out = c;
   return D2G_OK;
 }
@@ -1996,5 +2001,15 @@ int d2g_pairs_fetch(d2g_ctx* c, float* out_loglam32, double* out_lambda, double*
 namespace d2g {
 const std::vector<uint32_t>* ctx_host_reads(const d2g_ctx* c) {
   return (c && c->loaded) ? &c->h_reads : nullptr;
+}
+
+bool ctx_exec(const d2g_ctx* c, CtxExec* out) {
+  if (!c || !out) return false;
+  out->device = c->device;
+  out->stream = c->stream;
+  out->cu_count = c->cu_count;
+  out->timing = c->timing;
+  for (int i = 0; i < 3; ++i) out->ev[i] = c->ev[i];
+  return true;
 }
 }  // namespace d2g

@@ -4,6 +4,8 @@
 
 #include "d2g_internal.h"
 
+struct d2g_ctx;
+
 namespace d2g {
 
 // ---- prep.hip ----------------------------------------------------------
@@ -290,5 +292,19 @@ struct PairArgs {
   hipEvent_t ev_mid;        // nullable: recorded between the two kernels
 };
 hipError_t launch_pairs_lambda(const PairArgs& a, hipStream_t s);
+
+// ---- api.hip / noalloc.hip ------------------------------------------------
+// What a kernel launch needs from a context (api.hip owns the struct).
+struct CtxExec {
+  int device;
+  hipStream_t stream;
+  int cu_count;
+  bool timing;
+  hipEvent_t ev[3];
+};
+bool ctx_exec(const d2g_ctx* c, CtxExec* out);
+// Once per device, with that device current: the address of the
+// allocation-free entry point's device array and its constant table.
+int noalloc_setup(int device);
 
 }  // namespace d2g

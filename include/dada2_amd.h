@@ -444,6 +444,55 @@ int d2g_pairs_lambda(d2g_ctx* ctx, const double* err, uint32_t ncol, int use_qua
 int d2g_pairs_fetch(d2g_ctx* ctx, float* out_loglam32, double* out_lambda, double* out_pval,
                     uint8_t* out_recomputed);
 
+/* ---- allocation-free variant ----------------------------------------------
+ * No counterpart in the reference.  It is for callers that cannot allocate
+ * device memory: code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc.  d2g_pairs_lambda_noalloc computes what
+ * d2g_pairs_load + d2g_pairs_lambda + d2g_pairs_fetch compute, with the same
+ * kernels and the same bits, over a pair set the caller already holds in
+ * device memory, and writes the outputs into the caller's device memory.
+ * It calls no hipMalloc / hipFree: its intermediates (error matrix, the
+ * log(reads!) table, the list of pairs near omega_a, counters) live in one
+ * library-owned device array per GPU of about 12.1 MiB, held until the
+ * process ends and set up by d2g_ctx_create.  Every call on one GPU shares
+ * that array: calls that overlap on one GPU race.  Limits (beyond them the
+ * call returns D2G_ERR_UNSUPPORTED): */
+#define D2G_NOALLOC_MAX_PAIRS (1u << 20)
+#define D2G_NOALLOC_MAX_READS ((1u << 20) - 1u)
+#define D2G_NOALLOC_MAX_NCOL 256u
+
+/* The pair set in device memory, in d2g_pairs_load's resident layout:
+ * pair i's bases (1..4) at seq[off[i] .. off[i] + len[i]), off[i] a
+ * multiple of 16 and the row readable up to the next multiple of 16 inside
+ * pool_bytes; qual (nullable) in the same layout; its al2subs records
+ * subs[sub_off[i] .. sub_off[i+1]) as in d2g_pairs_load, inside n_subs
+ * records.  off and sub_off have n + 1 entries, len / reads /
+ * center_reads n.  Offsets, lengths, bases, qualities (< ncol when used),
+ * records and reads (>= 1) are checked on the device before any row is
+ * read: D2G_ERR_ARG / D2G_ERR_NONACGT as d2g_pairs_load returns them. */
+typedef struct d2g_pairs_dev {
+  uint32_t n;
+  uint32_t _pad;
+  uint64_t pool_bytes;
+  uint64_t n_subs;
+  const uint8_t* seq;
+  const uint8_t* qual;
+  const uint64_t* off;
+  const uint32_t* len;
+  const uint32_t* subs;
+  const uint64_t* sub_off;
+  const uint32_t* reads;
+  const uint32_t* center_reads;
+} d2g_pairs_dev;
+
+/* err: host memory, 16 x ncol f64.  out_*: device memory, n entries each,
+ * as d2g_pairs_fetch returns them.  The call returns after its stream has
+ * finished. */
+int d2g_pairs_lambda_noalloc(d2g_ctx* ctx, const d2g_pairs_dev* pairs, const double* err,
+                             uint32_t ncol, int use_quals, double omega_a, float* out_loglam32,
+                             double* out_lambda, double* out_pval, uint8_t* out_recomputed,
+                             d2g_pair_stats* stats);
+
 #ifdef __cplusplus
 }
 #endif

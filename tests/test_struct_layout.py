@@ -15,6 +15,7 @@ PAIRS = [  # (C struct, ctypes class name)
     ("d2g_dada_stats", "DadaStats"),
     ("d2g_bimera_params", "BimeraParams"),
     ("d2g_pair_stats", "PairStats"),
+    ("d2g_pairs_dev", "PairsDev"),
 ]
 
 
