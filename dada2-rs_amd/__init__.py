@@ -56,10 +56,33 @@ class Stats(C.Structure):
 
 def align_params(match=5, mismatch=-4, gap_p=-8, homo_gap_p=None, use_kmers=True, kmer_size=5,
                  kdist_cutoff=0.42, band=16, vectorized=True, gapless=True) -> AlignParams:
-    """Defaults of resolve_dada_params (main.rs:4393-4433)."""
-    return AlignParams(match, mismatch, gap_p, gap_p if homo_gap_p is None else homo_gap_p,
-                       int(use_kmers), kmer_size, kdist_cutoff, band, int(vectorized),
-                       int(gapless), 0)
+    """Defaults of resolve_dada_params (main.rs:4393-4433), which turns a
+    positive gap_p negative before homo_gap_p defaults to it, and a positive
+    homo_gap_p negative too (main.rs:4403-4418)."""
+    gap_p = -gap_p if gap_p > 0 else gap_p
+    homo = gap_p if homo_gap_p is None else homo_gap_p
+    homo = -homo if homo > 0 else homo
+    return AlignParams(match, mismatch, gap_p, homo, int(use_kmers), kmer_size, kdist_cutoff,
+                       band, int(vectorized), int(gapless), 0)
+
+
+def extend_err(err, qmax: int) -> np.ndarray:
+    """The error-model extension of dada_uniques_cached (dada.rs:337-384):
+    when the data's highest rounded quality qmax is at or above the model's
+    column count, the 16 x ncol matrix grows to qmax + 1 columns, the new
+    ones repeating the last learned column, with the reference's warning;
+    otherwise it is returned as is."""
+    e = np.ascontiguousarray(err, np.float64)
+    old = e.shape[1]
+    if old == 0 or qmax < old:
+        return e
+    ext = np.concatenate([e, np.repeat(e[:, old - 1:old], qmax + 1 - old, axis=1)], axis=1)
+    import warnings
+    warnings.warn(f"input has quality up to Q{qmax} but the error model covers Q0-Q{old - 1}. "
+                  f"Extending the model by repeating its last column (Q{old - 1}) for "
+                  f"Q{old}-Q{qmax}. Rates for the extended columns are extrapolated, not learned; "
+                  "re-run learn-errors on this data to avoid it.", stacklevel=3)
+    return np.ascontiguousarray(ext)
 
 
 class D2GError(RuntimeError):
@@ -99,6 +122,13 @@ def _lib():
                                                    C.POINTER(AlignParams), C.c_int, C.c_int, u8p,
                                                    C.POINTER(Stats)]),
             "d2g_fetch_results": (C.c_int, [vp, f64p, u32p, u8p]),
+            "d2g_store_reset": (C.c_int, [vp]),
+            "d2g_compare_store": (C.c_int, [vp, u32p, C.c_uint32, C.c_uint32, C.c_double, f64p,
+                                            C.c_uint32, C.POINTER(AlignParams), C.c_int, C.c_int,
+                                            u8p, C.c_uint64, C.POINTER(Comparison), u64p, f64p,
+                                            C.POINTER(Stats)]),
+            "d2g_store_size": (C.c_int, [vp, u64p]),
+            "d2g_store_fetch": (C.c_int, [vp, C.c_uint64, C.c_uint64, C.POINTER(Comparison)]),
             "d2g_align_vectorized": (C.c_int, [vp, C.c_uint32, u8p, u64p, u8p, u64p,
                                                C.POINTER(VecScores), u8p, u8p, u64p, u32p]),
             "d2g_vectorized_cells": (C.c_uint64, [C.c_uint32, C.c_uint32, C.c_int32]),
@@ -164,7 +194,8 @@ def exported_symbols() -> list[str]:
             "d2g_loglik_forward", "d2g_loglik_backward", "d2g_ctx_set_graphs",
             "d2g_ctx_set_screen_shape", "d2g_ctx_set_autotune", "d2g_run_dada",
             "d2g_transition_counts", "d2g_build_trans_mat", "d2g_transition_stats",
-            "d2g_dada_aux", "d2g_birth_subs", "d2g_pairs_lambda_noalloc"]
+            "d2g_dada_aux", "d2g_birth_subs", "d2g_pairs_lambda_noalloc", "d2g_store_reset",
+            "d2g_compare_store", "d2g_store_size", "d2g_store_fetch"]
 
 
 def _p(a: np.ndarray, t):
@@ -216,6 +247,17 @@ class PairStats(C.Structure):
 
     def as_dict(self) -> dict:
         return {name: getattr(self, name) for name, _ in self._fields_ if name != "_pad"}
+
+
+class Comparison(C.Structure):
+    """d2g_comparison: one kept comparison of the store (Comparison,
+    containers.rs:30-51)."""
+    _fields_ = [("index", C.c_uint32), ("cluster", C.c_uint32), ("hamming", C.c_uint32),
+                ("_pad", C.c_uint32), ("lambda", C.c_double)]
+
+
+COMPARISON_DTYPE = np.dtype([("index", "<u4"), ("cluster", "<u4"), ("hamming", "<u4"),
+                             ("_pad", "<u4"), ("lambda", "<f8")])
 
 
 class PairsDev(C.Structure):
@@ -312,6 +354,7 @@ class DadaResult:
     comp_lambda: np.ndarray
     comp_hamming: np.ndarray
     stats: dict
+    err_ncol: int = 41  # columns of the (possibly extended) error matrix of the run
 
 
 @dataclass
@@ -390,6 +433,8 @@ class Context:
                "d2g_load_uniques")
         self.n = len(seqs)
         self.has_quals = q is not None
+        self.max_qual = int(q.max()) if q is not None and q.size else 0
+        self.reads_total = float(int(r.astype(np.uint64).sum()) & 0xFFFFFFFF)  # B.reads, a u32 sum
         self.reads = r
         self.maxlen = int(np.diff(off).max())
 
@@ -425,6 +470,44 @@ class Context:
                "d2g_compare_batch")
         return CompareResult(lam, ham, skp, st.as_dict())
 
+    def store_reset(self):
+        """B::new's store: e_minmax = -999.0 per unique, nothing kept."""
+        _check(_lib().d2g_store_reset(self.h), "d2g_store_reset")
+
+    def compare_store(self, centers, err, first_cluster: int = 0,
+                      params: AlignParams | None = None, use_quals=True, greedy=True, lock=None,
+                      fetch=True):
+        """The compare of `centers` (successive clusters first_cluster, ...)
+        and b_compare_parallel's store (cluster.rs:351-397) on the device.
+        Returns (kept comparisons as a COMPARISON_DTYPE array in push order,
+        or their count without fetch; each center's self lambda; stats)."""
+        params = params or align_params()
+        e, lk = self._common(err, params, lock)
+        cs = np.ascontiguousarray(np.atleast_1d(centers), np.uint32)
+        cap = len(cs) * self.n if fetch else 0
+        out = np.zeros(cap, COMPARISON_DTYPE)
+        nst = np.zeros(1, np.uint64)
+        self_l = np.zeros(len(cs), np.float64)
+        st = Stats()
+        _check(_lib().d2g_compare_store(
+            self.h, _p(cs, u32p), len(cs), int(first_cluster), self.reads_total, _p(e, f64p),
+            e.shape[1], C.byref(params), int(use_quals), int(greedy),
+            _p(lk, u8p) if lk is not None else None, cap,
+            out.ctypes.data_as(C.POINTER(Comparison)) if fetch else None, _p(nst, u64p),
+            _p(self_l, f64p), C.byref(st)), "d2g_compare_store")
+        k = int(nst[0])
+        return (out[:k] if fetch else k), self_l, st.as_dict()
+
+    def store_fetch(self, first: int = 0, count: int | None = None) -> np.ndarray:
+        """Records [first, first + count) of the context's store."""
+        n = np.zeros(1, np.uint64)
+        _check(_lib().d2g_store_size(self.h, _p(n, u64p)), "d2g_store_size")
+        count = int(n[0]) - first if count is None else count
+        out = np.zeros(max(count, 0), COMPARISON_DTYPE)
+        _check(_lib().d2g_store_fetch(self.h, first, count,
+                                      out.ctypes.data_as(C.POINTER(Comparison))), "d2g_store_fetch")
+        return out
+
     def fetch_results(self, n_centers: int) -> CompareResult:
         shape = (n_centers, self.n)
         lam = np.zeros(shape, np.float64)
@@ -458,6 +541,8 @@ class Context:
         params = params or align_params()
         dada = dada or dada_params()
         e = np.ascontiguousarray(err, np.float64)
+        if self.has_quals:
+            e = extend_err(e, self.max_qual)
         n = self.n
         pr = None if prior is None else np.ascontiguousarray(prior, np.uint8)
         if pr is not None and pr.shape != (n,):
@@ -478,7 +563,7 @@ class Context:
                                    _p(ch, u32p), C.byref(st)), "d2g_run_dada")
         clusters = [{name: getattr(infos[i], name) for name, _ in ClusterInfo._fields_}
                     for i in range(ncl.value)]
-        return DadaResult(clusters, cof, pv, mp, cl, ch, st.as_dict())
+        return DadaResult(clusters, cof, pv, mp, cl, ch, st.as_dict(), int(e.shape[1]))
 
     def transition_counts(self, center_of, weight, params: AlignParams | None = None,
                           use_quals: bool = True, ncol: int = 41) -> np.ndarray:
@@ -509,13 +594,14 @@ class Context:
         return out.reshape(16, nq)
 
     def dada_aux(self, result: "DadaResult", params: AlignParams | None = None,
-                 use_quals: bool = True, ncol: int = 41, maxlen: int | None = None) -> dict:
+                 use_quals: bool = True, ncol: int | None = None, maxlen: int | None = None) -> dict:
         """compute_aux's per-raw part (dada.rs:610-631) for a run_dada
         result over the loaded uniques: the transitions, each raw's Sub.nsubs
         against its center (-1 when not counted) and cluster_quality
         (clusters x maxlen, NaN where uncovered; maxlen defaults to the
         longest unique, as the reference's)."""
         params = params or align_params()
+        ncol = result.err_ncol if ncol is None else ncol
         n = self.n
         ncl = len(result.clusters)
         centers = np.ascontiguousarray([c["center"] for c in result.clusters], np.uint32)
@@ -563,11 +649,13 @@ class Context:
         return recs, np.where(nsubs == NO_CENTER, -1, nsubs.astype(np.int64)), qave
 
     def compute_aux(self, result: "DadaResult", params: AlignParams | None = None,
-                    use_quals: bool = True, ncol: int = 41) -> dict:
+                    use_quals: bool = True, ncol: int | None = None) -> dict:
         """compute_aux (dada.rs:610-640) for a run_dada result: DadaAux's
         cluster_stats (error.rs:55-140), cluster_quality, birth_subs records
-        and transitions."""
+        and transitions.  ncol defaults to the run's error-matrix columns
+        (after any extension, as compute_aux reads params.err_ncol)."""
         params = params or align_params()
+        ncol = result.err_ncol if ncol is None else ncol
         a = self.dada_aux(result, params, use_quals, ncol)
         recs, _bn, qave = self.birth_subs(result, params, use_quals)
         reads = np.asarray(self.reads, np.int64)

@@ -8,6 +8,7 @@
 #include <algorithm>
 #include <cmath>
 #include <array>
+#include <cstddef>
 #include <cstring>
 #include <map>
 #include <numeric>
@@ -149,6 +150,18 @@ struct d2g_ctx {
   size_t tr_pin_cap = 0;
   const uint64_t* tr_aoff_for = nullptr;
   uint64_t tr_aoff_n = 0, tr_aoff_stride = 0;
+  // the device store of b_compare_parallel (d2g_store_reset / d2g_compare_store):
+  // e_minmax per raw (caller order) and the kept comparisons, appended
+  bool store_valid = false;
+  uint64_t store_n = 0;
+  DevBuf<uint32_t> inv;  // caller -> sorted
+  DevBuf<double> e_minmax, store_self;
+  ShuffleComp* store = nullptr;
+  size_t store_cap = 0;
+  DevBuf<uint8_t> store_flag;
+  DevBuf<uint32_t> store_bcount;
+  uint8_t* store_pin = nullptr;  // count, self lambdas, the first kept comparisons
+  size_t store_pin_cap = 0;
   // align mode buffers
   DevBuf<uint8_t> a_pool, a_al0, a_al1;
   DevBuf<uint64_t> a_aloff;
@@ -249,6 +262,13 @@ int d2g_ctx_destroy(d2g_ctx* c) {
   for (auto* b : {&c->ll_w, &c->ll_part, &c->ll_out}) b->release();
   c->ll_logE.release();
   for (auto* b : {&c->p_lambda, &c->p_pval, &c->p_err}) b->release();
+  c->inv.release();
+  c->e_minmax.release();
+  c->store_self.release();
+  c->store_flag.release();
+  c->store_bcount.release();
+  if (c->store) (void)hipFree(c->store);
+  if (c->store_pin) (void)hipHostFree(c->store_pin);
   for (int i = 0; i < kEvents; ++i)
     if (c->ev[i]) (void)hipEventDestroy(c->ev[i]);
   if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -398,6 +418,7 @@ int d2g_load_uniques(d2g_ctx* c, uint32_t n, const uint8_t* seq_concat, const ui
   HIP_TRY(c->len.ensure(n));
   HIP_TRY(c->reads.ensure(n));
   HIP_TRY(c->orig.ensure(n));
+  HIP_TRY(c->inv.ensure(n));
   HIP_TRY(c->kmer_row.ensure((size_t)n * u.kwords));
   HIP_TRY(c->kmer_il.ensure((size_t)u.npad * u.kwords));
   HIP_TRY(c->kord_row.ensure((size_t)n * u.owords));
@@ -412,6 +433,7 @@ int d2g_load_uniques(d2g_ctx* c, uint32_t n, const uint8_t* seq_concat, const ui
   HIP_TRY(hipMemcpyAsync(c->len.p, slen.data(), n * sizeof(uint32_t), hipMemcpyHostToDevice, s));
   HIP_TRY(hipMemcpyAsync(c->reads.p, sreads.data(), n * sizeof(uint32_t), hipMemcpyHostToDevice, s));
   HIP_TRY(hipMemcpyAsync(c->orig.p, order.data(), n * sizeof(uint32_t), hipMemcpyHostToDevice, s));
+  HIP_TRY(hipMemcpyAsync(c->inv.p, inv.data(), n * sizeof(uint32_t), hipMemcpyHostToDevice, s));
   HIP_TRY(launch_zero(c->kord_row.p, (size_t)n * u.owords, s));
   HIP_TRY(launch_zero(c->kord_il.p, (size_t)u.npad * u.owords, s));
   HIP_TRY(launch_zero(c->kmer_il.p, (size_t)u.npad * u.kwords, s));
@@ -439,6 +461,7 @@ int d2g_load_uniques(d2g_ctx* c, uint32_t n, const uint8_t* seq_concat, const ui
   c->has_qual = qual_concat != nullptr;
   c->loaded = true;
   c->cmp_valid = false;
+  c->store_valid = false;  // the store belongs to the previous uniques
   if (c->graph_exec) (void)hipGraphExecDestroy(c->graph_exec);  // unique buffers changed
   c->graph_exec = nullptr;
   c->graph_key.clear();
@@ -1007,6 +1030,138 @@ int d2g_compare_center(d2g_ctx* c, uint32_t center, const double* err, uint32_t 
                        d2g_stats* stats) {
   return d2g_compare_batch(c, &center, 1, err, ncol, p, use_quals, greedy, lock, out_lambda,
                            out_hamming, out_skipped, stats);
+}
+
+// ---- the device store ------------------------------------------------------
+static_assert(sizeof(d2g_comparison) == sizeof(ShuffleComp), "store record layout");
+static_assert(offsetof(d2g_comparison, index) == offsetof(ShuffleComp, raw), "store record layout");
+static_assert(offsetof(d2g_comparison, cluster) == offsetof(ShuffleComp, cl), "store record layout");
+static_assert(offsetof(d2g_comparison, hamming) == offsetof(ShuffleComp, ham), "store record layout");
+static_assert(offsetof(d2g_comparison, lambda) == offsetof(ShuffleComp, lambda), "store record layout");
+
+int d2g_store_reset(d2g_ctx* c) {
+  if (!c) return D2G_ERR_ARG;
+  if (!c->loaded) return D2G_ERR_STATE;
+  (void)hipSetDevice(c->device);
+  HIP_TRY(c->e_minmax.ensure(c->u.n));
+  HIP_TRY(launch_store_reset(c->e_minmax.p, c->u.n, c->stream));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  c->store_n = 0;
+  c->store_valid = true;
+  return D2G_OK;
+}
+
+// Room for `need` records, keeping the ones stored so far.
+static hipError_t store_reserve(d2g_ctx* c, size_t need) {
+  if (c->store && need <= c->store_cap) return hipSuccess;
+  const size_t cap = std::max(need, 2 * c->store_cap + 4096);
+  ShuffleComp* q = nullptr;
+  hipError_t e = hipMalloc((void**)&q, cap * sizeof(ShuffleComp));
+  if (e != hipSuccess) return e;
+  if (c->store_n) {
+    if ((e = hipMemcpyAsync(q, c->store, c->store_n * sizeof(ShuffleComp), hipMemcpyDeviceToDevice,
+                            c->stream)) != hipSuccess ||
+        (e = hipStreamSynchronize(c->stream)) != hipSuccess) {
+      (void)hipFree(q);
+      return e;
+    }
+  }
+  if (c->store) (void)hipFree(c->store);
+  c->store = q;
+  c->store_cap = cap;
+  return hipSuccess;
+}
+
+int d2g_compare_store(d2g_ctx* c, const uint32_t* centers, uint32_t nc, uint32_t first_cluster,
+                      double total_reads, const double* err, uint32_t ncol,
+                      const d2g_align_params* p, int use_quals, int greedy, const uint8_t* lock,
+                      uint64_t capacity, d2g_comparison* out_stored, uint64_t* out_nstored,
+                      double* out_self_lambda, d2g_stats* stats) {
+  if (!c || !out_nstored || nc == 0) return D2G_ERR_ARG;
+  *out_nstored = 0;
+  if (!c->loaded || !c->store_valid) return D2G_ERR_STATE;
+  const uint32_t n = c->u.n;
+  const size_t npairs = (size_t)nc * n;
+  // grid rows per center; the shuffle addresses the store with 32-bit indices
+  if (nc > 65535u || c->store_n + npairs >= ((uint64_t)1 << 32)) return D2G_ERR_UNSUPPORTED;
+  const int rc = compare_device(c, centers, nc, err, ncol, p, use_quals, greedy, lock, stats);
+  if (rc != D2G_OK) return rc;
+  hipStream_t s = c->stream;
+  const uint32_t nrb = store_blocks(n);
+  const size_t m = (size_t)nrb * nc;
+  HIP_TRY(store_reserve(c, c->store_n + npairs));
+  HIP_TRY(c->store_flag.ensure(npairs));
+  HIP_TRY(c->store_bcount.ensure(m + 1));
+  HIP_TRY(c->store_self.ensure(nc));
+  StoreArgs sa;
+  sa.n = n;
+  sa.nc = nc;
+  sa.inv = c->inv.p;
+  sa.centers = c->centers.p;
+  sa.reads = c->u.reads;
+  sa.lam = c->out_lambda.p;
+  sa.ham = c->out_hamming.p;
+  sa.total = total_reads;
+  sa.e_minmax = c->e_minmax.p;
+  sa.flag = c->store_flag.p;
+  sa.bcount = c->store_bcount.p;
+  sa.self_out = c->store_self.p;
+  sa.store = c->store + c->store_n;
+  sa.first_cluster = first_cluster;
+  HIP_TRY(launch_store(sa, s));
+  // one synchronisation for the count, the self lambdas and the first
+  // kFirst kept comparisons; a longer list takes a second copy
+  constexpr size_t kFirst = 4096;
+  const size_t first = out_stored ? std::min<size_t>({(size_t)capacity, kFirst, npairs}) : 0;
+  const size_t self_off = 8, rec_off = self_off + (size_t)nc * 8;
+  const size_t pin_need = rec_off + first * sizeof(ShuffleComp);
+  if (c->store_pin_cap < pin_need) {
+    if (c->store_pin) (void)hipHostFree(c->store_pin);
+    c->store_pin = nullptr;
+    c->store_pin_cap = 0;
+    HIP_TRY(hipHostMalloc((void**)&c->store_pin, std::max<size_t>(pin_need, 1 << 16),
+                          hipHostMallocDefault));
+    c->store_pin_cap = std::max<size_t>(pin_need, 1 << 16);
+  }
+  HIP_TRY(hipMemcpyAsync(c->store_pin, c->store_bcount.p + m, 4, hipMemcpyDeviceToHost, s));
+  HIP_TRY(hipMemcpyAsync(c->store_pin + self_off, c->store_self.p, (size_t)nc * 8,
+                         hipMemcpyDeviceToHost, s));
+  if (first)
+    HIP_TRY(hipMemcpyAsync(c->store_pin + rec_off, c->store + c->store_n,
+                           first * sizeof(ShuffleComp), hipMemcpyDeviceToHost, s));
+  HIP_TRY(hipStreamSynchronize(s));
+  uint32_t cnt;
+  std::memcpy(&cnt, c->store_pin, 4);
+  if (out_self_lambda) std::memcpy(out_self_lambda, c->store_pin + self_off, (size_t)nc * 8);
+  const uint64_t at = c->store_n;
+  c->store_n += cnt;
+  *out_nstored = cnt;
+  if (!out_stored) return D2G_OK;
+  if (cnt > capacity) return D2G_ERR_ARG;  // kept on the device: d2g_store_fetch
+  std::memcpy(out_stored, c->store_pin + rec_off, std::min<size_t>(cnt, first) * sizeof(ShuffleComp));
+  if (cnt > first)
+    HIP_TRY(hipMemcpy(out_stored + first, c->store + at + first, (cnt - first) * sizeof(ShuffleComp),
+                      hipMemcpyDeviceToHost));
+  return D2G_OK;
+}
+
+int d2g_store_size(d2g_ctx* c, uint64_t* out_count) {
+  if (!c || !out_count) return D2G_ERR_ARG;
+  if (!c->store_valid) return D2G_ERR_STATE;
+  *out_count = c->store_n;
+  return D2G_OK;
+}
+
+int d2g_store_fetch(d2g_ctx* c, uint64_t first, uint64_t count, d2g_comparison* out) {
+  if (!c || (count && !out)) return D2G_ERR_ARG;
+  if (!c->store_valid) return D2G_ERR_STATE;
+  if (first > c->store_n || count > c->store_n - first) return D2G_ERR_ARG;
+  if (count == 0) return D2G_OK;
+  (void)hipSetDevice(c->device);
+  HIP_TRY(hipMemcpyAsync(out, c->store + first, count * sizeof(ShuffleComp), hipMemcpyDeviceToHost,
+                         c->stream));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  return D2G_OK;
 }
 
 // The transition-by-quality counts of a finished partition: each counted
@@ -2001,6 +2156,13 @@ int d2g_pairs_fetch(d2g_ctx* c, float* out_loglam32, double* out_lambda, double*
 namespace d2g {
 const std::vector<uint32_t>* ctx_host_reads(const d2g_ctx* c) {
   return (c && c->loaded) ? &c->h_reads : nullptr;
+}
+
+bool ctx_store(const d2g_ctx* c, const ShuffleComp** comps, uint64_t* count) {
+  if (!c || !c->store_valid) return false;
+  *comps = c->store;
+  *count = c->store_n;
+  return true;
 }
 
 bool ctx_exec(const d2g_ctx* c, CtxExec* out) {

@@ -234,6 +234,29 @@ struct ShuffleArgs {
   uint32_t* where;           // per raw: its cluster (centers set by the pass)
   ShuffleMove* moves;        // slot 0 holds the count in `r`; movers from slot 1
 };
+// ---- store.hip ---------------------------------------------------------
+// The store of b_compare_parallel (cluster.rs:351-397) over the last
+// compare's results: nc centers acting as successive clusters.
+struct StoreArgs {
+  uint32_t n;               // raws
+  uint32_t nc;              // centers of the compare
+  const uint32_t* inv;      // caller -> sorted unique index
+  const uint32_t* centers;  // sorted indices of the centers
+  const uint32_t* reads;    // sorted order
+  const double* lam;        // [nc][n], sorted order (the compare's results)
+  const uint32_t* ham;      // [nc][n], sorted order
+  double total;             // B.reads as f64
+  double* e_minmax;         // [n] caller order, resident
+  uint8_t* flag;            // [nc][n] caller order
+  uint32_t* bcount;         // [nc][store_blocks(n)] + 1: counts, then offsets and the total
+  double* self_out;         // [nc]: each center's lambda against itself
+  ShuffleComp* store;       // where this call's first kept comparison goes
+  uint32_t first_cluster;   // cluster of center 0
+};
+uint32_t store_blocks(uint32_t n);
+hipError_t launch_store(const StoreArgs& a, hipStream_t s);
+hipError_t launch_store_reset(double* e_minmax, uint32_t n, hipStream_t s);
+
 hipError_t launch_shuffle_init(const ShuffleArgs& a, hipStream_t s);  // where = 0, scratch reset
 hipError_t launch_shuffle_pass(const ShuffleArgs& a, hipStream_t s);
 
@@ -303,6 +326,8 @@ struct CtxExec {
   hipEvent_t ev[3];
 };
 bool ctx_exec(const d2g_ctx* c, CtxExec* out);
+// The context's resident store of kept comparisons (d2g_compare_store).
+bool ctx_store(const d2g_ctx* c, const ShuffleComp** comps, uint64_t* count);
 // Once per device, with that device current: the address of the
 // allocation-free entry point's device array and its constant table.
 int noalloc_setup(int device);

@@ -17,7 +17,7 @@
 // exp(log lambda): the sentinels (singleton, hamming 0, lambda 0), the
 // Poisson upper tail P(X >= reads) = P(reads, E) (regularised lower
 // incomplete gamma, E = lambda * center reads), the reference's direct
-// series when that underflows (pval.rs:329-353), and the 1 - e^{-E}
+// series where that returns 0 (pval.rs:329-353), and the 1 - e^{-E}
 // conditioning with its Taylor replacement below 1e-7.  Pairs whose p-value
 // lands within a factor 10 of omega_a are appended to a list instead.
 //
@@ -32,12 +32,12 @@
 // owning lanes (0..15) patch their substitution positions, and each owning
 // lane runs its chain from its row.
 //
-// The incomplete gamma is evaluated with the series (x < a + 1) or the
-// modified-Lentz continued fraction (Numerical Recipes, 3rd ed., 6.2), using
-// the device lgamma/exp/log.
+// The incomplete gamma is statrs 0.18's gamma_lr (d2g_statrs.h) with the
+// device exp/log/sin.
 #include <float.h>
 
 #include "d2g_launch.h"
+#include "d2g_statrs.h"
 
 namespace d2g {
 
@@ -162,37 +162,6 @@ __device__ __forceinline__ void patch_pairs(const PairDev& a, const PairRow& me,
   }
 }
 
-// Regularised lower incomplete gamma P(a, x).
-__device__ double gamma_p(double a, double x) {
-  if (x <= 0.0) return 0.0;
-  const double gln = lgamma(a);
-  if (x < a + 1.0) {
-    double ap = a, del = 1.0 / a, sum = del;
-    for (int n = 0; n < 100000; ++n) {
-      ap += 1.0;
-      del *= x / ap;
-      sum += del;
-      if (fabs(del) < fabs(sum) * DBL_EPSILON) break;
-    }
-    return sum * exp(-x + a * log(x) - gln);
-  }
-  const double fpmin = DBL_MIN / DBL_EPSILON;
-  double bb = x + 1.0 - a, c = 1.0 / fpmin, d = 1.0 / bb, h = d;
-  for (int i = 1; i < 100000; ++i) {
-    const double an = -i * (i - a);
-    bb += 2.0;
-    d = an * d + bb;
-    if (fabs(d) < fpmin) d = fpmin;
-    c = bb + an / c;
-    if (fabs(c) < fpmin) c = fpmin;
-    d = 1.0 / d;
-    const double del = d * c;
-    h *= del;
-    if (fabs(del - 1.0) <= DBL_EPSILON) break;
-  }
-  return 1.0 - exp(-x + a * log(x) - gln) * h;
-}
-
 // poisson_upper_tail_direct, pval.rs:329-353.  log_kfact[r] = sum_{i=1..r}
 // log(i), the reference's running sum, tabulated on the host in the same
 // order.
@@ -219,7 +188,7 @@ __device__ double get_pA_dev(uint32_t reads, double lambda, uint32_t hamming, ui
   if (lambda == 0.0) return 0.0;
   const double e = lambda * (double)bi_reads;
   if (!(e > 0.0)) return 1.0;
-  double p = gamma_p((double)reads, e);
+  double p = statrs_gamma_lr((double)reads, e);
   if (p == 0.0) p = poisson_tail_direct(reads, e, log_kfact);
   double norm = 1.0 - exp(-e);
   if (norm < 1e-7) norm = e - 0.5 * e * e;

@@ -1,6 +1,7 @@
 // partition.hip — run_dada's partition loop (dada.rs:749-1150) and the final
 // per-raw p-value pass and read map (dada.rs:502-533), on the host, with the
-// compare of each new cluster center on the device (d2g_compare_center).
+// compare and the store of each new cluster center on the device
+// (d2g_compare_store).
 // Host code only; it links against the rest of the library through the C
 // entry points.
 //
@@ -10,12 +11,12 @@
 //     the center is the raw with the most reads, the last one among equals
 //     (assign_center, containers.rs:450-463: Rust's max_by_key keeps the
 //     last maximum).
-//   * b_compare (cluster.rs:106-164): the device returns each raw's lambda
-//     and hamming; the store is serial: a comparison is kept when
-//     lambda * B.reads > e_minmax[raw], e_minmax rises to lambda *
-//     center_reads, and Raw.comp follows it for cluster 0 and for the
-//     center itself.  Cluster 0's compare uses kdist_cutoff 1.0
-//     (dada.rs:835-838).
+//   * b_compare_parallel (cluster.rs:239-397): the device compares and
+//     stores: a comparison is kept when lambda * B.reads > e_minmax[raw],
+//     e_minmax (resident on the device) rises to lambda * center_reads, and
+//     only the kept comparisons come back, in raw order; Raw.comp follows
+//     them for cluster 0 and for the center itself.  Cluster 0's compare
+//     uses kdist_cutoff 1.0 (dada.rs:835-838).
 //   * b_p_update (pval.rs:45-130): clusters flagged update_e reprice their
 //     members with get_pA (pval.rs:376-398); with greedy, a cluster's first
 //     p-update locks the members its center explains and the center.
@@ -30,11 +31,10 @@
 //     MAX_SHUFFLE iteration on a reconcile (cluster.rs:1037, 1066-1073), so
 //     the first call makes up to max_shuffle move passes and later calls up
 //     to max_shuffle - 1.
-//   * calc_pA (pval.rs:208-241): statrs's Poisson survival function is the
-//     regularised lower incomplete gamma P(reads, e), evaluated by the
-//     series / Lentz continued fraction of Numerical Recipes (3rd ed.,
-//     section 6.2), with the reference's own small-lambda series when it
-//     returns 0 (pval.rs:329-353).
+//   * calc_pA (pval.rs:208-241): statrs's Poisson survival function is
+//     gamma_lr(reads, e), restated from statrs 0.18 (d2g_statrs.h), with the
+//     reference's own small-lambda series where it returns 0
+//     (pval.rs:329-353).
 #include <sched.h>
 
 #include <algorithm>
@@ -51,6 +51,7 @@
 
 #include "../../include/dada2_amd.h"
 #include "d2g_launch.h"
+#include "d2g_statrs.h"
 
 namespace d2g {
 const std::vector<uint32_t>* ctx_host_reads(const d2g_ctx* c);
@@ -186,8 +187,7 @@ struct Cluster {
 struct Part {
   std::vector<Raw> raws;
   std::vector<Cluster> cl;
-  std::vector<double> e_minmax;
-  uint32_t reads = 0;
+  uint32_t reads = 0;  // e_minmax lives on the device (d2g_compare_store)
   uint64_t nalign = 0, nshroud = 0;
   // each raw's cluster and position in it (the raw_cluster map of
   // containers.rs:302-316, plus the position)
@@ -223,37 +223,6 @@ struct Part {
 
 // ---- p-values (pval.rs) ----------------------------------------------------
 
-// Regularised lower incomplete gamma P(a, x).
-double gamma_p(double a, double x) {
-  if (x <= 0.0) return 0.0;
-  const double gln = std::lgamma(a);
-  if (x < a + 1.0) {
-    double ap = a, del = 1.0 / a, sum = del;
-    for (int n = 0; n < 100000; ++n) {
-      ap += 1.0;
-      del *= x / ap;
-      sum += del;
-      if (std::fabs(del) < std::fabs(sum) * DBL_EPSILON) break;
-    }
-    return sum * std::exp(-x + a * std::log(x) - gln);
-  }
-  const double fpmin = DBL_MIN / DBL_EPSILON;
-  double bb = x + 1.0 - a, c = 1.0 / fpmin, d = 1.0 / bb, h = d;
-  for (int i = 1; i < 100000; ++i) {
-    const double an = -i * (i - a);
-    bb += 2.0;
-    d = an * d + bb;
-    if (std::fabs(d) < fpmin) d = fpmin;
-    c = bb + an / c;
-    if (std::fabs(c) < fpmin) c = fpmin;
-    d = 1.0 / d;
-    const double del = d * c;
-    h *= del;
-    if (std::fabs(del - 1.0) <= DBL_EPSILON) break;
-  }
-  return 1.0 - std::exp(-x + a * std::log(x) - gln) * h;
-}
-
 // poisson_upper_tail_direct, pval.rs:329-353
 double poisson_tail_direct(uint32_t reads, double lambda) {
   const double k = (double)reads;
@@ -275,7 +244,7 @@ double poisson_tail_direct(uint32_t reads, double lambda) {
 // calc_pA, pval.rs:208-241
 double calc_pA(uint32_t reads, double e_reads, bool prior) {
   if (!(e_reads > 0.0)) return 1.0;  // Poisson::new rejects e_reads <= 0 and NaN
-  double pval = gamma_p((double)reads, e_reads);  // sf(reads - 1)
+  double pval = d2g::statrs_gamma_lr((double)reads, e_reads);  // sf(reads - 1)
   if (pval == 0.0) pval = poisson_tail_direct(reads, e_reads);
   if (prior) return pval;
   double norm = 1.0 - std::exp(-e_reads);
@@ -300,30 +269,28 @@ double get_pA(uint32_t reads, bool prior, double lambda, uint32_t hamming, uint3
 // (one copy each way per pass, one per compare).
 struct DevShuffle {
   hipStream_t s = nullptr;
-  uint32_t n = 0, n_comp = 0, cap = 0, ncl_cap = 0;
-  d2g::ShuffleComp* comps = nullptr;
+  const d2g_ctx* ctx = nullptr;  // its store holds the kept comparisons
+  uint32_t n = 0, ncl_cap = 0;
   uint8_t* cl_info = nullptr;  // creads (f64 x ncl_cap), then ccenter (u32)
   unsigned long long *best_bits = nullptr, *best_key = nullptr;
   uint32_t* where = nullptr;
   d2g::ShuffleMove* moves = nullptr;  // slot 0: the count
-  d2g::ShuffleComp* comp_pin = nullptr;
   uint8_t* pass_pin = nullptr;
   d2g::ShuffleMove* moves_pin = nullptr;
-  size_t comp_pin_cap = 0;
   static constexpr uint32_t kFirst = 1024;  // moves read back with the count
 
   ~DevShuffle() {
     if (s) (void)hipStreamSynchronize(s);
-    for (void* q : {(void*)comps, (void*)cl_info, (void*)best_bits, (void*)best_key,
-                    (void*)where, (void*)moves})
+    for (void* q : {(void*)cl_info, (void*)best_bits, (void*)best_key, (void*)where, (void*)moves})
       if (q) (void)hipFree(q);
-    for (void* q : {(void*)comp_pin, (void*)pass_pin, (void*)moves_pin})
+    for (void* q : {(void*)pass_pin, (void*)moves_pin})
       if (q) (void)hipHostFree(q);
   }
 
   static bool ok(hipError_t e) { return e == hipSuccess; }
 
-  bool init(hipStream_t stream, uint32_t nraw) {
+  bool init(const d2g_ctx* c, hipStream_t stream, uint32_t nraw) {
+    ctx = c;
     s = stream;
     n = nraw;
     if (!ok(hipMalloc((void**)&best_bits, (size_t)n * 8)) ||
@@ -333,13 +300,13 @@ struct DevShuffle {
         !ok(hipHostMalloc((void**)&moves_pin, (kFirst + 1) * sizeof(d2g::ShuffleMove),
                           hipHostMallocDefault)))
       return false;
-    return ok(d2g::launch_shuffle_init(args(0), s));
+    return ok(d2g::launch_shuffle_init(args(0, nullptr, 0), s));
   }
 
-  d2g::ShuffleArgs args(uint32_t ncl) const {
+  d2g::ShuffleArgs args(uint32_t ncl, const d2g::ShuffleComp* comps, uint64_t n_comp) const {
     d2g::ShuffleArgs a;
     a.comps = comps;
-    a.n_comp = n_comp;
+    a.n_comp = (uint32_t)n_comp;  // d2g_compare_store keeps the store below 2^32 records
     a.creads = reinterpret_cast<const double*>(cl_info);
     a.ccenter = cl_info ? reinterpret_cast<const uint32_t*>(cl_info + (size_t)ncl_cap * 8) : nullptr;
     a.n_cl = ncl;
@@ -349,40 +316,6 @@ struct DevShuffle {
     a.where = where;
     a.moves = moves;
     return a;
-  }
-
-  // Cluster `cl`'s stored comparisons.  The previous copy from the staging
-  // area has finished: every pass and compare ends with a synchronisation.
-  bool append(const std::vector<Comp>& cs, uint32_t cl) {
-    const uint32_t k = (uint32_t)cs.size();
-    if (k == 0) return true;
-    if (n_comp + k > cap) {
-      const uint32_t nc = std::max<uint32_t>(n_comp + k, 2 * cap + 4096);
-      d2g::ShuffleComp* q = nullptr;
-      if (!ok(hipMalloc((void**)&q, (size_t)nc * sizeof(d2g::ShuffleComp)))) return false;
-      if (n_comp && (!ok(hipMemcpyAsync(q, comps, (size_t)n_comp * sizeof(d2g::ShuffleComp),
-                                        hipMemcpyDeviceToDevice, s)) ||
-                     !ok(hipStreamSynchronize(s))))
-        return false;
-      if (comps) (void)hipFree(comps);
-      comps = q;
-      cap = nc;
-    }
-    if (comp_pin_cap < k) {
-      if (comp_pin) (void)hipHostFree(comp_pin);
-      comp_pin = nullptr;
-      if (!ok(hipHostMalloc((void**)&comp_pin, 2 * (size_t)k * sizeof(d2g::ShuffleComp),
-                            hipHostMallocDefault)))
-        return false;
-      comp_pin_cap = 2 * (size_t)k;
-    }
-    for (uint32_t j = 0; j < k; ++j)
-      comp_pin[j] = d2g::ShuffleComp{cs[j].index, cl, cs[j].hamming, 0u, cs[j].lambda};
-    if (!ok(hipMemcpyAsync(comps + n_comp, comp_pin, (size_t)k * sizeof(d2g::ShuffleComp),
-                           hipMemcpyHostToDevice, s)))
-      return false;
-    n_comp += k;
-    return true;
   }
 
   // One pass: the scan on the device, the moves read back (unordered).
@@ -406,9 +339,12 @@ struct DevShuffle {
       hc[ci] = B.cl[ci].center >= 0 ? (uint32_t)B.cl[ci].center : 0xFFFFFFFFu;
     }
     const uint32_t first = std::min(n, kFirst) + 1;
+    const d2g::ShuffleComp* comps = nullptr;
+    uint64_t n_comp = 0;
+    if (!d2g::ctx_store(ctx, &comps, &n_comp)) return false;
     if (!ok(hipMemcpyAsync(cl_info, pass_pin, (size_t)ncl_cap * 8 + (size_t)ncl * 4,
                            hipMemcpyHostToDevice, s)) ||
-        !ok(d2g::launch_shuffle_pass(args(ncl), s)) ||
+        !ok(d2g::launch_shuffle_pass(args(ncl, comps, n_comp), s)) ||
         !ok(hipMemcpyAsync(moves_pin, moves, (size_t)first * sizeof(d2g::ShuffleMove),
                            hipMemcpyDeviceToHost, s)) ||
         !ok(hipStreamSynchronize(s)))
@@ -456,15 +392,11 @@ struct Run {
   d2g_align_params align;
   d2g_dada_params dp;
   Part B;
-  Pinned<double> lam;
-  Pinned<uint32_t> ham;
-  Pinned<uint8_t> skp;
+  Pinned<d2g_comparison> stored;  // one compare's kept comparisons (at most one per raw)
   std::vector<uint8_t> lock;
   std::vector<std::vector<Cand>> cand;  // per raw
   Pool* pool = nullptr;
-  // per-chunk scratch of the store and the bud scan
-  std::vector<std::vector<Comp>> store_comps;
-  std::vector<uint64_t> store_counts;
+  // per-chunk scratch of the bud scan
   std::vector<size_t> bud_off;
   d2g_dada_stats st{};
   DevShuffle* dev = nullptr;  // the device scan, when asked for
@@ -474,18 +406,21 @@ struct Run {
     return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
   }
 
-  // b_compare, cluster.rs:106-164
+  // b_compare_parallel, cluster.rs:239-397: the map and the store on the
+  // device (d2g_compare_store); the kept comparisons come back in raw order
   int compare(size_t i) {
     const uint32_t n = (uint32_t)B.raws.size();
     const uint32_t center = (uint32_t)B.cl[i].center;
-    const uint32_t center_reads = B.raws[center].reads;
     for (uint32_t r = 0; r < n; ++r) lock[r] = B.raws[r].lock ? 1 : 0;
     d2g_align_params p = align;
     if (i == 0) p.kdist_cutoff = 1.0;  // dada.rs:835-838
     auto t0 = std::chrono::steady_clock::now();
     d2g_stats cs{};
-    const int rc = d2g_compare_center(ctx, center, err, ncol, &p, dp.use_quals, dp.greedy,
-                                      lock.data(), lam.data(), ham.data(), skp.data(), &cs);
+    uint64_t nst = 0;
+    double self = 0.0;
+    const int rc = d2g_compare_store(ctx, &center, 1, (uint32_t)i, (double)B.reads, err, ncol, &p,
+                                     dp.use_quals, dp.greedy, lock.data(), stored.n, stored.data(),
+                                     &nst, &self, &cs);
     st.ms_compare += ms_since(t0);
     ++st.compares;
     if (rc != D2G_OK) return rc;
@@ -495,47 +430,21 @@ struct Run {
     st.ms_dev_dp += cs.ms_dp;
     st.ms_dev_lambda += cs.ms_lambda;
     t0 = std::chrono::steady_clock::now();
-    // every raw's step is its own; the cluster's comp vector is the chunks'
-    // lists concatenated in raw order
-    constexpr size_t kChunk = 4096;
-    const double total = (double)B.reads;
+    B.nalign += cs.comparisons - cs.skipped;
+    B.nshroud += cs.shrouded;
     Cluster& C = B.cl[i];
-    const size_t nch = (n + kChunk - 1) / kChunk;
-    store_comps.resize(nch);
-    store_counts.assign(2 * nch, 0);
-    pool->run(n, kChunk, [&](size_t b, size_t e) {
-      std::vector<Comp>& out = store_comps[b / kChunk];
-      out.clear();
-      uint64_t na = 0, ns = 0;
-      for (size_t r = b; r < e; ++r) {
-        if (!skp[r]) {
-          ++na;
-          if (ham[r] == D2G_HAMMING_NONE) ++ns;
-        }
-        const double lambda = lam[r];
-        if (lambda * total > B.e_minmax[r]) {
-          const double ex = lambda * (double)center_reads;
-          if (ex > B.e_minmax[r]) B.e_minmax[r] = ex;
-          Comp c;
-          c.i = (uint32_t)i;
-          c.index = (uint32_t)r;
-          c.lambda = lambda;
-          c.hamming = ham[r] == D2G_HAMMING_NONE ? 0u : ham[r];
-          out.push_back(c);
-          if (!dev) cand[r].push_back(Cand{c.i, c.hamming, c.lambda});
-          if (i == 0 || r == center) B.raws[r].comp = c;
-        }
-      }
-      store_counts[2 * (b / kChunk)] = na;
-      store_counts[2 * (b / kChunk) + 1] = ns;
-    });
-    C.self = lam[center];
-    for (size_t k = 0; k < nch; ++k) {
-      B.nalign += store_counts[2 * k];
-      B.nshroud += store_counts[2 * k + 1];
-      C.comp.insert(C.comp.end(), store_comps[k].begin(), store_comps[k].end());
+    C.self = self;
+    C.comp.resize((size_t)nst);
+    for (size_t k = 0; k < (size_t)nst; ++k) {
+      const d2g_comparison& d = stored[k];
+      Comp& c = C.comp[k];
+      c.i = d.cluster;
+      c.index = d.index;
+      c.lambda = d.lambda;
+      c.hamming = d.hamming;
+      if (!dev) cand[d.index].push_back(Cand{c.i, c.hamming, c.lambda});
+      if (i == 0 || d.index == center) B.raws[d.index].comp = c;
     }
-    if (dev && !dev->append(C.comp, (uint32_t)i)) return D2G_ERR_HIP;
     st.ms_store += ms_since(t0);
     return D2G_OK;
   }
@@ -786,18 +695,19 @@ extern "C" int d2g_run_dada(d2g_ctx* ctx, const double* err, uint32_t ncol,
     B.raws[r].prior = prior ? prior[r] != 0 : false;
     B.reads += (*reads)[r];
   }
-  B.e_minmax.assign(n, -999.0);
   B.where.assign(n, 0);
   B.at.resize(n);
   for (uint32_t r = 0; r < n; ++r) B.at[r] = r;
-  if (!run.lam.resize(n) || !run.ham.resize(n) || !run.skp.resize(n)) return D2G_ERR_HIP;
+  if (!run.stored.resize(n)) return D2G_ERR_HIP;
+  int rc = d2g_store_reset(ctx);  // B::new's e_minmax (containers.rs:342)
+  if (rc != D2G_OK) return rc;
   run.lock.resize(n);
   run.cand.resize(n);
   Pool pool(pool_threads());
   run.pool = &pool;
   DevShuffle dev;
   if (dada->device_shuffle) {
-    if (!dev.init((hipStream_t)d2g_ctx_stream(ctx), n)) return D2G_ERR_HIP;
+    if (!dev.init(ctx, (hipStream_t)d2g_ctx_stream(ctx), n)) return D2G_ERR_HIP;
     run.dev = &dev;
   }
   // init, containers.rs:363-391
@@ -815,7 +725,7 @@ extern "C" int d2g_run_dada(d2g_ctx* ctx, const double* err, uint32_t ncol,
     B.assign_center(0);
   }
   // run_dada, dada.rs:822-1000
-  int rc = run.compare(0);
+  rc = run.compare(0);
   if (rc != D2G_OK) return rc;
   run.p_update();
   const size_t max_clust = dada->max_clust == 0 ? (size_t)n : (size_t)dada->max_clust;

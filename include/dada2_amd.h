@@ -153,6 +153,45 @@ int d2g_compare_batch_device(d2g_ctx* ctx, const uint32_t* centers, uint32_t n_c
 int d2g_fetch_results(d2g_ctx* ctx, double* out_lambda, uint32_t* out_hamming,
                       uint8_t* out_skipped);
 
+/* ---- the store on the device ---------------------------------------------
+ * Replaces the serial store of b_compare_parallel (cluster.rs:351-397): a
+ * compare keeps comparison (i, index) when lambda * B.reads > e_minmax[index]
+ * (strict), raises e_minmax[index] to lambda * center reads when that is
+ * larger, and pushes Comparison { i, index, lambda, hamming (0 when
+ * shrouded) } onto cluster i's comp vector in raw-index order.  e_minmax and
+ * the kept comparisons stay resident; only the kept ones come back.
+ * A record of the store: Comparison (containers.rs:30-51). */
+typedef struct d2g_comparison {
+  uint32_t index;    /* raw (unique, load order) */
+  uint32_t cluster;  /* Comparison.i */
+  uint32_t hamming;  /* 0 when shrouded (cluster.rs:389) */
+  uint32_t _pad;
+  double lambda;
+} d2g_comparison;
+
+/* B::new's store (containers.rs:342): e_minmax = -999.0 for every loaded
+ * unique, no kept comparisons.  Needed after each d2g_load_uniques. */
+int d2g_store_reset(d2g_ctx* ctx);
+
+/* The compare of centers[0 .. n_centers) (as d2g_compare_batch) followed by
+ * the store, the centers acting as successive clusters first_cluster,
+ * first_cluster + 1, ... (one b_compare_parallel per center, in order).
+ * total_reads is B.reads as f64.  *out_nstored receives the number kept by
+ * this call; they are appended to the context's store and, when out_stored
+ * is not null, copied to out_stored (capacity records; D2G_ERR_ARG when
+ * more were kept, which stay on the device for d2g_store_fetch).
+ * out_self_lambda (nullable, n_centers) receives each center's lambda
+ * against itself (Bi.self_, cluster.rs:372-374).  B.nalign / B.nshroud grow
+ * by stats->comparisons - stats->skipped and stats->shrouded. */
+int d2g_compare_store(d2g_ctx* ctx, const uint32_t* centers, uint32_t n_centers,
+                      uint32_t first_cluster, double total_reads, const double* err,
+                      uint32_t ncol, const d2g_align_params* params, int use_quals, int greedy,
+                      const uint8_t* lock, uint64_t capacity, d2g_comparison* out_stored,
+                      uint64_t* out_nstored, double* out_self_lambda, d2g_stats* stats);
+/* Kept comparisons so far, and a copy of records [first, first + count). */
+int d2g_store_size(d2g_ctx* ctx, uint64_t* out_count);
+int d2g_store_fetch(d2g_ctx* ctx, uint64_t first, uint64_t count, d2g_comparison* out);
+
 /* ---- the partition loop --------------------------------------------------
  * Replaces run_dada (dada.rs:749-1150) plus the final per-raw p-value pass
  * and read map (dada.rs:502-533), over the loaded uniques (raw index = load

@@ -118,6 +118,8 @@ def lib():
             "orc_get_pA": (C.c_double, [C.c_uint32, C.c_int, C.c_double, C.c_uint32, C.c_uint32,
                                         C.c_int]),
             "orc_gamma_q": (C.c_double, [C.c_double, C.c_double]),
+            "orc_ln_gamma": (C.c_double, [C.c_double]),
+            "orc_gamma_lr": (C.c_double, [C.c_double, C.c_double]),
             "orc_compare_center": (C.c_int, [C.POINTER(Raw), C.c_uint32, C.c_uint32, f64p,
                                              C.c_uint32, C.POINTER(AlignParams), C.c_int, C.c_int,
                                              f64p, u32p, u8p, C.c_int, C.c_int, u64p, u64p]),
@@ -425,6 +427,16 @@ def poisson_upper_tail_direct(reads: int, lam: float) -> float:
 
 def gamma_q(a: float, x: float) -> float:
     return lib().orc_gamma_q(a, x)
+
+
+def ln_gamma(x: float) -> float:
+    """statrs 0.18 ln_gamma (Lanczos), restated."""
+    return lib().orc_ln_gamma(x)
+
+
+def gamma_lr(a: float, x: float) -> float:
+    """statrs 0.18 gamma_lr (Cephes igam with the exp(ax) cut-off), restated."""
+    return lib().orc_gamma_lr(a, x)
 
 
 def bimera_params(allow_one_off=False, min_one_off_par_dist=4, match=5, mismatch=-4, gap_p=-8,

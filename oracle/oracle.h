@@ -164,6 +164,10 @@ double orc_get_pA(uint32_t reads, int prior, double lambda, uint32_t hamming, ui
                   int detect_singletons);
 /* regularised upper incomplete gamma Q(a, x) used for the Poisson tail */
 double orc_gamma_q(double a, double x);
+/* statrs 0.18 ln_gamma and gamma_lr (the regularised lower incomplete gamma
+ * P(a, x) behind Poisson::sf), restated */
+double orc_ln_gamma(double x);
+double orc_gamma_lr(double a, double x);
 
 /* ---------------- cluster.rs compare ---------------- */
 /* The per-raw map of b_compare_parallel (cluster.rs:279-330).  Outputs are

@@ -123,24 +123,33 @@ def _compare(B: Partition, i: int, err, align, greedy, use_quals, nthreads=1):
     center = B.cl[i].center
     lam, ham, skp, _, _ = compare_center(rs, center, err, err.shape[1], params,
                                          use_quals=use_quals, greedy=greedy, nthreads=nthreads)
-    center_reads = B.reads_of[center]
-    total = float(B.total)
     for r in range(len(rs)):
         if not skp[r]:
             B.nalign += 1
             if int(ham[r]) == HAMMING_NONE:
                 B.nshroud += 1
+    B.cl[i].self_lam = float(lam[center])
+    for c in store(B.e_minmax, lam, ham, float(B.total), B.reads_of[center], i):
+        B.cl[i].comp.append(c)
+        if i == 0 or c.index == center:
+            B.comp[c.index] = c
+
+
+def store(e_minmax, lam, ham, total: float, center_reads: int, i: int) -> list:
+    """The serial store of b_compare_parallel (cluster.rs:351-397) for
+    cluster i: raw r keeps its comparison when lam[r] * total > e_minmax[r]
+    (strict), e_minmax[r] then rises to lam[r] * center_reads if larger; a
+    shrouded comparison is kept with hamming 0.  Mutates e_minmax; returns
+    the kept comparisons in raw order."""
+    out = []
+    for r in range(len(lam)):
         lmb = float(lam[r])
-        if r == center:
-            B.cl[i].self_lam = lmb
-        if lmb * total > B.e_minmax[r]:
+        if lmb * total > e_minmax[r]:
             e = lmb * float(center_reads)
-            if e > B.e_minmax[r]:
-                B.e_minmax[r] = e
-            c = Comp(i, r, lmb, 0 if int(ham[r]) == HAMMING_NONE else int(ham[r]))
-            B.cl[i].comp.append(c)
-            if i == 0 or r == center:
-                B.comp[r] = c
+            if e > e_minmax[r]:
+                e_minmax[r] = e
+            out.append(Comp(i, r, lmb, 0 if int(ham[r]) == HAMMING_NONE else int(ham[r])))
+    return out
 
 
 def _p_update(B: Partition, greedy, detect_singletons):
@@ -266,6 +275,23 @@ class DadaOut:
     nshroud: int
     shuffle_passes: int
     partition: Partition
+
+
+def extend_err(err: np.ndarray, qmax: int) -> np.ndarray:
+    """dada_uniques_cached's error-model extension (dada.rs:337-384): when
+    the highest rounded quality qmax reaches the matrix's column count, the
+    matrix grows to qmax + 1 columns, each new column a copy of the last
+    learned one; otherwise it is returned unchanged."""
+    err = np.ascontiguousarray(err, np.float64)
+    old = err.shape[1]
+    if old == 0 or qmax < old:
+        return err
+    ext = np.zeros((16, qmax + 1), np.float64)
+    for t in range(16):
+        ext[t, :old] = err[t]
+        for q in range(old, qmax + 1):
+            ext[t, q] = err[t, old - 1]
+    return ext
 
 
 def run_dada(rs: RawSet, err: np.ndarray, align, prior=None, omega_a=1e-40, omega_p=1e-4,

@@ -2,13 +2,21 @@
  * pval.c — restatement of dada2-rs src/pval.rs (TEST INFRASTRUCTURE ONLY).
  *
  * compute_lambda is a left-to-right f64 product over query positions starting
- * from 1.0 (pval.rs:311-313).  calc_pA calls statrs 0.18 Poisson::sf, which
- * is not vendored with the reference: here the Poisson upper tail is the
- * regularised lower incomplete gamma P(reads, e) (sf(x) = gamma_lr(x+1, e),
- * pval.rs:211-218), evaluated with the series / Lentz continued fraction of
- * Numerical Recipes (Press et al., 3rd ed., section 6.2).  PARITY UNPINNED
- * against statrs bit patterns; pinned only by Poisson identities and the
- * reference's own small-lambda series (poisson_upper_tail_direct).
+ * from 1.0 (pval.rs:311-313).  calc_pA calls Poisson::sf of statrs 0.18
+ * (Cargo.toml:40, Cargo.lock:715-718), which is not vendored with the
+ * reference.  statrs defines sf(x) = gamma_lr(x + 1, lambda) (pval.rs:211-218),
+ * and its published gamma_lr is the Cephes incomplete gamma (igam: a power
+ * series for x <= 1 or x <= a, else a continued fraction for the complement,
+ * both stopped at a relative change of 1e-15), scaled by exp(ax) with
+ * ax = a ln x - x - ln_gamma(a), which returns exactly 0 (a >= x) or 1 when
+ * ax < -709.78271289338399 and 0 when x is within 1.1102230246251565e-15 of
+ * 0; ln_gamma is a Lanczos approximation (g = 10.900511, 11 terms).  Both are
+ * restated below in statrs's operation order.  The reference's own fallback
+ * poisson_upper_tail_direct runs exactly where that gamma_lr returns 0.
+ * PARITY UNPINNED at the bit level against statrs itself (not importable
+ * here): the restatement is pinned by lgamma and the regularised incomplete
+ * gamma of the system library / scipy to ~1e-14, by Poisson identities, and
+ * by the cut-off band's switch to the direct series.
  */
 #include <float.h>
 #include <math.h>
@@ -52,38 +60,83 @@ int orc_compute_lambda(const orc_raw* raw, const orc_sub* sub, const double* err
   return ORC_OK;
 }
 
-/* Regularised lower incomplete gamma P(a, x); NR 3rd ed. 6.2. */
-static double gamma_p_impl(double a, double x) {
-  if (x <= 0.0) return 0.0;
-  double gln = lgamma(a);
-  if (x < a + 1.0) {
-    double ap = a, del = 1.0 / a, sum = del;
-    for (int n = 0; n < 100000; ++n) {
-      ap += 1.0;
-      del *= x / ap;
-      sum += del;
-      if (fabs(del) < fabs(sum) * DBL_EPSILON) break;
-    }
-    return sum * exp(-x + a * log(x) - gln);
+/* statrs 0.18 ln_gamma: Lanczos approximation, g = GAMMA_R, coefficients
+ * GAMMA_DK, summed left to right from GAMMA_DK[0]. */
+static const double GAMMA_R = 10.900511;
+static const double GAMMA_DK[11] = {
+    2.48574089138753565546e-5,  1.05142378581721974210,    -3.45687097222016235469,
+    4.51227709466894823700,     -2.98285225323576655721,   1.05639711577126713077,
+    -1.95428773191645869583e-1, 1.70970543404441224307e-2, -5.71926117404305781283e-4,
+    4.63399473359905636708e-6,  -2.71994908488607703910e-9};
+static const double LN_PI = 1.1447298858494001741434273513530587116472948129153;
+static const double LN_2_SQRT_E_OVER_PI = 0.6207822376352452223455184457816472122518527279025978;
+static const double E_CONST = 2.71828182845904523536028747135266250;
+static const double PI_CONST = 3.14159265358979323846264338327950288;
+
+double orc_ln_gamma(double x) {
+  double s = GAMMA_DK[0];
+  if (x < 0.5) {
+    for (int i = 1; i < 11; ++i) s = s + GAMMA_DK[i] / ((double)i - x);
+    return LN_PI - log(sin(PI_CONST * x)) - log(s) - LN_2_SQRT_E_OVER_PI -
+           (0.5 - x) * log((0.5 - x + GAMMA_R) / E_CONST);
   }
-  /* Lentz continued fraction for Q(a, x) */
-  const double FPMIN = DBL_MIN / DBL_EPSILON;
-  double bb = x + 1.0 - a, c = 1.0 / FPMIN, d = 1.0 / bb, h = d;
-  for (int i = 1; i < 100000; ++i) {
-    double an = -i * (i - a);
-    bb += 2.0;
-    d = an * d + bb;
-    if (fabs(d) < FPMIN) d = FPMIN;
-    c = bb + an / c;
-    if (fabs(c) < FPMIN) c = FPMIN;
-    d = 1.0 / d;
-    double del = d * c;
-    h *= del;
-    if (fabs(del - 1.0) <= DBL_EPSILON) break;
-  }
-  double q = exp(-x + a * log(x) - gln) * h;
-  return 1.0 - q;
+  for (int i = 1; i < 11; ++i) s = s + GAMMA_DK[i] / (x + (double)i - 1.0);
+  return log(s) + LN_2_SQRT_E_OVER_PI + (x - 0.5) * log((x - 0.5 + GAMMA_R) / E_CONST);
 }
+
+/* statrs 0.18 gamma_lr(a, x) = P(a, x) for a > 0, x > 0 (the callers' range). */
+double orc_gamma_lr(double a, double x) {
+  if (isnan(a) || isnan(x)) return NAN;
+  const double eps = 0.000000000000001;
+  const double big = 4503599627370496.0;
+  const double big_inv = 2.22044604925031308085e-16;
+  const double acc = 0.0000000000000011102230246251565; /* prec::DEFAULT_F64_ACC */
+  if (fabs(a - 0.0) < acc) return 1.0;
+  if (fabs(x - 0.0) < acc) return 0.0;
+  const double ax = a * log(x) - x - orc_ln_gamma(a);
+  if (ax < -709.78271289338399) return a < x ? 1.0 : 0.0;
+  if (x <= 1.0 || x <= a) {
+    double r2 = a, c2 = 1.0, ans2 = 1.0;
+    for (;;) {
+      r2 += 1.0;
+      c2 *= x / r2;
+      ans2 += c2;
+      if (c2 / ans2 <= eps) break;
+    }
+    return exp(ax) * ans2 / a;
+  }
+  double y = 1.0 - a, z = x + y + 1.0;
+  int c = 0;
+  double p3 = 1.0, q3 = x, p2 = x + 1.0, q2 = z * x;
+  double ans = p2 / q2;
+  for (;;) {
+    y += 1.0;
+    z += 2.0;
+    c += 1;
+    const double yc = y * (double)c;
+    const double p = p2 * z - p3 * yc;
+    const double q = q2 * z - q3 * yc;
+    p3 = p2;
+    p2 = p;
+    q3 = q2;
+    q2 = q;
+    if (fabs(p) > big) {
+      p3 *= big_inv;
+      p2 *= big_inv;
+      q3 *= big_inv;
+      q2 *= big_inv;
+    }
+    if (q != 0.0) {
+      const double nextans = p / q;
+      const double error = fabs((ans - nextans) / nextans);
+      ans = nextans;
+      if (error <= eps) break;
+    }
+  }
+  return 1.0 - exp(ax) * ans;
+}
+
+static double gamma_p_impl(double a, double x) { return orc_gamma_lr(a, x); }
 
 double orc_gamma_q(double a, double x) { return 1.0 - gamma_p_impl(a, x); }
 

@@ -101,3 +101,16 @@ def test_error_model_json(F, tmp_path):
     q.write_text(json.dumps(dict(doc, err_out=err[:15].tolist())))
     with pytest.raises(ValueError):
         F.read_error_model(q)
+
+
+def test_positive_gap_penalties_are_normalised(pkg):
+    """resolve_dada_params (main.rs:4403-4418): gap_p = 8 means -8, and
+    homo_gap_p defaults to the normalised gap_p; a positive homo_gap_p is
+    normalised as well."""
+    a, b = pkg.align_params(gap_p=8), pkg.align_params(gap_p=-8)
+    for f, _ in pkg.AlignParams._fields_:
+        assert getattr(a, f) == getattr(b, f), f
+    assert a.gap_p == -8 and a.homo_gap_p == -8
+    c = pkg.align_params(gap_p=8, homo_gap_p=1)
+    assert c.gap_p == -8 and c.homo_gap_p == -1
+    assert pkg.align_params(homo_gap_p=-1).homo_gap_p == -1

@@ -150,3 +150,42 @@ def test_run_dada_errors(pkg):
         assert len(ok.clusters) >= 1
     finally:
         ctx.close()
+
+
+def test_quality_above_the_model_extends_it(pkg, oracle):
+    """Qualities at or above the model's columns (Q41-45 under a Q0-40
+    model): run_dada extends the matrix as dada_uniques_cached does
+    (dada.rs:337-384) instead of refusing, and the partition and the aux
+    transitions (now 46 columns) equal the oracle's on the extended matrix."""
+    import warnings
+    from oracle import learn
+    from oracle import partition as P
+    w = synth.workload_partition(n_true=8, n_near=2, n_prior=0, L=150, n_uniques=300,
+                                 seed=synth.SEED_BASE + 41)
+    quals = [np.asarray(q, np.uint8).copy() for q in w.quals]
+    for i in range(0, len(quals), 3):
+        quals[i][::7] = np.minimum(45, quals[i][::7].astype(np.int32) + 12).astype(np.uint8)
+    qmax = max(int(q.max()) for q in quals)
+    assert qmax >= 41
+    ext = P.extend_err(w.err, qmax)
+    rs = oracle.RawSet(w.seqs, quals, w.reads, k=5)
+    want = P.run_dada(rs, ext, oracle.make_params(band=16))
+    ctx = pkg.Context(0)
+    try:
+        ctx.load_uniques(w.seqs, quals, w.reads, k=5)
+        with warnings.catch_warnings(record=True):
+            warnings.simplefilter("always")
+            got = ctx.run_dada(w.err, pkg.align_params(band=16))
+        aux = ctx.compute_aux(got, pkg.align_params(band=16))
+    finally:
+        ctx.close()
+    assert got.err_ncol == qmax + 1
+    assert [c["center"] for c in got.clusters] == [c["center"] for c in want.clusters]
+    assert np.array_equal(got.p, want.p) and np.array_equal(got.map, want.map)
+    assert np.array_equal(got.comp_lambda, want.comp_lambda)
+    centers = [c["center"] for c in want.clusters]
+    weight = np.where(want.map >= 0, np.asarray(w.reads, np.int64), 0)
+    want_aux = learn.dada_aux(rs, centers, want.cluster_of.astype(np.int64), weight,
+                              oracle.make_params(band=16), True, qmax + 1, 150)
+    assert aux["transitions"].shape == (16, qmax + 1)
+    assert np.array_equal(aux["transitions"], want_aux["transitions"])

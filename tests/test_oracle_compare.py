@@ -157,3 +157,66 @@ def test_table_bimera2_single_sample(oracle):
     assert nflag[0] == 1 and nflag[1] == 0 and nflag[2] == 0
     # the same verdict as is_bimera on the qualifying parents
     assert oracle.is_bimera(seqs[0], [seqs[1], seqs[2]], p)
+
+
+def test_ln_gamma_is_statrs_lanczos(oracle):
+    """The restated statrs ln_gamma (Lanczos, g = 10.900511, 11 terms) agrees
+    with the system lgamma to the approximation's own accuracy on both of
+    its branches: a wrong or dropped coefficient moves it by far more."""
+    xs = list(np.geomspace(0.5, 1e7, 400)) + list(range(1, 2001)) + list(np.linspace(0.02, 0.49, 40))
+    worst = 0.0
+    for x in xs:
+        want = math.lgamma(x)
+        got = oracle.ln_gamma(float(x))
+        worst = max(worst, abs(got - want) / max(1.0, abs(want)))
+    # the approximation's rounding is ~1e-14 at its worst (near x = 5); a
+    # coefficient off in its 12th digit already moves it past 1e-13
+    assert worst < 1.5e-14, worst
+    assert abs(oracle.ln_gamma(1.0)) < 1e-15 and abs(oracle.ln_gamma(2.0)) < 1e-15
+
+
+def test_gamma_lr_is_statrs_igam(oracle):
+    """statrs 0.18 gamma_lr restated (Cephes igam): against scipy's
+    regularised lower incomplete gamma in the series branch (x <= 1 or
+    x <= a) and the continued-fraction branch (x > a, x > 1)."""
+    special = pytest.importorskip("scipy.special")
+    seen = {"series": 0, "cf": 0}
+    worst = 0.0
+    for a in (1, 2, 3, 5, 8, 13, 40, 100, 333, 1000, 5000):
+        for x in (1e-9, 1e-3, 0.5, 1.0, 0.3 * a, 0.9 * a, a, 1.1 * a + 1, 2.0 * a + 3, 5.0 * a + 10):
+            want = float(special.gammainc(a, x))
+            got = oracle.gamma_lr(float(a), float(x))
+            seen["series" if (x <= 1.0 or x <= a) else "cf"] += 1
+            if want > 1e-290:
+                # exp(ax) amplifies the rounding of ax's terms (each ~1e-16
+                # of its size): the method's own conditioning, not a bug
+                scale = abs(a * math.log(x)) + x + math.lgamma(a)
+                worst = max(worst, abs(got - want) / want / (1e-13 + 1e-15 * scale))
+    assert seen["series"] > 20 and seen["cf"] > 20
+    assert worst < 1.0, worst
+
+
+def test_gamma_lr_cutoff_hands_over_to_the_direct_series(oracle):
+    """statrs's gamma_lr returns exactly 0 when ax = a ln x - x - ln_gamma(a)
+    < -709.78271289338399 with a >= x, and when x is within 1.11e-15 of 0;
+    calc_pA then takes poisson_upper_tail_direct (pval.rs:219-226), bit for
+    bit.  Just inside the cut-off the series result stands."""
+    nonzero = 0
+    for reads, e in [(50, 1e-5), (50, 1e-6), (200, 0.5), (12, 1e-30), (2, 1e-16), (3, 1e-20)]:
+        ax = reads * math.log(e) - e - math.lgamma(reads)
+        assert ax < -709.78 or e < 1.1102230246251565e-15
+        assert oracle.gamma_lr(float(reads), e) == 0.0
+        got = oracle.calc_pA(reads, e, True)
+        want = oracle.poisson_upper_tail_direct(reads, e)
+        assert got == want
+        nonzero += want > 0.0  # (50, 1e-5): a subnormal the series still resolves
+    assert nonzero == 3
+    # ax just above the cut-off: a positive gamma_lr (exp(ax) ~ 1e-303)
+    reads, e = 200, 2.8
+    ax = reads * math.log(e) - e - math.lgamma(reads)
+    assert -709.78 < ax < -600
+    g = oracle.gamma_lr(float(reads), e)
+    assert g > 0.0 and oracle.calc_pA(reads, e, True) == g
+    # a >= x inside the band gives 0, a < x gives 1 (never reached by calc_pA
+    # at x <= 1e300: the band needs x far below a)
+    assert oracle.gamma_lr(3.0, 1e-300) == 0.0

@@ -155,3 +155,26 @@ def test_limits(oracle, P):
     rs = oracle.RawSet(w.seqs, w.quals, w.reads, k=5)
     three = P.run_dada(rs, w.err, oracle.make_params(), max_clust=3)
     assert len(three.clusters) == 3
+
+
+def test_extend_err_repeats_the_last_column(oracle, pkg):
+    """dada_uniques_cached's extension (dada.rs:337-384): a Q0-40 model met
+    with Q45 grows to 46 columns, columns 41..45 copies of column 40; data
+    within the model leaves it untouched.  The package's host mirror does
+    the same."""
+    import warnings
+    from oracle import partition as P
+    rng = np.random.default_rng(3)
+    err = rng.random((16, 41))
+    ext = P.extend_err(err, 45)
+    assert ext.shape == (16, 46)
+    assert np.array_equal(ext[:, :41], err)
+    for q in range(41, 46):
+        assert np.array_equal(ext[:, q], err[:, 40])
+    assert P.extend_err(err, 40) is not None and P.extend_err(err, 40).shape == (16, 41)
+    with warnings.catch_warnings(record=True) as w:
+        warnings.simplefilter("always")
+        mine = pkg.extend_err(err, 45)
+        assert any("Q45" in str(x.message) for x in w)
+    assert np.array_equal(mine, ext)
+    assert np.array_equal(pkg.extend_err(err, 40), err)

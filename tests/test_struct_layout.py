@@ -16,6 +16,7 @@ PAIRS = [  # (C struct, ctypes class name)
     ("d2g_bimera_params", "BimeraParams"),
     ("d2g_pair_stats", "PairStats"),
     ("d2g_pairs_dev", "PairsDev"),
+    ("d2g_comparison", "Comparison"),
 ]
 
 
