@@ -148,3 +148,46 @@ def test_ascii_chimera_constructs(ctx, oracle):
     got = ctx.align_vectorized([q] * len(parents), parents, band=16)
     for p, g in zip(parents, got):
         assert g == oracle.align_vectorized(q, p, band=16)
+
+
+def _dec(b: bytes) -> str:
+    return "".join("-" if x == 45 else "ACGTN"[x - 1] for x in b)
+
+
+def test_tie_cases_worked_by_hand_on_device(ctx, oracle):
+    """The hand-derived co-optimal placements (tests/tie_cases.py) on the
+    device aligners: vectorized with and without swap, scalar ends-free and
+    homopolymer."""
+    from tie_cases import TIE_CASES
+    for kind, s1, s2, w0, w1, _, why in TIE_CASES:
+        a, b = oracle.encode(s1), oracle.encode(s2)
+        if kind == "vectorized":
+            got = ctx.align_vectorized([a], [b], band=16)[0]
+        elif kind == "endsfree":
+            got = ctx.align_endsfree([a], [b], band=16)[0]
+        else:
+            got = ctx.align_endsfree([a], [b], homo_gap_p=-1, band=16)[0]
+        assert (_dec(got[0]), _dec(got[1])) == (w0, w1), (kind, s1, s2, why)
+
+
+def test_tie_rich_two_letter_pairs(ctx, oracle):
+    """Every pair over {A, T} with lengths 1-5 and 1-7 (ties everywhere),
+    both orders: the device equals the oracle, whose tie rules the
+    hand-worked cases pin."""
+    import itertools
+    s1s, s2s = [], []
+    for l1 in range(1, 6):
+        for l2 in range(l1, l1 + 3):
+            for x in itertools.product((1, 4), repeat=l1):
+                for y in itertools.product((1, 4), repeat=l2):
+                    s1s.append(bytes(x))
+                    s2s.append(bytes(y))
+    pairs = list(zip(s1s, s2s)) + list(zip(s2s, s1s))
+    print(f"{len(pairs)} tie-rich pairs", flush=True)
+    for band in (16, 2):
+        got = ctx.align_vectorized([p[0] for p in pairs], [p[1] for p in pairs], band=band)
+        for (a, b), g in zip(pairs, got):
+            assert g == oracle.align_vectorized(a, b, band=band), (a, b, band)
+    got = ctx.align_endsfree([p[0] for p in pairs], [p[1] for p in pairs], band=16)
+    for (a, b), g in zip(pairs, got):
+        assert g == oracle.align_endsfree(a, b, band=16), (a, b)

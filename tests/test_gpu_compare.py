@@ -184,3 +184,76 @@ def test_autotuned_shape_same_results(pkg, oracle, batched_small):
         _assert_same(c.compare_batch(cs, w.err), *want)
     finally:
         c.close()
+
+
+@pytest.mark.parametrize("k", [3, 6, 7, 8])
+def test_other_kmer_sizes(ctx, oracle, pkg, k):
+    """k = 3, 6, 7 (the reference's PacBio setting) and 8 (where the
+    reference switches to its sparse k-mer vectors, kmers.rs:29-45, whose
+    distances equal the dense ones bit for bit): the device's dense
+    screen against the oracle at each k."""
+    w = synth.subsample(synth.workload_batched(), 600, n_centers=8)
+    ctx.load_uniques(w.seqs, w.quals, w.reads, k=k)
+    print(f"k={k}: {w.n} uniques x {len(w.centers)} centers", flush=True)
+    for cutoff in (0.42, 0.1):
+        params = pkg.align_params(band=w.band, kmer_size=k, kdist_cutoff=cutoff)
+        got = ctx.compare_batch(w.centers, w.err, params)
+        rs = oracle.RawSet(w.seqs, w.quals, w.reads, k=k)
+        op = oracle.make_params(band=w.band, kmer_size=k, kdist_cutoff=cutoff)
+        rows = [oracle.compare_center(rs, int(c), w.err, 41, op, nthreads=8) for c in w.centers]
+        _assert_same(got, np.stack([r[0] for r in rows]), np.stack([r[1] for r in rows]),
+                     np.stack([r[2] for r in rows]))
+        assert got.stats["aligned_dp"] > 0
+        if k >= 6:  # at k = 3 every profile is close: nothing is shrouded
+            assert got.stats["shrouded"] > 0
+
+
+def _saturated_workload(seed=91):
+    """Uniques of 300-330 nt led by a 280-nt A run, so the AAAAA bucket
+    saturates kmer8 at 255 (kmers.rs:390-401: kmer_dist8 returns -1.0) and
+    the compare takes the u16 kmer_dist fallback (nwalign.rs:1270-1278);
+    mixed with ordinary random uniques of the same lengths."""
+    rng = np.random.default_rng(seed)
+    seqs = []
+    base_tail = rng.integers(1, 5, size=40).astype(np.uint8)
+    for i in range(60):
+        run = 280 - int(rng.integers(0, 8))
+        tail = base_tail[: 20 + int(rng.integers(0, 20))].copy()
+        flip = rng.random(len(tail)) < 0.05
+        tail[flip] = rng.integers(1, 5, size=int(flip.sum()))
+        s = np.concatenate([np.ones(run, np.uint8), tail])
+        if i % 3 == 0:  # a few substitutions inside the run keep its buckets >= 255
+            pos = rng.integers(0, run, size=2)
+            s[pos] = rng.integers(2, 5, size=2)
+        seqs.append(s)
+    for _ in range(140):
+        seqs.append(rng.integers(1, 5, size=int(rng.integers(300, 331))).astype(np.uint8))
+    uniq = {s.tobytes(): s for s in seqs}
+    seqs = list(uniq.values())
+    quals = [np.clip(rng.normal(34, 3, size=len(s)).round(), 0, 40).astype(np.uint8) for s in seqs]
+    reads = np.maximum(1, rng.lognormal(2, 1.5, size=len(seqs)).round()).astype(np.uint32)
+    return seqs, quals, reads
+
+
+def test_u16_fallback_on_saturated_buckets(ctx, oracle, pkg):
+    """Pairs where both k-mer vectors hold a 255 bucket: the screen's exact
+    min-sum meets the saturation, and the u16 kmer_dist decides (the
+    device's u16_min_dot branch); results equal the oracle's, and the
+    oracle's own fallback is checked to have fired."""
+    seqs, quals, reads = _saturated_workload()
+    kv = [oracle.kmer8(s) for s in seqs[:60]]
+    assert all(int(v.max()) == 255 for v in kv)
+    assert oracle.kmer_dist8(kv[0], len(seqs[0]), kv[1], len(seqs[1])) == -1.0
+    ctx.load_uniques(seqs, quals, reads, k=5)
+    centers = np.arange(0, 60, 6)
+    for cutoff in (0.42, 0.05):
+        params = pkg.align_params(band=16, kdist_cutoff=cutoff)
+        got = ctx.compare_batch(centers, synth.fixed_err(41), params)
+        rs = oracle.RawSet(seqs, quals, reads, k=5)
+        op = oracle.make_params(band=16, kdist_cutoff=cutoff)
+        rows = [oracle.compare_center(rs, int(c), synth.fixed_err(41), 41, op, nthreads=8)
+                for c in centers]
+        _assert_same(got, np.stack([r[0] for r in rows]), np.stack([r[1] for r in rows]),
+                     np.stack([r[2] for r in rows]))
+    # the saturated pairs aligned (the fallback passed them), the random ones shrouded
+    assert (got.hamming[:, :60] != 0xFFFFFFFF).sum() > 0

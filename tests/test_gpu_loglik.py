@@ -15,9 +15,20 @@ pytestmark = pytest.mark.gpu
 H = 1e-3
 
 
-@pytest.fixture(scope="module")
-def setup(ctx, oracle, pkg):
-    w = synth.subsample(synth.workload_batched(), 2000, n_centers=16)
+def _workload(name):
+    if name == "batched":
+        return synth.subsample(synth.workload_batched(), 2000, n_centers=16)
+    # the gradient workload: 500 variants at a fixed 250 nt (50,000 uniques
+    # generated, the 3,000 most abundant and the 40 centers among them used)
+    w = synth.workload_batched(n_variants=500, L=250, n_uniques=50000, lmin=250, lmax=250,
+                               seed=synth.SEED_BASE + 3)
+    return synth.subsample(w, 3000, n_centers=40)
+
+
+@pytest.fixture(scope="module", params=["batched", "gradient"])
+def setup(ctx, oracle, pkg, request):
+    w = _workload(request.param)
+    print(f"loglik workload {request.param}: {w.n} uniques x {len(w.centers)} centers", flush=True)
     ctx.load_uniques(w.seqs, w.quals, w.reads, k=5)
     got = ctx.compare_batch(w.centers, w.err)
     rng = np.random.default_rng(4)

@@ -223,3 +223,35 @@ def test_standard_nw_penalises_end_gaps(oracle):
     ef = oracle.align_endsfree(a, b, band=-1)
     assert oracle.score_alignment(*ef) == 55
     assert std[0][0] != 45 and std[1][0] == 45  # the leading gap is still placed
+
+
+def _dec(b: bytes) -> str:
+    return "".join("-" if x == 45 else "ACGTN"[x - 1] for x in b)
+
+
+def _score(al0: str, al1: str) -> int:
+    """Ends-free score with free leading and trailing gaps (5 / -4 / -8)."""
+    lo = min(i for i in range(len(al0)) if al0[i] != "-" and al1[i] != "-")
+    hi = max(i for i in range(len(al0)) if al0[i] != "-" and al1[i] != "-")
+    s = 0
+    for a, b in zip(al0[lo:hi + 1], al1[lo:hi + 1]):
+        s += -8 if "-" in (a, b) else (5 if a == b else -4)
+    return s
+
+
+@pytest.mark.parametrize("case", __import__("tie_cases").TIE_CASES, ids=lambda c: f"{c[0]}-{c[1]}-{c[2]}")
+def test_tie_cases_worked_by_hand(oracle, case):
+    """Co-optimal placements fixed by the tie-break lines, each derived by
+    hand on the full matrix (tests/tie_cases.py): the oracle reproduces
+    them string for string."""
+    kind, s1, s2, w0, w1, score, _ = case
+    a, b = enc(oracle, s1), enc(oracle, s2)
+    if kind == "vectorized":
+        got = oracle.align_vectorized(a, b, band=16)
+    elif kind == "endsfree":
+        got = oracle.align_endsfree(a, b, band=16)
+    else:
+        got = oracle.align_endsfree_homo(a, b, oracle.make_params(band=16, homo_gap_p=-1))
+    assert (_dec(got[0]), _dec(got[1])) == (w0, w1)
+    if kind != "homo":
+        assert _score(w0, w1) == score
