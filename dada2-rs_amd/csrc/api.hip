@@ -88,12 +88,34 @@ struct d2g_ctx {
   int sched_gap = 0, sched_end_gap = 0;
   size_t lds_bytes = 0;
   int sched_rows = 0;
+  std::vector<uint8_t> sched_present;  // lengths the cached schedule covers
+  DevBuf<uint64_t> sched_off;          // per bin: first record, or kSchedAbsent
   DevBuf<RowRec> sched;
   DevBuf<BinInfo> sched_info;
   // pointer-history scratch of the DP: nw_grid slots of hist_stride words
   uint32_t nw_grid = 0;
   size_t hist_stride = 0;
   DevBuf<uint32_t> hist;
+  // the same state for the other DP kernel (a compare over a unique set on
+  // both sides of the vectorized length limit alternates the two)
+  BinSpace st_key = {-1, -1, -1, -1, -1, -1};
+  int st_gap = 0, st_end_gap = 0;
+  size_t st_lds_bytes = 0;
+  int st_rows = 0;
+  std::vector<uint8_t> st_present;
+  DevBuf<uint64_t> st_off;
+  DevBuf<RowRec> st_sched;
+  DevBuf<BinInfo> st_info;
+  uint32_t st_grid = 0;
+  size_t st_hist_stride = 0;
+  DevBuf<uint32_t> st_hist;
+  // the one-pair kernel's pointer slots (vectorized bins nw_vec cannot hold)
+  WideNeed wide = {0, 0};
+  uint32_t wide_grid = 0;
+  DevBuf<uint8_t> wide_p;
+  WideNeed st_wide = {0, 0};
+  uint32_t st_wide_grid = 0;
+  DevBuf<uint8_t> st_wide_p;
   // resident aligned-pair set (mixed-precision lambda / p-values)
   uint32_t p_n = 0, p_max_qual = 0;
   bool p_has_qual = false;
@@ -249,7 +271,14 @@ int d2g_ctx_destroy(d2g_ctx* c) {
   c->dp_sorted.release();
   c->ctr.release();
   c->sched.release();
+  c->sched_off.release();
+  c->st_off.release();
   c->sched_info.release();
+  c->st_sched.release();
+  c->st_info.release();
+  c->st_hist.release();
+  c->wide_p.release();
+  c->st_wide_p.release();
   for (auto* b : {&c->p_seq, &c->p_qual, &c->p_flag}) b->release();
   for (auto* b : {&c->p_off, &c->p_suboff}) b->release();
   for (auto* b : {&c->p_len, &c->p_subs, &c->p_reads, &c->p_creads, &c->p_nrec, &c->p_list}) b->release();
@@ -316,54 +345,13 @@ int d2g_device_info(d2g_ctx* c, int* cu, int* clk, int* mclk, int* bus) {
 }
 
 uint64_t d2g_vectorized_cells(uint32_t l1_in, uint32_t l2_in, int32_t band) {
-  // replay of the index walk (nwalign.rs:865-1088): sum of active widths
-  VecGeom g = vec_geom((int)l1_in, (int)l2_in, band);
-  const int len1 = g.len1, len2 = g.len2, b = g.band, sc = g.start_col;
-  int row = 2, col_min = sc, col_max = sc, i_max = 0;
-  bool even = true;
-  uint64_t total = 0;
-  while (row <= len1 + len2) {
-    int n = col_max - col_min + 1;
-    if (n > 0) total += (uint64_t)n;
-    if (row == imin(b, len1)) {
-      col_min = imax(col_min - 1, 0);
-      i_max += 1;
-    }
-    if (row == g.top_limit) col_max += 1;
-    if (row < b && row < len1) {
-      if (even) col_min = imax(col_min - 1, 0);
-      i_max += 1;
-    } else if (i_max < len1 - 1) {
-      if (b % 2 == 0) {
-        if (!even) i_max += 1;
-      } else if (even) {
-        col_min = imax(col_min - 1, 0);
-        i_max += 1;
-      } else {
-        col_min += 1;
-      }
-    } else if (!even) {
-      col_min += 1;
-    }
-    if (row < g.top_limit) {
-      if (!even) col_max += 1;
-    } else if ((row + 1) / 2 + col_max - sc < len2) {
-      if ((b + len2 - len1) % 2 == 0) {
-        if (even) col_max = imax(col_max - 1, 0);
-        else col_max += 1;
-      }
-    } else if (even) {
-      col_max = imax(col_max - 1, 0);
-    }
-    ++row;
-    even = !even;
-  }
-  return total;
+  return vec_cells((int)l1_in, (int)l2_in, band);
 }
 
 int d2g_vectorized_supported(uint32_t l1, uint32_t l2, int32_t band) {
   if (l1 > kVectorizedMaxLen || l2 > kVectorizedMaxLen) return 0;
-  return nw_groups(vec_geom((int)l1, (int)l2, band)) > 0 ? 1 : 0;
+  const VecGeom g = vec_geom((int)l1, (int)l2, band);
+  return (nw_groups(g) > 0 || wide_fits(g)) ? 1 : 0;
 }
 
 int d2g_load_uniques(d2g_ctx* c, uint32_t n, const uint8_t* seq_concat, const uint64_t* offsets,
@@ -478,19 +466,62 @@ int d2g_load_uniques(d2g_ctx* c, uint32_t n, const uint8_t* seq_concat, const ui
 // kernel, the row schedule of a bin space, cached across calls with the same
 // space and scores; also sizes the persistent grid and its pointer-history
 // scratch.
-static int prepare_schedule(d2g_ctx* c, const BinSpace& bs, int end_gap, int gap) {
-  if (bs.lmin == c->lds_key.lmin && bs.range == c->lds_key.range && bs.band == c->lds_key.band &&
-      bs.kind == c->lds_key.kind && bs.homo == c->lds_key.homo && end_gap == c->sched_end_gap &&
-      gap == c->sched_gap && c->sched_info.p)
-    return D2G_OK;
+static void swap_schedule(d2g_ctx* c) {
+  std::swap(c->lds_key, c->st_key);
+  std::swap(c->sched_gap, c->st_gap);
+  std::swap(c->sched_end_gap, c->st_end_gap);
+  std::swap(c->lds_bytes, c->st_lds_bytes);
+  std::swap(c->sched_rows, c->st_rows);
+  std::swap(c->sched_present, c->st_present);
+  std::swap(c->sched_off, c->st_off);
+  std::swap(c->sched, c->st_sched);
+  std::swap(c->sched_info, c->st_info);
+  std::swap(c->nw_grid, c->st_grid);
+  std::swap(c->hist_stride, c->st_hist_stride);
+  std::swap(c->hist, c->st_hist);
+  std::swap(c->wide, c->st_wide);
+  std::swap(c->wide_grid, c->st_wide_grid);
+  std::swap(c->wide_p, c->st_wide_p);
+}
+
+static int prepare_schedule(d2g_ctx* c, const BinSpace& bs, int end_gap, int gap,
+                            const std::vector<uint8_t>& present) {
+  auto hit = [&]() {
+    return bs.lmin == c->lds_key.lmin && bs.range == c->lds_key.range &&
+           bs.band == c->lds_key.band && bs.kind == c->lds_key.kind && bs.homo == c->lds_key.homo &&
+           end_gap == c->sched_end_gap && gap == c->sched_gap && c->sched_info.p &&
+           present == c->sched_present;
+  };
+  if (hit()) return D2G_OK;
+  // keep one schedule per DP kernel: the other kernel's goes to the stash
+  if (bs.kind != c->lds_key.kind) {
+    swap_schedule(c);
+    if (hit()) return D2G_OK;
+  }
   const bool vec = bs.kind == kDpVectorized;
-  c->lds_bytes = vec ? nw_vec_lds_for(bs) : nw_band_lds_for(bs);
+  const uint8_t* pr = present.data();
+  c->lds_bytes = vec ? nw_vec_lds_for(bs, pr) : nw_band_lds_for(bs, pr);
   const int lmax = bs.lmin + bs.range - 1;
   if (c->sched_info.ensure((size_t)bs.nbins) != hipSuccess) return D2G_ERR_HIP;
+  std::vector<uint64_t> off;
   if (vec) {
+    // row schedules only for the length pairs the call's sequences form
     c->sched_rows = nw_sched_rows(lmax);
-    const size_t recs = (size_t)bs.nbins * (size_t)c->sched_rows;
-    if (c->sched.ensure(recs) != hipSuccess) return D2G_ERR_HIP;
+    off.assign((size_t)bs.nbins, kSchedAbsent);
+    uint64_t recs = 0;
+    for (int b = 0; b < bs.nbins; ++b) {
+      int l1, l2;
+      bin_lengths(bs, (uint32_t)b, &l1, &l2);
+      if (!bin_present(bs, pr, l1, l2)) continue;
+      const VecGeom g = vec_geom(l1, l2, bs.band);
+      off[(size_t)b] = recs;
+      if (nw_groups(g) > 0) recs += (uint64_t)g.nrow;
+    }
+    if (c->sched.ensure(std::max<uint64_t>(recs, 1)) != hipSuccess) return D2G_ERR_HIP;
+    if (c->sched_off.ensure((size_t)bs.nbins) != hipSuccess) return D2G_ERR_HIP;
+    if (hipMemcpyAsync(c->sched_off.p, off.data(), off.size() * sizeof(uint64_t),
+                       hipMemcpyHostToDevice, c->stream) != hipSuccess)
+      return D2G_ERR_HIP;
   }
   // persistent DP workgroups: 8 one-wave groups per SIMD where LDS allows,
   // and at most 1 GiB of pointer-history scratch
@@ -510,11 +541,20 @@ static int prepare_schedule(d2g_ctx* c, const BinSpace& bs, int end_gap, int gap
   while (c->nw_grid > (uint32_t)c->cu_count && (size_t)c->nw_grid * c->hist_stride * 4 > ((size_t)1 << 30))
     c->nw_grid /= 2;
   if (c->hist.ensure((size_t)c->nw_grid * c->hist_stride) != hipSuccess) return D2G_ERR_HIP;
-  const hipError_t e = vec ? launch_sched(bs, c->sched_rows, end_gap, gap, c->sched.p, c->sched_info.p, c->stream)
+  // the one-pair kernel: a pointer slot per wavefront, at most 1 GiB
+  c->wide = nw_wide_need(bs, pr);
+  c->wide_grid = 0;
+  if (c->wide.max_ncol > 0) {
+    c->wide_grid = (uint32_t)std::max<size_t>(
+        1, std::min<size_t>((size_t)c->cu_count * 2, ((size_t)1 << 30) / c->wide.pslot));
+    if (c->wide_p.ensure((size_t)c->wide_grid * c->wide.pslot) != hipSuccess) return D2G_ERR_HIP;
+  }
+  const hipError_t e = vec ? launch_sched(bs, c->sched_off.p, end_gap, gap, c->sched.p, c->sched_info.p, c->stream)
                            : launch_band_info(bs, c->sched_info.p, c->stream);
   if (e != hipSuccess) return D2G_ERR_HIP;
-  if (hipStreamSynchronize(c->stream) != hipSuccess) return D2G_ERR_HIP;
+  if (hipStreamSynchronize(c->stream) != hipSuccess) return D2G_ERR_HIP;  // off is pageable
   c->lds_key = bs;
+  c->sched_present = present;
   c->sched_end_gap = end_gap;
   c->sched_gap = gap;
   return D2G_OK;
@@ -604,38 +644,39 @@ struct TransMode {
   unsigned long long* qual_sum = nullptr;
 };
 
-// The compare pipeline; with fetch, the results are copied to the host
-// before the call's single synchronisation.
-static int compare_device(d2g_ctx* c, const uint32_t* centers, uint32_t nc, const double* err,
-                          uint32_t ncol, const d2g_align_params* p, int use_quals, int greedy,
-                          const uint8_t* lock, d2g_stats* stats, bool fetch = false,
-                          double* out_lambda = nullptr, uint32_t* out_hamming = nullptr,
-                          uint8_t* out_skipped = nullptr, const TransMode* tm = nullptr) {
-  if (!c || !centers || !err || !p || nc == 0 || ncol == 0) return D2G_ERR_ARG;
-  if (!c->loaded) return D2G_ERR_STATE;
-  (void)hipSetDevice(c->device);
+// One pass of the compare: the DP kernel and its bin space, and which
+// (center, unique) pairs the pass takes (see ScreenArgs).
+struct Route {
+  int kind, homo;
+  uint32_t lmin, lmax;  // lengths the bin space covers
+  uint32_t long_len, sr_lo, sr_hi, lr_lo, lr_hi;
+  bool graphs;          // replay from a captured graph (single-pass calls)
+};
+
+// One pass of the compare pipeline; with fetch, the results are copied to
+// the host before the pass's single synchronisation.
+static int compare_pass(d2g_ctx* c, const uint32_t* centers, uint32_t nc, const double* err,
+                        uint32_t ncol, const d2g_align_params* p, int use_quals, int greedy,
+                        const uint8_t* lock, d2g_stats* stats, bool fetch, double* out_lambda,
+                        uint32_t* out_hamming, uint8_t* out_skipped, const TransMode* tm,
+                        const Route& rt) {
   UniqueDev& u = c->u;
   const uint32_t n = u.n;
-  if ((uint32_t)p->kmer_size != u.k && p->use_kmers) return D2G_ERR_ARG;
-  if (use_quals && c->has_qual && c->max_qual >= ncol) return D2G_ERR_ARG;  // pval.rs:289, err extension dada.rs:337-384
-  int homo = 0;
-  const int kind = dp_kind(p, u.minlen, u.maxlen, &homo);
-  if (p->band != 0 && kind < 0) return D2G_ERR_UNSUPPORTED;
-  for (uint32_t i = 0; i < nc; ++i)
-    if (centers[i] >= n) return D2G_ERR_ARG;
   const size_t npairs = (size_t)nc * n;
-  // (the pairing passes form at most one pair per unique)
-  if (!tm && npairs > 0xFFFFFFFFull) return D2G_ERR_UNSUPPORTED;
   BinSpace bs;
-  bs.lmin = (int)u.minlen;
-  bs.range = (int)(u.maxlen - u.minlen + 1);
+  bs.lmin = (int)rt.lmin;
+  bs.range = (int)(rt.lmax - rt.lmin + 1);
   bs.band = p->band;
-  bs.kind = kind < 0 ? kDpVectorized : kind;  // band 0: every pair is gapless
-  bs.homo = homo;
+  bs.kind = rt.kind;
+  bs.homo = rt.homo;
   if ((size_t)bs.range * bs.range > (1u << 22)) return D2G_ERR_UNSUPPORTED;
   bs.nbins = bs.range * bs.range;
   const bool vec = bs.kind == kDpVectorized;
-  HIP_TRY_INT(prepare_schedule(c, bs, 0, p->gap_p));
+  // the lengths of the uniques in the bin space
+  std::vector<uint8_t> present((size_t)bs.range, 0);
+  for (uint32_t L : c->h_len)
+    if (L >= rt.lmin && L <= rt.lmax) present[L - rt.lmin] = 1;
+  HIP_TRY_INT(prepare_schedule(c, bs, 0, p->gap_p, present));
   HIP_TRY(c->centers.ensure(nc));
   if (c->err.cap < (size_t)16 * ncol) c->h_err.clear();  // reallocation loses the upload
   HIP_TRY(c->err.ensure((size_t)16 * ncol));
@@ -750,6 +791,11 @@ static int compare_device(d2g_ctx* c, const uint32_t* centers, uint32_t nc, cons
     sa.split = c->screen_split;
     sa.per_lane = c->screen_per_lane;
     sa.dstar = c->dstar.p;
+    sa.long_len = rt.long_len;
+    sa.sr_lo = rt.sr_lo;
+    sa.sr_hi = rt.sr_hi;
+    sa.lr_lo = rt.lr_lo;
+    sa.lr_hi = rt.lr_hi;
     HIP_TRY(tm ? launch_pair_screen(sa, tm->want, s) : launch_screen(sa, s));
     if (timed) HIP_TRY(hipEventRecord(c->ev[1], s));
     HIP_TRY(launch_plan(bs, c->bin_count.p, c->bin_start.p, c->wave_start.p, c->bin_fill.p, c->ctr.p,
@@ -784,8 +830,8 @@ static int compare_device(d2g_ctx* c, const uint32_t* centers, uint32_t nc, cons
     na.hist_stride = c->hist_stride;
     na.lds_bytes = c->lds_bytes;
     na.sched = c->sched.p;
+    na.sched_off = c->sched_off.p;
     na.info = c->sched_info.p;
-    na.sched_rows = c->sched_rows;
     if (tm) {
       na.emit_alignment = 1;
       na.al0 = tm->al0;
@@ -794,6 +840,7 @@ static int compare_device(d2g_ctx* c, const uint32_t* centers, uint32_t nc, cons
       na.al_len = tm->al_len;
     }
     HIP_TRY(vec ? launch_nw_vec(na, s) : launch_nw_band(na, s));
+    if (vec) HIP_TRY(launch_nw_wide(na, c->wide, c->wide_p.p, c->wide_grid, s));
     if (timed) HIP_TRY(hipEventRecord(c->ev[3], s));
     if (tm) {
       TransArgs ta;
@@ -893,7 +940,7 @@ static int compare_device(d2g_ctx* c, const uint32_t* centers, uint32_t nc, cons
   }
 
   for (int attempt = 0; attempt < 6; ++attempt) {
-    const bool graph = c->graphs && !c->timing && !tm;
+    const bool graph = c->graphs && !c->timing && !tm && rt.graphs;
     if (graph) {
       // everything a kernel argument depends on
       std::vector<uint64_t> key = {
@@ -902,6 +949,7 @@ static int compare_device(d2g_ctx* c, const uint32_t* centers, uint32_t nc, cons
           (uint64_t)bs.kind, (uint64_t)bs.homo, (uint64_t)p->use_kmers, (uint64_t)p->gapless,
           (uint64_t)p->match_score, (uint64_t)(uint32_t)p->mismatch, (uint64_t)(uint32_t)p->gap_p,
           (uint64_t)(uint32_t)p->homo_gap_p, (uint64_t)p->kmer_size, sub_stride, c->sub_capacity,
+          rt.long_len, rt.sr_lo, rt.sr_hi, rt.lr_lo, rt.lr_hi,
           c->nw_grid, c->lds_bytes, c->hist_stride, (uint64_t)c->sched_rows,
           (uint64_t)c->screen_split, (uint64_t)c->screen_qblock, (uint64_t)c->screen_per_lane,
           (uint64_t)(uintptr_t)c->dstar.p};
@@ -918,6 +966,7 @@ static int compare_device(d2g_ctx* c, const uint32_t* centers, uint32_t nc, cons
                             (const void*)c->wave_start.p, (const void*)c->bin_fill.p,
                             (const void*)c->screen_partials.p, (const void*)c->hist.p,
                             (const void*)c->sched.p, (const void*)c->sched_info.p,
+                            (const void*)c->sched_off.p, (const void*)c->wide_p.p,
                             (const void*)c->ctr.p})
         key.push_back((uint64_t)(uintptr_t)q);
       if (!c->graph_exec || key != c->graph_key) {
@@ -1000,6 +1049,70 @@ static int compare_device(d2g_ctx* c, const uint32_t* centers, uint32_t nc, cons
     return D2G_OK;
   }
   return D2G_ERR_HIP;
+}
+
+// The compare: raw_align_dp's routing (nwalign.rs:1315-1414) per pair.
+// With vectorized parameters, pairs whose longer side exceeds 3500 nt take
+// the scalar ends-free aligner (nwalign.rs:1379-1385): a unique set on both
+// sides of that length runs two passes, the vectorized one over the pairs
+// of short centers with short uniques, the row-band one over the rest.
+static int compare_device(d2g_ctx* c, const uint32_t* centers, uint32_t nc, const double* err,
+                          uint32_t ncol, const d2g_align_params* p, int use_quals, int greedy,
+                          const uint8_t* lock, d2g_stats* stats, bool fetch = false,
+                          double* out_lambda = nullptr, uint32_t* out_hamming = nullptr,
+                          uint8_t* out_skipped = nullptr, const TransMode* tm = nullptr) {
+  if (!c || !centers || !err || !p || nc == 0 || ncol == 0) return D2G_ERR_ARG;
+  if (!c->loaded) return D2G_ERR_STATE;
+  (void)hipSetDevice(c->device);
+  const UniqueDev& u = c->u;
+  const uint32_t n = u.n;
+  if ((uint32_t)p->kmer_size != u.k && p->use_kmers) return D2G_ERR_ARG;
+  if (use_quals && c->has_qual && c->max_qual >= ncol) return D2G_ERR_ARG;  // pval.rs:289, err extension dada.rs:337-384
+  for (uint32_t i = 0; i < nc; ++i)
+    if (centers[i] >= n) return D2G_ERR_ARG;
+  const size_t npairs = (size_t)nc * n;
+  // (the pairing passes form at most one pair per unique)
+  if (!tm && npairs > 0xFFFFFFFFull) return D2G_ERR_UNSUPPORTED;
+  int homo = 0;
+  const int kind = dp_kind(p, u.minlen, u.maxlen, &homo);
+  if (kind >= 0 || p->band == 0) {  // band 0: every pair is gapless
+    const Route rt{kind < 0 ? kDpVectorized : kind, homo, u.minlen, u.maxlen, 0xFFFFFFFFu, 0, n,
+                   0, n, true};
+    return compare_pass(c, centers, nc, err, ncol, p, use_quals, greedy, lock, stats, fetch,
+                        out_lambda, out_hamming, out_skipped, tm, rt);
+  }
+  const uint32_t n_short =
+      (uint32_t)(std::upper_bound(c->h_len.begin(), c->h_len.end(), kVectorizedMaxLen) - c->h_len.begin());
+  const Route rv{kDpVectorized, 0, u.minlen, c->h_len[n_short - 1], kVectorizedMaxLen, 0, n_short,
+                 0, 0, false};
+  const Route rr{kDpRowBand, 0, u.minlen, u.maxlen, kVectorizedMaxLen, n_short, n, 0, n, false};
+  d2g_stats s1{}, s2{};
+  TransMode tm1;
+  if (tm) {
+    tm1 = *tm;
+    tm1.aux = false;  // compute_aux's per-raw pass runs once, after both
+  }
+  int rc = compare_pass(c, centers, nc, err, ncol, p, use_quals, greedy, lock, &s1, false, nullptr,
+                        nullptr, nullptr, tm ? &tm1 : nullptr, rv);
+  if (rc != D2G_OK) return rc;
+  rc = compare_pass(c, centers, nc, err, ncol, p, use_quals, greedy, lock, &s2, fetch, out_lambda,
+                    out_hamming, out_skipped, tm, rr);
+  c->cmp_valid = false;  // the likelihood entries take a single-pass compare
+  if (stats) {
+    *stats = s2;
+    stats->comparisons = tm ? s1.comparisons + s2.comparisons : npairs;
+    stats->skipped += s1.skipped;
+    stats->shrouded += s1.shrouded;
+    stats->gapless += s1.gapless;
+    stats->aligned_dp += s1.aligned_dp;
+    stats->dp_cells += s1.dp_cells;
+    stats->ms_screen += s1.ms_screen;
+    stats->ms_plan += s1.ms_plan;
+    stats->ms_dp += s1.ms_dp;
+    stats->ms_lambda += s1.ms_lambda;
+    stats->ms_total += s1.ms_total;
+  }
+  return rc;
 }
 
 int d2g_compare_batch_device(d2g_ctx* c, const uint32_t* centers, uint32_t nc, const double* err,
@@ -1617,7 +1730,9 @@ static int align_pairs_device(d2g_ctx* c, uint32_t n, const PairSeqs& ps, int ki
     t.u_idx = 0;
     t.c_idx = 0;
     counts[t.bin] += 1;
-    if (bin_groups(bs, (int)l1[i], (int)l2[i]) == 0) return D2G_ERR_UNSUPPORTED;
+    if (bin_groups(bs, (int)l1[i], (int)l2[i]) == 0 &&
+        !(kind == kDpVectorized && wide_fits(vec_geom((int)l1[i], (int)l2[i], band))))
+      return D2G_ERR_UNSUPPORTED;
   }
   hipStream_t s = c->stream;
   HIP_TRY(c->a_pool.ensure(n1 + n2 + 1));
@@ -1643,7 +1758,12 @@ static int align_pairs_device(d2g_ctx* c, uint32_t n, const PairSeqs& ps, int ki
   h0.n_dp = n;
   HIP_TRY(hipMemcpyAsync(c->ctr.p, &h0, sizeof(Counters), hipMemcpyHostToDevice, s));
   HIP_TRY(hipStreamSynchronize(s));
-  HIP_TRY_INT(prepare_schedule(c, bs, end_gap, gap));
+  std::vector<uint8_t> present((size_t)bs.range, 0);
+  for (uint32_t i = 0; i < n; ++i) {
+    present[l1[i] - lmin] = 1;
+    present[l2[i] - lmin] = 1;
+  }
+  HIP_TRY_INT(prepare_schedule(c, bs, end_gap, gap, present));
   HIP_TRY(launch_plan(bs, c->bin_count.p, c->bin_start.p, c->wave_start.p, c->bin_fill.p, c->ctr.p,
                       c->sched_info.p, nullptr, 0, nullptr, s));
   HIP_TRY(launch_scatter(c->dp_unsorted.p, c->dp_sorted.p, c->bin_start.p, c->bin_fill.p, nullptr,
@@ -1671,9 +1791,10 @@ static int align_pairs_device(d2g_ctx* c, uint32_t n, const PairSeqs& ps, int ki
   na.hist_stride = c->hist_stride;
   na.lds_bytes = c->lds_bytes;
   na.sched = c->sched.p;
+  na.sched_off = c->sched_off.p;
   na.info = c->sched_info.p;
-  na.sched_rows = c->sched_rows;
   HIP_TRY(kind == kDpVectorized ? launch_nw_vec(na, s) : launch_nw_band(na, s));
+  if (kind == kDpVectorized) HIP_TRY(launch_nw_wide(na, c->wide, c->wide_p.p, c->wide_grid, s));
   Counters h;
   HIP_TRY(hipMemcpyAsync(&h, c->ctr.p, sizeof(Counters), hipMemcpyDeviceToHost, s));
   HIP_TRY(hipStreamSynchronize(s));

@@ -96,6 +96,64 @@ struct BinInfo {
 // Records of the schedule table per bin of a BinSpace whose longest length
 // is lmax: rows 2..len1+len2 and two lookahead records.
 D2G_HD int nw_sched_rows(int lmax) { return 2 * lmax + 1; }
+// Schedule offset of a bin whose length pair no task of the call can have.
+constexpr uint64_t kSchedAbsent = ~0ull;
+
+// Geometries nw_vec cannot hold (more than 64 compressed columns) go to the
+// one-pair-per-wavefront kernel (nw_wide.hip) up to these limits: its three
+// score rows in LDS and its pointer matrix in one global slot.
+constexpr int kWideMaxCol = 4096;
+constexpr size_t kWideMaxCells = (size_t)32 << 20;
+D2G_HD bool wide_fits(const VecGeom& g) {
+  return g.ncol <= kWideMaxCol && (size_t)g.ncol * (size_t)g.nrow <= kWideMaxCells;
+}
+
+// DP cells of the vectorized loop for one pair: the sum of the active widths
+// over the anti-diagonals, a replay of the index walk (nwalign.rs:865-1088).
+D2G_HD unsigned long long vec_cells(int l1_in, int l2_in, int band_in) {
+  const VecGeom g = vec_geom(l1_in, l2_in, band_in);
+  const int len1 = g.len1, len2 = g.len2, b = g.band, sc = g.start_col;
+  int row = 2, col_min = sc, col_max = sc, i_max = 0;
+  bool even = true;
+  unsigned long long total = 0;
+  while (row <= len1 + len2) {
+    const int n = col_max - col_min + 1;
+    if (n > 0) total += (unsigned long long)n;
+    if (row == imin(b, len1)) {
+      col_min = imax(col_min - 1, 0);
+      i_max += 1;
+    }
+    if (row == g.top_limit) col_max += 1;
+    if (row < b && row < len1) {
+      if (even) col_min = imax(col_min - 1, 0);
+      i_max += 1;
+    } else if (i_max < len1 - 1) {
+      if (b % 2 == 0) {
+        if (!even) i_max += 1;
+      } else if (even) {
+        col_min = imax(col_min - 1, 0);
+        i_max += 1;
+      } else {
+        col_min += 1;
+      }
+    } else if (!even) {
+      col_min += 1;
+    }
+    if (row < g.top_limit) {
+      if (!even) col_max += 1;
+    } else if ((row + 1) / 2 + col_max - sc < len2) {
+      if ((b + len2 - len1) % 2 == 0) {
+        if (even) col_max = imax(col_max - 1, 0);
+        else col_max += 1;
+      }
+    } else if (even) {
+      col_max = imax(col_max - 1, 0);
+    }
+    ++row;
+    even = !even;
+  }
+  return total;
+}
 
 // Pairs per wavefront for one geometry (0 = unsupported).
 D2G_HD int nw_groups(const VecGeom& g) {
@@ -214,6 +272,13 @@ struct BinSpace {
 D2G_HD void bin_lengths(const BinSpace& bs, uint32_t bin, int* l1, int* l2) {
   *l1 = bs.lmin + (int)(bin / (uint32_t)bs.range);
   *l2 = bs.lmin + (int)(bin % (uint32_t)bs.range);
+}
+
+// Whether a call's tasks can fall in the bin of lengths (l1, l2): present
+// (nullable: every length) flags the lengths lmin .. lmin + range - 1 the
+// call's sequences have.
+inline bool bin_present(const BinSpace& bs, const uint8_t* present, int l1, int l2) {
+  return !present || (present[l1 - bs.lmin] && present[l2 - bs.lmin]);
 }
 
 // Pairs per wavefront of a bin for its kernel (0 = unsupported geometry).

@@ -61,6 +61,10 @@ struct ScreenArgs {
   int split = 4;          // wavefront groups per workgroup sharing its 16 centers: 1, 2 or 4
   int per_lane = 1;       // uniques per lane: 1, or 2 (qblock 128 or 256)
   const uint32_t* dstar = nullptr;  // [maxlen - k + 2]: shroud iff dot < dstar[n] (use_kmers)
+  // the pairs of this pass (the compare splits at the vectorized aligner's
+  // length limit): a center of length <= long_len pairs with sorted uniques
+  // [sr_lo, sr_hi), a longer one with [lr_lo, lr_hi)
+  uint32_t long_len = 0xFFFFFFFFu, sr_lo = 0, sr_hi = 0xFFFFFFFFu, lr_lo = 0, lr_hi = 0xFFFFFFFFu;
 };
 // dstar[n] for n = 0 .. nmax: the smallest dot in 0 .. n with
 // 1.0 - (double)dot / (double)n <= cutoff in f64 (n + 1 when there is none);
@@ -110,9 +114,9 @@ struct NwArgs {
   const uint32_t* wave_start;  // nbins + 1
   const DpTask* tasks;         // sorted by bin
   const uint8_t* pool;         // sequence bytes
-  const RowRec* sched;         // per-bin row schedule (launch_sched)
+  const RowRec* sched;         // row schedules (launch_sched)
+  const uint64_t* sched_off;   // per bin: its first record in sched
   const BinInfo* info;         // per-bin steady range and cell count (launch_sched)
-  int sched_rows;              // records per bin
   int match, mismatch, gap_p, end_gap_p;
   int homo_gap_p;              // row-band kernel with bins.homo
   Counters* ctr;
@@ -135,10 +139,24 @@ struct NwArgs {
   size_t lds_bytes;        // from nw_vec_lds_for
 };
 hipError_t launch_nw_vec(const NwArgs& a, hipStream_t s);
-// Replays the vectorized row walk of every bin into table[bin * rows + row - 2].
-hipError_t launch_sched(const BinSpace& bs, int rows, int end_gap, int gap, RowRec* table,
-                        BinInfo* info, hipStream_t s);
-size_t nw_vec_lds_for(const BinSpace& bs);
+// Replays the vectorized row walk of every bin with off[bin] != kSchedAbsent
+// into table[off[bin] + row - 2] (nrow records per bin); a bin nw_vec cannot
+// hold gets only its cell count.
+hipError_t launch_sched(const BinSpace& bs, const uint64_t* off, int end_gap, int gap,
+                        RowRec* table, BinInfo* info, hipStream_t s);
+size_t nw_vec_lds_for(const BinSpace& bs, const uint8_t* present);
+
+// ---- nw_wide.hip -------------------------------------------------------
+// The vectorized aligner for the bins nw_vec cannot hold (wide_fits), one
+// pair per wavefront: the largest such geometry of a bin space sizes the
+// LDS rows and the per-wavefront pointer slot (0: none in the space).
+struct WideNeed {
+  int max_ncol;
+  size_t pslot;
+};
+WideNeed nw_wide_need(const BinSpace& bs, const uint8_t* present);
+hipError_t launch_nw_wide(const NwArgs& a, const WideNeed& need, uint8_t* pbuf, uint32_t grid,
+                          hipStream_t s);
 
 // ---- nw_band.hip -------------------------------------------------------
 // The scalar ends-free aligners (plain / homopolymer by bins.homo) over the
@@ -146,7 +164,7 @@ size_t nw_vec_lds_for(const BinSpace& bs);
 hipError_t launch_nw_band(const NwArgs& a, hipStream_t s);
 // BinInfo.cells of every bin for the row-band kernel.
 hipError_t launch_band_info(const BinSpace& bs, BinInfo* info, hipStream_t s);
-size_t nw_band_lds_for(const BinSpace& bs);
+size_t nw_band_lds_for(const BinSpace& bs, const uint8_t* present);
 
 // ---- lambda.hip --------------------------------------------------------
 struct LambdaArgs {

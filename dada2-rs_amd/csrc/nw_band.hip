@@ -772,11 +772,12 @@ hipError_t launch_nw_band(const NwArgs& s, hipStream_t st) {
   return hipGetLastError();
 }
 
-size_t nw_band_lds_for(const BinSpace& bs) {
+size_t nw_band_lds_for(const BinSpace& bs, const uint8_t* present) {
   size_t lds = 0;
   for (int b = 0; b < bs.nbins; ++b) {
     int l1, l2;
     bin_lengths(bs, (uint32_t)b, &l1, &l2);
+    if (!bin_present(bs, present, l1, l2)) continue;
     const BandGeom g = band_geom(l1, l2, bs.band);
     const int G = band_groups(g, bs.homo != 0);
     if (G > 0) lds = std::max(lds, band_wave_bytes(g, G, bs.homo != 0));

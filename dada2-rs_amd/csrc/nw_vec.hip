@@ -53,8 +53,8 @@ struct NwDev {
   const DpTask* tasks;
   const uint8_t* pool;
   const RowRec* sched;
+  const uint64_t* sched_off;
   const BinInfo* info;
-  int sched_rows;
   int match, mismatch, gap, end_gap;
   int steady_clamp;  // keep the i16 clamps in steady rows
   Counters* ctr;
@@ -154,15 +154,25 @@ D2G_HD void vec_schedule(const VecGeom& geo, int end_gap, int gap, RowRec* out) 
   }
 }
 
-__global__ void sched_kernel(BinSpace bs, int rows, int end_gap, int gap, RowRec* table,
-                             BinInfo* info) {
+__global__ void sched_kernel(BinSpace bs, const uint64_t* off, int end_gap, int gap,
+                             RowRec* table, BinInfo* info) {
   const int b = blockIdx.x * blockDim.x + threadIdx.x;
   if (b >= bs.nbins) return;
+  const uint64_t o = off[b];
+  if (o == kSchedAbsent) return;  // a length pair no task can have
   int l1, l2;
   bin_lengths(bs, (uint32_t)b, &l1, &l2);
   const VecGeom g = vec_geom(l1, l2, bs.band);
-  if (nw_groups(g) == 0 || g.nrow > rows) return;
-  RowRec* t = table + (size_t)b * rows;
+  if (nw_groups(g) == 0) {  // the one-pair kernel's bins: only their cell count
+    BinInfo bi;
+    bi.steady_lo = 0;
+    bi.steady_hi = 0;
+    bi.cells = wide_fits(g) ? (uint32_t)vec_cells(l1, l2, bs.band) : 0u;
+    bi._pad = 0;
+    info[b] = bi;
+    return;
+  }
+  RowRec* t = table + o;  // nrow records: rows 2 .. len1 + len2, two lookahead
   vec_schedule(g, end_gap, gap, t);
   const int last = g.len1 + g.len2;
   // active-lane masks over the G pairs of a wavefront (pair gi on lanes
@@ -224,11 +234,11 @@ __global__ void sched_kernel(BinSpace bs, int rows, int end_gap, int gap, RowRec
   info[b] = bi;
 }
 
-hipError_t launch_sched(const BinSpace& bs, int rows, int end_gap, int gap, RowRec* table,
-                        BinInfo* info, hipStream_t s) {
+hipError_t launch_sched(const BinSpace& bs, const uint64_t* off, int end_gap, int gap,
+                        RowRec* table, BinInfo* info, hipStream_t s) {
   const int threads = 64;
   const int blocks = (bs.nbins + threads - 1) / threads;
-  hipLaunchKernelGGL(sched_kernel, dim3(blocks), dim3(threads), 0, s, bs, rows, end_gap, gap,
+  hipLaunchKernelGGL(sched_kernel, dim3(blocks), dim3(threads), 0, s, bs, off, end_gap, gap,
                      table, info);
   return hipGetLastError();
 }
@@ -629,7 +639,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(8, 8))) void
     if (last < 2) hist_p[lane] = ls.hp << 28;
 
     // ---- forward: anti-diagonals 2..len1+len2 ----
-    CRowRec* rec = (CRowRec*)(a.sched + (size_t)bin * a.sched_rows);
+    CRowRec* rec = (CRowRec*)(a.sched + a.sched_off[bin]);
     const BinInfo bi = a.info[bin];
     const int b0 = uni((int)bi.steady_lo), b1 = uni((int)bi.steady_hi);
     const uint8_t* s1l = s1b - c;
@@ -729,7 +739,7 @@ hipError_t launch_nw_vec(const NwArgs& s, hipStream_t st) {
   d.pool = s.pool;
   d.sched = s.sched;
   d.info = s.info;
-  d.sched_rows = s.sched_rows;
+  d.sched_off = s.sched_off;
   d.match = s.match;
   d.mismatch = s.mismatch;
   d.gap = s.gap_p;
@@ -747,7 +757,8 @@ hipError_t launch_nw_vec(const NwArgs& s, hipStream_t st) {
   d.al1 = s.al1;
   d.al_off = s.al_off;
   d.al_len = s.al_len;
-  if (s.grid == 0 || s.hist == nullptr || s.sched == nullptr) return hipErrorInvalidValue;
+  if (s.grid == 0 || s.hist == nullptr || s.sched == nullptr || s.sched_off == nullptr)
+    return hipErrorInvalidValue;
   if (!s.emit_alignment && s.sub_stride == 0) return hipErrorInvalidValue;
   const size_t lds = s.lds_bytes;
   if (lds > 48 * 1024) {
@@ -761,11 +772,12 @@ hipError_t launch_nw_vec(const NwArgs& s, hipStream_t st) {
 
 // Dynamic LDS of a launch: the largest wavefront footprint over every bin of
 // the space (bins are not known on the host, only their range).
-size_t nw_vec_lds_for(const BinSpace& bs) {
+size_t nw_vec_lds_for(const BinSpace& bs, const uint8_t* present) {
   size_t lds = 0;
   for (int b = 0; b < bs.nbins; ++b) {
     int l1, l2;
     bin_lengths(bs, (uint32_t)b, &l1, &l2);
+    if (!bin_present(bs, present, l1, l2)) continue;
     VecGeom g = vec_geom(l1, l2, bs.band);
     int G = nw_groups(g);
     if (G > 0) {

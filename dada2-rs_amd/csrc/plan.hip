@@ -74,7 +74,8 @@ __global__ __launch_bounds__(kPlanThreads) void plan_kernel(BinSpace bs,
     int l1, l2;
     bin_lengths(bs, b, &l1, &l2);
     const int G = bin_groups(bs, l1, l2);
-    if (cnt && !G) bad = 1;
+    // vectorized bins too wide for nw_vec take the one-pair kernel
+    if (cnt && !G && !(bs.kind == kDpVectorized && wide_fits(vec_geom(l1, l2, bs.band)))) bad = 1;
     sp += cnt;
     sw += G ? (cnt + G - 1) / G : 0;
     if (cnt && info) cells += (unsigned long long)cnt * info[b].cells;

@@ -57,7 +57,14 @@ struct ScreenDev {
   uint32_t* bin_count;
   uint32_t* partials;
   Counters* ctr;
+  // the pairs of this pass: with a center of length <= long_len, uniques
+  // [sr_lo, sr_hi); with a longer center, uniques [lr_lo, lr_hi)
+  uint32_t long_len, sr_lo, sr_hi, lr_lo, lr_hi;
 };
+
+__device__ __forceinline__ bool pair_in_pass(const ScreenDev& a, uint32_t center_len, uint32_t q) {
+  return center_len <= a.long_len ? (q >= a.sr_lo && q < a.sr_hi) : (q >= a.lr_lo && q < a.lr_hi);
+}
 
 __device__ __forceinline__ uint32_t lane_id() { return __lane_id(); }
 
@@ -207,6 +214,7 @@ __global__ __launch_bounds__(QB / QL * SPLIT) void screen_kernel(ScreenDev a) {
 #pragma unroll
     for (int j = 0; j < QL; ++j) {
       const QueryFields& Q = qf[j];
+      const bool in = Q.valid && pair_in_pass(a, C.len, Q.qq);
       uint32_t sd = sad[j][0];
 #pragma unroll
       for (int cc = 1; cc < CW; ++cc) sd = (uint32_t)cc == c ? sad[j][cc] : sd;
@@ -233,7 +241,7 @@ __global__ __launch_bounds__(QB / QL * SPLIT) void screen_kernel(ScreenDev a) {
             }
           }
           dot = ovf ? u16_min_dot(u, Q.qq, C.ci) : d2;  // nwalign.rs:1270-1278
-          if (ovf && Q.valid) atomicAdd(&a.ctr->n_exact_fallback, 1u);
+          if (ovf && in) atomicAdd(&a.ctr->n_exact_fallback, 1u);
         }
         // kdist > cutoff (raw_align_with_buf, nwalign.rs:1296)
         shroud = !skip && dot < (Q.len <= C.len ? Q.thr : C.thr);
@@ -259,14 +267,14 @@ __global__ __launch_bounds__(QB / QL * SPLIT) void screen_kernel(ScreenDev a) {
         route_gapless = a.band == 0 || fabs(kodist - kdist) < DBL_EPSILON;
       }
       const bool route_dp = !skip && !shroud && !route_gapless;
-      if (Q.valid && (skip || shroud)) {
+      if (in && (skip || shroud)) {
         a.out_lambda[out_idx] = 0.0;
         a.out_hamming[out_idx] = kHammingNone;
         a.out_skipped[out_idx] = skip ? 1 : 0;
       }
-      n_skip += Q.valid && skip;
-      n_shroud += Q.valid && shroud;
-      const bool to_gl = Q.valid && route_gapless;
+      n_skip += in && skip;
+      n_shroud += in && shroud;
+      const bool to_gl = in && route_gapless;
       const uint32_t gpos = wave_append(&s_ngl, to_gl);
       if (to_gl) {
         GaplessTask t;
@@ -276,7 +284,7 @@ __global__ __launch_bounds__(QB / QL * SPLIT) void screen_kernel(ScreenDev a) {
         t._pad = 0;
         gl_region[gpos] = t;
       }
-      const bool to_dp = Q.valid && route_dp;
+      const bool to_dp = in && route_dp;
       const uint32_t dpos = wave_append(&s_ndp, to_dp);
       if (to_dp) {
         DpTask t;
@@ -346,7 +354,7 @@ __global__ __launch_bounds__(kPairBlock) void pair_kernel(ScreenDev a, const uin
   const UniqueDev& u = a.u;
   const uint32_t q = blockIdx.x * kPairBlock + threadIdx.x;
   const uint32_t ci = q < u.n ? want[q] : 0xFFFFFFFFu;
-  bool live = ci < u.n;
+  bool live = ci < u.n && pair_in_pass(a, u.len[ci], q);
   // raw_align_with_buf (nwalign.rs:1244-1310): without the k-mer screen
   // kdist is 0.0 and kodist -1.0, so only band 0 takes the gapless
   // alignment (nwalign.rs:1324); with it, the pair's own distances decide
@@ -428,6 +436,11 @@ hipError_t launch_pair_screen(const ScreenArgs& s, const uint32_t* want, hipStre
   d.bin_count = s.bin_count;
   d.partials = s.partials;
   d.ctr = s.ctr;
+  d.long_len = s.long_len;
+  d.sr_lo = s.sr_lo;
+  d.sr_hi = s.sr_hi;
+  d.lr_lo = s.lr_lo;
+  d.lr_hi = s.lr_hi;
   hipLaunchKernelGGL(pair_kernel, dim3(pair_blocks(d.u.n)), dim3(kPairBlock), 0, st, d, want);
   return hipGetLastError();
 }
@@ -462,6 +475,11 @@ hipError_t launch_screen(const ScreenArgs& s, hipStream_t st) {
   d.partials = s.partials;
   d.ctr = s.ctr;
   d.dstar = s.dstar;
+  d.long_len = s.long_len;
+  d.sr_lo = s.sr_lo;
+  d.sr_hi = s.sr_hi;
+  d.lr_lo = s.lr_lo;
+  d.lr_hi = s.lr_hi;
   if (s.split != 1 && s.split != 2 && s.split != 4) return hipErrorInvalidValue;
   if (s.use_kmers && s.dstar == nullptr) return hipErrorInvalidValue;
   const dim3 grid((d.u.n + s.qblock - 1) / s.qblock,

@@ -32,7 +32,11 @@ def test_supported_geometries(pkg):
     assert pkg.vectorized_supported(250, 250, 16)
     assert pkg.vectorized_supported(240, 260, 16)
     assert pkg.vectorized_supported(1450, 1480, 16)
-    assert not pkg.vectorized_supported(300, 299, -1)
+    # wider than one wavefront: the one-pair kernel
+    assert pkg.vectorized_supported(300, 299, -1)
+    assert pkg.vectorized_supported(250, 120, 16)
+    assert pkg.vectorized_supported(3500, 3500, -1)
+    # beyond the vectorized aligner's length limit (nwalign.rs:1383)
     assert not pkg.vectorized_supported(3501, 3400, 16)
 
 

@@ -75,12 +75,34 @@ def test_fuzz_grid_supported(ctx, oracle, pkg):
     assert n > 3000, n
 
 
-def test_unsupported_geometry_is_an_error(ctx, pkg):
-    # unbanded 300 x 299: compressed width far above one wavefront
-    assert not pkg.vectorized_supported(300, 299, -1)
-    with pytest.raises(pkg.D2GError) as e:
-        ctx.align_vectorized([bytes([1] * 300)], [bytes([2] * 299)], band=-1)
-    assert e.value.status == -5
+def test_wide_geometries_match_oracle(ctx, oracle, pkg):
+    """Compressed bands wider than one wavefront (nw_wide.hip): unbanded
+    pairs, wide bands, and length differences above ~90 at band 16, each in
+    both orders (swap), with and without shared sequence."""
+    rng = np.random.default_rng(17)
+    s1s, s2s, bands = [], [], []
+    for l1, l2, band in [(300, 299, -1), (250, 120, 16), (120, 250, 16), (400, 180, 32),
+                         (200, 200, 128), (333, 310, 256), (90, 260, -1), (1000, 700, 16),
+                         (61, 160, 16), (160, 61, 16)]:
+        for related in (True, False):
+            a = rng.integers(1, 5, size=max(l1, l2)).astype(np.uint8)
+            x = a[:l1].copy()
+            if related:
+                y = a[:l2].copy()
+                flip = rng.random(l2) < 0.02
+                y[flip] = rng.integers(1, 5, size=int(flip.sum()))
+            else:
+                y = rng.integers(1, 5, size=l2).astype(np.uint8)
+            assert pkg.vectorized_supported(l1, l2, band)
+            s1s.append(bytes(x))
+            s2s.append(bytes(y))
+            bands.append(band)
+    for band in sorted(set(bands)):
+        idx = [i for i, b in enumerate(bands) if b == band]
+        got = ctx.align_vectorized([s1s[i] for i in idx], [s2s[i] for i in idx], band=band)
+        for i, g in zip(idx, got):
+            assert g == oracle.align_vectorized(s1s[i], s2s[i], band=band), (len(s1s[i]),
+                                                                           len(s2s[i]), band)
 
 
 def test_buffer_reuse_descending_lengths(ctx, oracle):
