@@ -1850,6 +1850,9 @@ int d2g_band_supported(uint32_t len1, uint32_t len2, int32_t band, int homo) {
 static int loglik_args(d2g_ctx* c, uint32_t ncol, int use_quals, LoglikArgs* la) {
   if (!c->cmp_valid) return D2G_ERR_STATE;
   if (use_quals && c->has_qual && c->max_qual >= ncol) return D2G_ERR_ARG;
+  // LDS of the backward workgroup: four wavefronts' gradient rows and counts
+  // (12 bytes per table entry) and the 64 pairs' staged rows (33 KB)
+  if ((size_t)4 * 16 * ncol * 12 + 64 * 1024 > (size_t)160 * 1024) return D2G_ERR_UNSUPPORTED;
   // one workgroup per 64 aligned pairs, at most 4 per CU
   const uint32_t grid = (uint32_t)std::max<size_t>(
       1, std::min<size_t>((size_t)c->cu_count * 4, (c->cmp_pairs + 63) / 64));

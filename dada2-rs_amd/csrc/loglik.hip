@@ -14,10 +14,20 @@
 // again in the backward kernel, never stored.  The parameter is FP32; sums
 // are FP64.
 // The pairs are staged into LDS rows as d2g_stage.h describes (four
-// wavefronts per 64 pairs, lanes 0..15 own them), so the per-pair walks read
-// LDS only.  Positions past a pair's end index an extra table entry: 0.0 in
-// the forward's log table (adding +0.0 leaves the f64 sum unchanged) and a
-// discarded bin in the backward's histogram.
+// wavefronts per 64 pairs; each wavefront stages its 16 pairs segment by
+// segment), and then every staged pair is taken by all 64 lanes in turn,
+// four positions per lane:
+//   forward:  the lanes' FP64 partial sums of the log table meet in a fixed
+//             butterfly and the pair's running sum takes the segment's total,
+//             so log lambda_j does not depend on scheduling;
+//   backward: the lanes count the segment's table indices into an integer
+//             histogram (u32 LDS atomics: order-free), then each bin with a
+//             count takes one FP64 multiply-add, w_j x count, into the
+//             wavefront's gradient row.  Pairs and segments are visited in a
+//             fixed order and the workgroup adds its wavefronts' rows in
+//             wavefront order, so the gradient's bits repeat run to run.
+// Positions past a pair's end index an extra table entry: 0.0 in the
+// forward's log table and a discarded count in the backward.
 #include "d2g_stage.h"
 
 namespace d2g {
@@ -35,6 +45,7 @@ struct LoglikDev {
 __global__ __launch_bounds__(64 * kStWaves) void loglik_fwd_kernel(LoglikDev a, const float* logE) {
   extern __shared__ double le[];  // 16 ncol entries and 0.0 after them, then the rows
   __shared__ double red[kStWaves];
+  __shared__ double llp[kStWaves][kStPairsW];  // each staged pair's running sum
   const uint32_t lane = __lane_id(), wave = threadIdx.x / 64u;
   const uint32_t ncol = a.ncol, ne = 16u * ncol;
   for (uint32_t i = threadIdx.x; i <= ne; i += blockDim.x) le[i] = i < ne ? (double)logE[i] : 0.0;
@@ -43,39 +54,35 @@ __global__ __launch_bounds__(64 * kStWaves) void loglik_fwd_kernel(LoglikDev a, 
   const uint32_t n_g = a.ctr->n_gapless, n = n_g + a.ctr->n_dp;
   const bool quals = a.use_quals && a.ts.u.qual != nullptr;
   __syncthreads();
-  double acc = 0.0;
+  double acc = 0.0;  // lanes 0 .. 15: their pairs, in order
   for (uint32_t base = blockIdx.x * (uint32_t)kStTile; base < n; base += gridDim.x * (uint32_t)kStTile) {
     const TaskRow me = task_row(a.ts, n_g, n, base + wave * kStPairsW + lane,
                                 lane < (uint32_t)kStPairsW);
     const bool any_gl = __builtin_amdgcn_ballot_w64(me.gl) != 0;
-    double ll_lam = 0.0;
+    if (lane < (uint32_t)kStPairsW) llp[wave][lane] = 0.0;
     uint32_t ham = 0;
     SubCursor cur = sub_cursor(me);
     for (uint32_t s0 = 0; __builtin_amdgcn_ballot_w64(s0 < me.lq); s0 += kStSeg) {
       stage_segment(a.ts, me, s0, lane, quals, any_gl, rows, myb, ham, cur);
       wave_lds_sync();
-      const uint32_t seg_n = me.lq > s0 ? min(me.lq - s0, (uint32_t)kStSeg) : 0u;
-      if (lane < (uint32_t)kStPairsW) {
-        const uint32_t* my = reinterpret_cast<const uint32_t*>(myb);
-        for (uint32_t i = 0; __builtin_amdgcn_ballot_w64(i < seg_n); i += 32) {
-          uint32_t w16[16];
-#pragma unroll
-          for (int h = 0; h < 16; ++h) w16[h] = my[(i >> 1) + (uint32_t)h];
-          double f[32];
-#pragma unroll
-          for (int e = 0; e < 32; ++e) f[e] = le[code_index((w16[e >> 1] >> (16 * (e & 1))) & 0xFFFFu, ncol)];
-#pragma unroll
-          for (int e = 0; e < 32; ++e) ll_lam += f[e];
-        }
+      for (uint32_t pr = 0; pr < (uint32_t)kStPairsW; ++pr) {
+        // positions past the pair's end carry the 0.0 entry
+        const uint32_t w2 = *reinterpret_cast<const uint32_t*>(rows + pr * kStRow + 4u * lane);
+        const uint32_t w3 = *reinterpret_cast<const uint32_t*>(rows + pr * kStRow + 4u * lane + 256u);
+        double part = le[code_index(w2 & 0xFFFFu, ncol)] + le[code_index(w2 >> 16, ncol)];
+        part += le[code_index(w3 & 0xFFFFu, ncol)] + le[code_index(w3 >> 16, ncol)];
+        for (int off = 32; off > 0; off >>= 1) part += __shfl_xor(part, off);
+        if (lane == 0) llp[wave][pr] += part;
       }
       wave_lds_sync();
     }
     if (me.live) {
       const double av = (double)a.ts.u.reads[me.u_idx];
-      const double E = (double)a.ts.u.reads[me.c_idx] * exp(ll_lam);
+      const double E = (double)a.ts.u.reads[me.c_idx] * exp(llp[wave][lane]);
       a.w[me.t] = av - E;
       acc += av * log(E) - E - lgamma(av + 1.0);
     }
+    wave_lds_sync();
   }
   for (int off = 32; off > 0; off >>= 1) acc += __shfl_xor(acc, off);
   if (lane == 0) red[wave] = acc;
@@ -88,15 +95,24 @@ __global__ __launch_bounds__(64 * kStWaves) void loglik_fwd_kernel(LoglikDev a, 
 }
 
 __global__ __launch_bounds__(64 * kStWaves) void loglik_bwd_kernel(LoglikDev a) {
-  extern __shared__ double hist[];  // 16 ncol bins and a discarded one, then the rows
+  // per wavefront: 16 ncol gradient sums; then per wavefront 16 ncol + 1
+  // counts (the last one discarded); then the rows
+  extern __shared__ double grad[];
   const uint32_t lane = __lane_id(), wave = threadIdx.x / 64u;
   const uint32_t ncol = a.ncol, ne = 16u * ncol;
-  for (uint32_t i = threadIdx.x; i <= ne; i += blockDim.x) hist[i] = 0.0;
-  uint8_t* rows = reinterpret_cast<uint8_t*>(hist + ne + 1) + wave * kStPairsW * kStRow;
+  double* g = grad + (size_t)wave * ne;
+  uint32_t* cnt0 = reinterpret_cast<uint32_t*>(grad + (size_t)kStWaves * ne);
+  uint32_t* cnt = cnt0 + (size_t)wave * (ne + 1);
+  uint8_t* rows = reinterpret_cast<uint8_t*>(cnt0 + (size_t)kStWaves * (ne + 1) + 1) + wave * kStPairsW * kStRow;
+  rows = reinterpret_cast<uint8_t*>((reinterpret_cast<uintptr_t>(rows) + 7) & ~(uintptr_t)7);
   uint8_t* myb = rows + min(lane, (uint32_t)kStPairsW - 1u) * kStRow;
+  for (uint32_t i = lane; i <= ne; i += 64u) {
+    if (i < ne) g[i] = 0.0;
+    cnt[i] = 0u;
+  }
   const uint32_t n_g = a.ctr->n_gapless, n = n_g + a.ctr->n_dp;
   const bool quals = a.use_quals && a.ts.u.qual != nullptr;
-  __syncthreads();
+  wave_lds_sync();
   for (uint32_t base = blockIdx.x * (uint32_t)kStTile; base < n; base += gridDim.x * (uint32_t)kStTile) {
     const TaskRow me = task_row(a.ts, n_g, n, base + wave * kStPairsW + lane,
                                 lane < (uint32_t)kStPairsW);
@@ -107,30 +123,53 @@ __global__ __launch_bounds__(64 * kStWaves) void loglik_bwd_kernel(LoglikDev a) 
     for (uint32_t s0 = 0; __builtin_amdgcn_ballot_w64(s0 < me.lq); s0 += kStSeg) {
       stage_segment(a.ts, me, s0, lane, quals, any_gl, rows, myb, ham, cur);
       wave_lds_sync();
-      const uint32_t seg_n = me.lq > s0 ? min(me.lq - s0, (uint32_t)kStSeg) : 0u;
-      if (lane < (uint32_t)kStPairsW && me.live) {
-        const uint32_t* my = reinterpret_cast<const uint32_t*>(myb);
-        for (uint32_t i = 0; i < seg_n; i += 2) {
-          const uint32_t w2 = my[i >> 1];
-          atomicAdd(&hist[code_index(w2 & 0xFFFFu, ncol)], wt);
-          atomicAdd(&hist[code_index(w2 >> 16, ncol)], wt);  // past the end: the discarded bin
+      for (uint32_t pr = 0; pr < (uint32_t)kStPairsW; ++pr) {
+        const double wp = __shfl(wt, (int)pr);
+        if (!__builtin_amdgcn_readfirstlane(__shfl((int)me.live, (int)pr))) continue;  // an empty slot
+        const uint32_t w2 = *reinterpret_cast<const uint32_t*>(rows + pr * kStRow + 4u * lane);
+        const uint32_t w3 = *reinterpret_cast<const uint32_t*>(rows + pr * kStRow + 4u * lane + 256u);
+        // past the pair's end: index 16 ncol, the discarded count
+        atomicAdd(&cnt[code_index(w2 & 0xFFFFu, ncol)], 1u);
+        atomicAdd(&cnt[code_index(w2 >> 16, ncol)], 1u);
+        atomicAdd(&cnt[code_index(w3 & 0xFFFFu, ncol)], 1u);
+        atomicAdd(&cnt[code_index(w3 >> 16, ncol)], 1u);
+        wave_lds_sync();
+        for (uint32_t i = lane; i < ne; i += 64u) {
+          const uint32_t k = cnt[i];
+          if (k) {
+            g[i] += wp * (double)k;
+            cnt[i] = 0u;
+          }
         }
+        wave_lds_sync();
       }
-      wave_lds_sync();
     }
   }
   __syncthreads();
-  double* out = a.block_grad + (size_t)blockIdx.x * ne;
-  for (uint32_t i = threadIdx.x; i < ne; i += blockDim.x) out[i] = hist[i];
+  for (uint32_t i = threadIdx.x; i < ne; i += blockDim.x) {  // bin-major partials
+    double s = grad[i];
+    for (int w = 1; w < kStWaves; ++w) s += grad[(size_t)w * ne + i];
+    a.block_grad[(size_t)i * gridDim.x + blockIdx.x] = s;
+  }
 }
 
-// Sums per-block partials in block order: out[i] = sum_b part[b * m + i].
-__global__ void reduce_blocks_kernel(const double* part, uint32_t nblocks, uint32_t m, double* out) {
-  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < m; i += gridDim.x * blockDim.x) {
-    double s = 0.0;
-    for (uint32_t b = 0; b < nblocks; ++b) s += part[(size_t)b * m + i];
-    out[i] = s;
+// out[i] = sum over blocks b of part[i * nblocks + b], one workgroup per i:
+// thread t adds blocks t, t + 256, ... in order, then a fixed tree over the
+// threads, so the sum's bits do not depend on scheduling.
+__global__ __launch_bounds__(256) void reduce_blocks_kernel(const double* part, uint32_t nblocks,
+                                                            double* out) {
+  __shared__ double sh[256];
+  const uint32_t i = blockIdx.x, t = threadIdx.x;
+  const double* row = part + (size_t)i * nblocks;
+  double s = 0.0;
+  for (uint32_t b = t; b < nblocks; b += 256u) s += row[b];
+  sh[t] = s;
+  __syncthreads();
+  for (uint32_t h = 128; h > 0; h >>= 1) {
+    if (t < h) sh[t] += sh[t + h];
+    __syncthreads();
   }
+  if (t == 0) out[i] = sh[0];
 }
 
 static LoglikDev make_dev(const LoglikArgs& s) {
@@ -164,22 +203,24 @@ hipError_t launch_loglik_fwd(const LoglikArgs& s, const float* logE, double* los
   hipLaunchKernelGGL(loglik_fwd_kernel, dim3(s.grid), dim3(64 * kStWaves), lds, st, d, logE);
   e = hipGetLastError();
   if (e != hipSuccess) return e;
-  hipLaunchKernelGGL(reduce_blocks_kernel, dim3(1), dim3(64), 0, st, (const double*)s.block_part,
-                     s.grid, 1u, loss);
+  hipLaunchKernelGGL(reduce_blocks_kernel, dim3(1), dim3(256), 0, st, (const double*)s.block_part,
+                     s.grid, loss);
   return hipGetLastError();
 }
 
 hipError_t launch_loglik_bwd(const LoglikArgs& s, double* grad, hipStream_t st) {
   LoglikDev d = make_dev(s);
   const uint32_t m = 16u * s.ncol;
-  const size_t lds = ((size_t)m + 1) * sizeof(double) + (size_t)kStTile * kStRow;
+  const size_t lds = (size_t)kStWaves * m * sizeof(double) +
+                     ((size_t)kStWaves * (m + 1) + 1) * sizeof(uint32_t) + 8 +
+                     (size_t)kStTile * kStRow;
   hipError_t e = lds_attr((const void*)loglik_bwd_kernel, lds);
   if (e != hipSuccess) return e;
   hipLaunchKernelGGL(loglik_bwd_kernel, dim3(s.grid), dim3(64 * kStWaves), lds, st, d);
   e = hipGetLastError();
   if (e != hipSuccess) return e;
-  hipLaunchKernelGGL(reduce_blocks_kernel, dim3((m + 255) / 256), dim3(256), 0, st,
-                     (const double*)s.block_part, s.grid, m, grad);
+  hipLaunchKernelGGL(reduce_blocks_kernel, dim3(m), dim3(256), 0, st,
+                     (const double*)s.block_part, s.grid, grad);
   return hipGetLastError();
 }
 
