@@ -360,9 +360,13 @@ int d2g_align_vectorized(d2g_ctx* ctx, uint32_t n, const uint8_t* s1, const uint
  * of the active width; nwalign.rs:875-1088) — the GCUPS unit. */
 uint64_t d2g_vectorized_cells(uint32_t len1, uint32_t len2, int32_t band);
 
-/* 1 when the vectorized DP kernel handles this geometry (compressed width
- * ncol <= 64 and its wavefront fits in LDS), 0 otherwise.  Pairs outside it
- * make d2g_align_vectorized / d2g_compare_* return D2G_ERR_UNSUPPORTED. */
+/* 1 when a device kernel aligns this geometry with the vectorized aligner:
+ * both lengths <= 3500 (nwalign.rs:1383) and a compressed band of at most
+ * 4096 columns and 32 Mi pointer cells (up to 64 columns in the
+ * many-pairs-per-wavefront kernel, wider in the one-pair kernel), 0
+ * otherwise.  Pairs outside it make d2g_align_vectorized return
+ * D2G_ERR_UNSUPPORTED; the compare routes pairs above 3500 nt to the
+ * scalar ends-free aligner as raw_align_dp does. */
 int d2g_vectorized_supported(uint32_t len1, uint32_t len2, int32_t band);
 
 /* Backs align_endsfree_with_buf (nwalign.rs:336-417, when homo_gap_p ==
