@@ -160,6 +160,7 @@ def _lib():
             "d2g_ctx_set_graphs": (C.c_int, [vp, C.c_int]),
             "d2g_ctx_set_screen_shape": (C.c_int, [vp, C.c_int, C.c_int, C.c_int]),
             "d2g_ctx_set_autotune": (C.c_int, [vp, C.c_int]),
+            "d2g_ctx_set_dp_tile": (C.c_int, [vp, C.c_int]),
             "d2g_run_dada": (C.c_int, [vp, f64p, C.c_uint32, C.POINTER(AlignParams),
                                        C.POINTER(DadaParams), u8p, C.c_uint32,
                                        C.POINTER(C.c_uint32), C.POINTER(ClusterInfo), u32p, f64p,
@@ -195,7 +196,7 @@ def exported_symbols() -> list[str]:
             "d2g_ctx_set_screen_shape", "d2g_ctx_set_autotune", "d2g_run_dada",
             "d2g_transition_counts", "d2g_build_trans_mat", "d2g_transition_stats",
             "d2g_dada_aux", "d2g_birth_subs", "d2g_pairs_lambda_noalloc", "d2g_store_reset",
-            "d2g_compare_store", "d2g_store_size", "d2g_store_fetch"]
+            "d2g_compare_store", "d2g_store_size", "d2g_store_fetch", "d2g_ctx_set_dp_tile"]
 
 
 def _p(a: np.ndarray, t):
@@ -404,6 +405,12 @@ class Context:
         """Time the screen's launch shapes on the first compare of each
         workload class and keep the fastest (default off)."""
         _check(_lib().d2g_ctx_set_autotune(self.h, int(on)), "d2g_ctx_set_autotune")
+
+    def set_dp_tile(self, tile: int = -1):
+        """Fix the DP tile (vectorized: pairs per wavefront cap, 0 = all that
+        fit; row band: least cells per lane 1, 2, 4); -1 leaves it to the
+        autotuner."""
+        _check(_lib().d2g_ctx_set_dp_tile(self.h, int(tile)), "d2g_ctx_set_dp_tile")
 
     def set_screen_shape(self, qblock: int = 64, split: int = 4, per_lane: int = 1):
         """Screen launch shape: uniques per workgroup (64, 128, 256), the

@@ -84,7 +84,7 @@ struct d2g_ctx {
   size_t sub_capacity = 0;
   uint32_t last_nc = 0;
   // cached per bin space: LDS sizing and the row schedule of the DP kernel
-  BinSpace lds_key = {-1, -1, -1, -1, -1, -1};
+  BinSpace lds_key = {-1, -1, -1, -1, -1, -1, -1};
   int sched_gap = 0, sched_end_gap = 0;
   size_t lds_bytes = 0;
   int sched_rows = 0;
@@ -98,7 +98,7 @@ struct d2g_ctx {
   DevBuf<uint32_t> hist;
   // the same state for the other DP kernel (a compare over a unique set on
   // both sides of the vectorized length limit alternates the two)
-  BinSpace st_key = {-1, -1, -1, -1, -1, -1};
+  BinSpace st_key = {-1, -1, -1, -1, -1, -1, -1};
   int st_gap = 0, st_end_gap = 0;
   size_t st_lds_bytes = 0;
   int st_rows = 0;
@@ -135,6 +135,10 @@ struct d2g_ctx {
   // autotuned screen shape per workload class (qblock, split, per_lane)
   bool autotune = false;
   std::map<uint64_t, std::array<int, 3>> tuned;
+  // autotuned DP tile (BinSpace.tile) per workload class; forced_tile >= 0
+  // overrides it (d2g_ctx_set_dp_tile)
+  std::map<uint64_t, int> tuned_tile;
+  int forced_tile = -1;
   // the screen's shroud thresholds, for one (cutoff, k, longest unique)
   DevBuf<uint32_t> dstar;
   uint64_t dstar_cut = 0;
@@ -323,6 +327,12 @@ int d2g_ctx_set_autotune(d2g_ctx* c, int enable) {
   return D2G_OK;
 }
 
+int d2g_ctx_set_dp_tile(d2g_ctx* c, int tile) {
+  if (!c || tile < -1 || tile > 16) return D2G_ERR_ARG;
+  c->forced_tile = tile;
+  return D2G_OK;
+}
+
 int d2g_ctx_set_screen_shape(d2g_ctx* c, int qblock, int split, int per_lane) {
   if (!c || (split != 1 && split != 2 && split != 4)) return D2G_ERR_ARG;
   if (qblock != 64 && qblock != 128 && qblock != 256) return D2G_ERR_ARG;
@@ -489,6 +499,7 @@ static int prepare_schedule(d2g_ctx* c, const BinSpace& bs, int end_gap, int gap
   auto hit = [&]() {
     return bs.lmin == c->lds_key.lmin && bs.range == c->lds_key.range &&
            bs.band == c->lds_key.band && bs.kind == c->lds_key.kind && bs.homo == c->lds_key.homo &&
+           bs.tile == c->lds_key.tile &&
            end_gap == c->sched_end_gap && gap == c->sched_gap && c->sched_info.p &&
            present == c->sched_present;
   };
@@ -535,7 +546,7 @@ static int prepare_schedule(d2g_ctx* c, const BinSpace& bs, int end_gap, int gap
     for (int b = 0; b < bs.nbins; ++b) {
       int l1, l2;
       bin_lengths(bs, (uint32_t)b, &l1, &l2);
-      kmax = std::max(kmax, band_geom(l1, l2, bs.band).k);
+      kmax = std::max(kmax, band_geom(l1, l2, bs.band, bs.tile).k);
     }
   c->hist_stride = nw_hist_p_words(2 * lmax + 1) * (size_t)kmax;
   while (c->nw_grid > (uint32_t)c->cu_count && (size_t)c->nw_grid * c->hist_stride * 4 > ((size_t)1 << 30))
@@ -669,7 +680,13 @@ static int compare_pass(d2g_ctx* c, const uint32_t* centers, uint32_t nc, const 
   bs.band = p->band;
   bs.kind = rt.kind;
   bs.homo = rt.homo;
+  bs.tile = c->forced_tile >= 0 ? c->forced_tile : 0;
   if ((size_t)bs.range * bs.range > (1u << 22)) return D2G_ERR_UNSUPPORTED;
+  if (c->autotune && !tm && c->forced_tile < 0) {
+    bs.nbins = bs.range * bs.range;
+    const auto it = c->tuned_tile.find(tune_class(n, nc, u.k, bs));
+    if (it != c->tuned_tile.end()) bs.tile = it->second;
+  }
   bs.nbins = bs.range * bs.range;
   const bool vec = bs.kind == kDpVectorized;
   // the lengths of the uniques in the bin space
@@ -937,6 +954,33 @@ static int compare_pass(d2g_ctx* c, const uint32_t* centers, uint32_t nc, const 
     c->screen_qblock = (uint32_t)it->second[0];
     c->screen_split = it->second[1];
     c->screen_per_lane = it->second[2];
+    // then the DP tile: pairs per wavefront (vectorized: as many as fit, at
+    // most 2, 1) or cells per lane (row band: 1, 2, 4), the DP kernel timed
+    // on this call's pairs
+    if (c->forced_tile < 0 && c->tuned_tile.find(cls) == c->tuned_tile.end()) {
+      static const int kVecTiles[] = {0, 2, 1};
+      static const int kBandTiles[] = {1, 2, 4};
+      const int* tiles = vec ? kVecTiles : kBandTiles;
+      float best = 0.f;
+      int best_tile = tiles[0];
+      for (int i = 0; i < 3; ++i) {
+        bs.tile = tiles[i];
+        HIP_TRY_INT(prepare_schedule(c, bs, 0, p->gap_p, present));
+        float ms = 0.f;
+        for (int rep = 0; rep < 2; ++rep) {  // the second run is timed
+          HIP_TRY_INT(enqueue(true));
+          HIP_TRY(hipEventSynchronize(c->ev[4]));
+          ms = ev_ms(c, 2, 3);
+        }
+        if (i == 0 || ms < best) {
+          best = ms;
+          best_tile = tiles[i];
+        }
+      }
+      c->tuned_tile.emplace(cls, best_tile);
+      bs.tile = best_tile;
+      HIP_TRY_INT(prepare_schedule(c, bs, 0, p->gap_p, present));
+    }
   }
 
   for (int attempt = 0; attempt < 6; ++attempt) {
@@ -946,6 +990,7 @@ static int compare_pass(d2g_ctx* c, const uint32_t* centers, uint32_t nc, const 
       std::vector<uint64_t> key = {
           nc, n, (uint64_t)(uintptr_t)u.seq, (uint64_t)(uintptr_t)dlock, (uint64_t)greedy,
           (uint64_t)use_quals, ncol, (uint64_t)bs.lmin, (uint64_t)bs.range, (uint64_t)bs.band,
+          (uint64_t)(uint32_t)bs.tile,
           (uint64_t)bs.kind, (uint64_t)bs.homo, (uint64_t)p->use_kmers, (uint64_t)p->gapless,
           (uint64_t)p->match_score, (uint64_t)(uint32_t)p->mismatch, (uint64_t)(uint32_t)p->gap_p,
           (uint64_t)(uint32_t)p->homo_gap_p, (uint64_t)p->kmer_size, sub_stride, c->sub_capacity,
@@ -1717,6 +1762,7 @@ static int align_pairs_device(d2g_ctx* c, uint32_t n, const PairSeqs& ps, int ki
   bs.band = band;
   bs.kind = kind;
   bs.homo = homo;
+  bs.tile = 0;
   if ((size_t)bs.range * bs.range > (1u << 22)) return D2G_ERR_UNSUPPORTED;
   bs.nbins = bs.range * bs.range;
   std::vector<DpTask> tasks(n);

@@ -156,8 +156,9 @@ D2G_HD unsigned long long vec_cells(int l1_in, int l2_in, int band_in) {
 }
 
 // Pairs per wavefront for one geometry (0 = unsupported).
-D2G_HD int nw_groups(const VecGeom& g) {
+D2G_HD int nw_groups(const VecGeom& g, int cap = 0) {
   int G = vec_groups(g.ncol);
+  if (cap > 0 && G > cap) G = cap;
   if (G > kNwMaxGroups) G = kNwMaxGroups;
   while (G > 1 && nw_wave_bytes(g, G) > kNwLdsSoft) --G;
   if (G >= 1 && nw_wave_bytes(g, G) > kNwLdsHard) G = 0;
@@ -180,7 +181,7 @@ struct BandGeom {
   bool sentinels;
 };
 
-D2G_HD BandGeom band_geom(int l1, int l2, int band) {
+D2G_HD BandGeom band_geom(int l1, int l2, int band, int kmin = 1) {
   BandGeom g;
   g.len1 = l1;
   g.len2 = l2;
@@ -195,6 +196,7 @@ D2G_HD BandGeom band_geom(int l1, int l2, int band) {
   }
   g.dmax = g.lband + g.rband + 2;
   g.k = 1;
+  while (g.k < kBandMaxK && g.k < kmin) g.k *= 2;
   while (g.k < kBandMaxK && (g.dmax + 1 + 2 * g.k - 1) / (2 * g.k) > kWave) g.k *= 2;
   g.lanes = (g.dmax + 1 + 2 * g.k - 1) / (2 * g.k);
   g.nrow = l1 + l2 + 1;
@@ -260,6 +262,10 @@ struct GaplessTask {
 // Binning of DP tasks: bin = (len1_in - lmin) * range + (len2_in - lmin).
 // kind: the DP kernel of every bin (kDpVectorized or kDpRowBand); homo: the
 // row-band kernel applies homopolymer gap penalties.
+// tile: the DP tile shape the autotuner picks (results never depend on
+// it): for the vectorized kernel a cap on pairs per wavefront (0: as many
+// as fit), for the row-band kernel the least cells per lane (0 or 1: the
+// fewest that fit the band into one wavefront).
 struct BinSpace {
   int lmin;
   int range;
@@ -267,6 +273,7 @@ struct BinSpace {
   int band;
   int kind;
   int homo;
+  int tile;
 };
 
 D2G_HD void bin_lengths(const BinSpace& bs, uint32_t bin, int* l1, int* l2) {
@@ -283,8 +290,8 @@ inline bool bin_present(const BinSpace& bs, const uint8_t* present, int l1, int 
 
 // Pairs per wavefront of a bin for its kernel (0 = unsupported geometry).
 D2G_HD int bin_groups(const BinSpace& bs, int l1, int l2) {
-  if (bs.kind == kDpRowBand) return band_groups(band_geom(l1, l2, bs.band), bs.homo != 0);
-  return nw_groups(vec_geom(l1, l2, bs.band));
+  if (bs.kind == kDpRowBand) return band_groups(band_geom(l1, l2, bs.band, bs.tile), bs.homo != 0);
+  return nw_groups(vec_geom(l1, l2, bs.band), bs.tile);
 }
 
 // Device-side counters of one compare call.

@@ -71,7 +71,7 @@ __global__ void band_info_kernel(BinSpace bs, BinInfo* info) {
   if (b >= bs.nbins) return;
   int l1, l2;
   bin_lengths(bs, (uint32_t)b, &l1, &l2);
-  const BandGeom g = band_geom(l1, l2, bs.band);
+  const BandGeom g = band_geom(l1, l2, bs.band, bs.tile);
   BinInfo bi;
   bi.steady_lo = 0;
   bi.steady_hi = 0;
@@ -682,7 +682,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4, 8))) void
     const int bin = uni(lo);
     int l1, l2;
     bin_lengths(bs, (uint32_t)bin, &l1, &l2);
-    const BandGeom geo = band_geom(l1, l2, bs.band);
+    const BandGeom geo = band_geom(l1, l2, bs.band, bs.tile);
     const int G = band_groups(geo, HOMO);
     const uint32_t first = a.bin_start[bin] + (w - a.wave_start[bin]) * (uint32_t)G;
     const int count = uni((int)min((uint32_t)G, a.bin_start[bin + 1] - first));
@@ -778,7 +778,7 @@ size_t nw_band_lds_for(const BinSpace& bs, const uint8_t* present) {
     int l1, l2;
     bin_lengths(bs, (uint32_t)b, &l1, &l2);
     if (!bin_present(bs, present, l1, l2)) continue;
-    const BandGeom g = band_geom(l1, l2, bs.band);
+    const BandGeom g = band_geom(l1, l2, bs.band, bs.tile);
     const int G = band_groups(g, bs.homo != 0);
     if (G > 0) lds = std::max(lds, band_wave_bytes(g, G, bs.homo != 0));
   }

@@ -163,7 +163,7 @@ __global__ void sched_kernel(BinSpace bs, const uint64_t* off, int end_gap, int 
   int l1, l2;
   bin_lengths(bs, (uint32_t)b, &l1, &l2);
   const VecGeom g = vec_geom(l1, l2, bs.band);
-  if (nw_groups(g) == 0) {  // the one-pair kernel's bins: only their cell count
+  if (nw_groups(g, bs.tile) == 0) {  // the one-pair kernel's bins: only their cell count
     BinInfo bi;
     bi.steady_lo = 0;
     bi.steady_hi = 0;
@@ -177,7 +177,7 @@ __global__ void sched_kernel(BinSpace bs, const uint64_t* off, int end_gap, int 
   const int last = g.len1 + g.len2;
   // active-lane masks over the G pairs of a wavefront (pair gi on lanes
   // gi * ncol .. gi * ncol + ncol - 1)
-  const int G = nw_groups(g);
+  const int G = nw_groups(g, bs.tile);
   for (int row = 2; row <= last; ++row) {
     RowRec& R = t[row - 2];
     uint64_t base = 0;
@@ -779,7 +779,7 @@ size_t nw_vec_lds_for(const BinSpace& bs, const uint8_t* present) {
     bin_lengths(bs, (uint32_t)b, &l1, &l2);
     if (!bin_present(bs, present, l1, l2)) continue;
     VecGeom g = vec_geom(l1, l2, bs.band);
-    int G = nw_groups(g);
+    int G = nw_groups(g, bs.tile);
     if (G > 0) {
       size_t need = nw_wave_bytes(g, G);
       if (need > lds) lds = need;

@@ -101,6 +101,13 @@ int d2g_ctx_set_screen_shape(d2g_ctx* ctx, int qblock, int split, int per_lane);
  * fastest.  Results do not depend on it.  No counterpart in the
  * reference. */
 int d2g_ctx_set_autotune(d2g_ctx* ctx, int enable);
+/* The DP tile shape the autotuner would pick, fixed: for the vectorized
+ * kernel a cap on pairs per wavefront (0: as many as fit), for the scalar
+ * row-band kernel the least cells per lane (1, 2 or 4); -1 (the default)
+ * leaves it to the autotuner (with autotuning off: as many pairs / as few
+ * cells as fit).  Results do not depend on it.  No counterpart in the
+ * reference. */
+int d2g_ctx_set_dp_tile(d2g_ctx* ctx, int tile);
 /* The hipStream_t every kernel of this context is launched on. */
 void* d2g_ctx_stream(d2g_ctx* ctx);
 /* Device properties used for roofline figures. */

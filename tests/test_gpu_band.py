@@ -119,3 +119,25 @@ def test_band_wide_geometries(ctx, oracle, pkg):
         _check_pairs(ctx, oracle, pairs, band)
         _check_pairs(ctx, oracle, pairs, band, homo_gap=-1)
     assert not pkg.band_supported(600, 600, -1, True)  # 1203 diagonals: beyond 4 x 128
+
+
+@pytest.mark.parametrize("tile", [1, 2, 4])
+def test_compare_long_reads_every_dp_tile(pkg, oracle, tile):
+    """The row-band kernel at 1, 2 and 4 cells per lane (the DP tiles the
+    autotuner chooses among): the compare's results do not change."""
+    from dada2_rs_amd import synth
+    w = synth.workload_longread(n_variants=4, n_uniques=120, seed=synth.SEED_BASE + 42)
+    rs = oracle.RawSet(w.seqs, w.quals, w.reads, k=w.kmer_size)
+    oparams = oracle.make_params(band=32, homo_gap_p=-1)
+    c = pkg.Context(0)
+    try:
+        c.load_uniques(w.seqs, w.quals, w.reads, k=w.kmer_size)
+        c.set_dp_tile(tile)
+        got = c.compare_batch(w.centers, w.err, pkg.align_params(band=32, homo_gap_p=-1))
+    finally:
+        c.close()
+    assert got.stats["aligned_dp"] > 0
+    for r, cc in enumerate(int(x) for x in w.centers):
+        lam, ham, skp = oracle.compare_center(rs, cc, w.err, w.err.shape[1], oparams, nthreads=8)[:3]
+        assert np.array_equal(got.lam[r].view(np.uint64), lam.view(np.uint64)), cc
+        assert np.array_equal(got.hamming[r], ham), cc

@@ -257,3 +257,34 @@ def test_u16_fallback_on_saturated_buckets(ctx, oracle, pkg):
                      np.stack([r[2] for r in rows]))
     # the saturated pairs aligned (the fallback passed them), the random ones shrouded
     assert (got.hamming[:, :60] != 0xFFFFFFFF).sum() > 0
+
+
+@pytest.mark.parametrize("tile", [0, 2, 1])
+def test_vectorized_dp_tiles_same_results(pkg, oracle, batched_small, tile):
+    """nw_vec with every pair count per wavefront the autotuner tries."""
+    w = batched_small
+    c = pkg.Context(0)
+    try:
+        c.load_uniques(w.seqs, w.quals, w.reads, k=5)
+        c.set_dp_tile(tile)
+        cs = w.centers[:8]
+        _assert_same(c.compare_batch(cs, w.err), *_oracle_rows(oracle, w, cs))
+    finally:
+        c.close()
+
+
+def test_autotuned_long_read_dp_tile(pkg, oracle):
+    """Autotuning on the long-read shape times the row-band tiles; the
+    first and the next call match the oracle."""
+    w = synth.workload_longread(n_variants=3, n_uniques=90, seed=synth.SEED_BASE + 43)
+    params = pkg.align_params(band=32, homo_gap_p=-1)
+    op = oracle.make_params(band=32, homo_gap_p=-1)
+    want = _oracle_rows(oracle, w, w.centers, params=op)
+    c = pkg.Context(0)
+    try:
+        c.set_autotune(True)
+        c.load_uniques(w.seqs, w.quals, w.reads, k=w.kmer_size)
+        _assert_same(c.compare_batch(w.centers, w.err, params), *want)
+        _assert_same(c.compare_batch(w.centers, w.err, params), *want)
+    finally:
+        c.close()
