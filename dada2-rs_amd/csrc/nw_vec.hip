@@ -467,6 +467,7 @@ __device__ __forceinline__ void forward(CRowRec* rec, const int b0, const int b1
 // Per-wave constants of the traceback.
 struct BwConst {
   CRowRec* rec;
+  const uint32_t* rec_v;  // the same records, for per-lane (vector) loads
   int gpos;  // lane of column 0 of this lane's pair
   int c, start_col;
   bool swap, real, sub_ok;
@@ -560,6 +561,82 @@ __device__ __forceinline__ void backward(const int last, const uint32_t* hist_p,
   if (top >= 1) {
     hp = hp_next;
     for (int r = 15; r >= 1; --r) bw_row<EMIT>(r, r, r & 1, hp, ps, B, a);
+  }
+}
+
+// The compare-mode traceback by runs.  A diagonal step keeps the compressed
+// column and goes two rows up, so within one 16-row history word the
+// pointers of rows r, r-2, r-4, ... of a column sit 4 bits apart: shifted to
+// row r, a run of diagonal-over-match pointers (1) is the count of trailing
+// zero fields of (t ^ 0x11111111) & 0x33333333.  Each iteration moves every
+// pair's path over its run in the current block and then takes at most one
+// other step (a mismatching diagonal: an al2subs substitution; a gap; or the
+// reference's invalid pointer), so a block costs one iteration per non-run
+// step of the wavefront's busiest pair instead of one per row.  The path
+// visits the cells of the row sweep above in the same order, so the
+// substitution records and the error flag are the same.
+__device__ __forceinline__ void backward_runs(const int last, const uint32_t* hist_p,
+                                              const int lane, PathState& ps, const BwConst& B,
+                                              const NwDev& a) {
+  int b = last >> 4;
+  uint32_t hp = hist_p[b * 64 + lane];
+  uint32_t hp_next = b >= 1 ? hist_p[(b - 1) * 64 + lane] : 0u;
+  for (; b >= 0; --b) {
+    const int base = b << 4;
+    const int lo = b == 0 ? 1 : base;  // row 0 is the path's end
+    for (;;) {
+      // the path cell is in this block (rc <= base + 15 holds: rc starts at
+      // last and leaves a block only downwards)
+      const bool on = ps.rc >= lo;
+      if (!__builtin_amdgcn_ballot_w64(on)) break;
+      const uint32_t w = (uint32_t)__builtin_amdgcn_ds_bpermute(ps.addr, (int)hp);
+      const int k = ps.rc - base;
+      // field j of t: the pointer of row rc - 2j; fields past row 0 / 1 of
+      // the block are zero, which ends a run (k = 15 alone can fill all 8)
+      const uint32_t t = w >> ((30 - 2 * k) & 31);
+      const uint32_t m = (t ^ 0x11111111u) & 0x33333333u;
+      const int n = m ? (__builtin_ctz(m) >> 2) : 8;
+      const int r = ps.rc - 2 * n;  // first cell after the run
+      const int pp = (int)((t >> ((4 * n) & 31)) & 3u);
+      const bool step = on && r >= lo;  // a non-run pointer inside the block
+      const bool odd = (r & 1) != 0;
+      bool sub = false, invalid = false;
+      if (__builtin_amdgcn_ballot_w64(step && pp == 0)) {
+        // pointer 0: a mismatching diagonal on an active cell, else invalid
+        const int cc = (ps.addr >> 2) - B.gpos;
+        bool act = false;
+        if (step && pp == 0 && r >= 2) {
+          const int rcmin = (int)B.rec_v[kRecWords * (r - 2) + 2];
+          const uint32_t rn = B.rec_v[kRecWords * (r - 2) + 3];
+          act = (unsigned)(cc - rcmin) < rn;
+        }
+        sub = step && pp == 0 && act;
+        invalid = step && pp == 0 && !act;
+        if (sub && B.c == cc && B.real && B.sub_ok) {
+          const int dd = 2 * (cc - B.start_col) + (odd ? 1 : 0);  // j - i on row r
+          const int i = (r - dd) >> 1, j = (r + dd) >> 1;
+          const int qp = B.swap ? i - 1 : j - 1;
+          const int cp = B.swap ? j - 1 : i - 1;
+          a.sub_pool[B.my_sub + ps.ns] = ((uint32_t)qp << 16) | (uint32_t)cp;
+        }
+        ps.ns += sub ? 1 : 0;
+      }
+      // gaps: left (2) goes one column down on even rows, up (3) one column
+      // up on odd rows; both go one row up
+      const bool gap = step && pp >= 2;
+      ps.addr += (gap && odd && pp == 3) ? 4 : 0;
+      ps.addr -= (gap && !odd && pp == 2) ? 4 : 0;
+      int rc = on ? r : ps.rc;
+      rc = sub ? r - 2 : rc;
+      rc = gap ? r - 1 : rc;
+      rc = invalid ? -1 : rc;
+      ps.bad = ps.bad || invalid;
+      ps.rc = rc;
+    }
+    if (b >= 1) {
+      hp = hp_next;
+      hp_next = b >= 2 ? hist_p[(b - 2) * 64 + lane] : 0u;
+    }
   }
 }
 
@@ -667,6 +744,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(8, 8))) void
       atomicOr(&a.ctr->err_flags, kErrSubOverflow);
     BwConst B;
     B.rec = rec;
+    B.rec_v = reinterpret_cast<const uint32_t*>(a.sched + a.sched_off[bin]);
     B.gpos = g_raw * ncol;
     B.c = c;
     B.start_col = start_col;
@@ -686,7 +764,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(8, 8))) void
     if (a.emit)
       backward<true>(last, hist_p, lane, ps, B, a);
     else
-      backward<false>(last, hist_p, lane, ps, B, a);
+      backward_runs(last, hist_p, lane, ps, B, a);
     const int ns = ps.ns, nt = ps.nt;
     const bool bad = ps.bad;
     if (__builtin_amdgcn_ballot_w64(bad && real) && lane == 0)
