@@ -60,6 +60,7 @@ struct ScreenDev {
   // the pairs of this pass: with a center of length <= long_len, uniques
   // [sr_lo, sr_hi); with a longer center, uniques [lr_lo, lr_hi)
   uint32_t long_len, sr_lo, sr_hi, lr_lo, lr_hi;
+  uint32_t nx, ny;  // screen_kernel's grid: tiles of uniques x rows of 16 centers
 };
 
 __device__ __forceinline__ bool pair_in_pass(const ScreenDev& a, uint32_t center_len, uint32_t q) {
@@ -124,35 +125,30 @@ struct CenterFields {
   bool has255;
 };
 
-template <int QB, int SPLIT, int QL>
-__global__ __launch_bounds__(QB / QL * SPLIT) void screen_kernel(ScreenDev a) {
-  constexpr int CW = (int)kScreenCenters / SPLIT;    // centers per wavefront
-  constexpr int GT = QB / QL;                          // threads per wavefront group
-  constexpr uint32_t kRegion = QB * kScreenCenters;  // one task slot per pair of the block
+// Indices of a wavefront's CW centers from c0 (centers past the end repeat
+// the last one; their results are unused) and of its lanes' QL uniques
+// wave_base + 64 j + lane (lanes past the end repeat the last unique).
+template <int QL, int CW>
+__device__ __forceinline__ void load_ids(const ScreenDev& a, const uint32_t c0,
+                                         const uint32_t wave_base, const uint32_t lane,
+                                         CenterFields (&cf)[CW], QueryFields (&qf)[QL]) {
   const UniqueDev& u = a.u;
-  // this block's task lists: kRegion slots each, counted in LDS
-  __shared__ uint32_t s_ngl, s_ndp;
-  if (threadIdx.x == 0) {
-    s_ngl = 0;
-    s_ndp = 0;
-  }
-  __syncthreads();
-  const uint32_t blk = blockIdx.y * gridDim.x + blockIdx.x;
-  GaplessTask* gl_region = a.gapless_tasks + (size_t)blk * kRegion;
-  DpTask* dp_region = a.dp_unsorted + (size_t)blk * kRegion;
-  const uint32_t part = uni(threadIdx.x / GT);
-  const uint32_t in_group = threadIdx.x % GT;
-  const uint32_t wave_base = blockIdx.x * QB + (in_group / 64u) * 64u * QL;
-  const uint32_t lane = lane_id();
-  const uint32_t c0 = blockIdx.y * kScreenCenters + part * CW;
-  const uint32_t nc = c0 < a.n_centers ? min((uint32_t)CW, a.n_centers - c0) : 0u;
-  const uint32_t k = u.k;
-
-  // center fields, requested together (centers past the end repeat the
-  // last one; their results are unused)
-  CenterFields cf[CW];
 #pragma unroll
   for (int c = 0; c < CW; ++c) cf[c].ci = a.centers[min(c0 + (uint32_t)c, a.n_centers - 1)];
+#pragma unroll
+  for (int j = 0; j < QL; ++j) {
+    const uint32_t q = wave_base + 64u * (uint32_t)j + lane;
+    qf[j].valid = q < u.n;
+    qf[j].qq = qf[j].valid ? q : u.n - 1;
+  }
+}
+
+// The other fields of those centers and uniques.
+template <int QL, int CW>
+__device__ __forceinline__ void load_rest(const ScreenDev& a, CenterFields (&cf)[CW],
+                                          QueryFields (&qf)[QL]) {
+  const UniqueDev& u = a.u;
+  const uint32_t k = u.k;
 #pragma unroll
   for (int c = 0; c < CW; ++c) {
     cf[c].len = u.len[cf[c].ci];
@@ -160,12 +156,8 @@ __global__ __launch_bounds__(QB / QL * SPLIT) void screen_kernel(ScreenDev a) {
     cf[c].sum = u.ksum[cf[c].ci];
     cf[c].has255 = u.has255[cf[c].ci] != 0;
   }
-  QueryFields qf[QL];
 #pragma unroll
   for (int j = 0; j < QL; ++j) {
-    const uint32_t q = wave_base + 64u * (uint32_t)j + lane;
-    qf[j].valid = q < u.n;
-    qf[j].qq = qf[j].valid ? q : u.n - 1;
     qf[j].len = u.len[qf[j].qq];
     qf[j].reads = u.reads[qf[j].qq];
     qf[j].lock = a.lock ? a.lock[qf[j].qq] != 0 : false;
@@ -178,46 +170,33 @@ __global__ __launch_bounds__(QB / QL * SPLIT) void screen_kernel(ScreenDev a) {
 #pragma unroll
     for (int j = 0; j < QL; ++j) qf[j].thr = a.dstar[qf[j].len - k + 1];
   }
+}
 
-  // sum |a - b| over the dense vectors, four buckets per v_sad_u8
-  uint32_t sad[QL][CW];
+// The decision for each (center, unique) pair of a lane, once its sums of
+// |a - b| are in: greedy skip, shroud (with the exact kmer_dist8 where both
+// vectors hold a 255 bucket), gapless or DP route, and the append to the
+// block's task lists.  All lanes of the wavefront call it (the appends use
+// ballots); nc centers of cf are live.
+template <int QL, int CW>
+__device__ __forceinline__ void screen_pairs(const ScreenDev& a, const QueryFields (&qf)[QL],
+                                             const CenterFields (&cf)[CW],
+                                             const uint32_t (&sad)[QL][CW], const uint32_t nc,
+                                             const uint32_t c0, uint32_t* s_ngl, uint32_t* s_ndp,
+                                             GaplessTask* gl_region, DpTask* dp_region,
+                                             uint32_t& n_skip, uint32_t& n_shroud) {
+  const UniqueDev& u = a.u;
+  const uint32_t k = u.k;
+  // unrolled over the centers: the per-center arrays are read at constant
+  // indices (a runtime index would put them in scratch memory)
 #pragma unroll
-  for (int j = 0; j < QL; ++j)
-#pragma unroll
-    for (int c = 0; c < CW; ++c) sad[j][c] = 0;
-  if (a.use_kmers && nc > 0) {
-    for (uint32_t w0 = 0; w0 < u.kwords; w0 += kScreenChunk) {
-      uint32_t qw[QL][kScreenChunk];
-#pragma unroll
-      for (int j = 0; j < QL; ++j)
-#pragma unroll
-        for (int t = 0; t < kScreenChunk; ++t)
-          qw[j][t] = u.kmer_il[(size_t)(w0 + t) * u.npad + qf[j].qq];
-#pragma unroll
-      for (int c = 0; c < CW; ++c) {
-        const uint32_t* cw = u.kmer_row + (size_t)cf[c].ci * u.kwords + w0;
-#pragma unroll
-        for (int t = 0; t < kScreenChunk; ++t)
-#pragma unroll
-          for (int j = 0; j < QL; ++j) sad[j][c] = __builtin_amdgcn_sad_u8(qw[j][t], cw[t], sad[j][c]);
-      }
-    }
-  }
-
-  uint32_t n_skip = 0, n_shroud = 0;
-  for (uint32_t c = 0; c < nc; ++c) {
-    // (the per-center arrays are read with a uniform index: select chains)
-    CenterFields C = cf[0];
-#pragma unroll
-    for (int cc = 1; cc < CW; ++cc)
-      if ((uint32_t)cc == c) C = cf[cc];
+  for (int c = 0; c < CW; ++c) {
+    if ((uint32_t)c >= nc) continue;
+    const CenterFields& C = cf[c];
 #pragma unroll
     for (int j = 0; j < QL; ++j) {
       const QueryFields& Q = qf[j];
       const bool in = Q.valid && pair_in_pass(a, C.len, Q.qq);
-      uint32_t sd = sad[j][0];
-#pragma unroll
-      for (int cc = 1; cc < CW; ++cc) sd = (uint32_t)cc == c ? sad[j][cc] : sd;
+      const uint32_t sd = sad[j][c];
       // results stay in the length-sorted order (coalesced); d2g_fetch_results
       // restores the caller's unique order
       const uint32_t out_idx = (c0 + c) * u.n + Q.qq;
@@ -275,7 +254,7 @@ __global__ __launch_bounds__(QB / QL * SPLIT) void screen_kernel(ScreenDev a) {
       n_skip += in && skip;
       n_shroud += in && shroud;
       const bool to_gl = in && route_gapless;
-      const uint32_t gpos = wave_append(&s_ngl, to_gl);
+      const uint32_t gpos = wave_append(s_ngl, to_gl);
       if (to_gl) {
         GaplessTask t;
         t.c_idx = C.ci;
@@ -285,7 +264,7 @@ __global__ __launch_bounds__(QB / QL * SPLIT) void screen_kernel(ScreenDev a) {
         gl_region[gpos] = t;
       }
       const bool to_dp = in && route_dp;
-      const uint32_t dpos = wave_append(&s_ndp, to_dp);
+      const uint32_t dpos = wave_append(s_ndp, to_dp);
       if (to_dp) {
         DpTask t;
         t.s1_off = u.off[C.ci];
@@ -299,9 +278,16 @@ __global__ __launch_bounds__(QB / QL * SPLIT) void screen_kernel(ScreenDev a) {
       }
     }
   }
-  // block totals of skipped / shrouded pairs, summed by the plan kernel
-  // (no device-scope atomics on shared counter words)
-  __shared__ uint32_t s_cnt[2][GT * SPLIT / 64];
+}
+
+// Block totals of skipped / shrouded pairs and the block's list lengths,
+// summed by the plan kernel (no device-scope atomics on shared counter
+// words).  All threads of the block call it.
+template <int NW>
+__device__ __forceinline__ void screen_block_totals(const ScreenDev& a, const uint32_t blk,
+                                                    const uint32_t n_skip, const uint32_t n_shroud,
+                                                    const uint32_t& s_ngl, const uint32_t& s_ndp) {
+  __shared__ uint32_t s_cnt[2][NW];
   uint32_t s1 = n_skip, s2 = n_shroud;
   for (int off = 32; off > 0; off >>= 1) {
     s1 += __shfl_xor(s1, off);
@@ -314,7 +300,7 @@ __global__ __launch_bounds__(QB / QL * SPLIT) void screen_kernel(ScreenDev a) {
   __syncthreads();
   if (threadIdx.x == 0) {
     uint32_t t1 = 0, t2 = 0;
-    for (int w = 0; w < GT * SPLIT / 64; ++w) {
+    for (int w = 0; w < NW; ++w) {
       t1 += s_cnt[0][w];
       t2 += s_cnt[1][w];
     }
@@ -323,6 +309,73 @@ __global__ __launch_bounds__(QB / QL * SPLIT) void screen_kernel(ScreenDev a) {
     a.partials[4 * blk + 2] = s_ngl;
     a.partials[4 * blk + 3] = s_ndp;
   }
+}
+
+template <int QB, int SPLIT, int QL>
+__global__ __launch_bounds__(QB / QL * SPLIT) void screen_kernel(ScreenDev a) {
+  constexpr int CW = (int)kScreenCenters / SPLIT;    // centers per wavefront
+  constexpr int GT = QB / QL;                          // threads per wavefront group
+  constexpr uint32_t kRegion = QB * kScreenCenters;  // one task slot per pair of the block
+  const UniqueDev& u = a.u;
+  // this block's task lists: kRegion slots each, counted in LDS
+  // Workgroup order: workgroup L runs on XCD L mod 8, and XCD x takes the
+  // logical blocks [x per, (x + 1) per) in unique-tile-major order, so the
+  // center rows of one tile of uniques run on one XCD close in time and
+  // re-read the tile's k-mer vectors from that XCD's L2.
+  const uint32_t total = a.nx * a.ny;
+  const uint32_t per = (total + 7u) / 8u;
+  const uint32_t blk = (blockIdx.x & 7u) * per + (blockIdx.x >> 3);
+  if (blk >= total) return;  // the whole workgroup: the grid is 8 per
+  const uint32_t tile = blk / a.ny, row = blk % a.ny;
+  __shared__ uint32_t s_ngl, s_ndp;
+  if (threadIdx.x == 0) {
+    s_ngl = 0;
+    s_ndp = 0;
+  }
+  __syncthreads();
+  GaplessTask* gl_region = a.gapless_tasks + (size_t)blk * kRegion;
+  DpTask* dp_region = a.dp_unsorted + (size_t)blk * kRegion;
+  const uint32_t part = uni(threadIdx.x / GT);
+  const uint32_t in_group = threadIdx.x % GT;
+  const uint32_t wave_base = tile * QB + (in_group / 64u) * 64u * QL;
+  const uint32_t lane = lane_id();
+  const uint32_t c0 = row * kScreenCenters + part * CW;
+  const uint32_t nc = c0 < a.n_centers ? min((uint32_t)CW, a.n_centers - c0) : 0u;
+
+  CenterFields cf[CW];
+  QueryFields qf[QL];
+  load_ids<QL, CW>(a, c0, wave_base, lane, cf, qf);
+  load_rest<QL, CW>(a, cf, qf);
+
+  // sum |a - b| over the dense vectors, four buckets per v_sad_u8
+  uint32_t sad[QL][CW];
+#pragma unroll
+  for (int j = 0; j < QL; ++j)
+#pragma unroll
+    for (int c = 0; c < CW; ++c) sad[j][c] = 0;
+  if (a.use_kmers && nc > 0) {
+    for (uint32_t w0 = 0; w0 < u.kwords; w0 += kScreenChunk) {
+      uint32_t qw[QL][kScreenChunk];
+#pragma unroll
+      for (int j = 0; j < QL; ++j)
+#pragma unroll
+        for (int t = 0; t < kScreenChunk; ++t)
+          qw[j][t] = u.kmer_il[(size_t)(w0 + t) * u.npad + qf[j].qq];
+#pragma unroll
+      for (int c = 0; c < CW; ++c) {
+        const uint32_t* cw = u.kmer_row + (size_t)cf[c].ci * u.kwords + w0;
+#pragma unroll
+        for (int t = 0; t < kScreenChunk; ++t)
+#pragma unroll
+          for (int j = 0; j < QL; ++j) sad[j][c] = __builtin_amdgcn_sad_u8(qw[j][t], cw[t], sad[j][c]);
+      }
+    }
+  }
+
+  uint32_t n_skip = 0, n_shroud = 0;
+  screen_pairs<QL, CW>(a, qf, cf, sad, nc, c0, &s_ngl, &s_ndp, gl_region, dp_region, n_skip,
+                       n_shroud);
+  screen_block_totals<GT * SPLIT / 64>(a, blk, n_skip, n_shroud, s_ngl, s_ndp);
 }
 
 void screen_thresholds(double cutoff, uint32_t nmax, uint32_t* out) {
@@ -482,8 +535,9 @@ hipError_t launch_screen(const ScreenArgs& s, hipStream_t st) {
   d.lr_hi = s.lr_hi;
   if (s.split != 1 && s.split != 2 && s.split != 4) return hipErrorInvalidValue;
   if (s.use_kmers && s.dstar == nullptr) return hipErrorInvalidValue;
-  const dim3 grid((d.u.n + s.qblock - 1) / s.qblock,
-                  (s.n_centers + kScreenCenters - 1) / kScreenCenters);
+  d.nx = (d.u.n + s.qblock - 1) / s.qblock;
+  d.ny = (s.n_centers + kScreenCenters - 1) / kScreenCenters;
+  const dim3 grid(8u * ((d.nx * d.ny + 7u) / 8u));
   const int key = (int)s.qblock * 4 + s.per_lane;
   switch (key) {
     case 64 * 4 + 1: launch_qb<64, 1>(d, grid, s.split, st); break;
