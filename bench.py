@@ -165,11 +165,11 @@ def copy_calibration(mib: int) -> None:
     hip.hipFree(b)
 
 
-def traffic_from_profile(kernel: str):
-    """HBM bytes per launch of `kernel` from the committed PMC summary
-    (profiles/traffic_batched.txt: 'kernel fetch_kib write_kib' lines already
+def traffic_from_profile(kernel: str, name: str = "traffic_batched.txt"):
+    """HBM bytes per launch of `kernel` from a committed PMC summary
+    (profiles/traffic_*.txt: 'kernel fetch_kib write_kib' lines already
     scaled by the copy-kernel factor), or None."""
-    path = ROOT / "profiles" / "traffic_batched.txt"
+    path = ROOT / "profiles" / name
     if not path.exists():
         return None
     for line in path.read_text().splitlines():
@@ -322,11 +322,15 @@ def bench_gradient(args, pkg):
         h = (float(lp[t, q]) - float(lm[t, q])) / 2
         fd = (ctx.loglik_forward(lp) - ctx.loglik_forward(lm)) / (2 * h)
         fd_err = max(fd_err, abs(fd - g[t, q]) / abs(g[t, q]))
+    cpu = None if args.no_cpu else gradient_cpu_baseline(ctx, w, centers, logE, args.cpu_seconds)
     ctx.close()
     npairs = cst["gapless"] + cst["aligned_dp"]
     ms = wall / args.steps * 1e3
     kms = float(np.mean(fwd)) + float(np.mean(bwd))
     hbm = npairs * 2 * 250.0 * 2  # seq + qual per pair, read by forward and backward
+    traffic = [traffic_from_profile(k, "traffic_gradient.txt")
+               for k in ("loglik_fwd_kernel", "loglik_bwd_kernel")]
+    traffic = sum(traffic) if all(t is not None for t in traffic) else None
     return _line("aligned pairs/sec, log-likelihood forward + backward", npairs / (ms * 1e-3),
                  "pairs/s", args, ms, "f32 parameter / f64 sums",
                  {"workload": "gradient: 500 centers x 50000 uniques, 250 nt, 16x41 log error "
@@ -336,10 +340,67 @@ def bench_gradient(args, pkg):
                  fd_rel_err_top4=fd_err,
                  roofline={"bound": "hbm", "achieved": hbm / (kms * 1e-3) / 1e9,
                            "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                           "frac": hbm / (kms * 1e-3) / 1e9 / HBM_PEAK_GBS, "traffic": None,
+                           "frac": hbm / (kms * 1e-3) / 1e9 / HBM_PEAK_GBS, "traffic": traffic,
                            "kernel": "loglik_fwd_kernel + loglik_bwd_kernel",
                            "algorithmic_bytes_per_launch": hbm},
-                 cpu_baseline=None)
+                 cpu_baseline=cpu)
+
+
+def gradient_cpu_baseline(ctx, w, centers, logE, min_seconds: float, max_pairs: int = 10000):
+    """The forward and backward on the host: numpy FP64 over the same aligned
+    pairs (up to max_pairs of them), each pair's (transition, quality) index
+    row built from the oracle's alignment (sub_new, nwalign.rs:1429-1455;
+    compute_lambda's transitions, pval.rs:294-309), which stays fixed as on
+    the device.  Whole passes for at least min_seconds; one thread."""
+    import math
+
+    import oracle as O
+
+    res = ctx.compare_batch(centers, w.err)
+    rs = O.RawSet(w.seqs, w.quals, w.reads, k=w.kmer_size)
+    params = O.make_params(band=w.band)
+    ncol = logE.shape[1]
+    rows, a, nc = [], [], []
+    for r, c in enumerate(int(x) for x in centers):
+        for u in np.nonzero(res.hamming[r] != 0xFFFFFFFF)[0]:
+            if len(rows) >= max_pairs:
+                break
+            sub, _ = O.sub_new(rs, c, int(u), params)
+            q = np.asarray(w.seqs[u], np.int64)
+            t = 5 * (q - 1)
+            for s in range(sub.nsubs):
+                t[sub.map[sub.pos[s]]] = 4 * (sub.nt0[s] - 1) + (sub.nt1[s] - 1)
+            rows.append(t * ncol + np.asarray(w.quals[u], np.int64))
+            a.append(float(w.reads[u]))
+            nc.append(float(w.reads[c]))
+    # rows padded to the longest query with a spare table slot (0.0 in the
+    # forward; its gradient count is dropped)
+    le = np.append(logE.astype(np.float64).ravel(), 0.0)
+    width = max(len(r) for r in rows)
+    idx = np.full((len(rows), width), le.size - 1, np.int64)
+    for i, r in enumerate(rows):
+        idx[i, :len(r)] = r
+    a, nc = np.asarray(a), np.asarray(nc)
+    lg = np.array([math.lgamma(x + 1.0) for x in a])
+
+    def one_pass():
+        E = nc * np.exp(le[idx].sum(axis=1))
+        loss = float(np.sum(a * np.log(E) - E - lg))
+        grad = np.bincount(idx.ravel(), weights=np.repeat(a - E, width), minlength=le.size)
+        return loss, grad[:-1]
+
+    one_pass()
+    passes, t0 = 0, time.perf_counter()
+    while True:
+        one_pass()
+        passes += 1
+        el = time.perf_counter() - t0
+        if el >= min_seconds:
+            break
+    return {"value": passes * len(idx) / el, "unit": "pairs/s", "cores": 1, "kind": "port",
+            "sample": f"{passes} passes over {len(idx)} of the aligned pairs, {el:.1f} s: numpy "
+                      "FP64 gather-sum forward and bincount backward over index rows built once "
+                      "from the oracle's alignments"}
 
 
 def bench_dada(args, pkg):
