@@ -79,18 +79,27 @@ struct RowRec {
   int32_t edge_t;   // top ends-free edge column, -1 when none (nwalign.rs:895-898)
   int32_t row_free; // accumulated ends-free edge score (nwalign.rs:884)
   uint32_t flags;   // bit0 left recalc, bit1 right recalc, bit2 band wedge,
+                    // bit3 kRowMasked (the DP may take the masked row update),
                     // bits 8-15 col_min after the wedge, bits 16-23 col_max after it
   uint32_t act_lo;  // lanes of the bin's wavefront whose column is active
   uint32_t act_hi;  // (columns col_min..col_max of each of its pairs), bits 0-31 / 32-63
 };
 
+// A row whose active columns are exactly its in-sequence columns, lie off
+// each pair's outer columns, and whose recalculated columns (if any) read
+// only lanes of their own pair: the DP updates it with lane masks from the
+// record instead of the per-lane window tests.
+constexpr uint32_t kRowMasked = 8u;
+
 // Per bin: the rows [steady_lo, steady_hi) (multiples of 16) where no
 // ends-free edge or recalculation applies, every active cell lies inside
 // both sequences, and the s1 / s2 offsets advance by one after each odd /
-// even row, so the DP runs its short row update there; and the DP
-// cells of one pair (sum of the active widths).
+// even row, so the DP runs its short row update there; the DP cells of
+// one pair (sum of the active widths); and masked = 1 when every other row
+// of the bin is kRowMasked, so the DP runs its masked row update on all of
+// them (vectorized bins; 0 elsewhere).
 struct BinInfo {
-  uint32_t steady_lo, steady_hi, cells, _pad;
+  uint32_t steady_lo, steady_hi, cells, masked;
 };
 
 // Records of the schedule table per bin of a BinSpace whose longest length

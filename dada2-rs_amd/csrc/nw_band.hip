@@ -76,7 +76,7 @@ __global__ void band_info_kernel(BinSpace bs, BinInfo* info) {
   bi.steady_lo = 0;
   bi.steady_hi = 0;
   bi.cells = band_cells(g, bs.band);
-  bi._pad = 0;
+  bi.masked = 0;
   info[b] = bi;
 }
 

@@ -168,7 +168,7 @@ __global__ void sched_kernel(BinSpace bs, const uint64_t* off, int end_gap, int 
     bi.steady_lo = 0;
     bi.steady_hi = 0;
     bi.cells = wide_fits(g) ? (uint32_t)vec_cells(l1, l2, bs.band) : 0u;
-    bi._pad = 0;
+    bi.masked = 0;
     info[b] = bi;
     return;
   }
@@ -187,6 +187,12 @@ __global__ void sched_kernel(BinSpace bs, const uint64_t* off, int end_gap, int 
     for (int gi = 0; gi < G; ++gi) m |= base << (gi * g.ncol);
     R.act_lo = (uint32_t)m;
     R.act_hi = (uint32_t)(m >> 32);
+    const bool window = R.inr_lo == R.cmin && R.inr_n == R.n;
+    const bool inside = R.n == 0 || (R.cmin >= 1 && R.cmin + (int)R.n <= g.ncol - 1);
+    const int cmin_new = (int)((R.flags >> 8) & 0xFFu), cmax_new = (int)((R.flags >> 16) & 0xFFu);
+    const bool rec_l = !(R.flags & 1u) || cmin_new >= 1;
+    const bool rec_r = !(R.flags & 2u) || cmax_new + 1 <= g.ncol - 1;
+    if (window && inside && rec_l && rec_r) R.flags |= kRowMasked;
   }
   // two lookahead records past the last row (rows last+1, last+2)
   t[last - 1] = t[last - 2];
@@ -226,11 +232,14 @@ __global__ void sched_kernel(BinSpace bs, const uint64_t* off, int end_gap, int 
   }
   int lo = (best_lo + 15) & ~15, hi = best_hi & ~15;
   if (hi <= lo) lo = hi = last + 1;
+  bool all_masked = true;
+  for (int row = 2; row <= last; ++row)
+    if ((row < lo || row >= hi) && !(t[row - 2].flags & kRowMasked)) all_masked = false;
   BinInfo bi;
   bi.steady_lo = (uint32_t)lo;
   bi.steady_hi = (uint32_t)hi;
   bi.cells = cells;
-  bi._pad = 0;
+  bi.masked = all_masked ? 1u : 0u;
   info[b] = bi;
 }
 
@@ -340,7 +349,95 @@ __device__ __forceinline__ RowRec load_rec(CWord* w) {
   R.edge_t = (int32_t)w[7];
   R.row_free = (int32_t)w[8];
   R.flags = w[9];
+  R.act_lo = w[10];
+  R.act_hi = w[11];
   return R;
+}
+
+// hp = 2 hp + (1 on the lanes of mask): one add with the mask as carry-in.
+__device__ __forceinline__ uint32_t shift_in(uint32_t hp, uint64_t mask) {
+  uint32_t r;
+  uint64_t carry_out;
+  asm("v_addc_co_u32_e64 %0, %1, %2, %2, %3" : "=v"(r), "=s"(carry_out) : "v"(hp), "s"(mask));
+  return r;
+}
+
+// A kRowMasked row (see d2g_internal.h): nw_row's update with the window
+// tests replaced by the record's lane masks.  Every active cell is inside
+// both sequences (score = match / mismatch) and reads lanes of its own pair,
+// so the values and pointers equal nw_row's: the same saturating i16
+// candidates and dploop precedence for active lanes, the ends-free edge
+// score (pointer 2 on the top edge, else 3 on the left) or fill elsewhere.
+// Each condition becomes a lane mask where it is formed, and rows without
+// recalculation shift the pointer in as two mask bits (a diagonal over a
+// mismatch is 0); recalculation rows take nw_row's per-lane pointer and its
+// tie rules on the recalculated columns.
+template <bool EVEN, bool SWAP>
+__device__ __forceinline__ void masked_row(const int row, const RowRec R, LaneState& ls,
+                                           const RowConst K, const int ca, const int cb,
+                                           const int c, const int lane, uint32_t* hist_p) {
+  const bool eq = ca == cb;
+  const uint64_t meq = __builtin_amdgcn_ballot_w64(eq);
+  const int dg = clamp16(ls.prev2 + (eq ? K.match : K.mismatch));
+  const int sh = EVEN ? shift_from_lower(ls.prev) : shift_from_upper(ls.prev);
+  const int lraw = EVEN ? sh : ls.prev;
+  const int uraw = EVEN ? ls.prev : sh;
+  const int l = clamp16(lraw + K.gap), u = clamp16(uraw + K.gap);
+  const bool upf = SWAP ? (l < u) : (u >= l);
+  const uint64_t mup = __builtin_amdgcn_ballot_w64(upf);
+  const int m = l > u ? l : u;
+  const bool dw = dg > m;
+  const uint64_t mdw = __builtin_amdgcn_ballot_w64(dw);
+  const int val = dw ? dg : m;
+  const uint64_t act = ((uint64_t)R.act_hi << 32) | R.act_lo;
+  // ends-free edge lanes (no lane matches a column of -1)
+  const uint64_t mt = __builtin_amdgcn_ballot_w64(c == R.edge_t);
+  const uint64_t ml = __builtin_amdgcn_ballot_w64(c == R.edge_l);
+  const int outside = __builtin_amdgcn_inverse_ballot_w64(mt | ml) ? R.row_free : K.fill;
+  int e = __builtin_amdgcn_inverse_ballot_w64(act) ? val : outside;
+  if (R.flags & 3u) {
+    // (from the masks only: the compare results stay scalar masks)
+    const bool a = __builtin_amdgcn_inverse_ballot_w64(act);
+    const bool isT = __builtin_amdgcn_inverse_ballot_w64(mt);
+    const bool isL = __builtin_amdgcn_inverse_ballot_w64(ml);
+    const bool bdw = __builtin_amdgcn_inverse_ballot_w64(mdw);
+    const bool bup = __builtin_amdgcn_inverse_ballot_w64(mup);
+    const bool beq = __builtin_amdgcn_inverse_ballot_w64(meq);
+    int p = a ? (bdw ? 1 : (bup ? 3 : 2)) : (isT ? 2 : (isL ? 3 : 0));
+    const bool trans_min = (R.flags & 4u) != 0;
+    const int cmin_new = (int)((R.flags >> 8) & 0xFFu);
+    const int cmax_new = (int)((R.flags >> 16) & 0xFFu);
+    if (R.flags & 1u) {
+      const int d_free = clamp16((trans_min ? uraw : lraw) + K.end_gap);
+      const bool here = c == cmin_new;
+      const bool take = d_free > e;
+      const bool tie = d_free == e && (SWAP ? p != 2 : p == 1);
+      e = (here && take) ? d_free : e;
+      p = (here && (take || tie)) ? 2 : p;
+    }
+    if (R.flags & 2u) {
+      const int src = trans_min ? shift_from_upper(uraw) : uraw;
+      const int d_free = clamp16(src + K.end_gap);
+      const bool here = c == cmax_new;
+      const bool take = d_free > e;
+      const bool tie = d_free == e && (SWAP ? p == 1 : p != 3);
+      e = (here && take) ? d_free : e;
+      p = (here && (take || tie)) ? 3 : p;
+    }
+    p = (p == 1 && !beq) ? 0 : p;  // p == 1 only on active (in-sequence) cells
+    ls.hp = ls.hp * 4u + (uint32_t)p;
+  } else {
+    const uint64_t hi = (act & ~mdw) | (~act & (mt | ml));
+    const uint64_t lo = (act & ((mdw & meq) | (~mdw & mup))) | (~act & ml & ~mt);
+    ls.hp = shift_in(ls.hp, hi);
+    ls.hp = shift_in(ls.hp, lo);
+  }
+  if ((row & 15) == 15 || row == K.last) {
+    hist_p[(row >> 4) * 64 + lane] = ls.hp << (2 * (15 - (row & 15)));
+    ls.hp = 0;
+  }
+  ls.prev2 = ls.prev;
+  ls.prev = e;
 }
 
 // Rows [r0, r1) through the full row update.  Software pipelined: the bases
@@ -371,13 +468,43 @@ __device__ __forceinline__ void generic_rows(const int r0, const int r1, CRowRec
   }
 }
 
-// hp = 2 hp + (1 on the lanes of mask): one add with the mask as carry-in.
-__device__ __forceinline__ uint32_t shift_in(uint32_t hp, uint64_t mask) {
-  uint32_t r;
-  uint64_t carry_out;
-  asm("v_addc_co_u32_e64 %0, %1, %2, %2, %3" : "=v"(r), "=s"(carry_out) : "v"(hp), "s"(mask));
-  return r;
+// The same rows of a bin whose rows are all kRowMasked, unrolled by row
+// pairs: r0 is even (row 2, or the end of a steady run, a multiple of 16),
+// so each pair is an even row and then an odd one, each with its own code
+// and no branch between them (merged, the two parities' compares would
+// leave their lane masks).  Loads are pipelined as in generic_rows.
+template <bool SWAP>
+__device__ __forceinline__ void masked_rows(const int r0, const int r1, CRowRec* rec,
+                                            LaneState& ls, const RowConst K, const uint8_t* s1l,
+                                            const uint8_t* s2l, const int c, const int lane,
+                                            uint32_t* hist_p) {
+  if (r0 >= r1) return;
+  RowRec cur = load_rec(rec + kRecWords * (r0 - 2));
+  RowRec nxt = load_rec(rec + kRecWords * (r0 - 1));
+  int ca = s1l[cur.a1], cb = s2l[cur.a2];
+  for (int row = r0; row < r1; row += 2) {
+    {
+      const int ca1 = s1l[nxt.a1], cb1 = s2l[nxt.a2];
+      const RowRec nx2 = load_rec(rec + kRecWords * row);  // row + 2
+      masked_row<true, SWAP>(row, cur, ls, K, ca, cb, c, lane, hist_p);
+      cur = nxt;
+      nxt = nx2;
+      ca = ca1;
+      cb = cb1;
+    }
+    if (row + 1 >= r1) break;
+    {
+      const int ca1 = s1l[nxt.a1], cb1 = s2l[nxt.a2];
+      const RowRec nx2 = load_rec(rec + kRecWords * (row + 1));  // row + 3
+      masked_row<false, SWAP>(row + 1, cur, ls, K, ca, cb, c, lane, hist_p);
+      cur = nxt;
+      nxt = nx2;
+      ca = ca1;
+      cb = cb1;
+    }
+  }
 }
+
 
 // A steady row: no ends-free edge or recalculation, every active cell
 // inside both sequences and off its pair's outer columns, with an active
@@ -453,12 +580,19 @@ __device__ __forceinline__ void steady_rows(const int b0, const int b1, CRowRec*
 }
 
 template <bool SWAP, bool CLAMP>
-__device__ __forceinline__ void forward(CRowRec* rec, const int b0, const int b1, LaneState& ls,
-                                        const RowConst K, const uint8_t* s1l, const uint8_t* s2l,
-                                        const int c, const int lane, uint32_t* hist_p) {
-  generic_rows<SWAP>(2, b0, rec, ls, K, s1l, s2l, c, lane, hist_p);
-  steady_rows<SWAP, CLAMP>(b0, b1, rec, ls, K, s1l, s2l, lane, hist_p);
-  generic_rows<SWAP>(b1, K.last + 1, rec, ls, K, s1l, s2l, c, lane, hist_p);
+__device__ __forceinline__ void forward(CRowRec* rec, const int b0, const int b1,
+                                        const bool masked, LaneState& ls, const RowConst K,
+                                        const uint8_t* s1l, const uint8_t* s2l, const int c,
+                                        const int lane, uint32_t* hist_p) {
+  if (masked) {
+    masked_rows<SWAP>(2, b0, rec, ls, K, s1l, s2l, c, lane, hist_p);
+    steady_rows<SWAP, CLAMP>(b0, b1, rec, ls, K, s1l, s2l, lane, hist_p);
+    masked_rows<SWAP>(b1, K.last + 1, rec, ls, K, s1l, s2l, c, lane, hist_p);
+  } else {
+    generic_rows<SWAP>(2, b0, rec, ls, K, s1l, s2l, c, lane, hist_p);
+    steady_rows<SWAP, CLAMP>(b0, b1, rec, ls, K, s1l, s2l, lane, hist_p);
+    generic_rows<SWAP>(b1, K.last + 1, rec, ls, K, s1l, s2l, c, lane, hist_p);
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -719,18 +853,19 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(8, 8))) void
     CRowRec* rec = (CRowRec*)(a.sched + a.sched_off[bin]);
     const BinInfo bi = a.info[bin];
     const int b0 = uni((int)bi.steady_lo), b1 = uni((int)bi.steady_hi);
+    const bool masked = uni((int)bi.masked) != 0;
     const uint8_t* s1l = s1b - c;
     const uint8_t* s2l = s2b + c;
     if (a.steady_clamp) {
       if (swap)
-        forward<true, true>(rec, b0, b1, ls, K, s1l, s2l, c, lane, hist_p);
+        forward<true, true>(rec, b0, b1, masked, ls, K, s1l, s2l, c, lane, hist_p);
       else
-        forward<false, true>(rec, b0, b1, ls, K, s1l, s2l, c, lane, hist_p);
+        forward<false, true>(rec, b0, b1, masked, ls, K, s1l, s2l, c, lane, hist_p);
     } else {
       if (swap)
-        forward<true, false>(rec, b0, b1, ls, K, s1l, s2l, c, lane, hist_p);
+        forward<true, false>(rec, b0, b1, masked, ls, K, s1l, s2l, c, lane, hist_p);
       else
-        forward<false, false>(rec, b0, b1, ls, K, s1l, s2l, c, lane, hist_p);
+        forward<false, false>(rec, b0, b1, masked, ls, K, s1l, s2l, c, lane, hist_p);
     }
     __syncthreads();
 
