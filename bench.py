@@ -403,6 +403,31 @@ def gradient_cpu_baseline(ctx, w, centers, logE, min_seconds: float, max_pairs: 
                       "from the oracle's alignments"}
 
 
+def p_margins(r, dp) -> dict:
+    """How far the run's p-values sit from the thresholds that decide its
+    calls, in decades (log10 ratios): each born cluster's birth p against
+    omega_a (dada.rs:966-1010, all below it), the smallest final p of a
+    non-center raw against omega_a (the bud test that ended the loop), and
+    every raw's final p against omega_c (the read map, dada.rs:502-533).  A
+    p-value routine with a relative error e can flip a call only within
+    about e / ln(10) decades of a threshold."""
+    centers = {c["center"] for c in r.clusters}
+    births = [c["birth_pval"] for k, c in enumerate(r.clusters) if k > 0]
+    p = np.asarray(r.p, np.float64)
+    raw = np.array([i not in centers for i in range(len(p))])
+    pos = p[raw & (p > 0)]
+    out = {"omega_a": dp.omega_a, "omega_c": dp.omega_c,
+           "births": len(births), "raws_p_zero": int(np.count_nonzero(raw & (p == 0)))}
+    bpos = [b for b in births if b > 0]
+    if bpos:
+        out["birth_closest_log10_p_over_omega_a"] = float(max(np.log10(np.array(bpos) / dp.omega_a)))
+    if len(pos):
+        out["raw_min_log10_p_over_omega_a"] = float(np.log10(pos.min() / dp.omega_a))
+        out["raw_closest_abs_log10_p_over_omega_c"] = float(np.abs(np.log10(pos / dp.omega_c)).min())
+    out["note"] = "omega_p applies to prior sequences; this workload has none"
+    return out
+
+
 def bench_dada(args, pkg):
     """The partition loop: run_dada over 50,000 uniques (250 nt; 300 planted
     variants, 60 near variants that bud, error copies), every cluster's
@@ -434,6 +459,7 @@ def bench_dada(args, pkg):
     rt = ctx.run_dada(w.err, ap, dp)
     ctx.close()
     st, stt = r.stats, rt.stats
+    margins = p_margins(r, dp)
     ms = wall / args.steps * 1e3
     # the dominant kernel by device time is the DP (VALU); the screen's HBM
     # rate is reported beside it
@@ -465,6 +491,7 @@ def bench_dada(args, pkg):
                               "variants, band 16, (5,-4,-8), k=5 cutoff 0.42, greedy, "
                               "omega_a 1e-40"},
                  clusters=len(r.clusters),
+                 p_margins=margins,
                  counts={"nalign": st["nalign"], "nshroud": st["nshroud"],
                          "compares": st["compares"], "shuffle_passes": st["shuffle_passes"],
                          "dp_cells": st["dp_cells"]},
