@@ -29,21 +29,34 @@ hipError_t launch_zero(uint32_t* p, size_t n_words, hipStream_t s) {
 constexpr int kPlanThreads = 1024;
 
 // Single-workgroup exclusive scan of v over the 1024 threads (returns the
-// exclusive prefix; *total gets the sum).  Every thread must call it.
+// exclusive prefix; *total gets the sum).  Each wavefront scans its lanes
+// with shuffles, wavefront 0 scans the 16 wavefront totals, and three
+// barriers order the LDS traffic.  Every thread must call it.
 __device__ uint32_t block_scan(uint32_t v, uint32_t* sh, uint32_t* total) {
-  const int tid = threadIdx.x;
-  sh[tid] = v;
-  __syncthreads();
-  for (int off = 1; off < kPlanThreads; off <<= 1) {
-    const uint32_t x = tid >= off ? sh[tid - off] : 0;
-    __syncthreads();
-    sh[tid] += x;
-    __syncthreads();
+  constexpr int kWaves = kPlanThreads / 64;
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  uint32_t x = v;
+#pragma unroll
+  for (int off = 1; off < 64; off <<= 1) {
+    const uint32_t y = (uint32_t)__shfl_up((int)x, off);
+    x += lane >= off ? y : 0u;
   }
-  const uint32_t incl = sh[tid];
-  *total = sh[kPlanThreads - 1];
+  if (lane == 63) sh[w] = x;
   __syncthreads();
-  return incl - v;
+  if (w == 0) {
+    uint32_t t = lane < kWaves ? sh[lane] : 0u;
+#pragma unroll
+    for (int off = 1; off < kWaves; off <<= 1) {
+      const uint32_t y = (uint32_t)__shfl_up((int)t, off);
+      t += lane >= off ? y : 0u;
+    }
+    if (lane < kWaves) sh[kWaves + lane] = t;  // inclusive wavefront prefixes
+  }
+  __syncthreads();
+  const uint32_t before = w ? sh[kWaves + w - 1] : 0u;
+  *total = sh[2 * kWaves - 1];
+  __syncthreads();  // sh is reused by the next scan
+  return before + x - v;
 }
 
 // Exclusive scans of the pair counts and of the wavefront counts over bins;
@@ -144,33 +157,39 @@ hipError_t launch_plan(const BinSpace& bs, const uint32_t* bin_count, uint32_t* 
   return hipGetLastError();
 }
 
-// One workgroup per task region: DP tasks go to their bin (bin_start plus a
+// One wavefront per task region (regions strided over a bounded grid; most
+// regions hold a few tasks): DP tasks go to their bin (bin_start plus a
 // per-bin fill counter); gapless tasks are compacted at gl_off[region].
 // Region r holds partials[4r+3] DP tasks and partials[4r+2] gapless tasks;
 // without partials, `count` DP tasks fill the regions in order.
-__global__ __launch_bounds__(256) void scatter_kernel(const DpTask* unsorted, DpTask* sorted,
-                                                      const uint32_t* bin_start, uint32_t* bin_fill,
-                                                      const uint32_t* partials, uint32_t count,
-                                                      uint32_t region,
-                                                      const GaplessTask* gl_unsorted,
-                                                      GaplessTask* gl, const uint32_t* gl_off) {
-  const uint32_t r = blockIdx.x;
-  const uint32_t ndp = partials ? partials[4 * r + 3] : min(region, count - r * region);
-  const DpTask* src = unsorted + (size_t)r * region;
-  // (a block-uniform trip count: every lane takes part in the keyed add)
-  for (uint32_t i0 = 0; i0 < ndp; i0 += blockDim.x) {
-    const uint32_t i = i0 + threadIdx.x;
-    const bool ok = i < ndp;
-    DpTask t;
-    if (ok) t = src[i];
-    const uint32_t rank = wave_add_by_key(bin_fill, ok ? t.bin : 0u, ok);
-    if (ok) sorted[bin_start[t.bin] + rank] = t;
-  }
-  if (partials) {
-    const uint32_t ngl = partials[4 * r + 2];
-    const GaplessTask* gsrc = gl_unsorted + (size_t)r * region;
-    GaplessTask* gdst = gl + gl_off[r];
-    for (uint32_t i = threadIdx.x; i < ngl; i += blockDim.x) gdst[i] = gsrc[i];
+constexpr uint32_t kScatterWaves = 4;
+constexpr uint32_t kScatterMaxBlocks = 1024;
+
+__global__ __launch_bounds__(64 * kScatterWaves) void scatter_kernel(
+    const DpTask* unsorted, DpTask* sorted, const uint32_t* bin_start, uint32_t* bin_fill,
+    const uint32_t* partials, uint32_t n_regions, uint32_t count, uint32_t region,
+    const GaplessTask* gl_unsorted, GaplessTask* gl, const uint32_t* gl_off) {
+  const uint32_t lane = __lane_id();
+  const uint32_t stride = gridDim.x * kScatterWaves;
+  for (uint32_t r = (uint32_t)uni((int)(blockIdx.x * kScatterWaves + threadIdx.x / 64u));
+       r < n_regions; r += stride) {
+    const uint32_t ndp = partials ? partials[4 * r + 3] : min(region, count - r * region);
+    const DpTask* src = unsorted + (size_t)r * region;
+    // (a wavefront-uniform trip count: every lane takes part in the keyed add)
+    for (uint32_t i0 = 0; i0 < ndp; i0 += 64u) {
+      const uint32_t i = i0 + lane;
+      const bool ok = i < ndp;
+      DpTask t;
+      if (ok) t = src[i];
+      const uint32_t rank = wave_add_by_key(bin_fill, ok ? t.bin : 0u, ok);
+      if (ok) sorted[bin_start[t.bin] + rank] = t;
+    }
+    if (partials) {
+      const uint32_t ngl = partials[4 * r + 2];
+      const GaplessTask* gsrc = gl_unsorted + (size_t)r * region;
+      GaplessTask* gdst = gl + gl_off[r];
+      for (uint32_t i = lane; i < ngl; i += 64u) gdst[i] = gsrc[i];
+    }
   }
 }
 
@@ -179,8 +198,11 @@ hipError_t launch_scatter(const DpTask* unsorted, DpTask* sorted, const uint32_t
                           uint32_t region, uint32_t count, const GaplessTask* gl_unsorted,
                           GaplessTask* gl, const uint32_t* gl_off, hipStream_t s) {
   if (n_regions == 0) return hipSuccess;
-  hipLaunchKernelGGL(scatter_kernel, dim3(n_regions), dim3(256), 0, s, unsorted, sorted,
-                     bin_start, bin_fill, partials, count, region, gl_unsorted, gl, gl_off);
+  uint32_t blocks = (n_regions + kScatterWaves - 1) / kScatterWaves;
+  if (blocks > kScatterMaxBlocks) blocks = kScatterMaxBlocks;
+  hipLaunchKernelGGL(scatter_kernel, dim3(blocks), dim3(64 * kScatterWaves), 0, s, unsorted,
+                     sorted, bin_start, bin_fill, partials, n_regions, count, region,
+                     gl_unsorted, gl, gl_off);
   return hipGetLastError();
 }
 
