@@ -151,6 +151,13 @@ struct d2g_ctx {
   uint8_t* h_pin = nullptr;
   size_t h_pin_cap = 0;
   std::vector<double> h_err;
+  // the center list (sorted indices) last uploaded into `centers`, and the
+  // buffer it went to (skipped when both are unchanged)
+  std::vector<uint32_t> h_centers;
+  const uint32_t* h_centers_buf = nullptr;
+  // pinned landing place of the compare's counters (a pageable target costs
+  // a staged copy per call)
+  Counters* h_ctr = nullptr;
   // the last compare's aligned pairs (for the log-likelihood and gradient)
   bool cmp_valid = false;
   uint32_t cmp_sub_stride = 0;
@@ -291,6 +298,7 @@ int d2g_ctx_destroy(d2g_ctx* c) {
   c->b_out.release();
   c->last_ctr.release();
   if (c->h_pin) (void)hipHostFree(c->h_pin);
+  if (c->h_ctr) (void)hipHostFree(c->h_ctr);
   if (c->graph_exec) (void)hipGraphExecDestroy(c->graph_exec);
   for (auto* b : {&c->ll_w, &c->ll_part, &c->ll_out}) b->release();
   c->ll_logE.release();
@@ -743,7 +751,12 @@ static int compare_pass(d2g_ctx* c, const uint32_t* centers, uint32_t nc, const 
   }
   uint32_t* hc = reinterpret_cast<uint32_t*>(c->h_pin);
   for (uint32_t i = 0; i < nc; ++i) hc[i] = c->h_inv[centers[i]];
-  HIP_TRY(hipMemcpyAsync(c->centers.p, hc, nc * sizeof(uint32_t), hipMemcpyHostToDevice, s));
+  if (c->h_centers_buf != c->centers.p || c->h_centers.size() != nc ||
+      std::memcmp(c->h_centers.data(), hc, (size_t)nc * sizeof(uint32_t)) != 0) {
+    HIP_TRY(hipMemcpyAsync(c->centers.p, hc, nc * sizeof(uint32_t), hipMemcpyHostToDevice, s));
+    c->h_centers.assign(hc, hc + nc);
+    c->h_centers_buf = c->centers.p;
+  }
   const uint8_t* dlock = nullptr;
   if (lock) {
     uint8_t* sl = c->h_pin + (size_t)nc * 4;
@@ -1034,9 +1047,10 @@ static int compare_pass(d2g_ctx* c, const uint32_t* centers, uint32_t nc, const 
       HIP_TRY_INT(enqueue(c->timing));
     }
     if (fetch) HIP_TRY(enqueue_fetch(c, nc, out_lambda, out_hamming, out_skipped));
-    Counters h;
-    HIP_TRY(hipMemcpyAsync(&h, c->ctr.p, sizeof(Counters), hipMemcpyDeviceToHost, s));
+    if (!c->h_ctr) HIP_TRY(hipHostMalloc((void**)&c->h_ctr, sizeof(Counters), hipHostMallocDefault));
+    HIP_TRY(hipMemcpyAsync(c->h_ctr, c->ctr.p, sizeof(Counters), hipMemcpyDeviceToHost, s));
     HIP_TRY(hipStreamSynchronize(s));
+    const Counters h = *c->h_ctr;
     if (h.err_flags & kErrSubOverflow) {
       // every DP pair owns sub_stride records
       c->sub_capacity = std::max(c->sub_capacity * 2, (size_t)h.n_dp * sub_stride);
