@@ -796,8 +796,8 @@ static int compare_pass(d2g_ctx* c, const uint32_t* centers, uint32_t nc, const 
     const uint32_t n_part = tm ? pair_blocks(n) : screen_blocks(n, nc, c->screen_qblock);
     const uint32_t region = tm ? kPairBlock : screen_region(c->screen_qblock);
     if (timed) HIP_TRY(hipEventRecord(c->ev[0], s));
-    HIP_TRY(launch_zero(reinterpret_cast<uint32_t*>(c->ctr.p), sizeof(Counters) / 4, s));
-    HIP_TRY(launch_zero(c->bin_count.p, bs.nbins + 1, s));
+    HIP_TRY(launch_zero2(reinterpret_cast<uint32_t*>(c->ctr.p), sizeof(Counters) / 4,
+                         c->bin_count.p, bs.nbins + 1, s));
     ScreenArgs sa;
     sa.u = &u;
     sa.centers = c->centers.p;

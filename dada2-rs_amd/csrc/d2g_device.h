@@ -29,23 +29,34 @@ __device__ __forceinline__ int uni(int v) { return __builtin_amdgcn_readfirstlan
 // with equal keys share one atomic (device-scope atomics on one word
 // serialise across the chip, and equal-length pairs all share one bin).
 // Returns each such lane's slot: the counter's old value plus the lane's
-// rank among its key's lanes.  Every lane of the wavefront must call this.
+// rank among its key's lanes.  The key groups are found with ballots first,
+// then every group's leader issues its atomic in one instruction, so a call
+// costs one memory round trip however many keys the wavefront holds.  Every
+// lane of the wavefront must call this.
 __device__ __forceinline__ uint32_t wave_add_by_key(uint32_t* counter, uint32_t key, bool pred) {
   const uint32_t lane = __lane_id();
   uint64_t pending = __builtin_amdgcn_ballot_w64(pred);
-  uint32_t slot = 0;
+  bool leads = false;
+  uint32_t count = 0, rank = 0;
+  int leader_of_mine = (int)lane;
   while (pending) {
     const int leader = __builtin_ffsll((long long)pending) - 1;
     const uint32_t k = (uint32_t)__shfl((int)key, leader);
     const uint64_t grp = __builtin_amdgcn_ballot_w64(pred && key == k) & pending;
-    uint32_t base = 0;
-    if ((int)lane == leader) base = atomicAdd(&counter[k], (uint32_t)__builtin_popcountll(grp));
-    base = (uint32_t)__shfl((int)base, leader);
-    if ((grp >> lane) & 1ull)
-      slot = base + (uint32_t)__builtin_popcountll(grp & ((1ull << lane) - 1ull));
+    if ((int)lane == leader) {
+      leads = true;
+      count = (uint32_t)__builtin_popcountll(grp);
+    }
+    if ((grp >> lane) & 1ull) {
+      leader_of_mine = leader;
+      rank = (uint32_t)__builtin_popcountll(grp & ((1ull << lane) - 1ull));
+    }
     pending &= ~grp;
   }
-  return slot;
+  uint32_t base = 0;
+  if (leads) base = atomicAdd(&counter[key], count);
+  base = (uint32_t)__shfl((int)base, leader_of_mine);
+  return pred ? base + rank : 0u;
 }
 
 }  // namespace d2g

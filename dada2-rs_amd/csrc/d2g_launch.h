@@ -89,6 +89,7 @@ hipError_t launch_pair_screen(const ScreenArgs& a, const uint32_t* want, hipStre
 
 // ---- plan.hip ----------------------------------------------------------
 hipError_t launch_zero(uint32_t* p, size_t n_words, hipStream_t s);
+hipError_t launch_zero2(uint32_t* a, size_t na, uint32_t* b, size_t nb, hipStream_t s);
 // Also totals the DP cells (info: per-bin cells, nullable) and, for a
 // compare, the screen's per-block counts (partials: 4 words per block,
 // nullable), writing each block's gapless offset to gl_off.

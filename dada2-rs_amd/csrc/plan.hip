@@ -18,6 +18,26 @@ __global__ void zero_kernel(uint32_t* p, size_t n) {
   for (; i < n; i += stride) p[i] = 0;
 }
 
+// Two buffers in one launch: words [0, na) of a, then [0, nb) of b.
+__global__ void zero2_kernel(uint32_t* a, size_t na, uint32_t* b, size_t nb) {
+  size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const size_t stride = (size_t)gridDim.x * blockDim.x;
+  for (; i < na + nb; i += stride) {
+    if (i < na)
+      a[i] = 0;
+    else
+      b[i - na] = 0;
+  }
+}
+
+hipError_t launch_zero2(uint32_t* a, size_t na, uint32_t* b, size_t nb, hipStream_t s) {
+  if (na + nb == 0) return hipSuccess;
+  size_t blocks = (na + nb + 255) / 256;
+  if (blocks > 4096) blocks = 4096;
+  hipLaunchKernelGGL(zero2_kernel, dim3((unsigned)blocks), dim3(256), 0, s, a, na, b, nb);
+  return hipGetLastError();
+}
+
 hipError_t launch_zero(uint32_t* p, size_t n_words, hipStream_t s) {
   if (n_words == 0) return hipSuccess;
   size_t blocks = (n_words + 255) / 256;
