@@ -726,26 +726,34 @@ __device__ __forceinline__ void backward_runs(const int last, const uint32_t* hi
       const uint32_t w = (uint32_t)__builtin_amdgcn_ds_bpermute(ps.addr, (int)hp);
       const int k = ps.rc - base;
       // field j of t: the pointer of row rc - 2j; fields past row 0 / 1 of
-      // the block are zero, which ends a run (k = 15 alone can fill all 8)
+      // the block are zero, which ends a run
       const uint32_t t = w >> ((30 - 2 * k) & 31);
       const uint32_t m = (t ^ 0x11111111u) & 0x33333333u;
-      const int n = m ? (__builtin_ctz(m) >> 2) : 8;
+      // the lowest set bit sits at 4j or 4j + 1 of the first non-1 field j;
+      // bit 31 stands for field 8 (all 8 fields 1, k = 15)
+      const int n = (int)((__builtin_ctz(m | 0x80000000u) + 1u) >> 2);
       const int r = ps.rc - 2 * n;  // first cell after the run
-      const int pp = (int)((t >> ((4 * n) & 31)) & 3u);
+      const int pp = (int)((t >> ((4 * n) & 31)) & 3u);  // 0, 2 or 3 there (a run stops at non-1)
       const bool step = on && r >= lo;  // a non-run pointer inside the block
       const bool odd = (r & 1) != 0;
-      bool sub = false, invalid = false;
+      // gaps: up (3) one column up on odd rows, left (2) one column down on
+      // even rows; both one row up.  Lanes that only ran move to r.
+      const bool gap = step && pp != 0;
+      const int delta = odd ? (pp == 3 ? 4 : 0) : (pp == 2 ? -4 : 0);
+      ps.addr += gap ? delta : 0;
+      ps.rc = on ? (gap ? r - 1 : r) : ps.rc;
       if (__builtin_amdgcn_ballot_w64(step && pp == 0)) {
         // pointer 0: a mismatching diagonal on an active cell, else invalid
+        const bool zero = step && pp == 0;
         const int cc = (ps.addr >> 2) - B.gpos;
         bool act = false;
-        if (step && pp == 0 && r >= 2) {
+        if (zero && r >= 2) {
           const int rcmin = (int)B.rec_v[kRecWords * (r - 2) + 2];
           const uint32_t rn = B.rec_v[kRecWords * (r - 2) + 3];
           act = (unsigned)(cc - rcmin) < rn;
         }
-        sub = step && pp == 0 && act;
-        invalid = step && pp == 0 && !act;
+        const bool sub = zero && act;
+        const bool invalid = zero && !act;
         if (sub && B.c == cc && B.real && B.sub_ok) {
           const int dd = 2 * (cc - B.start_col) + (odd ? 1 : 0);  // j - i on row r
           const int i = (r - dd) >> 1, j = (r + dd) >> 1;
@@ -754,18 +762,9 @@ __device__ __forceinline__ void backward_runs(const int last, const uint32_t* hi
           a.sub_pool[B.my_sub + ps.ns] = ((uint32_t)qp << 16) | (uint32_t)cp;
         }
         ps.ns += sub ? 1 : 0;
+        ps.rc = sub ? r - 2 : (invalid ? -1 : ps.rc);
+        ps.bad = ps.bad || invalid;
       }
-      // gaps: left (2) goes one column down on even rows, up (3) one column
-      // up on odd rows; both go one row up
-      const bool gap = step && pp >= 2;
-      ps.addr += (gap && odd && pp == 3) ? 4 : 0;
-      ps.addr -= (gap && !odd && pp == 2) ? 4 : 0;
-      int rc = on ? r : ps.rc;
-      rc = sub ? r - 2 : rc;
-      rc = gap ? r - 1 : rc;
-      rc = invalid ? -1 : rc;
-      ps.bad = ps.bad || invalid;
-      ps.rc = rc;
     }
     if (b >= 1) {
       hp = hp_next;

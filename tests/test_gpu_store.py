@@ -108,3 +108,22 @@ def test_store_count_only_and_state(ctx, pkg, work):
     ctx.store_reset()
     k, _, _ = ctx.compare_store(w.centers[:4], w.err, fetch=False)
     assert k == len(ctx.store_fetch())
+
+
+def test_store_rejects_calls_past_its_index_range(ctx, pkg, work):
+    """The store's index-range guard (d2g_compare_store checks, in 64-bit
+    arithmetic, that a call has at most 65,535 centers and that the store
+    stays under 2^32 comparisons): a call with 65,536 centers is refused
+    with D2G_ERR_UNSUPPORTED before any launch, and the store is left as it
+    was."""
+    w = work
+    ctx.load_uniques(w.seqs, w.quals, w.reads, k=5)
+    ctx.store_reset()
+    kept, _, _ = ctx.compare_store(w.centers[:2], w.err)
+    before = ctx.store_fetch()
+    too_many = np.zeros(65536, np.uint32)
+    with pytest.raises(pkg.D2GError) as exc:
+        ctx.compare_store(too_many, w.err, first_cluster=2, fetch=False)
+    assert exc.value.status == -5  # D2G_ERR_UNSUPPORTED
+    after = ctx.store_fetch()
+    assert np.array_equal(before, after) and len(after) == len(kept)
