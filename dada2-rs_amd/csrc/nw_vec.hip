@@ -604,6 +604,8 @@ struct BwConst {
   const uint32_t* rec_v;  // the same records, for per-lane (vector) loads
   int gpos;  // lane of column 0 of this lane's pair
   int c, start_col;
+  int st_lo;     // the bin's steady rows: [st_lo, st_lo + st_n)
+  uint32_t st_n;
   bool swap, real, sub_ok;
   size_t my_sub;  // first substitution slot of this lane's pair
   const uint8_t* s1b;
@@ -709,67 +711,87 @@ __device__ __forceinline__ void backward(const int last, const uint32_t* hist_p,
 // step of the wavefront's busiest pair instead of one per row.  The path
 // visits the cells of the row sweep above in the same order, so the
 // substitution records and the error flag are the same.
+//
+// Block b of the history (rows 16b .. 16b+15), word hp of this lane.
+__device__ __forceinline__ void runs_block(const int b, const uint32_t hp, PathState& ps,
+                                           const BwConst& B, const NwDev& a) {
+  const int base = b << 4;
+  const int lo = b == 0 ? 1 : base;  // row 0 is the path's end
+  for (;;) {
+    // the path cell is in this block (rc <= base + 15 holds: rc starts at
+    // last and leaves a block only downwards)
+    const bool on = ps.rc >= lo;
+    if (!__builtin_amdgcn_ballot_w64(on)) break;
+    const uint32_t w = (uint32_t)__builtin_amdgcn_ds_bpermute(ps.addr, (int)hp);
+    const int k = ps.rc - base;
+    // field j of t: the pointer of row rc - 2j; fields past row 0 / 1 of
+    // the block are zero, which ends a run
+    const uint32_t t = w >> ((30 - 2 * k) & 31);
+    const uint32_t m = (t ^ 0x11111111u) & 0x33333333u;
+    // the lowest set bit sits at 4j or 4j + 1 of the first non-1 field j;
+    // bit 31 stands for field 8 (all 8 fields 1, k = 15)
+    const int n = (int)((__builtin_ctz(m | 0x80000000u) + 1u) >> 2);
+    const int r = ps.rc - 2 * n;  // first cell after the run
+    const bool step = on && r >= lo;  // a non-run pointer inside the block
+    if (!__builtin_amdgcn_ballot_w64(step)) {
+      // the common case: every path runs out of the block
+      ps.rc = on ? r : ps.rc;
+      break;
+    }
+    const int pp = (int)((t >> ((4 * n) & 31)) & 3u);  // 0, 2 or 3 there (a run stops at non-1)
+    const bool odd = (r & 1) != 0;
+    // gaps: up (3) one column up on odd rows, left (2) one column down on
+    // even rows; both one row up.  Lanes that only ran move to r.
+    const bool gap = step && pp != 0;
+    const int delta = odd ? (pp == 3 ? 4 : 0) : (pp == 2 ? -4 : 0);
+    ps.addr += gap ? delta : 0;
+    ps.rc = on ? (gap ? r - 1 : r) : ps.rc;
+    if (__builtin_amdgcn_ballot_w64(step && pp == 0)) {
+      // pointer 0: a mismatching diagonal on an active cell, else invalid.
+      // A path cell of a steady row is active (no path enters an inactive
+      // lane there), so only rows outside the steady run read the record.
+      const bool zero = step && pp == 0;
+      const int cc = (ps.addr >> 2) - B.gpos;
+      bool act = (unsigned)(r - B.st_lo) < B.st_n;
+      if (zero && !act && r >= 2) {
+        const int rcmin = (int)B.rec_v[kRecWords * (r - 2) + 2];
+        const uint32_t rn = B.rec_v[kRecWords * (r - 2) + 3];
+        act = (unsigned)(cc - rcmin) < rn;
+      }
+      const bool sub = zero && act;
+      const bool invalid = zero && !act;
+      if (sub && B.c == cc && B.real && B.sub_ok) {
+        const int dd = 2 * (cc - B.start_col) + (odd ? 1 : 0);  // j - i on row r
+        const int i = (r - dd) >> 1, j = (r + dd) >> 1;
+        const int qp = B.swap ? i - 1 : j - 1;
+        const int cp = B.swap ? j - 1 : i - 1;
+        a.sub_pool[B.my_sub + ps.ns] = ((uint32_t)qp << 16) | (uint32_t)cp;
+      }
+      ps.ns += sub ? 1 : 0;
+      ps.rc = sub ? r - 2 : (invalid ? -1 : ps.rc);
+      ps.bad = ps.bad || invalid;
+    }
+  }
+}
+
+// Blocks last/16 .. 0.  Two history words alternate, each reloaded (for the
+// block two below) as soon as its block is done, so every load has a whole
+// block of work to arrive in and no register copy waits for it.
 __device__ __forceinline__ void backward_runs(const int last, const uint32_t* hist_p,
                                               const int lane, PathState& ps, const BwConst& B,
                                               const NwDev& a) {
   int b = last >> 4;
-  uint32_t hp = hist_p[b * 64 + lane];
-  uint32_t hp_next = b >= 1 ? hist_p[(b - 1) * 64 + lane] : 0u;
-  for (; b >= 0; --b) {
-    const int base = b << 4;
-    const int lo = b == 0 ? 1 : base;  // row 0 is the path's end
-    for (;;) {
-      // the path cell is in this block (rc <= base + 15 holds: rc starts at
-      // last and leaves a block only downwards)
-      const bool on = ps.rc >= lo;
-      if (!__builtin_amdgcn_ballot_w64(on)) break;
-      const uint32_t w = (uint32_t)__builtin_amdgcn_ds_bpermute(ps.addr, (int)hp);
-      const int k = ps.rc - base;
-      // field j of t: the pointer of row rc - 2j; fields past row 0 / 1 of
-      // the block are zero, which ends a run
-      const uint32_t t = w >> ((30 - 2 * k) & 31);
-      const uint32_t m = (t ^ 0x11111111u) & 0x33333333u;
-      // the lowest set bit sits at 4j or 4j + 1 of the first non-1 field j;
-      // bit 31 stands for field 8 (all 8 fields 1, k = 15)
-      const int n = (int)((__builtin_ctz(m | 0x80000000u) + 1u) >> 2);
-      const int r = ps.rc - 2 * n;  // first cell after the run
-      const int pp = (int)((t >> ((4 * n) & 31)) & 3u);  // 0, 2 or 3 there (a run stops at non-1)
-      const bool step = on && r >= lo;  // a non-run pointer inside the block
-      const bool odd = (r & 1) != 0;
-      // gaps: up (3) one column up on odd rows, left (2) one column down on
-      // even rows; both one row up.  Lanes that only ran move to r.
-      const bool gap = step && pp != 0;
-      const int delta = odd ? (pp == 3 ? 4 : 0) : (pp == 2 ? -4 : 0);
-      ps.addr += gap ? delta : 0;
-      ps.rc = on ? (gap ? r - 1 : r) : ps.rc;
-      if (__builtin_amdgcn_ballot_w64(step && pp == 0)) {
-        // pointer 0: a mismatching diagonal on an active cell, else invalid
-        const bool zero = step && pp == 0;
-        const int cc = (ps.addr >> 2) - B.gpos;
-        bool act = false;
-        if (zero && r >= 2) {
-          const int rcmin = (int)B.rec_v[kRecWords * (r - 2) + 2];
-          const uint32_t rn = B.rec_v[kRecWords * (r - 2) + 3];
-          act = (unsigned)(cc - rcmin) < rn;
-        }
-        const bool sub = zero && act;
-        const bool invalid = zero && !act;
-        if (sub && B.c == cc && B.real && B.sub_ok) {
-          const int dd = 2 * (cc - B.start_col) + (odd ? 1 : 0);  // j - i on row r
-          const int i = (r - dd) >> 1, j = (r + dd) >> 1;
-          const int qp = B.swap ? i - 1 : j - 1;
-          const int cp = B.swap ? j - 1 : i - 1;
-          a.sub_pool[B.my_sub + ps.ns] = ((uint32_t)qp << 16) | (uint32_t)cp;
-        }
-        ps.ns += sub ? 1 : 0;
-        ps.rc = sub ? r - 2 : (invalid ? -1 : ps.rc);
-        ps.bad = ps.bad || invalid;
-      }
-    }
-    if (b >= 1) {
-      hp = hp_next;
-      hp_next = b >= 2 ? hist_p[(b - 2) * 64 + lane] : 0u;
-    }
+  uint32_t ha = hist_p[b * 64 + lane];
+  uint32_t hb = hist_p[imax(b - 1, 0) * 64 + lane];
+  for (;;) {
+    runs_block(b, ha, ps, B, a);
+    if (b == 0) break;
+    ha = hist_p[imax(b - 2, 0) * 64 + lane];
+    --b;
+    runs_block(b, hb, ps, B, a);
+    if (b == 0) break;
+    hb = hist_p[imax(b - 2, 0) * 64 + lane];
+    --b;
   }
 }
 
@@ -882,6 +904,8 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(8, 8))) void
     B.gpos = g_raw * ncol;
     B.c = c;
     B.start_col = start_col;
+    B.st_lo = b0;
+    B.st_n = (uint32_t)(b1 - b0);
     B.swap = swap;
     B.real = real;
     B.sub_ok = sub_ok;
