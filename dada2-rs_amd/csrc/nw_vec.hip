@@ -368,10 +368,10 @@ __device__ __forceinline__ uint32_t shift_in(uint32_t hp, uint64_t mask) {
 // so the values and pointers equal nw_row's: the same saturating i16
 // candidates and dploop precedence for active lanes, the ends-free edge
 // score (pointer 2 on the top edge, else 3 on the left) or fill elsewhere.
-// Each condition becomes a lane mask where it is formed, and rows without
-// recalculation shift the pointer in as two mask bits (a diagonal over a
-// mismatch is 0); recalculation rows take nw_row's per-lane pointer and its
-// tie rules on the recalculated columns.
+// Each condition becomes a lane mask where it is formed, and the pointer is
+// shifted in as two mask bits (a diagonal over a mismatch is 0); on
+// recalculation rows the pointer classes are lane masks too, updated by
+// nw_row's take and tie rules on the recalculated columns.
 template <bool EVEN, bool SWAP>
 __device__ __forceinline__ void masked_row(const int row, const RowRec R, LaneState& ls,
                                            const RowConst K, const int ca, const int cb,
@@ -395,43 +395,48 @@ __device__ __forceinline__ void masked_row(const int row, const RowRec R, LaneSt
   const uint64_t ml = __builtin_amdgcn_ballot_w64(c == R.edge_l);
   const int outside = __builtin_amdgcn_inverse_ballot_w64(mt | ml) ? R.row_free : K.fill;
   int e = __builtin_amdgcn_inverse_ballot_w64(act) ? val : outside;
+  uint64_t hi, lo;
   if (R.flags & 3u) {
-    // (from the masks only: the compare results stay scalar masks)
-    const bool a = __builtin_amdgcn_inverse_ballot_w64(act);
-    const bool isT = __builtin_amdgcn_inverse_ballot_w64(mt);
-    const bool isL = __builtin_amdgcn_inverse_ballot_w64(ml);
-    const bool bdw = __builtin_amdgcn_inverse_ballot_w64(mdw);
-    const bool bup = __builtin_amdgcn_inverse_ballot_w64(mup);
-    const bool beq = __builtin_amdgcn_inverse_ballot_w64(meq);
-    int p = a ? (bdw ? 1 : (bup ? 3 : 2)) : (isT ? 2 : (isL ? 3 : 0));
+    // pointer classes as lane masks (p1: diagonal, before a mismatch makes
+    // it 0; p2: left; p3: up; none: 0); the recalculated columns then follow
+    // nw_row's take and tie rules
+    uint64_t p1 = act & mdw;
+    uint64_t p2 = (act & ~mdw & ~mup) | (~act & mt);
+    uint64_t p3 = (act & ~mdw & mup) | (~act & ~mt & ml);
     const bool trans_min = (R.flags & 4u) != 0;
-    const int cmin_new = (int)((R.flags >> 8) & 0xFFu);
-    const int cmax_new = (int)((R.flags >> 16) & 0xFFu);
     if (R.flags & 1u) {
+      const int cmin_new = (int)((R.flags >> 8) & 0xFFu);
       const int d_free = clamp16((trans_min ? uraw : lraw) + K.end_gap);
-      const bool here = c == cmin_new;
-      const bool take = d_free > e;
-      const bool tie = d_free == e && (SWAP ? p != 2 : p == 1);
-      e = (here && take) ? d_free : e;
-      p = (here && (take || tie)) ? 2 : p;
+      const uint64_t here = __builtin_amdgcn_ballot_w64(c == cmin_new);
+      const uint64_t take = here & __builtin_amdgcn_ballot_w64(d_free > e);
+      const uint64_t tie = here & __builtin_amdgcn_ballot_w64(d_free == e) & (SWAP ? ~p2 : p1);
+      e = __builtin_amdgcn_inverse_ballot_w64(take) ? d_free : e;
+      const uint64_t upd = take | tie;
+      p1 &= ~upd;
+      p2 |= upd;
+      p3 &= ~upd;
     }
     if (R.flags & 2u) {
+      const int cmax_new = (int)((R.flags >> 16) & 0xFFu);
       const int src = trans_min ? shift_from_upper(uraw) : uraw;
       const int d_free = clamp16(src + K.end_gap);
-      const bool here = c == cmax_new;
-      const bool take = d_free > e;
-      const bool tie = d_free == e && (SWAP ? p == 1 : p != 3);
-      e = (here && take) ? d_free : e;
-      p = (here && (take || tie)) ? 3 : p;
+      const uint64_t here = __builtin_amdgcn_ballot_w64(c == cmax_new);
+      const uint64_t take = here & __builtin_amdgcn_ballot_w64(d_free > e);
+      const uint64_t tie = here & __builtin_amdgcn_ballot_w64(d_free == e) & (SWAP ? p1 : ~p3);
+      e = __builtin_amdgcn_inverse_ballot_w64(take) ? d_free : e;
+      const uint64_t upd = take | tie;
+      p1 &= ~upd;
+      p2 &= ~upd;
+      p3 |= upd;
     }
-    p = (p == 1 && !beq) ? 0 : p;  // p == 1 only on active (in-sequence) cells
-    ls.hp = ls.hp * 4u + (uint32_t)p;
+    hi = p2 | p3;
+    lo = (p1 & meq) | p3;
   } else {
-    const uint64_t hi = (act & ~mdw) | (~act & (mt | ml));
-    const uint64_t lo = (act & ((mdw & meq) | (~mdw & mup))) | (~act & ml & ~mt);
-    ls.hp = shift_in(ls.hp, hi);
-    ls.hp = shift_in(ls.hp, lo);
+    hi = (act & ~mdw) | (~act & (mt | ml));
+    lo = (act & ((mdw & meq) | (~mdw & mup))) | (~act & ml & ~mt);
   }
+  ls.hp = shift_in(ls.hp, hi);
+  ls.hp = shift_in(ls.hp, lo);
   if ((row & 15) == 15 || row == K.last) {
     hist_p[(row >> 4) * 64 + lane] = ls.hp << (2 * (15 - (row & 15)));
     ls.hp = 0;
