@@ -91,7 +91,7 @@ struct RowRec {
 // record instead of the per-lane window tests.
 constexpr uint32_t kRowMasked = 8u;
 
-// Per bin: the rows [steady_lo, steady_hi) (multiples of 16) where no
+// Per bin: the rows [steady_lo, steady_hi) (even bounds, before the last row) where no
 // ends-free edge or recalculation applies, every active cell lies inside
 // both sequences, and the s1 / s2 offsets advance by one after each odd /
 // even row, so the DP runs its short row update there; the DP cells of

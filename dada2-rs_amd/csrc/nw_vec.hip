@@ -198,7 +198,8 @@ __global__ void sched_kernel(BinSpace bs, const uint64_t* off, int end_gap, int 
   t[last - 1] = t[last - 2];
   t[last] = t[last - 2];
   // longest run of steady rows whose s1 / s2 offsets advance by one after
-  // each odd / even row, trimmed to whole 16-row history words
+  // each odd / even row, trimmed to even bounds and kept off the last row
+  // (whose history word the generic and masked rows flush)
   int best_lo = 0, best_hi = 0, run_lo = -1;
   uint32_t cells = 0;
   for (int row = 2; row <= last; ++row) {
@@ -230,7 +231,7 @@ __global__ void sched_kernel(BinSpace bs, const uint64_t* off, int end_gap, int 
       best_hi = row + 1;
     }
   }
-  int lo = (best_lo + 15) & ~15, hi = best_hi & ~15;
+  int lo = (best_lo + 1) & ~1, hi = imin(best_hi, last) & ~1;
   if (hi <= lo) lo = hi = last + 1;
   bool all_masked = true;
   for (int row = 2; row <= last; ++row)
@@ -474,7 +475,7 @@ __device__ __forceinline__ void generic_rows(const int r0, const int r1, CRowRec
 }
 
 // The same rows of a bin whose rows are all kRowMasked, unrolled by row
-// pairs: r0 is even (row 2, or the end of a steady run, a multiple of 16),
+// pairs: r0 is even (row 2, or the end of a steady run, which is even),
 // so each pair is an even row and then an odd one, each with its own code
 // and no branch between them (merged, the two parities' compares would
 // leave their lane masks).  Loads are pipelined as in generic_rows.
@@ -552,12 +553,42 @@ __device__ __forceinline__ void steady_cell(const int ca, const int cb, const ui
   hp = shift_in(hp, (mdw & meq) | (~mdw & mup));
 }
 
-// Steady rows [b0, b1), both multiples of 16: one history word per 16-row
-// block, fully unrolled.  Within a run the s1 offset advances after odd rows
-// and the s2 offset after even rows, so row b0 + k reads s1 at a1(b0) + k/2
-// and s2 at a2(b0) + (k+1)/2 (immediate LDS offsets from per-block bases),
-// and the active lanes of every even / odd row are those of row b0 / b0 + 1.
-// Nothing is read from the schedule inside the run.
+// Steady rows [r0, r1) inside one history block (r0 and r1 even), by row
+// pairs: the pointer word continues the one in ls.hp and is flushed when r1
+// ends the block.  p1 / p2 point at row r0's bases; the next pair's bases are
+// read one pair ahead (the reads past the run stay in the padded rows).
+template <bool SWAP, bool CLAMP>
+__device__ __forceinline__ void steady_part(const int r0, const int r1, const uint8_t* p1,
+                                            const uint8_t* p2, const uint64_t act_even,
+                                            const uint64_t act_odd, LaneState& ls,
+                                            const RowConst K, const int lane, uint32_t* hist_p) {
+  if (r0 >= r1) return;
+  uint32_t hp = ls.hp;
+  int ca = p1[0], cb = p2[0], cb1 = p2[1];
+  for (int row = r0; row < r1; row += 2) {
+    const int na = p1[1], nb1 = p2[2];
+    steady_cell<true, SWAP, CLAMP>(ca, cb, act_even, ls, K, hp);
+    steady_cell<false, SWAP, CLAMP>(ca, cb1, act_odd, ls, K, hp);
+    ca = na;
+    cb = cb1;
+    cb1 = nb1;
+    ++p1;
+    ++p2;
+  }
+  if ((r1 & 15) == 0) {
+    hist_p[((r1 - 1) >> 4) * 64 + lane] = hp;
+    hp = 0;
+  }
+  ls.hp = hp;
+}
+
+// Steady rows [b0, b1) (both even).  Within a run the s1 offset advances
+// after odd rows and the s2 offset after even rows, so row b0 + k reads s1
+// at a1(b0) + k/2 and s2 at a2(b0) + (k+1)/2, and the active lanes of every
+// even / odd row are those of row b0 / b0 + 1: nothing is read from the
+// schedule inside the run.  Whole 16-row history blocks run fully unrolled
+// (immediate LDS offsets from per-block bases, one history word each); the
+// rows before the first whole block and after the last go by row pairs.
 template <bool SWAP, bool CLAMP>
 __device__ __forceinline__ void steady_rows(const int b0, const int b1, CRowRec* rec,
                                             LaneState& ls, const RowConst K, const uint8_t* s1l,
@@ -568,7 +599,12 @@ __device__ __forceinline__ void steady_rows(const int b0, const int b1, CRowRec*
   const uint8_t* p2 = s2l + (int)r0[1];
   const uint64_t act_even = ((uint64_t)r0[11] << 32) | r0[10];
   const uint64_t act_odd = ((uint64_t)r0[kRecWords + 11] << 32) | r0[kRecWords + 10];
-  for (int row = b0; row < b1; row += 16) {
+  const int f0 = imin((b0 + 15) & ~15, b1);  // the first whole block
+  const int f1 = imax(b1 & ~15, f0);         // the end of the whole blocks
+  steady_part<SWAP, CLAMP>(b0, f0, p1, p2, act_even, act_odd, ls, K, lane, hist_p);
+  p1 += (f0 - b0) >> 1;
+  p2 += (f0 - b0) >> 1;
+  for (int row = f0; row < f1; row += 16) {
     uint32_t hp = 0;
 #pragma unroll
     for (int k = 0; k < 16; ++k) {
@@ -582,6 +618,7 @@ __device__ __forceinline__ void steady_rows(const int b0, const int b1, CRowRec*
     p1 += 8;
     p2 += 8;
   }
+  steady_part<SWAP, CLAMP>(f1, b1, p1, p2, act_even, act_odd, ls, K, lane, hist_p);
 }
 
 template <bool SWAP, bool CLAMP>
