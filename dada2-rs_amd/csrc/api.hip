@@ -158,11 +158,19 @@ struct d2g_ctx {
   // pinned landing place of the compare's counters (a pageable target costs
   // a staged copy per call)
   Counters* h_ctr = nullptr;
-  // the last compare's aligned pairs (for the log-likelihood and gradient)
+  // the last compare's aligned pairs (for the log-likelihood and gradient);
+  // its counters go up to last_ctr when a likelihood entry first needs them
   bool cmp_valid = false;
   uint32_t cmp_sub_stride = 0;
   uint64_t cmp_pairs = 0;
+  Counters cmp_ctr{};
+  bool cmp_ctr_uploaded = false;
   DevBuf<Counters> last_ctr;
+  // small result sets come back as one packed block (lambda, hamming,
+  // skipped, counters) through pinned memory
+  DevBuf<uint8_t> fetch_dev;
+  uint8_t* fetch_pin = nullptr;
+  size_t fetch_pin_cap = 0;
   DevBuf<double> ll_w, ll_part, ll_out;
   DevBuf<float> ll_logE;
   uint32_t ll_ncol = 0;
@@ -195,6 +203,14 @@ struct d2g_ctx {
   DevBuf<uint32_t> store_bcount;
   uint8_t* store_pin = nullptr;  // count, self lambdas, the first kept comparisons
   size_t store_pin_cap = 0;
+  // resident lock flags (the partition loop): compares with a null lock
+  // argument read `lock`, which ctx_lock_add updates from a short index list
+  bool lock_resident = false;
+  DevBuf<uint32_t> lock_idx;
+  uint32_t* lock_pin = nullptr;
+  size_t lock_pin_cap = 0;
+  hipEvent_t lock_ev = nullptr;  // the last index upload (staging reuse)
+  bool lock_ev_pending = false;
   // align mode buffers
   DevBuf<uint8_t> a_pool, a_al0, a_al1;
   DevBuf<uint64_t> a_aloff;
@@ -299,6 +315,8 @@ int d2g_ctx_destroy(d2g_ctx* c) {
   c->last_ctr.release();
   if (c->h_pin) (void)hipHostFree(c->h_pin);
   if (c->h_ctr) (void)hipHostFree(c->h_ctr);
+  c->fetch_dev.release();
+  if (c->fetch_pin) (void)hipHostFree(c->fetch_pin);
   if (c->graph_exec) (void)hipGraphExecDestroy(c->graph_exec);
   for (auto* b : {&c->ll_w, &c->ll_part, &c->ll_out}) b->release();
   c->ll_logE.release();
@@ -310,6 +328,9 @@ int d2g_ctx_destroy(d2g_ctx* c) {
   c->store_bcount.release();
   if (c->store) (void)hipFree(c->store);
   if (c->store_pin) (void)hipHostFree(c->store_pin);
+  c->lock_idx.release();
+  if (c->lock_pin) (void)hipHostFree(c->lock_pin);
+  if (c->lock_ev) (void)hipEventDestroy(c->lock_ev);
   for (int i = 0; i < kEvents; ++i)
     if (c->ev[i]) (void)hipEventDestroy(c->ev[i]);
   if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -636,6 +657,61 @@ static hipError_t enqueue_fetch(d2g_ctx* c, uint32_t nc, double* out_lambda,
   return hipSuccess;
 }
 
+// A result set of at most kPackedFetchMax bytes returns as one packed block:
+// the unpermute writes lambda, hamming, skipped and the counters into
+// fetch_dev, one copy lands it in pinned memory, and the host spreads it
+// into the caller's arrays after the synchronisation (one short copy instead
+// of three staged pageable ones and the counters' own).
+constexpr size_t kPackedFetchMax = (size_t)4 << 20;
+struct PackedLayout {
+  size_t ham, skp, ctr, total;
+};
+static PackedLayout packed_layout(size_t np) {
+  PackedLayout L;
+  L.ham = np * sizeof(double);
+  L.skp = L.ham + np * sizeof(uint32_t);
+  L.ctr = (L.skp + np + 15) & ~(size_t)15;
+  L.total = L.ctr + sizeof(Counters);
+  return L;
+}
+
+static hipError_t enqueue_fetch_packed(d2g_ctx* c, uint32_t nc) {
+  const size_t np = (size_t)nc * c->u.n;
+  const PackedLayout L = packed_layout(np);
+  hipError_t e;
+  if ((e = c->fetch_dev.ensure(L.total)) != hipSuccess) return e;
+  if (c->fetch_pin_cap < L.total) {
+    if (c->fetch_pin) (void)hipHostFree(c->fetch_pin);  // no copy into it is pending
+    c->fetch_pin = nullptr;
+    c->fetch_pin_cap = 0;
+    if ((e = hipHostMalloc((void**)&c->fetch_pin, L.total, hipHostMallocDefault)) != hipSuccess)
+      return e;
+    c->fetch_pin_cap = L.total;
+  }
+  uint8_t* d = c->fetch_dev.p;
+  if ((e = launch_unpermute(nc, c->u.n, c->u.orig, c->out_lambda.p, c->out_hamming.p,
+                            c->out_skipped.p, reinterpret_cast<double*>(d),
+                            reinterpret_cast<uint32_t*>(d + L.ham), d + L.skp, c->stream,
+                            reinterpret_cast<const uint32_t*>(c->ctr.p),
+                            reinterpret_cast<uint32_t*>(d + L.ctr),
+                            (uint32_t)(sizeof(Counters) / 4))) != hipSuccess)
+    return e;
+  return hipMemcpyAsync(c->fetch_pin, d, L.total, hipMemcpyDeviceToHost, c->stream);
+}
+
+// After the synchronisation: the packed block into the caller's arrays and
+// the counters into h_ctr.
+static void finish_fetch_packed(d2g_ctx* c, uint32_t nc, double* out_lambda,
+                                uint32_t* out_hamming, uint8_t* out_skipped) {
+  const size_t np = (size_t)nc * c->u.n;
+  const PackedLayout L = packed_layout(np);
+  const uint8_t* h = c->fetch_pin;
+  if (out_lambda) std::memcpy(out_lambda, h, np * sizeof(double));
+  if (out_hamming) std::memcpy(out_hamming, h + L.ham, np * sizeof(uint32_t));
+  if (out_skipped) std::memcpy(out_skipped, h + L.skp, np);
+  std::memcpy(c->h_ctr, h + L.ctr, sizeof(Counters));
+}
+
 // The transition pass's variant of the compare (transition_counts,
 // error.rs:194-236): the screen keeps each unique's pair with its own center
 // (want, sorted order), the DP writes alignment strings per unique, and the
@@ -764,6 +840,8 @@ static int compare_pass(d2g_ctx* c, const uint32_t* centers, uint32_t nc, const 
     HIP_TRY(c->lock.ensure(n));
     HIP_TRY(hipMemcpyAsync(c->lock.p, sl, n, hipMemcpyHostToDevice, s));
     dlock = c->lock.p;
+  } else if (c->lock_resident) {
+    dlock = c->lock.p;  // kept current by ctx_lock_add
   }
   const size_t ne = (size_t)16 * ncol;
   if (c->h_err.size() != ne || std::memcmp(c->h_err.data(), err, ne * sizeof(double)) != 0) {
@@ -1046,10 +1124,16 @@ static int compare_pass(d2g_ctx* c, const uint32_t* centers, uint32_t nc, const 
     } else {
       HIP_TRY_INT(enqueue(c->timing));
     }
-    if (fetch) HIP_TRY(enqueue_fetch(c, nc, out_lambda, out_hamming, out_skipped));
     if (!c->h_ctr) HIP_TRY(hipHostMalloc((void**)&c->h_ctr, sizeof(Counters), hipHostMallocDefault));
-    HIP_TRY(hipMemcpyAsync(c->h_ctr, c->ctr.p, sizeof(Counters), hipMemcpyDeviceToHost, s));
+    const bool packed = fetch && packed_layout(npairs).total <= kPackedFetchMax;
+    if (packed) {
+      HIP_TRY(enqueue_fetch_packed(c, nc));
+    } else {
+      if (fetch) HIP_TRY(enqueue_fetch(c, nc, out_lambda, out_hamming, out_skipped));
+      HIP_TRY(hipMemcpyAsync(c->h_ctr, c->ctr.p, sizeof(Counters), hipMemcpyDeviceToHost, s));
+    }
     HIP_TRY(hipStreamSynchronize(s));
+    if (packed) finish_fetch_packed(c, nc, out_lambda, out_hamming, out_skipped);
     const Counters h = *c->h_ctr;
     if (h.err_flags & kErrSubOverflow) {
       // every DP pair owns sub_stride records
@@ -1082,8 +1166,8 @@ static int compare_pass(d2g_ctx* c, const uint32_t* centers, uint32_t nc, const 
     }
     c->last_nc = nc;
     // the aligned pairs of this call stay resident for d2g_loglik_*
-    HIP_TRY(c->last_ctr.ensure(1));
-    HIP_TRY(hipMemcpyAsync(c->last_ctr.p, c->ctr.p, sizeof(Counters), hipMemcpyDeviceToDevice, s));
+    c->cmp_ctr = h;
+    c->cmp_ctr_uploaded = false;
     c->cmp_valid = true;
     c->cmp_sub_stride = sub_stride;
     c->cmp_pairs = (uint64_t)h.n_gapless + h.n_dp;
@@ -1909,6 +1993,12 @@ int d2g_band_supported(uint32_t len1, uint32_t len2, int32_t band, int homo) {
 // ---- log-likelihood and gradient ------------------------------------------
 static int loglik_args(d2g_ctx* c, uint32_t ncol, int use_quals, LoglikArgs* la) {
   if (!c->cmp_valid) return D2G_ERR_STATE;
+  if (!c->cmp_ctr_uploaded) {  // the compare's counters (n_gapless, n_dp)
+    HIP_TRY(c->last_ctr.ensure(1));
+    HIP_TRY(hipMemcpyAsync(c->last_ctr.p, &c->cmp_ctr, sizeof(Counters), hipMemcpyHostToDevice,
+                           c->stream));
+    c->cmp_ctr_uploaded = true;
+  }
   if (use_quals && c->has_qual && c->max_qual >= ncol) return D2G_ERR_ARG;
   // LDS of the backward workgroup: four wavefronts' gradient rows and counts
   // (12 bytes per table entry) and the 64 pairs' staged rows (33 KB)
@@ -2347,6 +2437,54 @@ bool ctx_store(const d2g_ctx* c, const ShuffleComp** comps, uint64_t* count) {
   *comps = c->store;
   *count = c->store_n;
   return true;
+}
+
+bool ctx_lock_begin(d2g_ctx* c) {
+  if (!c || !c->loaded) return false;
+  if (c->lock.ensure(c->u.n) != hipSuccess) return false;
+  if (hipMemsetAsync(c->lock.p, 0, c->u.n, c->stream) != hipSuccess) return false;
+  c->lock_resident = true;
+  return true;
+}
+
+bool ctx_lock_add(d2g_ctx* c, const uint32_t* raws, uint32_t k) {
+  if (!c || !c->lock_resident) return false;
+  if (k == 0) return true;
+  // the staging area is free once the previous upload has run
+  if (c->lock_ev_pending && hipEventSynchronize(c->lock_ev) != hipSuccess) return false;
+  c->lock_ev_pending = false;
+  if (!c->lock_ev && hipEventCreateWithFlags(&c->lock_ev, hipEventDisableTiming) != hipSuccess)
+    return false;
+  if (c->lock_pin_cap < k) {
+    if (c->lock_pin) (void)hipHostFree(c->lock_pin);
+    c->lock_pin = nullptr;
+    c->lock_pin_cap = 0;
+    const size_t cap = std::max<size_t>(k, 4096);
+    if (hipHostMalloc((void**)&c->lock_pin, cap * sizeof(uint32_t), hipHostMallocDefault) !=
+        hipSuccess)
+      return false;
+    c->lock_pin_cap = cap;
+  }
+  if (c->lock_idx.cap < k) {
+    // a kernel still reading the old index list must finish first
+    if (hipStreamSynchronize(c->stream) != hipSuccess) return false;
+    if (c->lock_idx.ensure(std::max<size_t>(k, 4096)) != hipSuccess) return false;
+  }
+  for (uint32_t j = 0; j < k; ++j) {
+    if (raws[j] >= c->u.n) return false;
+    c->lock_pin[j] = c->h_inv[raws[j]];
+  }
+  if (hipMemcpyAsync(c->lock_idx.p, c->lock_pin, (size_t)k * sizeof(uint32_t),
+                     hipMemcpyHostToDevice, c->stream) != hipSuccess ||
+      launch_lock_set(c->lock.p, c->lock_idx.p, k, c->stream) != hipSuccess ||
+      hipEventRecord(c->lock_ev, c->stream) != hipSuccess)
+    return false;
+  c->lock_ev_pending = true;
+  return true;
+}
+
+void ctx_lock_end(d2g_ctx* c) {
+  if (c) c->lock_resident = false;
 }
 
 bool ctx_exec(const d2g_ctx* c, CtxExec* out) {

@@ -106,7 +106,9 @@ hipError_t launch_scatter(const DpTask* unsorted, DpTask* sorted, const uint32_t
                           GaplessTask* gl, const uint32_t* gl_off, hipStream_t s);
 hipError_t launch_unpermute(uint32_t nc, uint32_t n, const uint32_t* orig, const double* src_lam,
                             const uint32_t* src_ham, const uint8_t* src_skp, double* dst_lam,
-                            uint32_t* dst_ham, uint8_t* dst_skp, hipStream_t s);
+                            uint32_t* dst_ham, uint8_t* dst_skp, hipStream_t s,
+                            const uint32_t* ctr_src = nullptr, uint32_t* ctr_dst = nullptr,
+                            uint32_t ctr_words = 0);
 
 // ---- nw_vec.hip --------------------------------------------------------
 struct NwArgs {
@@ -275,6 +277,9 @@ struct StoreArgs {
 uint32_t store_blocks(uint32_t n);
 hipError_t launch_store(const StoreArgs& a, hipStream_t s);
 hipError_t launch_store_reset(double* e_minmax, uint32_t n, hipStream_t s);
+// lock[idx[j]] = 1 for the k sorted indices idx (the partition loop's new
+// greedy locks).
+hipError_t launch_lock_set(uint8_t* lock, const uint32_t* idx, uint32_t k, hipStream_t s);
 
 hipError_t launch_shuffle_init(const ShuffleArgs& a, hipStream_t s);  // where = 0, scratch reset
 hipError_t launch_shuffle_pass(const ShuffleArgs& a, hipStream_t s);
@@ -347,6 +352,13 @@ struct CtxExec {
 bool ctx_exec(const d2g_ctx* c, CtxExec* out);
 // The context's resident store of kept comparisons (d2g_compare_store).
 bool ctx_store(const d2g_ctx* c, const ShuffleComp** comps, uint64_t* count);
+// Resident lock flags for the partition loop: begin clears the device lock
+// array and makes compares with a null lock argument read it; add sets the
+// flags of k raws (caller order; flags are only ever set); end returns the
+// context to per-call lock uploads.
+bool ctx_lock_begin(d2g_ctx* c);
+bool ctx_lock_add(d2g_ctx* c, const uint32_t* raws, uint32_t k);
+void ctx_lock_end(d2g_ctx* c);
 // Once per device, with that device current: the address of the
 // allocation-free entry point's device array and its constant table.
 int noalloc_setup(int device);

@@ -223,12 +223,40 @@ struct Part {
 
 // ---- p-values (pval.rs) ----------------------------------------------------
 
+// ln(k!) as poisson_upper_tail_direct sums it (ln 1 + ln 2 + ... + ln k,
+// left to right), tabulated once per run up to the run's largest read
+// count (at most kMax entries; larger k continue the sum from the table's
+// end in the same order), so every entry has the loop's bits.
+class LogFact {
+ public:
+  static constexpr uint32_t kMax = 1u << 22;
+  void build(uint32_t rmax) {
+    const uint32_t m = std::min(rmax, kMax - 1u);
+    t_.resize((size_t)m + 1);
+    double s = 0.0;
+    t_[0] = 0.0;
+    for (uint32_t i = 1; i <= m; ++i) {
+      s += std::log((double)i);
+      t_[i] = s;
+    }
+  }
+  double operator()(uint32_t k) const {
+    const uint32_t m = (uint32_t)t_.size() - 1u;
+    if (k <= m) return t_[k];
+    double s = t_[m];
+    for (uint32_t i = m + 1; i <= k; ++i) s += std::log((double)i);
+    return s;
+  }
+
+ private:
+  std::vector<double> t_{0.0};
+};
+
 // poisson_upper_tail_direct, pval.rs:329-353
-double poisson_tail_direct(uint32_t reads, double lambda) {
+double poisson_tail_direct(uint32_t reads, double lambda, const LogFact& lf) {
   const double k = (double)reads;
   const double log_lambda = std::log(lambda);
-  double log_kfact = 0.0;
-  for (uint32_t i = 1; i <= reads; ++i) log_kfact += std::log((double)i);
+  const double log_kfact = lf(reads);
   const double log_leading = -lambda + k * log_lambda - log_kfact;
   double series = 1.0, term = 1.0;
   for (uint32_t j = 1; j < 10000u; ++j) {
@@ -242,10 +270,10 @@ double poisson_tail_direct(uint32_t reads, double lambda) {
 }
 
 // calc_pA, pval.rs:208-241
-double calc_pA(uint32_t reads, double e_reads, bool prior) {
+double calc_pA(uint32_t reads, double e_reads, bool prior, const LogFact& lf) {
   if (!(e_reads > 0.0)) return 1.0;  // Poisson::new rejects e_reads <= 0 and NaN
   double pval = d2g::statrs_gamma_lr((double)reads, e_reads);  // sf(reads - 1)
-  if (pval == 0.0) pval = poisson_tail_direct(reads, e_reads);
+  if (pval == 0.0) pval = poisson_tail_direct(reads, e_reads, lf);
   if (prior) return pval;
   double norm = 1.0 - std::exp(-e_reads);
   if (norm < 1e-7) norm = e_reads - 0.5 * e_reads * e_reads;  // TAIL_APPROX_CUTOFF, pval.rs:18
@@ -254,11 +282,11 @@ double calc_pA(uint32_t reads, double e_reads, bool prior) {
 
 // get_pA, pval.rs:376-398
 double get_pA(uint32_t reads, bool prior, double lambda, uint32_t hamming, uint32_t bi_reads,
-              bool detect_singletons) {
+              bool detect_singletons, const LogFact& lf) {
   if (reads == 1 && !prior && !detect_singletons) return 1.0;
   if (hamming == 0) return 1.0;
   if (lambda == 0.0) return 0.0;
-  return calc_pA(reads, lambda * (double)bi_reads, prior || detect_singletons);
+  return calc_pA(reads, lambda * (double)bi_reads, prior || detect_singletons, lf);
 }
 
 // ---- the loop's steps ------------------------------------------------------
@@ -393,11 +421,29 @@ struct Run {
   d2g_dada_params dp;
   Part B;
   Pinned<d2g_comparison> stored;  // one compare's kept comparisons (at most one per raw)
-  std::vector<uint8_t> lock;
+  // raws locked since the last compare (the device's lock flags follow them)
+  std::vector<uint32_t> new_locks;
   std::vector<std::vector<Cand>> cand;  // per raw
   Pool* pool = nullptr;
-  // per-chunk scratch of the bud scan
-  std::vector<size_t> bud_off;
+  // A bud candidate; `better` is the scan's replacement test (smaller p,
+  // then more reads).
+  struct Best {
+    double p;
+    uint32_t reads;
+    size_t ci, pos;
+    bool have;
+  };
+  static bool better(double p, uint32_t reads, const Best& b) {
+    return p < b.p || (p == b.p && reads > b.reads);
+  }
+  // a p-update's work: members [b, e) of cluster ci
+  struct Piece {
+    uint32_t ci, b, e;
+  };
+  std::vector<Piece> pieces;
+  std::vector<Best> piece_best;  // per piece: abundance and prior candidates
+  std::vector<Best> cl_best;     // per cluster: the same, kept between p-updates
+  LogFact lf;                    // ln(k!) up to the largest read count
   d2g_dada_stats st{};
   DevShuffle* dev = nullptr;  // the device scan, when asked for
   bool dev_failed = false;
@@ -409,17 +455,19 @@ struct Run {
   // b_compare_parallel, cluster.rs:239-397: the map and the store on the
   // device (d2g_compare_store); the kept comparisons come back in raw order
   int compare(size_t i) {
-    const uint32_t n = (uint32_t)B.raws.size();
     const uint32_t center = (uint32_t)B.cl[i].center;
-    for (uint32_t r = 0; r < n; ++r) lock[r] = B.raws[r].lock ? 1 : 0;
     d2g_align_params p = align;
     if (i == 0) p.kdist_cutoff = 1.0;  // dada.rs:835-838
     auto t0 = std::chrono::steady_clock::now();
+    // the lock flags live on the device: only the raws locked since the
+    // last compare go up
+    if (!d2g::ctx_lock_add(ctx, new_locks.data(), (uint32_t)new_locks.size())) return D2G_ERR_HIP;
+    new_locks.clear();
     d2g_stats cs{};
     uint64_t nst = 0;
     double self = 0.0;
     const int rc = d2g_compare_store(ctx, &center, 1, (uint32_t)i, (double)B.reads, err, ncol, &p,
-                                     dp.use_quals, dp.greedy, lock.data(), stored.n, stored.data(),
+                                     dp.use_quals, dp.greedy, nullptr, stored.n, stored.data(),
                                      &nst, &self, &cs);
     st.ms_compare += ms_since(t0);
     ++st.compares;
@@ -449,30 +497,79 @@ struct Run {
     return D2G_OK;
   }
 
-  // b_p_update, pval.rs:45-130
+  // b_p_update, pval.rs:45-130, fused with the candidate scan of b_bud.
+  //   * Each member's p depends only on its own comparison and its
+  //     cluster's reads, so the members of every cluster flagged update_e
+  //     are repriced in one pass over the pool, in pieces of at most kPiece
+  //     members of one cluster (most clusters are far smaller).
+  //   * The same pass keeps each repriced cluster's bud candidates (see
+  //     bud()): a cluster's members, their order, p and comparisons and its
+  //     reads change only through add_raw / pop_raw, which flag update_e,
+  //     so a cluster that is not repriced keeps its candidates.
+  //   * Locks are only ever set; the newly set ones are queued for the
+  //     device.
   void p_update() {
+    constexpr uint32_t kPiece = 2048;
     const auto t0 = std::chrono::steady_clock::now();
-    for (Cluster& C : B.cl) {
+    pieces.clear();
+    cl_best.resize(2 * B.cl.size(), Best{0.0, 0u, 0, 0, false});
+    for (size_t ci = 0; ci < B.cl.size(); ++ci) {
+      Cluster& C = B.cl[ci];
       if (C.update_e) {
-        const uint32_t bi_reads = C.reads;
-        const bool ds = dp.detect_singletons != 0;
-        pool->run(C.raws.size(), 1024, [&](size_t b, size_t e) {
-          for (size_t k = b; k < e; ++k) {
-            Raw& R = B.raws[C.raws[k]];
-            R.p = get_pA(R.reads, R.prior, R.comp.lambda, R.comp.hamming, bi_reads, ds);
-          }
-        });
+        const uint32_t m = (uint32_t)C.raws.size();
+        if (m == 0) cl_best[2 * ci] = cl_best[2 * ci + 1] = Best{0.0, 0u, 0, 0, false};
+        for (uint32_t b = 0; b < m; b += kPiece)
+          pieces.push_back(Piece{(uint32_t)ci, b, std::min(m, b + kPiece)});
         C.update_e = false;
       }
       if (dp.greedy && C.check_locks) {
         const uint32_t center_reads = C.center >= 0 ? B.raws[(size_t)C.center].reads : 0u;
         for (uint32_t r : C.raws) {
           Raw& R = B.raws[r];
-          if ((double)center_reads * R.comp.lambda > (double)R.reads) R.lock = true;
-          if ((int64_t)r == C.center) R.lock = true;
+          const bool lk = (double)center_reads * R.comp.lambda > (double)R.reads ||
+                          (int64_t)r == C.center;
+          if (lk && !R.lock) {
+            R.lock = true;
+            new_locks.push_back(r);
+          }
         }
         C.check_locks = false;
       }
+    }
+    const bool ds = dp.detect_singletons != 0;
+    piece_best.resize(2 * pieces.size());
+    pool->run(pieces.size(), 1, [&](size_t b, size_t e) {
+      for (size_t k = b; k < e; ++k) {
+        const Piece& pc = pieces[k];
+        const Cluster& C = B.cl[pc.ci];
+        Best ba{0.0, 0u, 0, 0, false}, bp{0.0, 0u, 0, 0, false};
+        for (uint32_t pos = pc.b; pos < pc.e; ++pos) {
+          Raw& R = B.raws[C.raws[pos]];
+          R.p = get_pA(R.reads, R.prior, R.comp.lambda, R.comp.hamming, C.reads, ds, lf);
+          if (pos == 0) continue;  // the first member is never a candidate
+          if (R.reads < dp.min_abund) continue;
+          if (R.comp.hamming < dp.min_hamming) continue;
+          const bool fold_ok = dp.min_fold <= 1.0 ||
+                               (double)R.reads >= dp.min_fold * R.comp.lambda * (double)C.reads;
+          if (!fold_ok) continue;
+          if (!ba.have || better(R.p, R.reads, ba)) ba = Best{R.p, R.reads, pc.ci, pos, true};
+          if (R.prior && (!bp.have || better(R.p, R.reads, bp)))
+            bp = Best{R.p, R.reads, pc.ci, pos, true};
+        }
+        piece_best[2 * k] = ba;
+        piece_best[2 * k + 1] = bp;
+      }
+    });
+    // each repriced cluster's candidates: its pieces folded in order
+    for (size_t k = 0; k < pieces.size(); ++k) {
+      const size_t ci = pieces[k].ci;
+      Best& ca = cl_best[2 * ci];
+      Best& cp = cl_best[2 * ci + 1];
+      if (pieces[k].b == 0) ca = cp = Best{0.0, 0u, 0, 0, false};
+      const Best& a = piece_best[2 * k];
+      const Best& q = piece_best[2 * k + 1];
+      if (a.have && (!ca.have || better(a.p, a.reads, ca))) ca = a;
+      if (q.have && (!cp.have || better(q.p, q.reads, cp))) cp = q;
     }
     st.ms_pupdate += ms_since(t0);
   }
@@ -496,58 +593,20 @@ struct Run {
     return ni;
   }
 
-  // b_bud, cluster.rs:1331-1480; returns the new cluster or -1
-  // A bud candidate; `better` is the scan's replacement test (smaller p,
-  // then more reads).  Keeping the first candidate of the smallest key is an
-  // associative fold, so chunks of the scan combine in scan order.
-  struct Best {
-    double p;
-    uint32_t reads;
-    size_t ci, pos;
-    bool have;
-  };
-  static bool better(double p, uint32_t reads, const Best& b) {
-    return p < b.p || (p == b.p && reads > b.reads);
-  }
-  std::vector<Best> bud_best;
-
+  // b_bud, cluster.rs:1331-1480; returns the new cluster or -1.
+  // The scan visits every cluster's members at positions >= 1 in order and
+  // keeps the first candidate of the smallest key (p, then more reads).
+  // That is an associative fold, so it runs over each cluster's kept
+  // candidates (p_update) in cluster order and gives the whole scan's pick.
   int64_t bud() {
-    constexpr size_t kChunk = 4096;
     const auto t0 = std::chrono::steady_clock::now();
     const double nraw = (double)B.raws.size();
     const uint32_t c0 = (uint32_t)B.cl[0].center;
-    // the scan visits (cluster, position >= 1) in order: flatten it
-    bud_off.resize(B.cl.size() + 1);
-    bud_off[0] = 0;
-    for (size_t ci = 0; ci < B.cl.size(); ++ci) bud_off[ci + 1] = bud_off[ci] + B.cl[ci].raws.size();
-    const size_t m = bud_off.back();
-    const size_t nch = (m + kChunk - 1) / kChunk;
-    bud_best.assign(2 * nch, Best{0.0, 0u, 0, 0, false});
-    pool->run(m, kChunk, [&](size_t b, size_t e) {
-      Best ba{0.0, 0u, 0, 0, false}, bp{0.0, 0u, 0, 0, false};
-      size_t ci = (size_t)(std::upper_bound(bud_off.begin(), bud_off.end(), b) - bud_off.begin()) - 1;
-      for (size_t x = b; x < e; ++x) {
-        while (x >= bud_off[ci + 1]) ++ci;
-        const size_t pos = x - bud_off[ci];
-        if (pos == 0) continue;  // the first member is never a candidate
-        const Cluster& C = B.cl[ci];
-        const Raw& R = B.raws[C.raws[pos]];
-        if (R.reads < dp.min_abund) continue;
-        if (R.comp.hamming < dp.min_hamming) continue;
-        const bool fold_ok =
-            dp.min_fold <= 1.0 || (double)R.reads >= dp.min_fold * R.comp.lambda * (double)C.reads;
-        if (!fold_ok) continue;
-        if (!ba.have || better(R.p, R.reads, ba)) ba = Best{R.p, R.reads, ci, pos, true};
-        if (R.prior && (!bp.have || better(R.p, R.reads, bp))) bp = Best{R.p, R.reads, ci, pos, true};
-      }
-      bud_best[2 * (b / kChunk)] = ba;
-      bud_best[2 * (b / kChunk) + 1] = bp;
-    });
     // seeded with cluster 0's center, which is never chosen itself
     Best ma{B.raws[c0].p, B.raws[c0].reads, 0, 0, false}, mp = ma;
-    for (size_t k = 0; k < nch; ++k) {
-      const Best& a = bud_best[2 * k];
-      const Best& q = bud_best[2 * k + 1];
+    for (size_t ci = 0; ci < B.cl.size(); ++ci) {
+      const Best& a = cl_best[2 * ci];
+      const Best& q = cl_best[2 * ci + 1];
       if (a.have && better(a.p, a.reads, ma)) ma = a;
       if (q.have && better(q.p, q.reads, mp)) mp = q;
     }
@@ -695,13 +754,23 @@ extern "C" int d2g_run_dada(d2g_ctx* ctx, const double* err, uint32_t ncol,
     B.raws[r].prior = prior ? prior[r] != 0 : false;
     B.reads += (*reads)[r];
   }
+  {
+    uint32_t rmax = 0;
+    for (uint32_t r = 0; r < n; ++r) rmax = std::max(rmax, (*reads)[r]);
+    run.lf.build(rmax);
+  }
   B.where.assign(n, 0);
   B.at.resize(n);
   for (uint32_t r = 0; r < n; ++r) B.at[r] = r;
   if (!run.stored.resize(n)) return D2G_ERR_HIP;
   int rc = d2g_store_reset(ctx);  // B::new's e_minmax (containers.rs:342)
   if (rc != D2G_OK) return rc;
-  run.lock.resize(n);
+  // every raw starts unlocked; the flags stay on the device for the run
+  struct LockScope {
+    d2g_ctx* c;
+    ~LockScope() { d2g::ctx_lock_end(c); }
+  } lock_scope{ctx};
+  if (!d2g::ctx_lock_begin(ctx)) return D2G_ERR_HIP;
   run.cand.resize(n);
   Pool pool(pool_threads());
   run.pool = &pool;
@@ -748,7 +817,7 @@ extern "C" int d2g_run_dada(d2g_ctx* ctx, const double* err, uint32_t ncol,
         R.p = 1.0;
         R.correct = true;
       } else {
-        R.p = calc_pA(R.reads, R.comp.lambda * (double)C.reads, true);
+        R.p = calc_pA(R.reads, R.comp.lambda * (double)C.reads, true, run.lf);
         R.correct = R.p >= dada->omega_c;
       }
     }
@@ -781,7 +850,8 @@ extern "C" int d2g_run_dada(d2g_ctx* ctx, const double* err, uint32_t ncol,
     o.birth_e = C.birth_e;
     o.birth_lambda = C.birth_comp.lambda;
     o.self_lambda = C.self;
-    o.posthoc_pval = C.center >= 0 ? calc_pA(B.raws[(size_t)C.center].reads, tot_e[ci], true) : 1.0;
+    o.posthoc_pval =
+        C.center >= 0 ? calc_pA(B.raws[(size_t)C.center].reads, tot_e[ci], true, run.lf) : 1.0;
     for (uint32_t r : C.raws) {
       out_cluster_of[r] = (uint32_t)ci;
       out_map[r] = B.raws[r].correct ? (int32_t)ci : -1;

@@ -231,11 +231,16 @@ hipError_t launch_scatter(const DpTask* unsorted, DpTask* sorted, const uint32_t
 namespace d2g {
 
 // Results are produced in the length-sorted unique order; this restores the
-// caller's order: dst[c * n + orig[s]] = src[c * n + s].
+// caller's order: dst[c * n + orig[s]] = src[c * n + s].  With ctr_dst, the
+// first workgroup also copies ctr_words words of the compare's counters
+// (final by now) next to the results, so one copy brings both to the host.
 __global__ void unpermute_kernel(uint32_t nc, uint32_t n, const uint32_t* orig,
                                  const double* src_lam, const uint32_t* src_ham,
                                  const uint8_t* src_skp, double* dst_lam, uint32_t* dst_ham,
-                                 uint8_t* dst_skp) {
+                                 uint8_t* dst_skp, const uint32_t* ctr_src, uint32_t* ctr_dst,
+                                 uint32_t ctr_words) {
+  if (ctr_dst && blockIdx.x == 0)
+    for (uint32_t t = threadIdx.x; t < ctr_words; t += blockDim.x) ctr_dst[t] = ctr_src[t];
   const size_t total = (size_t)nc * n;
   for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < total;
        i += (size_t)gridDim.x * blockDim.x) {
@@ -249,13 +254,15 @@ __global__ void unpermute_kernel(uint32_t nc, uint32_t n, const uint32_t* orig,
 
 hipError_t launch_unpermute(uint32_t nc, uint32_t n, const uint32_t* orig, const double* src_lam,
                             const uint32_t* src_ham, const uint8_t* src_skp, double* dst_lam,
-                            uint32_t* dst_ham, uint8_t* dst_skp, hipStream_t s) {
+                            uint32_t* dst_ham, uint8_t* dst_skp, hipStream_t s,
+                            const uint32_t* ctr_src, uint32_t* ctr_dst, uint32_t ctr_words) {
   size_t total = (size_t)nc * n;
   size_t blocks = (total + 255) / 256;
   if (blocks > 8192) blocks = 8192;
   if (blocks == 0) blocks = 1;
   hipLaunchKernelGGL(unpermute_kernel, dim3((unsigned)blocks), dim3(256), 0, s, nc, n, orig,
-                     src_lam, src_ham, src_skp, dst_lam, dst_ham, dst_skp);
+                     src_lam, src_ham, src_skp, dst_lam, dst_ham, dst_skp, ctr_src, ctr_dst,
+                     ctr_words);
   return hipGetLastError();
 }
 

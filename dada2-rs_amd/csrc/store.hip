@@ -120,7 +120,20 @@ __global__ void store_reset_kernel(double* e_minmax, uint32_t n) {
     e_minmax[r] = -999.0;  // containers.rs:342
 }
 
+// lock[idx[j]] = 1 for j < k (sorted unique order).
+__global__ void lock_set_kernel(uint8_t* lock, const uint32_t* idx, uint32_t k) {
+  for (uint32_t j = blockIdx.x * blockDim.x + threadIdx.x; j < k; j += gridDim.x * blockDim.x)
+    lock[idx[j]] = 1;
+}
+
 }  // namespace
+
+hipError_t launch_lock_set(uint8_t* lock, const uint32_t* idx, uint32_t k, hipStream_t s) {
+  if (k == 0) return hipSuccess;
+  const uint32_t g = std::max(1u, std::min((k + 255u) / 256u, 1024u));
+  hipLaunchKernelGGL(lock_set_kernel, dim3(g), dim3(256), 0, s, lock, idx, k);
+  return hipGetLastError();
+}
 
 uint32_t store_blocks(uint32_t n) { return (n + kStoreBlock - 1) / kStoreBlock; }
 
