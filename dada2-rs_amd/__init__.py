@@ -483,16 +483,23 @@ class Context:
 
     def compare_store(self, centers, err, first_cluster: int = 0,
                       params: AlignParams | None = None, use_quals=True, greedy=True, lock=None,
-                      fetch=True):
+                      fetch=True, out: np.ndarray | None = None):
         """The compare of `centers` (successive clusters first_cluster, ...)
         and b_compare_parallel's store (cluster.rs:351-397) on the device.
         Returns (kept comparisons as a COMPARISON_DTYPE array in push order,
-        or their count without fetch; each center's self lambda; stats)."""
+        or their count without fetch; each center's self lambda; stats).
+        `out` (COMPARISON_DTYPE, at least len(centers) * n records) lets a
+        caller reuse one landing array across calls; the result is a view of it."""
         params = params or align_params()
         e, lk = self._common(err, params, lock)
         cs = np.ascontiguousarray(np.atleast_1d(centers), np.uint32)
         cap = len(cs) * self.n if fetch else 0
-        out = np.zeros(cap, COMPARISON_DTYPE)
+        if out is None or not fetch:
+            out = np.zeros(cap, COMPARISON_DTYPE)
+        elif out.dtype != COMPARISON_DTYPE or out.ndim != 1 or len(out) < cap \
+                or not out.flags.c_contiguous:
+            raise ValueError("out must be a contiguous COMPARISON_DTYPE array of "
+                             f"at least {cap} records")
         nst = np.zeros(1, np.uint64)
         self_l = np.zeros(len(cs), np.float64)
         st = Stats()

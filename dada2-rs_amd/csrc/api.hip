@@ -192,7 +192,7 @@ struct d2g_ctx {
   const uint64_t* tr_aoff_for = nullptr;
   uint64_t tr_aoff_n = 0, tr_aoff_stride = 0;
   // the device store of b_compare_parallel (d2g_store_reset / d2g_compare_store):
-  // e_minmax per raw (caller order) and the kept comparisons, appended
+  // e_minmax per raw (sorted order) and the kept comparisons, appended
   bool store_valid = false;
   uint64_t store_n = 0;
   DevBuf<uint32_t> inv;  // caller -> sorted
@@ -800,6 +800,12 @@ static int compare_pass(d2g_ctx* c, const uint32_t* centers, uint32_t nc, const 
     max_part = std::max<size_t>(max_part, screen_blocks(n, nc, qb));
     max_slots = std::max<size_t>(max_slots, (size_t)screen_blocks(n, nc, qb) * screen_region(qb));
   }
+  // one center: the single-center screen's regions
+  const bool single = !tm && screen1_fits(nc, p->use_kmers, u.kwords);
+  if (single) {
+    max_part = std::max<size_t>(max_part, screen1_regions(n));
+    max_slots = std::max<size_t>(max_slots, (size_t)screen1_regions(n) * kS1Uniques);
+  }
   HIP_TRY(c->gl.ensure(ntask));
   HIP_TRY(c->gl_unsorted.ensure(max_slots));
   HIP_TRY(c->gl_off.ensure(max_part));
@@ -871,8 +877,9 @@ static int compare_pass(d2g_ctx* c, const uint32_t* centers, uint32_t nc, const 
   // lambda).  Timed runs bracket the kernels with events; otherwise the
   // sequence is captured once per shape into a HIP graph and replayed.
   auto enqueue = [&](bool timed) -> int {
-    const uint32_t n_part = tm ? pair_blocks(n) : screen_blocks(n, nc, c->screen_qblock);
-    const uint32_t region = tm ? kPairBlock : screen_region(c->screen_qblock);
+    const uint32_t n_part = tm ? pair_blocks(n)
+                            : single ? screen1_regions(n) : screen_blocks(n, nc, c->screen_qblock);
+    const uint32_t region = tm ? kPairBlock : single ? kS1Uniques : screen_region(c->screen_qblock);
     if (timed) HIP_TRY(hipEventRecord(c->ev[0], s));
     HIP_TRY(launch_zero2(reinterpret_cast<uint32_t*>(c->ctr.p), sizeof(Counters) / 4,
                          c->bin_count.p, bs.nbins + 1, s));
@@ -904,6 +911,7 @@ static int compare_pass(d2g_ctx* c, const uint32_t* centers, uint32_t nc, const 
     sa.sr_hi = rt.sr_hi;
     sa.lr_lo = rt.lr_lo;
     sa.lr_hi = rt.lr_hi;
+    sa.single = single;
     HIP_TRY(tm ? launch_pair_screen(sa, tm->want, s) : launch_screen(sa, s));
     if (timed) HIP_TRY(hipEventRecord(c->ev[1], s));
     HIP_TRY(launch_plan(bs, c->bin_count.p, c->bin_start.p, c->wave_start.p, c->bin_fill.p, c->ctr.p,
@@ -1394,10 +1402,22 @@ int d2g_compare_store(d2g_ctx* c, const uint32_t* centers, uint32_t nc, uint32_t
   *out_nstored = cnt;
   if (!out_stored) return D2G_OK;
   if (cnt > capacity) return D2G_ERR_ARG;  // kept on the device: d2g_store_fetch
-  std::memcpy(out_stored, c->store_pin + rec_off, std::min<size_t>(cnt, first) * sizeof(ShuffleComp));
-  if (cnt > first)
-    HIP_TRY(hipMemcpy(out_stored + first, c->store + at + first, (cnt - first) * sizeof(ShuffleComp),
-                      hipMemcpyDeviceToHost));
+  if (cnt > first) {
+    // the rest through the pinned area too (grown as needed), one copy
+    const size_t need = rec_off + (size_t)cnt * sizeof(ShuffleComp);
+    if (c->store_pin_cap < need) {
+      (void)hipHostFree(c->store_pin);
+      c->store_pin = nullptr;
+      c->store_pin_cap = 0;
+      const size_t cap = std::max(need, 2 * pin_need);
+      HIP_TRY(hipHostMalloc((void**)&c->store_pin, cap, hipHostMallocDefault));
+      c->store_pin_cap = cap;
+    }
+    HIP_TRY(hipMemcpyAsync(c->store_pin + rec_off, c->store + at, (size_t)cnt * sizeof(ShuffleComp),
+                           hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipStreamSynchronize(s));
+  }
+  std::memcpy(out_stored, c->store_pin + rec_off, (size_t)cnt * sizeof(ShuffleComp));
   return D2G_OK;
 }
 

@@ -65,6 +65,9 @@ struct ScreenArgs {
   // length limit): a center of length <= long_len pairs with sorted uniques
   // [sr_lo, sr_hi), a longer one with [lr_lo, lr_hi)
   uint32_t long_len = 0xFFFFFFFFu, sr_lo = 0, sr_hi = 0xFFFFFFFFu, lr_lo = 0, lr_hi = 0xFFFFFFFFu;
+  // one center (screen1_fits): the single-center screen, whose task regions
+  // are screen1_regions(n) of kS1Uniques slots
+  bool single = false;
 };
 // dstar[n] for n = 0 .. nmax: the smallest dot in 0 .. n with
 // 1.0 - (double)dot / (double)n <= cutoff in f64 (n + 1 when there is none);
@@ -76,6 +79,13 @@ hipError_t launch_screen(const ScreenArgs& a, hipStream_t s);
 uint32_t screen_blocks(uint32_t n_uniques, uint32_t n_centers, uint32_t qblock);
 constexpr uint32_t kScreenCenters = 16;  // centers per screen workgroup
 inline uint32_t screen_region(uint32_t qblock) { return qblock * kScreenCenters; }
+// The single-center screen: kS1Uniques uniques (task slots) per wavefront
+// region, kS1Waves wavefronts per workgroup; k <= 5 (dense vectors of at
+// most 256 words).
+constexpr uint32_t kS1Uniques = 16;
+constexpr uint32_t kS1Waves = 4;
+bool screen1_fits(uint32_t n_centers, int use_kmers, uint32_t kwords);
+uint32_t screen1_regions(uint32_t n_uniques);
 
 // The pairing of the final-subs and birth-subs passes in place of the
 // screen: unique q (sorted index) with the one center want[q] (0xFFFFFFFF:
@@ -267,8 +277,8 @@ struct StoreArgs {
   const double* lam;        // [nc][n], sorted order (the compare's results)
   const uint32_t* ham;      // [nc][n], sorted order
   double total;             // B.reads as f64
-  double* e_minmax;         // [n] caller order, resident
-  uint8_t* flag;            // [nc][n] caller order
+  double* e_minmax;         // [n] sorted order, resident
+  uint8_t* flag;            // [nc][n] sorted order
   uint32_t* bcount;         // [nc][store_blocks(n)] + 1: counts, then offsets and the total
   double* self_out;         // [nc]: each center's lambda against itself
   ShuffleComp* store;       // where this call's first kept comparison goes

@@ -177,7 +177,7 @@ __device__ __forceinline__ void load_rest(const ScreenDev& a, CenterFields (&cf)
 // vectors hold a 255 bucket), gapless or DP route, and the append to the
 // block's task lists.  All lanes of the wavefront call it (the appends use
 // ballots); nc centers of cf are live.
-template <int QL, int CW>
+template <int QL, int CW, int KC>
 __device__ __forceinline__ void screen_pairs(const ScreenDev& a, const QueryFields (&qf)[QL],
                                              const CenterFields (&cf)[CW],
                                              const uint32_t (&sad)[QL][CW], const uint32_t nc,
@@ -207,17 +207,26 @@ __device__ __forceinline__ void screen_pairs(const ScreenDev& a, const QueryFiel
       if (a.use_kmers) {
         dot = (Q.sum + C.sum - sd) >> 1;
         if (Q.has255 && C.has255 && !skip) {
-          // exact kmer_dist8 (kmers.rs:254-269): overflow when min == 255
+          // exact kmer_dist8 (kmers.rs:254-269): overflow when min == 255;
+          // the words are loaded KC at a time (kwords is a multiple of 16),
+          // so one round trip serves KC words
           bool ovf = false;
           uint32_t d2 = 0;
-          for (uint32_t w = 0; w < u.kwords; ++w) {
-            uint32_t x = u.kmer_il[(size_t)w * u.npad + Q.qq];
-            uint32_t y = u.kmer_row[(size_t)C.ci * u.kwords + w];
-            for (int b = 0; b < 4; ++b) {
-              uint32_t m = min((x >> (8 * b)) & 255u, (y >> (8 * b)) & 255u);
-              ovf |= m == 255;
-              d2 += m;
+          for (uint32_t w0 = 0; w0 < u.kwords; w0 += KC) {
+            uint32_t xs[KC], ys[KC];
+#pragma unroll
+            for (int t = 0; t < KC; ++t) {
+              xs[t] = u.kmer_il[(size_t)(w0 + t) * u.npad + Q.qq];
+              ys[t] = u.kmer_row[(size_t)C.ci * u.kwords + w0 + t];
             }
+#pragma unroll
+            for (int t = 0; t < KC; ++t)
+#pragma unroll
+              for (int b = 0; b < 4; ++b) {
+                const uint32_t m = min((xs[t] >> (8 * b)) & 255u, (ys[t] >> (8 * b)) & 255u);
+                ovf |= m == 255;
+                d2 += m;
+              }
           }
           dot = ovf ? u16_min_dot(u, Q.qq, C.ci) : d2;  // nwalign.rs:1270-1278
           if (ovf && in) atomicAdd(&a.ctr->n_exact_fallback, 1u);
@@ -232,14 +241,24 @@ __device__ __forceinline__ void screen_pairs(const ScreenDev& a, const QueryFiel
         double kodist = -1.0;
         if (a.use_kmers && a.gapless && Q.len == C.len) {
           // kord_dist, kmers.rs:314-328
+          // (loaded KC words at a time: one round trip each)
           uint32_t klen = Q.len - (k - 1);
           uint32_t eq = 0;
           uint32_t nw = (klen + 1) / 2;
           const uint32_t* crow = u.kord_row + (size_t)C.ci * u.owords;
-          for (uint32_t w = 0; w < nw; ++w) {
-            uint32_t x = u.kord_il[(size_t)w * u.npad + Q.qq] ^ crow[w];
-            eq += (x & 0xFFFFu) == 0;
-            if (2 * w + 1 < klen) eq += (x >> 16) == 0;
+          for (uint32_t w0 = 0; w0 < nw; w0 += KC) {
+            uint32_t xs[KC];
+#pragma unroll
+            for (int t = 0; t < KC; ++t) {
+              const uint32_t w = w0 + (uint32_t)t;
+              xs[t] = w < nw ? u.kord_il[(size_t)w * u.npad + Q.qq] ^ crow[w] : 0xFFFFFFFFu;
+            }
+#pragma unroll
+            for (int t = 0; t < KC; ++t) {
+              const uint32_t w = w0 + (uint32_t)t;
+              eq += (xs[t] & 0xFFFFu) == 0;  // 0 past the end: both halves nonzero
+              if (2 * w + 1 < klen) eq += (xs[t] >> 16) == 0;
+            }
           }
           kodist = 1.0 - (double)eq / (double)klen;
         }
@@ -373,9 +392,103 @@ __global__ __launch_bounds__(QB / QL * SPLIT) void screen_kernel(ScreenDev a) {
   }
 
   uint32_t n_skip = 0, n_shroud = 0;
-  screen_pairs<QL, CW>(a, qf, cf, sad, nc, c0, &s_ngl, &s_ndp, gl_region, dp_region, n_skip,
-                       n_shroud);
+  screen_pairs<QL, CW, 1>(a, qf, cf, sad, nc, c0, &s_ngl, &s_ndp, gl_region, dp_region, n_skip,
+                          n_shroud);
   screen_block_totals<GT * SPLIT / 64>(a, blk, n_skip, n_shroud, s_ngl, s_ndp);
+}
+
+// One center against every unique (the plain path and each compare of the
+// partition loop).  A wavefront owns kS1Uniques consecutive uniques, its
+// task region.  It reads each unique's whole k-mer vector in one coalesced
+// row load (lane l holds words [l KL, l KL + KL)) with all of the region's
+// loads in flight at once, and forms the per-lane sums of |a - b| against
+// the center's words, held in registers.  A transposing reduction
+// (reduce-scatter over lane bits 2, 3, 4, 5, then sums over bits 0 and 1)
+// leaves unique j's total on lanes 4j .. 4j + 3, and lane 4j makes unique
+// j's decision through screen_pairs: the region's task lists come out in
+// unique order, like the multi-center screen's, for the same plan and
+// scatter.
+template <int KL>
+__global__ __launch_bounds__(64 * kS1Waves) void screen1_kernel(ScreenDev a) {
+  static_assert(kS1Uniques == 16, "the reduction maps 16 uniques onto lane bits 2..5");
+  const UniqueDev& u = a.u;
+  const uint32_t lane = lane_id();
+  const uint32_t wave = threadIdx.x / 64u;
+  const uint32_t region = blockIdx.x * kS1Waves + wave;
+  const uint32_t q0 = region * kS1Uniques;
+  __shared__ uint32_t s_ngl[kS1Waves], s_ndp[kS1Waves];
+  if (q0 >= u.n) return;  // wavefront-uniform; no workgroup barrier below
+  if (lane == 0) {
+    s_ngl[wave] = 0;
+    s_ndp[wave] = 0;
+  }
+  CenterFields cf[1];
+  QueryFields qf[1];
+  cf[0].ci = a.centers[0];
+  const uint32_t j_me = lane >> 2;  // the unique this lane decides for (lanes 4j)
+  qf[0].valid = (lane & 3u) == 0 && q0 + j_me < u.n;
+  qf[0].qq = min(q0 + j_me, u.n - 1);
+  load_rest<1, 1>(a, cf, qf);
+
+  uint32_t v[kS1Uniques];
+#pragma unroll
+  for (uint32_t j = 0; j < kS1Uniques; ++j) v[j] = 0;
+  if (a.use_kmers) {
+    const uint32_t w0 = lane * (uint32_t)KL;
+    const bool mine = w0 < u.kwords;  // k < 5: lanes past the vector hold zeros
+    uint32_t cw[KL];
+    const uint32_t* crow = u.kmer_row + (size_t)cf[0].ci * u.kwords + w0;
+#pragma unroll
+    for (int t = 0; t < KL; ++t) cw[t] = mine ? crow[t] : 0u;
+    uint32_t qw[kS1Uniques][KL];
+#pragma unroll
+    for (uint32_t j = 0; j < kS1Uniques; ++j) {
+      const uint32_t q = min(q0 + j, u.n - 1);
+      const uint32_t* qrow = u.kmer_row + (size_t)q * u.kwords + w0;
+#pragma unroll
+      for (int t = 0; t < KL; ++t) qw[j][t] = mine ? qrow[t] : 0u;
+    }
+#pragma unroll
+    for (uint32_t j = 0; j < kS1Uniques; ++j)
+#pragma unroll
+      for (int t = 0; t < KL; ++t) v[j] = __builtin_amdgcn_sad_u8(qw[j][t], cw[t], v[j]);
+  }
+  // reduce-scatter: at lane bit b, pairs (v[2i], v[2i+1]) become one vector
+  // whose lanes with bit b clear sum v[2i] over lanes l and l ^ (1 << b)
+  // and whose lanes with bit b set sum v[2i+1]; after bits 2..5, vector 0
+  // holds unique (l >> 2) on lane l
+#pragma unroll
+  for (uint32_t step = 0; step < 4; ++step) {
+    const uint32_t bit = 4u << step;
+    const bool hi = (lane & bit) != 0;
+    const uint32_t pairs = kS1Uniques >> (step + 1);
+#pragma unroll
+    for (uint32_t i = 0; i < kS1Uniques / 2; ++i) {
+      if (i >= pairs) break;
+      const uint32_t give = hi ? v[2 * i] : v[2 * i + 1];
+      const uint32_t keep = hi ? v[2 * i + 1] : v[2 * i];
+      v[i] = keep + (uint32_t)__shfl_xor((int)give, (int)bit);
+    }
+  }
+  uint32_t tot = v[0];
+  tot += (uint32_t)__shfl_xor((int)tot, 1);
+  tot += (uint32_t)__shfl_xor((int)tot, 2);
+
+  const uint32_t sad[1][1] = {{tot}};
+  uint32_t n_skip = 0, n_shroud = 0;
+  GaplessTask* gl_region = a.gapless_tasks + (size_t)region * kS1Uniques;
+  DpTask* dp_region = a.dp_unsorted + (size_t)region * kS1Uniques;
+  screen_pairs<1, 1, 16>(a, qf, cf, sad, 1u, 0u, &s_ngl[wave], &s_ndp[wave], gl_region,
+                         dp_region, n_skip, n_shroud);
+  const uint32_t t_skip = (uint32_t)__builtin_popcountll(__builtin_amdgcn_ballot_w64(n_skip != 0));
+  const uint32_t t_shroud =
+      (uint32_t)__builtin_popcountll(__builtin_amdgcn_ballot_w64(n_shroud != 0));
+  if (lane == 0) {
+    a.partials[4 * region] = t_skip;
+    a.partials[4 * region + 1] = t_shroud;
+    a.partials[4 * region + 2] = s_ngl[wave];
+    a.partials[4 * region + 3] = s_ndp[wave];
+  }
 }
 
 void screen_thresholds(double cutoff, uint32_t nmax, uint32_t* out) {
@@ -507,6 +620,11 @@ static void launch_qb(const ScreenDev& d, dim3 grid, int split, hipStream_t st) 
   }
 }
 
+bool screen1_fits(uint32_t n_centers, int use_kmers, uint32_t kwords) {
+  return n_centers == 1 && (!use_kmers || kwords <= 256);
+}
+uint32_t screen1_regions(uint32_t n_uniques) { return (n_uniques + kS1Uniques - 1) / kS1Uniques; }
+
 hipError_t launch_screen(const ScreenArgs& s, hipStream_t st) {
   ScreenDev d;
   d.u = *s.u;
@@ -535,6 +653,14 @@ hipError_t launch_screen(const ScreenArgs& s, hipStream_t st) {
   d.lr_hi = s.lr_hi;
   if (s.split != 1 && s.split != 2 && s.split != 4) return hipErrorInvalidValue;
   if (s.use_kmers && s.dstar == nullptr) return hipErrorInvalidValue;
+  if (s.single && screen1_fits(s.n_centers, s.use_kmers, d.u.kwords)) {
+    const uint32_t blocks = (screen1_regions(d.u.n) + kS1Waves - 1) / kS1Waves;
+    if (d.u.kwords > 64)
+      hipLaunchKernelGGL(screen1_kernel<4>, dim3(blocks), dim3(64 * kS1Waves), 0, st, d);
+    else
+      hipLaunchKernelGGL(screen1_kernel<1>, dim3(blocks), dim3(64 * kS1Waves), 0, st, d);
+    return hipGetLastError();
+  }
   d.nx = (d.u.n + s.qblock - 1) / s.qblock;
   d.ny = (s.n_centers + kScreenCenters - 1) / kScreenCenters;
   const dim3 grid(8u * ((d.nx * d.ny + 7u) / 8u));
