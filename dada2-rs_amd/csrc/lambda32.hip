@@ -7,11 +7,13 @@
 // product over query positions of err[t][q], t = 5 (nt-1) on the diagonal and
 // 4 (nt0-1) + (nt-1) at a substitution.
 //
-// Pass 1 (pairs_loglam_kernel): log lambda as an FP32 Kahan (compensated) sum,
-// in the same position order, of a table log_err[t][q] = (float)log(err[t][q])
-// computed in FP64.  Each table entry carries one FP32 rounding (<= 2^-24
-// relative) and every term has the same sign, so the sum's relative error
-// stays near 2^-24 plus the compensated-summation bound.  pairs_pval_kernel
+// Pass 1 (pairs_loglam_kernel): log lambda as an FP32 compensated sum of a
+// table log_err[t][q] = (float)log(err[t][q]) computed in FP64, the terms
+// split over all 64 lanes and the lanes' (sum, error) accumulators joined by
+// TwoSum in a fixed tree.  Each table entry carries one FP32 rounding
+// (<= 2^-24 relative) and every term has the same sign, so the sum's
+// relative error stays near 2^-24 plus the compensated-summation bound.
+// pairs_pval_kernel
 // (one pair per lane) then takes get_pA / calc_pA (pval.rs:376-398, 208-241)
 // in FP64 from lambda =
 // exp(log lambda): the sentinels (singleton, hamming 0, lambda 0), the
@@ -25,7 +27,7 @@
 // exact ordered FP64 product and the p-value again, so the p < omega_a call
 // is taken on FP64 lambda wherever it is close.
 //
-// Both passes feed their per-pair chains from LDS, as the compare's lambda
+// The recompute feeds its per-pair chains from LDS, as the compare's lambda
 // kernel does: a workgroup of four wavefronts takes 64 pairs, each wavefront
 // stages its 16 pairs' (transition row << 2 | nt - 1, quality) codes with
 // coalesced loads (lane l: positions 4l .. 4l + 3 of a pair's rows), the
@@ -195,53 +197,175 @@ __device__ double get_pA_dev(uint32_t reads, double lambda, uint32_t hamming, ui
   return p / norm;
 }
 
-// Pass 1: FP32 Kahan log-lambda of every pair.
+// Pass 1: FP32 compensated log-lambda of every pair, all 64 lanes.
+// A wavefront takes 8 pairs at a time; lane l reads positions 4l .. 4l + 3
+// of each pair's segment (coalesced words of the sequence and quality rows)
+// and looks its four terms up in the log table, with the segment's
+// substitutions patched in through a per-pair byte row in LDS (the record's
+// center base; 0 = diagonal).  Each lane adds its four terms in FP32 and
+// keeps, per pair, a sum of those partials and its compensation (each
+// addition's exact rounding error, by TwoSum), and one
+// transposing reduction combines the lanes' accumulators pairwise over lane
+// bits 3..5, then 0..2, so pair j's total lands on lanes 8j .. 8j + 7.  Every
+// pair is summed by the same tree whatever its slot or wavefront, so the
+// result repeats run to run and does not depend on the grid.  An err entry
+// of 0 (log -inf) makes lambda 0: it is flagged, not summed.
+struct F2 {
+  float s, c;
+};
+// a + t: the rounded sum, the rounding error added to the compensation
+__device__ __forceinline__ F2 f2_add(const F2 a, const float t) {
+  const float s = a.s + t;
+  const float bb = s - a.s;
+  const float e = (a.s - (s - bb)) + (t - bb);
+  return F2{s, a.c + e};
+}
+// a + b of two accumulators; symmetric in a and b (TwoSum's error is exact)
+__device__ __forceinline__ F2 f2_join(const F2 a, const F2 b) {
+  const float s = a.s + b.s;
+  const float bb = s - a.s;
+  const float e = (a.s - (s - bb)) + (b.s - bb);
+  return F2{s, (a.c + b.c) + e};
+}
+__device__ __forceinline__ F2 f2_xor(const F2 v, const int m) {
+  return F2{__shfl_xor(v.s, m), __shfl_xor(v.c, m)};
+}
+
+constexpr int kLPairs = 8;   // pairs per wavefront group (lane bits 3..5 in the reduction)
+constexpr int kLShift = 3;   // log2(64 / kLPairs): pair j's total lands on lanes 8j .. 8j + 7
+
 __global__ __launch_bounds__(64 * kPWaves) void pairs_loglam_kernel(PairDev a, const double* err,
                                                                      uint32_t ncol, int use_quals,
                                                                      float* out_loglam) {
-  extern __shared__ float logerr[];  // 16 ncol entries and a 0 after them
+  extern __shared__ float logerr[];  // 16 ncol entries and a 0 after them, then the patch rows
   const uint32_t lane = __lane_id(), wave = threadIdx.x / 64u;
   const uint32_t ne = 16u * ncol;
   for (uint32_t i = threadIdx.x; i <= ne; i += 64u * kPWaves)
     logerr[i] = i < ne ? (float)log(err[i]) : 0.0f;
-  uint8_t* rows = reinterpret_cast<uint8_t*>(logerr + ((ne + 2) & ~1u)) + wave * kPPairsW * kPRow;
-  uint8_t* myb = rows + min(lane, (uint32_t)kPPairsW - 1u) * kPRow;
+  // per wavefront: one kPSeg-byte patch row per pair of the group, all zero
+  // between uses
+  uint8_t* patch = reinterpret_cast<uint8_t*>(logerr + ((ne + 4) & ~3u)) +
+                   (size_t)wave * kLPairs * kPSeg;
+  for (uint32_t i = lane; i < (uint32_t)(kLPairs * kPSeg / 4); i += 64u)
+    reinterpret_cast<uint32_t*>(patch)[i] = 0u;
   const bool quals = use_quals && a.qual != nullptr;
   __syncthreads();
-  for (uint32_t base = blockIdx.x * (uint32_t)kPTile; base < a.n; base += gridDim.x * (uint32_t)kPTile) {
-    const uint32_t t = base + wave * kPPairsW + lane;
-    const PairRow me = pair_row(a, lane < (uint32_t)kPPairsW && t < a.n, t);
-    float sum = 0.0f, comp = 0.0f;
-    bool zero = false;
-    for (uint32_t s0 = 0; __builtin_amdgcn_ballot_w64(s0 < me.lq); s0 += kPSeg) {
-      stage_pairs(a, me, s0, lane, quals, rows);
-      patch_pairs(a, me, s0, myb);
-      pwave_sync();
-      const uint32_t seg_n = me.lq > s0 ? min(me.lq - s0, (uint32_t)kPSeg) : 0u;
-      if (lane < (uint32_t)kPPairsW) {
-        const uint32_t* w = reinterpret_cast<const uint32_t*>(myb);
-        for (uint32_t i = 0; __builtin_amdgcn_ballot_w64(i < seg_n); i += 16) {
-          uint32_t w8[8];
+  const uint32_t gw = blockIdx.x * kPWaves + wave, nwv = gridDim.x * kPWaves;
+  for (uint32_t base = gw * kLPairs; base < a.n; base += nwv * kLPairs) {
+    // lanes 0..15 fetch their pair's fields; the pair loops read them by readlane
+    const uint32_t tl = base + lane;
+    const bool livel = lane < (uint32_t)kLPairs && tl < a.n;
+    const uint32_t my_lq = livel ? a.len[tl] : 0u;
+    const uint32_t my_qoff = livel ? (uint32_t)a.off[tl] : 0u;
+    const uint64_t my_so = livel ? a.sub_off[tl] : 0ull;
+    const uint32_t my_ns = livel ? (uint32_t)(a.sub_off[tl + 1] - my_so) : 0u;
+    uint32_t lqmax = 0;
 #pragma unroll
-          for (int h = 0; h < 8; ++h) w8[h] = w[(i >> 1) + (uint32_t)h];
+    for (int j = 0; j < kLPairs; ++j) lqmax = max(lqmax, prl(my_lq, (uint32_t)j));
+    F2 acc[kLPairs];
 #pragma unroll
-          for (int e = 0; e < 16; ++e) {
-            const uint32_t code = (w8[e >> 1] >> (16 * (e & 1))) & 0xFFFFu;
-            const float term = logerr[((code & 0xFFu) >> 2) * ncol + (code >> 8)];
-            if (i + (uint32_t)e < seg_n) {
-              zero = zero || term == -INFINITY;
-              // Kahan: compensated sum in FP32 (no contraction: no products)
-              const float y = term - comp;
-              const float s1 = sum + y;
-              comp = (s1 - sum) - y;
-              sum = s1;
-            }
+    for (int j = 0; j < kLPairs; ++j) acc[j] = F2{0.0f, 0.0f};
+    uint32_t zmask = 0;
+    for (uint32_t s0 = 0; s0 < lqmax; s0 += kPSeg) {
+      // every load of the segment in flight together: the sequence and
+      // quality words, and record `lane` of each pair's substitution list
+      const uint32_t pos = s0 + 4u * lane;
+      uint32_t qw[kLPairs], uw[kLPairs], rq[kLPairs];
+#pragma unroll
+      for (int j = 0; j < kLPairs; ++j) {
+        const uint32_t lq = prl(my_lq, (uint32_t)j);
+        const uint32_t o = prl(my_qoff, (uint32_t)j) + pos;
+        const bool in = pos < lq;
+        qw[j] = in ? pld<uint32_t>(a.seq, o) : 0u;
+        uw[j] = (in && quals) ? pld<uint32_t>(a.qual, o) : 0u;
+        const uint32_t ns = prl(my_ns, (uint32_t)j);
+        const uint32_t sb = prl((uint32_t)my_so, (uint32_t)j);
+        rq[j] = lane < ns ? a.subs[(size_t)sb + lane] : 0xFFFFFFFFu;
+      }
+      // the segment's substitutions into the pairs' patch rows (records
+      // past the first 64 of a pair, rare, in a loop)
+#pragma unroll
+      for (int j = 0; j < kLPairs; ++j) {
+        const uint32_t qp = rq[j] >> 16;
+        if (rq[j] != 0xFFFFFFFFu && qp >= s0 && qp - s0 < (uint32_t)kPSeg)
+          patch[j * kPSeg + (qp - s0)] = (uint8_t)(rq[j] & 0xFFu);
+        const uint32_t ns = prl(my_ns, (uint32_t)j);
+        if (ns > 64u) {
+          const uint32_t sb = prl((uint32_t)my_so, (uint32_t)j);
+          for (uint32_t k = lane + 64u; k < ns; k += 64u) {
+            const uint32_t rec = a.subs[(size_t)sb + k];
+            const uint32_t q2 = rec >> 16;
+            if (q2 >= s0 && q2 - s0 < (uint32_t)kPSeg) patch[j * kPSeg + (q2 - s0)] = (uint8_t)(rec & 0xFFu);
           }
         }
       }
-      pwave_sync();  // rows are restaged for the next segment
+      pwave_sync();
+#pragma unroll
+      for (int j = 0; j < kLPairs; ++j) {
+        const uint32_t lq = prl(my_lq, (uint32_t)j);
+        const uint32_t pw = *reinterpret_cast<const uint32_t*>(patch + j * kPSeg + 4u * lane);
+#pragma unroll
+        for (int b = 0; b < 4; ++b) {
+          // positions past the end read the table's 0 entry
+          const bool in = pos + (uint32_t)b < lq;
+          // bases are 1..4 (checked when the pairs are loaded); the masks
+          // keep every table index inside the 16 rows
+          const uint32_t nt1 = (((qw[j] >> (8 * b)) & 0xFFu) - 1u) & 3u;
+          const uint32_t cb = (pw >> (8 * b)) & 0xFFu;
+          const uint32_t tr = cb ? 4u * ((cb - 1u) & 3u) + nt1 : 5u * nt1;
+          const uint32_t qi = (uw[j] >> (8 * b)) & 0xFFu;
+          const float term = logerr[in ? tr * ncol + qi : ne];
+          if (term == -INFINITY)
+            zmask |= 1u << j;
+          else
+            acc[j] = f2_add(acc[j], term);
+        }
+      }
+      pwave_sync();
+      // the patched bytes back to zero (LDS operations of one wavefront
+      // complete in order, so the next segment's writes come after)
+#pragma unroll
+      for (int j = 0; j < kLPairs; ++j) {
+        const uint32_t qp = rq[j] >> 16;
+        if (rq[j] != 0xFFFFFFFFu && qp >= s0 && qp - s0 < (uint32_t)kPSeg)
+          patch[j * kPSeg + (qp - s0)] = 0u;
+        const uint32_t ns = prl(my_ns, (uint32_t)j);
+        if (ns > 64u) {
+          const uint32_t sb = prl((uint32_t)my_so, (uint32_t)j);
+          for (uint32_t k = lane + 64u; k < ns; k += 64u) {
+            const uint32_t q2 = a.subs[(size_t)sb + k] >> 16;
+            if (q2 >= s0 && q2 - s0 < (uint32_t)kPSeg) patch[j * kPSeg + (q2 - s0)] = 0u;
+          }
+        }
+      }
+      pwave_sync();
     }
-    if (me.live) out_loglam[me.t] = zero ? -INFINITY : sum;
+    // reduce-scatter over lane bits 3..5: at bit b, pairs (2i, 2i + 1) become
+    // slot i, lanes with bit b clear holding pair 2i and the others 2i + 1,
+    // each the join of its own and its partner lane's accumulators
+#pragma unroll
+    for (int step = 0; step < 6 - kLShift; ++step) {
+      const int bit = (1 << kLShift) << step;
+      const bool hi = (lane & (uint32_t)bit) != 0;
+#pragma unroll
+      for (int i = 0; i < kLPairs / 2; ++i) {
+        if (i >= (kLPairs >> (step + 1))) break;
+        const F2 give = hi ? acc[2 * i] : acc[2 * i + 1];
+        const F2 keep = hi ? acc[2 * i + 1] : acc[2 * i];
+        acc[i] = f2_join(keep, f2_xor(give, bit));
+      }
+    }
+    F2 tot = acc[0];
+#pragma unroll
+    for (int bit = 1; bit < (1 << kLShift); bit <<= 1) tot = f2_join(tot, f2_xor(tot, bit));
+    const uint32_t j_me = lane >> kLShift;
+    bool zero = false;
+#pragma unroll
+    for (int j = 0; j < kLPairs; ++j)
+      if (__builtin_amdgcn_ballot_w64((zmask >> j) & 1u) && j_me == (uint32_t)j) zero = true;
+    const uint32_t t = base + j_me;
+    if ((lane & ((1u << kLShift) - 1u)) == 0 && t < a.n)
+      out_loglam[t] = zero ? -INFINITY : tot.s + tot.c;
   }
 }
 
@@ -345,7 +469,8 @@ hipError_t launch_pairs_lambda(const PairArgs& s, hipStream_t st) {
   const uint32_t tiles = (s.n + kPTile - 1) / kPTile;
   const uint32_t blocks = min(tiles, s.max_blocks);
   const size_t rows = (size_t)kPTile * kPRow;
-  const size_t lds1 = (((size_t)16 * s.ncol + 2) & ~(size_t)1) * sizeof(float) + rows;
+  const size_t lds1 = (((size_t)16 * s.ncol + 4) & ~(size_t)3) * sizeof(float) +
+                      (size_t)kPWaves * kLPairs * kPSeg;
   const size_t lds2 = ((size_t)16 * s.ncol + 1) * sizeof(double) + rows;
   if (lds1 > 48 * 1024) {
     hipError_t e = hipFuncSetAttribute((const void*)pairs_loglam_kernel,

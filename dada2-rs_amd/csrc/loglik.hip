@@ -15,17 +15,22 @@
 // are FP64.
 // The pairs are staged into LDS rows as d2g_stage.h describes (four
 // wavefronts per 64 pairs; each wavefront stages its 16 pairs segment by
-// segment), and then every staged pair is taken by all 64 lanes in turn,
-// four positions per lane:
-//   forward:  the lanes' FP64 partial sums of the log table meet in a fixed
-//             butterfly and the pair's running sum takes the segment's total,
-//             so log lambda_j does not depend on scheduling;
-//   backward: the lanes count the segment's table indices into an integer
-//             histogram (u32 LDS atomics: order-free), then each bin with a
-//             count takes one FP64 multiply-add, w_j x count, into the
-//             wavefront's gradient row.  Pairs and segments are visited in a
-//             fixed order and the workgroup adds its wavefronts' rows in
-//             wavefront order, so the gradient's bits repeat run to run.
+// segment), and then every staged pair is taken by all 64 lanes, four
+// positions per lane:
+//   forward:  each lane sums its four table entries of every staged pair in
+//             FP64; one transposing reduction over the 16 pairs (pairwise
+//             joins over lane bits 2..5, then 0 and 1) leaves pair j's
+//             segment total on lanes 4j .. 4j + 3, the same tree for every
+//             pair, and the pair's running sum takes it, so log lambda_j does
+//             not depend on scheduling;
+//   backward: per staged pair the lanes count the segment's table indices
+//             into an integer histogram (u32 LDS atomics: order-free); the
+//             lane whose increment found a bin at 0 owns that bin and, once
+//             every lane has counted, applies its one FP64 multiply-add,
+//             w_j x count, to the wavefront's gradient row and clears it.
+//             Pairs and segments are visited in a fixed order and the
+//             workgroup adds its wavefronts' rows in wavefront order, so the
+//             gradient's bits repeat run to run.
 // Positions past a pair's end index an extra table entry: 0.0 in the
 // forward's log table and a discarded count in the backward.
 #include "d2g_stage.h"
@@ -65,15 +70,36 @@ __global__ __launch_bounds__(64 * kStWaves) void loglik_fwd_kernel(LoglikDev a, 
     for (uint32_t s0 = 0; __builtin_amdgcn_ballot_w64(s0 < me.lq); s0 += kStSeg) {
       stage_segment(a.ts, me, s0, lane, quals, any_gl, rows, myb, ham, cur);
       wave_lds_sync();
-      for (uint32_t pr = 0; pr < (uint32_t)kStPairsW; ++pr) {
+      double part[kStPairsW];
+#pragma unroll
+      for (int pr = 0; pr < kStPairsW; ++pr) {
+        // eight pairs' table reads in flight at a time
+        if (pr == kStPairsW / 2) __builtin_amdgcn_sched_barrier(0);
         // positions past the pair's end carry the 0.0 entry
         const uint32_t w2 = *reinterpret_cast<const uint32_t*>(rows + pr * kStRow + 4u * lane);
         const uint32_t w3 = *reinterpret_cast<const uint32_t*>(rows + pr * kStRow + 4u * lane + 256u);
-        double part = le[code_index(w2 & 0xFFFFu, ncol)] + le[code_index(w2 >> 16, ncol)];
-        part += le[code_index(w3 & 0xFFFFu, ncol)] + le[code_index(w3 >> 16, ncol)];
-        for (int off = 32; off > 0; off >>= 1) part += __shfl_xor(part, off);
-        if (lane == 0) llp[wave][pr] += part;
+        part[pr] = le[code_index(w2 & 0xFFFFu, ncol)] + le[code_index(w2 >> 16, ncol)];
+        part[pr] += le[code_index(w3 & 0xFFFFu, ncol)] + le[code_index(w3 >> 16, ncol)];
       }
+      // reduce-scatter: at lane bit b, pairs (2i, 2i + 1) become slot i,
+      // lanes with bit b clear holding pair 2i, the others 2i + 1
+      static_assert(kStPairsW == 16, "16 staged pairs map onto lane bits 2..5");
+#pragma unroll
+      for (int step = 0; step < 4; ++step) {
+        const int bit = 4 << step;
+        const bool hi = (lane & (uint32_t)bit) != 0;
+#pragma unroll
+        for (int i = 0; i < kStPairsW / 2; ++i) {
+          if (i >= (kStPairsW >> (step + 1))) break;
+          const double give = hi ? part[2 * i] : part[2 * i + 1];
+          const double keep = hi ? part[2 * i + 1] : part[2 * i];
+          part[i] = keep + __shfl_xor(give, bit);
+        }
+      }
+      double segt = part[0];
+      segt += __shfl_xor(segt, 1);
+      segt += __shfl_xor(segt, 2);
+      if ((lane & 3u) == 0) llp[wave][lane >> 2] += segt;
       wave_lds_sync();
     }
     if (me.live) {
@@ -129,16 +155,18 @@ __global__ __launch_bounds__(64 * kStWaves) void loglik_bwd_kernel(LoglikDev a) 
         const uint32_t w2 = *reinterpret_cast<const uint32_t*>(rows + pr * kStRow + 4u * lane);
         const uint32_t w3 = *reinterpret_cast<const uint32_t*>(rows + pr * kStRow + 4u * lane + 256u);
         // past the pair's end: index 16 ncol, the discarded count
-        atomicAdd(&cnt[code_index(w2 & 0xFFFFu, ncol)], 1u);
-        atomicAdd(&cnt[code_index(w2 >> 16, ncol)], 1u);
-        atomicAdd(&cnt[code_index(w3 & 0xFFFFu, ncol)], 1u);
-        atomicAdd(&cnt[code_index(w3 >> 16, ncol)], 1u);
+        const uint32_t ix[4] = {code_index(w2 & 0xFFFFu, ncol), code_index(w2 >> 16, ncol),
+                                code_index(w3 & 0xFFFFu, ncol), code_index(w3 >> 16, ncol)};
+        uint32_t old[4];
+#pragma unroll
+        for (int e = 0; e < 4; ++e) old[e] = atomicAdd(&cnt[ix[e]], 1u);
         wave_lds_sync();
-        for (uint32_t i = lane; i < ne; i += 64u) {
-          const uint32_t k = cnt[i];
-          if (k) {
-            g[i] += wp * (double)k;
-            cnt[i] = 0u;
+        // the bin's owner (its first increment) applies the pair's count
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+          if (old[e] == 0u && ix[e] < ne) {
+            g[ix[e]] += wp * (double)cnt[ix[e]];
+            cnt[ix[e]] = 0u;
           }
         }
         wave_lds_sync();
