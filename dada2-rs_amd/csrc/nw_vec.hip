@@ -816,23 +816,34 @@ __device__ __forceinline__ void runs_block(const int b, const uint32_t hp, PathS
   }
 }
 
-// Blocks last/16 .. 0.  Two history words alternate, each reloaded (for the
-// block two below) as soon as its block is done, so every load has a whole
-// block of work to arrive in and no register copy waits for it.
+// Blocks last/16 .. 0.  Four history words rotate, each reloaded (for the
+// block four below) as soon as its block is done, so every load has three
+// blocks of work to arrive in (a lone wavefront's blocks are short next to
+// an L2 round trip) and no register copy waits for it.
 __device__ __forceinline__ void backward_runs(const int last, const uint32_t* hist_p,
                                               const int lane, PathState& ps, const BwConst& B,
                                               const NwDev& a) {
   int b = last >> 4;
-  uint32_t ha = hist_p[b * 64 + lane];
-  uint32_t hb = hist_p[imax(b - 1, 0) * 64 + lane];
+  uint32_t h0 = hist_p[b * 64 + lane];
+  uint32_t h1 = hist_p[imax(b - 1, 0) * 64 + lane];
+  uint32_t h2 = hist_p[imax(b - 2, 0) * 64 + lane];
+  uint32_t h3 = hist_p[imax(b - 3, 0) * 64 + lane];
   for (;;) {
-    runs_block(b, ha, ps, B, a);
+    runs_block(b, h0, ps, B, a);
     if (b == 0) break;
-    ha = hist_p[imax(b - 2, 0) * 64 + lane];
+    h0 = hist_p[imax(b - 4, 0) * 64 + lane];
     --b;
-    runs_block(b, hb, ps, B, a);
+    runs_block(b, h1, ps, B, a);
     if (b == 0) break;
-    hb = hist_p[imax(b - 2, 0) * 64 + lane];
+    h1 = hist_p[imax(b - 4, 0) * 64 + lane];
+    --b;
+    runs_block(b, h2, ps, B, a);
+    if (b == 0) break;
+    h2 = hist_p[imax(b - 4, 0) * 64 + lane];
+    --b;
+    runs_block(b, h3, ps, B, a);
+    if (b == 0) break;
+    h3 = hist_p[imax(b - 4, 0) * 64 + lane];
     --b;
   }
 }

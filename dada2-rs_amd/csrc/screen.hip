@@ -180,7 +180,8 @@ __device__ __forceinline__ void load_rest(const ScreenDev& a, CenterFields (&cf)
 template <int QL, int CW, int KC>
 __device__ __forceinline__ void screen_pairs(const ScreenDev& a, const QueryFields (&qf)[QL],
                                              const CenterFields (&cf)[CW],
-                                             const uint32_t (&sad)[QL][CW], const uint32_t nc,
+                                             const uint32_t (&sad)[QL][CW],
+                                             const int32_t (&keq)[QL][CW], const uint32_t nc,
                                              const uint32_t c0, uint32_t* s_ngl, uint32_t* s_ndp,
                                              GaplessTask* gl_region, DpTask* dp_region,
                                              uint32_t& n_skip, uint32_t& n_shroud) {
@@ -240,24 +241,26 @@ __device__ __forceinline__ void screen_pairs(const ScreenDev& a, const QueryFiel
         const double kdist = a.use_kmers ? 1.0 - (double)dot / (double)(mn - k + 1) : 0.0;
         double kodist = -1.0;
         if (a.use_kmers && a.gapless && Q.len == C.len) {
-          // kord_dist, kmers.rs:314-328
-          // (loaded KC words at a time: one round trip each)
+          // kord_dist, kmers.rs:314-328: the count of equal positions comes
+          // precomputed (keq >= 0) or is counted here, KC words per round trip
           uint32_t klen = Q.len - (k - 1);
-          uint32_t eq = 0;
-          uint32_t nw = (klen + 1) / 2;
-          const uint32_t* crow = u.kord_row + (size_t)C.ci * u.owords;
-          for (uint32_t w0 = 0; w0 < nw; w0 += KC) {
-            uint32_t xs[KC];
+          uint32_t eq = keq[j][c] >= 0 ? (uint32_t)keq[j][c] : 0u;
+          if (keq[j][c] < 0) {
+            const uint32_t nw = (klen + 1) / 2;
+            const uint32_t* crow = u.kord_row + (size_t)C.ci * u.owords;
+            for (uint32_t w0 = 0; w0 < nw; w0 += KC) {
+              uint32_t xs[KC];
 #pragma unroll
-            for (int t = 0; t < KC; ++t) {
-              const uint32_t w = w0 + (uint32_t)t;
-              xs[t] = w < nw ? u.kord_il[(size_t)w * u.npad + Q.qq] ^ crow[w] : 0xFFFFFFFFu;
-            }
+              for (int t = 0; t < KC; ++t) {
+                const uint32_t w = w0 + (uint32_t)t;
+                xs[t] = w < nw ? u.kord_il[(size_t)w * u.npad + Q.qq] ^ crow[w] : 0xFFFFFFFFu;
+              }
 #pragma unroll
-            for (int t = 0; t < KC; ++t) {
-              const uint32_t w = w0 + (uint32_t)t;
-              eq += (xs[t] & 0xFFFFu) == 0;  // 0 past the end: both halves nonzero
-              if (2 * w + 1 < klen) eq += (xs[t] >> 16) == 0;
+              for (int t = 0; t < KC; ++t) {
+                const uint32_t w = w0 + (uint32_t)t;
+                eq += (xs[t] & 0xFFFFu) == 0;  // 0 past the end: both halves nonzero
+                if (2 * w + 1 < klen) eq += (xs[t] >> 16) == 0;
+              }
             }
           }
           kodist = 1.0 - (double)eq / (double)klen;
@@ -391,8 +394,13 @@ __global__ __launch_bounds__(QB / QL * SPLIT) void screen_kernel(ScreenDev a) {
     }
   }
 
+  int32_t keq[QL][CW];
+#pragma unroll
+  for (int j = 0; j < QL; ++j)
+#pragma unroll
+    for (int c = 0; c < CW; ++c) keq[j][c] = -1;
   uint32_t n_skip = 0, n_shroud = 0;
-  screen_pairs<QL, CW, 1>(a, qf, cf, sad, nc, c0, &s_ngl, &s_ndp, gl_region, dp_region, n_skip,
+  screen_pairs<QL, CW, 1>(a, qf, cf, sad, keq, nc, c0, &s_ngl, &s_ndp, gl_region, dp_region, n_skip,
                           n_shroud);
   screen_block_totals<GT * SPLIT / 64>(a, blk, n_skip, n_shroud, s_ngl, s_ndp);
 }
@@ -422,6 +430,17 @@ __global__ __launch_bounds__(64 * kS1Waves) void screen1_kernel(ScreenDev a) {
     s_ngl[wave] = 0;
     s_ndp[wave] = 0;
   }
+  // the region's k-mer rows first: they depend on nothing loaded
+  const uint32_t w0 = lane * (uint32_t)KL;
+  const bool mine = a.use_kmers && w0 < u.kwords;  // k < 5: lanes past the vector hold zeros
+  uint32_t qw[kS1Uniques][KL];
+#pragma unroll
+  for (uint32_t j = 0; j < kS1Uniques; ++j) {
+    const uint32_t q = min(q0 + j, u.n - 1);
+    const uint32_t* qrow = u.kmer_row + (size_t)q * u.kwords + w0;
+#pragma unroll
+    for (int t = 0; t < KL; ++t) qw[j][t] = mine ? qrow[t] : 0u;
+  }
   CenterFields cf[1];
   QueryFields qf[1];
   cf[0].ci = a.centers[0];
@@ -434,20 +453,10 @@ __global__ __launch_bounds__(64 * kS1Waves) void screen1_kernel(ScreenDev a) {
 #pragma unroll
   for (uint32_t j = 0; j < kS1Uniques; ++j) v[j] = 0;
   if (a.use_kmers) {
-    const uint32_t w0 = lane * (uint32_t)KL;
-    const bool mine = w0 < u.kwords;  // k < 5: lanes past the vector hold zeros
     uint32_t cw[KL];
     const uint32_t* crow = u.kmer_row + (size_t)cf[0].ci * u.kwords + w0;
 #pragma unroll
     for (int t = 0; t < KL; ++t) cw[t] = mine ? crow[t] : 0u;
-    uint32_t qw[kS1Uniques][KL];
-#pragma unroll
-    for (uint32_t j = 0; j < kS1Uniques; ++j) {
-      const uint32_t q = min(q0 + j, u.n - 1);
-      const uint32_t* qrow = u.kmer_row + (size_t)q * u.kwords + w0;
-#pragma unroll
-      for (int t = 0; t < KL; ++t) qw[j][t] = mine ? qrow[t] : 0u;
-    }
 #pragma unroll
     for (uint32_t j = 0; j < kS1Uniques; ++j)
 #pragma unroll
@@ -474,11 +483,71 @@ __global__ __launch_bounds__(64 * kS1Waves) void screen1_kernel(ScreenDev a) {
   tot += (uint32_t)__shfl_xor((int)tot, 1);
   tot += (uint32_t)__shfl_xor((int)tot, 2);
 
+  // The gapless dispatch's kord_dist (kmers.rs:314-328) for the pairs
+  // that pass the screen with equal lengths: the whole wavefront reads their
+  // k-mer order rows (one coalesced pass per 64 words, every needing pair's
+  // loads in flight together) and the same transposing reduction puts pair
+  // j's count of equal positions on lanes 4j.  Lanes that decide no pair
+  // skip it (keq 0, unused); a pair whose vectors both hold a 255 count
+  // takes its exact dot first and counts its own (keq -1).
+  int32_t keq[1][1] = {{qf[0].valid ? -1 : 0}};
+  if (a.use_kmers && a.gapless) {
+    const QueryFields& Q = qf[0];
+    const CenterFields& C = cf[0];
+    const bool skip = a.greedy && (Q.reads > C.reads || Q.lock);
+    const bool exact = Q.has255 && C.has255;
+    const uint32_t dot = (Q.sum + C.sum - tot) >> 1;
+    const bool shroud = !skip && dot < (Q.len <= C.len ? Q.thr : C.thr);
+    const bool need = Q.valid && pair_in_pass(a, C.len, Q.qq) && !skip && !exact && !shroud &&
+                      Q.len == C.len;
+    const uint64_t nm = __builtin_amdgcn_ballot_w64(need);
+    if (nm) {
+      // every needing pair has the center's length
+      const uint32_t klen = C.len - (u.k - 1);
+      const uint32_t nw = (klen + 1) / 2;
+      const uint32_t* crow = u.kord_row + (size_t)C.ci * u.owords;
+      uint32_t part[kS1Uniques];
+#pragma unroll
+      for (uint32_t j = 0; j < kS1Uniques; ++j) part[j] = 0;
+      for (uint32_t w0 = 0; w0 < nw; w0 += 64u) {
+        const uint32_t w = w0 + lane;
+        const bool inw = w < nw;
+        const uint32_t cw = inw ? crow[w] : 0u;
+        uint32_t x[kS1Uniques];
+#pragma unroll
+        for (uint32_t j = 0; j < kS1Uniques; ++j) {
+          const uint32_t q = min(q0 + j, u.n - 1);
+          x[j] = (inw && ((nm >> (4 * j)) & 1ull)) ? u.kord_row[(size_t)q * u.owords + w] ^ cw
+                                                     : 0xFFFFFFFFu;
+        }
+#pragma unroll
+        for (uint32_t j = 0; j < kS1Uniques; ++j)
+          part[j] += ((x[j] & 0xFFFFu) == 0 ? 1u : 0u) +
+                     ((2 * w + 1 < klen && (x[j] >> 16) == 0) ? 1u : 0u);
+      }
+#pragma unroll
+      for (uint32_t step = 0; step < 4; ++step) {
+        const uint32_t bit = 4u << step;
+        const bool hi = (lane & bit) != 0;
+#pragma unroll
+        for (uint32_t i = 0; i < kS1Uniques / 2; ++i) {
+          if (i >= (kS1Uniques >> (step + 1))) break;
+          const uint32_t give = hi ? part[2 * i] : part[2 * i + 1];
+          const uint32_t keep = hi ? part[2 * i + 1] : part[2 * i];
+          part[i] = keep + (uint32_t)__shfl_xor((int)give, (int)bit);
+        }
+      }
+      uint32_t eq = part[0];
+      eq += (uint32_t)__shfl_xor((int)eq, 1);
+      eq += (uint32_t)__shfl_xor((int)eq, 2);
+      if (need) keq[0][0] = (int32_t)eq;
+    }
+  }
   const uint32_t sad[1][1] = {{tot}};
   uint32_t n_skip = 0, n_shroud = 0;
   GaplessTask* gl_region = a.gapless_tasks + (size_t)region * kS1Uniques;
   DpTask* dp_region = a.dp_unsorted + (size_t)region * kS1Uniques;
-  screen_pairs<1, 1, 16>(a, qf, cf, sad, 1u, 0u, &s_ngl[wave], &s_ndp[wave], gl_region,
+  screen_pairs<1, 1, 16>(a, qf, cf, sad, keq, 1u, 0u, &s_ngl[wave], &s_ndp[wave], gl_region,
                          dp_region, n_skip, n_shroud);
   const uint32_t t_skip = (uint32_t)__builtin_popcountll(__builtin_amdgcn_ballot_w64(n_skip != 0));
   const uint32_t t_shroud =
