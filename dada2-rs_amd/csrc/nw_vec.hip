@@ -262,6 +262,7 @@ __device__ __forceinline__ int clamp16(int v) { return v > 32767 ? 32767 : (v < 
 // Row constants of one wavefront (uniform).
 struct RowConst {
   int match, mismatch, gap, end_gap, fill, last;
+  int dmatch, dmis;  // match - 2 gap, mismatch - 2 gap: the steady rows' offset diagonal
 };
 // Per-lane column state.
 struct LaneState {
@@ -519,27 +520,41 @@ __device__ __forceinline__ void masked_rows(const int r0, const int r1, CRowRec*
 //    pointer bits are never read: no path of such a row enters them;
 //  - without CLAMP every candidate is an active cell's score or fill plus
 //    one increment, inside i16 by the launch's score bound, so the clamps
-//    are the identity and the gap is added once, after max(left, up).
+//    are the identity.  The run then keeps every score of row r offset by
+//    -r gap (steady_rows converts on entry and exit): each candidate of a
+//    row carries the same offset, so every comparison, pointer and maximum
+//    is unchanged, a gap move adds nothing and a diagonal adds score - 2 gap.
+//    Real cells stay at or above 0 there (no move costs more than -gap per
+//    row) and fill far below, so a fill-derived candidate still never wins
+//    or ties.  With the gap add gone, max(left, up) takes the shifted
+//    neighbour as a DPP operand, and up-versus-left is read off it against
+//    the unshifted one (or off the min on the other parity).
 // The pointer (dploop precedence, nwalign.rs:700-737; a diagonal over a
 // mismatch is 0) is shifted into hp as two bits.
 template <bool EVEN, bool SWAP, bool CLAMP>
 __device__ __forceinline__ void steady_cell(const int ca, const int cb, const uint64_t act,
                                             LaneState& ls, const RowConst K, uint32_t& hp) {
   const bool eq = ca == cb;
-  int dg = ls.prev2 + (eq ? K.match : K.mismatch);
+  int dg;
   const int sh = EVEN ? shift_from_lower(ls.prev) : shift_from_upper(ls.prev);
-  const int lraw = EVEN ? sh : ls.prev;
-  const int uraw = EVEN ? ls.prev : sh;
   bool upf;
   int m;
   if (CLAMP) {
-    dg = clamp16(dg);
+    dg = clamp16(ls.prev2 + (eq ? K.match : K.mismatch));
+    const int lraw = EVEN ? sh : ls.prev;
+    const int uraw = EVEN ? ls.prev : sh;
     const int l = clamp16(lraw + K.gap), u = clamp16(uraw + K.gap);
     upf = SWAP ? (l < u) : (u >= l);
     m = l > u ? l : u;
   } else {
-    upf = SWAP ? (lraw < uraw) : (uraw >= lraw);
-    m = (lraw > uraw ? lraw : uraw) + K.gap;
+    dg = ls.prev2 + (eq ? K.dmatch : K.dmis);
+    m = max(sh, ls.prev);
+    // left/up: (sh, prev) on even rows, (prev, sh) on odd rows; upf is
+    // u >= l, or l < u with SWAP
+    if (EVEN)
+      upf = SWAP ? min(sh, ls.prev) != ls.prev : m == ls.prev;
+    else
+      upf = SWAP ? m != ls.prev : min(sh, ls.prev) == ls.prev;
   }
   const bool dw = dg > m;
   const int val = dw ? dg : m;
@@ -599,6 +614,10 @@ __device__ __forceinline__ void steady_rows(const int b0, const int b1, CRowRec*
   const uint8_t* p2 = s2l + (int)r0[1];
   const uint64_t act_even = ((uint64_t)r0[11] << 32) | r0[10];
   const uint64_t act_odd = ((uint64_t)r0[kRecWords + 11] << 32) | r0[kRecWords + 10];
+  if (!CLAMP) {  // into the offset domain: row r's scores minus r gap
+    ls.prev -= (b0 - 1) * K.gap;
+    ls.prev2 -= (b0 - 2) * K.gap;
+  }
   const int f0 = imin((b0 + 15) & ~15, b1);  // the first whole block
   const int f1 = imax(b1 & ~15, f0);         // the end of the whole blocks
   steady_part<SWAP, CLAMP>(b0, f0, p1, p2, act_even, act_odd, ls, K, lane, hist_p);
@@ -619,6 +638,12 @@ __device__ __forceinline__ void steady_rows(const int b0, const int b1, CRowRec*
     p2 += 8;
   }
   steady_part<SWAP, CLAMP>(f1, b1, p1, p2, act_even, act_odd, ls, K, lane, hist_p);
+  if (!CLAMP) {
+    // back from the offset domain: rows b1 - 1 (odd) and b1 - 2 (even);
+    // their inactive lanes hold fill, as the plain update leaves them
+    ls.prev = __builtin_amdgcn_inverse_ballot_w64(act_odd) ? ls.prev + (b1 - 1) * K.gap : K.fill;
+    ls.prev2 = __builtin_amdgcn_inverse_ballot_w64(act_even) ? ls.prev2 + (b1 - 2) * K.gap : K.fill;
+  }
 }
 
 template <bool SWAP, bool CLAMP>
@@ -909,6 +934,8 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(8, 8))) void
     min_score = min_score < 0 ? min_score : 0;
     K.fill = (int)(int16_t)(uint16_t)(uint32_t)(-32768 - min_score);
     K.last = len1 + len2;
+    K.dmatch = K.match - 2 * K.gap;
+    K.dmis = K.mismatch - 2 * K.gap;
     const int last = K.last;
 
     // ---- rows 0 and 1 (nwalign.rs:842-861) ----
