@@ -208,6 +208,26 @@ def test_other_kmer_sizes(ctx, oracle, pkg, k):
             assert got.stats["shrouded"] > 0
 
 
+@pytest.mark.parametrize("k", [3, 4, 5])
+def test_one_center_screen_kmer_sizes(ctx, oracle, pkg, k):
+    """One center per call takes the one-center screen (a wavefront per 16
+    uniques, whole-vector row loads; one word per lane at k <= 4, four at
+    k = 5): every center of a small batch alone, at two cutoffs, against the
+    oracle."""
+    w = synth.subsample(synth.workload_batched(), 600, n_centers=6)
+    ctx.load_uniques(w.seqs, w.quals, w.reads, k=k)
+    rs = oracle.RawSet(w.seqs, w.quals, w.reads, k=k)
+    for cutoff in (0.42, 0.1):
+        params = pkg.align_params(band=w.band, kmer_size=k, kdist_cutoff=cutoff)
+        op = oracle.make_params(band=w.band, kmer_size=k, kdist_cutoff=cutoff)
+        for c in w.centers:
+            got = ctx.compare_center(int(c), w.err, params)
+            lam, ham, skp, _, _ = oracle.compare_center(rs, int(c), w.err, 41, op, nthreads=8)
+            assert np.array_equal(got.lam.view(np.uint64), lam.view(np.uint64)), (k, cutoff, c)
+            assert np.array_equal(got.hamming, ham), (k, cutoff, c)
+            assert np.array_equal(got.skipped, skp), (k, cutoff, c)
+
+
 def _saturated_workload(seed=91):
     """Uniques of 300-330 nt led by a 280-nt A run, so the AAAAA bucket
     saturates kmer8 at 255 (kmers.rs:390-401: kmer_dist8 returns -1.0) and
@@ -257,6 +277,16 @@ def test_u16_fallback_on_saturated_buckets(ctx, oracle, pkg):
                      np.stack([r[2] for r in rows]))
     # the saturated pairs aligned (the fallback passed them), the random ones shrouded
     assert (got.hamming[:, :60] != 0xFFFFFFFF).sum() > 0
+    # the same centers one at a time: the one-center screen's own fallback
+    params = pkg.align_params(band=16, kdist_cutoff=0.42)
+    op = oracle.make_params(band=16, kdist_cutoff=0.42)
+    rs = oracle.RawSet(seqs, quals, reads, k=5)
+    for c in centers[:4]:
+        one = ctx.compare_center(int(c), synth.fixed_err(41), params)
+        lam, ham, skp, _, _ = oracle.compare_center(rs, int(c), synth.fixed_err(41), 41, op,
+                                                    nthreads=8)
+        assert np.array_equal(one.lam.view(np.uint64), lam.view(np.uint64))
+        assert np.array_equal(one.hamming, ham) and np.array_equal(one.skipped, skp)
 
 
 @pytest.mark.parametrize("tile", [0, 2, 1])
