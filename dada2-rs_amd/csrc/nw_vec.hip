@@ -678,6 +678,7 @@ struct BwConst {
   const uint8_t* s1b;
   const uint8_t* s2b;
   uint64_t ao;  // alignment string offset (parity mode)
+  int ncol;     // lanes of a pair (alignment-output mode writes a run's bytes over them)
 };
 // Path state of this lane's pair, replicated over the pair's lanes.
 struct PathState {
@@ -779,7 +780,15 @@ __device__ __forceinline__ void backward(const int last, const uint32_t* hist_p,
 // visits the cells of the row sweep above in the same order, so the
 // substitution records and the error flag are the same.
 //
+// Alignment-output mode (EMIT) walks the same runs and writes the strings
+// instead of substitution records: a run's n diagonal columns hold the
+// bases s1[i-1-k], s2[j-1-k] (k < n, the same compressed column on rows of
+// one parity), written by the pair's lanes in parallel; a gap or a
+// mismatching diagonal writes one column, as the row sweep above does, in
+// the same order, so the strings are the same.
+//
 // Block b of the history (rows 16b .. 16b+15), word hp of this lane.
+template <bool EMIT>
 __device__ __forceinline__ void runs_block(const int b, const uint32_t hp, PathState& ps,
                                            const BwConst& B, const NwDev& a) {
   const int base = b << 4;
@@ -800,6 +809,20 @@ __device__ __forceinline__ void runs_block(const int b, const uint32_t hp, PathS
     const int n = (int)((__builtin_ctz(m | 0x80000000u) + 1u) >> 2);
     const int r = ps.rc - 2 * n;  // first cell after the run
     const bool step = on && r >= lo;  // a non-run pointer inside the block
+    if (EMIT && on && n > 0) {
+      // the run's columns: cell (i0 - k, j0 - k) for k < n, one byte each
+      const int cc = (ps.addr >> 2) - B.gpos;
+      const int dd = 2 * (cc - B.start_col) + (ps.rc & 1);  // j - i
+      const int i0 = (ps.rc - dd) >> 1, j0 = (ps.rc + dd) >> 1;
+      if (B.real) {
+        for (int k = B.c; k < n; k += B.ncol) {
+          const uint8_t x0 = B.s1b[i0 - 1 - k], x1 = B.s2b[j0 - 1 - k];
+          a.al0[B.ao + ps.nt + k] = B.swap ? x1 : x0;
+          a.al1[B.ao + ps.nt + k] = B.swap ? x0 : x1;
+        }
+      }
+      ps.nt += n;
+    }
     if (!__builtin_amdgcn_ballot_w64(step)) {
       // the common case: every path runs out of the block
       ps.rc = on ? r : ps.rc;
@@ -807,6 +830,20 @@ __device__ __forceinline__ void runs_block(const int b, const uint32_t hp, PathS
     }
     const int pp = (int)((t >> ((4 * n) & 31)) & 3u);  // 0, 2 or 3 there (a run stops at non-1)
     const bool odd = (r & 1) != 0;
+    if (EMIT && step && pp != 0) {
+      // a gap column at row r, in the path's column before the move: left
+      // (2) pairs a gap with s2[j-1], up (3) s1[i-1] with a gap
+      const int cc = (ps.addr >> 2) - B.gpos;
+      const int dd = 2 * (cc - B.start_col) + (odd ? 1 : 0);
+      const int i = (r - dd) >> 1, j = (r + dd) >> 1;
+      if (B.real && B.c == 0) {
+        const uint8_t x0 = pp == 2 ? (uint8_t)'-' : B.s1b[i - 1];
+        const uint8_t x1 = pp == 3 ? (uint8_t)'-' : B.s2b[j - 1];
+        a.al0[B.ao + ps.nt] = B.swap ? x1 : x0;
+        a.al1[B.ao + ps.nt] = B.swap ? x0 : x1;
+      }
+      ps.nt += 1;
+    }
     // gaps: up (3) one column up on odd rows, left (2) one column down on
     // even rows; both one row up.  Lanes that only ran move to r.
     const bool gap = step && pp != 0;
@@ -827,9 +864,17 @@ __device__ __forceinline__ void runs_block(const int b, const uint32_t hp, PathS
       }
       const bool sub = zero && act;
       const bool invalid = zero && !act;
-      if (sub && B.c == cc && B.real && B.sub_ok) {
-        const int dd = 2 * (cc - B.start_col) + (odd ? 1 : 0);  // j - i on row r
-        const int i = (r - dd) >> 1, j = (r + dd) >> 1;
+      const int dd = 2 * (cc - B.start_col) + (odd ? 1 : 0);  // j - i on row r
+      const int i = (r - dd) >> 1, j = (r + dd) >> 1;
+      if (EMIT) {
+        // a mismatching diagonal column: both bases
+        if (sub && B.c == 0 && B.real) {
+          const uint8_t x0 = B.s1b[i - 1], x1 = B.s2b[j - 1];
+          a.al0[B.ao + ps.nt] = B.swap ? x1 : x0;
+          a.al1[B.ao + ps.nt] = B.swap ? x0 : x1;
+        }
+        ps.nt += sub ? 1 : 0;
+      } else if (sub && B.c == cc && B.real && B.sub_ok) {
         const int qp = B.swap ? i - 1 : j - 1;
         const int cp = B.swap ? j - 1 : i - 1;
         a.sub_pool[B.my_sub + ps.ns] = ((uint32_t)qp << 16) | (uint32_t)cp;
@@ -845,6 +890,7 @@ __device__ __forceinline__ void runs_block(const int b, const uint32_t hp, PathS
 // block four below) as soon as its block is done, so every load has three
 // blocks of work to arrive in (a lone wavefront's blocks are short next to
 // an L2 round trip) and no register copy waits for it.
+template <bool EMIT>
 __device__ __forceinline__ void backward_runs(const int last, const uint32_t* hist_p,
                                               const int lane, PathState& ps, const BwConst& B,
                                               const NwDev& a) {
@@ -854,19 +900,19 @@ __device__ __forceinline__ void backward_runs(const int last, const uint32_t* hi
   uint32_t h2 = hist_p[imax(b - 2, 0) * 64 + lane];
   uint32_t h3 = hist_p[imax(b - 3, 0) * 64 + lane];
   for (;;) {
-    runs_block(b, h0, ps, B, a);
+    runs_block<EMIT>(b, h0, ps, B, a);
     if (b == 0) break;
     h0 = hist_p[imax(b - 4, 0) * 64 + lane];
     --b;
-    runs_block(b, h1, ps, B, a);
+    runs_block<EMIT>(b, h1, ps, B, a);
     if (b == 0) break;
     h1 = hist_p[imax(b - 4, 0) * 64 + lane];
     --b;
-    runs_block(b, h2, ps, B, a);
+    runs_block<EMIT>(b, h2, ps, B, a);
     if (b == 0) break;
     h2 = hist_p[imax(b - 4, 0) * 64 + lane];
     --b;
-    runs_block(b, h3, ps, B, a);
+    runs_block<EMIT>(b, h3, ps, B, a);
     if (b == 0) break;
     h3 = hist_p[imax(b - 4, 0) * 64 + lane];
     --b;
@@ -982,6 +1028,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(8, 8))) void
     B.rec = rec;
     B.rec_v = reinterpret_cast<const uint32_t*>(a.sched + a.sched_off[bin]);
     B.gpos = g_raw * ncol;
+    B.ncol = ncol;
     B.c = c;
     B.start_col = start_col;
     B.st_lo = b0;
@@ -1000,9 +1047,9 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(8, 8))) void
     ps.nt = 0;
     ps.bad = false;
     if (a.emit)
-      backward<true>(last, hist_p, lane, ps, B, a);
+      backward_runs<true>(last, hist_p, lane, ps, B, a);
     else
-      backward_runs(last, hist_p, lane, ps, B, a);
+      backward_runs<false>(last, hist_p, lane, ps, B, a);
     const int ns = ps.ns, nt = ps.nt;
     const bool bad = ps.bad;
     if (__builtin_amdgcn_ballot_w64(bad && real) && lane == 0)
