@@ -688,104 +688,27 @@ struct PathState {
   bool bad;
 };
 
-// One row of the traceback (nwalign.rs:1101-1139) for every pair: each
-// lane fetches the history word of its pair's path column from the owning
-// lane (ds_bpermute) and advances the path when the path cell is on row r.
-// k = r mod 16 selects the pointer within the word; odd = r mod 2.
-template <bool EMIT>
-__device__ __forceinline__ void bw_row(const int r, const int k, const int odd, const uint32_t hp,
-                                       PathState& ps, const BwConst& B, const NwDev& a) {
-  const uint32_t w = (uint32_t)__builtin_amdgcn_ds_bpermute(ps.addr, (int)hp);
-  int pp = (int)__builtin_amdgcn_ubfe(w, (uint32_t)(30 - 2 * k), 2u);
-  bool on = ps.rc == r;
-  // pointer 0: a mismatching diagonal on an active cell, else the
-  // reference's invalid pointer (row 1 holds no active cells), which ends
-  // the path here
-  if (__builtin_amdgcn_ballot_w64(on && pp == 0)) {
-    const int cc = (ps.addr >> 2) - B.gpos;
-    bool act = false;
-    if (r >= 2) {
-      const int rcmin = (int)B.rec[kRecWords * (r - 2) + 2];
-      const uint32_t rn = B.rec[kRecWords * (r - 2) + 3];
-      act = (unsigned)(cc - rcmin) < rn;
-    }
-    const bool sub = on && pp == 0 && act;
-    if (!EMIT && sub && B.c == cc && B.real && B.sub_ok) {
-      const int dd = 2 * (cc - B.start_col) + odd;  // j - i on this row
-      const int i = (r - dd) >> 1, j = (r + dd) >> 1;
-      const int qp = B.swap ? i - 1 : j - 1;
-      const int cp = B.swap ? j - 1 : i - 1;
-      a.sub_pool[B.my_sub + ps.ns] = ((uint32_t)qp << 16) | (uint32_t)cp;
-    }
-    ps.ns += sub ? 1 : 0;
-    pp = sub ? 1 : pp;
-    const bool invalid = on && pp == 0;
-    ps.bad = ps.bad || invalid;
-    ps.rc = invalid ? -1 : ps.rc;
-    on = on && !invalid;
-  }
-  if (EMIT) {
-    const int cc = (ps.addr >> 2) - B.gpos;
-    if (on && B.c == cc && B.real) {
-      const int dd = 2 * (cc - B.start_col) + odd;
-      const int i = (r - dd) >> 1, j = (r + dd) >> 1;
-      const uint8_t x0 = pp == 2 ? (uint8_t)'-' : B.s1b[i - 1];
-      const uint8_t x1 = pp == 3 ? (uint8_t)'-' : B.s2b[j - 1];
-      a.al0[B.ao + ps.nt] = B.swap ? x1 : x0;
-      a.al1[B.ao + ps.nt] = B.swap ? x0 : x1;
-    }
-    ps.nt += on ? 1 : 0;
-  }
-  // step (pointer 1, 2 or 3 where on): diag (i-1, j-1) two rows up; left
-  // (i, j-1) and up (i-1, j) one row up, i.e. one column down on even rows
-  // / up on odd rows
-  if (odd)
-    ps.addr += (on && pp == 3) ? 4 : 0;
-  else
-    ps.addr -= (on && pp == 2) ? 4 : 0;
-  ps.rc = on ? r - 1 - (pp == 1 ? 1 : 0) : ps.rc;
-}
-
-// Rows last..1 in 16-row history blocks; each block's words are loaded one
-// block ahead, and whole blocks run fully unrolled.
-template <bool EMIT>
-__device__ __forceinline__ void backward(const int last, const uint32_t* hist_p, const int lane,
-                                         PathState& ps, const BwConst& B, const NwDev& a) {
-  const int top = last >> 4;
-  uint32_t hp = hist_p[top * 64 + lane];
-  uint32_t hp_next = top >= 1 ? hist_p[(top - 1) * 64 + lane] : 0u;
-  const int rlo = top * 16 > 1 ? top * 16 : 1;
-  for (int r = last; r >= rlo; --r) bw_row<EMIT>(r, r & 15, r & 1, hp, ps, B, a);
-  for (int b = top - 1; b >= 1; --b) {
-    hp = hp_next;
-    hp_next = hist_p[(b - 1) * 64 + lane];
-#pragma unroll
-    for (int k = 15; k >= 0; --k) bw_row<EMIT>(b * 16 + k, k, k & 1, hp, ps, B, a);
-  }
-  if (top >= 1) {
-    hp = hp_next;
-    for (int r = 15; r >= 1; --r) bw_row<EMIT>(r, r, r & 1, hp, ps, B, a);
-  }
-}
-
-// The compare-mode traceback by runs.  A diagonal step keeps the compressed
-// column and goes two rows up, so within one 16-row history word the
-// pointers of rows r, r-2, r-4, ... of a column sit 4 bits apart: shifted to
-// row r, a run of diagonal-over-match pointers (1) is the count of trailing
-// zero fields of (t ^ 0x11111111) & 0x33333333.  Each iteration moves every
-// pair's path over its run in the current block and then takes at most one
-// other step (a mismatching diagonal: an al2subs substitution; a gap; or the
-// reference's invalid pointer), so a block costs one iteration per non-run
-// step of the wavefront's busiest pair instead of one per row.  The path
-// visits the cells of the row sweep above in the same order, so the
-// substitution records and the error flag are the same.
+// The traceback (nwalign.rs:1101-1139) by runs.  The path starts at the
+// last cell and takes pointer p of its cell: 1 a diagonal (i-1, j-1), two
+// rows up in the same compressed column; 2 left (i, j-1) and 3 up (i-1, j),
+// one row up, i.e. one column down on even rows / up on odd rows; 0 a
+// mismatching diagonal on an active cell (the al2subs substitution), else
+// the reference's invalid pointer, which ends the path with the error flag.
+// Each lane fetches its pair's path word from the lane owning the column
+// (ds_bpermute).  Within one 16-row history word the pointers of rows r,
+// r-2, r-4, ... of a column sit 4 bits apart: shifted to row r, a run of
+// diagonal-over-match pointers (1) is the count of trailing zero fields of
+// (t ^ 0x11111111) & 0x33333333.  Each iteration moves every pair's path
+// over its run in the current block and then takes at most one other step,
+// so a block costs one iteration per non-run step of the wavefront's
+// busiest pair instead of one per row.
 //
-// Alignment-output mode (EMIT) walks the same runs and writes the strings
-// instead of substitution records: a run's n diagonal columns hold the
-// bases s1[i-1-k], s2[j-1-k] (k < n, the same compressed column on rows of
-// one parity), written by the pair's lanes in parallel; a gap or a
-// mismatching diagonal writes one column, as the row sweep above does, in
-// the same order, so the strings are the same.
+// Compare mode records each substitution (query position << 16 | center
+// position) from the lane owning the path column.  Alignment-output mode
+// (EMIT) writes the alignment strings in path order (reversed afterwards):
+// a run's n diagonal columns hold s1[i-1-k], s2[j-1-k] (k < n), written by
+// the pair's lanes in parallel; a gap or a mismatching diagonal writes one
+// column from the pair's first lane.
 //
 // Block b of the history (rows 16b .. 16b+15), word hp of this lane.
 template <bool EMIT>
