@@ -403,6 +403,18 @@ __device__ __forceinline__ void wait_vm_all() { __builtin_amdgcn_s_waitcnt(0xF70
 // The path's moves inside history block blk (steps [16 blk, 16 blk + 16)),
 // whose words are hb.  Same moves, records and end conditions as
 // band_bw_step; returns false once the path has ended.
+//
+// A diagonal move keeps the diagonal D and goes two steps back, so within
+// one block the pointers of steps t, t-2, t-4, ... of D sit 4 bits apart in
+// the owner's word: shifted to step t, a run of diagonal-over-match
+// pointers (1) is the count of trailing zero fields of
+// (x ^ 0x11111111) & 0x33333333.  Each iteration takes the whole run and
+// then at most one other move, so a block costs one iteration per
+// mismatch or gap on the path.  Steps below the block read as zero fields,
+// which ends a run at the block's edge; row 0 and column 0 hold pointers 2
+// and 3, so no run reaches step 1 or 0.  In alignment-output mode a run's n
+// columns (the bases q1[i-1-l], q2[j-1-l], l < n) are written by lanes
+// 0 .. n-1.
 template <bool EMIT, int K>
 __device__ __forceinline__ bool band_solo_block(const int blk, const uint32_t (&hb)[K],
                                                 BandSolo& st, const bool writer, const int lane,
@@ -417,8 +429,21 @@ __device__ __forceinline__ bool band_solo_block(const int blk, const uint32_t (&
       const uint32_t x = (uint32_t)__builtin_amdgcn_readlane((int)hb[m], owner);
       w = mi == m ? x : w;
     }
-    int pp = (int)((w >> (30 - 2 * (st.rc & 15))) & 3u);
+    const uint32_t x = w >> ((30 - 2 * (st.rc & 15)) & 31);
+    const uint32_t rm = (x ^ 0x11111111u) & 0x33333333u;
+    const int n = (int)((__builtin_ctz(rm | 0x80000000u) + 1u) >> 2);
     const int d = st.D - B.lband - 1;
+    if (EMIT && n > 0) {
+      const int i0 = (st.rc - d) >> 1, j0 = i0 + d;
+      if (writer && lane < n) {
+        a.al0[B.ao + st.nt + lane] = (uint8_t)B.q1[i0 - 1 - lane];
+        a.al1[B.ao + st.nt + lane] = (uint8_t)B.q2[j0 - 1 - lane];
+      }
+      st.nt += n;
+    }
+    st.rc -= 2 * n;
+    if (st.rc < lo) break;
+    int pp = (int)((x >> ((4 * n) & 31)) & 3u);  // 0, 2 or 3: a run stops at non-1
     const int i = (st.rc - d) >> 1, j = i + d;
     if (pp == 0) {  // a mismatching diagonal on an interior cell, else unset
       const bool sentd = B.sentinels && (st.D == 0 || st.D == B.dmax);
@@ -436,7 +461,7 @@ __device__ __forceinline__ bool band_solo_block(const int blk, const uint32_t (&
       pp = interior ? 1 : 0;
     }
     if (EMIT && pp != 0) {
-      if (writer) {
+      if (writer && lane == 0) {
         a.al0[B.ao + st.nt] = pp == 2 ? (uint8_t)'-' : (uint8_t)B.q1[max(i - 1, 0)];
         a.al1[B.ao + st.nt] = pp == 3 ? (uint8_t)'-' : (uint8_t)B.q2[max(j - 1, 0)];
       }
@@ -490,8 +515,10 @@ __device__ __forceinline__ void band_backward_solo(const int last, const uint32_
   st.flushed = 0;
   st.bad = false;
   st.sbuf[0] = st.sbuf[1] = 0;
-  const bool writer = lane == 0 && B.real;
-  const bool store = !EMIT && B.sub_ok && B.real;
+  // the wavefront holds one pair, on lanes 0 .. lanes-1; record n sits in
+  // lane n & 63 whatever the pair's width, so every lane writes for it
+  const bool writer = __builtin_amdgcn_readfirstlane(B.real ? 1 : 0) != 0;
+  const bool store = !EMIT && B.sub_ok && writer;
   bool more = st.rc >= 1;
   while (more) {
 #pragma unroll

@@ -141,3 +141,33 @@ def test_compare_long_reads_every_dp_tile(pkg, oracle, tile):
         lam, ham, skp = oracle.compare_center(rs, cc, w.err, w.err.shape[1], oparams, nthreads=8)[:3]
         assert np.array_equal(got.lam[r].view(np.uint64), lam.view(np.uint64)), cc
         assert np.array_equal(got.hamming[r], ham), cc
+
+
+@pytest.mark.parametrize("sub", [0.02, 0.03])
+def test_compare_long_reads_dense_substitutions(ctx, oracle, pkg, sub):
+    """Long-read pairs 2-6% apart (35 to about 105 substitutions): more
+    substitution records than the pair has lanes (34 + |len1 - len2| / 2 at
+    band 32) and more than one 64-record register of the one-pair
+    traceback.  Lambda bits, hamming and skipped flags against the oracle's
+    compare, then the alignment strings of the same pairs."""
+    from dada2_rs_amd import synth
+    w = synth.workload_longread(n_variants=2, n_uniques=30, sub=sub, seed=synth.SEED_BASE + 43)
+    rs = oracle.RawSet(w.seqs, w.quals, w.reads, k=w.kmer_size)
+    ctx.load_uniques(w.seqs, w.quals, w.reads, k=w.kmer_size)
+    params = pkg.align_params(band=32, homo_gap_p=-1)
+    oparams = oracle.make_params(band=32, homo_gap_p=-1)
+    got = ctx.compare_batch(w.centers, w.err, params)
+    assert got.stats["aligned_dp"] > 0
+    hams = []
+    for r, c in enumerate(int(x) for x in w.centers):
+        lam, ham, skp = oracle.compare_center(rs, c, w.err, w.err.shape[1], oparams, nthreads=8)[:3]
+        assert np.array_equal(got.lam[r].view(np.uint64), lam.view(np.uint64)), c
+        assert np.array_equal(got.hamming[r], ham), c
+        assert np.array_equal(got.skipped[r], skp), c
+        hams += [int(h) for h, s in zip(ham, skp) if not s and h != 0xFFFFFFFF]
+    assert max(hams) > 64 and sum(h > 34 for h in hams) > 10, hams
+    pairs = [(w.seqs[int(c)].tobytes(), w.seqs[i].tobytes())
+             for c in w.centers for i in range(0, w.n, 2)]
+    pairs = [p for p in pairs if pkg.band_supported(len(p[0]), len(p[1]), 32, True)]
+    _check_pairs(ctx, oracle, pairs, 32, homo_gap=-1)
+    _check_pairs(ctx, oracle, pairs, 32)
