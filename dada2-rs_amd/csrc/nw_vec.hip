@@ -72,9 +72,11 @@ struct NwDev {
 };
 
 // ---------------------------------------------------------------------------
-// Schedule: a replay of the reference's row walk, one record per row 2..last.
+// Schedule: a replay of the reference's row walk, one record per row 2..last,
+// handed to sink(row, R) in row order.
 // ---------------------------------------------------------------------------
-D2G_HD void vec_schedule(const VecGeom& geo, int end_gap, int gap, RowRec* out) {
+template <class Sink>
+D2G_HD void vec_schedule(const VecGeom& geo, int end_gap, int gap, Sink&& sink) {
   const int len1 = geo.len1, len2 = geo.len2, band = geo.band, start_col = geo.start_col;
   const int lfl = geo.left_fill_limit, tfl = geo.top_fill_limit, top_limit = geo.top_limit;
   const int band_min_len1 = imin(band, len1);
@@ -120,7 +122,7 @@ D2G_HD void vec_schedule(const VecGeom& geo, int end_gap, int gap, RowRec* out) 
     flags |= ((uint32_t)col_min & 0xFFu) << 8;
     flags |= ((uint32_t)col_max & 0xFFu) << 16;
     R.flags = flags;
-    out[row - 2] = R;
+    sink(row, R);
     // index walk (nwalign.rs:1032-1079)
     if (row < band && row < len1) {
       if (even) col_min = imax(col_min - 1, 0);
@@ -173,13 +175,21 @@ __global__ void sched_kernel(BinSpace bs, const uint64_t* off, int end_gap, int 
     return;
   }
   RowRec* t = table + o;  // nrow records: rows 2 .. len1 + len2, two lookahead
-  vec_schedule(g, end_gap, gap, t);
   const int last = g.len1 + g.len2;
-  // active-lane masks over the G pairs of a wavefront (pair gi on lanes
-  // gi * ncol .. gi * ncol + ncol - 1)
   const int G = nw_groups(g, bs.tile);
-  for (int row = 2; row <= last; ++row) {
-    RowRec& R = t[row - 2];
+  // One pass over the walk: each record gets its lane masks and masked-row
+  // flag and is stored; the steady-run search and the masked-row check keep
+  // what they need of the two rows above in registers, so nothing is read
+  // back from the table.
+  uint64_t act_p1 = 0, act_p2 = 0;  // active lanes of rows row-1, row-2
+  int a1_p1 = 0, a2_p1 = 0;         // s1 / s2 offsets of row row-1
+  int best_lo = 0, best_hi = 0, run_lo = -1;
+  int nm_first = -1, nm_last = -1;  // rows without the masked-row flag
+  uint32_t cells = 0;
+  RowRec last_rec;
+  auto sink = [&](const int row, RowRec& R) {
+    // active-lane masks over the G pairs of a wavefront (pair gi on lanes
+    // gi * ncol .. gi * ncol + ncol - 1)
     uint64_t base = 0;
     if (R.n > 0 && R.cmin < 64)
       base = (R.n >= 64 ? ~0ull : ((1ull << R.n) - 1ull)) << R.cmin;
@@ -192,50 +202,54 @@ __global__ void sched_kernel(BinSpace bs, const uint64_t* off, int end_gap, int 
     const int cmin_new = (int)((R.flags >> 8) & 0xFFu), cmax_new = (int)((R.flags >> 16) & 0xFFu);
     const bool rec_l = !(R.flags & 1u) || cmin_new >= 1;
     const bool rec_r = !(R.flags & 2u) || cmax_new + 1 <= g.ncol - 1;
-    if (window && inside && rec_l && rec_r) R.flags |= kRowMasked;
-  }
-  // two lookahead records past the last row (rows last+1, last+2)
-  t[last - 1] = t[last - 2];
-  t[last] = t[last - 2];
-  // longest run of steady rows whose s1 / s2 offsets advance by one after
-  // each odd / even row, trimmed to even bounds and kept off the last row
-  // (whose history word the generic and masked rows flush)
-  int best_lo = 0, best_hi = 0, run_lo = -1;
-  uint32_t cells = 0;
-  for (int row = 2; row <= last; ++row) {
-    const RowRec& R = t[row - 2];
+    if (window && inside && rec_l && rec_r) {
+      R.flags |= kRowMasked;
+    } else {
+      if (nm_first < 0) nm_first = row;
+      nm_last = row;
+    }
+    t[row - 2] = R;
+    // longest run of steady rows whose s1 / s2 offsets advance by one after
+    // each odd / even row.  A steady row's active cells keep off each pair's
+    // outer columns (a lane-shift that leaves the wavefront lands on an
+    // inactive lane), and its active lanes are those of the row two up (so
+    // every active cell has an active diagonal predecessor, and a run needs
+    // one mask per parity).
     cells += R.n;
-    // a steady row's active cells keep off each pair's outer columns (a
-    // lane-shift that leaves the wavefront lands on an inactive lane), and
-    // its active lanes are those of the row two up (so every active cell has
-    // an active diagonal predecessor, and a run needs one mask per parity)
-    const uint64_t act = ((uint64_t)R.act_hi << 32) | R.act_lo;
-    const uint64_t act2 = row >= 4 ? (((uint64_t)t[row - 4].act_hi << 32) | t[row - 4].act_lo) : 0ull;
-    const bool steady = R.edge_l < 0 && R.edge_t < 0 && (R.flags & 3u) == 0 &&
-                        R.inr_lo == R.cmin && R.inr_n == R.n && R.cmin >= 1 &&
-                        R.cmin + (int)R.n <= g.ncol - 1 && act == act2;
+    const uint64_t act2 = row >= 4 ? act_p2 : 0ull;
+    const bool steady = R.edge_l < 0 && R.edge_t < 0 && (R.flags & 3u) == 0 && window &&
+                        R.cmin >= 1 && R.cmin + (int)R.n <= g.ncol - 1 && m == act2;
     if (!steady) {
       run_lo = -1;
-      continue;
-    }
-    if (run_lo >= 0) {
-      const RowRec& P = t[row - 3];
-      const bool odd_prev = ((row - 1) & 1) != 0;
-      const bool linked = R.a1 - P.a1 == (odd_prev ? 1 : 0) && R.a2 - P.a2 == (odd_prev ? 0 : 1);
-      if (!linked) run_lo = row;
     } else {
-      run_lo = row;
+      if (run_lo >= 0) {
+        const bool odd_prev = ((row - 1) & 1) != 0;
+        const bool linked = R.a1 - a1_p1 == (odd_prev ? 1 : 0) && R.a2 - a2_p1 == (odd_prev ? 0 : 1);
+        if (!linked) run_lo = row;
+      } else {
+        run_lo = row;
+      }
+      if (row + 1 - run_lo > best_hi - best_lo) {
+        best_lo = run_lo;
+        best_hi = row + 1;
+      }
     }
-    if (row + 1 - run_lo > best_hi - best_lo) {
-      best_lo = run_lo;
-      best_hi = row + 1;
-    }
-  }
+    act_p2 = act_p1;
+    act_p1 = m;
+    a1_p1 = R.a1;
+    a2_p1 = R.a2;
+    last_rec = R;
+  };
+  vec_schedule(g, end_gap, gap, sink);
+  // two lookahead records past the last row (rows last+1, last+2)
+  t[last - 1] = last_rec;
+  t[last] = last_rec;
+  // the run trimmed to even bounds and kept off the last row (whose history
+  // word the generic and masked rows flush)
   int lo = (best_lo + 1) & ~1, hi = imin(best_hi, last) & ~1;
   if (hi <= lo) lo = hi = last + 1;
-  bool all_masked = true;
-  for (int row = 2; row <= last; ++row)
-    if ((row < lo || row >= hi) && !(t[row - 2].flags & kRowMasked)) all_masked = false;
+  // every row outside [lo, hi) carries the masked-row flag
+  const bool all_masked = nm_first < 0 || (nm_first >= lo && nm_last < hi);
   BinInfo bi;
   bi.steady_lo = (uint32_t)lo;
   bi.steady_hi = (uint32_t)hi;
