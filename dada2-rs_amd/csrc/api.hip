@@ -914,11 +914,19 @@ static int compare_pass(d2g_ctx* c, const uint32_t* centers, uint32_t nc, const 
     sa.single = single;
     HIP_TRY(tm ? launch_pair_screen(sa, tm->want, s) : launch_screen(sa, s));
     if (timed) HIP_TRY(hipEventRecord(c->ev[1], s));
-    HIP_TRY(launch_plan(bs, c->bin_count.p, c->bin_start.p, c->wave_start.p, c->bin_fill.p, c->ctr.p,
-                        c->sched_info.p, c->screen_partials.p, n_part, c->gl_off.p, s));
-    HIP_TRY(launch_scatter(c->dp_unsorted.p, c->dp_sorted.p, c->bin_start.p, c->bin_fill.p,
-                           c->screen_partials.p, n_part, region, 0, c->gl_unsorted.p, c->gl.p,
-                           c->gl_off.p, s));
+    if (single) {
+      // one center: the plan kernel compacts the length-ordered regions
+      const PlanCopy cp{c->dp_unsorted.p, c->dp_sorted.p, c->gl_unsorted.p, c->gl.p, region};
+      HIP_TRY(launch_plan(bs, c->bin_count.p, c->bin_start.p, c->wave_start.p, c->bin_fill.p,
+                          c->ctr.p, c->sched_info.p, c->screen_partials.p, n_part, c->gl_off.p, s,
+                          &cp));
+    } else {
+      HIP_TRY(launch_plan(bs, c->bin_count.p, c->bin_start.p, c->wave_start.p, c->bin_fill.p,
+                          c->ctr.p, c->sched_info.p, c->screen_partials.p, n_part, c->gl_off.p, s));
+      HIP_TRY(launch_scatter(c->dp_unsorted.p, c->dp_sorted.p, c->bin_start.p, c->bin_fill.p,
+                             c->screen_partials.p, n_part, region, 0, c->gl_unsorted.p, c->gl.p,
+                             c->gl_off.p, s));
+    }
     if (timed) HIP_TRY(hipEventRecord(c->ev[2], s));
     NwArgs na;
     std::memset(&na, 0, sizeof(na));

@@ -100,13 +100,25 @@ hipError_t launch_pair_screen(const ScreenArgs& a, const uint32_t* want, hipStre
 // ---- plan.hip ----------------------------------------------------------
 hipError_t launch_zero(uint32_t* p, size_t n_words, hipStream_t s);
 hipError_t launch_zero2(uint32_t* a, size_t na, uint32_t* b, size_t nb, hipStream_t s);
+// The task lists to compact in the plan kernel itself, for a screen whose
+// regions already list their DP tasks in bin order (one center: uniques are
+// length-sorted, so a region's tasks and the regions follow the bins).
+struct PlanCopy {
+  const DpTask* dp_unsorted;
+  DpTask* dp_sorted;
+  const GaplessTask* gl_unsorted;
+  GaplessTask* gl;
+  uint32_t region;  // task slots per region
+};
 // Also totals the DP cells (info: per-bin cells, nullable) and, for a
 // compare, the screen's per-block counts (partials: 4 words per block,
-// nullable), writing each block's gapless offset to gl_off.
+// nullable), writing each block's gapless offset to gl_off.  With copy
+// (partials required), the plan kernel also moves the regions' DP and
+// gapless tasks to their places, and no scatter is needed.
 hipError_t launch_plan(const BinSpace& bs, const uint32_t* bin_count, uint32_t* bin_start,
                        uint32_t* wave_start, uint32_t* bin_fill, Counters* ctr,
                        const BinInfo* info, const uint32_t* partials, uint32_t n_partials,
-                       uint32_t* gl_off, hipStream_t s);
+                       uint32_t* gl_off, hipStream_t s, const PlanCopy* copy = nullptr);
 // Bins the DP tasks of n_regions regions of `region` slots (counts from
 // partials, or `count` tasks filling the regions in order when partials is
 // null) and compacts the gapless tasks (compare only).

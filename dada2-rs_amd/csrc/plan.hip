@@ -93,7 +93,8 @@ __global__ __launch_bounds__(kPlanThreads) void plan_kernel(BinSpace bs,
                                                             const BinInfo* info,
                                                             const uint32_t* partials,
                                                             uint32_t n_partials,
-                                                            uint32_t* gl_off) {
+                                                            uint32_t* gl_off, PlanCopy copy,
+                                                            int do_copy) {
   __shared__ uint32_t sh[kPlanThreads];
   __shared__ unsigned long long s_cells[kPlanThreads / 64];
   const int tid = threadIdx.x;
@@ -130,7 +131,7 @@ __global__ __launch_bounds__(kPlanThreads) void plan_kernel(BinSpace bs,
     rw += G ? (cnt + G - 1) / G : 0;
   }
   // screen blocks: skip / shroud totals, gapless offsets
-  uint32_t skip = 0, shroud = 0, ngl = 0;
+  uint32_t skip = 0, shroud = 0, ngl = 0, ndp = 0;
   const uint32_t pchunk = (n_partials + kPlanThreads - 1) / kPlanThreads;
   const uint32_t p0 = min(n_partials, tid * pchunk), p1 = min(n_partials, p0 + pchunk);
   if (partials) {
@@ -138,13 +139,29 @@ __global__ __launch_bounds__(kPlanThreads) void plan_kernel(BinSpace bs,
       skip += partials[4 * i];
       shroud += partials[4 * i + 1];
       ngl += partials[4 * i + 2];
+      ndp += partials[4 * i + 3];
     }
   }
   uint32_t tot_skip, tot_shroud, tot_gl;
   (void)block_scan(skip, sh, &tot_skip);
   (void)block_scan(shroud, sh, &tot_shroud);
   uint32_t go = block_scan(ngl, sh, &tot_gl);
-  if (partials) {
+  if (do_copy) {
+    // regions in order hold the DP tasks in bin order: region i's tasks go
+    // to the exclusive prefix of the regions' DP counts, which is where
+    // bin_start and the in-region order put them
+    uint32_t tot_dp;
+    uint32_t dpo = block_scan(ndp, sh, &tot_dp);
+    for (uint32_t i = p0; i < p1; ++i) {
+      const uint32_t nd = partials[4 * i + 3], ng = partials[4 * i + 2];
+      const DpTask* ds = copy.dp_unsorted + (size_t)i * copy.region;
+      const GaplessTask* gs = copy.gl_unsorted + (size_t)i * copy.region;
+      for (uint32_t j = 0; j < nd; ++j) copy.dp_sorted[dpo + j] = ds[j];
+      for (uint32_t j = 0; j < ng; ++j) copy.gl[go + j] = gs[j];
+      dpo += nd;
+      go += ng;
+    }
+  } else if (partials) {
     for (uint32_t i = p0; i < p1; ++i) {
       gl_off[i] = go;
       go += partials[4 * i + 2];
@@ -171,9 +188,13 @@ __global__ __launch_bounds__(kPlanThreads) void plan_kernel(BinSpace bs,
 hipError_t launch_plan(const BinSpace& bs, const uint32_t* bin_count, uint32_t* bin_start,
                        uint32_t* wave_start, uint32_t* bin_fill, Counters* ctr,
                        const BinInfo* info, const uint32_t* partials, uint32_t n_partials,
-                       uint32_t* gl_off, hipStream_t s) {
+                       uint32_t* gl_off, hipStream_t s, const PlanCopy* copy) {
+  if (copy && !partials) return hipErrorInvalidValue;
+  PlanCopy cp{};
+  if (copy) cp = *copy;
   hipLaunchKernelGGL(plan_kernel, dim3(1), dim3(kPlanThreads), 0, s, bs, bin_count, bin_start,
-                     wave_start, bin_fill, ctr, info, partials, n_partials, gl_off);
+                     wave_start, bin_fill, ctr, info, partials, n_partials, gl_off, cp,
+                     copy ? 1 : 0);
   return hipGetLastError();
 }
 
