@@ -1876,6 +1876,7 @@ static int align_pairs_device(d2g_ctx* c, uint32_t n, const PairSeqs& ps, int ki
     lmax = std::max({lmax, l1[i], l2[i]});
   }
   const uint64_t n1 = ps.s1_off[ps.n1] - ps.s1_off[0], n2 = ps.s2_off[ps.n2] - ps.s2_off[0];
+  if (n1 + n2 >= ((uint64_t)1 << 32)) return D2G_ERR_UNSUPPORTED;  // 32-bit pool offsets
   std::vector<uint64_t> aoff(n + 1, 0);
   for (uint32_t i = 0; i < n; ++i) aoff[i + 1] = aoff[i] + l1[i] + l2[i];
   if (al_offsets)

@@ -856,6 +856,8 @@ __device__ __forceinline__ void backward_runs(const int last, const uint32_t* hi
   }
 }
 
+constexpr int kStage = 4;  // sequence bytes per lane requested together when staging a pair
+
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(8, 8))) void nw_vec_kernel(NwDev a) {
   extern __shared__ uint32_t lds_words[];
   const int lane = (int)__lane_id();
@@ -898,11 +900,30 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(8, 8))) void
     uint8_t* gbase = gbase0 + (size_t)g * gbytes;
     uint8_t* s1b = gbase + pad;
     uint8_t* s2b = gbase + pad + len1 + 2 * pad;
-    const uint8_t* src1 = a.pool + (swap ? task.s2_off : task.s1_off);
-    const uint8_t* src2 = a.pool + (swap ? task.s1_off : task.s2_off);
     if (real) {
-      for (int t = c; t < len1; t += ncol) s1b[t] = src1[t];
-      for (int t = c; t < len2; t += ncol) s2b[t] = src2[t];
+      // kStage bytes of each sequence per lane are requested before any is
+      // stored (indices clamped into the sequence, so every load is in
+      // bounds): a 250-nt pair takes four round trips, not eight (more in
+      // flight would spill at 64 registers)
+      // (the pool is below 2^32 bytes: 32-bit offsets from its uniform base)
+      const int lmax = imax(len1, len2);
+      const uint32_t o1 = (uint32_t)(swap ? task.s2_off : task.s1_off);
+      const uint32_t o2 = (uint32_t)(swap ? task.s1_off : task.s2_off);
+      for (int t0 = c; t0 < lmax; t0 += kStage * ncol) {
+        uint32_t b1[kStage], b2[kStage];
+#pragma unroll
+        for (int k = 0; k < kStage; ++k) {
+          const int t = t0 + k * ncol;
+          b1[k] = a.pool[o1 + (uint32_t)imin(t, len1 - 1)];
+          b2[k] = a.pool[o2 + (uint32_t)imin(t, len2 - 1)];
+        }
+#pragma unroll
+        for (int k = 0; k < kStage; ++k) {
+          const int t = t0 + k * ncol;
+          if (t < len1) s1b[t] = (uint8_t)b1[k];
+          if (t < len2) s2b[t] = (uint8_t)b2[k];
+        }
+      }
     }
     __syncthreads();
 
