@@ -31,13 +31,13 @@
 // 16)) flushed to global memory every 16 rows.  A diagonal step over a
 // mismatch is stored as pointer 0; only active cells carry diagonal
 // pointers, so the traceback tells it from the reference's invalid pointer
-// by the row's active range.  The traceback (nwalign.rs:1101-1139) sweeps
-// the rows downwards once: per row two ballots rebuild the row's pointer
-// mask and every lane advances its pair's path cell (the path state is
-// replicated across the pair's lanes).  Mismatching diagonal steps become
-// al2subs substitutions (nwalign.rs:1169-1224), written straight to global
-// memory as (query position << 16 | center position) for the lambda
-// kernel; in parity mode the alignment strings are written instead.
+// by the row's active range.  The traceback (nwalign.rs:1101-1139) walks
+// every pair's path down the history blocks a run of diagonal matches at a
+// time (the path state is replicated across the pair's lanes).  Mismatching
+// diagonal steps become al2subs substitutions (nwalign.rs:1169-1224),
+// written straight to global memory as (query position << 16 | center
+// position) for the lambda kernel; in alignment-output mode the alignment
+// strings are written instead.
 #include <algorithm>
 #include <type_traits>
 
