@@ -357,7 +357,9 @@ int d2g_transition_stats(d2g_ctx* ctx, d2g_stats* out);
  * (n+1 entries each).  al0/al1 receive each alignment at al_offsets[i]
  * (caller supplies capacity len1+len2 per pair: al_offsets[i] =
  * sum_{j<i} (len1_j + len2_j)); al_len[i] its length.  al_offsets may be
- * NULL, meaning exactly that packing. */
+ * NULL, meaning exactly that packing.  The two sets together must stay
+ * below 4 GiB of sequence bytes (D2G_ERR_UNSUPPORTED otherwise), as the
+ * compare's unique set must. */
 int d2g_align_vectorized(d2g_ctx* ctx, uint32_t n, const uint8_t* s1, const uint64_t* s1_off,
                          const uint8_t* s2, const uint64_t* s2_off, const d2g_vec_scores* scores,
                          uint8_t* al0, uint8_t* al1, const uint64_t* al_offsets,
