@@ -235,34 +235,38 @@ __device__ __forceinline__ void screen_pairs(const ScreenDev& a, const QueryFiel
         // kdist > cutoff (raw_align_with_buf, nwalign.rs:1296)
         shroud = !skip && dot < (Q.len <= C.len ? Q.thr : C.thr);
       }
+      // kord_dist (kmers.rs:314-328) needs the count of equal positions of an
+      // equal-length pair that passes: precomputed (keq >= 0) or counted
+      // here by the whole wavefront, one such pair at a time, from
+      // coalesced reads of the two order rows and one wavefront sum (a
+      // lane's own loop would wait on one load per word)
+      int32_t eqc = keq[j][c];
+      const bool needk = in && !skip && !shroud && a.use_kmers && a.gapless && Q.len == C.len &&
+                         eqc < 0;
+      for (uint64_t need = __builtin_amdgcn_ballot_w64(needk); need; need &= need - 1ull) {
+        const int l = (int)__builtin_ctzll(need);
+        const uint32_t qq = (uint32_t)__builtin_amdgcn_readlane((int)Q.qq, l);
+        const uint32_t klen = (uint32_t)__builtin_amdgcn_readlane((int)Q.len, l) - (k - 1);
+        const uint32_t nw = (klen + 1) / 2;
+        const uint32_t* qrow = u.kord_row + (size_t)qq * u.owords;
+        const uint32_t* crow = u.kord_row + (size_t)C.ci * u.owords;
+        uint32_t cnt = 0;
+        for (uint32_t w = lane_id(); w < nw; w += 64u) {
+          const uint32_t x = qrow[w] ^ crow[w];
+          cnt += (x & 0xFFFFu) == 0;
+          if (2 * w + 1 < klen) cnt += (x >> 16) == 0;
+        }
+        for (int off = 32; off > 0; off >>= 1) cnt += (uint32_t)__shfl_xor((int)cnt, off);
+        if ((int)lane_id() == l) eqc = (int32_t)cnt;
+      }
       bool route_gapless = false;
       if (!skip && !shroud) {
         const uint32_t mn = Q.len < C.len ? Q.len : C.len;
         const double kdist = a.use_kmers ? 1.0 - (double)dot / (double)(mn - k + 1) : 0.0;
         double kodist = -1.0;
         if (a.use_kmers && a.gapless && Q.len == C.len) {
-          // kord_dist, kmers.rs:314-328: the count of equal positions comes
-          // precomputed (keq >= 0) or is counted here, KC words per round trip
-          uint32_t klen = Q.len - (k - 1);
-          uint32_t eq = keq[j][c] >= 0 ? (uint32_t)keq[j][c] : 0u;
-          if (keq[j][c] < 0) {
-            const uint32_t nw = (klen + 1) / 2;
-            const uint32_t* crow = u.kord_row + (size_t)C.ci * u.owords;
-            for (uint32_t w0 = 0; w0 < nw; w0 += KC) {
-              uint32_t xs[KC];
-#pragma unroll
-              for (int t = 0; t < KC; ++t) {
-                const uint32_t w = w0 + (uint32_t)t;
-                xs[t] = w < nw ? u.kord_il[(size_t)w * u.npad + Q.qq] ^ crow[w] : 0xFFFFFFFFu;
-              }
-#pragma unroll
-              for (int t = 0; t < KC; ++t) {
-                const uint32_t w = w0 + (uint32_t)t;
-                eq += (xs[t] & 0xFFFFu) == 0;  // 0 past the end: both halves nonzero
-                if (2 * w + 1 < klen) eq += (xs[t] >> 16) == 0;
-              }
-            }
-          }
+          const uint32_t klen = Q.len - (k - 1);
+          const uint32_t eq = eqc >= 0 ? (uint32_t)eqc : 0u;
           kodist = 1.0 - (double)eq / (double)klen;
         }
         route_gapless = a.band == 0 || fabs(kodist - kdist) < DBL_EPSILON;
